@@ -1,0 +1,9 @@
+"""MI355X-native MB-pol force + heat-current path (gfx950 HIP kernels behind a C ABI).
+
+The directory name is not a Python identifier, so import it with
+``importlib.import_module("dlpoly2-mbpol-heat_amd")`` after putting the repository root on
+``sys.path``.
+"""
+from . import waterbox  # noqa: F401
+from .mbgpu import (Context, MbgpuError, load_library, mbpol_2b_cutoff, mbpol_2b_poly,  # noqa: F401
+                    mbpol_3b_cutoff, mbpol_3b_poly)

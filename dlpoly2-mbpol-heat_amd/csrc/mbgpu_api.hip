@@ -1,0 +1,548 @@
+// mbgpu_api.hip -- C ABI of libmbgpu.so: contexts, device buffers and the host-side
+// orchestration of the MB-pol kernels.  Declarations and reference citations live in
+// include/mbgpu.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mbgpu.h"
+#include "mbgpu_internal.h"
+
+int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s);
+int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipStream_t s);
+int mb_launch_trimers(MbBuffers& B, int nt, const CellGeom& g, bool gather, hipStream_t s);
+int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s);
+int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const double* vz, const double* ekin,
+                        const double* epot, const double* epot2, const double* w, const double* w2,
+                        const double* sew, double* blk, int nblk_cap, double* out, hipStream_t s);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code ? code : -1;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) return fail((int)e_, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+bool make_geom(int imcon, const double* cell, CellGeom& g)
+{
+    if (imcon < 1 || imcon > 3) return false;
+    g.imcon = imcon;
+    for (int i = 0; i < 9; ++i) g.c[i] = cell[i];
+    g.r[0] = 1.0 / cell[0];
+    g.r[1] = 1.0 / cell[4];
+    g.r[2] = 1.0 / cell[8];
+    const double* a = cell;
+    double* b = g.rc;
+    b[0] = a[4] * a[8] - a[5] * a[7];
+    b[1] = a[2] * a[7] - a[1] * a[8];
+    b[2] = a[1] * a[5] - a[2] * a[4];
+    b[3] = a[5] * a[6] - a[3] * a[8];
+    b[4] = a[0] * a[8] - a[2] * a[6];
+    b[5] = a[2] * a[3] - a[0] * a[5];
+    b[6] = a[3] * a[7] - a[4] * a[6];
+    b[7] = a[1] * a[6] - a[0] * a[7];
+    b[8] = a[0] * a[4] - a[1] * a[3];
+    const double det = a[0] * b[0] + a[3] * b[1] + a[6] * b[2];
+    const double r = det != 0.0 ? 1.0 / det : 0.0;
+    for (int i = 0; i < 9; ++i) b[i] *= r;
+    return true;
+}
+
+}  // namespace
+
+struct mbgpu_ctx {
+    MbBuffers B;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // host-side staging for the host-pointer entry points
+    double *hx = nullptr, *hy = nullptr, *hz = nullptr;   // device copies of positions
+    int* hlist = nullptr;
+    long nd = 0, nt = 0;
+    CellGeom geom{};
+    bool have_inputs = false;
+    // heat-flux scratch
+    double *dv = nullptr, *dek = nullptr, *dep = nullptr, *dw = nullptr, *dsew = nullptr, *dout = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+};
+
+namespace {
+
+int alloc_fixed(mbgpu_ctx* c)
+{
+    MbBuffers& B = c->B;
+    const int nw = B.nw, na = B.natms;
+    B.cap2 = 128;
+    B.cap3 = 96;
+    HIPCHK(dalloc(&B.mon, (size_t)9 * nw));
+    HIPCHK(dalloc(&B.nbr2, (size_t)nw * B.cap2));
+    HIPCHK(dalloc(&B.cnt2, nw));
+    HIPCHK(dalloc(&B.off2, nw));
+    HIPCHK(dalloc(&B.nbr3, (size_t)nw * B.cap3));
+    HIPCHK(dalloc(&B.cnt3, nw));
+    HIPCHK(dalloc(&B.tcnt, nw));
+    HIPCHK(dalloc(&B.toff, nw));
+    HIPCHK(dalloc(&B.counts, 4));
+    HIPCHK(dalloc(&B.acc_f, (size_t)13 * na));
+    B.acc_e = B.acc_f + 3 * (size_t)na;
+    B.acc_w = B.acc_f + 4 * (size_t)na;
+    HIPCHK(dalloc(&B.scal, 16));
+    HIPCHK(dalloc(&B.atomic_energy, na));
+    HIPCHK(dalloc(&B.heat_w, (size_t)9 * na));
+    HIPCHK(dalloc(&B.fout, (size_t)3 * na));
+    HIPCHK(dalloc(&c->hx, na));
+    HIPCHK(dalloc(&c->hy, na));
+    HIPCHK(dalloc(&c->hz, na));
+    HIPCHK(dalloc(&c->hlist, (size_t)3 * nw));
+    HIPCHK(dalloc(&c->dv, (size_t)3 * na));
+    HIPCHK(dalloc(&c->dek, na));
+    HIPCHK(dalloc(&c->dep, na));
+    HIPCHK(dalloc(&c->dw, (size_t)9 * na));
+    HIPCHK(dalloc(&c->dsew, (size_t)9 * na));
+    HIPCHK(dalloc(&c->dout, 16));
+    // coefficient streams
+    const int l2 = pip2b_stream_len(), l3 = pip3b_stream_len();
+    HIPCHK(dalloc(&B.cs2, l2));
+    HIPCHK(dalloc(&B.cs3, l3));
+    HIPCHK(hipMemcpy(B.cs2, pip2b_stream_values(), sizeof(double) * l2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B.cs3, pip3b_stream_values(), sizeof(double) * l3, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
+{
+    MbBuffers& B = c->B;
+    if (nd > B.dcap) {
+        dfree(B.dimers);
+        dfree(B.clus2);
+        dfree(B.xv2);
+        dfree(B.part2);
+        dfree(B.e2);
+        dfree(B.g2);
+        B.dcap = std::max<long>(nd + nd / 2, 1024);
+        HIPCHK(dalloc(&B.dimers, (size_t)2 * B.dcap));
+        HIPCHK(dalloc(&B.clus2, (size_t)18 * B.dcap));
+        HIPCHK(dalloc(&B.xv2, (size_t)31 * B.dcap));
+        HIPCHK(dalloc(&B.part2, (size_t)pip2b_nslices() * 32 * B.dcap));
+        HIPCHK(dalloc(&B.e2, B.dcap));
+        HIPCHK(dalloc(&B.g2, (size_t)18 * B.dcap));
+    }
+    if (nt > B.tcap) {
+        dfree(B.trimers);
+        dfree(B.clus3);
+        dfree(B.xv3);
+        dfree(B.part3);
+        dfree(B.e3);
+        dfree(B.g3);
+        B.tcap = std::max<long>(nt + nt / 2, 1024);
+        HIPCHK(dalloc(&B.trimers, (size_t)3 * B.tcap));
+        HIPCHK(dalloc(&B.clus3, (size_t)27 * B.tcap));
+        HIPCHK(dalloc(&B.xv3, (size_t)36 * B.tcap));
+        HIPCHK(dalloc(&B.part3, (size_t)pip3b_nslices() * 37 * B.tcap));
+        HIPCHK(dalloc(&B.e3, B.tcap));
+        HIPCHK(dalloc(&B.g3, (size_t)27 * B.tcap));
+    }
+    const int nblk = (int)((B.dcap + 255) / 256 + (B.tcap + 255) / 256 + (B.natms + 255) / 256 + 8);
+    if (nblk > B.nblk_cap) {
+        dfree(B.blk);
+        B.nblk_cap = nblk;
+        HIPCHK(dalloc(&B.blk, (size_t)8 * nblk));
+    }
+    return 0;
+}
+
+// Build the lists and size the cluster buffers; returns nd, nt through the context.
+int build_lists(mbgpu_ctx* c)
+{
+    MbBuffers& B = c->B;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        if (B.dcap == 0 && ensure_clusters(c, 1024, 1024)) return -1;
+        int rc = mb_launch_lists(B, c->geom, c->stream);
+        if (rc) return fail(rc, "list kernels failed: %s", hipGetErrorString((hipError_t)rc));
+        int cnt[4];
+        HIPCHK(hipMemcpyAsync(cnt, B.counts, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (cnt[2]) return fail(-2, "neighbour list capacity exceeded (more than %d/%d O-O neighbours)", B.cap2,
+                                B.cap3);
+        c->nd = cnt[0];
+        c->nt = cnt[1];
+        if (c->nd <= B.dcap && c->nt <= B.tcap) return 0;
+        if (ensure_clusters(c, c->nd, c->nt)) return -1;  // lists were truncated: rebuild
+    }
+    return fail(-3, "cluster list sizing did not converge");
+}
+
+int run_clusters(mbgpu_ctx* c, double* fx, double* fy, double* fz)
+{
+    MbBuffers& B = c->B;
+    int rc = mb_launch_dimers(B, (int)c->nd, c->geom, true, c->stream);
+    if (!rc) rc = mb_launch_trimers(B, (int)c->nt, c->geom, true, c->stream);
+    if (!rc) rc = mb_launch_accumulate(B, (int)c->nd, (int)c->nt, fx, fy, fz, c->stream);
+    if (rc) return fail(rc, "cluster kernels failed (%d)", rc);
+    return 0;
+}
+
+int finish_scalars(mbgpu_ctx* c, double* engsrp, double* virsrp, double* stress)
+{
+    double s[16];
+    HIPCHK(hipMemcpyAsync(s, c->B.scal, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (engsrp) *engsrp += s[0] + s[8];
+    if (virsrp) *virsrp += s[1] + s[9];
+    if (stress) {
+        const double t[6] = {s[2] + s[10], s[3] + s[11], s[4] + s[12], s[5] + s[13], s[6] + s[14], s[7] + s[15]};
+        stress[0] += t[0];
+        stress[1] += t[1];
+        stress[2] += t[2];
+        stress[3] += t[1];
+        stress[4] += t[3];
+        stress[5] += t[4];
+        stress[6] += t[2];
+        stress[7] += t[4];
+        stress[8] += t[5];
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- leaf context ----
+std::mutex g_leaf_mu;
+mbgpu_ctx* g_leaf = nullptr;
+
+mbgpu_ctx* leaf_ctx()
+{
+    if (!g_leaf) {
+        g_leaf = mbgpu_create(3, 12);
+        if (g_leaf && ensure_clusters(g_leaf, 1, 1)) {
+            mbgpu_destroy(g_leaf);
+            g_leaf = nullptr;
+        }
+    }
+    return g_leaf;
+}
+
+int leaf_eval(int nmon, const double* const* w, double* E, double* gout)
+{
+    std::lock_guard<std::mutex> lk(g_leaf_mu);
+    mbgpu_ctx* c = leaf_ctx();
+    if (!c) return -1;
+    MbBuffers& B = c->B;
+    // SoA with one cluster: element k of cluster 0 sits at k*nc with nc = 1
+    std::vector<double> soa(9 * nmon);
+    for (int m = 0; m < nmon; ++m)
+        for (int k = 0; k < 9; ++k) soa[9 * m + k] = w[m][k];
+    double* dclus = nmon == 2 ? B.clus2 : B.clus3;
+    HIPCHK(hipMemcpyAsync(dclus, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, c->stream));
+    int rc = nmon == 2 ? mb_launch_dimers(B, 1, c->geom, false, c->stream)
+                       : mb_launch_trimers(B, 1, c->geom, false, c->stream);
+    if (rc) return fail(rc, "leaf kernels failed");
+    HIPCHK(hipMemcpyAsync(E, nmon == 2 ? B.e2 : B.e3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(gout, nmon == 2 ? B.g2 : B.g3, sizeof(double) * 9 * nmon, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI =======
+extern "C" {
+
+const char* mbgpu_last_error(void) { return g_err.c_str(); }
+
+mbgpu_ctx* mbgpu_create(int nw, int natms)
+{
+    if (nw < 1 || natms < 3 * nw) {
+        fail(-1, "mbgpu_create: need nw >= 1 and natms >= 3*nw (got %d, %d)", nw, natms);
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        fail(-1, "mbgpu_create: no HIP device");
+        return nullptr;
+    }
+    mbgpu_ctx* c = new mbgpu_ctx();
+    c->B.nw = nw;
+    c->B.natms = natms;
+    (void)hipGetDevice(&c->device);
+    if (hipStreamCreate(&c->stream) != hipSuccess || alloc_fixed(c) ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
+        std::string msg = g_err;
+        mbgpu_destroy(c);
+        g_err = "mbgpu_create: " + msg;
+        return nullptr;
+    }
+    double cell[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    make_geom(1, cell, c->geom);
+    return c;
+}
+
+void mbgpu_destroy(mbgpu_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    MbBuffers& B = c->B;
+    dfree(B.mon); dfree(B.nbr2); dfree(B.cnt2); dfree(B.off2); dfree(B.nbr3); dfree(B.cnt3);
+    dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
+    dfree(B.clus2); dfree(B.xv2); dfree(B.part2); dfree(B.e2); dfree(B.g2);
+    dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3);
+    dfree(B.acc_f); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
+    dfree(B.cs2); dfree(B.cs3);
+    dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
+    dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev3) (void)hipEventDestroy(c->ev3);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int mbgpu_mbpol_forces_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
+                           const double* x_dev, const double* y_dev, const double* z_dev, double* fx_dev,
+                           double* fy_dev, double* fz_dev, double* engsrp, double* virsrp, double stress[9])
+{
+    if (!c) return fail(-1, "null context");
+    if (nttm2 != 3 * c->B.nw) return fail(-1, "nttm2 = %d does not match the context (3*%d)", nttm2, c->B.nw);
+    if (!make_geom(imcon, cell, c->geom))
+        return fail(-4, "imcon = %d is not supported by image_vectors (reference stops in fixme)", imcon);
+    MbBuffers& B = c->B;
+    B.list = const_cast<int*>(listttm2_dev);
+    B.x = const_cast<double*>(x_dev);
+    B.y = const_cast<double*>(y_dev);
+    B.z = const_cast<double*>(z_dev);
+    c->have_inputs = true;
+    if (build_lists(c)) return -1;
+    if (run_clusters(c, fx_dev, fy_dev, fz_dev)) return -1;
+    return finish_scalars(c, engsrp, virsrp, stress);
+}
+
+int mbgpu_mbpol_forces(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2,
+                       const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
+                       double* fzz, double* engsrp, double* virsrp, double stress[9])
+{
+    if (!c) return fail(-1, "null context");
+    const int na = c->B.natms;
+    if (nttm2 != 3 * c->B.nw) return fail(-1, "nttm2 = %d does not match the context (3*%d)", nttm2, c->B.nw);
+    for (int t = 0; t < nttm2; ++t)
+        if (listttm2[t] < 1 || listttm2[t] + 2 - (t % 3) > na)
+            return fail(-5, "listttm2(%d) = %d outside 1..%d", t + 1, listttm2[t], na);
+    HIPCHK(hipMemcpyAsync(c->hlist, listttm2, sizeof(int) * nttm2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hx, xxx, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hy, yyy, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hz, zzz, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));
+    int rc = mbgpu_mbpol_forces_dev(c, imcon, cell, nttm2, c->hlist, c->hx, c->hy, c->hz, nullptr, nullptr,
+                                    nullptr, engsrp, virsrp, stress);
+    if (rc) return rc;
+    if (fxx || fyy || fzz) {
+        std::vector<double> f((size_t)3 * na);
+        HIPCHK(hipMemcpyAsync(f.data(), c->B.fout, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < na; ++i) {
+            if (fxx) fxx[i] += f[3 * i];
+            if (fyy) fyy[i] += f[3 * i + 1];
+            if (fzz) fzz[i] += f[3 * i + 2];
+        }
+    }
+    return 0;
+}
+
+int mbgpu_get_decomposition(mbgpu_ctx* c, double* ae, double* hw, double* forces)
+{
+    if (!c) return fail(-1, "null context");
+    const int na = c->B.natms;
+    if (ae) HIPCHK(hipMemcpyAsync(ae, c->B.atomic_energy, sizeof(double) * na, hipMemcpyDeviceToHost, c->stream));
+    if (hw) HIPCHK(hipMemcpyAsync(hw, c->B.heat_w, sizeof(double) * 9 * na, hipMemcpyDeviceToHost, c->stream));
+    if (forces)
+        HIPCHK(hipMemcpyAsync(forces, c->B.fout, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_get_lists(mbgpu_ctx* c, int* dimers, long dcap, long* nd, int* trimers, long tcap, long* nt)
+{
+    if (!c) return fail(-1, "null context");
+    if (nd) *nd = c->nd;
+    if (nt) *nt = c->nt;
+    if (dimers && c->nd)
+        HIPCHK(hipMemcpyAsync(dimers, c->B.dimers, sizeof(int) * 2 * std::min(dcap, c->nd), hipMemcpyDeviceToHost,
+                              c->stream));
+    if (trimers && c->nt)
+        HIPCHK(hipMemcpyAsync(trimers, c->B.trimers, sizeof(int) * 3 * std::min(tcap, c->nt),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_heat_flux(mbgpu_ctx* c, const double* vxx, const double* vyy, const double* vzz, const double* e_kin,
+                    const double* e_pot_extra, const double* w_extra, const double* stress_ew1p, double jkin[3],
+                    double jpot[3], double jtot[3])
+{
+    if (!c) return fail(-1, "null context");
+    const int nr = 3 * c->B.nw;
+    double* v = c->dv;
+    HIPCHK(hipMemcpyAsync(v, vxx, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(v + nr, vyy, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(v + 2 * nr, vzz, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dek, e_kin, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    if (e_pot_extra) HIPCHK(hipMemcpyAsync(c->dep, e_pot_extra, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    if (w_extra) HIPCHK(hipMemcpyAsync(c->dw, w_extra, sizeof(double) * 9 * nr, hipMemcpyHostToDevice, c->stream));
+    if (stress_ew1p)
+        HIPCHK(hipMemcpyAsync(c->dsew, stress_ew1p, sizeof(double) * 9 * nr, hipMemcpyHostToDevice, c->stream));
+    if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
+    int rc = mb_launch_heat_flux(nr, v, v + nr, v + 2 * nr, c->dek, c->B.atomic_energy,
+                                 e_pot_extra ? c->dep : nullptr, c->B.heat_w, w_extra ? c->dw : nullptr,
+                                 stress_ew1p ? c->dsew : nullptr, c->B.blk, c->B.nblk_cap, c->dout, c->stream);
+    if (rc) return fail(rc, "heat-flux kernels failed");
+    double out[8];
+    HIPCHK(hipMemcpyAsync(out, c->dout, sizeof out, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) {
+        jkin[a] = out[a];
+        jpot[a] = out[3 + a];
+        jtot[a] = out[a] + out[3 + a];
+    }
+    return 0;
+}
+
+int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const double* vz, const double* e_kin,
+                        const double* e_pot_extra, const double* w_extra, const double* stress_ew1p, double jkin[3],
+                        double jpot[3], double jtot[3])
+{
+    if (!c) return fail(-1, "null context");
+    const int nr = 3 * c->B.nw;
+    if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
+    int rc = mb_launch_heat_flux(nr, vx, vy, vz, e_kin, c->B.atomic_energy, e_pot_extra, c->B.heat_w, w_extra,
+                                 stress_ew1p, c->B.blk, c->B.nblk_cap, c->dout, c->stream);
+    if (rc) return fail(rc, "heat-flux kernels failed");
+    double out[8];
+    HIPCHK(hipMemcpyAsync(out, c->dout, sizeof out, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) {
+        jkin[a] = out[a];
+        jpot[a] = out[3 + a];
+        jtot[a] = out[a] + out[3 + a];
+    }
+    return 0;
+}
+
+int mbgpu_time_forces(mbgpu_ctx* c, int reps, double* ms_per_eval, double* ms_pip3b)
+{
+    if (!c || !c->have_inputs) return fail(-1, "mbgpu_time_forces: run mbgpu_mbpol_forces first");
+    MbBuffers& B = c->B;
+    float total = 0.f, k3 = 0.f;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (build_lists(c)) return -1;
+        int rc = mb_launch_dimers(B, (int)c->nd, c->geom, true, c->stream);
+        if (!rc) rc = mb_launch_trimers(B, (int)c->nt, c->geom, true, c->stream);
+        if (!rc) rc = mb_launch_accumulate(B, (int)c->nd, (int)c->nt, nullptr, nullptr, nullptr, c->stream);
+        if (rc) return fail(rc, "timing run failed");
+        double s[16];
+        HIPCHK(hipMemcpyAsync(s, B.scal, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&total, c->ev0, c->ev1));
+    // the 3-body polynomial kernel alone, same stream
+    HIPCHK(hipEventRecord(c->ev2, c->stream));
+    for (int r = 0; r < reps; ++r) launch_pip3b_f64(B.xv3, B.part3, (int)c->nt, B.cs3, c->stream);
+    HIPCHK(hipEventRecord(c->ev3, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev3));
+    HIPCHK(hipEventElapsedTime(&k3, c->ev2, c->ev3));
+    if (ms_per_eval) *ms_per_eval = total / reps;
+    if (ms_pip3b) *ms_pip3b = k3 / reps;
+    return 0;
+}
+
+// ------------------------------------------------------------------------- leaves ----
+void mbpol_2b_cutoff_(double* r) { *r = MBGPU_R2F; }
+void mbpol_3b_cutoff_(double* r) { *r = MBGPU_R3F; }
+void mbpol_2b_cutoff(double* r) { *r = MBGPU_R2F; }
+void mbpol_3b_cutoff(double* r) { *r = MBGPU_R3F; }
+
+void mbpol_2b_poly_(const double* w1, const double* w2, double* E, double* g1, double* g2, double* force_matrix)
+{
+    (void)force_matrix;
+    const double d[3] = {w1[0] - w2[0], w1[1] - w2[1], w1[2] - w2[2]};
+    if (sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) > MBGPU_R2F) {  // leaf early return
+        *E = 0.0;
+        return;
+    }
+    const double* w[2] = {w1, w2};
+    double g[18];
+    if (leaf_eval(2, w, E, g)) {
+        fprintf(stderr, "mbpol_2b_poly_: %s\n", g_err.c_str());
+        *E = NAN;
+        return;
+    }
+    for (int k = 0; k < 9; ++k) {
+        g1[k] = g[k];
+        g2[k] = g[9 + k];
+    }
+}
+
+void mbpol_3b_poly_(const double* w1, const double* w2, const double* w3, double* E, double* g1, double* g2,
+                    double* g3, double* force_matrix)
+{
+    (void)force_matrix;
+    const double* w[3] = {w1, w2, w3};
+    double g[27];
+    if (leaf_eval(3, w, E, g)) {
+        fprintf(stderr, "mbpol_3b_poly_: %s\n", g_err.c_str());
+        *E = NAN;
+        return;
+    }
+    for (int k = 0; k < 9; ++k) {
+        g1[k] += g[k];
+        g2[k] += g[9 + k];
+        g3[k] += g[18 + k];
+    }
+}
+
+void mbpol_2b_poly(const double* w1, const double* w2, double* E, double* g1, double* g2, double* fm)
+{
+    mbpol_2b_poly_(w1, w2, E, g1, g2, fm);
+}
+
+void mbpol_3b_poly(const double* w1, const double* w2, const double* w3, double* E, double* g1, double* g2,
+                   double* g3, double* fm)
+{
+    mbpol_3b_poly_(w1, w2, w3, E, g1, g2, g3, fm);
+}
+
+}  // extern "C"

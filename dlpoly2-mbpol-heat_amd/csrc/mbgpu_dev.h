@@ -1,0 +1,64 @@
+// mbgpu_dev.h -- device-side helpers shared by the MB-pol kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MBGPU_ENGUNIT 418.4             // kcal/mol -> 10 J/mol (mbpol.f90:37)
+#define MBGPU_R2F 6.5                   // 2B cutoff (mbpol-2b-poly.cpp:1863)
+#define MBGPU_R3F 4.5                   // 3B cutoff (mbpol-3b-poly.cpp:1489)
+
+// Per-atom sums are accumulated in 64-bit fixed point so that the result does not depend
+// on the order in which clusters finish (integer addition is associative).
+#define MBGPU_FX_SCALE 4294967296.0     // 2^32: forces and per-atom energies
+#define MBGPU_FXW_SCALE 16777216.0      // 2^24: per-atom heat-flux tensors (larger range)
+
+struct CellGeom {
+    int imcon;          // 1 cubic, 2 orthorhombic, 3 general (mbpol.f90:1319-1357)
+    double c[9];        // cell vectors, Fortran cell(1..9)
+    double rc[9];       // inverse (imcon 3)
+    double r[3];        // 1/cell(1), 1/cell(5), 1/cell(9)
+};
+
+// Minimum image of one vector exactly as image_vectors (mbpol.f90:1305-1361):
+// d - L*nint(d*(1/L)), nint rounding half away from zero, no FMA contraction.
+__device__ __forceinline__ void image_vec(const CellGeom& g, double& dx, double& dy, double& dz)
+{
+#pragma clang fp contract(off)
+    if (g.imcon == 1) {
+        dx = dx - g.c[0] * round(dx * g.r[0]);
+        dy = dy - g.c[0] * round(dy * g.r[0]);
+        dz = dz - g.c[0] * round(dz * g.r[0]);
+    } else if (g.imcon == 2) {
+        dx = dx - g.c[0] * round(dx * g.r[0]);
+        dy = dy - g.c[4] * round(dy * g.r[1]);
+        dz = dz - g.c[8] * round(dz * g.r[2]);
+    } else {
+        const double ssx = g.rc[0] * dx + g.rc[3] * dy + g.rc[6] * dz;
+        const double ssy = g.rc[1] * dx + g.rc[4] * dy + g.rc[7] * dz;
+        const double ssz = g.rc[2] * dx + g.rc[5] * dy + g.rc[8] * dz;
+        const double xss = ssx - round(ssx), yss = ssy - round(ssy), zss = ssz - round(ssz);
+        dx = g.c[0] * xss + g.c[3] * yss + g.c[6] * zss;
+        dy = g.c[1] * xss + g.c[4] * yss + g.c[7] * zss;
+        dz = g.c[2] * xss + g.c[5] * yss + g.c[8] * zss;
+    }
+}
+
+// Squared length summed in the reference's order ((x^2 + y^2) + z^2), no contraction.
+__device__ __forceinline__ double norm2_nc(double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    const double a = x * x;
+    const double b = y * y;
+    const double c = z * z;
+    return (a + b) + c;
+}
+
+__device__ __forceinline__ long long to_fx(double v, double scale)
+{
+    return __double2ll_rn(v * scale);
+}
+
+__device__ __forceinline__ void fx_add(long long* p, double v, double scale)
+{
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)to_fx(v, scale));
+}

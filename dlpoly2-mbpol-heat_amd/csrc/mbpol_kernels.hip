@@ -1,0 +1,742 @@
+// mbpol_kernels.hip -- MB-pol 2B/3B force path on gfx950: monomer imaging, dimer and
+// trimer enumeration, cluster set-up (extra points and polynomial variables), chain rule
+// and switching, per-atom accumulation of forces / energies / heat-flux tensors.
+//
+// Reference semantics restated (file:line in the reference tree):
+//   image_monomers   mbpol.f90:1275-1301     k_image_monomers
+//   pair predicate   mbpol.f90:496-503       k_neighbours (|img(O_i-O_j)|^2 <= r2f^2)
+//   triple predicate mbpol.f90:843-869       k_trimers (some monomer within r3f of both)
+//   re-imaging       mbpol.f90:505-509,875-882  k_gather2 / k_gather3
+//   2B leaf          mbpol-2b-poly.cpp:358-682  k_setup2 + pip2b + k_finish2
+//   3B leaf          mbpol-3b-poly.cpp:148-312  k_setup3 + pip3b + k_finish3
+//   accumulation     mbpol.f90:519-653, 896-1067  k_accum2 / k_accum3
+//   heat-current bookkeeping  heatcurrent.f90 update_energy_mbpol / update_forces, contracted
+//                    per atom as W_i = sum_j F_ij (x) (x_i - x_j) (heatcurrent.f90:716-719)
+#include <stdio.h>
+
+#include "mbgpu_internal.h"
+
+namespace {
+
+// ---------------------------------------------------------------- model constants ----
+// 2B (mbpol-2b-poly.cpp:1848-1863)
+__constant__ double c2_k[8] = {
+    -6.480884773303821e-01,  // 0 HH intra
+    1.674518993682975e+00,   // 1 OH intra
+    1.148231864355956e+00,   // 2 HH coul
+    1.205989761123099e+00,   // 3 OH coul
+    1.395357065790959e+00,   // 4 OO coul
+    7.347036852042255e-01,   // 5 XH main
+    7.998249864422826e-01,   // 6 XO main
+    7.960663960630585e-01};  // 7 XX main
+#define IN_PLANE_G (-9.721486914088159e-02)
+#define OUT_PLANE_G (9.859272078406150e-02)
+#define R2I 4.5
+
+// the 31 variables (mbpol-2b-poly.cpp:413-445): kind (0 exp, 1 exp/r), k index, d0, sites
+// sites: 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Xa1 7 Xa2 8 Xb1 9 Xb2
+struct V2Def { signed char kind, k, d0, s1, s2; };
+__constant__ V2Def c2_vars[31] = {
+    {0, 0, 1, 1, 2}, {0, 0, 1, 4, 5}, {0, 1, 1, 0, 1}, {0, 1, 1, 0, 2}, {0, 1, 1, 3, 4},
+    {0, 1, 1, 3, 5}, {1, 2, 4, 1, 4}, {1, 2, 4, 1, 5}, {1, 2, 4, 2, 4}, {1, 2, 4, 2, 5},
+    {1, 3, 4, 0, 4}, {1, 3, 4, 0, 5}, {1, 3, 4, 3, 1}, {1, 3, 4, 3, 2}, {1, 4, 4, 0, 3},
+    {0, 5, 4, 6, 4}, {0, 5, 4, 6, 5}, {0, 5, 4, 7, 4}, {0, 5, 4, 7, 5}, {0, 5, 4, 8, 1},
+    {0, 5, 4, 8, 2}, {0, 5, 4, 9, 1}, {0, 5, 4, 9, 2}, {0, 6, 4, 0, 8}, {0, 6, 4, 0, 9},
+    {0, 6, 4, 3, 6}, {0, 6, 4, 3, 7}, {0, 7, 4, 6, 8}, {0, 7, 4, 6, 9}, {0, 7, 4, 7, 8},
+    {0, 7, 4, 7, 9}};
+
+// 3B (mbpol-3b-poly.cpp:1488-1504): k and r0 per variable class
+// class 0 HH intra, 1 OH intra, 2 HH, 3 OH, 4 OO
+__constant__ double c3_k[5] = {-2.254303257872797e+00, -2.564771901404151e-01, 4.247920544074333e-01,
+                               8.128985941165371e-01, 3.749457984616480e-02};
+__constant__ double c3_r0[5] = {1.690594510379166e+00, -2.999999868517452e+00, 3.499031358429095e+00,
+                                4.854042963514281e+00, 4.816312044947604e-08};
+// the 36 variables (mbpol-3b-poly.cpp:170-206): class, atom 1, atom 2
+// atoms: 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Oc 7 Hc1 8 Hc2
+struct V3Def { signed char cls, a1, a2; };
+__constant__ V3Def c3_vars[36] = {
+    {0, 1, 2}, {0, 4, 5}, {0, 7, 8}, {1, 0, 1}, {1, 0, 2}, {1, 3, 4}, {1, 3, 5}, {1, 6, 7},
+    {1, 6, 8}, {2, 1, 4}, {2, 1, 5}, {2, 1, 7}, {2, 1, 8}, {2, 2, 4}, {2, 2, 5}, {2, 2, 7},
+    {2, 2, 8}, {2, 4, 7}, {2, 4, 8}, {2, 5, 7}, {2, 5, 8}, {3, 0, 4}, {3, 0, 5}, {3, 0, 7},
+    {3, 0, 8}, {3, 3, 1}, {3, 3, 2}, {3, 3, 7}, {3, 3, 8}, {3, 6, 1}, {3, 6, 2}, {3, 6, 4},
+    {3, 6, 5}, {4, 0, 3}, {4, 0, 6}, {4, 3, 6}};
+
+// ------------------------------------------------------------------ small helpers ----
+__device__ __forceinline__ void load_mon(const double* __restrict__ mon, int w, double* m)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = mon[9 * w + k];
+}
+
+// monomer::setup (mbpol-2b-poly.cpp:118-140)
+__device__ __forceinline__ void extra_points(const double* ohh, double* oh1, double* oh2, double* x1,
+                                             double* x2)
+{
+    for (int i = 0; i < 3; ++i) {
+        oh1[i] = ohh[i + 3] - ohh[i];
+        oh2[i] = ohh[i + 6] - ohh[i];
+    }
+    const double v[3] = {oh1[1] * oh2[2] - oh1[2] * oh2[1], oh1[2] * oh2[0] - oh1[0] * oh2[2],
+                         oh1[0] * oh2[1] - oh1[1] * oh2[0]};
+    for (int i = 0; i < 3; ++i) {
+        const double in_plane = ohh[i] + 0.5 * IN_PLANE_G * (oh1[i] + oh2[i]);
+        const double out_of_plane = OUT_PLANE_G * v[i];
+        x1[i] = in_plane + out_of_plane;
+        x2[i] = in_plane - out_of_plane;
+    }
+}
+
+// monomer::grads (mbpol-2b-poly.cpp:144-176)
+__device__ __forceinline__ void extra_point_grads(const double* oh1, const double* oh2, const double* g1,
+                                                  const double* g2, double* grd)
+{
+    const double gm[3] = {g1[0] - g2[0], g1[1] - g2[1], g1[2] - g2[2]};
+    const double t1[3] = {oh2[1] * gm[2] - oh2[2] * gm[1], oh2[2] * gm[0] - oh2[0] * gm[2],
+                          oh2[0] * gm[1] - oh2[1] * gm[0]};
+    const double t2[3] = {oh1[1] * gm[2] - oh1[2] * gm[1], oh1[2] * gm[0] - oh1[0] * gm[2],
+                          oh1[0] * gm[1] - oh1[1] * gm[0]};
+    for (int i = 0; i < 3; ++i) {
+        const double gsum = g1[i] + g2[i];
+        const double in_plane = 0.5 * IN_PLANE_G * gsum;
+        const double gh1 = in_plane + OUT_PLANE_G * t1[i];
+        const double gh2 = in_plane - OUT_PLANE_G * t2[i];
+        grd[i + 0] += gsum - (gh1 + gh2);
+        grd[i + 3] += gh1;
+        grd[i + 6] += gh2;
+    }
+}
+
+// f_switch (mbpol-2b-poly.cpp:340-354, mbpol-3b-poly.cpp:130-144)
+__device__ __forceinline__ double f_switch(double r, double ri, double rf, double& g)
+{
+    if (r > rf) {
+        g = 0.0;
+        return 0.0;
+    } else if (r > ri) {
+        const double t1 = M_PI / (rf - ri);
+        const double x = (r - ri) * t1;
+        g = -sin(x) * t1 / 2.0;
+        return (1.0 + cos(x)) / 2.0;
+    }
+    g = 0.0;
+    return 1.0;
+}
+
+// 10 sites (30 coordinates) of a dimer: O,H,H of both waters then the 4 extra points
+__device__ __forceinline__ void dimer_sites(const double* w, double* s, double* oha1, double* oha2,
+                                           double* ohb1, double* ohb2)
+{
+    for (int k = 0; k < 18; ++k) s[k] = w[k];
+    extra_points(s + 0, oha1, oha2, s + 18, s + 21);
+    extra_points(s + 9, ohb1, ohb2, s + 24, s + 27);
+}
+
+__device__ __forceinline__ void block_reduce_store(double* vals, int n, double* out)
+{
+    // vals: per-thread values; wavefront then block reduction in a fixed order
+    __shared__ double red[8][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int q = 0; q < n; ++q) {
+        double v = vals[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wid][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < n) {
+        double s = 0.0;
+        const int nwv = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nwv; ++w) s += red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+}  // namespace
+
+// ===================================================================== kernels ======
+
+__global__ void k_zero_ll(long long* __restrict__ p, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ void k_zero_i(int* __restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// image_monomers (mbpol.f90:1275-1301) on the waters gathered through listttm2
+__global__ void k_image_monomers(int nw, const int* __restrict__ list, const double* __restrict__ x,
+                                 const double* __restrict__ y, const double* __restrict__ z, CellGeom g,
+                                 double* __restrict__ mon)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    double m[9];
+    for (int a = 0; a < 3; ++a) {
+        const int id = list[3 * w + a] - 1;
+        m[3 * a] = x[id];
+        m[3 * a + 1] = y[id];
+        m[3 * a + 2] = z[id];
+    }
+    for (int h = 1; h < 3; ++h) {
+        double dx = m[3 * h] - m[0], dy = m[3 * h + 1] - m[1], dz = m[3 * h + 2] - m[2];
+        image_vec(g, dx, dy, dz);
+        m[3 * h] = m[0] + dx;
+        m[3 * h + 1] = m[1] + dy;
+        m[3 * h + 2] = m[2] + dz;
+    }
+    for (int k = 0; k < 9; ++k) mon[9 * w + k] = m[k];
+}
+
+// O-O neighbour lists.  One wavefront per water i scans all partners in chunks of 64; a
+// ballot + popcount compacts the hits in ascending partner order.
+//   list 2: j > i with |img(O_i - O_j)|^2 <= r2f^2          (do_dimer predicate)
+//   list 3: j != i with sqrt(|img(O_lo - O_hi)|^2) <= r3f    (do_trimer edge test)
+__global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restrict__ mon, CellGeom g,
+                                                   int cap2, int* __restrict__ nbr2, int* __restrict__ cnt2,
+                                                   int cap3, int* __restrict__ nbr3, int* __restrict__ cnt3,
+                                                   int* __restrict__ err)
+{
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (i >= nw) return;
+    int n2 = 0, n3 = 0;
+    for (int base = 0; base < nw; base += 64) {
+        const int j = base + lane;
+        bool in2 = false, in3 = false;
+        if (j < nw && j != i) {
+            const int lo = min(i, j), hi = max(i, j);
+            double dx = mon[9 * lo] - mon[9 * hi];
+            double dy = mon[9 * lo + 1] - mon[9 * hi + 1];
+            double dz = mon[9 * lo + 2] - mon[9 * hi + 2];
+            image_vec(g, dx, dy, dz);
+            const double r2 = norm2_nc(dx, dy, dz);
+            in2 = (j > i) && !(r2 > MBGPU_R2F * MBGPU_R2F);
+            in3 = !(sqrt(r2) > MBGPU_R3F);
+        }
+        const unsigned long long b2 = __ballot(in2), b3 = __ballot(in3);
+        const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (in2) {
+            const int p = n2 + __popcll(b2 & below);
+            if (p < cap2) nbr2[(long)i * cap2 + p] = j;
+        }
+        if (in3) {
+            const int p = n3 + __popcll(b3 & below);
+            if (p < cap3) nbr3[(long)i * cap3 + p] = j;
+        }
+        n2 += __popcll(b2);
+        n3 += __popcll(b3);
+    }
+    if (lane == 0) {
+        cnt2[i] = n2;
+        cnt3[i] = n3;
+        if (n2 > cap2 || n3 > cap3) atomicOr(err, 1);
+    }
+}
+
+// Trimer enumeration around a centre water c: every pair a < b of c's short neighbours
+// forms a triple with two short edges (c-a, c-b).  A triple whose third edge is short too
+// is reached from all three vertices and kept only from its smallest index.  This yields
+// exactly the triples do_trimer keeps: at least two of the three O-O edges <= r3f.
+template <bool FILL>
+__global__ void k_trimers(int nw, const double* __restrict__ mon, CellGeom g, int cap3,
+                          const int* __restrict__ nbr3, const int* __restrict__ cnt3, int* __restrict__ tcnt,
+                          const int* __restrict__ toff, int* __restrict__ trimers, long tcap)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nw) return;
+    const int n = min(cnt3[c], cap3);
+    const int* S = nbr3 + (long)c * cap3;
+    int cnt = 0;
+    long o = FILL ? toff[c] : 0;
+    for (int p = 0; p < n; ++p) {
+        const int a = S[p];
+        for (int q = p + 1; q < n; ++q) {
+            const int b = S[q];
+            double dx = mon[9 * a] - mon[9 * b];
+            double dy = mon[9 * a + 1] - mon[9 * b + 1];
+            double dz = mon[9 * a + 2] - mon[9 * b + 2];
+            image_vec(g, dx, dy, dz);
+            const double rab = sqrt(norm2_nc(dx, dy, dz));
+            const bool keep = (rab > MBGPU_R3F) || (c < a);
+            if (!keep) continue;
+            if (FILL) {
+                int i0, i1, i2;
+                if (c < a) { i0 = c; i1 = a; i2 = b; }
+                else if (c < b) { i0 = a; i1 = c; i2 = b; }
+                else { i0 = a; i1 = b; i2 = c; }
+                const long t = o + cnt;
+                if (t < tcap) {
+                    trimers[3 * t] = i0;
+                    trimers[3 * t + 1] = i1;
+                    trimers[3 * t + 2] = i2;
+                }
+            }
+            ++cnt;
+        }
+    }
+    if (!FILL) tcnt[c] = cnt;
+}
+
+// Exclusive prefix sum of n counts (single block); total written to *total.
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ cnt, int n, int* __restrict__ off,
+                                               int* __restrict__ total)
+{
+    __shared__ long part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + blockDim.x - 1) / blockDim.x;
+    const int b = t * per, e = min(n, b + per);
+    long s = 0;
+    for (int i = b; i < e; ++i) s += cnt[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        long acc = 0;
+        for (int i = 0; i < (int)blockDim.x; ++i) {
+            const long v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        *total = (int)acc;
+    }
+    __syncthreads();
+    long acc = part[t];
+    for (int i = b; i < e; ++i) {
+        off[i] = (int)acc;
+        acc += cnt[i];
+    }
+}
+
+__global__ void k_fill_dimers(int nw, int cap2, const int* __restrict__ nbr2, const int* __restrict__ cnt2,
+                              const int* __restrict__ off2, int* __restrict__ dimers, long dcap)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nw) return;
+    const int n = min(cnt2[i], cap2);
+    for (int q = 0; q < n; ++q) {
+        const long d = (long)off2[i] + q;
+        if (d < dcap) {
+            dimers[2 * d] = i;
+            dimers[2 * d + 1] = nbr2[(long)i * cap2 + q];
+        }
+    }
+}
+
+// Re-image a dimer around its second water (mbpol.f90:496-509); SoA [18][nd].
+__global__ void k_gather2(int nd, const int* __restrict__ dimers, const double* __restrict__ mon, CellGeom g,
+                          double* __restrict__ clus)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    double w1[9], w2[9];
+    load_mon(mon, dimers[2 * d], w1);
+    load_mon(mon, dimers[2 * d + 1], w2);
+    double r[3] = {w1[0] - w2[0], w1[1] - w2[1], w1[2] - w2[2]};
+    image_vec(g, r[0], r[1], r[2]);
+    for (int k = 0; k < 3; ++k) {
+        w1[3 + k] = (w1[3 + k] - w1[k]) + (w2[k] + r[k]);
+        w1[6 + k] = (w1[6 + k] - w1[k]) + (w2[k] + r[k]);
+        w1[k] = w2[k] + r[k];
+    }
+    for (int k = 0; k < 9; ++k) {
+        clus[(long)k * nd + d] = w1[k];
+        clus[(long)(9 + k) * nd + d] = w2[k];
+    }
+}
+
+// Re-image a trimer around its middle water (mbpol.f90:843-882); SoA [27][nt].
+__global__ void k_gather3(int nt, const int* __restrict__ trimers, const double* __restrict__ mon, CellGeom g,
+                          double* __restrict__ clus)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double w1[9], w2[9], w3[9];
+    load_mon(mon, trimers[3 * t], w1);
+    load_mon(mon, trimers[3 * t + 1], w2);
+    load_mon(mon, trimers[3 * t + 2], w3);
+    double ra[3] = {w1[0] - w2[0], w1[1] - w2[1], w1[2] - w2[2]};
+    double rb[3] = {w2[0] - w3[0], w2[1] - w3[1], w2[2] - w3[2]};
+    image_vec(g, ra[0], ra[1], ra[2]);
+    image_vec(g, rb[0], rb[1], rb[2]);
+    for (int k = 0; k < 3; ++k) {
+        w1[3 + k] = (w1[3 + k] - w1[k]) + (w2[k] + ra[k]);
+        w1[6 + k] = (w1[6 + k] - w1[k]) + (w2[k] + ra[k]);
+        w1[k] = w2[k] + ra[k];
+        w3[3 + k] = (w3[3 + k] - w3[k]) + (w2[k] - rb[k]);
+        w3[6 + k] = (w3[6 + k] - w3[k]) + (w2[k] - rb[k]);
+        w3[k] = w2[k] - rb[k];
+    }
+    for (int k = 0; k < 9; ++k) {
+        clus[(long)k * nt + t] = w1[k];
+        clus[(long)(9 + k) * nt + t] = w2[k];
+        clus[(long)(18 + k) * nt + t] = w3[k];
+    }
+}
+
+// 2B variables (mbpol-2b-poly.cpp:358-445): SoA [31][nd]
+__global__ void k_setup2(int nd, const double* __restrict__ clus, double* __restrict__ xv)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
+    for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * nd + d];
+    dimer_sites(w, s, a1, a2, b1, b2);
+    for (int n = 0; n < 31; ++n) {
+        const V2Def v = c2_vars[n];
+        const double dx = s[3 * v.s1] - s[3 * v.s2], dy = s[3 * v.s1 + 1] - s[3 * v.s2 + 1],
+                     dz = s[3 * v.s1 + 2] - s[3 * v.s2 + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double e = exp(c2_k[v.k] * ((double)v.d0 - r));
+        xv[(long)n * nd + d] = v.kind ? e * (1.0 / r) : e;
+    }
+}
+
+// Sum the slice partials, apply the chain rule through the variables and extra points and
+// the switch (mbpol-2b-poly.cpp:447-682).  Record: E (kcal/mol) and dE/dx of the 6 atoms.
+__global__ void k_finish2(int nd, int nsl, const double* __restrict__ clus, const double* __restrict__ xv,
+                          const double* __restrict__ part, double* __restrict__ erec, double* __restrict__ grec)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= nd) return;
+    double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
+    for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * nd + d];
+    const double dOO[3] = {w[0] - w[9], w[1] - w[10], w[2] - w[11]};
+    const double rOO = sqrt(dOO[0] * dOO[0] + dOO[1] * dOO[1] + dOO[2] * dOO[2]);
+    if (rOO > MBGPU_R2F) {  // the leaf returns 0 without touching the gradients
+        erec[d] = 0.0;
+        for (int k = 0; k < 18; ++k) grec[(long)k * nd + d] = 0.0;
+        return;
+    }
+    dimer_sites(w, s, a1, a2, b1, b2);
+    double E = 0.0;
+    for (int q = 0; q < nsl; ++q) E += part[((long)q * 32) * nd + d];
+    double xg[30];
+    for (int k = 0; k < 30; ++k) xg[k] = 0.0;
+    for (int n = 0; n < 31; ++n) {
+        double gn = 0.0;
+        for (int q = 0; q < nsl; ++q) gn += part[((long)q * 32 + 1 + n) * nd + d];
+        const V2Def v = c2_vars[n];
+        double dd[3] = {s[3 * v.s1] - s[3 * v.s2], s[3 * v.s1 + 1] - s[3 * v.s2 + 1],
+                        s[3 * v.s1 + 2] - s[3 * v.s2 + 2]};
+        const double r = sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+        const double val = xv[(long)n * nd + d];
+        const double k = c2_k[v.k];
+        double gg;
+        if (v.kind) {
+            const double rinv = 1.0 / r;
+            gg = -(k + rinv) * val * rinv;
+        } else {
+            gg = -k * val / r;
+        }
+        for (int i = 0; i < 3; ++i) {
+            const double t = gn * (dd[i] * gg);
+            xg[3 * v.s1 + i] += t;
+            xg[3 * v.s2 + i] -= t;
+        }
+    }
+    extra_point_grads(a1, a2, xg + 18, xg + 21, xg + 0);
+    extra_point_grads(b1, b2, xg + 24, xg + 27, xg + 9);
+    double gsw;
+    const double sw = f_switch(rOO, R2I, MBGPU_R2F, gsw);
+    double gout[18];
+    for (int i = 0; i < 18; ++i) gout[i] = sw * xg[i];
+    gsw *= E / rOO;
+    for (int i = 0; i < 3; ++i) {
+        const double t = gsw * dOO[i];
+        gout[i] += t;
+        gout[9 + i] -= t;
+    }
+    erec[d] = sw * E;
+    for (int k = 0; k < 18; ++k) grec[(long)k * nd + d] = gout[k];
+}
+
+// 3B variables (mbpol-3b-poly.cpp:170-206): SoA [36][nt]
+__global__ void k_setup3(int nt, const double* __restrict__ clus, double* __restrict__ xv)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double w[27];
+    for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * nt + t];
+    for (int n = 0; n < 36; ++n) {
+        const V3Def v = c3_vars[n];
+        const double dx = w[3 * v.a1] - w[3 * v.a2], dy = w[3 * v.a1 + 1] - w[3 * v.a2 + 1],
+                     dz = w[3 * v.a1 + 2] - w[3 * v.a2 + 2];
+        const double dd = sqrt(dx * dx + dy * dy + dz * dz);
+        xv[(long)n * nt + t] = exp(-c3_k[v.cls] * (dd - c3_r0[v.cls]));
+    }
+}
+
+// mbpol-3b-poly.cpp:208-312: switch s = sab*sac + sab*sbc + sac*sbc, chain rule, switch
+// gradients.  Record: E (kcal/mol) and dE/dx of the 9 atoms.
+__global__ void k_finish3(int nt, int nsl, const double* __restrict__ clus, const double* __restrict__ xv,
+                          const double* __restrict__ part, double* __restrict__ erec, double* __restrict__ grec)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double w[27];
+    for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * nt + t];
+    double E = 0.0;
+    for (int q = 0; q < nsl; ++q) E += part[((long)q * 37) * nt + t];
+    double rab[3], rac[3], rbc[3];
+    for (int n = 0; n < 3; ++n) {
+        rab[n] = w[n] - w[9 + n];
+        rac[n] = w[n] - w[18 + n];
+        rbc[n] = w[9 + n] - w[18 + n];
+    }
+    const double drab = sqrt(rab[0] * rab[0] + rab[1] * rab[1] + rab[2] * rab[2]);
+    const double drac = sqrt(rac[0] * rac[0] + rac[1] * rac[1] + rac[2] * rac[2]);
+    const double drbc = sqrt(rbc[0] * rbc[0] + rbc[1] * rbc[1] + rbc[2] * rbc[2]);
+    double gab, gac, gbc;
+    const double sab = f_switch(drab, 0.0, MBGPU_R3F, gab);
+    const double sac = f_switch(drac, 0.0, MBGPU_R3F, gac);
+    const double sbc = f_switch(drbc, 0.0, MBGPU_R3F, gbc);
+    const double sw = sab * sac + sab * sbc + sac * sbc;
+    double gc[27];
+    for (int k = 0; k < 27; ++k) gc[k] = 0.0;
+    for (int n = 0; n < 36; ++n) {
+        double gn = 0.0;
+        for (int q = 0; q < nsl; ++q) gn += part[((long)q * 37 + 1 + n) * nt + t];
+        gn *= sw;
+        const V3Def v = c3_vars[n];
+        const double dx[3] = {w[3 * v.a1] - w[3 * v.a2], w[3 * v.a1 + 1] - w[3 * v.a2 + 1],
+                              w[3 * v.a1 + 2] - w[3 * v.a2 + 2]};
+        const double dd = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        const double k = c3_k[v.cls];
+        const double gg = -k * gn * xv[(long)n * nt + t] / dd;
+        for (int i = 0; i < 3; ++i) {
+            gc[3 * v.a1 + i] += gg * dx[i];
+            gc[3 * v.a2 + i] -= gg * dx[i];
+        }
+    }
+    gab *= (sac + sbc) * E / drab;
+    gac *= (sab + sbc) * E / drac;
+    gbc *= (sab + sac) * E / drbc;
+    for (int n = 0; n < 3; ++n) {
+        gc[n] += gab * rab[n] + gac * rac[n];
+        gc[9 + n] += gbc * rbc[n] - gab * rab[n];
+        gc[18 + n] -= gac * rac[n] + gbc * rbc[n];
+    }
+    erec[t] = E * sw;
+    for (int k = 0; k < 27; ++k) grec[(long)k * nt + t] = gc[k];
+}
+
+// Per-cluster accumulation (do_dimer mbpol.f90:519-653 / do_trimer :896-1067):
+// forces f -= g*engunit, per-atom energy E*engunit/nat, heat-flux tensor
+// W_a += F_a (x) sum_{b != a}(x_a - x_b) with F_a = -(g_a/nat)*engunit and raw positions,
+// virial sum w.g and stress -sum w (x) g (upper triangle, mbpol.f90:605-653), per block.
+template <int NMON>
+__global__ __launch_bounds__(256) void k_accum(int nc, const int* __restrict__ clist, const int* __restrict__ list,
+                                               const double* __restrict__ clus, const double* __restrict__ erec,
+                                               const double* __restrict__ grec, const double* __restrict__ x,
+                                               const double* __restrict__ y, const double* __restrict__ z,
+                                               long long* __restrict__ acc_f, long long* __restrict__ acc_e,
+                                               long long* __restrict__ acc_w, double* __restrict__ blk, int natms)
+{
+    constexpr int NA = 3 * NMON;
+    constexpr double FRAC = NMON == 2 ? 1.0 / 6.0 : 1.0 / 9.0;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // energy, virial, s11 s12 s13 s22 s23 s33
+    if (c < nc) {
+        const double E = erec[c];
+        int id[NA];
+        for (int m = 0; m < NMON; ++m) {
+            const int o = list[3 * clist[NMON * c + m]] - 1;
+            for (int a = 0; a < 3; ++a) id[3 * m + a] = o + a;
+        }
+        double g[3 * NA], wc[3 * NA];
+        for (int k = 0; k < 3 * NA; ++k) {
+            g[k] = grec[(long)k * nc + c];
+            wc[k] = clus[(long)k * nc + c];
+        }
+        red[0] = E * MBGPU_ENGUNIT;
+        const double epa = FRAC * E * MBGPU_ENGUNIT;
+        double S[3] = {0, 0, 0};
+        double xa[NA][3];
+        for (int a = 0; a < NA; ++a) {
+            xa[a][0] = x[id[a]];
+            xa[a][1] = y[id[a]];
+            xa[a][2] = z[id[a]];
+            S[0] += xa[a][0];
+            S[1] += xa[a][1];
+            S[2] += xa[a][2];
+        }
+        for (int a = 0; a < NA; ++a) {
+            const int i = id[a];
+            fx_add(acc_e + i, epa, MBGPU_FX_SCALE);
+            double fa[3];
+            for (int q = 0; q < 3; ++q) {
+                const double gi = g[3 * a + q] * MBGPU_ENGUNIT;
+                fx_add(acc_f + 3 * i + q, -gi, MBGPU_FX_SCALE);
+                fa[q] = -(FRAC * g[3 * a + q]) * MBGPU_ENGUNIT;
+                red[1] += wc[3 * a + q] * gi;
+            }
+            red[2] -= wc[3 * a] * g[3 * a] * MBGPU_ENGUNIT;
+            red[3] -= wc[3 * a] * g[3 * a + 1] * MBGPU_ENGUNIT;
+            red[4] -= wc[3 * a] * g[3 * a + 2] * MBGPU_ENGUNIT;
+            red[5] -= wc[3 * a + 1] * g[3 * a + 1] * MBGPU_ENGUNIT;
+            red[6] -= wc[3 * a + 1] * g[3 * a + 2] * MBGPU_ENGUNIT;
+            red[7] -= wc[3 * a + 2] * g[3 * a + 2] * MBGPU_ENGUNIT;
+            const double rr[3] = {NA * xa[a][0] - S[0], NA * xa[a][1] - S[1], NA * xa[a][2] - S[2]};
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) fx_add(acc_w + 9 * i + 3 * p + q, fa[p] * rr[q], MBGPU_FXW_SCALE);
+        }
+    }
+    block_reduce_store(red, 8, blk + 8 * blockIdx.x);
+}
+
+// Fixed order sum of nb block partials (8 each) into out[0..7]
+__global__ void k_reduce_blocks(const double* __restrict__ blk, int nb, double* __restrict__ out)
+{
+    const int q = threadIdx.x;
+    if (q >= 8) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += blk[8 * b + q];
+    out[q] = s;
+}
+
+// Convert the fixed-point per-atom sums and add the forces into the caller's arrays.
+__global__ void k_finalize(int natms, const long long* __restrict__ acc_f, const long long* __restrict__ acc_e,
+                           const long long* __restrict__ acc_w, double* __restrict__ fx, double* __restrict__ fy,
+                           double* __restrict__ fz, double* __restrict__ fout, double* __restrict__ ae,
+                           double* __restrict__ hw)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natms) return;
+    const double inv = 1.0 / MBGPU_FX_SCALE, invw = 1.0 / MBGPU_FXW_SCALE;
+    const double f0 = (double)acc_f[3 * i] * inv, f1 = (double)acc_f[3 * i + 1] * inv,
+                 f2 = (double)acc_f[3 * i + 2] * inv;
+    fout[3 * i] = f0;
+    fout[3 * i + 1] = f1;
+    fout[3 * i + 2] = f2;
+    if (fx) {
+        fx[i] += f0;
+        fy[i] += f1;
+        fz[i] += f2;
+    }
+    ae[i] = (double)acc_e[i] * inv;
+    for (int k = 0; k < 9; ++k) hw[9 * i + k] = (double)acc_w[9 * i + k] * invw;
+}
+
+// Heat flux (compute_heat_flux2, heatcurrent.f90:703-728) from per-atom quantities:
+//   J_kin = sum_i (e_kin,i + e_pot,i) v_i
+//   J_pot,b = sum_i sum_a v_i,a W_i[a][b] - (sigma_ew1p,i . v_i)_b (upper triangle of sigma)
+// over the real atoms i < nreal.  Per-block partials, reduced by k_reduce_blocks.
+__global__ __launch_bounds__(256) void k_heat_flux(int nreal, const double* __restrict__ vx,
+                                                   const double* __restrict__ vy, const double* __restrict__ vz,
+                                                   const double* __restrict__ ekin, const double* __restrict__ epot,
+                                                   const double* __restrict__ epot2, const double* __restrict__ w,
+                                                   const double* __restrict__ w2, const double* __restrict__ sew,
+                                                   double* __restrict__ blk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < nreal) {
+        const double v[3] = {vx[i], vy[i], vz[i]};
+        double e = ekin[i] + epot[i];
+        if (epot2) e += epot2[i];
+        for (int a = 0; a < 3; ++a) red[a] = e * v[a];
+        for (int b = 0; b < 3; ++b) {
+            double s = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                s += v[a] * w[9 * i + 3 * a + b];
+                if (w2) s += v[a] * w2[9 * i + 3 * a + b];
+            }
+            red[3 + b] = s;
+        }
+        if (sew) {
+            const double* sg = sew + 9 * i;
+            red[3] -= sg[0] * v[0] + sg[1] * v[1] + sg[2] * v[2];
+            red[4] -= sg[1] * v[0] + sg[4] * v[1] + sg[5] * v[2];
+            red[5] -= sg[2] * v[0] + sg[5] * v[1] + sg[8] * v[2];
+        }
+    }
+    block_reduce_store(red, 8, blk + 8 * blockIdx.x);
+}
+
+// ============================================================ host-side launchers ====
+
+#define CHK(x)                                                   \
+    do {                                                         \
+        hipError_t e_ = (x);                                     \
+        if (e_ != hipSuccess) return (int)e_;                    \
+    } while (0)
+
+int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s)
+{
+    const int nw = B.nw;
+    hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, B.counts, 4);
+    hipLaunchKernelGGL(k_image_monomers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.list, B.x, B.y, B.z, g,
+                       B.mon);
+    hipLaunchKernelGGL(k_neighbours, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap2, B.nbr2, B.cnt2, B.cap3,
+                       B.nbr3, B.cnt3, B.counts + 2);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.cnt2, nw, B.off2, B.counts + 0);
+    hipLaunchKernelGGL(k_fill_dimers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.cap2, B.nbr2, B.cnt2, B.off2,
+                       B.dimers, B.dcap);
+    hipLaunchKernelGGL(k_trimers<false>, dim3((nw + 63) / 64), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3,
+                       B.cnt3, B.tcnt, B.toff, B.trimers, B.tcap);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.tcnt, nw, B.toff, B.counts + 1);
+    hipLaunchKernelGGL(k_trimers<true>, dim3((nw + 63) / 64), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3,
+                       B.cnt3, B.tcnt, B.toff, B.trimers, B.tcap);
+    return (int)hipGetLastError();
+}
+
+int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipStream_t s)
+{
+    if (nd <= 0) return 0;
+    const int nb = (nd + 127) / 128;
+    if (gather) hipLaunchKernelGGL(k_gather2, dim3(nb), dim3(128), 0, s, nd, B.dimers, B.mon, g, B.clus2);
+    hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2);
+    launch_pip2b_f64(B.xv2, B.part2, nd, B.cs2, s);
+    hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, pip2b_nslices(), B.clus2, B.xv2, B.part2, B.e2,
+                       B.g2);
+    return (int)hipGetLastError();
+}
+
+int mb_launch_trimers(MbBuffers& B, int nt, const CellGeom& g, bool gather, hipStream_t s)
+{
+    if (nt <= 0) return 0;
+    const int nb = (nt + 127) / 128;
+    if (gather) hipLaunchKernelGGL(k_gather3, dim3(nb), dim3(128), 0, s, nt, B.trimers, B.mon, g, B.clus3);
+    hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3);
+    launch_pip3b_f64(B.xv3, B.part3, nt, B.cs3, s);
+    hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, pip3b_nslices(), B.clus3, B.xv3, B.part3, B.e3,
+                       B.g3);
+    return (int)hipGetLastError();
+}
+
+// Accumulate the cluster records into per-atom sums and scalars; scal[0..7] = 2B
+// (energy, virial, stress 6), scal[8..15] = 3B.
+int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s)
+{
+    const int na = B.natms;
+    hipLaunchKernelGGL(k_zero_ll, dim3(256), dim3(256), 0, s, B.acc_f, (long)13 * na);
+    const int nb2 = nd > 0 ? (nd + 255) / 256 : 0, nb3 = nt > 0 ? (nt + 255) / 256 : 0;
+    if (nb2 + nb3 > B.nblk_cap) return -1;
+    if (nb2) {
+        hipLaunchKernelGGL(k_accum<2>, dim3(nb2), dim3(256), 0, s, nd, B.dimers, B.list, B.clus2, B.e2, B.g2, B.x,
+                           B.y, B.z, B.acc_f, B.acc_e, B.acc_w, B.blk, na);
+    }
+    if (nb3) {
+        hipLaunchKernelGGL(k_accum<3>, dim3(nb3), dim3(256), 0, s, nt, B.trimers, B.list, B.clus3, B.e3, B.g3,
+                           B.x, B.y, B.z, B.acc_f, B.acc_e, B.acc_w, B.blk + 8 * nb2, na);
+    }
+    hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(B.scal), 32);
+    if (nb2) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, B.blk, nb2, B.scal);
+    if (nb3) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, B.blk + 8 * nb2, nb3, B.scal + 8);
+    hipLaunchKernelGGL(k_finalize, dim3((na + 255) / 256), dim3(256), 0, s, na, B.acc_f, B.acc_e, B.acc_w, fx, fy,
+                       fz, B.fout, B.atomic_energy, B.heat_w);
+    return (int)hipGetLastError();
+}
+
+int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const double* vz, const double* ekin,
+                        const double* epot, const double* epot2, const double* w, const double* w2,
+                        const double* sew, double* blk, int nblk_cap, double* out, hipStream_t s)
+{
+    const int nb = (nreal + 255) / 256;
+    if (nb > nblk_cap) return -1;
+    hipLaunchKernelGGL(k_heat_flux, dim3(nb), dim3(256), 0, s, nreal, vx, vy, vz, ekin, epot, epot2, w, w2, sew,
+                       blk);
+    hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, blk, nb, out);
+    return (int)hipGetLastError();
+}

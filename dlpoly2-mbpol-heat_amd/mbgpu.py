@@ -1,0 +1,219 @@
+"""ctypes binding of libmbgpu.so, the gfx950 MB-pol force / heat-current library.
+
+Host-side mirror of the reference's interface for this path:
+
+  * ``mbpol_2b_poly`` / ``mbpol_3b_poly`` / ``mbpol_2b_cutoff`` / ``mbpol_3b_cutoff``: the
+    C++ leaves (mbpol-2b-poly.cpp:1878-1908, mbpol-3b-poly.cpp:1518-1546);
+  * ``Context.mbpol_forces``: the Fortran ``mbpol_forces`` subroutine (mbpol.f90:98-449)
+    with the same argument meaning and accumulate-into semantics;
+  * ``Context.heat_flux``: ``compute_heat_flux2`` (heatcurrent.f90:703-728).
+
+There is no CPU fallback: loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmbgpu.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+# name -> (restype, argtypes); every function declared in include/mbgpu.h
+_SIGS = {
+    "mbpol_2b_poly_": (None, [_dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbpol_3b_poly_": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbpol_2b_cutoff_": (None, [_dp]),
+    "mbpol_3b_cutoff_": (None, [_dp]),
+    "mbpol_2b_poly": (None, [_dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbpol_3b_poly": (None, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbpol_2b_cutoff": (None, [_dp]),
+    "mbpol_3b_cutoff": (None, [_dp]),
+    "mbgpu_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int]),
+    "mbgpu_destroy": (None, [ctypes.c_void_p]),
+    "mbgpu_last_error": (ctypes.c_char_p, []),
+    "mbgpu_mbpol_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
+                                          _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbgpu_mbpol_forces_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp]),
+    "mbgpu_get_decomposition": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "mbgpu_get_lists": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _ip,
+                                       ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
+    "mbgpu_heat_flux": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbgpu_heat_flux_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp,
+                                           _dp, _dp]),
+    "mbgpu_time_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+}
+
+
+class MbgpuError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load libmbgpu.so (built by __graft_entry__.build()); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MbgpuError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise MbgpuError(f"{what} failed ({rc}): {load_library().mbgpu_last_error().decode()}")
+
+
+# ------------------------------------------------------------------------ leaves ----
+def mbpol_2b_cutoff():
+    r = ctypes.c_double()
+    load_library().mbpol_2b_cutoff_(ctypes.byref(r))
+    return r.value
+
+
+def mbpol_3b_cutoff():
+    r = ctypes.c_double()
+    load_library().mbpol_3b_cutoff_(ctypes.byref(r))
+    return r.value
+
+
+def mbpol_2b_poly(w1, w2, g1=None, g2=None):
+    """E (kcal/mol), g1, g2 (overwritten unless r_OO > 6.5, as the reference leaf)."""
+    w1, w2 = _f64(w1, 9), _f64(w2, 9)
+    g1 = np.zeros(9) if g1 is None else g1
+    g2 = np.zeros(9) if g2 is None else g2
+    e = ctypes.c_double()
+    fm = np.zeros(108)
+    load_library().mbpol_2b_poly_(_p(w1), _p(w2), ctypes.byref(e), _p(g1), _p(g2), _p(fm))
+    return e.value, g1, g2
+
+
+def mbpol_3b_poly(w1, w2, w3, g1=None, g2=None, g3=None):
+    """E (kcal/mol); g1..g3 are accumulated into (zeros if not given), as the reference."""
+    w1, w2, w3 = _f64(w1, 9), _f64(w2, 9), _f64(w3, 9)
+    g1 = np.zeros(9) if g1 is None else g1
+    g2 = np.zeros(9) if g2 is None else g2
+    g3 = np.zeros(9) if g3 is None else g3
+    e = ctypes.c_double()
+    fm = np.zeros(243)
+    load_library().mbpol_3b_poly_(_p(w1), _p(w2), _p(w3), ctypes.byref(e), _p(g1), _p(g2), _p(g3), _p(fm))
+    return e.value, g1, g2, g3
+
+
+# ----------------------------------------------------------------------- context ----
+class Context:
+    """One device context for a system of nw waters and natms sites."""
+
+    def __init__(self, nw, natms):
+        lib = load_library()
+        self.nw, self.natms = nw, natms
+        self._h = lib.mbgpu_create(nw, natms)
+        if not self._h:
+            raise MbgpuError("mbgpu_create failed: " + lib.mbgpu_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().mbgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mbpol_forces(self, imcon, cell, nttm2, listttm2, xxx, yyy, zzz, fxx, fyy, fzz, engsrp=0.0, virsrp=0.0,
+                     stress=None):
+        """mbpol_forces (mbpol.f90:98-449): fxx/fyy/fzz and stress accumulate in place;
+        returns (engsrp, virsrp) with this call's energy and virial added."""
+        cell = _f64(cell, 9)
+        lst = np.ascontiguousarray(listttm2, dtype=np.int32)
+        x, y, z = _f64(xxx, self.natms), _f64(yyy, self.natms), _f64(zzz, self.natms)
+        for f in (fxx, fyy, fzz):
+            if f.dtype != np.float64 or not f.flags.c_contiguous or f.size != self.natms:
+                raise ValueError("force arrays must be contiguous float64 of length natms")
+        stress = np.zeros(9) if stress is None else stress
+        e, v = ctypes.c_double(engsrp), ctypes.c_double(virsrp)
+        rc = load_library().mbgpu_mbpol_forces(self._h, imcon, _p(cell), nttm2, lst.ctypes.data_as(_ip), _p(x),
+                                               _p(y), _p(z), _p(fxx), _p(fyy), _p(fzz), ctypes.byref(e),
+                                               ctypes.byref(v), _p(stress))
+        _check(rc, "mbgpu_mbpol_forces")
+        return e.value, v.value
+
+    def mbpol_forces_dev(self, imcon, cell, list_ptr, x_ptr, y_ptr, z_ptr, fx_ptr=None, fy_ptr=None, fz_ptr=None):
+        """Device-pointer variant (ints are raw device addresses, e.g. torch data_ptr())."""
+        cell = _f64(cell, 9)
+        e, v = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        stress = np.zeros(9)
+        rc = load_library().mbgpu_mbpol_forces_dev(self._h, imcon, _p(cell), 3 * self.nw, list_ptr, x_ptr, y_ptr,
+                                                   z_ptr, fx_ptr, fy_ptr, fz_ptr, ctypes.byref(e), ctypes.byref(v),
+                                                   _p(stress))
+        _check(rc, "mbgpu_mbpol_forces_dev")
+        return e.value, v.value, stress
+
+    def decomposition(self):
+        ae = np.zeros(self.natms)
+        hw = np.zeros(9 * self.natms)
+        f = np.zeros(3 * self.natms)
+        _check(load_library().mbgpu_get_decomposition(self._h, _p(ae), _p(hw), _p(f)), "mbgpu_get_decomposition")
+        return ae, hw.reshape(self.natms, 3, 3), f.reshape(self.natms, 3)
+
+    def lists(self):
+        nd, nt = ctypes.c_long(), ctypes.c_long()
+        lib = load_library()
+        _check(lib.mbgpu_get_lists(self._h, None, 0, ctypes.byref(nd), None, 0, ctypes.byref(nt)), "lists")
+        d = np.zeros((max(nd.value, 1), 2), dtype=np.int32)
+        t = np.zeros((max(nt.value, 1), 3), dtype=np.int32)
+        _check(lib.mbgpu_get_lists(self._h, d.ctypes.data_as(_ip), nd.value, ctypes.byref(nd),
+                                   t.ctypes.data_as(_ip), nt.value, ctypes.byref(nt)), "lists")
+        return d[:nd.value], t[:nt.value]
+
+    def heat_flux(self, vxx, vyy, vzz, e_kin, e_pot_extra=None, w_extra=None, stress_ew1p=None):
+        """compute_heat_flux2 (heatcurrent.f90:703-728) over the real atoms: (jkin, jpot, jtot)."""
+        nr = 3 * self.nw
+        vx, vy, vz = _f64(vxx)[:nr].copy(), _f64(vyy)[:nr].copy(), _f64(vzz)[:nr].copy()
+        ek = _f64(e_kin)[:nr].copy()
+        ep = None if e_pot_extra is None else _f64(e_pot_extra)[:nr].copy()
+        we = None if w_extra is None else _f64(w_extra).reshape(-1)[:9 * nr].copy()
+        se = None if stress_ew1p is None else _f64(stress_ew1p).reshape(-1)[:9 * nr].copy()
+        jk, jp, jt = np.zeros(3), np.zeros(3), np.zeros(3)
+        rc = load_library().mbgpu_heat_flux(self._h, _p(vx), _p(vy), _p(vz), _p(ek), _p(ep), _p(we), _p(se),
+                                            _p(jk), _p(jp), _p(jt))
+        _check(rc, "mbgpu_heat_flux")
+        return jk, jp, jt
+
+    def heat_flux_dev(self, vx_ptr, vy_ptr, vz_ptr, ekin_ptr, epot_ptr=None, w_ptr=None, sew_ptr=None):
+        """Device-pointer variant of heat_flux (raw device addresses)."""
+        jk, jp, jt = np.zeros(3), np.zeros(3), np.zeros(3)
+        rc = load_library().mbgpu_heat_flux_dev(self._h, vx_ptr, vy_ptr, vz_ptr, ekin_ptr, epot_ptr, w_ptr, sew_ptr,
+                                                _p(jk), _p(jp), _p(jt))
+        _check(rc, "mbgpu_heat_flux_dev")
+        return jk, jp, jt
+
+    def time_forces(self, reps):
+        ms, k3 = ctypes.c_double(), ctypes.c_double()
+        _check(load_library().mbgpu_time_forces(self._h, reps, ctypes.byref(ms), ctypes.byref(k3)), "time_forces")
+        return ms.value, k3.value

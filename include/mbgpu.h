@@ -1,0 +1,124 @@
+/*
+ * mbgpu.h -- C ABI of libmbgpu.so, the MI355X (gfx950) MB-pol force / heat-current path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success and a nonzero
+ * status otherwise; mbgpu_last_error() gives a message for the calling thread's last
+ * failure.  Units follow the reference: Angstrom in, DL_POLY internal energy units
+ * (10 J/mol; engunit = 418.4 per kcal/mol) out, except the leaves, which work in kcal/mol
+ * like the reference's C++ leaves.
+ *
+ * Ownership: a context owns its device buffers (created by mbgpu_create, released by
+ * mbgpu_destroy); caller arrays stay the caller's.  One context per host thread.
+ */
+#ifndef MBGPU_H
+#define MBGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mbgpu_ctx mbgpu_ctx;
+
+/* ------------------------------------------------------------------------------------
+ * Leaf entry points: identical signatures and conventions to the reference's C ABI.
+ * Each call evaluates one cluster on the GPU (device 0, the library's own stream).
+ * ---------------------------------------------------------------------------------- */
+
+/* Replaces mbpol_2b_poly_ (mbpol-2b-poly.cpp:1878-1893), called from do_dimer
+ * (mbpol.f90:517).  w1, w2: 9 doubles each (O, H1, H2; x y z; Angstrom), already imaged
+ * into one cluster.  E in kcal/mol; g1, g2 (9 each) are overwritten with dE/dx, except
+ * when r_OO > 6.5 A, where E = 0 and g1, g2 are left untouched (as the reference).
+ * force_matrix (108 doubles) is not written: the reference fills it only partially and
+ * nothing on the active path reads it (mbpol.f90:533-556 uses g instead). */
+void mbpol_2b_poly_(const double* w1, const double* w2, double* E, double* g1, double* g2,
+                    double* force_matrix);
+
+/* Replaces mbpol_3b_poly_ (mbpol-3b-poly.cpp:1518-1531), called from do_trimer
+ * (mbpol.f90:894).  g1..g3 are ACCUMULATED (the caller zeroes them, mbpol.f90:890-892).
+ * force_matrix (243 doubles) is not written (see above). */
+void mbpol_3b_poly_(const double* w1, const double* w2, const double* w3, double* E, double* g1,
+                    double* g2, double* g3, double* force_matrix);
+
+/* Replace mbpol_2b_cutoff_ / mbpol_3b_cutoff_ (mbpol-2b-poly.cpp:1900-1904,
+ * mbpol-3b-poly.cpp:1538-1542), called from mbpol_init (mbpol.f90:64-65): 6.5 and 4.5. */
+void mbpol_2b_cutoff_(double* r);
+void mbpol_3b_cutoff_(double* r);
+
+/* Spellings without the trailing underscore (the reference's -DBGQ build,
+ * mbpol-2b-poly.cpp:1884-1901, mbpol-3b-poly.cpp:1522-1539). */
+void mbpol_2b_poly(const double* w1, const double* w2, double* E, double* g1, double* g2,
+                   double* force_matrix);
+void mbpol_3b_poly(const double* w1, const double* w2, const double* w3, double* E, double* g1,
+                   double* g2, double* g3, double* force_matrix);
+void mbpol_2b_cutoff(double* r);
+void mbpol_3b_cutoff(double* r);
+
+/* ------------------------------------------------------------------------------------
+ * Context (stands in for mbpol_init / mbpol_fini, mbpol.f90:52-94, and init_heat,
+ * heatcurrent.f90:47-110, for the MB-pol part of the decomposition).
+ * ---------------------------------------------------------------------------------- */
+
+/* nw waters, natms sites in the per-atom arrays (4*nw with M sites).  Returns NULL on
+ * failure (no GPU, out of memory, tables missing); see mbgpu_last_error(). */
+mbgpu_ctx* mbgpu_create(int nw, int natms);
+void mbgpu_destroy(mbgpu_ctx* ctx);
+const char* mbgpu_last_error(void);
+
+/* Batched MB-pol forces: stands in for mbpol_forces (mbpol.f90:98-449), called from
+ * dlpoly_cmd.f:2504, single rank.  Host arrays, Fortran semantics:
+ *   listttm2: 3*nttm2/3 1-based atom ids (O, H, H per water; consecutive ids per water);
+ *   x/y/zzz: natms positions (Angstrom);
+ *   f{x,y,z}: += MB-pol forces (internal units);
+ *   engsrp, virsrp: += 2B+3B energy and virial; stress[9]: += stress.
+ * imcon 1, 2, 3 as image_vectors (mbpol.f90:1319-1357); other values return an error
+ * (the reference stops in fixme).  The per-atom energies (E/6 per dimer atom, E/9 per
+ * trimer atom) and the per-atom heat-flux tensors stay on the device for
+ * mbgpu_heat_flux and can be read with mbgpu_get_decomposition. */
+int mbgpu_mbpol_forces(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm2, const int* listttm2,
+                       const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
+                       double* fzz, double* engsrp, double* virsrp, double stress[9]);
+
+/* Same with device pointers (inputs resident in HBM; f{x,y,z} device arrays or NULL). */
+int mbgpu_mbpol_forces_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm2,
+                           const int* listttm2_dev, const double* x_dev, const double* y_dev,
+                           const double* z_dev, double* fx_dev, double* fy_dev, double* fz_dev,
+                           double* engsrp, double* virsrp, double stress[9]);
+
+/* Per-atom results of the last mbgpu_mbpol_forces call (host arrays of natms, natms and
+ * 9*natms, any may be NULL):
+ *   atomic_energy_mbpol  (heatcurrent.f90 update_energy_mbpol sums),
+ *   heat_w[i][a][b] = sum_j F_ij,a (x_i - x_j)_b over the MB-pol force-matrix entries
+ *     (the contraction compute_heat_flux2 makes, heatcurrent.f90:716-719),
+ *   forces: the MB-pol forces alone, 3*natms interleaved. */
+int mbgpu_get_decomposition(mbgpu_ctx* ctx, double* atomic_energy_mbpol, double* heat_w, double* forces);
+
+/* Dimer / trimer sets of the last call (0-based water indices; dimers i<j, trimers
+ * i<j<k, in the order evaluated).  *nd, *nt receive the counts; arrays may be NULL. */
+int mbgpu_get_lists(mbgpu_ctx* ctx, int* dimers, long dcap, long* nd, int* trimers, long tcap, long* nt);
+
+/* Heat flux: stands in for compute_heat_flux2 (heatcurrent.f90:703-728) restricted to the
+ * terms this library computes, plus caller-supplied per-atom terms:
+ *   vxx..vzz  velocities (natms, only the first nreal = 3*nw are used),
+ *   e_kin     per-atom kinetic energy (update_kinetic_energy sums), nreal,
+ *   e_pot_extra  other per-atom potential energies to add (NULL = none), nreal,
+ *   w_extra      other per-atom tensors W_i (9 per atom, NULL = none),
+ *   stress_ew1p  per-atom reciprocal-space stress (9 per atom, NULL = none).
+ * jkin, jpot, jtot receive the three components (internal units, not scaled by junit). */
+int mbgpu_heat_flux(mbgpu_ctx* ctx, const double* vxx, const double* vyy, const double* vzz,
+                    const double* e_kin, const double* e_pot_extra, const double* w_extra,
+                    const double* stress_ew1p, double jkin[3], double jpot[3], double jtot[3]);
+
+/* Same with device pointers (arrays of nreal = 3*nw; extras may be NULL). */
+int mbgpu_heat_flux_dev(mbgpu_ctx* ctx, const double* vx_dev, const double* vy_dev, const double* vz_dev,
+                        const double* e_kin_dev, const double* e_pot_extra_dev, const double* w_extra_dev,
+                        const double* stress_ew1p_dev, double jkin[3], double jpot[3], double jtot[3]);
+
+/* Timing helper: runs the MB-pol force evaluation `reps` times on device-resident inputs
+ * already uploaded by a previous mbgpu_mbpol_forces(_dev) call and returns the mean
+ * milliseconds per evaluation, measured with HIP events on the library's stream. */
+int mbgpu_time_forces(mbgpu_ctx* ctx, int reps, double* ms_per_eval, double* ms_pip3b_kernel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,111 @@
+/*
+ * oracle.h -- CPU restatement of the reference's MB-pol force / heat-current path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load it.  The
+ * product library (libmbgpu.so) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Units follow the
+ * reference: Angstrom, kcal/mol inside the polynomial leaves, DL_POLY internal units
+ * (10 J/mol, engunit = 418.4 per kcal/mol) at the mbpol_forces level.
+ */
+#ifndef MBPOL_ORACLE_H
+#define MBPOL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- polynomial tables (pip_oracle.c) -------------------------------------- */
+
+/* Load the monomial and coefficient tables; returns 0 on success. */
+int ora_pip_load(const char *mono2b, const char *coef2b,
+                 const char *mono3b, const char *coef3b);
+
+/* E(a, x) = sum_i a_i p_i(x) and dE/dx for an arbitrary coefficient vector
+ * (poly_2b_v6x::eval, poly-2b-v6x.h:15-17; poly_3b_v2x::eval, poly-3b-v2x.h:14-16). */
+double ora_poly_eval(int which, const double *a, const double *x, double *g);
+
+/* Sum over monomials of |a_b * monomial(x)| with the fitted coefficients: the magnitude
+ * against which E(x) cancels (used to scale parity tolerances). */
+double ora_poly_scale(int which, const double *x);
+
+/* Leaf restatements: x2b_v6x_p::operator() (mbpol-2b-poly.cpp:358-682) and
+ * x3b_v2x_p::operator() (mbpol-3b-poly.cpp:148-312).  Same conventions as the
+ * reference C ABI: 2B overwrites g1,g2 (unless r_OO > 6.5, then untouched);
+ * 3B accumulates into g1..g3.  Returns E in kcal/mol. */
+double ora_mbpol_2b(const double *w1, const double *w2, double *g1, double *g2);
+double ora_mbpol_3b(const double *w1, const double *w2, const double *w3,
+                    double *g1, double *g2, double *g3);
+
+/* The 31 / 36 polynomial variables of one cluster (for kernel-stage parity). */
+void ora_vars_2b(const double *w1, const double *w2, double *v31);
+void ora_vars_3b(const double *w1, const double *w2, const double *w3, double *v36);
+
+/* ---- mbpol_forces (mbpol_oracle.c) ------------------------------------------ */
+
+typedef struct {
+    /* inputs */
+    int imcon;               /* 1, 2 or 3 (mbpol.f90:1319-1357) */
+    double cell[9];
+    int nw;                  /* number of waters */
+    const int *listttm2;     /* 3*nw, 1-based atom ids O,H,H per water */
+    const double *xxx, *yyy, *zzz;
+    int natms;               /* length of per-atom arrays below */
+    int compat_inplace;      /* 1: keep the reference's in-place re-imaging of the
+                                shared monomer array (mbpol.f90:505-509,875-882) */
+    /* outputs (accumulated, reference semantics) */
+    double *fxx, *fyy, *fzz; /* internal units */
+    double engsrp, virsrp, stress[9];
+    double eng2, eng3;       /* 2B and 3B energies, internal units */
+    double *atomic_energy;   /* natms: += E/6, E/9 (heatcurrent.f90 update_energy_mbpol) */
+    double *heat_w;          /* natms*9 row-major [a][alpha][beta]: per-atom
+                                sum_j F_ij,alpha (x_i - x_j)_beta of the MB-pol force-matrix
+                                entries (the contraction compute_heat_flux2 makes) */
+    double *force_matrix;    /* optional dense natms*natms*3, layout [k][j][i] as
+                                Fortran force_matrix(i,j,k); NULL to skip */
+    /* lists (optional outputs, 0-based water indices) */
+    int *dimers;  long ndimers, cap_dimers;    /* pairs i<j */
+    int *trimers; long ntrimers, cap_trimers;  /* triples i<j<k */
+    int skip_3b;             /* debugging aid: 1 = 2B only */
+} ora_mbpol_args;
+
+/* mbpol_forces (mbpol.f90:98-449) with do_dimer (:453-790) and do_trimer (:795-1271),
+ * single rank (bead_size = 1).  Returns 0, or -1 for an unsupported imcon (the
+ * reference stops in fixme, mbpol.f90:1357). */
+int ora_mbpol_forces(ora_mbpol_args *a);
+
+/* Route the oracle's mbpol_forces through other leaf implementations with the reference
+ * C ABI (e.g. the reference's own compiled mbpol_2b_poly_ / mbpol_3b_poly_); NULL
+ * restores the oracle's leaves. */
+typedef void (*ora_leaf2_fn)(const double *, const double *, double *, double *, double *, double *);
+typedef void (*ora_leaf3_fn)(const double *, const double *, const double *, double *, double *, double *,
+                             double *, double *);
+void ora_set_leaves(ora_leaf2_fn f2, ora_leaf3_fn f3);
+
+/* image_vectors (mbpol.f90:1305-1361) */
+int ora_image_vectors(int imcon, const double *cell, int n, double *dx);
+
+/* ---- heat current (heat_oracle.c) ------------------------------------------ */
+
+/* compute_heat_flux2 (heatcurrent.f90:703-728) from the dense force matrix, layout
+ * [k][j][i] over natms_f = 4*nw sites (3*nw real atoms first).  stress_ew1p is
+ * natms*9 row-major [i][a][b] (NULL = zero). */
+void ora_heat_flux_dense(int natms, int nreal, const double *force_matrix,
+                         const double *x, const double *y, const double *z,
+                         const double *vx, const double *vy, const double *vz,
+                         const double *e_kin, const double *e_pot,
+                         const double *stress_ew1p,
+                         double jkin[3], double jpot[3], double jtot[3]);
+
+/* Same flux from per-atom contracted tensors W_i[a][b] = sum_j F_ij,a (x_i-x_j)_b. */
+void ora_heat_flux_tensor(int nreal, const double *heat_w,
+                          const double *vx, const double *vy, const double *vz,
+                          const double *e_kin, const double *e_pot,
+                          const double *stress_ew1p,
+                          double jkin[3], double jpot[3], double jtot[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
