@@ -117,9 +117,7 @@ int alloc_fixed(mbgpu_ctx* c)
     HIPCHK(dalloc(&B.tcnt, nw));
     HIPCHK(dalloc(&B.toff, nw));
     HIPCHK(dalloc(&B.counts, 4));
-    HIPCHK(dalloc(&B.acc_f, (size_t)13 * na));
-    B.acc_e = B.acc_f + 3 * (size_t)na;
-    B.acc_w = B.acc_f + 4 * (size_t)na;
+    HIPCHK(dalloc(&B.acc, (size_t)26 * na));
     HIPCHK(dalloc(&B.scal, 16));
     HIPCHK(dalloc(&B.atomic_energy, na));
     HIPCHK(dalloc(&B.heat_w, (size_t)9 * na));
@@ -320,7 +318,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
     dfree(B.clus2); dfree(B.xv2); dfree(B.part2); dfree(B.e2); dfree(B.g2);
     dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3);
-    dfree(B.acc_f); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
+    dfree(B.acc); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
     dfree(B.cs2); dfree(B.cs3);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);

@@ -7,10 +7,13 @@
 #define MBGPU_R2F 6.5                   // 2B cutoff (mbpol-2b-poly.cpp:1863)
 #define MBGPU_R3F 4.5                   // 3B cutoff (mbpol-3b-poly.cpp:1489)
 
-// Per-atom sums are accumulated in 64-bit fixed point so that the result does not depend
-// on the order in which clusters finish (integer addition is associative).
-#define MBGPU_FX_SCALE 4294967296.0     // 2^32: forces and per-atom energies
-#define MBGPU_FXW_SCALE 16777216.0      // 2^24: per-atom heat-flux tensors (larger range)
+// Per-atom sums are accumulated in fixed point so that the result does not depend on the
+// order in which clusters finish (integer addition is associative).  A contribution v is
+// rounded to q = nint(v * 2^32) (|v| < 2^31) and q is split into its high and low 32-bit
+// halves, which go to two int64 accumulators: hi collects q >> 32, lo collects q & (2^32-1).
+// The sum is hi + lo * 2^-32 exactly, with 2^-32 resolution and no overflow of either
+// accumulator for any realistic number of contributions.
+#define MBGPU_FX_SCALE 4294967296.0     // 2^32
 
 struct CellGeom {
     int imcon;          // 1 cubic, 2 orthorhombic, 3 general (mbpol.f90:1319-1357)
@@ -53,12 +56,15 @@ __device__ __forceinline__ double norm2_nc(double x, double y, double z)
     return (a + b) + c;
 }
 
-__device__ __forceinline__ long long to_fx(double v, double scale)
+// add v to the fixed-point accumulator whose high half is hi[k] and low half lo[k]
+__device__ __forceinline__ void fx_add(long long* hi, long long* lo, long k, double v)
 {
-    return __double2ll_rn(v * scale);
+    const long long q = __double2ll_rn(v * MBGPU_FX_SCALE);
+    atomicAdd(reinterpret_cast<unsigned long long*>(hi + k), (unsigned long long)(q >> 32));
+    atomicAdd(reinterpret_cast<unsigned long long*>(lo + k), (unsigned long long)(q & 0xffffffffll));
 }
 
-__device__ __forceinline__ void fx_add(long long* p, double v, double scale)
+__device__ __forceinline__ double fx_value(const long long* hi, const long long* lo, long k)
 {
-    atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)to_fx(v, scale));
+    return (double)hi[k] + (double)lo[k] * (1.0 / MBGPU_FX_SCALE);
 }

@@ -37,7 +37,8 @@ struct MbBuffers {
     double *clus2 = nullptr, *xv2 = nullptr, *part2 = nullptr, *e2 = nullptr, *g2 = nullptr;
     double *clus3 = nullptr, *xv3 = nullptr, *part3 = nullptr, *e3 = nullptr, *g3 = nullptr;
     // accumulators
-    long long *acc_f = nullptr, *acc_e = nullptr, *acc_w = nullptr;   // 3*natms, natms, 9*natms
+    long long* acc = nullptr;      // fixed-point sums: [2][13*natms] (high halves, low halves) of
+                                   // forces (3n), energies (n), heat-flux tensors (9n)
     double* blk = nullptr;         // per-block partial scalars
     int nblk_cap = 0;
     double* scal = nullptr;        // [16] reduced scalars

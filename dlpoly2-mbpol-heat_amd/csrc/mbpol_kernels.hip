@@ -530,11 +530,14 @@ __global__ __launch_bounds__(256) void k_accum(int nc, const int* __restrict__ c
                                                const double* __restrict__ clus, const double* __restrict__ erec,
                                                const double* __restrict__ grec, const double* __restrict__ x,
                                                const double* __restrict__ y, const double* __restrict__ z,
-                                               long long* __restrict__ acc_f, long long* __restrict__ acc_e,
-                                               long long* __restrict__ acc_w, double* __restrict__ blk, int natms)
+                                               long long* __restrict__ acc, double* __restrict__ blk, int natms)
 {
     constexpr int NA = 3 * NMON;
     constexpr double FRAC = NMON == 2 ? 1.0 / 6.0 : 1.0 / 9.0;
+    // accumulator layout (high halves, then low halves): forces [0,3n), energies [3n,4n),
+    // heat-flux tensors [4n,13n)
+    long long* const hi = acc;
+    long long* const lo = acc + 13 * (long)natms;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // energy, virial, s11 s12 s13 s22 s23 s33
     if (c < nc) {
@@ -563,11 +566,11 @@ __global__ __launch_bounds__(256) void k_accum(int nc, const int* __restrict__ c
         }
         for (int a = 0; a < NA; ++a) {
             const int i = id[a];
-            fx_add(acc_e + i, epa, MBGPU_FX_SCALE);
+            fx_add(hi, lo, 3L * natms + i, epa);
             double fa[3];
             for (int q = 0; q < 3; ++q) {
                 const double gi = g[3 * a + q] * MBGPU_ENGUNIT;
-                fx_add(acc_f + 3 * i + q, -gi, MBGPU_FX_SCALE);
+                fx_add(hi, lo, 3L * i + q, -gi);
                 fa[q] = -(FRAC * g[3 * a + q]) * MBGPU_ENGUNIT;
                 red[1] += wc[3 * a + q] * gi;
             }
@@ -579,7 +582,7 @@ __global__ __launch_bounds__(256) void k_accum(int nc, const int* __restrict__ c
             red[7] -= wc[3 * a + 2] * g[3 * a + 2] * MBGPU_ENGUNIT;
             const double rr[3] = {NA * xa[a][0] - S[0], NA * xa[a][1] - S[1], NA * xa[a][2] - S[2]};
             for (int p = 0; p < 3; ++p)
-                for (int q = 0; q < 3; ++q) fx_add(acc_w + 9 * i + 3 * p + q, fa[p] * rr[q], MBGPU_FXW_SCALE);
+                for (int q = 0; q < 3; ++q) fx_add(hi, lo, 4L * natms + 9L * i + 3 * p + q, fa[p] * rr[q]);
         }
     }
     block_reduce_store(red, 8, blk + 8 * blockIdx.x);
@@ -596,16 +599,15 @@ __global__ void k_reduce_blocks(const double* __restrict__ blk, int nb, double* 
 }
 
 // Convert the fixed-point per-atom sums and add the forces into the caller's arrays.
-__global__ void k_finalize(int natms, const long long* __restrict__ acc_f, const long long* __restrict__ acc_e,
-                           const long long* __restrict__ acc_w, double* __restrict__ fx, double* __restrict__ fy,
-                           double* __restrict__ fz, double* __restrict__ fout, double* __restrict__ ae,
-                           double* __restrict__ hw)
+__global__ void k_finalize(int natms, const long long* __restrict__ acc, double* __restrict__ fx,
+                           double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ fout,
+                           double* __restrict__ ae, double* __restrict__ hw)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natms) return;
-    const double inv = 1.0 / MBGPU_FX_SCALE, invw = 1.0 / MBGPU_FXW_SCALE;
-    const double f0 = (double)acc_f[3 * i] * inv, f1 = (double)acc_f[3 * i + 1] * inv,
-                 f2 = (double)acc_f[3 * i + 2] * inv;
+    const long long* hi = acc;
+    const long long* lo = acc + 13 * (long)natms;
+    const double f0 = fx_value(hi, lo, 3L * i), f1 = fx_value(hi, lo, 3L * i + 1), f2 = fx_value(hi, lo, 3L * i + 2);
     fout[3 * i] = f0;
     fout[3 * i + 1] = f1;
     fout[3 * i + 2] = f2;
@@ -614,8 +616,8 @@ __global__ void k_finalize(int natms, const long long* __restrict__ acc_f, const
         fy[i] += f1;
         fz[i] += f2;
     }
-    ae[i] = (double)acc_e[i] * inv;
-    for (int k = 0; k < 9; ++k) hw[9 * i + k] = (double)acc_w[9 * i + k] * invw;
+    ae[i] = fx_value(hi, lo, 3L * natms + i);
+    for (int k = 0; k < 9; ++k) hw[9 * i + k] = fx_value(hi, lo, 4L * natms + 9L * i + k);
 }
 
 // Heat flux (compute_heat_flux2, heatcurrent.f90:703-728) from per-atom quantities:
@@ -710,22 +712,22 @@ int mb_launch_trimers(MbBuffers& B, int nt, const CellGeom& g, bool gather, hipS
 int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s)
 {
     const int na = B.natms;
-    hipLaunchKernelGGL(k_zero_ll, dim3(256), dim3(256), 0, s, B.acc_f, (long)13 * na);
+    hipLaunchKernelGGL(k_zero_ll, dim3(256), dim3(256), 0, s, B.acc, (long)26 * na);
     const int nb2 = nd > 0 ? (nd + 255) / 256 : 0, nb3 = nt > 0 ? (nt + 255) / 256 : 0;
     if (nb2 + nb3 > B.nblk_cap) return -1;
     if (nb2) {
         hipLaunchKernelGGL(k_accum<2>, dim3(nb2), dim3(256), 0, s, nd, B.dimers, B.list, B.clus2, B.e2, B.g2, B.x,
-                           B.y, B.z, B.acc_f, B.acc_e, B.acc_w, B.blk, na);
+                           B.y, B.z, B.acc, B.blk, na);
     }
     if (nb3) {
         hipLaunchKernelGGL(k_accum<3>, dim3(nb3), dim3(256), 0, s, nt, B.trimers, B.list, B.clus3, B.e3, B.g3,
-                           B.x, B.y, B.z, B.acc_f, B.acc_e, B.acc_w, B.blk + 8 * nb2, na);
+                           B.x, B.y, B.z, B.acc, B.blk + 8 * nb2, na);
     }
     hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(B.scal), 32);
     if (nb2) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, B.blk, nb2, B.scal);
     if (nb3) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, B.blk + 8 * nb2, nb3, B.scal + 8);
-    hipLaunchKernelGGL(k_finalize, dim3((na + 255) / 256), dim3(256), 0, s, na, B.acc_f, B.acc_e, B.acc_w, fx, fy,
-                       fz, B.fout, B.atomic_energy, B.heat_w);
+    hipLaunchKernelGGL(k_finalize, dim3((na + 255) / 256), dim3(256), 0, s, na, B.acc, fx, fy, fz, B.fout,
+                       B.atomic_energy, B.heat_w);
     return (int)hipGetLastError();
 }
 
