@@ -16,7 +16,7 @@
 
 int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s);
 int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipStream_t s);
-int mb_launch_trimers(MbBuffers& B, int nt, const CellGeom& g, bool gather, hipStream_t s);
+int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gather, hipStream_t s);
 int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s);
 int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const double* vz, const double* ekin,
                         const double* epot, const double* epot2, const double* w, const double* w2,
@@ -117,7 +117,9 @@ int alloc_fixed(mbgpu_ctx* c)
     HIPCHK(dalloc(&B.tcnt, nw));
     HIPCHK(dalloc(&B.toff, nw));
     HIPCHK(dalloc(&B.counts, 4));
-    HIPCHK(dalloc(&B.acc, (size_t)26 * na));
+    HIPCHK(dalloc(&B.icnt, (size_t)2 * nw));
+    HIPCHK(dalloc(&B.ioff, nw));
+    HIPCHK(dalloc(&B.is_water, (size_t)4 * ((na + 3) / 4)));
     HIPCHK(dalloc(&B.scal, 16));
     HIPCHK(dalloc(&B.atomic_energy, na));
     HIPCHK(dalloc(&B.heat_w, (size_t)9 * na));
@@ -174,7 +176,12 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
         HIPCHK(dalloc(&B.e3, B.tcap));
         HIPCHK(dalloc(&B.g3, (size_t)27 * B.tcap));
     }
-    const int nblk = (int)((B.dcap + 255) / 256 + (B.tcap + 255) / 256 + (B.natms + 255) / 256 + 8);
+    if (2 * B.dcap + 3 * B.tcap > B.inc_cap) {
+        dfree(B.inc);
+        B.inc_cap = 2 * B.dcap + 3 * B.tcap;
+        HIPCHK(dalloc(&B.inc, B.inc_cap));
+    }
+    const int nblk = (int)((B.dcap + 127) / 128 + (B.tcap + 127) / 128 + (B.natms + 255) / 256 + 8);
     if (nblk > B.nblk_cap) {
         dfree(B.blk);
         B.nblk_cap = nblk;
@@ -208,7 +215,7 @@ int run_clusters(mbgpu_ctx* c, double* fx, double* fy, double* fz)
 {
     MbBuffers& B = c->B;
     int rc = mb_launch_dimers(B, (int)c->nd, c->geom, true, c->stream);
-    if (!rc) rc = mb_launch_trimers(B, (int)c->nt, c->geom, true, c->stream);
+    if (!rc) rc = mb_launch_trimers(B, (int)c->nt, (int)c->nd, c->geom, true, c->stream);
     if (!rc) rc = mb_launch_accumulate(B, (int)c->nd, (int)c->nt, fx, fy, fz, c->stream);
     if (rc) return fail(rc, "cluster kernels failed (%d)", rc);
     return 0;
@@ -265,7 +272,7 @@ int leaf_eval(int nmon, const double* const* w, double* E, double* gout)
     double* dclus = nmon == 2 ? B.clus2 : B.clus3;
     HIPCHK(hipMemcpyAsync(dclus, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, c->stream));
     int rc = nmon == 2 ? mb_launch_dimers(B, 1, c->geom, false, c->stream)
-                       : mb_launch_trimers(B, 1, c->geom, false, c->stream);
+                       : mb_launch_trimers(B, 1, 0, c->geom, false, c->stream);
     if (rc) return fail(rc, "leaf kernels failed");
     HIPCHK(hipMemcpyAsync(E, nmon == 2 ? B.e2 : B.e3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(gout, nmon == 2 ? B.g2 : B.g3, sizeof(double) * 9 * nmon, hipMemcpyDeviceToHost,
@@ -318,7 +325,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
     dfree(B.clus2); dfree(B.xv2); dfree(B.part2); dfree(B.e2); dfree(B.g2);
     dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3);
-    dfree(B.acc); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
+    dfree(B.icnt); dfree(B.ioff); dfree(B.inc); dfree(B.is_water); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
     dfree(B.cs2); dfree(B.cs3);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
@@ -467,7 +474,7 @@ int mbgpu_time_forces(mbgpu_ctx* c, int reps, double* ms_per_eval, double* ms_pi
     for (int r = 0; r < reps; ++r) {
         if (build_lists(c)) return -1;
         int rc = mb_launch_dimers(B, (int)c->nd, c->geom, true, c->stream);
-        if (!rc) rc = mb_launch_trimers(B, (int)c->nt, c->geom, true, c->stream);
+        if (!rc) rc = mb_launch_trimers(B, (int)c->nt, (int)c->nd, c->geom, true, c->stream);
         if (!rc) rc = mb_launch_accumulate(B, (int)c->nd, (int)c->nt, nullptr, nullptr, nullptr, c->stream);
         if (rc) return fail(rc, "timing run failed");
         double s[16];

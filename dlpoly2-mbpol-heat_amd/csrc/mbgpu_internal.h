@@ -39,6 +39,10 @@ struct MbBuffers {
     // accumulators
     long long* acc = nullptr;      // fixed-point sums: [2][13*natms] (high halves, low halves) of
                                    // forces (3n), energies (n), heat-flux tensors (9n)
+    int *icnt = nullptr, *ioff = nullptr, *inc = nullptr;  // per-water cluster incidence (CSR);
+                                   // icnt holds [counts | fill cursors], 2*nw
+    long inc_cap = 0;
+    unsigned char* is_water = nullptr;  // natms (rounded up to 4)
     double* blk = nullptr;         // per-block partial scalars
     int nblk_cap = 0;
     double* scal = nullptr;        // [16] reduced scalars

@@ -20,7 +20,7 @@ namespace {
 
 // ---------------------------------------------------------------- model constants ----
 // 2B (mbpol-2b-poly.cpp:1848-1863)
-__constant__ double c2_k[8] = {
+constexpr double c2_k[8] = {
     -6.480884773303821e-01,  // 0 HH intra
     1.674518993682975e+00,   // 1 OH intra
     1.148231864355956e+00,   // 2 HH coul
@@ -35,8 +35,8 @@ __constant__ double c2_k[8] = {
 
 // the 31 variables (mbpol-2b-poly.cpp:413-445): kind (0 exp, 1 exp/r), k index, d0, sites
 // sites: 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Xa1 7 Xa2 8 Xb1 9 Xb2
-struct V2Def { signed char kind, k, d0, s1, s2; };
-__constant__ V2Def c2_vars[31] = {
+struct V2Def { int kind, k, d0, s1, s2; };
+constexpr V2Def c2_vars[31] = {
     {0, 0, 1, 1, 2}, {0, 0, 1, 4, 5}, {0, 1, 1, 0, 1}, {0, 1, 1, 0, 2}, {0, 1, 1, 3, 4},
     {0, 1, 1, 3, 5}, {1, 2, 4, 1, 4}, {1, 2, 4, 1, 5}, {1, 2, 4, 2, 4}, {1, 2, 4, 2, 5},
     {1, 3, 4, 0, 4}, {1, 3, 4, 0, 5}, {1, 3, 4, 3, 1}, {1, 3, 4, 3, 2}, {1, 4, 4, 0, 3},
@@ -47,14 +47,14 @@ __constant__ V2Def c2_vars[31] = {
 
 // 3B (mbpol-3b-poly.cpp:1488-1504): k and r0 per variable class
 // class 0 HH intra, 1 OH intra, 2 HH, 3 OH, 4 OO
-__constant__ double c3_k[5] = {-2.254303257872797e+00, -2.564771901404151e-01, 4.247920544074333e-01,
+constexpr double c3_k[5] = {-2.254303257872797e+00, -2.564771901404151e-01, 4.247920544074333e-01,
                                8.128985941165371e-01, 3.749457984616480e-02};
-__constant__ double c3_r0[5] = {1.690594510379166e+00, -2.999999868517452e+00, 3.499031358429095e+00,
+constexpr double c3_r0[5] = {1.690594510379166e+00, -2.999999868517452e+00, 3.499031358429095e+00,
                                 4.854042963514281e+00, 4.816312044947604e-08};
 // the 36 variables (mbpol-3b-poly.cpp:170-206): class, atom 1, atom 2
 // atoms: 0 Oa 1 Ha1 2 Ha2 3 Ob 4 Hb1 5 Hb2 6 Oc 7 Hc1 8 Hc2
-struct V3Def { signed char cls, a1, a2; };
-__constant__ V3Def c3_vars[36] = {
+struct V3Def { int cls, a1, a2; };
+constexpr V3Def c3_vars[36] = {
     {0, 1, 2}, {0, 4, 5}, {0, 7, 8}, {1, 0, 1}, {1, 0, 2}, {1, 3, 4}, {1, 3, 5}, {1, 6, 7},
     {1, 6, 8}, {2, 1, 4}, {2, 1, 5}, {2, 1, 7}, {2, 1, 8}, {2, 2, 4}, {2, 2, 5}, {2, 2, 7},
     {2, 2, 8}, {2, 4, 7}, {2, 4, 8}, {2, 5, 7}, {2, 5, 8}, {3, 0, 4}, {3, 0, 5}, {3, 0, 7},
@@ -131,10 +131,12 @@ __device__ __forceinline__ void dimer_sites(const double* w, double* s, double* 
     extra_points(s + 9, ohb1, ohb2, s + 24, s + 27);
 }
 
-__device__ __forceinline__ void block_reduce_store(double* vals, int n, double* out)
+// Block sum of n <= 16 per-thread values in a fixed order (wavefront shuffles, then the
+// wavefronts in index order); thread q < n writes sum q to out[q].  Every thread of the
+// block must call it.
+__device__ __forceinline__ void block_reduce_store(const double* vals, int n, double* out)
 {
-    // vals: per-thread values; wavefront then block reduction in a fixed order
-    __shared__ double red[8][16];
+    __shared__ double red[16][16];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int q = 0; q < n; ++q) {
         double v = vals[q];
@@ -239,69 +241,86 @@ __global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restr
 // forms a triple with two short edges (c-a, c-b).  A triple whose third edge is short too
 // is reached from all three vertices and kept only from its smallest index.  This yields
 // exactly the triples do_trimer keeps: at least two of the three O-O edges <= r3f.
+// One wavefront per centre: the outer neighbour index runs sequentially, the inner one
+// across the lanes, and a ballot keeps the output in (a, b) order.
 template <bool FILL>
-__global__ void k_trimers(int nw, const double* __restrict__ mon, CellGeom g, int cap3,
-                          const int* __restrict__ nbr3, const int* __restrict__ cnt3, int* __restrict__ tcnt,
-                          const int* __restrict__ toff, int* __restrict__ trimers, long tcap)
+__global__ __launch_bounds__(64) void k_trimers(int nw, const double* __restrict__ mon, CellGeom g, int cap3,
+                                                const int* __restrict__ nbr3, const int* __restrict__ cnt3,
+                                                int* __restrict__ tcnt, const int* __restrict__ toff,
+                                                int* __restrict__ trimers, long tcap)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
     if (c >= nw) return;
     const int n = min(cnt3[c], cap3);
     const int* S = nbr3 + (long)c * cap3;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int cnt = 0;
-    long o = FILL ? toff[c] : 0;
-    for (int p = 0; p < n; ++p) {
+    const long o = FILL ? toff[c] : 0;
+    for (int p = 0; p + 1 < n; ++p) {
         const int a = S[p];
-        for (int q = p + 1; q < n; ++q) {
-            const int b = S[q];
-            double dx = mon[9 * a] - mon[9 * b];
-            double dy = mon[9 * a + 1] - mon[9 * b + 1];
-            double dz = mon[9 * a + 2] - mon[9 * b + 2];
-            image_vec(g, dx, dy, dz);
-            const double rab = sqrt(norm2_nc(dx, dy, dz));
-            const bool keep = (rab > MBGPU_R3F) || (c < a);
-            if (!keep) continue;
-            if (FILL) {
+        const double ax = mon[9 * a], ay = mon[9 * a + 1], az = mon[9 * a + 2];
+        for (int base = p + 1; base < n; base += 64) {
+            const int q = base + lane;
+            bool keep = false;
+            int b = 0;
+            if (q < n) {
+                b = S[q];
+                double dx = ax - mon[9 * b];
+                double dy = ay - mon[9 * b + 1];
+                double dz = az - mon[9 * b + 2];
+                image_vec(g, dx, dy, dz);
+                const double rab = sqrt(norm2_nc(dx, dy, dz));
+                keep = (rab > MBGPU_R3F) || (c < a);
+            }
+            const unsigned long long bal = __ballot(keep);
+            if (FILL && keep) {
                 int i0, i1, i2;
                 if (c < a) { i0 = c; i1 = a; i2 = b; }
                 else if (c < b) { i0 = a; i1 = c; i2 = b; }
                 else { i0 = a; i1 = b; i2 = c; }
-                const long t = o + cnt;
+                const long t = o + cnt + __popcll(bal & below);
                 if (t < tcap) {
                     trimers[3 * t] = i0;
                     trimers[3 * t + 1] = i1;
                     trimers[3 * t + 2] = i2;
                 }
             }
-            ++cnt;
+            cnt += __popcll(bal);
         }
     }
-    if (!FILL) tcnt[c] = cnt;
+    if (!FILL && lane == 0) tcnt[c] = cnt;
 }
 
-// Exclusive prefix sum of n counts (single block); total written to *total.
+// Exclusive prefix sum of n counts (one block of 1024 threads); total to *total.
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ cnt, int n, int* __restrict__ off,
                                                int* __restrict__ total)
 {
-    __shared__ long part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + blockDim.x - 1) / blockDim.x;
+    __shared__ long wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int per = (n + 1023) / 1024;
     const int b = t * per, e = min(n, b + per);
     long s = 0;
     for (int i = b; i < e; ++i) s += cnt[i];
-    part[t] = s;
+    // inclusive scan of s across the wavefront
+    long inc = s;
+    for (int d = 1; d < 64; d <<= 1) {
+        const long v = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += v;
+    }
+    if (lane == 63) wsum[wid] = inc;
     __syncthreads();
     if (t == 0) {
         long acc = 0;
-        for (int i = 0; i < (int)blockDim.x; ++i) {
-            const long v = part[i];
-            part[i] = acc;
+        for (int w = 0; w < 16; ++w) {
+            const long v = wsum[w];
+            wsum[w] = acc;
             acc += v;
         }
         *total = (int)acc;
     }
     __syncthreads();
-    long acc = part[t];
+    long acc = wsum[wid] + inc - s;
     for (int i = b; i < e; ++i) {
         off[i] = (int)acc;
         acc += cnt[i];
@@ -334,11 +353,13 @@ __global__ void k_gather2(int nd, const int* __restrict__ dimers, const double* 
     load_mon(mon, dimers[2 * d + 1], w2);
     double r[3] = {w1[0] - w2[0], w1[1] - w2[1], w1[2] - w2[2]};
     image_vec(g, r[0], r[1], r[2]);
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         w1[3 + k] = (w1[3 + k] - w1[k]) + (w2[k] + r[k]);
         w1[6 + k] = (w1[6 + k] - w1[k]) + (w2[k] + r[k]);
         w1[k] = w2[k] + r[k];
     }
+#pragma unroll
     for (int k = 0; k < 9; ++k) {
         clus[(long)k * nd + d] = w1[k];
         clus[(long)(9 + k) * nd + d] = w2[k];
@@ -359,6 +380,7 @@ __global__ void k_gather3(int nt, const int* __restrict__ trimers, const double*
     double rb[3] = {w2[0] - w3[0], w2[1] - w3[1], w2[2] - w3[2]};
     image_vec(g, ra[0], ra[1], ra[2]);
     image_vec(g, rb[0], rb[1], rb[2]);
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         w1[3 + k] = (w1[3 + k] - w1[k]) + (w2[k] + ra[k]);
         w1[6 + k] = (w1[6 + k] - w1[k]) + (w2[k] + ra[k]);
@@ -367,6 +389,7 @@ __global__ void k_gather3(int nt, const int* __restrict__ trimers, const double*
         w3[6 + k] = (w3[6 + k] - w3[k]) + (w2[k] - rb[k]);
         w3[k] = w2[k] - rb[k];
     }
+#pragma unroll
     for (int k = 0; k < 9; ++k) {
         clus[(long)k * nt + t] = w1[k];
         clus[(long)(9 + k) * nt + t] = w2[k];
@@ -380,8 +403,10 @@ __global__ void k_setup2(int nd, const double* __restrict__ clus, double* __rest
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= nd) return;
     double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
+#pragma unroll
     for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * nd + d];
     dimer_sites(w, s, a1, a2, b1, b2);
+#pragma unroll
     for (int n = 0; n < 31; ++n) {
         const V2Def v = c2_vars[n];
         const double dx = s[3 * v.s1] - s[3 * v.s2], dy = s[3 * v.s1 + 1] - s[3 * v.s2 + 1],
@@ -392,63 +417,99 @@ __global__ void k_setup2(int nd, const double* __restrict__ clus, double* __rest
     }
 }
 
+// Per-cluster virial and stress (do_dimer mbpol.f90:582-653, do_trimer :972-1067) with g
+// in kcal/mol/A: vir = sum w.g*engunit, strs = -sum w (x) g*engunit (upper triangle).
+template <int NC>
+__device__ __forceinline__ void cluster_virial(const double* w, const double* g, double* red)
+{
+#pragma unroll
+    for (int a = 0; a < NC / 3; ++a) {
+        const double g0 = g[3 * a] * MBGPU_ENGUNIT, g1 = g[3 * a + 1] * MBGPU_ENGUNIT,
+                     g2 = g[3 * a + 2] * MBGPU_ENGUNIT;
+        red[1] += w[3 * a] * g0 + w[3 * a + 1] * g1 + w[3 * a + 2] * g2;
+        red[2] -= w[3 * a] * g0;
+        red[3] -= w[3 * a] * g1;
+        red[4] -= w[3 * a] * g2;
+        red[5] -= w[3 * a + 1] * g1;
+        red[6] -= w[3 * a + 1] * g2;
+        red[7] -= w[3 * a + 2] * g2;
+    }
+}
+
 // Sum the slice partials, apply the chain rule through the variables and extra points and
-// the switch (mbpol-2b-poly.cpp:447-682).  Record: E (kcal/mol) and dE/dx of the 6 atoms.
-__global__ void k_finish2(int nd, int nsl, const double* __restrict__ clus, const double* __restrict__ xv,
-                          const double* __restrict__ part, double* __restrict__ erec, double* __restrict__ grec)
+// the switch (mbpol-2b-poly.cpp:447-682).  Record: E (kcal/mol) and dE/dx of the 6 atoms;
+// block partials of energy, virial and stress.
+__global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* __restrict__ clus,
+                                                 const double* __restrict__ xv, const double* __restrict__ part,
+                                                 double* __restrict__ erec, double* __restrict__ grec,
+                                                 double* __restrict__ blk)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= nd) return;
-    double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
-    for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * nd + d];
-    const double dOO[3] = {w[0] - w[9], w[1] - w[10], w[2] - w[11]};
-    const double rOO = sqrt(dOO[0] * dOO[0] + dOO[1] * dOO[1] + dOO[2] * dOO[2]);
-    if (rOO > MBGPU_R2F) {  // the leaf returns 0 without touching the gradients
-        erec[d] = 0.0;
-        for (int k = 0; k < 18; ++k) grec[(long)k * nd + d] = 0.0;
-        return;
-    }
-    dimer_sites(w, s, a1, a2, b1, b2);
-    double E = 0.0;
-    for (int q = 0; q < nsl; ++q) E += part[((long)q * 32) * nd + d];
-    double xg[30];
-    for (int k = 0; k < 30; ++k) xg[k] = 0.0;
-    for (int n = 0; n < 31; ++n) {
-        double gn = 0.0;
-        for (int q = 0; q < nsl; ++q) gn += part[((long)q * 32 + 1 + n) * nd + d];
-        const V2Def v = c2_vars[n];
-        double dd[3] = {s[3 * v.s1] - s[3 * v.s2], s[3 * v.s1 + 1] - s[3 * v.s2 + 1],
-                        s[3 * v.s1 + 2] - s[3 * v.s2 + 2]};
-        const double r = sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
-        const double val = xv[(long)n * nd + d];
-        const double k = c2_k[v.k];
-        double gg;
-        if (v.kind) {
-            const double rinv = 1.0 / r;
-            gg = -(k + rinv) * val * rinv;
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (d < nd) {
+        double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
+#pragma unroll
+        for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * nd + d];
+        const double dOO[3] = {w[0] - w[9], w[1] - w[10], w[2] - w[11]};
+        const double rOO = sqrt(dOO[0] * dOO[0] + dOO[1] * dOO[1] + dOO[2] * dOO[2]);
+        double gout[18];
+        double Eout = 0.0;
+        if (rOO > MBGPU_R2F) {  // the leaf returns 0 without touching the gradients
+#pragma unroll
+            for (int k = 0; k < 18; ++k) gout[k] = 0.0;
         } else {
-            gg = -k * val / r;
+            dimer_sites(w, s, a1, a2, b1, b2);
+            double E = 0.0;
+            for (int q = 0; q < nsl; ++q) E += part[((long)q * 32) * nd + d];
+            double xg[30];
+#pragma unroll
+            for (int k = 0; k < 30; ++k) xg[k] = 0.0;
+#pragma unroll
+            for (int n = 0; n < 31; ++n) {
+                double gn = 0.0;
+                for (int q = 0; q < nsl; ++q) gn += part[((long)q * 32 + 1 + n) * nd + d];
+                const V2Def v = c2_vars[n];
+                const double dd[3] = {s[3 * v.s1] - s[3 * v.s2], s[3 * v.s1 + 1] - s[3 * v.s2 + 1],
+                                      s[3 * v.s1 + 2] - s[3 * v.s2 + 2]};
+                const double r = sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+                const double val = xv[(long)n * nd + d];
+                const double k = c2_k[v.k];
+                double gg;
+                if (v.kind) {
+                    const double rinv = 1.0 / r;
+                    gg = -(k + rinv) * val * rinv;
+                } else {
+                    gg = -k * val / r;
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const double t = gn * (dd[i] * gg);
+                    xg[3 * v.s1 + i] += t;
+                    xg[3 * v.s2 + i] -= t;
+                }
+            }
+            extra_point_grads(a1, a2, xg + 18, xg + 21, xg + 0);
+            extra_point_grads(b1, b2, xg + 24, xg + 27, xg + 9);
+            double gsw;
+            const double sw = f_switch(rOO, R2I, MBGPU_R2F, gsw);
+#pragma unroll
+            for (int i = 0; i < 18; ++i) gout[i] = sw * xg[i];
+            gsw *= E / rOO;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double t = gsw * dOO[i];
+                gout[i] += t;
+                gout[9 + i] -= t;
+            }
+            Eout = sw * E;
         }
-        for (int i = 0; i < 3; ++i) {
-            const double t = gn * (dd[i] * gg);
-            xg[3 * v.s1 + i] += t;
-            xg[3 * v.s2 + i] -= t;
-        }
+        erec[d] = Eout;
+#pragma unroll
+        for (int k = 0; k < 18; ++k) grec[(long)k * nd + d] = gout[k];
+        red[0] = Eout * MBGPU_ENGUNIT;
+        cluster_virial<18>(w, gout, red);
     }
-    extra_point_grads(a1, a2, xg + 18, xg + 21, xg + 0);
-    extra_point_grads(b1, b2, xg + 24, xg + 27, xg + 9);
-    double gsw;
-    const double sw = f_switch(rOO, R2I, MBGPU_R2F, gsw);
-    double gout[18];
-    for (int i = 0; i < 18; ++i) gout[i] = sw * xg[i];
-    gsw *= E / rOO;
-    for (int i = 0; i < 3; ++i) {
-        const double t = gsw * dOO[i];
-        gout[i] += t;
-        gout[9 + i] -= t;
-    }
-    erec[d] = sw * E;
-    for (int k = 0; k < 18; ++k) grec[(long)k * nd + d] = gout[k];
+    block_reduce_store(red, 8, blk + 8 * blockIdx.x);
 }
 
 // 3B variables (mbpol-3b-poly.cpp:170-206): SoA [36][nt]
@@ -457,7 +518,9 @@ __global__ void k_setup3(int nt, const double* __restrict__ clus, double* __rest
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     double w[27];
+#pragma unroll
     for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * nt + t];
+#pragma unroll
     for (int n = 0; n < 36; ++n) {
         const V3Def v = c3_vars[n];
         const double dx = w[3 * v.a1] - w[3 * v.a2], dy = w[3 * v.a1 + 1] - w[3 * v.a2 + 1],
@@ -468,156 +531,240 @@ __global__ void k_setup3(int nt, const double* __restrict__ clus, double* __rest
 }
 
 // mbpol-3b-poly.cpp:208-312: switch s = sab*sac + sab*sbc + sac*sbc, chain rule, switch
-// gradients.  Record: E (kcal/mol) and dE/dx of the 9 atoms.
-__global__ void k_finish3(int nt, int nsl, const double* __restrict__ clus, const double* __restrict__ xv,
-                          const double* __restrict__ part, double* __restrict__ erec, double* __restrict__ grec)
+// gradients.  Record: E (kcal/mol) and dE/dx of the 9 atoms; block partials as k_finish2.
+__global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* __restrict__ clus,
+                                                 const double* __restrict__ xv, const double* __restrict__ part,
+                                                 double* __restrict__ erec, double* __restrict__ grec,
+                                                 double* __restrict__ blk)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    double w[27];
-    for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * nt + t];
-    double E = 0.0;
-    for (int q = 0; q < nsl; ++q) E += part[((long)q * 37) * nt + t];
-    double rab[3], rac[3], rbc[3];
-    for (int n = 0; n < 3; ++n) {
-        rab[n] = w[n] - w[9 + n];
-        rac[n] = w[n] - w[18 + n];
-        rbc[n] = w[9 + n] - w[18 + n];
-    }
-    const double drab = sqrt(rab[0] * rab[0] + rab[1] * rab[1] + rab[2] * rab[2]);
-    const double drac = sqrt(rac[0] * rac[0] + rac[1] * rac[1] + rac[2] * rac[2]);
-    const double drbc = sqrt(rbc[0] * rbc[0] + rbc[1] * rbc[1] + rbc[2] * rbc[2]);
-    double gab, gac, gbc;
-    const double sab = f_switch(drab, 0.0, MBGPU_R3F, gab);
-    const double sac = f_switch(drac, 0.0, MBGPU_R3F, gac);
-    const double sbc = f_switch(drbc, 0.0, MBGPU_R3F, gbc);
-    const double sw = sab * sac + sab * sbc + sac * sbc;
-    double gc[27];
-    for (int k = 0; k < 27; ++k) gc[k] = 0.0;
-    for (int n = 0; n < 36; ++n) {
-        double gn = 0.0;
-        for (int q = 0; q < nsl; ++q) gn += part[((long)q * 37 + 1 + n) * nt + t];
-        gn *= sw;
-        const V3Def v = c3_vars[n];
-        const double dx[3] = {w[3 * v.a1] - w[3 * v.a2], w[3 * v.a1 + 1] - w[3 * v.a2 + 1],
-                              w[3 * v.a1 + 2] - w[3 * v.a2 + 2]};
-        const double dd = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
-        const double k = c3_k[v.cls];
-        const double gg = -k * gn * xv[(long)n * nt + t] / dd;
-        for (int i = 0; i < 3; ++i) {
-            gc[3 * v.a1 + i] += gg * dx[i];
-            gc[3 * v.a2 + i] -= gg * dx[i];
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (t < nt) {
+        double w[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * nt + t];
+        double E = 0.0;
+        for (int q = 0; q < nsl; ++q) E += part[((long)q * 37) * nt + t];
+        double rab[3], rac[3], rbc[3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            rab[n] = w[n] - w[9 + n];
+            rac[n] = w[n] - w[18 + n];
+            rbc[n] = w[9 + n] - w[18 + n];
         }
-    }
-    gab *= (sac + sbc) * E / drab;
-    gac *= (sab + sbc) * E / drac;
-    gbc *= (sab + sac) * E / drbc;
-    for (int n = 0; n < 3; ++n) {
-        gc[n] += gab * rab[n] + gac * rac[n];
-        gc[9 + n] += gbc * rbc[n] - gab * rab[n];
-        gc[18 + n] -= gac * rac[n] + gbc * rbc[n];
-    }
-    erec[t] = E * sw;
-    for (int k = 0; k < 27; ++k) grec[(long)k * nt + t] = gc[k];
-}
-
-// Per-cluster accumulation (do_dimer mbpol.f90:519-653 / do_trimer :896-1067):
-// forces f -= g*engunit, per-atom energy E*engunit/nat, heat-flux tensor
-// W_a += F_a (x) sum_{b != a}(x_a - x_b) with F_a = -(g_a/nat)*engunit and raw positions,
-// virial sum w.g and stress -sum w (x) g (upper triangle, mbpol.f90:605-653), per block.
-template <int NMON>
-__global__ __launch_bounds__(256) void k_accum(int nc, const int* __restrict__ clist, const int* __restrict__ list,
-                                               const double* __restrict__ clus, const double* __restrict__ erec,
-                                               const double* __restrict__ grec, const double* __restrict__ x,
-                                               const double* __restrict__ y, const double* __restrict__ z,
-                                               long long* __restrict__ acc, double* __restrict__ blk, int natms)
-{
-    constexpr int NA = 3 * NMON;
-    constexpr double FRAC = NMON == 2 ? 1.0 / 6.0 : 1.0 / 9.0;
-    // accumulator layout (high halves, then low halves): forces [0,3n), energies [3n,4n),
-    // heat-flux tensors [4n,13n)
-    long long* const hi = acc;
-    long long* const lo = acc + 13 * (long)natms;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // energy, virial, s11 s12 s13 s22 s23 s33
-    if (c < nc) {
-        const double E = erec[c];
-        int id[NA];
-        for (int m = 0; m < NMON; ++m) {
-            const int o = list[3 * clist[NMON * c + m]] - 1;
-            for (int a = 0; a < 3; ++a) id[3 * m + a] = o + a;
-        }
-        double g[3 * NA], wc[3 * NA];
-        for (int k = 0; k < 3 * NA; ++k) {
-            g[k] = grec[(long)k * nc + c];
-            wc[k] = clus[(long)k * nc + c];
-        }
-        red[0] = E * MBGPU_ENGUNIT;
-        const double epa = FRAC * E * MBGPU_ENGUNIT;
-        double S[3] = {0, 0, 0};
-        double xa[NA][3];
-        for (int a = 0; a < NA; ++a) {
-            xa[a][0] = x[id[a]];
-            xa[a][1] = y[id[a]];
-            xa[a][2] = z[id[a]];
-            S[0] += xa[a][0];
-            S[1] += xa[a][1];
-            S[2] += xa[a][2];
-        }
-        for (int a = 0; a < NA; ++a) {
-            const int i = id[a];
-            fx_add(hi, lo, 3L * natms + i, epa);
-            double fa[3];
-            for (int q = 0; q < 3; ++q) {
-                const double gi = g[3 * a + q] * MBGPU_ENGUNIT;
-                fx_add(hi, lo, 3L * i + q, -gi);
-                fa[q] = -(FRAC * g[3 * a + q]) * MBGPU_ENGUNIT;
-                red[1] += wc[3 * a + q] * gi;
+        const double drab = sqrt(rab[0] * rab[0] + rab[1] * rab[1] + rab[2] * rab[2]);
+        const double drac = sqrt(rac[0] * rac[0] + rac[1] * rac[1] + rac[2] * rac[2]);
+        const double drbc = sqrt(rbc[0] * rbc[0] + rbc[1] * rbc[1] + rbc[2] * rbc[2]);
+        double gab, gac, gbc;
+        const double sab = f_switch(drab, 0.0, MBGPU_R3F, gab);
+        const double sac = f_switch(drac, 0.0, MBGPU_R3F, gac);
+        const double sbc = f_switch(drbc, 0.0, MBGPU_R3F, gbc);
+        const double sw = sab * sac + sab * sbc + sac * sbc;
+        double gc[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) gc[k] = 0.0;
+#pragma unroll
+        for (int n = 0; n < 36; ++n) {
+            double gn = 0.0;
+            for (int q = 0; q < nsl; ++q) gn += part[((long)q * 37 + 1 + n) * nt + t];
+            gn *= sw;
+            const V3Def v = c3_vars[n];
+            const double dx[3] = {w[3 * v.a1] - w[3 * v.a2], w[3 * v.a1 + 1] - w[3 * v.a2 + 1],
+                                  w[3 * v.a1 + 2] - w[3 * v.a2 + 2]};
+            const double dd = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+            const double k = c3_k[v.cls];
+            const double gg = -k * gn * xv[(long)n * nt + t] / dd;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                gc[3 * v.a1 + i] += gg * dx[i];
+                gc[3 * v.a2 + i] -= gg * dx[i];
             }
-            red[2] -= wc[3 * a] * g[3 * a] * MBGPU_ENGUNIT;
-            red[3] -= wc[3 * a] * g[3 * a + 1] * MBGPU_ENGUNIT;
-            red[4] -= wc[3 * a] * g[3 * a + 2] * MBGPU_ENGUNIT;
-            red[5] -= wc[3 * a + 1] * g[3 * a + 1] * MBGPU_ENGUNIT;
-            red[6] -= wc[3 * a + 1] * g[3 * a + 2] * MBGPU_ENGUNIT;
-            red[7] -= wc[3 * a + 2] * g[3 * a + 2] * MBGPU_ENGUNIT;
-            const double rr[3] = {NA * xa[a][0] - S[0], NA * xa[a][1] - S[1], NA * xa[a][2] - S[2]};
-            for (int p = 0; p < 3; ++p)
-                for (int q = 0; q < 3; ++q) fx_add(hi, lo, 4L * natms + 9L * i + 3 * p + q, fa[p] * rr[q]);
         }
+        gab *= (sac + sbc) * E / drab;
+        gac *= (sab + sbc) * E / drac;
+        gbc *= (sab + sac) * E / drbc;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            gc[n] += gab * rab[n] + gac * rac[n];
+            gc[9 + n] += gbc * rbc[n] - gab * rab[n];
+            gc[18 + n] -= gac * rac[n] + gbc * rbc[n];
+        }
+        const double Eout = E * sw;
+        erec[t] = Eout;
+#pragma unroll
+        for (int k = 0; k < 27; ++k) grec[(long)k * nt + t] = gc[k];
+        red[0] = Eout * MBGPU_ENGUNIT;
+        cluster_virial<27>(w, gc, red);
     }
     block_reduce_store(red, 8, blk + 8 * blockIdx.x);
 }
 
-// Fixed order sum of nb block partials (8 each) into out[0..7]
-__global__ void k_reduce_blocks(const double* __restrict__ blk, int nb, double* __restrict__ out)
+// ------------------------------------------------------------ per-water incidence ----
+// Every (cluster, slot) pair is entered in the list of the water at that slot:
+// entry = (global cluster index << 2) | slot, dimers first (0..nd-1), then trimers.
+__global__ void k_inc_count(int nd, int nt, const int* __restrict__ dimers, const int* __restrict__ trimers,
+                            int* __restrict__ icnt)
 {
-    const int q = threadIdx.x;
-    if (q >= 8) return;
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += blk[8 * b + q];
-    out[q] = s;
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e < 2L * nd) atomicAdd(icnt + dimers[e], 1);
+    else if (e < 2L * nd + 3L * nt) atomicAdd(icnt + trimers[e - 2L * nd], 1);
 }
 
-// Convert the fixed-point per-atom sums and add the forces into the caller's arrays.
-__global__ void k_finalize(int natms, const long long* __restrict__ acc, double* __restrict__ fx,
-                           double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ fout,
-                           double* __restrict__ ae, double* __restrict__ hw)
+__global__ void k_inc_fill(int nd, int nt, const int* __restrict__ dimers, const int* __restrict__ trimers,
+                           const int* __restrict__ ioff, int* __restrict__ icur, int* __restrict__ inc)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int w, entry;
+    if (e < 2L * nd) {
+        w = dimers[e];
+        entry = (int)((e >> 1) << 2) | (int)(e & 1);
+    } else if (e < 2L * nd + 3L * nt) {
+        const long t = e - 2L * nd;
+        w = trimers[t];
+        entry = (int)((nd + t / 3) << 2) | (int)(t % 3);
+    } else {
+        return;
+    }
+    inc[ioff[w] + atomicAdd(icur + w, 1)] = entry;
+}
+
+// Per-water gather of the cluster records (replaces update_energy_mbpol / fxx updates /
+// update_forces of do_dimer and do_trimer).  One wavefront per water; each lane sums its
+// share of the water's clusters in 2^-32 fixed point (order-independent, hence
+// deterministic), then the lanes are combined exactly.  Per atom a of the water:
+//   f_a   += -g_a * engunit
+//   e_a   += E * engunit / k                          (k = 6 dimer, 9 trimer)
+//   W_a   += F_a (x) sum_{b in cluster, b != a} (x_a - x_b),  F_a = -(g_a / k) engunit
+// with raw (un-imaged) positions x, as compute_heat_flux2 contracts force_matrix.
+__global__ __launch_bounds__(64) void k_gather_water(int nw, int nd, int nt, const int* __restrict__ list,
+                                                     const int* __restrict__ dimers, const int* __restrict__ trimers,
+                                                     const int* __restrict__ ioff, const int* __restrict__ icnt,
+                                                     const int* __restrict__ inc, const double* __restrict__ e2,
+                                                     const double* __restrict__ g2, const double* __restrict__ e3,
+                                                     const double* __restrict__ g3, const double* __restrict__ x,
+                                                     const double* __restrict__ y, const double* __restrict__ z,
+                                                     double* __restrict__ fx, double* __restrict__ fy,
+                                                     double* __restrict__ fz, double* __restrict__ fout,
+                                                     double* __restrict__ ae, double* __restrict__ hw)
+{
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (w >= nw) return;
+    const int o = list[3 * w] - 1;
+    double xa[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        xa[a][0] = x[o + a];
+        xa[a][1] = y[o + a];
+        xa[a][2] = z[o + a];
+    }
+    // fixed-point accumulators: 9 force components, 1 energy, 27 tensor components
+    long long hi[37], lo[37];
+#pragma unroll
+    for (int k = 0; k < 37; ++k) hi[k] = lo[k] = 0;
+    auto acc = [&](int k, double v) {
+        const long long q = __double2ll_rn(v * MBGPU_FX_SCALE);
+        hi[k] += q >> 32;
+        lo[k] += q & 0xffffffffll;
+    };
+    const int n = icnt[w], base = ioff[w];
+    for (int p = lane; p < n; p += 64) {
+        const int entry = inc[base + p];
+        const int cg = entry >> 2, slot = entry & 3;
+        const bool tri = cg >= nd;
+        const int c = tri ? cg - nd : cg;
+        const int nmon = tri ? 3 : 2;
+        const double frac = tri ? 1.0 / 9.0 : 1.0 / 6.0;
+        const double kk = tri ? 9.0 : 6.0;
+        const double E = tri ? e3[c] : e2[c];
+        const double* G = tri ? g3 : g2;
+        const long nc = tri ? nt : nd;
+        const int* cl = tri ? trimers + 3L * c : dimers + 2L * c;
+        // sum of the raw positions of all atoms of the cluster
+        double S[3] = {0, 0, 0};
+        for (int m = 0; m < nmon; ++m) {
+            const int om = list[3 * cl[m]] - 1;
+            for (int a = 0; a < 3; ++a) {
+                S[0] += x[om + a];
+                S[1] += y[om + a];
+                S[2] += z[om + a];
+            }
+        }
+        acc(9, frac * E * MBGPU_ENGUNIT);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double ga[3], fa[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                ga[q] = G[(long)(9 * slot + 3 * a + q) * nc + c];
+                acc(3 * a + q, -(ga[q] * MBGPU_ENGUNIT));
+                fa[q] = -(frac * ga[q]) * MBGPU_ENGUNIT;
+            }
+            const double rr[3] = {kk * xa[a][0] - S[0], kk * xa[a][1] - S[1], kk * xa[a][2] - S[2]};
+#pragma unroll
+            for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) acc(10 + 9 * a + 3 * pp + q, fa[pp] * rr[q]);
+        }
+    }
+    // exact combination across the wavefront
+#pragma unroll
+    for (int k = 0; k < 37; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            hi[k] += __shfl_down(hi[k], off, 64);
+            lo[k] += __shfl_down(lo[k], off, 64);
+        }
+    }
+    if (lane == 0) {
+        const double inv = 1.0 / MBGPU_FX_SCALE;
+        double v[37];
+#pragma unroll
+        for (int k = 0; k < 37; ++k) v[k] = (double)hi[k] + (double)lo[k] * inv;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int i = o + a;
+            fout[3 * i] = v[3 * a];
+            fout[3 * i + 1] = v[3 * a + 1];
+            fout[3 * i + 2] = v[3 * a + 2];
+            if (fx) {
+                fx[i] += v[3 * a];
+                fy[i] += v[3 * a + 1];
+                fz[i] += v[3 * a + 2];
+            }
+            ae[i] = v[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) hw[9L * i + k] = v[10 + 9 * a + k];
+        }
+    }
+}
+
+// Zero the per-atom outputs of the sites that are not water atoms (M sites).
+__global__ void k_zero_other_sites(int natms, const unsigned char* __restrict__ is_water, double* __restrict__ fout,
+                                   double* __restrict__ ae, double* __restrict__ hw)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natms) return;
-    const long long* hi = acc;
-    const long long* lo = acc + 13 * (long)natms;
-    const double f0 = fx_value(hi, lo, 3L * i), f1 = fx_value(hi, lo, 3L * i + 1), f2 = fx_value(hi, lo, 3L * i + 2);
-    fout[3 * i] = f0;
-    fout[3 * i + 1] = f1;
-    fout[3 * i + 2] = f2;
-    if (fx) {
-        fx[i] += f0;
-        fy[i] += f1;
-        fz[i] += f2;
-    }
-    ae[i] = fx_value(hi, lo, 3L * natms + i);
-    for (int k = 0; k < 9; ++k) hw[9 * i + k] = fx_value(hi, lo, 4L * natms + 9L * i + k);
+    if (i >= natms || is_water[i]) return;
+    fout[3 * i] = fout[3 * i + 1] = fout[3 * i + 2] = 0.0;
+    ae[i] = 0.0;
+    for (int k = 0; k < 9; ++k) hw[9L * i + k] = 0.0;
+}
+
+__global__ void k_mark_water(int nw, const int* __restrict__ list, unsigned char* __restrict__ is_water)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const int o = list[3 * w] - 1;
+    is_water[o] = is_water[o + 1] = is_water[o + 2] = 1;
+}
+
+// Fixed-order sum of nb block partials (8 values each): one wavefront per value.
+__global__ __launch_bounds__(512) void k_reduce_blocks(const double* __restrict__ blk, int nb, double* __restrict__ out)
+{
+    const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int b = lane; b < nb; b += 64) s += blk[8L * b + q];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) out[q] = s;
 }
 
 // Heat flux (compute_heat_flux2, heatcurrent.f90:703-728) from per-atom quantities:
@@ -658,12 +805,6 @@ __global__ __launch_bounds__(256) void k_heat_flux(int nreal, const double* __re
 
 // ============================================================ host-side launchers ====
 
-#define CHK(x)                                                   \
-    do {                                                         \
-        hipError_t e_ = (x);                                     \
-        if (e_ != hipSuccess) return (int)e_;                    \
-    } while (0)
-
 int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s)
 {
     const int nw = B.nw;
@@ -675,14 +816,16 @@ int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s)
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.cnt2, nw, B.off2, B.counts + 0);
     hipLaunchKernelGGL(k_fill_dimers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.cap2, B.nbr2, B.cnt2, B.off2,
                        B.dimers, B.dcap);
-    hipLaunchKernelGGL(k_trimers<false>, dim3((nw + 63) / 64), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3,
-                       B.cnt3, B.tcnt, B.toff, B.trimers, B.tcap);
+    hipLaunchKernelGGL(k_trimers<false>, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3, B.cnt3, B.tcnt,
+                       B.toff, B.trimers, B.tcap);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.tcnt, nw, B.toff, B.counts + 1);
-    hipLaunchKernelGGL(k_trimers<true>, dim3((nw + 63) / 64), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3,
-                       B.cnt3, B.tcnt, B.toff, B.trimers, B.tcap);
+    hipLaunchKernelGGL(k_trimers<true>, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3, B.cnt3, B.tcnt,
+                       B.toff, B.trimers, B.tcap);
     return (int)hipGetLastError();
 }
 
+// Cluster pipeline of the dimers; block partials of (energy, virial, stress) go to
+// B.blk[0 .. 8*ceil(nd/128)).
 int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipStream_t s)
 {
     if (nd <= 0) return 0;
@@ -691,43 +834,49 @@ int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipSt
     hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2);
     launch_pip2b_f64(B.xv2, B.part2, nd, B.cs2, s);
     hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, pip2b_nslices(), B.clus2, B.xv2, B.part2, B.e2,
-                       B.g2);
+                       B.g2, B.blk);
     return (int)hipGetLastError();
 }
 
-int mb_launch_trimers(MbBuffers& B, int nt, const CellGeom& g, bool gather, hipStream_t s)
+// Same for trimers; block partials after those of the dimers (offset from nd).
+int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gather, hipStream_t s)
 {
     if (nt <= 0) return 0;
     const int nb = (nt + 127) / 128;
+    const int nb2 = (nd + 127) / 128;
     if (gather) hipLaunchKernelGGL(k_gather3, dim3(nb), dim3(128), 0, s, nt, B.trimers, B.mon, g, B.clus3);
     hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3);
     launch_pip3b_f64(B.xv3, B.part3, nt, B.cs3, s);
     hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, pip3b_nslices(), B.clus3, B.xv3, B.part3, B.e3,
-                       B.g3);
+                       B.g3, B.blk + 8L * nb2);
     return (int)hipGetLastError();
 }
 
-// Accumulate the cluster records into per-atom sums and scalars; scal[0..7] = 2B
-// (energy, virial, stress 6), scal[8..15] = 3B.
+// Per-water gather of the cluster records into per-atom forces, energies and heat-flux
+// tensors; scal[0..7] = 2B (energy, virial, stress 6), scal[8..15] = 3B.
 int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s)
 {
-    const int na = B.natms;
-    hipLaunchKernelGGL(k_zero_ll, dim3(256), dim3(256), 0, s, B.acc, (long)26 * na);
-    const int nb2 = nd > 0 ? (nd + 255) / 256 : 0, nb3 = nt > 0 ? (nt + 255) / 256 : 0;
-    if (nb2 + nb3 > B.nblk_cap) return -1;
-    if (nb2) {
-        hipLaunchKernelGGL(k_accum<2>, dim3(nb2), dim3(256), 0, s, nd, B.dimers, B.list, B.clus2, B.e2, B.g2, B.x,
-                           B.y, B.z, B.acc, B.blk, na);
+    const int na = B.natms, nw = B.nw;
+    const long ne = 2L * nd + 3L * nt;
+    const int nb2 = (nd + 127) / 128, nb3 = (nt + 127) / 128;
+    if (nb2 + nb3 > B.nblk_cap || ne > B.inc_cap) return -1;
+    hipLaunchKernelGGL(k_zero_i, dim3((2 * nw + 255) / 256), dim3(256), 0, s, B.icnt, 2 * nw);  // icnt + icur
+    if (ne > 0) {
+        hipLaunchKernelGGL(k_inc_count, dim3((ne + 255) / 256), dim3(256), 0, s, nd, nt, B.dimers, B.trimers, B.icnt);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.icnt, nw, B.ioff, B.counts + 3);
+        hipLaunchKernelGGL(k_inc_fill, dim3((ne + 255) / 256), dim3(256), 0, s, nd, nt, B.dimers, B.trimers, B.ioff,
+                           B.icnt + nw, B.inc);
     }
-    if (nb3) {
-        hipLaunchKernelGGL(k_accum<3>, dim3(nb3), dim3(256), 0, s, nt, B.trimers, B.list, B.clus3, B.e3, B.g3,
-                           B.x, B.y, B.z, B.acc, B.blk + 8 * nb2, na);
-    }
-    hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(B.scal), 32);
-    if (nb2) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, B.blk, nb2, B.scal);
-    if (nb3) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, B.blk + 8 * nb2, nb3, B.scal + 8);
-    hipLaunchKernelGGL(k_finalize, dim3((na + 255) / 256), dim3(256), 0, s, na, B.acc, fx, fy, fz, B.fout,
+    hipLaunchKernelGGL(k_zero_i, dim3((na + 255) / 256), dim3(256), 0, s, reinterpret_cast<int*>(B.is_water), (na + 3) / 4);
+    hipLaunchKernelGGL(k_mark_water, dim3((nw + 255) / 256), dim3(256), 0, s, nw, B.list, B.is_water);
+    hipLaunchKernelGGL(k_zero_other_sites, dim3((na + 255) / 256), dim3(256), 0, s, na, B.is_water, B.fout,
                        B.atomic_energy, B.heat_w);
+    hipLaunchKernelGGL(k_gather_water, dim3(nw), dim3(64), 0, s, nw, nd, nt, B.list, B.dimers, B.trimers, B.ioff,
+                       B.icnt, B.inc, B.e2, B.g2, B.e3, B.g3, B.x, B.y, B.z, fx, fy, fz, B.fout, B.atomic_energy,
+                       B.heat_w);
+    hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(B.scal), 32);
+    if (nb2) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(512), 0, s, B.blk, nb2, B.scal);
+    if (nb3) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(512), 0, s, B.blk + 8L * nb2, nb3, B.scal + 8);
     return (int)hipGetLastError();
 }
 
@@ -739,6 +888,6 @@ int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const dou
     if (nb > nblk_cap) return -1;
     hipLaunchKernelGGL(k_heat_flux, dim3(nb), dim3(256), 0, s, nreal, vx, vy, vz, ekin, epot, epot2, w, w2, sew,
                        blk);
-    hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(64), 0, s, blk, nb, out);
+    hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(512), 0, s, blk, nb, out);
     return (int)hipGetLastError();
 }
