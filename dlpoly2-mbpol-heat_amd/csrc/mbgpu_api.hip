@@ -107,18 +107,20 @@ int alloc_fixed(mbgpu_ctx* c)
     MbBuffers& B = c->B;
     const int nw = B.nw, na = B.natms;
     B.cap2 = 128;
-    B.cap3 = 96;
+    B.cap3 = 64;
     HIPCHK(dalloc(&B.mon, (size_t)9 * nw));
     HIPCHK(dalloc(&B.nbr2, (size_t)nw * B.cap2));
     HIPCHK(dalloc(&B.cnt2, nw));
     HIPCHK(dalloc(&B.off2, nw));
     HIPCHK(dalloc(&B.nbr3, (size_t)nw * B.cap3));
+    HIPCHK(dalloc(&B.nbr2l, (size_t)nw * B.cap2));
+    HIPCHK(dalloc(&B.cnt2l, nw));
+    HIPCHK(dalloc(&B.kmask, (size_t)nw * 64));
+    HIPCHK(dalloc(&B.rowoff, (size_t)nw * 64));
     HIPCHK(dalloc(&B.cnt3, nw));
     HIPCHK(dalloc(&B.tcnt, nw));
     HIPCHK(dalloc(&B.toff, nw));
     HIPCHK(dalloc(&B.counts, 4));
-    HIPCHK(dalloc(&B.icnt, (size_t)2 * nw));
-    HIPCHK(dalloc(&B.ioff, nw));
     HIPCHK(dalloc(&B.is_water, (size_t)4 * ((na + 3) / 4)));
     HIPCHK(dalloc(&B.scal, 16));
     HIPCHK(dalloc(&B.atomic_energy, na));
@@ -175,11 +177,6 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
         HIPCHK(dalloc(&B.part3, (size_t)pip3b_nslices() * 37 * B.tcap));
         HIPCHK(dalloc(&B.e3, B.tcap));
         HIPCHK(dalloc(&B.g3, (size_t)27 * B.tcap));
-    }
-    if (2 * B.dcap + 3 * B.tcap > B.inc_cap) {
-        dfree(B.inc);
-        B.inc_cap = 2 * B.dcap + 3 * B.tcap;
-        HIPCHK(dalloc(&B.inc, B.inc_cap));
     }
     const int nblk = (int)((B.dcap + 127) / 128 + (B.tcap + 127) / 128 + (B.natms + 255) / 256 + 8);
     if (nblk > B.nblk_cap) {
@@ -325,7 +322,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
     dfree(B.clus2); dfree(B.xv2); dfree(B.part2); dfree(B.e2); dfree(B.g2);
     dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3);
-    dfree(B.icnt); dfree(B.ioff); dfree(B.inc); dfree(B.is_water); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
+    dfree(B.is_water); dfree(B.nbr2l); dfree(B.cnt2l); dfree(B.kmask); dfree(B.rowoff); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
     dfree(B.cs2); dfree(B.cs3);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);

@@ -27,9 +27,12 @@ struct MbBuffers {
     int* list = nullptr;           // [3*nw] listttm2 (1-based atom ids)
     double *x = nullptr, *y = nullptr, *z = nullptr;   // [natms] raw positions
     // neighbour lists
-    int *nbr2 = nullptr, *cnt2 = nullptr, *off2 = nullptr;
-    int *nbr3 = nullptr, *cnt3 = nullptr;
+    int *nbr2 = nullptr, *cnt2 = nullptr, *off2 = nullptr;   // partners j > i within 6.5 A
+    int *nbr2l = nullptr, *cnt2l = nullptr;                  // partners j < i within 6.5 A
+    int *nbr3 = nullptr, *cnt3 = nullptr;                    // partners within 4.5 A (64 per water)
     int *tcnt = nullptr, *toff = nullptr;
+    unsigned long long* kmask = nullptr;  // [nw][64] kept-pair masks of the trimer enumeration
+    int* rowoff = nullptr;                // [nw][64] kept pairs before each row
     int* counts = nullptr;         // [4]: ndimers, ntrimers, error flags, spare
     int* dimers = nullptr;         // [dcap][2]
     int* trimers = nullptr;        // [tcap][3]
@@ -39,9 +42,6 @@ struct MbBuffers {
     // accumulators
     long long* acc = nullptr;      // fixed-point sums: [2][13*natms] (high halves, low halves) of
                                    // forces (3n), energies (n), heat-flux tensors (9n)
-    int *icnt = nullptr, *ioff = nullptr, *inc = nullptr;  // per-water cluster incidence (CSR);
-                                   // icnt holds [counts | fill cursors], 2*nw
-    long inc_cap = 0;
     unsigned char* is_water = nullptr;  // natms (rounded up to 4)
     double* blk = nullptr;         // per-block partial scalars
     int nblk_cap = 0;

@@ -193,20 +193,23 @@ __global__ void k_image_monomers(int nw, const int* __restrict__ list, const dou
 
 // O-O neighbour lists.  One wavefront per water i scans all partners in chunks of 64; a
 // ballot + popcount compacts the hits in ascending partner order.
-//   list 2: j > i with |img(O_i - O_j)|^2 <= r2f^2          (do_dimer predicate)
-//   list 3: j != i with sqrt(|img(O_lo - O_hi)|^2) <= r3f    (do_trimer edge test)
+//   list 2:  j > i with |img(O_i - O_j)|^2 <= r2f^2          (do_dimer predicate)
+//   list 2l: j < i with the same predicate (the partners of which i is the second water)
+//   list 3:  j != i with sqrt(|img(O_lo - O_hi)|^2) <= r3f    (do_trimer edge test)
 __global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restrict__ mon, CellGeom g,
                                                    int cap2, int* __restrict__ nbr2, int* __restrict__ cnt2,
+                                                   int* __restrict__ nbr2l, int* __restrict__ cnt2l,
                                                    int cap3, int* __restrict__ nbr3, int* __restrict__ cnt3,
                                                    int* __restrict__ err)
 {
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     if (i >= nw) return;
-    int n2 = 0, n3 = 0;
+    int n2 = 0, n2l = 0, n3 = 0;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int base = 0; base < nw; base += 64) {
         const int j = base + lane;
-        bool in2 = false, in3 = false;
+        bool in2 = false, in2l = false, in3 = false;
         if (j < nw && j != i) {
             const int lo = min(i, j), hi = max(i, j);
             double dx = mon[9 * lo] - mon[9 * hi];
@@ -214,26 +217,33 @@ __global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restr
             double dz = mon[9 * lo + 2] - mon[9 * hi + 2];
             image_vec(g, dx, dy, dz);
             const double r2 = norm2_nc(dx, dy, dz);
-            in2 = (j > i) && !(r2 > MBGPU_R2F * MBGPU_R2F);
+            const bool pair = !(r2 > MBGPU_R2F * MBGPU_R2F);
+            in2 = (j > i) && pair;
+            in2l = (j < i) && pair;
             in3 = !(sqrt(r2) > MBGPU_R3F);
         }
-        const unsigned long long b2 = __ballot(in2), b3 = __ballot(in3);
-        const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        const unsigned long long b2 = __ballot(in2), b2l = __ballot(in2l), b3 = __ballot(in3);
         if (in2) {
             const int p = n2 + __popcll(b2 & below);
             if (p < cap2) nbr2[(long)i * cap2 + p] = j;
+        }
+        if (in2l) {
+            const int p = n2l + __popcll(b2l & below);
+            if (p < cap2) nbr2l[(long)i * cap2 + p] = j;
         }
         if (in3) {
             const int p = n3 + __popcll(b3 & below);
             if (p < cap3) nbr3[(long)i * cap3 + p] = j;
         }
         n2 += __popcll(b2);
+        n2l += __popcll(b2l);
         n3 += __popcll(b3);
     }
     if (lane == 0) {
         cnt2[i] = n2;
+        cnt2l[i] = n2l;
         cnt3[i] = n3;
-        if (n2 > cap2 || n3 > cap3) atomicOr(err, 1);
+        if (n2 > cap2 || n2l > cap2 || n3 > cap3) atomicOr(err, 1);
     }
 }
 
@@ -241,40 +251,44 @@ __global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restr
 // forms a triple with two short edges (c-a, c-b).  A triple whose third edge is short too
 // is reached from all three vertices and kept only from its smallest index.  This yields
 // exactly the triples do_trimer keeps: at least two of the three O-O edges <= r3f.
-// One wavefront per centre: the outer neighbour index runs sequentially, the inner one
-// across the lanes, and a ballot keeps the output in (a, b) order.
+// One wavefront per centre (at most 64 short neighbours, checked when the lists are
+// built): row p of the pair table runs sequentially, the partner index q across the lanes.
+// The count pass records, per row, the bit mask of kept partners and the number of kept
+// pairs before the row, so that the position of any kept pair in c's output range is
+// toff[c] + rowoff[c][p] + popcount(mask[c][p] below bit q); k_gather_water uses this to
+// find the trimers of a water without atomics.
 template <bool FILL>
-__global__ __launch_bounds__(64) void k_trimers(int nw, const double* __restrict__ mon, CellGeom g, int cap3,
+__global__ __launch_bounds__(64) void k_trimers(int nw, const double* __restrict__ mon, CellGeom g,
                                                 const int* __restrict__ nbr3, const int* __restrict__ cnt3,
                                                 int* __restrict__ tcnt, const int* __restrict__ toff,
+                                                unsigned long long* __restrict__ kmask, int* __restrict__ rowoff,
                                                 int* __restrict__ trimers, long tcap)
 {
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
     if (c >= nw) return;
-    const int n = min(cnt3[c], cap3);
-    const int* S = nbr3 + (long)c * cap3;
+    const int n = min(cnt3[c], 64);
+    const int* S = nbr3 + (long)c * 64;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int cnt = 0;
     const long o = FILL ? toff[c] : 0;
     for (int p = 0; p + 1 < n; ++p) {
         const int a = S[p];
-        const double ax = mon[9 * a], ay = mon[9 * a + 1], az = mon[9 * a + 2];
-        for (int base = p + 1; base < n; base += 64) {
-            const int q = base + lane;
-            bool keep = false;
-            int b = 0;
-            if (q < n) {
-                b = S[q];
-                double dx = ax - mon[9 * b];
-                double dy = ay - mon[9 * b + 1];
-                double dz = az - mon[9 * b + 2];
-                image_vec(g, dx, dy, dz);
-                const double rab = sqrt(norm2_nc(dx, dy, dz));
-                keep = (rab > MBGPU_R3F) || (c < a);
-            }
-            const unsigned long long bal = __ballot(keep);
-            if (FILL && keep) {
+        const int q = p + 1 + lane;
+        bool keep = false;
+        int b = 0;
+        if (q < n) {
+            b = S[q];
+            double dx = mon[9 * a] - mon[9 * b];
+            double dy = mon[9 * a + 1] - mon[9 * b + 1];
+            double dz = mon[9 * a + 2] - mon[9 * b + 2];
+            image_vec(g, dx, dy, dz);
+            const double rab = sqrt(norm2_nc(dx, dy, dz));
+            keep = (rab > MBGPU_R3F) || (c < a);
+        }
+        const unsigned long long bal = __ballot(keep);
+        if (FILL) {
+            if (keep) {
                 int i0, i1, i2;
                 if (c < a) { i0 = c; i1 = a; i2 = b; }
                 else if (c < b) { i0 = a; i1 = c; i2 = b; }
@@ -286,10 +300,19 @@ __global__ __launch_bounds__(64) void k_trimers(int nw, const double* __restrict
                     trimers[3 * t + 2] = i2;
                 }
             }
-            cnt += __popcll(bal);
+        } else if (lane == 0) {
+            kmask[(long)c * 64 + p] = bal << (p + 1);
+            rowoff[(long)c * 64 + p] = cnt;
+        }
+        cnt += __popcll(bal);
+    }
+    if (!FILL && lane == 0) {
+        tcnt[c] = cnt;
+        for (int p = max(n - 1, 0); p < 64; ++p) {
+            kmask[(long)c * 64 + p] = 0ull;
+            rowoff[(long)c * 64 + p] = cnt;
         }
     }
-    if (!FILL && lane == 0) tcnt[c] = cnt;
 }
 
 // Exclusive prefix sum of n counts (one block of 1024 threads); total to *total.
@@ -599,112 +622,155 @@ __global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* 
     block_reduce_store(red, 8, blk + 8 * blockIdx.x);
 }
 
-// ------------------------------------------------------------ per-water incidence ----
-// Every (cluster, slot) pair is entered in the list of the water at that slot:
-// entry = (global cluster index << 2) | slot, dimers first (0..nd-1), then trimers.
-__global__ void k_inc_count(int nd, int nt, const int* __restrict__ dimers, const int* __restrict__ trimers,
-                            int* __restrict__ icnt)
+// position of value v in the ascending list L[0..n) (v is known to be present)
+__device__ __forceinline__ int find_rank(const int* __restrict__ L, int n, int v)
 {
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (e < 2L * nd) atomicAdd(icnt + dimers[e], 1);
-    else if (e < 2L * nd + 3L * nt) atomicAdd(icnt + trimers[e - 2L * nd], 1);
-}
-
-__global__ void k_inc_fill(int nd, int nt, const int* __restrict__ dimers, const int* __restrict__ trimers,
-                           const int* __restrict__ ioff, int* __restrict__ icur, int* __restrict__ inc)
-{
-    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    int w, entry;
-    if (e < 2L * nd) {
-        w = dimers[e];
-        entry = (int)((e >> 1) << 2) | (int)(e & 1);
-    } else if (e < 2L * nd + 3L * nt) {
-        const long t = e - 2L * nd;
-        w = trimers[t];
-        entry = (int)((nd + t / 3) << 2) | (int)(t % 3);
-    } else {
-        return;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (L[mid] < v) lo = mid + 1;
+        else hi = mid;
     }
-    inc[ioff[w] + atomicAdd(icur + w, 1)] = entry;
+    return lo;
 }
 
 // Per-water gather of the cluster records (replaces update_energy_mbpol / fxx updates /
-// update_forces of do_dimer and do_trimer).  One wavefront per water; each lane sums its
-// share of the water's clusters in 2^-32 fixed point (order-independent, hence
-// deterministic), then the lanes are combined exactly.  Per atom a of the water:
+// update_forces of do_dimer and do_trimer).  One wavefront per water w; the clusters of w
+// are found from the lists, without atomics:
+//   dimers (w, j), j > w:  off2[w] + rank of j in nbr2[w]         (contiguous)
+//   dimers (j, w), j < w:  off2[j] + rank of w in nbr2[j]          (j from list 2l)
+//   trimers centred on w:  toff[w] .. toff[w] + tcnt[w]            (contiguous)
+//   trimers with w as a leaf of centre c (c a short neighbour of w): the kept pairs of c
+//     that contain w's rank r in c's short list, indexed through kmask / rowoff.
+// Each lane sums its share in 2^-32 fixed point (order-independent, hence deterministic)
+// and the lanes are combined exactly.  Per atom a of w:
 //   f_a   += -g_a * engunit
 //   e_a   += E * engunit / k                          (k = 6 dimer, 9 trimer)
 //   W_a   += F_a (x) sum_{b in cluster, b != a} (x_a - x_b),  F_a = -(g_a / k) engunit
 // with raw (un-imaged) positions x, as compute_heat_flux2 contracts force_matrix.
-__global__ __launch_bounds__(64) void k_gather_water(int nw, int nd, int nt, const int* __restrict__ list,
-                                                     const int* __restrict__ dimers, const int* __restrict__ trimers,
-                                                     const int* __restrict__ ioff, const int* __restrict__ icnt,
-                                                     const int* __restrict__ inc, const double* __restrict__ e2,
-                                                     const double* __restrict__ g2, const double* __restrict__ e3,
-                                                     const double* __restrict__ g3, const double* __restrict__ x,
-                                                     const double* __restrict__ y, const double* __restrict__ z,
-                                                     double* __restrict__ fx, double* __restrict__ fy,
-                                                     double* __restrict__ fz, double* __restrict__ fout,
-                                                     double* __restrict__ ae, double* __restrict__ hw)
+struct GatherArgs {
+    int nw, nd, nt, cap2;
+    const int *list, *dimers, *trimers;
+    const int *nbr2, *cnt2, *off2, *nbr2l, *cnt2l;
+    const int *nbr3, *cnt3, *toff, *tcnt, *rowoff;
+    const unsigned long long* kmask;
+    const double *e2, *g2, *e3, *g3, *x, *y, *z;
+    double *fx, *fy, *fz, *fout, *ae, *hw;
+};
+
+__global__ __launch_bounds__(64) void k_gather_water(GatherArgs A)
 {
     const int w = blockIdx.x;
     const int lane = threadIdx.x;
-    if (w >= nw) return;
-    const int o = list[3 * w] - 1;
+    if (w >= A.nw) return;
+    const int o = A.list[3 * w] - 1;
     double xa[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        xa[a][0] = x[o + a];
-        xa[a][1] = y[o + a];
-        xa[a][2] = z[o + a];
+        xa[a][0] = A.x[o + a];
+        xa[a][1] = A.y[o + a];
+        xa[a][2] = A.z[o + a];
     }
     // fixed-point accumulators: 9 force components, 1 energy, 27 tensor components
     long long hi[37], lo[37];
 #pragma unroll
     for (int k = 0; k < 37; ++k) hi[k] = lo[k] = 0;
-    auto acc = [&](int k, double v) {
-        const long long q = __double2ll_rn(v * MBGPU_FX_SCALE);
-        hi[k] += q >> 32;
-        lo[k] += q & 0xffffffffll;
-    };
-    const int n = icnt[w], base = ioff[w];
-    for (int p = lane; p < n; p += 64) {
-        const int entry = inc[base + p];
-        const int cg = entry >> 2, slot = entry & 3;
-        const bool tri = cg >= nd;
-        const int c = tri ? cg - nd : cg;
+
+    auto add_cluster = [&](bool tri, long c, int slot) {
         const int nmon = tri ? 3 : 2;
         const double frac = tri ? 1.0 / 9.0 : 1.0 / 6.0;
         const double kk = tri ? 9.0 : 6.0;
-        const double E = tri ? e3[c] : e2[c];
-        const double* G = tri ? g3 : g2;
-        const long nc = tri ? nt : nd;
-        const int* cl = tri ? trimers + 3L * c : dimers + 2L * c;
-        // sum of the raw positions of all atoms of the cluster
+        const double E = tri ? A.e3[c] : A.e2[c];
+        const double* G = tri ? A.g3 : A.g2;
+        const long nc = tri ? A.nt : A.nd;
+        const int* cl = tri ? A.trimers + 3 * c : A.dimers + 2 * c;
         double S[3] = {0, 0, 0};
         for (int m = 0; m < nmon; ++m) {
-            const int om = list[3 * cl[m]] - 1;
+            const int om = A.list[3 * cl[m]] - 1;
             for (int a = 0; a < 3; ++a) {
-                S[0] += x[om + a];
-                S[1] += y[om + a];
-                S[2] += z[om + a];
+                S[0] += A.x[om + a];
+                S[1] += A.y[om + a];
+                S[2] += A.z[om + a];
             }
         }
+        auto acc = [&](int k, double v) {
+            const long long q = __double2ll_rn(v * MBGPU_FX_SCALE);
+            hi[k] += q >> 32;
+            lo[k] += q & 0xffffffffll;
+        };
         acc(9, frac * E * MBGPU_ENGUNIT);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            double ga[3], fa[3];
+            double fa[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                ga[q] = G[(long)(9 * slot + 3 * a + q) * nc + c];
-                acc(3 * a + q, -(ga[q] * MBGPU_ENGUNIT));
-                fa[q] = -(frac * ga[q]) * MBGPU_ENGUNIT;
+                const double ga = G[(long)(9 * slot + 3 * a + q) * nc + c];
+                acc(3 * a + q, -(ga * MBGPU_ENGUNIT));
+                fa[q] = -(frac * ga) * MBGPU_ENGUNIT;
             }
             const double rr[3] = {kk * xa[a][0] - S[0], kk * xa[a][1] - S[1], kk * xa[a][2] - S[2]};
 #pragma unroll
             for (int pp = 0; pp < 3; ++pp)
 #pragma unroll
                 for (int q = 0; q < 3; ++q) acc(10 + 9 * a + 3 * pp + q, fa[pp] * rr[q]);
+        }
+    };
+    auto tslot = [&](long t) {
+        const int* tr = A.trimers + 3 * t;
+        return tr[0] == w ? 0 : (tr[1] == w ? 1 : 2);
+    };
+
+    // dimers in which w is the first water
+    {
+        const int n = A.cnt2[w];
+        for (int p = lane; p < n; p += 64) add_cluster(false, (long)A.off2[w] + p, 0);
+    }
+    // dimers in which w is the second water
+    {
+        const int n = A.cnt2l[w];
+        for (int p = lane; p < n; p += 64) {
+            const int j = A.nbr2l[(long)w * A.cap2 + p];
+            const int r = find_rank(A.nbr2 + (long)j * A.cap2, A.cnt2[j], w);
+            add_cluster(false, (long)A.off2[j] + r, 1);
+        }
+    }
+    // trimers centred on w
+    {
+        const int n = A.tcnt[w];
+        for (int p = lane; p < n; p += 64) {
+            const long t = (long)A.toff[w] + p;
+            add_cluster(true, t, tslot(t));
+        }
+    }
+    // trimers in which w is a leaf: one lane per centre c among w's short neighbours
+    {
+        const int n3 = min(A.cnt3[w], 64);
+        for (int p = lane; p < n3; p += 64) {
+            const int c = A.nbr3[(long)w * 64 + p];
+            const int nc3 = min(A.cnt3[c], 64);
+            const int r = find_rank(A.nbr3 + (long)c * 64, nc3, w);
+            const unsigned long long* km = A.kmask + (long)c * 64;
+            const int* ro = A.rowoff + (long)c * 64;
+            const long base = A.toff[c];
+            // pairs (r, q), q > r
+            unsigned long long m = km[r];
+            const long t0 = base + ro[r];
+            int k = 0;
+            while (m) {
+                add_cluster(true, t0 + k, tslot(t0 + k));
+                m &= m - 1;
+                ++k;
+            }
+            // pairs (p, r), p < r
+            const unsigned long long bit = 1ull << r;
+            const unsigned long long lower = bit - 1ull;
+            for (int pp = 0; pp < r; ++pp) {
+                const unsigned long long mp = km[pp];
+                if (mp & bit) {
+                    const long t = base + ro[pp] + __popcll(mp & lower);
+                    add_cluster(true, t, tslot(t));
+                }
+            }
         }
     }
     // exact combination across the wavefront
@@ -723,19 +789,29 @@ __global__ __launch_bounds__(64) void k_gather_water(int nw, int nd, int nt, con
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int i = o + a;
-            fout[3 * i] = v[3 * a];
-            fout[3 * i + 1] = v[3 * a + 1];
-            fout[3 * i + 2] = v[3 * a + 2];
-            if (fx) {
-                fx[i] += v[3 * a];
-                fy[i] += v[3 * a + 1];
-                fz[i] += v[3 * a + 2];
+            A.fout[3 * i] = v[3 * a];
+            A.fout[3 * i + 1] = v[3 * a + 1];
+            A.fout[3 * i + 2] = v[3 * a + 2];
+            if (A.fx) {
+                A.fx[i] += v[3 * a];
+                A.fy[i] += v[3 * a + 1];
+                A.fz[i] += v[3 * a + 2];
             }
-            ae[i] = v[9];
+            A.ae[i] = v[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) hw[9L * i + k] = v[10 + 9 * a + k];
+            for (int k = 0; k < 9; ++k) A.hw[9L * i + k] = v[10 + 9 * a + k];
         }
     }
+}
+
+// Sum the nsl slice partials of every (variable, cluster) into slice 0, in slice order.
+__global__ void k_slice_sum(long n, int nsl, double* __restrict__ part)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = part[i];
+    for (int q = 1; q < nsl; ++q) s += part[q * n + i];
+    part[i] = s;
 }
 
 // Zero the per-atom outputs of the sites that are not water atoms (M sites).
@@ -811,16 +887,16 @@ int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s)
     hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, B.counts, 4);
     hipLaunchKernelGGL(k_image_monomers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.list, B.x, B.y, B.z, g,
                        B.mon);
-    hipLaunchKernelGGL(k_neighbours, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap2, B.nbr2, B.cnt2, B.cap3,
-                       B.nbr3, B.cnt3, B.counts + 2);
+    hipLaunchKernelGGL(k_neighbours, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap2, B.nbr2, B.cnt2, B.nbr2l,
+                       B.cnt2l, 64, B.nbr3, B.cnt3, B.counts + 2);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.cnt2, nw, B.off2, B.counts + 0);
     hipLaunchKernelGGL(k_fill_dimers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.cap2, B.nbr2, B.cnt2, B.off2,
                        B.dimers, B.dcap);
-    hipLaunchKernelGGL(k_trimers<false>, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3, B.cnt3, B.tcnt,
-                       B.toff, B.trimers, B.tcap);
+    hipLaunchKernelGGL(k_trimers<false>, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.nbr3, B.cnt3, B.tcnt, B.toff,
+                       B.kmask, B.rowoff, B.trimers, B.tcap);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.tcnt, nw, B.toff, B.counts + 1);
-    hipLaunchKernelGGL(k_trimers<true>, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap3, B.nbr3, B.cnt3, B.tcnt,
-                       B.toff, B.trimers, B.tcap);
+    hipLaunchKernelGGL(k_trimers<true>, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.nbr3, B.cnt3, B.tcnt, B.toff,
+                       B.kmask, B.rowoff, B.trimers, B.tcap);
     return (int)hipGetLastError();
 }
 
@@ -833,8 +909,9 @@ int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipSt
     if (gather) hipLaunchKernelGGL(k_gather2, dim3(nb), dim3(128), 0, s, nd, B.dimers, B.mon, g, B.clus2);
     hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2);
     launch_pip2b_f64(B.xv2, B.part2, nd, B.cs2, s);
-    hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, pip2b_nslices(), B.clus2, B.xv2, B.part2, B.e2,
-                       B.g2, B.blk);
+    const long n = 32L * nd;
+    hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2);
+    hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, 1, B.clus2, B.xv2, B.part2, B.e2, B.g2, B.blk);
     return (int)hipGetLastError();
 }
 
@@ -847,8 +924,10 @@ int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gath
     if (gather) hipLaunchKernelGGL(k_gather3, dim3(nb), dim3(128), 0, s, nt, B.trimers, B.mon, g, B.clus3);
     hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3);
     launch_pip3b_f64(B.xv3, B.part3, nt, B.cs3, s);
-    hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, pip3b_nslices(), B.clus3, B.xv3, B.part3, B.e3,
-                       B.g3, B.blk + 8L * nb2);
+    const long n = 37L * nt;
+    hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3);
+    hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, 1, B.clus3, B.xv3, B.part3, B.e3, B.g3,
+                       B.blk + 8L * nb2);
     return (int)hipGetLastError();
 }
 
@@ -857,23 +936,21 @@ int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gath
 int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s)
 {
     const int na = B.natms, nw = B.nw;
-    const long ne = 2L * nd + 3L * nt;
     const int nb2 = (nd + 127) / 128, nb3 = (nt + 127) / 128;
-    if (nb2 + nb3 > B.nblk_cap || ne > B.inc_cap) return -1;
-    hipLaunchKernelGGL(k_zero_i, dim3((2 * nw + 255) / 256), dim3(256), 0, s, B.icnt, 2 * nw);  // icnt + icur
-    if (ne > 0) {
-        hipLaunchKernelGGL(k_inc_count, dim3((ne + 255) / 256), dim3(256), 0, s, nd, nt, B.dimers, B.trimers, B.icnt);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.icnt, nw, B.ioff, B.counts + 3);
-        hipLaunchKernelGGL(k_inc_fill, dim3((ne + 255) / 256), dim3(256), 0, s, nd, nt, B.dimers, B.trimers, B.ioff,
-                           B.icnt + nw, B.inc);
-    }
-    hipLaunchKernelGGL(k_zero_i, dim3((na + 255) / 256), dim3(256), 0, s, reinterpret_cast<int*>(B.is_water), (na + 3) / 4);
+    if (nb2 + nb3 > B.nblk_cap) return -1;
+    hipLaunchKernelGGL(k_zero_i, dim3((na + 1023) / 1024), dim3(256), 0, s, reinterpret_cast<int*>(B.is_water),
+                       (na + 3) / 4);
     hipLaunchKernelGGL(k_mark_water, dim3((nw + 255) / 256), dim3(256), 0, s, nw, B.list, B.is_water);
     hipLaunchKernelGGL(k_zero_other_sites, dim3((na + 255) / 256), dim3(256), 0, s, na, B.is_water, B.fout,
                        B.atomic_energy, B.heat_w);
-    hipLaunchKernelGGL(k_gather_water, dim3(nw), dim3(64), 0, s, nw, nd, nt, B.list, B.dimers, B.trimers, B.ioff,
-                       B.icnt, B.inc, B.e2, B.g2, B.e3, B.g3, B.x, B.y, B.z, fx, fy, fz, B.fout, B.atomic_energy,
-                       B.heat_w);
+    GatherArgs A;
+    A.nw = nw; A.nd = nd; A.nt = nt; A.cap2 = B.cap2;
+    A.list = B.list; A.dimers = B.dimers; A.trimers = B.trimers;
+    A.nbr2 = B.nbr2; A.cnt2 = B.cnt2; A.off2 = B.off2; A.nbr2l = B.nbr2l; A.cnt2l = B.cnt2l;
+    A.nbr3 = B.nbr3; A.cnt3 = B.cnt3; A.toff = B.toff; A.tcnt = B.tcnt; A.rowoff = B.rowoff; A.kmask = B.kmask;
+    A.e2 = B.e2; A.g2 = B.g2; A.e3 = B.e3; A.g3 = B.g3; A.x = B.x; A.y = B.y; A.z = B.z;
+    A.fx = fx; A.fy = fy; A.fz = fz; A.fout = B.fout; A.ae = B.atomic_energy; A.hw = B.heat_w;
+    hipLaunchKernelGGL(k_gather_water, dim3(nw), dim3(64), 0, s, A);
     hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(B.scal), 32);
     if (nb2) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(512), 0, s, B.blk, nb2, B.scal);
     if (nb3) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(512), 0, s, B.blk + 8L * nb2, nb3, B.scal + 8);
