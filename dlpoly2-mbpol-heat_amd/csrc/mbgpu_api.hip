@@ -155,6 +155,7 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
         dfree(B.part2);
         dfree(B.e2);
         dfree(B.g2);
+        dfree(B.s2);
         B.dcap = std::max<long>(nd + nd / 2, 1024);
         HIPCHK(dalloc(&B.dimers, (size_t)2 * B.dcap));
         HIPCHK(dalloc(&B.clus2, (size_t)18 * B.dcap));
@@ -162,6 +163,7 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
         HIPCHK(dalloc(&B.part2, (size_t)pip2b_nslices() * 32 * B.dcap));
         HIPCHK(dalloc(&B.e2, B.dcap));
         HIPCHK(dalloc(&B.g2, (size_t)18 * B.dcap));
+        HIPCHK(dalloc(&B.s2, (size_t)3 * B.dcap));
     }
     if (nt > B.tcap) {
         dfree(B.trimers);
@@ -170,6 +172,7 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
         dfree(B.part3);
         dfree(B.e3);
         dfree(B.g3);
+        dfree(B.s3);
         B.tcap = std::max<long>(nt + nt / 2, 1024);
         HIPCHK(dalloc(&B.trimers, (size_t)3 * B.tcap));
         HIPCHK(dalloc(&B.clus3, (size_t)27 * B.tcap));
@@ -177,6 +180,7 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
         HIPCHK(dalloc(&B.part3, (size_t)pip3b_nslices() * 37 * B.tcap));
         HIPCHK(dalloc(&B.e3, B.tcap));
         HIPCHK(dalloc(&B.g3, (size_t)27 * B.tcap));
+        HIPCHK(dalloc(&B.s3, (size_t)3 * B.tcap));
     }
     const int nblk = (int)((B.dcap + 127) / 128 + (B.tcap + 127) / 128 + (B.natms + 255) / 256 + 8);
     if (nblk > B.nblk_cap) {
@@ -320,8 +324,8 @@ void mbgpu_destroy(mbgpu_ctx* c)
     MbBuffers& B = c->B;
     dfree(B.mon); dfree(B.nbr2); dfree(B.cnt2); dfree(B.off2); dfree(B.nbr3); dfree(B.cnt3);
     dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
-    dfree(B.clus2); dfree(B.xv2); dfree(B.part2); dfree(B.e2); dfree(B.g2);
-    dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3);
+    dfree(B.clus2); dfree(B.xv2); dfree(B.part2); dfree(B.e2); dfree(B.g2); dfree(B.s2);
+    dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3); dfree(B.s3);
     dfree(B.is_water); dfree(B.nbr2l); dfree(B.cnt2l); dfree(B.kmask); dfree(B.rowoff); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
     dfree(B.cs2); dfree(B.cs3);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);

@@ -37,8 +37,8 @@ struct MbBuffers {
     int* dimers = nullptr;         // [dcap][2]
     int* trimers = nullptr;        // [tcap][3]
     // cluster data
-    double *clus2 = nullptr, *xv2 = nullptr, *part2 = nullptr, *e2 = nullptr, *g2 = nullptr;
-    double *clus3 = nullptr, *xv3 = nullptr, *part3 = nullptr, *e3 = nullptr, *g3 = nullptr;
+    double *clus2 = nullptr, *xv2 = nullptr, *part2 = nullptr, *e2 = nullptr, *g2 = nullptr, *s2 = nullptr;
+    double *clus3 = nullptr, *xv3 = nullptr, *part3 = nullptr, *e3 = nullptr, *g3 = nullptr, *s3 = nullptr;
     // accumulators
     long long* acc = nullptr;      // fixed-point sums: [2][13*natms] (high halves, low halves) of
                                    // forces (3n), energies (n), heat-flux tensors (9n)

@@ -462,13 +462,42 @@ __device__ __forceinline__ void cluster_virial(const double* w, const double* g,
 // Sum the slice partials, apply the chain rule through the variables and extra points and
 // the switch (mbpol-2b-poly.cpp:447-682).  Record: E (kcal/mol) and dE/dx of the 6 atoms;
 // block partials of energy, virial and stress.
+// sum of the raw (un-imaged) positions of the atoms of the NMON waters ids[0..NMON)
+template <int NMON>
+__device__ __forceinline__ void cluster_raw_sum(const int* ids, const int* __restrict__ list,
+                                                const double* __restrict__ x, const double* __restrict__ y,
+                                                const double* __restrict__ z, double* S)
+{
+    S[0] = S[1] = S[2] = 0.0;
+#pragma unroll
+    for (int m = 0; m < NMON; ++m) {
+        const int o = list[3 * ids[m]] - 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            S[0] += x[o + a];
+            S[1] += y[o + a];
+            S[2] += z[o + a];
+        }
+    }
+}
+
 __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* __restrict__ clus,
                                                  const double* __restrict__ xv, const double* __restrict__ part,
                                                  double* __restrict__ erec, double* __restrict__ grec,
-                                                 double* __restrict__ blk)
+                                                 double* __restrict__ blk, const int* __restrict__ dimers,
+                                                 const int* __restrict__ list, const double* __restrict__ x,
+                                                 const double* __restrict__ y, const double* __restrict__ z,
+                                                 double* __restrict__ srec)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (d < nd && dimers) {
+        double S[3];
+        cluster_raw_sum<2>(dimers + 2L * d, list, x, y, z, S);
+        srec[d] = S[0];
+        srec[(long)nd + d] = S[1];
+        srec[2L * nd + d] = S[2];
+    }
     if (d < nd) {
         double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
 #pragma unroll
@@ -558,10 +587,20 @@ __global__ void k_setup3(int nt, const double* __restrict__ clus, double* __rest
 __global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* __restrict__ clus,
                                                  const double* __restrict__ xv, const double* __restrict__ part,
                                                  double* __restrict__ erec, double* __restrict__ grec,
-                                                 double* __restrict__ blk)
+                                                 double* __restrict__ blk, const int* __restrict__ trimers,
+                                                 const int* __restrict__ list, const double* __restrict__ x,
+                                                 const double* __restrict__ y, const double* __restrict__ z,
+                                                 double* __restrict__ srec)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (t < nt && trimers) {
+        double S[3];
+        cluster_raw_sum<3>(trimers + 3L * t, list, x, y, z, S);
+        srec[t] = S[0];
+        srec[(long)nt + t] = S[1];
+        srec[2L * nt + t] = S[2];
+    }
     if (t < nt) {
         double w[27];
 #pragma unroll
@@ -635,34 +674,67 @@ __device__ __forceinline__ int find_rank(const int* __restrict__ L, int n, int v
 }
 
 // Per-water gather of the cluster records (replaces update_energy_mbpol / fxx updates /
-// update_forces of do_dimer and do_trimer).  One wavefront per water w; the clusters of w
-// are found from the lists, without atomics:
-//   dimers (w, j), j > w:  off2[w] + rank of j in nbr2[w]         (contiguous)
-//   dimers (j, w), j < w:  off2[j] + rank of w in nbr2[j]          (j from list 2l)
-//   trimers centred on w:  toff[w] .. toff[w] + tcnt[w]            (contiguous)
-//   trimers with w as a leaf of centre c (c a short neighbour of w): the kept pairs of c
-//     that contain w's rank r in c's short list, indexed through kmask / rowoff.
-// Each lane sums its share in 2^-32 fixed point (order-independent, hence deterministic)
-// and the lanes are combined exactly.  Per atom a of w:
+// update_forces of do_dimer and do_trimer).  One block of 256 threads per water w; the
+// clusters of w are found from the lists, without atomics on global memory:
+//   A  dimers (w, j), j > w:  off2[w] + rank of j in nbr2[w]         (contiguous)
+//   B  dimers (j, w), j < w:  off2[j] + rank of w in nbr2[j]          (j from list 2l)
+//   C  trimers centred on w:  toff[w] .. toff[w] + tcnt[w]            (contiguous)
+//   D  trimers with w as a leaf of centre c (c a short neighbour of w): the kept pairs of
+//      c that contain w's rank r in c's short list, through kmask / rowoff; listed in LDS
+//      first (one thread per centre), then processed like the others.
+// Each thread sums its items in 2^-32 fixed point (order-independent, hence
+// deterministic) and the block combines the thread sums exactly.  Per atom a of w:
 //   f_a   += -g_a * engunit
 //   e_a   += E * engunit / k                          (k = 6 dimer, 9 trimer)
-//   W_a   += F_a (x) sum_{b in cluster, b != a} (x_a - x_b),  F_a = -(g_a / k) engunit
-// with raw (un-imaged) positions x, as compute_heat_flux2 contracts force_matrix.
+//   W_a   += F_a (x) (k x_a - S_c),  F_a = -(g_a / k) engunit,  S_c = sum of the raw
+//            positions of the cluster's atoms, i.e. F_a (x) sum_{b != a} (x_a - x_b) as
+//            compute_heat_flux2 contracts force_matrix with un-imaged positions.
 struct GatherArgs {
     int nw, nd, nt, cap2;
     const int *list, *dimers, *trimers;
     const int *nbr2, *cnt2, *off2, *nbr2l, *cnt2l;
     const int *nbr3, *cnt3, *toff, *tcnt, *rowoff;
     const unsigned long long* kmask;
-    const double *e2, *g2, *e3, *g3, *x, *y, *z;
+    const double *e2, *g2, *s2, *e3, *g3, *s3, *x, *y, *z;
     double *fx, *fy, *fz, *fout, *ae, *hw;
 };
 
-__global__ __launch_bounds__(64) void k_gather_water(GatherArgs A)
+#define GATHER_THREADS 256
+#define GATHER_MAXLEAF 4096   // >= 64 centres x 63 pairs: cannot overflow with 64-entry short lists
+
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather_water(GatherArgs A)
 {
+    __shared__ int leaf[GATHER_MAXLEAF];
+    __shared__ int nleaf;
+    __shared__ long long red[GATHER_THREADS / 64][74];
     const int w = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (w >= A.nw) return;
+    const int tid = threadIdx.x;
+    if (tid == 0) nleaf = 0;
+    __syncthreads();
+    // D: list the leaf trimers, one thread per centre
+    {
+        const int n3 = min(A.cnt3[w], 64);
+        if (tid < n3) {
+            const int c = A.nbr3[(long)w * 64 + tid];
+            const int nc3 = min(A.cnt3[c], 64);
+            const int r = find_rank(A.nbr3 + (long)c * 64, nc3, w);
+            const unsigned long long* km = A.kmask + (long)c * 64;
+            const int* ro = A.rowoff + (long)c * 64;
+            const int base = A.toff[c];
+            const int k0 = __popcll(km[r]);
+            const unsigned long long bit = 1ull << r, lower = bit - 1ull;
+            int cnt = k0;
+            for (int pp = 0; pp < r; ++pp) cnt += (km[pp] & bit) ? 1 : 0;
+            int pos = atomicAdd(&nleaf, cnt);
+            for (int k = 0; k < k0; ++k)
+                if (pos < GATHER_MAXLEAF) leaf[pos++] = base + ro[r] + k;
+            for (int pp = 0; pp < r; ++pp) {
+                const unsigned long long mp = km[pp];
+                if ((mp & bit) && pos < GATHER_MAXLEAF) leaf[pos++] = base + ro[pp] + __popcll(mp & lower);
+            }
+        }
+    }
+    __syncthreads();
     const int o = A.list[3 * w] - 1;
     double xa[3][3];
 #pragma unroll
@@ -671,33 +743,42 @@ __global__ __launch_bounds__(64) void k_gather_water(GatherArgs A)
         xa[a][1] = A.y[o + a];
         xa[a][2] = A.z[o + a];
     }
-    // fixed-point accumulators: 9 force components, 1 energy, 27 tensor components
     long long hi[37], lo[37];
 #pragma unroll
     for (int k = 0; k < 37; ++k) hi[k] = lo[k] = 0;
-
-    auto add_cluster = [&](bool tri, long c, int slot) {
-        const int nmon = tri ? 3 : 2;
+    auto acc = [&](int k, double v) {
+        const long long q = __double2ll_rn(v * MBGPU_FX_SCALE);
+        hi[k] += q >> 32;
+        lo[k] += q & 0xffffffffll;
+    };
+    const int nA = A.cnt2[w], nB = A.cnt2l[w], nC = A.tcnt[w], nD = min(nleaf, GATHER_MAXLEAF);
+    const int ntot = nA + nB + nC + nD;
+    for (int it = tid; it < ntot; it += GATHER_THREADS) {
+        bool tri;
+        long c;
+        int slot;
+        if (it < nA) {
+            tri = false;
+            c = (long)A.off2[w] + it;
+            slot = 0;
+        } else if (it < nA + nB) {
+            const int j = A.nbr2l[(long)w * A.cap2 + (it - nA)];
+            tri = false;
+            c = (long)A.off2[j] + find_rank(A.nbr2 + (long)j * A.cap2, A.cnt2[j], w);
+            slot = 1;
+        } else {
+            tri = true;
+            c = it < nA + nB + nC ? (long)A.toff[w] + (it - nA - nB) : (long)leaf[it - nA - nB - nC];
+            const int* tr = A.trimers + 3 * c;
+            slot = tr[0] == w ? 0 : (tr[1] == w ? 1 : 2);
+        }
         const double frac = tri ? 1.0 / 9.0 : 1.0 / 6.0;
         const double kk = tri ? 9.0 : 6.0;
+        const long nc = tri ? A.nt : A.nd;
         const double E = tri ? A.e3[c] : A.e2[c];
         const double* G = tri ? A.g3 : A.g2;
-        const long nc = tri ? A.nt : A.nd;
-        const int* cl = tri ? A.trimers + 3 * c : A.dimers + 2 * c;
-        double S[3] = {0, 0, 0};
-        for (int m = 0; m < nmon; ++m) {
-            const int om = A.list[3 * cl[m]] - 1;
-            for (int a = 0; a < 3; ++a) {
-                S[0] += A.x[om + a];
-                S[1] += A.y[om + a];
-                S[2] += A.z[om + a];
-            }
-        }
-        auto acc = [&](int k, double v) {
-            const long long q = __double2ll_rn(v * MBGPU_FX_SCALE);
-            hi[k] += q >> 32;
-            lo[k] += q & 0xffffffffll;
-        };
+        const double* Sp = tri ? A.s3 : A.s2;
+        const double S[3] = {Sp[c], Sp[nc + c], Sp[2 * nc + c]};
         acc(9, frac * E * MBGPU_ENGUNIT);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -714,92 +795,37 @@ __global__ __launch_bounds__(64) void k_gather_water(GatherArgs A)
 #pragma unroll
                 for (int q = 0; q < 3; ++q) acc(10 + 9 * a + 3 * pp + q, fa[pp] * rr[q]);
         }
-    };
-    auto tslot = [&](long t) {
-        const int* tr = A.trimers + 3 * t;
-        return tr[0] == w ? 0 : (tr[1] == w ? 1 : 2);
-    };
-
-    // dimers in which w is the first water
-    {
-        const int n = A.cnt2[w];
-        for (int p = lane; p < n; p += 64) add_cluster(false, (long)A.off2[w] + p, 0);
     }
-    // dimers in which w is the second water
-    {
-        const int n = A.cnt2l[w];
-        for (int p = lane; p < n; p += 64) {
-            const int j = A.nbr2l[(long)w * A.cap2 + p];
-            const int r = find_rank(A.nbr2 + (long)j * A.cap2, A.cnt2[j], w);
-            add_cluster(false, (long)A.off2[j] + r, 1);
-        }
-    }
-    // trimers centred on w
-    {
-        const int n = A.tcnt[w];
-        for (int p = lane; p < n; p += 64) {
-            const long t = (long)A.toff[w] + p;
-            add_cluster(true, t, tslot(t));
-        }
-    }
-    // trimers in which w is a leaf: one lane per centre c among w's short neighbours
-    {
-        const int n3 = min(A.cnt3[w], 64);
-        for (int p = lane; p < n3; p += 64) {
-            const int c = A.nbr3[(long)w * 64 + p];
-            const int nc3 = min(A.cnt3[c], 64);
-            const int r = find_rank(A.nbr3 + (long)c * 64, nc3, w);
-            const unsigned long long* km = A.kmask + (long)c * 64;
-            const int* ro = A.rowoff + (long)c * 64;
-            const long base = A.toff[c];
-            // pairs (r, q), q > r
-            unsigned long long m = km[r];
-            const long t0 = base + ro[r];
-            int k = 0;
-            while (m) {
-                add_cluster(true, t0 + k, tslot(t0 + k));
-                m &= m - 1;
-                ++k;
-            }
-            // pairs (p, r), p < r
-            const unsigned long long bit = 1ull << r;
-            const unsigned long long lower = bit - 1ull;
-            for (int pp = 0; pp < r; ++pp) {
-                const unsigned long long mp = km[pp];
-                if (mp & bit) {
-                    const long t = base + ro[pp] + __popcll(mp & lower);
-                    add_cluster(true, t, tslot(t));
-                }
-            }
-        }
-    }
-    // exact combination across the wavefront
+    // exact combination: wavefront shuffles, then the wavefronts through LDS
+    const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
     for (int k = 0; k < 37; ++k) {
         for (int off = 32; off > 0; off >>= 1) {
             hi[k] += __shfl_down(hi[k], off, 64);
             lo[k] += __shfl_down(lo[k], off, 64);
         }
+        if (lane == 0) {
+            red[wid][k] = hi[k];
+            red[wid][37 + k] = lo[k];
+        }
     }
-    if (lane == 0) {
-        const double inv = 1.0 / MBGPU_FX_SCALE;
-        double v[37];
-#pragma unroll
-        for (int k = 0; k < 37; ++k) v[k] = (double)hi[k] + (double)lo[k] * inv;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int i = o + a;
-            A.fout[3 * i] = v[3 * a];
-            A.fout[3 * i + 1] = v[3 * a + 1];
-            A.fout[3 * i + 2] = v[3 * a + 2];
-            if (A.fx) {
-                A.fx[i] += v[3 * a];
-                A.fy[i] += v[3 * a + 1];
-                A.fz[i] += v[3 * a + 2];
-            }
-            A.ae[i] = v[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) A.hw[9L * i + k] = v[10 + 9 * a + k];
+    __syncthreads();
+    if (tid < 37) {
+        long long h = 0, l = 0;
+        for (int q = 0; q < GATHER_THREADS / 64; ++q) {
+            h += red[q][tid];
+            l += red[q][37 + tid];
+        }
+        const double v = (double)h + (double)l * (1.0 / MBGPU_FX_SCALE);
+        if (tid < 9) {
+            const int i = o + tid / 3, q = tid % 3;
+            A.fout[3 * i + q] = v;
+            if (A.fx) (q == 0 ? A.fx : (q == 1 ? A.fy : A.fz))[i] += v;
+        } else if (tid == 9) {
+            A.ae[o] = A.ae[o + 1] = A.ae[o + 2] = v;
+        } else {
+            const int k = tid - 10, a = k / 9;
+            A.hw[9L * (o + a) + (k % 9)] = v;
         }
     }
 }
@@ -911,7 +937,8 @@ int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipSt
     launch_pip2b_f64(B.xv2, B.part2, nd, B.cs2, s);
     const long n = 32L * nd;
     hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2);
-    hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, 1, B.clus2, B.xv2, B.part2, B.e2, B.g2, B.blk);
+    hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, 1, B.clus2, B.xv2, B.part2, B.e2, B.g2, B.blk,
+                       gather ? B.dimers : nullptr, B.list, B.x, B.y, B.z, B.s2);
     return (int)hipGetLastError();
 }
 
@@ -927,7 +954,7 @@ int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gath
     const long n = 37L * nt;
     hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3);
     hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, 1, B.clus3, B.xv3, B.part3, B.e3, B.g3,
-                       B.blk + 8L * nb2);
+                       B.blk + 8L * nb2, gather ? B.trimers : nullptr, B.list, B.x, B.y, B.z, B.s3);
     return (int)hipGetLastError();
 }
 
@@ -948,9 +975,10 @@ int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, d
     A.list = B.list; A.dimers = B.dimers; A.trimers = B.trimers;
     A.nbr2 = B.nbr2; A.cnt2 = B.cnt2; A.off2 = B.off2; A.nbr2l = B.nbr2l; A.cnt2l = B.cnt2l;
     A.nbr3 = B.nbr3; A.cnt3 = B.cnt3; A.toff = B.toff; A.tcnt = B.tcnt; A.rowoff = B.rowoff; A.kmask = B.kmask;
-    A.e2 = B.e2; A.g2 = B.g2; A.e3 = B.e3; A.g3 = B.g3; A.x = B.x; A.y = B.y; A.z = B.z;
+    A.e2 = B.e2; A.g2 = B.g2; A.s2 = B.s2; A.e3 = B.e3; A.g3 = B.g3; A.s3 = B.s3;
+    A.x = B.x; A.y = B.y; A.z = B.z;
     A.fx = fx; A.fy = fy; A.fz = fz; A.fout = B.fout; A.ae = B.atomic_energy; A.hw = B.heat_w;
-    hipLaunchKernelGGL(k_gather_water, dim3(nw), dim3(64), 0, s, A);
+    hipLaunchKernelGGL(k_gather_water, dim3(nw), dim3(GATHER_THREADS), 0, s, A);
     hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(B.scal), 32);
     if (nb2) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(512), 0, s, B.blk, nb2, B.scal);
     if (nb3) hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(512), 0, s, B.blk + 8L * nb2, nb3, B.scal + 8);
