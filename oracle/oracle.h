@@ -86,6 +86,69 @@ void ora_set_leaves(ora_leaf2_fn f2, ora_leaf3_fn f3);
 /* image_vectors (mbpol.f90:1305-1361) */
 int ora_image_vectors(int imcon, const double *cell, int n, double *dx);
 
+/* ---- polarizable electrostatics (elec_oracle.c) ------------------------------ */
+
+/* erfcc.f, gammq.f (gammln, gammq), erfcgenp.f with the grid of parset.f:210-217
+ * (ora_erfc_grid returns mxegrd; the table holds (mxegrd+1)*4 doubles, row i at r=i*drewd). */
+double ora_erfcc(double x);
+double ora_gammln(double x);
+double ora_gammq(double a, double x);
+int ora_erfc_grid(double rcut, double *drewd);
+void ora_erfc_table(double alpha, double rcut, double *table);
+
+/* DMS table (dms_table.txt: 84 rows "i1 i2 i3 coef"); returns 0 on success. */
+int ora_dms_load(const char *path);
+/* DMS charges of one water (w9: O, H1, H2 xyz) -> q(H1), q(H2), q(M) (qdetermine.f90). */
+int ora_dms_charges(int imcon, const double *cell, const double *w9, double gamma, double *q3);
+
+typedef struct {
+    /* inputs */
+    int imcon;                 /* 1, 2, 3 */
+    double cell[9];
+    int nw;                    /* waters; sites O,H1,H2 of water w at 3w..3w+2, M at 3nw+w */
+    double *x, *y, *z;         /* 4*nw positions; the M-site entries are (re)computed */
+    double rcut, alpha;        /* real-space cutoff, Ewald alpha (1/A) */
+    int kmax[3];               /* Ewald kmax1..3 */
+    double toler, omega;       /* dipole tolerance (e A), SOR mixing (old-dipole weight) */
+    int max_iter;              /* stop after this many iterations (reference: 22581) */
+    double gamma, q_o;         /* M-site gamma; oxygen charge from the FIELD */
+    double polr_site[3];       /* smearing polarizabilities O, H, M (A^3) */
+    double polr2_site[3];      /* induction polarizabilities O, H, M (A^3) */
+    double athole, athole12, athole13, ascc, ascd;
+    int compat_qd_column;      /* 1: the M column of qdforce's force-matrix entries is
+                                  n_water+imol as in qdforce.f90:332-337 */
+    /* in/out */
+    double *dip;               /* 3*4nw dipoles: start values in, converged out */
+    /* outputs (site level, before the M-site redistribution; f and the per-atom arrays
+     * accumulate, the scalars are set) */
+    double *f;                 /* 3*4nw interleaved forces, internal units */
+    double *chge;              /* 4nw charges */
+    double *vesp;              /* 4nw electrostatic potential at the sites */
+    double *efield;            /* optional 3*4nw total field at convergence */
+    double *f_ew1p;            /* optional 3*4nw reciprocal-space forces (force_ew1p) */
+    double eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe;
+    double vir_cpe, vir_ew1p, vir_qd, stress[9];
+    double stress_ew1p[9];     /* the reciprocal-space part of stress */
+    double *e_ew1p, *e_ew2p, *e_ew3p, *e_polar;  /* optional 4nw per-atom energies */
+    double *s_ew1p;            /* optional 4nw*9 per-atom reciprocal stress [i][a][b] */
+    double *heat_w;            /* optional 3nw*9 per-atom W_i over the electrostatic
+                                  force-matrix entries (M rows redistributed) */
+    double *force_matrix;      /* optional dense [k][j][i] over 4nw sites (M rows
+                                  redistributed) */
+    long npairs;               /* site pairs within rcut */
+    int niter;                 /* solver iterations */
+    double last_delta;         /* RMS dipole change of the last iteration */
+    double volm;
+} ora_elec_args;
+
+/* The polarizable-Ewald part of forces (forces.f:268-404,585-610,700-712,874-892) with
+ * qdetermine and the M sites.  Returns 0, -1 (imcon), -2 (no convergence within
+ * max_iter: the reference stops), -3 (DMS table not loaded). */
+int ora_elec_forces(ora_elec_args *a);
+
+/* M-site redistribution of forces (3 per site), energies and 9-component stresses. */
+void ora_msite_distribute(int nw, double gamma, double *f, double *e, double *s9);
+
 /* ---- heat current (heat_oracle.c) ------------------------------------------ */
 
 /* compute_heat_flux2 (heatcurrent.f90:703-728) from the dense force matrix, layout

@@ -148,3 +148,126 @@ def heat_flux_tensor(box, heat_w, vel, e_kin, e_pot, stress_ew1p=None):
     oracle().ora_heat_flux_tensor(box.nreal, P(hw), P(v[0]), P(v[1]), P(v[2]), P(np.ascontiguousarray(e_kin)),
                                   P(np.ascontiguousarray(e_pot)), P(stress_ew1p), P(jk), P(jp), P(jt))
     return jk, jp, jt
+
+
+# ---------------------------------------------------------------- electrostatics ----
+
+class ElecArgs(ctypes.Structure):
+    _fields_ = [("imcon", ctypes.c_int), ("cell", ctypes.c_double * 9), ("nw", ctypes.c_int),
+                ("x", _dp), ("y", _dp), ("z", _dp), ("rcut", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("kmax", ctypes.c_int * 3), ("toler", ctypes.c_double), ("omega", ctypes.c_double),
+                ("max_iter", ctypes.c_int), ("gamma", ctypes.c_double), ("q_o", ctypes.c_double),
+                ("polr_site", ctypes.c_double * 3), ("polr2_site", ctypes.c_double * 3),
+                ("athole", ctypes.c_double), ("athole12", ctypes.c_double), ("athole13", ctypes.c_double),
+                ("ascc", ctypes.c_double), ("ascd", ctypes.c_double), ("compat_qd_column", ctypes.c_int),
+                ("dip", _dp), ("f", _dp), ("chge", _dp), ("vesp", _dp), ("efield", _dp), ("f_ew1p", _dp),
+                ("eng_ew1p", ctypes.c_double), ("eng_ew2p", ctypes.c_double), ("eng_ew3p", ctypes.c_double),
+                ("eng_polar", ctypes.c_double), ("eng_cpe", ctypes.c_double), ("vir_cpe", ctypes.c_double),
+                ("vir_ew1p", ctypes.c_double), ("vir_qd", ctypes.c_double), ("stress", ctypes.c_double * 9),
+                ("stress_ew1p", ctypes.c_double * 9),
+                ("e_ew1p", _dp), ("e_ew2p", _dp), ("e_ew3p", _dp), ("e_polar", _dp), ("s_ew1p", _dp),
+                ("heat_w", _dp), ("force_matrix", _dp), ("npairs", ctypes.c_long), ("niter", ctypes.c_int),
+                ("last_delta", ctypes.c_double), ("volm", ctypes.c_double)]
+
+
+_dms_loaded = False
+
+
+def elec_lib():
+    global _dms_loaded
+    lib = oracle()
+    if not _dms_loaded:
+        for name in ("ora_erfcc", "ora_gammln", "ora_gammq"):
+            getattr(lib, name).restype = ctypes.c_double
+        lib.ora_erfcc.argtypes = [ctypes.c_double]
+        lib.ora_gammln.argtypes = [ctypes.c_double]
+        lib.ora_gammq.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.ora_erfc_grid.argtypes = [ctypes.c_double, _dp]
+        lib.ora_erfc_table.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
+        lib.ora_msite_distribute.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        if lib.ora_dms_load(os.path.join(DATA, "dms_table.txt").encode()) != 0:
+            raise RuntimeError("oracle could not load the DMS table")
+        _dms_loaded = True
+    return lib
+
+
+def erfc_table(alpha, rcut):
+    lib = elec_lib()
+    dr = ctypes.c_double()
+    n = lib.ora_erfc_grid(rcut, ctypes.byref(dr))
+    t = np.zeros((n + 1) * 4)
+    lib.ora_erfc_table(alpha, rcut, P(t))
+    return t.reshape(n + 1, 4), dr.value
+
+
+def dms_charges(w9, gamma=0.426706882, imcon=1, cell=None):
+    lib = elec_lib()
+    cell = np.asarray(cell if cell is not None else [100.0, 0, 0, 0, 100.0, 0, 0, 0, 100.0], float)
+    q = np.zeros(3)
+    rc = lib.ora_dms_charges(imcon, P(cell), P(np.ascontiguousarray(w9, float)), ctypes.c_double(gamma), P(q))
+    assert rc == 0
+    return q
+
+
+def site_xyz(box):
+    """4*nw site positions (O, H1, H2 per water, then M sites), interleaved (n, 3)."""
+    return np.ascontiguousarray(box.xyz[:4 * box.nw], float).copy()
+
+
+def elec_forces(box, p, dip0=None, dense=False, xyz=None, cell=None, imcon=None, distribute=True):
+    """Run the oracle's polarizable electrostatics on a WaterBox with ElecParams p.
+    Returns a dict; per-atom arrays are redistributed from the M sites when
+    distribute=True (forces of real atoms, energies, ew1p stresses)."""
+    lib = elec_lib()
+    nw = box.nw
+    n = 4 * nw
+    a = ElecArgs()
+    a.imcon = box.imcon if imcon is None else imcon
+    c = box.cell if cell is None else cell
+    for q in range(9):
+        a.cell[q] = c[q]
+    a.nw = nw
+    r = site_xyz(box) if xyz is None else np.array(xyz, float)
+    x, y, z = (np.ascontiguousarray(r[:, k]) for k in range(3))
+    a.x, a.y, a.z = P(x), P(y), P(z)
+    a.rcut, a.alpha = p.rcut, p.alpha
+    for q in range(3):
+        a.kmax[q] = p.kmax[q]
+    a.toler, a.omega, a.max_iter = p.toler, p.omega, p.max_iter
+    a.gamma, a.q_o = p.gamma, p.q_o
+    for q in range(3):
+        a.polr_site[q] = p.polr[q]
+        a.polr2_site[q] = p.polr2[q]
+    a.athole, a.athole12, a.athole13, a.ascc, a.ascd = p.athole, p.athole12, p.athole13, p.ascc, p.ascd
+    a.compat_qd_column = p.compat_qd_column
+    dip = np.zeros(3 * n) if dip0 is None else np.array(dip0, float).reshape(-1).copy()
+    f = np.zeros(3 * n)
+    chge, vesp, efield, fk = np.zeros(n), np.zeros(n), np.zeros(3 * n), np.zeros(3 * n)
+    e1, e2, e3, ep = (np.zeros(n) for _ in range(4))
+    s1 = np.zeros(9 * n)
+    hw = np.zeros(9 * 3 * nw)
+    fm = np.zeros(3 * n * n) if dense else None
+    a.dip, a.f, a.chge, a.vesp, a.efield, a.f_ew1p = P(dip), P(f), P(chge), P(vesp), P(efield), P(fk)
+    a.e_ew1p, a.e_ew2p, a.e_ew3p, a.e_polar, a.s_ew1p = P(e1), P(e2), P(e3), P(ep), P(s1)
+    a.heat_w, a.force_matrix = P(hw), P(fm)
+    rc = lib.ora_elec_forces(ctypes.byref(a))
+    if rc != 0:
+        raise RuntimeError(f"oracle elec_forces returned {rc} (niter {a.niter})")
+    out = dict(f_sites=f.reshape(n, 3).copy(), chge=chge, vesp=vesp, dip=dip.reshape(n, 3), efield=efield.reshape(n, 3),
+               eng_ew1p=a.eng_ew1p, eng_ew2p=a.eng_ew2p, eng_ew3p=a.eng_ew3p, eng_polar=a.eng_polar,
+               eng=a.eng_cpe, vir=a.vir_cpe, vir_ew1p=a.vir_ew1p, vir_qd=a.vir_qd, stress=np.array(a.stress[:]),
+               stress_ew1p=np.array(a.stress_ew1p[:]),
+               e_sites=np.stack([e1, e2, e3, ep]), s_ew1p_sites=s1.reshape(n, 9).copy(), heat_w=hw.reshape(3 * nw, 3, 3),
+               npairs=a.npairs, niter=a.niter, last_delta=a.last_delta, xyz=np.stack([x, y, z], 1))
+    out["f_ew1p_sites"] = fk.reshape(n, 3).copy()
+    if distribute:
+        e = (e1 + e2 + e3 + ep).copy()
+        lib.ora_msite_distribute(nw, p.gamma, P(f), P(e), P(s1))
+        lib.ora_msite_distribute(nw, p.gamma, P(fk), None, None)
+        out["f_ew1p"] = fk.reshape(n, 3)[:3 * nw].copy()
+        out["f"] = f.reshape(n, 3)[:3 * nw].copy()
+        out["atomic_energy"] = e[:3 * nw].copy()
+        out["s_ew1p"] = s1.reshape(n, 9)[:3 * nw].copy()
+    if dense:
+        out["force_matrix"] = fm
+    return out
