@@ -1,0 +1,493 @@
+// elec_api.hip -- host-side set-up and driving of the electrostatics kernels
+// (elec_kernels.hip): buffers, the B-function table, the k-vector list, the solver loop
+// and the assembly of the scalar results.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/mbgpu.h"
+#include "mbgpu_internal.h"
+
+#define R4PIE0 138935.4835
+#define SQRPI 1.7724538509055159
+#define PI_D 3.141592653589793
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+#define ECHK(x)                                                                            \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            err = std::string(#x) + ": " + hipGetErrorString(e_);                          \
+            return (int)e_;                                                                \
+        }                                                                                  \
+    } while (0)
+
+// erfcc.f with its default-REAL literals (float values), as the table builder uses it
+double erfcc_host(double x)
+{
+    const double z = fabs(x);
+    const double t = 1.0 / (1.0 + 0.5 * z);
+    double r = t * exp(-z * z - (double)1.26551223f +
+                       t * ((double)1.00002368f +
+                            t * ((double).37409196f +
+                                 t * ((double).09678418f +
+                                      t * ((double)-.18628806f +
+                                           t * ((double).27886807f +
+                                                t * ((double)-1.13520398f +
+                                                     t * ((double)1.48851587f +
+                                                          t * ((double)-0.82215223f + t * (double).17087277f)))))))));
+    return x < 0.0 ? 2.0 - r : r;
+}
+
+// gammln of gammq.f (Lanczos)
+double gammln_host(double xx)
+{
+    static const double cof[6] = {76.18009172947146, -86.50532032941677, 24.01409824083091,
+                                  -1.231739572450155, .1208650973866179e-2, -.5395239384953e-5};
+    double x = xx, y = x;
+    double tmp = x + 5.5;
+    tmp = (x + 0.5) * log(tmp) - tmp;
+    double ser = 1.000000000190015;
+    for (int j = 0; j < 6; ++j) {
+        y = y + 1.0;
+        ser = ser + cof[j] / y;
+    }
+    return tmp + log(2.5066282746310005 * ser / x);
+}
+
+void free_buffers(ElecBuffers& E)
+{
+    dfree(E.x); dfree(E.y); dfree(E.z); dfree(E.chge); dfree(E.polr); dfree(E.polr2); dfree(E.dg); dfree(E.tab);
+    dfree(E.nbrA); dfree(E.cntA); dfree(E.nbrP); dfree(E.cntP); dfree(E.tens); dfree(E.tin);
+    dfree(E.dip); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd); dfree(E.state); dfree(E.zero_state);
+    dfree(E.kv); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs); dfree(E.pht);
+    dfree(E.S); dfree(E.Spart); dfree(E.Sq); dfree(E.kc); dfree(E.kf); dfree(E.Fpart);
+    dfree(E.vk); dfree(E.vr); dfree(E.vs); dfree(E.vc); dfree(E.vdck); dfree(E.vdcr); dfree(E.vesp);
+    dfree(E.fk); dfree(E.f2); dfree(E.f3); dfree(E.fsite); dfree(E.e1); dfree(E.e2); dfree(E.e3); dfree(E.ep);
+    dfree(E.s1); dfree(E.w2); dfree(E.w3); dfree(E.fmsum); dfree(E.fmx); dfree(E.eout); dfree(E.wout);
+    dfree(E.sout); dfree(E.blk); dfree(E.scal); dfree(E.err);
+    E.n = E.capA = E.capP = E.K = E.nrun = 0;
+}
+
+// site-sized buffers (allocated once per context size)
+int alloc_sites(ElecState& S, int nw, std::string& err)
+{
+    ElecBuffers& E = S.E;
+    const int n = 4 * nw, nreal = 3 * nw;
+    E.n = n;
+    ECHK(dalloc(&E.x, n)); ECHK(dalloc(&E.y, n)); ECHK(dalloc(&E.z, n));
+    ECHK(dalloc(&E.chge, n)); ECHK(dalloc(&E.dg, (size_t)6 * nw)); ECHK(dalloc(&E.tin, (size_t)36 * nw));
+    ECHK(dalloc(&E.cntA, n)); ECHK(dalloc(&E.cntP, n));
+    ECHK(dalloc(&E.dip, (size_t)3 * n)); ECHK(dalloc(&E.efk, (size_t)3 * n)); ECHK(dalloc(&E.emu, (size_t)3 * n));
+    ECHK(dalloc(&E.efdc, (size_t)3 * n)); ECHK(dalloc(&E.efdd, (size_t)3 * n));
+    ECHK(dalloc(&E.state, 8)); ECHK(dalloc(&E.zero_state, 8));
+    ECHK(dalloc(&E.vk, n)); ECHK(dalloc(&E.vr, n)); ECHK(dalloc(&E.vs, n)); ECHK(dalloc(&E.vc, n));
+    ECHK(dalloc(&E.vdck, n)); ECHK(dalloc(&E.vdcr, n)); ECHK(dalloc(&E.vesp, n));
+    ECHK(dalloc(&E.fk, (size_t)3 * n)); ECHK(dalloc(&E.f2, (size_t)3 * n)); ECHK(dalloc(&E.f3, (size_t)3 * n));
+    ECHK(dalloc(&E.fsite, (size_t)3 * n));
+    ECHK(dalloc(&E.e1, n)); ECHK(dalloc(&E.e2, n)); ECHK(dalloc(&E.e3, n)); ECHK(dalloc(&E.ep, n));
+    ECHK(dalloc(&E.s1, (size_t)9 * n)); ECHK(dalloc(&E.w2, (size_t)9 * n)); ECHK(dalloc(&E.w3, (size_t)9 * n));
+    ECHK(dalloc(&E.fmsum, (size_t)3 * n)); ECHK(dalloc(&E.fmx, (size_t)9 * n));
+    ECHK(dalloc(&E.eout, nreal)); ECHK(dalloc(&E.wout, (size_t)9 * nreal)); ECHK(dalloc(&E.sout, (size_t)9 * nreal));
+    ECHK(dalloc(&E.scal, 64)); ECHK(dalloc(&E.err, 4)); ECHK(dalloc(&S.rcell, 9));
+    ECHK(hipMemset(E.zero_state, 0, 8 * sizeof(double)));
+    ECHK(hipMemset(E.w2, 0, sizeof(double) * 9 * n));
+    ECHK(hipMemset(E.fmsum, 0, sizeof(double) * 3 * n));
+    ECHK(hipMemset(E.fmx, 0, sizeof(double) * 9 * n));
+    S.nw = nw;
+    return 0;
+}
+
+int alloc_rows(ElecState& S, int capA, int capP, std::string& err)
+{
+    ElecBuffers& E = S.E;
+    dfree(E.nbrA);
+    dfree(E.nbrP);
+    dfree(E.tens);
+    E.capA = capA;
+    E.capP = capP;
+    ECHK(dalloc(&E.nbrA, (size_t)E.n * capA));
+    ECHK(dalloc(&E.nbrP, (size_t)E.n * capP));
+    ECHK(dalloc(&E.tens, (size_t)6 * E.n * capP));
+    return 0;
+}
+
+// erfcgenp.f (table of B0..B3 at r_i = i * drewd, i = 1..mxegrd), grid of parset.f:210-217
+int build_table(ElecState& S, double alpha, double rcut, std::string& err)
+{
+    ElecBuffers& E = S.E;
+    int mxegrd = (int)(rcut / 0.0001 + 0.5) + 4;
+    if (mxegrd < 1000) mxegrd = 1000;
+    const double drewd = rcut / (double)(mxegrd - 4);
+    std::vector<double> t((size_t)4 * (mxegrd + 1), 0.0);
+    for (int i = 1; i <= mxegrd; ++i) {
+        const double rrr = (double)i * drewd;
+        const double rsqi = 1.0 / rrr / rrr;
+        const double a = alpha * rrr;
+        double* e = &t[(size_t)4 * i];
+        e[0] = (erfcc_host(a) - 1.0) / rrr;
+        const double exp2a = exp(-a * a);
+        for (int n = 1; n <= 3; ++n) {
+            const double fn = (double)n;
+            e[n] = rsqi * ((2.0 * fn - 1.0) * e[n - 1] + pow(2.0 * alpha * alpha, fn) * exp2a / alpha / SQRPI);
+        }
+    }
+    dfree(E.tab);
+    ECHK(dalloc(&E.tab, t.size()));
+    ECHK(hipMemcpy(E.tab, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    E.ntab = mxegrd;
+    S.P.drewd = drewd;
+    S.tab_alpha = alpha;
+    S.tab_rcut = rcut;
+    return 0;
+}
+
+// k vectors of ewald1p.f:151-159, 248-345 in the reference's order (l = 0..kmax1, then
+// m, then n, half space), grouped into runs of consecutive n at fixed (l, m).
+int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha, std::string& err)
+{
+    ElecBuffers& E = S.E;
+    const double* a = cell;
+    double rc[9];
+    rc[0] = a[4] * a[8] - a[5] * a[7];
+    rc[1] = a[2] * a[7] - a[1] * a[8];
+    rc[2] = a[1] * a[5] - a[2] * a[4];
+    rc[3] = a[5] * a[6] - a[3] * a[8];
+    rc[4] = a[0] * a[8] - a[2] * a[6];
+    rc[5] = a[2] * a[3] - a[0] * a[5];
+    rc[6] = a[3] * a[7] - a[4] * a[6];
+    rc[7] = a[1] * a[6] - a[0] * a[7];
+    rc[8] = a[0] * a[4] - a[1] * a[3];
+    const double det = a[0] * rc[0] + a[3] * rc[1] + a[6] * rc[2];
+    if (fabs(det) < 1e-6) {
+        err = "cell is singular";
+        return -1;
+    }
+    for (int i = 0; i < 9; ++i) rc[i] /= det;
+    const double volm = fabs(det);
+    // dcell(rcell): widths of the reciprocal cell
+    const double ax[3] = {rc[0], rc[1], rc[2]}, bx[3] = {rc[3], rc[4], rc[5]}, cx[3] = {rc[6], rc[7], rc[8]};
+    const double bc[3] = {bx[1] * cx[2] - bx[2] * cx[1], bx[2] * cx[0] - bx[0] * cx[2], bx[0] * cx[1] - bx[1] * cx[0]};
+    const double ca[3] = {cx[1] * ax[2] - cx[2] * ax[1], cx[2] * ax[0] - cx[0] * ax[2], cx[0] * ax[1] - cx[1] * ax[0]};
+    const double ab[3] = {ax[1] * bx[2] - ax[2] * bx[1], ax[2] * bx[0] - ax[0] * bx[2], ax[0] * bx[1] - ax[1] * bx[0]};
+    const double vol = fabs(ax[0] * bc[0] + ax[1] * bc[1] + ax[2] * bc[2]);
+    const double w0 = vol / sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
+    const double w1 = vol / sqrt(ca[0] * ca[0] + ca[1] * ca[1] + ca[2] * ca[2]);
+    const double w2 = vol / sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+    const double twopi = 2.0 * PI_D;
+    double rcpcut = std::min(std::min(kmax[0] * w0, kmax[1] * w1), kmax[2] * w2);
+    rcpcut = rcpcut * 1.05 * twopi;
+    const double rcpct2 = rcpcut * rcpcut;
+    const double ralph = -0.25 / (alpha * alpha);
+
+    std::vector<double> kv;
+    std::vector<int> runs, runk;
+    int K = 0;
+    int mmin = 0, nmin = 1;
+    for (int ll = 0; ll <= kmax[0]; ++ll) {
+        const double tl = twopi * (double)ll;
+        const double rkx1 = tl * rc[0], rky1 = tl * rc[3], rkz1 = tl * rc[6];
+        for (int mm = mmin; mm <= kmax[1]; ++mm) {
+            const double tm = twopi * (double)mm;
+            const double rkx2 = rkx1 + tm * rc[1], rky2 = rky1 + tm * rc[4], rkz2 = rkz1 + tm * rc[7];
+            int run_open = -1;
+            for (int nn = nmin; nn <= kmax[2]; ++nn) {
+                const double tn = twopi * (double)nn;
+                const double k0 = rkx2 + tn * rc[2], k1 = rky2 + tn * rc[5], k2 = rkz2 + tn * rc[8];
+                const double rksq = k0 * k0 + k1 * k1 + k2 * k2;
+                if (!(rksq <= rcpct2)) {
+                    run_open = -1;
+                    continue;
+                }
+                const double rrksq = 1.0 / rksq;
+                const double akk = exp(ralph * rksq) * rrksq;
+                const double akv = 2.0 * akk * (rrksq - ralph);
+                const double akv2 = 2.0 * (rrksq - ralph);
+                const double rec[8] = {k0, k1, k2, akk, akv, akv2, 0.0, 0.0};
+                kv.insert(kv.end(), rec, rec + 8);
+                if (run_open < 0 || runs[4 * run_open + 3] == 8) {
+                    run_open = (int)runk.size();
+                    const int r4[4] = {ll, mm, nn, 0};
+                    runs.insert(runs.end(), r4, r4 + 4);
+                    runk.push_back(K);
+                }
+                runs[4 * run_open + 3] += 1;
+                ++K;
+            }
+            nmin = -kmax[2];
+        }
+        mmin = -kmax[1];
+    }
+    dfree(E.kv);
+    dfree(E.runs);
+    dfree(E.runk);
+    dfree(E.S);
+    dfree(E.Sq);
+    dfree(E.kc);
+    dfree(E.kf);
+    dfree(E.Spart);
+    E.K = K;
+    E.nrun = (int)runk.size();
+    ECHK(dalloc(&E.kv, kv.size()));
+    ECHK(dalloc(&E.runs, runs.size()));
+    ECHK(dalloc(&E.runk, runk.size()));
+    ECHK(hipMemcpy(E.kv, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice));
+    ECHK(hipMemcpy(E.runs, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice));
+    ECHK(hipMemcpy(E.runk, runk.data(), sizeof(int) * runk.size(), hipMemcpyHostToDevice));
+    E.schunks = (E.n + 63) / 64;
+    ECHK(dalloc(&E.S, (size_t)8 * K));
+    ECHK(dalloc(&E.Sq, (size_t)2 * K));
+    ECHK(dalloc(&E.kc, (size_t)16 * K));
+    ECHK(dalloc(&E.kf, (size_t)16 * K));
+    ECHK(dalloc(&E.Spart, (size_t)E.schunks * K * 8));
+    ECHK(hipMemset(E.S, 0, sizeof(double) * 8 * K));
+    ECHK(hipMemset(E.kc, 0, sizeof(double) * 16 * K));
+    // phase tables
+    const int ks = std::max(std::max(kmax[0], kmax[1]), kmax[2]) + 1;
+    if (ks != E.kstride) {
+        dfree(E.phc);
+        dfree(E.phs);
+        dfree(E.pht);
+        E.kstride = ks;
+        ECHK(dalloc(&E.phc, (size_t)3 * ks * E.n));
+        ECHK(dalloc(&E.phs, (size_t)3 * ks * E.n));
+        ECHK(dalloc(&E.pht, (size_t)E.n * 3 * ks * 2));
+    }
+    // k split of the site-parallel passes: about 1024 wavefronts over the sites
+    E.kchunks = std::max(1, std::min(E.nrun, 1024 / std::max(1, (E.n + 63) / 64)));
+    dfree(E.Fpart);
+    ECHK(dalloc(&E.Fpart, (size_t)E.kchunks * 16 * E.n));
+    // block partials of the final passes
+    const int nsb = (E.n + 255) / 256, nkb = (K + 255) / 256, nwb = (S.nw + 63) / 64;
+    dfree(E.blk);
+    E.nblk = nsb + 8 * nkb + 2 * nsb + 8 * E.n + 16 * nwb + 2 * nsb + 64;
+    ECHK(dalloc(&E.blk, (size_t)E.nblk));
+    ECHK(hipMemcpy(S.rcell, rc, sizeof(double) * 9, hipMemcpyHostToDevice));
+    S.P.rvolm = twopi / volm;
+    for (int i = 0; i < 9; ++i) S.k_cell[i] = cell[i];
+    for (int i = 0; i < 3; ++i) S.k_kmax[i] = kmax[i];
+    S.k_alpha = alpha;
+    return 0;
+}
+
+}  // namespace
+
+void elec_release(ElecState& S)
+{
+    free_buffers(S.E);
+    dfree(S.rcell);
+    S.ready = false;
+}
+
+// Prepare buffers, tables and the k list for a call; positions and dipoles are uploaded by
+// the caller into S.E.x/y/z (real atoms) and S.E.dip afterwards.
+int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, const mbgpu_elec_params* p,
+                   std::string& err)
+{
+    if (!p) {
+        err = "null parameters";
+        return -1;
+    }
+    if (!(p->rcut > 0.0) || !(p->alpha > 1e-6) || p->kmax[0] < 1 || p->kmax[1] < 1 || p->kmax[2] < 1) {
+        err = "electrostatics parameters: need rcut > 0, alpha > 1e-6 and kmax >= 1";
+        return -1;
+    }
+    if (p->polr2[2] > 1e-6) {
+        err = "electrostatics parameters: the M site must be unpolarizable (polr2[2] = 0)";
+        return -1;
+    }
+    if (!(p->polr[0] > 0.0 && p->polr[1] > 0.0 && p->polr[2] > 0.0)) {
+        err = "electrostatics parameters: smearing polarizabilities must be positive";
+        return -1;
+    }
+    if (S.nw != nw) {
+        elec_release(S);
+        if (alloc_sites(S, nw, err)) return -1;
+    }
+    ElecDev& P = S.P;
+    P.nw = nw;
+    P.n = 4 * nw;
+    P.nreal = 3 * nw;
+    P.nrow = 3 * nw;
+    P.g = g;
+    P.rcut = p->rcut;
+    P.alpha = p->alpha;
+    P.gamma = p->gamma;
+    P.q_o = p->q_o;
+    for (int i = 0; i < 3; ++i) {
+        P.polr[i] = p->polr[i];
+        P.polr2[i] = p->polr2[i];
+        P.km[i] = p->kmax[i];
+    }
+    P.athole = p->athole;
+    P.athole12 = p->athole12;
+    P.athole13 = p->athole13;
+    P.ascc = p->ascc;
+    P.ascd = p->ascd;
+    P.toler = p->toler;
+    P.omega = p->omega;
+    P.compat_qd = p->compat_qd_column;
+    if (S.tab_alpha != p->alpha || S.tab_rcut != p->rcut)
+        if (build_table(S, p->alpha, p->rcut, err)) return -1;
+    bool same_k = S.k_alpha == p->alpha && S.E.K > 0;
+    for (int i = 0; i < 9; ++i) same_k = same_k && S.k_cell[i] == cell[i];
+    for (int i = 0; i < 3; ++i) same_k = same_k && S.k_kmax[i] == p->kmax[i];
+    if (!same_k && build_klist(S, cell, p->kmax, p->alpha, err)) return -1;
+    if (S.E.capA == 0) {
+        // rows: sites within rcut at a density of up to 0.15 sites per A^3, rounded to 64
+        const double vol = 4.0 / 3.0 * PI_D * p->rcut * p->rcut * p->rcut;
+        int cap = (int)(0.15 * vol) + 64;
+        cap = std::min(((cap + 63) / 64) * 64, ((P.n + 63) / 64) * 64);
+        if (alloc_rows(S, cap, cap, err)) return -1;
+    }
+    P.capA = S.E.capA;
+    P.capP = S.E.capP;
+    P.K = S.E.K;
+    P.nrun = S.E.nrun;
+    P.kchunks = S.E.kchunks;
+    P.kstride = S.E.kstride;
+    P.scale = 4.0 * P.rvolm * R4PIE0;
+    P.scal1 = 2.0 * P.rvolm * R4PIE0;
+    P.self_dd = 4.0 * R4PIE0 * P.alpha * P.alpha * P.alpha / SQRPI / 3.0;
+    return 0;
+}
+
+// One evaluation with positions (real atoms) in E.x/y/z and start dipoles in E.dip.
+int elec_evaluate(ElecState& S, int max_iter, int distribute, hipStream_t s, std::string& err)
+{
+    ElecBuffers& E = S.E;
+    ElecDev& P = S.P;
+    S.ready = false;
+    for (int attempt = 0;; ++attempt) {
+        ECHK(hipMemsetAsync(E.err, 0, 4 * sizeof(int), s));
+        int rc = el_launch_setup(E, P, s);
+        if (rc) {
+            err = std::string("electrostatics set-up kernels failed: ") + hipGetErrorString((hipError_t)rc);
+            return rc;
+        }
+        int herr = 0;
+        ECHK(hipMemcpyAsync(&herr, E.err, sizeof(int), hipMemcpyDeviceToHost, s));
+        ECHK(hipStreamSynchronize(s));
+        if (!herr) break;
+        const int full = ((P.n + 63) / 64) * 64;
+        if (attempt >= 2 || E.capA >= full) {
+            err = "electrostatics neighbour rows exceed their capacity";
+            return -2;
+        }
+        if (alloc_rows(S, std::min(full, 2 * E.capA), std::min(full, 2 * E.capP), err)) return -1;
+        P.capA = E.capA;
+        P.capP = E.capP;
+    }
+    ECHK(hipMemsetAsync(E.state, 0, 8 * sizeof(double), s));
+    ECHK(hipMemsetAsync(E.emu, 0, sizeof(double) * 3 * P.n, s));
+    // M sites carry no induced dipole (polr2 = 0): clear any start value
+    ECHK(hipMemsetAsync(E.dip + 3 * P.nreal, 0, sizeof(double) * 3 * (P.n - P.nreal), s));
+    int rc = el_launch_prep(E, P, S.rcell, s);
+    if (rc) {
+        err = std::string("electrostatics preparation kernels failed: ") + hipGetErrorString((hipError_t)rc);
+        return rc;
+    }
+    // solver loop in batches; a converged state turns the rest of a batch into no-ops
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int launched = 0, batch = 12;
+    for (;;) {
+        for (int b = 0; b < batch; ++b) {
+            rc = el_launch_iteration(E, P, launched == 0, s);
+            ++launched;
+            if (rc) {
+                err = std::string("electrostatics iteration kernels failed: ") + hipGetErrorString((hipError_t)rc);
+                return rc;
+            }
+        }
+        ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
+        ECHK(hipStreamSynchronize(s));
+        if (st[2] != 0.0) break;
+        if (!(st[1] == st[1])) {
+            err = "induced dipoles became NaN";
+            return -3;
+        }
+        if (st[0] >= (double)max_iter) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "induced dipoles did not converge in %d iterations (rms change %.3e)",
+                     (int)st[0], st[1]);
+            err = buf;
+            return -3;
+        }
+        batch = 8;
+    }
+    const double g34 = exp(gammln_host(0.75)), gln34 = gammln_host(0.75);
+    rc = el_launch_final(E, P, distribute, g34, gln34, s);
+    if (rc) {
+        err = std::string("electrostatics final kernels failed: ") + hipGetErrorString((hipError_t)rc);
+        return rc;
+    }
+    ECHK(hipMemcpyAsync(S.scal, E.scal, sizeof(double) * 35, hipMemcpyDeviceToHost, s));
+    ECHK(hipStreamSynchronize(s));
+    const double* q = S.scal;
+    const double engsic = -R4PIE0 * P.alpha * q[9] / SQRPI;
+    const double eng1 = P.scal1 * q[1] + engsic;
+    const double engdum = q[2];
+    const double omg_xx = q[3], omg_yy = q[4], omg_zz = q[5], omg_xy = q[6], omg_xz = q[7], omg_yz = q[8];
+    double st9[9];
+    st9[0] = P.scal1 * (omg_xx + engdum);
+    st9[1] = P.scal1 * omg_xy;
+    st9[2] = P.scal1 * omg_xz;
+    st9[3] = st9[1];
+    st9[4] = P.scal1 * (omg_yy + engdum);
+    st9[5] = P.scal1 * omg_yz;
+    st9[6] = st9[2];
+    st9[7] = st9[5];
+    st9[8] = P.scal1 * (omg_zz + engdum);
+    const double vir1 = -P.scal1 * (omg_xx + omg_yy + omg_zz + 3.0 * engdum);
+    // pair, excluded-pair and qd sums: [energy, virial, xx, xy, xz, yy, yz, zz]
+    const double* pp[3] = {q + 11, q + 19, q + 27};
+    double vir = vir1;
+    for (int t = 0; t < 3; ++t) {
+        const double* r = pp[t];
+        vir += r[1];
+        st9[0] += r[2];
+        st9[1] += r[3];
+        st9[2] += r[4];
+        st9[3] += r[3];
+        st9[4] += r[5];
+        st9[5] += r[6];
+        st9[6] += r[4];
+        st9[7] += r[6];
+        st9[8] += r[7];
+    }
+    double* o = S.out;
+    o[0] = eng1;
+    o[1] = q[11];
+    o[2] = q[19];
+    o[3] = q[0];
+    o[4] = q[0] + eng1 + q[11] + q[19];
+    o[5] = vir;
+    o[6] = vir1;
+    o[7] = q[28];
+    o[8] = st[0];
+    o[9] = st[1];
+    for (int i = 0; i < 9; ++i) o[10 + i] = st9[i];
+    o[19] = 0.0;
+    S.ready = true;
+    return 0;
+}

@@ -1,0 +1,1499 @@
+// elec_kernels.hip -- polarizable Ewald electrostatics of MB-pol water on gfx950.
+//
+// The reference computes, once per force evaluation (forces.f:268-404 with the
+// subroutines it calls):  M sites and DMS charges (dlpoly_cmd.f:2041-2057,
+// qdetermine.f90), the induced dipoles by SOR over real-space (rfield.f) and
+// reciprocal-space (ewald1p.f, iflag = 0) fields (iter.f), then the polarization energy
+// (coulomb_polar.f), reciprocal energy / forces / per-atom stress (ewald1p.f, iflag = 1),
+// real-space pair terms (ewald2p.f), intramolecular corrections (ewald3p.f) and the
+// charge-derivative forces (qdforce.f90).
+//
+// GPU organisation (one context, one stream, no atomics on floating-point data: every
+// per-site sum is owned by one thread or one wavefront and summed in a fixed order):
+//   * real space: full neighbour rows per site (both directions of each pair), so that a
+//     wavefront owns a site's row; the Thole-damped dipole tensors of the polarizable
+//     pairs are built once per evaluation and reused by every solver iteration;
+//   * reciprocal space: the k vectors are grouped into runs of consecutive n at fixed
+//     (l, m); exp(i k.r_j) comes from per-site tables of exp(i l a*.r_j) (a* a reciprocal
+//     vector) and a complex recurrence along n.  Structure factors are k-parallel with a
+//     site split (partials summed in a fixed order); fields and forces are site-parallel
+//     with a k split (likewise);
+//   * the solver loop runs on the device; a done flag set by the convergence check turns
+//     the remaining kernels of a batch of iterations into no-ops.
+#include <math.h>
+
+#include "mbgpu_internal.h"
+#include "gen/dms_table.inc"
+
+#define R4PIE0 138935.4835            // dl_params.inc
+#define SQRPI 1.7724538509055159      // dl_params.inc
+#define TWOPI (2.0 * 3.141592653589793)
+#define EL_RUN 8                      // k vectors per run (consecutive n)
+#define EL_SCHUNK 64                  // sites per chunk of the structure-factor pass
+
+namespace {
+
+__device__ __forceinline__ double wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// fixed-order block reduction of NV values per thread into out[0..NV) (blockDim <= 1024)
+template <int NV>
+__device__ void block_sum_store(double (&v)[NV], double* out)
+{
+    __shared__ double red[16][NV];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (blockDim.x + 63) >> 6;
+    for (int q = 0; q < NV; ++q) {
+        const double s = wsum(v[q]);
+        if (lane == 0) red[wv][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int w = 0; w < nwv; ++w) s += red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ int water_of(const ElecDev& P, int i) { return i < P.nreal ? i / 3 : i - P.nreal; }
+
+__device__ __forceinline__ double site_polr(const ElecDev& P, int i)
+{
+    return i >= P.nreal ? P.polr[2] : ((i % 3) == 0 ? P.polr[0] : P.polr[1]);
+}
+__device__ __forceinline__ double site_polr2(const ElecDev& P, int i)
+{
+    return i >= P.nreal ? P.polr2[2] : ((i % 3) == 0 ? P.polr2[0] : P.polr2[1]);
+}
+
+// erfcc.f: Numerical Recipes erfc with single-precision literals promoted to double.
+__device__ double erfcc_nr(double x)
+{
+    const double z = fabs(x);
+    const double t = 1.0 / (1.0 + 0.5 * z);
+    double r = t * exp(-z * z - (double)1.26551223f +
+                       t * ((double)1.00002368f +
+                            t * ((double).37409196f +
+                                 t * ((double).09678418f +
+                                      t * ((double)-.18628806f +
+                                           t * ((double).27886807f +
+                                                t * ((double)-1.13520398f +
+                                                     t * ((double)1.48851587f +
+                                                          t * ((double)-0.82215223f + t * (double).17087277f)))))))));
+    return x < 0.0 ? 2.0 - r : r;
+}
+
+// gammq.f: gammln, gser, gcf, gammq (EPS, FPMIN single-precision values)
+__device__ double gammln_nr(double xx)
+{
+    const double cof[6] = {76.18009172947146, -86.50532032941677, 24.01409824083091,
+                           -1.231739572450155, .1208650973866179e-2, -.5395239384953e-5};
+    double x = xx, y = x;
+    double tmp = x + 5.5;
+    tmp = (x + 0.5) * log(tmp) - tmp;
+    double ser = 1.000000000190015;
+    for (int j = 0; j < 6; ++j) {
+        y = y + 1.0;
+        ser = ser + cof[j] / y;
+    }
+    return tmp + log(2.5066282746310005 * ser / x);
+}
+
+__device__ double gammq_nr(double a, double x, double gln)
+{
+    const double EPS = (double)3.e-7f, FPMIN = (double)1.e-30f;
+    if (x < a + 1.0) {
+        if (x <= 0.0) return 1.0;
+        double ap = a, sum = 1.0 / a, del = sum;
+        for (int n = 1; n <= 800; ++n) {
+            ap = ap + 1.0;
+            del = del * x / ap;
+            sum = sum + del;
+            if (fabs(del) < fabs(sum) * EPS) break;
+        }
+        return 1.0 - sum * exp(-x + a * log(x) - gln);
+    }
+    double b = x + 1.0 - a, c = 1.0 / FPMIN, d = 1.0 / b, h = d;
+    for (int i = 1; i <= 100; ++i) {
+        const double an = -i * (i - a);
+        b = b + 2.0;
+        d = an * d + b;
+        if (fabs(d) < FPMIN) d = FPMIN;
+        c = b + an / c;
+        if (fabs(c) < FPMIN) c = FPMIN;
+        d = 1.0 / d;
+        const double del = d * c;
+        h = h * del;
+        if (fabs(del - 1.0) < EPS) break;
+    }
+    return exp(-x + a * log(x) - gln) * h;
+}
+
+// 3-point interpolation of the B-function table (rfield.f:102-120, ewald2p.f:160-178)
+__device__ __forceinline__ void table_interp(const double* __restrict__ t, double drewd, double rr, double* bbb)
+{
+    const int ll = (int)(rr / drewd);
+    const double ppp = rr / drewd - (double)ll;
+    const double* r0 = t + 4 * ll;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const double vk = r0[n], vk1 = r0[4 + n], vk2 = r0[8 + n];
+        const double t1 = vk + (vk1 - vk) * ppp;
+        const double t2 = vk1 + (vk2 - vk1) * (ppp - 1.0);
+        bbb[n] = t1 + (t2 - t1) * ppp * 0.5;
+    }
+}
+
+// B functions computed directly (excluded pairs, rfield.f:365-374, ewald3p.f:100-110)
+__device__ __forceinline__ void bfun_direct(double alpha, double rr, double rsqi, double* bbb)
+{
+    const double a = alpha * rr;
+    bbb[0] = (erfcc_nr(a) - 1.0) / rr;
+    const double exp2a = exp(-a * a);
+    const double c = 2.0 * alpha * alpha;
+    double cm = 1.0;
+#pragma unroll
+    for (int m = 1; m <= 3; ++m) {
+        cm *= c;
+        bbb[m] = rsqi * ((2.0 * m - 1.0) * bbb[m - 1] + cm * exp2a / alpha / SQRPI);
+    }
+}
+
+// imaged separation of sites i, j in the reference's arithmetic
+__device__ __forceinline__ double sep(const ElecDev& P, const double* __restrict__ x, const double* __restrict__ y,
+                                      const double* __restrict__ z, int i, int j, double* d)
+{
+#pragma clang fp contract(off)
+    double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+    image_vec(P.g, dx, dy, dz);
+    d[0] = dx;
+    d[1] = dy;
+    d[2] = dz;
+    return norm2_nc(dx, dy, dz);
+}
+
+}  // namespace
+
+// ====================================================================== set-up ======
+
+// M sites (dlpoly_cmd.f:2041-2057) and DMS charges with their derivatives
+// (qdetermine.f90:118-259; new charge assignment).  One thread per water.
+__global__ void k_el_sites(ElecDev P, double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+                           double* __restrict__ chge, double* __restrict__ dg)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= P.nw) return;
+    const int io = 3 * w, m = P.nreal + w;
+    double r1v[3], r2v[3];
+    {
+#pragma clang fp contract(off)
+        double d1x = x[io + 1] - x[io], d1y = y[io + 1] - y[io], d1z = z[io + 1] - z[io];
+        double d2x = x[io + 2] - x[io], d2y = y[io + 2] - y[io], d2z = z[io + 2] - z[io];
+        image_vec(P.g, d1x, d1y, d1z);
+        image_vec(P.g, d2x, d2y, d2z);
+        x[m] = x[io] + (d1x + d2x) * P.gamma / 2.0;
+        y[m] = y[io] + (d1y + d2y) * P.gamma / 2.0;
+        z[m] = z[io] + (d1z + d2z) * P.gamma / 2.0;
+        r1v[0] = d1x;
+        r1v[1] = d1y;
+        r1v[2] = d1z;
+        r2v[0] = d2x;
+        r2v[1] = d2y;
+        r2v[2] = d2z;
+    }
+    const double r1 = sqrt(r1v[0] * r1v[0] + r1v[1] * r1v[1] + r1v[2] * r1v[2]);
+    const double r2 = sqrt(r2v[0] * r2v[0] + r2v[1] * r2v[1] + r2v[2] * r2v[2]);
+    const double cost = (r1v[0] * r2v[0] + r1v[1] * r2v[1] + r1v[2] * r2v[2]) / (r1 * r2);
+    // reference geometry of the expansion: ps_type_pes reoh, thetae (module.f90:480-481)
+    const double reoh = 0.958649, cose = cos(104.3475 * 3.141592653589793 / 180.0);
+    const double x1 = (r1 - reoh) / reoh, x2 = (r2 - reoh) / reoh, x3 = cost - cose;
+    double f1[8], f2[8], f3[8];
+    f1[0] = f2[0] = f3[0] = 0.0;
+    f1[1] = f2[1] = f3[1] = 1.0;
+#pragma unroll
+    for (int j = 2; j < 8; ++j) {
+        f1[j] = f1[j - 1] * x1;
+        f2[j] = f2[j - 1] * x2;
+        f3[j] = f3[j - 1] * x3;
+    }
+    double p1 = 0.0, p2 = 0.0, g11 = 0.0, g12 = 0.0, g21 = 0.0, g22 = 0.0, g1c = 0.0, g2c = 0.0;
+#pragma unroll
+    for (int j = 1; j < DMS_NTERMS; ++j) {
+        const int a = dms_idx[j][0], b = dms_idx[j][1], c = dms_idx[j][2];
+        const double cf = dms_coef[j];
+        p1 = p1 + cf * f1[a] * f2[b] * f3[c];
+        p2 = p2 + cf * f1[b] * f2[a] * f3[c];
+        g11 = g11 + cf * (a - 1) * f1[a - 1] * f2[b] * f3[c] / reoh;
+        g12 = g12 + cf * (b - 1) * f1[a] * f2[b - 1] * f3[c] / reoh;
+        g21 = g21 + cf * (b - 1) * f1[b - 1] * f2[a] * f3[c] / reoh;
+        g22 = g22 + cf * (a - 1) * f1[b] * f2[a - 1] * f3[c] / reoh;
+        g1c = g1c + cf * (c - 1) * f1[a] * f2[b] * f3[c - 1];
+        g2c = g2c + cf * (c - 1) * f1[b] * f2[a] * f3[c - 1];
+    }
+    // DMS parameters (module.f90:818-826)
+    const double b1dms = 1.0, adms = 0.2999 * 0.529177249, bdms = -0.6932;
+    const double c0 = 1.0099, c1 = -0.1801, c2 = 0.0892;
+    const double pl1 = cost, pl2 = 0.5 * (3.0 * pl1 * pl1 - 1.0);
+    const double damp = exp(-b1dms * ((r1 - reoh) * (r1 - reoh) + (r2 - reoh) * (r2 - reoh)));
+    const double ddampdr1 = -2.0 * b1dms * (r1 - reoh) * damp;
+    const double ddampdr2 = -2.0 * b1dms * (r2 - reoh) * damp;
+    const double leg = c0 + pl1 * c1 + pl2 * c2;
+    const double pr1 = pow(r1, bdms), pr2 = pow(r2, bdms);
+    const double pc0 = adms * (pr1 + pr2) * leg;
+    const double dpc0dr1 = adms * (bdms * pow(r1, bdms - 1)) * leg;
+    const double dpc0dr2 = adms * (bdms * pow(r2, bdms - 1)) * leg;
+    const double dpc0dcos = adms * (pr1 + pr2) * (c1 + 3.0 * c2 * pl1);
+    double* G = dg + 6 * w;
+    G[0] = g11 * damp + p1 * ddampdr1 + dpc0dr1;
+    G[1] = g12 * damp + p1 * ddampdr2 + dpc0dr2;
+    G[2] = g21 * damp + p2 * ddampdr1 + dpc0dr1;
+    G[3] = g22 * damp + p2 * ddampdr2 + dpc0dr2;
+    G[4] = g1c * damp + dpc0dcos;
+    G[5] = g2c * damp + dpc0dcos;
+    p1 = dms_coef[0] + p1 * damp + pc0;
+    p2 = dms_coef[0] + p2 * damp + pc0;
+    const double factor = P.gamma / 2.0 / (1.0 - P.gamma);
+    const double q1 = p1 + factor * (p1 + p2), q2 = p2 + factor * (p1 + p2);
+    chge[io] = P.q_o;
+    chge[io + 1] = q1;
+    chge[io + 2] = q2;
+    chge[m] = -(q1 + q2);
+}
+
+// Neighbour rows: every site of another water within rcut (superset by a relative 1e-9 of
+// rcut^2; each kernel applies the reference's own test), ascending partner index.  Row P
+// holds the partners of a polarizable site that are polarizable.  One wavefront per site.
+__global__ __launch_bounds__(64) void k_el_nbr(ElecDev P, const double* __restrict__ x,
+                                               const double* __restrict__ y, const double* __restrict__ z,
+                                               int* __restrict__ nbrA, int* __restrict__ cntA,
+                                               int* __restrict__ nbrP, int* __restrict__ cntP, int* __restrict__ err)
+{
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (i >= P.n) return;
+    const int wi = water_of(P, i);
+    const bool poli = site_polr2(P, i) > 1.e-6;
+    const double rl2 = P.rcut * P.rcut * (1.0 + 1e-9);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int nA = 0, nP = 0;
+    for (int base = 0; base < P.n; base += 64) {
+        const int j = base + lane;
+        bool inA = false, inP = false;
+        if (j < P.n && water_of(P, j) != wi) {
+            double d[3];
+            const double r2 = sep(P, x, y, z, i, j, d);
+            inA = r2 < rl2;
+            inP = inA && poli && site_polr2(P, j) > 1.e-6;
+        }
+        const unsigned long long bA = __ballot(inA), bP = __ballot(inP);
+        if (inA) {
+            const int p = nA + __popcll(bA & below);
+            if (p < P.capA) nbrA[(long)i * P.capA + p] = j;
+        }
+        if (inP) {
+            const int p = nP + __popcll(bP & below);
+            if (p < P.capP) nbrP[(long)i * P.capP + p] = j;
+        }
+        nA += __popcll(bA);
+        nP += __popcll(bP);
+    }
+    if (lane == 0) {
+        cntA[i] = min(nA, P.capA);
+        cntP[i] = min(nP, P.capP);
+        if (nA > P.capA || nP > P.capP) atomicOr(err, 1);
+    }
+}
+
+// Per-row set-up of the real-space fields (rfield.f:62-323): the charge field of the row's
+// partners (iloop = 1 part, with the lads smearing) and the Thole-damped dipole tensors of
+// its polarizable partners (stored SoA [6][n][capP]).  One wavefront per site.
+__global__ __launch_bounds__(64) void k_el_prep(ElecDev P, const double* __restrict__ x,
+                                                const double* __restrict__ y, const double* __restrict__ z,
+                                                const double* __restrict__ chge, const double* __restrict__ tab,
+                                                const int* __restrict__ nbrA, const int* __restrict__ cntA,
+                                                const int* __restrict__ nbrP, const int* __restrict__ cntP,
+                                                double* __restrict__ tens, double* __restrict__ efk)
+{
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (i >= P.n) return;
+    const double pi_ = site_polr(P, i);
+    double e[3] = {0.0, 0.0, 0.0};
+    const int na = cntA[i];
+    for (int s = lane; s < na; s += 64) {
+        const int j = nbrA[(long)i * P.capA + s];
+        double d[3];
+        const double r2 = sep(P, x, y, z, i, j, d);
+        const double rr = sqrt(r2);
+        if (!(P.rcut > rr)) continue;
+        double bbb[4];
+        table_interp(tab, P.drewd, rr, bbb);
+        const double r3i = 1.0 / (rr * rr * rr);
+        const double bb = pow(pi_ * site_polr(P, j), 1.0 / 6.0);
+        const double u = rr / bb;
+        const double arob4 = P.ascd * (u * u) * (u * u);
+        const double dum1 = exp(-arob4);
+        double eki = R4PIE0 * chge[j] * (bbb[1] + r3i);
+        eki = eki - R4PIE0 * chge[j] * dum1 * r3i;
+        e[0] += eki * d[0];
+        e[1] += eki * d[1];
+        e[2] += eki * d[2];
+    }
+    for (int q = 0; q < 3; ++q) e[q] = wsum(e[q]);
+    if (lane == 0)
+        for (int q = 0; q < 3; ++q) efk[3 * i + q] = e[q];
+    const int np = cntP[i];
+    const long nc = (long)P.n * P.capP;
+    for (int s = lane; s < np; s += 64) {
+        const int j = nbrP[(long)i * P.capP + s];
+        double d[3];
+        const double r2 = sep(P, x, y, z, i, j, d);
+        const double rr = sqrt(r2);
+        double dt[6] = {0, 0, 0, 0, 0, 0};
+        if (P.rcut > rr) {
+            double bbb[4];
+            table_interp(tab, P.drewd, rr, bbb);
+            const double rsqi = 1.0 / r2;
+            const double r3i = 1.0 / (rr * rr * rr);
+            const double r5i = r3i * rsqi;
+            const double bb = pow(pi_ * site_polr(P, j), 1.0 / 6.0);
+            const double u = rr / bb;
+            const double arob4 = P.athole * (u * u) * (u * u);
+            const double dum1 = exp(-arob4);
+            const double dum2 = (1.0 + (4.0 / 3.0) * arob4) * dum1;
+            const double rij[6] = {d[0] * d[0], d[0] * d[1], d[0] * d[2], d[1] * d[1], d[1] * d[2], d[2] * d[2]};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const bool dg_ = (q == 0 || q == 3 || q == 5);
+                dt[q] = R4PIE0 * ((3.0 * rij[q] * r5i - (dg_ ? r3i : 0.0)) + rij[q] * bbb[2] - (dg_ ? bbb[1] : 0.0));
+                dt[q] = dt[q] - (3.0 * rij[q] * r5i * dum2 - (dg_ ? dum1 * r3i : 0.0)) * R4PIE0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) tens[q * nc + (long)i * P.capP + s] = dt[q];
+    }
+}
+
+// Excluded (intramolecular) pairs of each water (rfield.f:327-525): charge field with the
+// -erf kernel and the dipole tensors (Ewald + Applequist - Thole with athole12/athole13),
+// stored [nw][pair][6] for pairs (O,H1) (O,H2) (O,M) (H1,H2) (H1,M) (H2,M).
+__global__ void k_el_intra_prep(ElecDev P, const double* __restrict__ x, const double* __restrict__ y,
+                                const double* __restrict__ z, const double* __restrict__ chge,
+                                double* __restrict__ tin, double* __restrict__ efk)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= P.nw) return;
+    const int s[4] = {3 * w, 3 * w + 1, 3 * w + 2, P.nreal + w};
+    int pr = 0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = a + 1; b < 4; ++b, ++pr) {
+            const int i = s[a], j = s[b];
+            double d[3];
+            const double r2 = sep(P, x, y, z, i, j, d);
+            const double rr = sqrt(r2);
+            const double rsqi = 1.0 / r2;
+            const double r3i = 1.0 / (rr * r2);
+            const double r5i = r3i * rsqi;
+            double bbb[4];
+            bfun_direct(P.alpha, rr, rsqi, bbb);
+            const double eki = R4PIE0 * chge[j] * bbb[1];
+            const double ekj = R4PIE0 * chge[i] * bbb[1];
+            for (int q = 0; q < 3; ++q) {
+                efk[3 * i + q] += eki * d[q];
+                efk[3 * j + q] -= ekj * d[q];
+            }
+            double dt[6] = {0, 0, 0, 0, 0, 0};
+            if (site_polr2(P, i) > 1.e-6 && site_polr2(P, j) > 1.e-6) {
+                const double bb = pow(site_polr(P, i) * site_polr(P, j), 1.0 / 6.0);
+                const double u = rr / bb;
+                const double ath = (i % 3 == 0 || j % 3 == 0) ? P.athole12 : P.athole13;
+                const double arob4 = ath * (u * u) * (u * u);
+                const double dum1 = exp(-arob4);
+                const double dum2 = (1.0 + (4.0 / 3.0) * arob4) * dum1;
+                const double rij[6] = {d[0] * d[0], d[0] * d[1], d[0] * d[2], d[1] * d[1], d[1] * d[2], d[2] * d[2]};
+                for (int q = 0; q < 6; ++q) {
+                    const bool dg_ = (q == 0 || q == 3 || q == 5);
+                    dt[q] = rij[q] * bbb[2] - (dg_ ? bbb[1] : 0.0);
+                    dt[q] = dt[q] + 3.0 * rij[q] * r5i - (dg_ ? r3i : 0.0);
+                    dt[q] = dt[q] - 3.0 * rij[q] * r5i * dum2 + (dg_ ? dum1 * r3i : 0.0);
+                }
+            }
+            for (int q = 0; q < 6; ++q) tin[36 * w + 6 * pr + q] = dt[q];
+        }
+}
+
+// Reciprocal-space phase tables of each site: cos / sin (l * 2 pi * s_a), s = rcell . r,
+// l = 0..kmax_a, by the reference's recurrence (ewald1p.f:196-241, 262-282).  Two layouts:
+// phc/phs [a][l][n] for site-parallel kernels and pht [n][a][l][2] for k-parallel kernels.
+__global__ void k_el_phase(ElecDev P, const double* __restrict__ x, const double* __restrict__ y,
+                           const double* __restrict__ z, const double* __restrict__ rcell,
+                           double* __restrict__ phc, double* __restrict__ phs, double* __restrict__ pht)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.n) return;
+    const int ks = P.kstride;
+    for (int a = 0; a < 3; ++a) {
+        const double ss = rcell[a] * x[j] + rcell[3 + a] * y[j] + rcell[6 + a] * z[j];
+        const double c1 = cos(TWOPI * ss), s1 = sin(TWOPI * ss);
+        double c = 1.0, s = 0.0;
+        for (int l = 0; l <= P.km[a]; ++l) {
+            if (l == 1) {
+                c = c1;
+                s = s1;
+            } else if (l > 1) {
+                const double cs = c;
+                c = cs * c1 - s * s1;
+                s = s * c1 + cs * s1;
+            }
+            phc[((long)a * ks + l) * P.n + j] = c;
+            phs[((long)a * ks + l) * P.n + j] = s;
+            pht[(((long)j * 3 + a) * ks + l) * 2] = c;
+            pht[(((long)j * 3 + a) * ks + l) * 2 + 1] = s;
+        }
+    }
+}
+
+// ======================================================= reciprocal-space passes ======
+
+namespace {
+
+// exp(i k.r_j) at the first k of a run (l, m, n0), with the reference's sign rules
+// (ewald1p.f:290-306 for m, 346-352 / 392-398 for n).
+__device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c, double em_s, double en_c, double en_s,
+                                          int m, int n0, double& ckr, double& skr)
+{
+    double clm, slm;
+    if (m >= 0) {
+        clm = el_c * em_c - el_s * em_s;
+        slm = el_s * em_c + em_s * el_c;
+    } else {
+        clm = el_c * em_c + el_s * em_s;
+        slm = el_s * em_c - em_s * el_c;
+    }
+    if (n0 >= 0) {
+        ckr = clm * en_c - slm * en_s;
+        skr = slm * en_c + clm * en_s;
+    } else {
+        ckr = clm * en_c + slm * en_s;
+        skr = slm * en_c - clm * en_s;
+    }
+}
+
+}  // namespace
+
+// Structure factors over a chunk of sites, k-parallel (one thread per run of <= EL_RUN
+// consecutive k).  Writes Spart[chunk][k][8] = (sum q ckr, sum q skr, sum mu ckr (3),
+// sum mu skr (3)) with the dipole sums over polarizable sites only (ewald1p.f:342-438).
+template <bool CHG>
+__global__ __launch_bounds__(64) void k_el_sfac(ElecDev P, const int* __restrict__ runs,
+                                                const int* __restrict__ runk, const double* __restrict__ pht,
+                                                const double* __restrict__ chge, const double* __restrict__ dip,
+                                                double* __restrict__ Spart, const double* __restrict__ state)
+{
+    if (!CHG && state[2] != 0.0) return;
+    const int r = blockIdx.x * 64 + threadIdx.x;
+    const int chunk = blockIdx.y;
+    if (r >= P.nrun) return;
+    const int l = runs[4 * r], m = runs[4 * r + 1], n0 = runs[4 * r + 2], cnt = runs[4 * r + 3];
+    const int am = abs(m), an0 = abs(n0);
+    const int j0 = chunk * EL_SCHUNK, j1 = min(P.n, j0 + EL_SCHUNK);
+    const int ks = P.kstride;
+    double acc[EL_RUN][8];
+#pragma unroll
+    for (int t = 0; t < EL_RUN; ++t)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[t][q] = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const bool pol = site_polr2(P, j) > 1.e-6;
+        if (!CHG && !pol) continue;
+        const double* pj = pht + (long)j * 3 * ks * 2;
+        const double* pa = pj;
+        const double* pb = pj + ks * 2;
+        const double* pc = pj + 2 * ks * 2;
+        double ckr, skr;
+        run_phase(pa[2 * l], pa[2 * l + 1], pb[2 * am], pb[2 * am + 1], pc[2 * an0], pc[2 * an0 + 1], m, n0, ckr, skr);
+        const double c1 = pc[2], s1 = pc[3];
+        const double q = chge[j];
+        const double mx = dip[3 * j], my = dip[3 * j + 1], mz = dip[3 * j + 2];
+#pragma unroll
+        for (int t = 0; t < EL_RUN; ++t) {
+            if (t < cnt) {
+                if (CHG) {
+                    acc[t][0] += q * ckr;
+                    acc[t][1] += q * skr;
+                }
+                if (pol) {
+                    acc[t][2] += ckr * mx;
+                    acc[t][3] += ckr * my;
+                    acc[t][4] += ckr * mz;
+                    acc[t][5] += skr * mx;
+                    acc[t][6] += skr * my;
+                    acc[t][7] += skr * mz;
+                }
+                const double cn = ckr * c1 - skr * s1;
+                skr = skr * c1 + ckr * s1;
+                ckr = cn;
+            }
+        }
+    }
+    const int k0 = runk[r];
+#pragma unroll
+    for (int t = 0; t < EL_RUN; ++t)
+        if (t < cnt)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Spart[((long)chunk * P.K + k0 + t) * 8 + q] = acc[t][q];
+}
+
+// Sum the chunk partials in chunk order and derive the per-k constants of the field
+// passes: kc[k][0..5] = (-k_a akk k.Ms, -k_a akk k.Mc) for the dipole field (ewald1p.f:759-767).
+// With CHG the charge sums are stored in Sq[k][2] and kc[k][6..11] receives the charge-
+// field constants (k_a akk ckcs, -k_a akk ckss) (ewald1p.f:749-757).
+template <bool CHG>
+__global__ void k_el_sreduce(ElecDev P, int schunks, const double* __restrict__ Spart, const double* __restrict__ kv,
+                             double* __restrict__ S, double* __restrict__ Sq, double* __restrict__ kc,
+                             const double* __restrict__ state)
+{
+    if (!CHG && state[2] != 0.0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.K) return;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < schunks; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] += Spart[((long)c * P.K + k) * 8 + q];
+    if (CHG) {
+        Sq[2 * k] = s[0];
+        Sq[2 * k + 1] = s[1];
+    }
+#pragma unroll
+    for (int q = 2; q < 8; ++q) S[8 * k + q] = s[q];
+    const double kx = kv[8 * k], ky = kv[8 * k + 1], kz = kv[8 * k + 2], akk = kv[8 * k + 3];
+    const double skdmuckrs = kx * s[2] + ky * s[3] + kz * s[4];
+    const double skdmuskrs = kx * s[5] + ky * s[6] + kz * s[7];
+    double* c = kc + 16 * k;
+    c[0] = -kx * akk * skdmuskrs;
+    c[1] = -ky * akk * skdmuskrs;
+    c[2] = -kz * akk * skdmuskrs;
+    c[3] = -kx * akk * skdmuckrs;
+    c[4] = -ky * akk * skdmuckrs;
+    c[5] = -kz * akk * skdmuckrs;
+    if (CHG) {
+        c[6] = kx * akk * s[0];
+        c[7] = ky * akk * s[0];
+        c[8] = kz * akk * s[0];
+        c[9] = -kx * akk * s[1];
+        c[10] = -ky * akk * s[1];
+        c[11] = -kz * akk * s[1];
+    }
+}
+
+// Reciprocal-space fields over a chunk of runs, site-parallel: efdd_a += c_a skr + d_a ckr
+// (dipole field) and with CHG also efdc_a += e_a skr + f_a ckr (charge field).  Partials
+// Fpart[chunk][6][n].
+template <bool CHG>
+__global__ __launch_bounds__(64) void k_el_kfield(ElecDev P, const int* __restrict__ runs,
+                                                  const int* __restrict__ runk, const double* __restrict__ phc,
+                                                  const double* __restrict__ phs, const double* __restrict__ kc,
+                                                  double* __restrict__ Fpart, const double* __restrict__ state)
+{
+    if (!CHG && state[2] != 0.0) return;
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int rpc = (P.nrun + P.kchunks - 1) / P.kchunks;
+    const int r0 = chunk * rpc, r1 = min(P.nrun, r0 + rpc);
+    const int nrow = CHG ? P.n : P.nrow;
+    if (j >= nrow) return;
+    const int ks = P.kstride;
+    double fd[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+    const double c1 = phc[((long)2 * ks + 1) * P.n + j], s1 = phs[((long)2 * ks + 1) * P.n + j];
+    for (int r = r0; r < r1; ++r) {
+        const int l = runs[4 * r], m = runs[4 * r + 1], n0 = runs[4 * r + 2], cnt = runs[4 * r + 3];
+        const int am = abs(m), an0 = abs(n0);
+        double ckr, skr;
+        run_phase(phc[(long)l * P.n + j], phs[(long)l * P.n + j], phc[((long)ks + am) * P.n + j],
+                  phs[((long)ks + am) * P.n + j], phc[((long)2 * ks + an0) * P.n + j],
+                  phs[((long)2 * ks + an0) * P.n + j], m, n0, ckr, skr);
+        const int k0 = runk[r];
+        for (int t = 0; t < cnt; ++t) {
+            const double* c = kc + 16 * (k0 + t);
+            fd[0] += c[0] * skr + c[3] * ckr;
+            fd[1] += c[1] * skr + c[4] * ckr;
+            fd[2] += c[2] * skr + c[5] * ckr;
+            if (CHG) {
+                fc[0] += c[6] * skr + c[9] * ckr;
+                fc[1] += c[7] * skr + c[10] * ckr;
+                fc[2] += c[8] * skr + c[11] * ckr;
+            }
+            const double cn = ckr * c1 - skr * s1;
+            skr = skr * c1 + ckr * s1;
+            ckr = cn;
+        }
+    }
+    double* out = Fpart + (long)chunk * 6 * P.n;
+    for (int q = 0; q < 3; ++q) out[(long)q * P.n + j] = fd[q];
+    if (CHG)
+        for (int q = 0; q < 3; ++q) out[(long)(3 + q) * P.n + j] = fc[q];
+}
+
+// Sum the field partials in chunk order and scale (ewald1p.f:941-976): efdd =
+// scale * sum + self term; with CHG also efdc = scale * sum (iloop = 1).
+template <bool CHG>
+__global__ void k_el_kfield_reduce(ElecDev P, const double* __restrict__ Fpart, const double* __restrict__ dip,
+                                   double* __restrict__ efdd, double* __restrict__ efdc, const double* __restrict__ state)
+{
+    if (!CHG && state[2] != 0.0) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nrow = CHG ? P.n : P.nrow;
+    if (j >= nrow) return;
+    double fd[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
+    for (int c = 0; c < P.kchunks; ++c) {
+        const double* in = Fpart + (long)c * 6 * P.n;
+        for (int q = 0; q < 3; ++q) {
+            fd[q] += in[(long)q * P.n + j];
+            if (CHG) fc[q] += in[(long)(3 + q) * P.n + j];
+        }
+    }
+    const bool pol = site_polr2(P, j) > 1.e-6;
+    for (int q = 0; q < 3; ++q) {
+        efdd[3 * j + q] = (pol ? P.scale * fd[q] : 0.0) + P.self_dd * dip[3 * j + q];
+        if (CHG) efdc[3 * j + q] = P.scale * fc[q];
+    }
+}
+
+// ============================================================ solver iteration ======
+
+// Real-space dipole field of one row (rfield.f:201-317 list pairs; 419-520 excluded
+// pairs): emu_i = sum_j T_ij mu_j.  One wavefront per site.
+__global__ __launch_bounds__(64) void k_el_emu(ElecDev P, const int* __restrict__ nbrP, const int* __restrict__ cntP,
+                                               const double* __restrict__ tens, const double* __restrict__ tin,
+                                               const double* __restrict__ dip, double* __restrict__ emu,
+                                               const double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (i >= P.nrow) return;
+    const long nc = (long)P.n * P.capP;
+    double e[3] = {0, 0, 0};
+    const int np = cntP[i];
+    for (int s = lane; s < np; s += 64) {
+        const long o = (long)i * P.capP + s;
+        const int j = nbrP[o];
+        const double mx = dip[3 * j], my = dip[3 * j + 1], mz = dip[3 * j + 2];
+        const double t0 = tens[o], t1 = tens[nc + o], t2 = tens[2 * nc + o];
+        const double t3 = tens[3 * nc + o], t4 = tens[4 * nc + o], t5 = tens[5 * nc + o];
+        e[0] += t0 * mx + t1 * my + t2 * mz;
+        e[1] += t1 * mx + t3 * my + t4 * mz;
+        e[2] += t2 * mx + t4 * my + t5 * mz;
+    }
+    for (int q = 0; q < 3; ++q) e[q] = wsum(e[q]);
+    if (lane == 0) {
+        // excluded partners of the same water (pair order (O,H1) (O,H2) (O,M) (H1,H2) (H1,M) (H2,M))
+        const int w = water_of(P, i);
+        const int si = i < P.nreal ? i - 3 * w : 3;
+        const int s[4] = {3 * w, 3 * w + 1, 3 * w + 2, P.nreal + w};
+        int pr = 0;
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b, ++pr) {
+                if (a != si && b != si) continue;
+                const int j = (a == si) ? s[b] : s[a];
+                const double* dt = tin + 36 * w + 6 * pr;
+                const double mx = dip[3 * j] * R4PIE0, my = dip[3 * j + 1] * R4PIE0, mz = dip[3 * j + 2] * R4PIE0;
+                e[0] += dt[0] * mx + dt[1] * my + dt[2] * mz;
+                e[1] += dt[1] * mx + dt[3] * my + dt[4] * mz;
+                e[2] += dt[2] * mx + dt[4] * my + dt[5] * mz;
+            }
+        for (int q = 0; q < 3; ++q) emu[3 * i + q] = e[q];
+    }
+}
+
+// SOR update (iter.f:58-89) and per-block sums of |dmu|^2 and of the dipole count.
+__global__ __launch_bounds__(256) void k_el_sor(ElecDev P, const double* __restrict__ efk,
+                                                const double* __restrict__ emu, const double* __restrict__ efdd,
+                                                const double* __restrict__ efdc, double* __restrict__ dip,
+                                                double* __restrict__ blk, const double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < P.nrow) {
+        const double p2 = site_polr2(P, i);
+        if (p2 > P.toler) {
+            const double oldfac = P.omega, fac = 1.0 - oldfac;
+            double dd = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                const double old = dip[3 * i + q];
+                const double nw = oldfac * old + fac * p2 *
+                                                      (efk[3 * i + q] + emu[3 * i + q] + efdd[3 * i + q] +
+                                                       efdc[3 * i + q] + 0.0) /
+                                                      R4PIE0;
+                dip[3 * i + q] = nw;
+                dd += (nw - old) * (nw - old);
+            }
+            v[0] = dd;
+            v[1] = 1.0;
+        }
+    }
+    block_sum_store(v, blk + 2 * blockIdx.x);
+}
+
+// Convergence test (iter.f:95-101): state = [iterations, rms dmu, done, ...].
+__global__ __launch_bounds__(64) void k_el_converge(ElecDev P, int nb, const double* __restrict__ blk,
+                                                    double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    double s = 0.0, c = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 64) {
+        s += blk[2 * b];
+        c += blk[2 * b + 1];
+    }
+    s = wsum(s);
+    c = wsum(c);
+    if (threadIdx.x == 0) {
+        if (c == 0.0) c = 1.0;
+        const double delta = sqrt(s / c);
+        state[0] += 1.0;
+        state[1] = delta;
+        if (delta <= P.toler) state[2] = 1.0;
+    }
+}
+
+// ================================================================ final passes ======
+
+// Polarization energy per site (coulomb_polar.f:58-103).
+__global__ __launch_bounds__(256) void k_el_polar(ElecDev P, const double* __restrict__ efk,
+                                                  const double* __restrict__ emu, const double* __restrict__ efdd,
+                                                  const double* __restrict__ efdc, const double* __restrict__ dip,
+                                                  double* __restrict__ ep, double* __restrict__ blk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < P.n) {
+        double e = 0.0;
+        const double p2 = site_polr2(P, i);
+        if (p2 > 1.e-6) {
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q)
+                s += dip[3 * i + q] * (efk[3 * i + q] + efdc[3 * i + q] + 0.5 * (emu[3 * i + q] + efdd[3 * i + q]) + 0.0);
+            const double edum = dip[3 * i] * dip[3 * i] + dip[3 * i + 1] * dip[3 * i + 1] + dip[3 * i + 2] * dip[3 * i + 2];
+            e = -s;
+            e = e + R4PIE0 * edum / p2 / 2.0;
+        }
+        ep[i] = e;
+        v[0] = e;
+    }
+    block_sum_store(v, blk + blockIdx.x);
+}
+
+// Per-k scalars of the final reciprocal pass (ewald1p.f:696-730): akk (Qc^2 + Qs^2),
+// akk |summu|^2 and the six omg components; block partials [nb][8].  Also stores, per k,
+// the constants the site pass needs in kf[k][16].
+__global__ __launch_bounds__(256) void k_el_kscal(ElecDev P, const double* __restrict__ kv,
+                                                  const double* __restrict__ S, const double* __restrict__ Sq,
+                                                  double* __restrict__ kf, double* __restrict__ blk)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < P.K) {
+        const double kx = kv[8 * k], ky = kv[8 * k + 1], kz = kv[8 * k + 2], akk = kv[8 * k + 3];
+        const double akv = kv[8 * k + 4];
+        const double Qc = Sq[2 * k], Qs = Sq[2 * k + 1];
+        const double Mc[3] = {S[8 * k + 2], S[8 * k + 3], S[8 * k + 4]};
+        const double Ms[3] = {S[8 * k + 5], S[8 * k + 6], S[8 * k + 7]};
+        const double kk[3] = {kx, ky, kz};
+        const double skdmuckrs = kx * Mc[0] + ky * Mc[1] + kz * Mc[2];
+        const double skdmuskrs = kx * Ms[0] + ky * Ms[1] + kz * Ms[2];
+        const double summu1 = Qc - skdmuskrs, summu2 = Qs + skdmuckrs;
+        v[0] = akk * (Qc * Qc + Qs * Qs);
+        v[1] = akk * (summu1 * summu1 + summu2 * summu2);
+        const double virprs = akv * (summu1 * summu1 + summu2 * summu2);
+        const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+        for (int t = 0; t < 6; ++t)
+            v[2 + t] = -virprs * kk[ia[t]] * kk[ib[t]] - 2.0 * akk * kk[ia[t]] * Ms[ib[t]] * summu1 +
+                       2.0 * akk * kk[ia[t]] * Mc[ib[t]] * summu2;
+        double* f = kf + 16 * k;
+        f[0] = Qc;
+        f[1] = Qs;
+        f[2] = summu1;
+        f[3] = summu2;
+        f[4] = skdmuckrs;
+        f[5] = skdmuskrs;
+        f[6] = skdmuckrs;   // dpkMc
+        f[7] = skdmuskrs;   // dpkMs
+        for (int q = 0; q < 3; ++q) {
+            f[8 + q] = Mc[q];
+            f[11 + q] = Ms[q];
+        }
+    }
+    block_sum_store(v, blk + 8 * blockIdx.x);
+}
+
+
+// Final reciprocal pass per site over a chunk of runs (ewald1p.f:550-693, 771-825):
+// forces, ESP parts vesp_k / vesp_dc_k, per-atom energy and per-atom stress (scaled by
+// akk, without the final prefactors).  Partials Fpart[chunk][16][n]:
+// f(3), vk, vdck, e, s(9).
+__global__ __launch_bounds__(64) void k_el_kfinal(ElecDev P, const int* __restrict__ runs,
+                                                  const int* __restrict__ runk, const double* __restrict__ phc,
+                                                  const double* __restrict__ phs, const double* __restrict__ kv,
+                                                  const double* __restrict__ kf, const double* __restrict__ chge,
+                                                  const double* __restrict__ dip, double* __restrict__ Fpart)
+{
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int rpc = (P.nrun + P.kchunks - 1) / P.kchunks;
+    const int r0 = chunk * rpc, r1 = min(P.nrun, r0 + rpc);
+    if (j >= P.n) return;
+    const int ks = P.kstride;
+    const bool pol = site_polr2(P, j) > 1.e-6;
+    const double q = chge[j];
+    const double mu[3] = {dip[3 * j], dip[3 * j + 1], dip[3 * j + 2]};
+    double acc[15];
+#pragma unroll
+    for (int t = 0; t < 15; ++t) acc[t] = 0.0;
+    const double c1 = phc[((long)2 * ks + 1) * P.n + j], s1 = phs[((long)2 * ks + 1) * P.n + j];
+    for (int r = r0; r < r1; ++r) {
+        const int l = runs[4 * r], m = runs[4 * r + 1], n0 = runs[4 * r + 2], cnt = runs[4 * r + 3];
+        const int am = abs(m), an0 = abs(n0);
+        double ckr, skr;
+        run_phase(phc[(long)l * P.n + j], phs[(long)l * P.n + j], phc[((long)ks + am) * P.n + j],
+                  phs[((long)ks + am) * P.n + j], phc[((long)2 * ks + an0) * P.n + j],
+                  phs[((long)2 * ks + an0) * P.n + j], m, n0, ckr, skr);
+        const int k0 = runk[r];
+        for (int t = 0; t < cnt; ++t) {
+            const int k = k0 + t;
+            const double kk[3] = {kv[8 * k], kv[8 * k + 1], kv[8 * k + 2]};
+            const double akk = kv[8 * k + 3], akv2 = kv[8 * k + 5];
+            const double* f = kf + 16 * k;
+            const double Qc = f[0], Qs = f[1], summu1 = f[2], summu2 = f[3];
+            const double skdmuckrs = f[4], skdmuskrs = f[5];
+            const double* Mc = f + 8;
+            const double* Ms = f + 11;
+            const double djmuk = pol ? (mu[0] * kk[0] + mu[1] * kk[1] + mu[2] * kk[2]) : 0.0;
+            const double ssmu = summu1 * (-skr * q - ckr * djmuk) + summu2 * (-skr * djmuk + ckr * q);
+            acc[0] -= kk[0] * akk * ssmu;
+            acc[1] -= kk[1] * akk * ssmu;
+            acc[2] -= kk[2] * akk * ssmu;
+            acc[3] += akk * (ckr * Qc + skr * Qs);
+            acc[4] += akk * (skr * skdmuckrs - ckr * skdmuskrs);
+            const double Cqi = q * ckr, Sqi = q * skr;
+            const double CMi[3] = {mu[0] * ckr, mu[1] * ckr, mu[2] * ckr};
+            const double SMi[3] = {mu[0] * skr, mu[1] * skr, mu[2] * skr};
+            const double dpkCMi = kk[0] * CMi[0] + kk[1] * CMi[1] + kk[2] * CMi[2];
+            const double dpkSMi = kk[0] * SMi[0] + kk[1] * SMi[1] + kk[2] * SMi[2];
+            const double dpkMc = skdmuckrs, dpkMs = skdmuskrs;
+            const double cc = Cqi * Qc + Sqi * Qs;
+            const double cd = Sqi * dpkMc - Cqi * dpkMs - dpkSMi * Qc + dpkCMi * Qs;
+            const double dd = dpkCMi * dpkMc + dpkSMi * dpkMs;
+            acc[5] += akk * cc;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const double B = (a == b ? 1.0 : 0.0) - akv2 * kk[a] * kk[b];
+                    const double sCC = B * cc;
+                    const double sCD = B * cd + Sqi * kk[a] * Mc[b] - Cqi * kk[a] * Ms[b] - kk[a] * SMi[b] * Qc +
+                                       kk[a] * CMi[b] * Qs;
+                    const double sDD = B * dd + kk[a] * CMi[b] * dpkMc + kk[a] * SMi[b] * dpkMs +
+                                       dpkCMi * kk[a] * Mc[b] + dpkSMi * kk[a] * Ms[b];
+                    acc[6 + 3 * a + b] += akk * (sCC + sCD + sDD);
+                }
+            const double cn = ckr * c1 - skr * s1;
+            skr = skr * c1 + ckr * s1;
+            ckr = cn;
+        }
+    }
+    double* out = Fpart + (long)chunk * 16 * P.n;
+#pragma unroll
+    for (int t = 0; t < 15; ++t) out[(long)t * P.n + j] = acc[t];
+}
+
+// Sum the final-pass partials in chunk order and apply the prefactors (ewald1p.f:941-1001):
+// fk = scale * f, vesp_k, vesp_dc_k = scale * ..., e_ew1p = scal1 * e + self energy,
+// s_ew1p = -scal1 * s, vesp_s.  Block partials [nb][2]: sum of q^2 (self energy).
+__global__ __launch_bounds__(256) void k_el_kfinal_reduce(ElecDev P, const double* __restrict__ Fpart,
+                                                          const double* __restrict__ chge, double* __restrict__ fk,
+                                                          double* __restrict__ vk, double* __restrict__ vdck,
+                                                          double* __restrict__ vs, double* __restrict__ e1,
+                                                          double* __restrict__ s1, double* __restrict__ blk)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (j < P.n) {
+        double a[15];
+        for (int t = 0; t < 15; ++t) a[t] = 0.0;
+        for (int c = 0; c < P.kchunks; ++c) {
+            const double* in = Fpart + (long)c * 16 * P.n;
+            for (int t = 0; t < 15; ++t) a[t] += in[(long)t * P.n + j];
+        }
+        const double q = chge[j];
+        for (int t = 0; t < 3; ++t) fk[3 * j + t] = P.scale * a[t];
+        vk[j] = a[3] * P.scale;
+        vdck[j] = a[4] * P.scale;
+        vs[j] = -2.0 * R4PIE0 * q * P.alpha / SQRPI;
+        e1[j] = -R4PIE0 * P.alpha * q * q / SQRPI + P.scal1 * a[5];
+        for (int t = 0; t < 9; ++t) s1[9 * j + t] = -(P.scal1 * a[6 + t]);
+        v[0] = q * q;
+    }
+    block_sum_store(v, blk + 2 * blockIdx.x);
+}
+
+// Real-space pair terms of one row (ewald2p.f:118-600; nthole = 4, lacs, lads): forces,
+// per-atom energy, ESP parts, and the row's force-matrix contraction W_i (real partners
+// only; for an M-site row the partner sums sum_j F and sum_j F (x) x_j, from which the
+// M-row redistribution builds W of O, H1, H2).  Each pair is seen from both of its rows,
+// so block partials carry half of each pair's energy, virial and stress.  One wavefront
+// per site; partials [n][8]: energy, virial, stress (xx xy xz yy yz zz).
+__global__ __launch_bounds__(64) void k_el_pairs(ElecDev P, const double* __restrict__ x,
+                                                 const double* __restrict__ y, const double* __restrict__ z,
+                                                 const double* __restrict__ chge, const double* __restrict__ dip,
+                                                 const double* __restrict__ tab, const int* __restrict__ nbrA,
+                                                 const int* __restrict__ cntA, double g34, double gln34,
+                                                 double* __restrict__ f2, double* __restrict__ e2,
+                                                 double* __restrict__ vr, double* __restrict__ vdcr,
+                                                 double* __restrict__ w2, double* __restrict__ fmsum,
+                                                 double* __restrict__ fmx, double* __restrict__ blk)
+{
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (i >= P.n) return;
+    const double qi = chge[i];
+    const double pi_ = site_polr(P, i);
+    const bool poli = site_polr2(P, i) > 1.e-6;
+    const double mi[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
+    const bool mrow = i >= P.nreal;
+    double fsum[3] = {0, 0, 0}, esum = 0.0, vrs = 0.0, vdcs = 0.0, W[9], red[8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) W[t] = 0.0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) red[t] = 0.0;
+    double fm[3] = {0, 0, 0};
+    const int na = cntA[i];
+    for (int s = lane; s < na; s += 64) {
+        const int j = nbrA[(long)i * P.capA + s];
+        double d[3];
+        const double rsq = sep(P, x, y, z, i, j, d);
+        if (!(P.rcut * P.rcut > rsq)) continue;
+        const double rrr = sqrt(rsq);
+        double bbb[4];
+        table_interp(tab, P.drewd, rrr, bbb);
+        const double rri = 1.0 / rrr, rsqi = 1.0 / rsq;
+        const double r3i = 1.0 / (rrr * rrr * rrr);
+        const double r5i = r3i * rsqi;
+        const double qj = chge[j];
+        const double cicj = qi * qj;
+        const double edum = cicj * (bbb[0] + 1.0 / rrr) * R4PIE0;
+        double espdum = (bbb[0] + 1.0 / rrr) * R4PIE0;
+        double ffac = cicj * (r3i + bbb[1]) * R4PIE0;
+        const double bb = pow(pi_ * site_polr(P, j), 1.0 / 6.0);
+        const double u = rrr / bb, u4 = (u * u) * (u * u);
+        double eng;
+        {
+            const double arob4 = P.ascc * u4;
+            const double dum1 = exp(-arob4);
+            const double sm = (dum1 - pow(arob4, 0.25) * g34 * gammq_nr(0.75, arob4, gln34)) / rrr * R4PIE0;
+            eng = edum - cicj * sm;
+            esum += 0.5 * edum;
+            esum += -cicj * 0.5 * sm;
+            espdum -= sm;
+            ffac -= cicj * r3i * dum1 * R4PIE0;
+        }
+        const double mj[3] = {dip[3 * j], dip[3 * j + 1], dip[3 * j + 2]};
+        const double rdmui = mi[0] * d[0] + mi[1] * d[1] + mi[2] * d[2];
+        const double rdmuj = mj[0] * d[0] + mj[1] * d[1] + mj[2] * d[2];
+        const double facmu0 = (qj * rdmui - qi * rdmuj) * (-3.0) * r5i * R4PIE0;
+        const double facmui0 = r3i * qj * R4PIE0;
+        const double facmuj0 = -r3i * qi * R4PIE0;
+        double facmu = facmu0 - (qj * rdmui - qi * rdmuj) * bbb[2] * R4PIE0;
+        double dfacmui = facmui0 + bbb[1] * qj * R4PIE0;
+        double dfacmuj = facmuj0 - bbb[1] * qi * R4PIE0;
+        double espdcdum = (bbb[1] + r3i) * R4PIE0;
+        {
+            const double arob4 = P.ascd * u4;
+            const double dum1 = exp(-arob4);
+            const double dum2 = (1.0 + (4.0 / 3.0) * arob4) * dum1;
+            facmu -= facmu0 * dum2;
+            dfacmui -= facmui0 * dum1;
+            dfacmuj -= facmuj0 * dum1;
+            espdcdum -= r3i * dum1 * R4PIE0;
+        }
+        if (poli && site_polr2(P, j) > 1.e-6) {
+            const double dpidpj = mi[0] * mj[0] + mi[1] * mj[1] + mi[2] * mj[2];
+            facmu += ((dpidpj * 3.0 + rdmui * rdmuj * rsqi * (-15.0)) * r5i + bbb[2] * dpidpj - bbb[3] * rdmui * rdmuj) *
+                     R4PIE0;
+            dfacmui += (3.0 * r5i + bbb[2]) * rdmuj * R4PIE0;
+            dfacmuj += (3.0 * r5i + bbb[2]) * rdmui * R4PIE0;
+            const double facmu1 = -15.0 * rdmui * rdmuj * rsqi * r5i * R4PIE0;
+            const double facmu2 = 3.0 * dpidpj * r5i * R4PIE0;
+            const double dfacmui1 = 3.0 * rdmuj * r5i * R4PIE0;
+            const double dfacmuj1 = 3.0 * rdmui * r5i * R4PIE0;
+            const double arob4 = P.athole * u4;
+            const double dum1 = exp(-arob4);
+            const double dum2 = (1.0 + (4.0 / 3.0) * arob4) * dum1;
+            const double bb4 = (bb * bb) * (bb * bb);
+            const double dum3 = 4.0 * P.athole * (4.0 * arob4 - 1.0) * dum1 * rsqi / bb4;
+            const double dum4 = 4.0 * P.athole * dum1 * rrr / bb4;
+            facmu = facmu - facmu1 * dum2 - facmu2 * dum1 + rdmui * rdmuj * dum3 * rri * R4PIE0 -
+                    dpidpj * rsqi * dum4 * R4PIE0;
+            dfacmui -= dfacmui1 * dum2;
+            dfacmuj -= dfacmuj1 * dum2;
+        }
+        const double dfac[3] = {dfacmui * mi[0] + dfacmuj * mj[0], dfacmui * mi[1] + dfacmuj * mj[1],
+                                dfacmui * mi[2] + dfacmuj * mj[2]};
+        ffac += facmu;
+        const double virdum = dfac[0] * d[0] + dfac[1] * d[1] + dfac[2] * d[2];
+        const double fv[3] = {ffac * d[0] + dfac[0], ffac * d[1] + dfac[1], ffac * d[2] + dfac[2]};
+        for (int q = 0; q < 3; ++q) fsum[q] += fv[q];
+        vrs += qj * espdum;
+        vdcs += rdmuj * espdcdum;
+        red[0] += 0.5 * eng;
+        red[1] += 0.5 * (-ffac * rsq - virdum);
+        red[2] += 0.5 * d[0] * fv[0];
+        red[3] += 0.5 * d[0] * fv[1];
+        red[4] += 0.5 * d[0] * fv[2];
+        red[5] += 0.5 * d[1] * fv[1];
+        red[6] += 0.5 * d[1] * fv[2];
+        red[7] += 0.5 * d[2] * fv[2];
+        if (j < P.nreal) {
+            if (!mrow) {
+                const double dr[3] = {x[i] - x[j], y[i] - y[j], z[i] - z[j]};
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) W[3 * a + b] += fv[a] * dr[b];
+            } else {
+                const double xj[3] = {x[j], y[j], z[j]};
+                for (int a = 0; a < 3; ++a) {
+                    fm[a] += fv[a];
+                    for (int b = 0; b < 3; ++b) W[3 * a + b] += fv[a] * xj[b];
+                }
+            }
+        }
+    }
+    for (int q = 0; q < 3; ++q) fsum[q] = wsum(fsum[q]);
+    for (int q = 0; q < 3; ++q) fm[q] = wsum(fm[q]);
+    for (int t = 0; t < 9; ++t) W[t] = wsum(W[t]);
+    esum = wsum(esum);
+    vrs = wsum(vrs);
+    vdcs = wsum(vdcs);
+    for (int t = 0; t < 8; ++t) red[t] = wsum(red[t]);
+    if (lane == 0) {
+        for (int q = 0; q < 3; ++q) f2[3 * i + q] = fsum[q];
+        e2[i] = esum;
+        vr[i] = vrs;
+        vdcr[i] = vdcs;
+        if (!mrow) {
+            for (int t = 0; t < 9; ++t) w2[9 * i + t] = W[t];
+        } else {
+            for (int q = 0; q < 3; ++q) fmsum[3 * i + q] = fm[q];
+            for (int t = 0; t < 9; ++t) fmx[9 * i + t] = W[t];
+        }
+        for (int t = 0; t < 8; ++t) blk[8 * i + t] = red[t];
+    }
+}
+
+namespace {
+
+// force-matrix entry F at (row a, column b) of water-local work: contraction into the
+// per-atom tensors of the water's real atoms (M rows moved to O, H1, H2), columns of
+// real atoms only, diagonal entries skipped (heatcurrent.f90:541-555, 711-719).
+__device__ void fm_local(const ElecDev& P, const double* x, const double* y, const double* z, int a, int b,
+                         const double* F, double* w3)
+{
+    if (b >= P.nreal) return;
+    if (a < P.nreal) {
+        if (a == b) return;
+        const double dr[3] = {x[a] - x[b], y[a] - y[b], z[a] - z[b]};
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) w3[9 * a + 3 * p + q] += F[p] * dr[q];
+        return;
+    }
+    const int w = a - P.nreal;
+    const double wt[3] = {1.0 - P.gamma, 0.5 * P.gamma, 0.5 * P.gamma};
+    for (int t = 0; t < 3; ++t) {
+        const int r = 3 * w + t;
+        if (r == b) continue;
+        const double dr[3] = {x[r] - x[b], y[r] - y[b], z[r] - z[b]};
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) w3[9 * r + 3 * p + q] += wt[t] * F[p] * dr[q];
+    }
+}
+
+}  // namespace
+
+// Excluded-pair corrections of each water (ewald3p.f:80-391).  Writes the water's rows of
+// f3, e3, vc, w3 (all zero-initialised by this kernel) and block partials [nb][8]:
+// energy, virial, stress.
+__global__ __launch_bounds__(64) void k_el_intra_final(ElecDev P, const double* __restrict__ x,
+                                                       const double* __restrict__ y, const double* __restrict__ z,
+                                                       const double* __restrict__ chge, const double* __restrict__ dip,
+                                                       double* __restrict__ f3, double* __restrict__ e3,
+                                                       double* __restrict__ vc, double* __restrict__ w3,
+                                                       double* __restrict__ blk)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (w < P.nw) {
+        const int s[4] = {3 * w, 3 * w + 1, 3 * w + 2, P.nreal + w};
+        for (int a = 0; a < 4; ++a) {
+            const int i = s[a];
+            for (int q = 0; q < 3; ++q) f3[3 * i + q] = 0.0;
+            e3[i] = 0.0;
+            vc[i] = 0.0;
+            if (i < P.nreal)
+                for (int t = 0; t < 9; ++t) w3[9 * i + t] = 0.0;
+        }
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b) {
+                const int i = s[a], j = s[b];
+                double d[3];
+                const double rsq = sep(P, x, y, z, i, j, d);
+                const double rrr = sqrt(rsq);
+                const double rri = 1.0 / rrr, rsqi = 1.0 / rsq;
+                const double r3i = 1.0 / (rrr * rrr * rrr);
+                const double r5i = r3i * rsqi;
+                double bbb[4];
+                bfun_direct(P.alpha, rrr, rsqi, bbb);
+                const double qi = chge[i], qj = chge[j];
+                const double cicj = qi * qj;
+                red[0] += cicj * bbb[0] * R4PIE0;
+                e3[i] += 0.5 * cicj * bbb[0] * R4PIE0;
+                e3[j] += 0.5 * cicj * bbb[0] * R4PIE0;
+                const double espdum = bbb[0] * R4PIE0;
+                const double espdcdum = bbb[1] * R4PIE0;
+                double ffac = cicj * bbb[1] * R4PIE0;
+                const double mi[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
+                const double mj[3] = {dip[3 * j], dip[3 * j + 1], dip[3 * j + 2]};
+                const double rdmui = mi[0] * d[0] + mi[1] * d[1] + mi[2] * d[2];
+                const double rdmuj = mj[0] * d[0] + mj[1] * d[1] + mj[2] * d[2];
+                double facmu = (qj * rdmui - qi * rdmuj) * (-bbb[2]) * R4PIE0;
+                double dfacmui = bbb[1] * qj * R4PIE0;
+                double dfacmuj = -bbb[1] * qi * R4PIE0;
+                if (site_polr2(P, i) > 1.e-6 && site_polr2(P, j) > 1.e-6) {
+                    const double dpidpj = mi[0] * mj[0] + mi[1] * mj[1] + mi[2] * mj[2];
+                    facmu += (bbb[2] * dpidpj - bbb[3] * rdmui * rdmuj) * R4PIE0;
+                    dfacmui += bbb[2] * rdmuj * R4PIE0;
+                    dfacmuj += bbb[2] * rdmui * R4PIE0;
+                    const double facmu1 = -15.0 * rdmui * rdmuj * rsqi * r5i * R4PIE0;
+                    const double facmu2 = 3.0 * dpidpj * r5i * R4PIE0;
+                    facmu = facmu + facmu1 + facmu2;
+                    const double dfacmuidd = 3.0 * rdmuj * r5i * R4PIE0;
+                    const double dfacmujdd = 3.0 * rdmui * r5i * R4PIE0;
+                    dfacmui += dfacmuidd;
+                    dfacmuj += dfacmujdd;
+                    const double bb = pow(site_polr(P, i) * site_polr(P, j), 1.0 / 6.0);
+                    const double ath = (i % 3 == 0 || j % 3 == 0) ? P.athole12 : P.athole13;
+                    const double u = rrr / bb;
+                    const double arob4 = ath * (u * u) * (u * u);
+                    const double dum1 = exp(-arob4);
+                    const double dum2 = (1.0 + (4.0 / 3.0) * arob4) * dum1;
+                    const double bb4 = (bb * bb) * (bb * bb);
+                    const double dum3 = 4.0 * ath * (4.0 * arob4 - 1.0) * dum1 * rsqi / bb4;
+                    const double dum4 = 4.0 * ath * dum1 * rrr / bb4;
+                    facmu = facmu - facmu1 * dum2 - facmu2 * dum1 + rdmui * rdmuj * dum3 * rri * R4PIE0 -
+                            dpidpj * rsqi * dum4 * R4PIE0;
+                    dfacmui -= dfacmuidd * dum2;
+                    dfacmuj -= dfacmujdd * dum2;
+                }
+                const double dfac[3] = {dfacmui * mi[0] + dfacmuj * mj[0], dfacmui * mi[1] + dfacmuj * mj[1],
+                                        dfacmui * mi[2] + dfacmuj * mj[2]};
+                ffac += facmu;
+                const double fv[3] = {ffac * d[0] + dfac[0], ffac * d[1] + dfac[1], ffac * d[2] + dfac[2]};
+                const double virdum = dfac[0] * d[0] + dfac[1] * d[1] + dfac[2] * d[2];
+                for (int q = 0; q < 3; ++q) {
+                    f3[3 * i + q] += fv[q];
+                    f3[3 * j + q] -= fv[q];
+                }
+                const double mf[3] = {-fv[0], -fv[1], -fv[2]};
+                fm_local(P, x, y, z, i, j, fv, w3);
+                fm_local(P, x, y, z, j, i, mf, w3);
+                red[1] += -ffac * rsq - virdum;
+                red[2] += d[0] * fv[0];
+                red[3] += d[0] * fv[1];
+                red[4] += d[0] * fv[2];
+                red[5] += d[1] * fv[1];
+                red[6] += d[1] * fv[2];
+                red[7] += d[2] * fv[2];
+                vc[i] += qj * espdum;
+                vc[j] += qi * espdum;
+                vc[i] += rdmuj * espdcdum;
+                vc[j] -= rdmui * espdcdum;
+            }
+    }
+    block_sum_store(red, blk + 8 * blockIdx.x);
+}
+
+// ESP at every site (forces.f:876-880).
+__global__ void k_el_vesp(int n, const double* __restrict__ vk, const double* __restrict__ vr,
+                          const double* __restrict__ vs, const double* __restrict__ vc, const double* __restrict__ vdck,
+                          const double* __restrict__ vdcr, double* __restrict__ vesp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) vesp[i] = vk[i] + vr[i] + vs[i] + vc[i] + vdck[i] + vdcr[i] + 0.0;
+}
+
+// Charge-derivative forces of each water (qdforce.f90:70-566): adds to the water's rows
+// of f3 and w3; block partials [nb][8]: 0, virial, stress.
+__global__ __launch_bounds__(64) void k_el_qd(ElecDev P, const double* __restrict__ x, const double* __restrict__ y,
+                                              const double* __restrict__ z, const double* __restrict__ dg,
+                                              const double* __restrict__ vesp, double* __restrict__ f3,
+                                              double* __restrict__ w3, double* __restrict__ blk)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (w < P.nw) {
+        const int io = 3 * w, ih1 = io + 1, ih2 = io + 2, im = P.nreal + w;
+        const int site[4] = {io, ih1, ih2, im};
+        const double* G = dg + 6 * w;   // dp1dr1, dp1dr2, dp2dr1, dp2dr2, dp1dcos, dp2dcos
+        double r1v[3], r2v[3];
+        sep(P, x, y, z, ih1, io, r1v);
+        sep(P, x, y, z, ih2, io, r2v);
+        const double r1 = sqrt(r1v[0] * r1v[0] + r1v[1] * r1v[1] + r1v[2] * r1v[2]);
+        const double r2 = sqrt(r2v[0] * r2v[0] + r2v[1] * r2v[1] + r2v[2] * r2v[2]);
+        const double rdot = r1v[0] * r2v[0] + r1v[1] * r2v[1] + r1v[2] * r2v[2];
+        const double cost = rdot / (r1 * r2);
+        const double r3 = sqrt(r1 * r1 + r2 * r2 - 2.0 * r1 * r2 * cost);
+        const double sint = sqrt(1.0 - cost * cost);
+        const double cott = cost / sint;
+        const double r1sqr = r1 * r1, r2sqr = r2 * r2;
+        double dr[3][4][3];
+        for (int k = 0; k < 3; ++k)
+            for (int s = 0; s < 4; ++s)
+                for (int q = 0; q < 3; ++q) dr[k][s][q] = 0.0;
+        for (int q = 0; q < 3; ++q) {
+            dr[0][1][q] = r1v[q] / r1;
+            dr[0][0][q] = -dr[0][1][q];
+            dr[1][2][q] = r2v[q] / r2;
+            dr[1][0][q] = -dr[1][2][q];
+            dr[2][1][q] = cott * (r1v[q] / r1sqr - r2v[q] / rdot);
+            dr[2][2][q] = cott * (r2v[q] / r2sqr - r1v[q] / rdot);
+            dr[2][0][q] = -dr[2][1][q] - dr[2][2][q];
+        }
+        const double tmp = P.gamma / 2.0 / (1.0 - P.gamma);
+        double dq[4][4][3];
+        for (int s = 0; s < 4; ++s)
+            for (int q = 0; q < 3; ++q) {
+                const double dp1 = G[0] * dr[0][s][q] + G[1] * dr[1][s][q] - sint * G[4] * dr[2][s][q];
+                const double dp2 = G[2] * dr[0][s][q] + G[3] * dr[1][s][q] - sint * G[5] * dr[2][s][q];
+                dq[0][s][q] = 0.0;
+                dq[1][s][q] = dp1 * (tmp + 1.0) + dp2 * tmp;
+                dq[2][s][q] = dp2 * (tmp + 1.0) + dp1 * tmp;
+                dq[3][s][q] = -(dq[1][s][q] + dq[2][s][q]);
+            }
+        for (int j = 0; j < 4; ++j) {
+            const double vt = vesp[site[j]];
+            const int col = (j < 3) ? site[j] : (P.compat_qd ? P.nw + w : im);
+            for (int s = 0; s < 4; ++s) {
+                const double F[3] = {-dq[j][s][0] * vt, -dq[j][s][1] * vt, -dq[j][s][2] * vt};
+                for (int q = 0; q < 3; ++q) f3[3 * site[s] + q] += F[q];
+                fm_local(P, x, y, z, site[s], col, F, w3);
+            }
+        }
+        double dqdr12[4] = {0, 0, 0, 0}, dqdr13[4] = {0, 0, 0, 0}, dqdcos[4] = {0, 0, 0, 0};
+        dqdr12[1] = G[0] + (G[0] + G[2]) * tmp;
+        dqdr13[1] = G[1] + (G[1] + G[3]) * tmp;
+        dqdcos[1] = G[4] + (G[4] + G[5]) * tmp;
+        dqdr12[2] = G[2] + (G[0] + G[2]) * tmp;
+        dqdr13[2] = G[3] + (G[1] + G[3]) * tmp;
+        dqdcos[2] = G[5] + (G[4] + G[5]) * tmp;
+        dqdr12[3] = -(G[0] + G[2]) / (1.0 - P.gamma);
+        dqdr13[3] = -(G[1] + G[3]) / (1.0 - P.gamma);
+        dqdcos[3] = -(G[4] + G[5]) / (1.0 - P.gamma);
+        double rv[2][3], r3v[3];
+        sep(P, x, y, z, io, ih1, rv[0]);
+        sep(P, x, y, z, io, ih2, rv[1]);
+        sep(P, x, y, z, ih1, ih2, r3v);
+        const double rr1 = sqrt(rv[0][0] * rv[0][0] + rv[0][1] * rv[0][1] + rv[0][2] * rv[0][2]);
+        const double rr2 = sqrt(rv[1][0] * rv[1][0] + rv[1][1] * rv[1][1] + rv[1][2] * rv[1][2]);
+        const double rr3 = sqrt(r3v[0] * r3v[0] + r3v[1] * r3v[1] + r3v[2] * r3v[2]);
+        const double dcosdr12 = (r1 - r2 * cost) / r1 / r2;
+        const double dcosdr13 = (r2 - r1 * cost) / r1 / r2;
+        const double dcosdr23 = -r3 / r1 / r2;
+        const int sa[6] = {0, 0, 0, 1, 1, 2}, sb[6] = {0, 1, 2, 1, 2, 2};
+        for (int i = 0; i < 4; ++i) {
+            const double vt = vesp[site[i]];
+            for (int pr = 0; pr < 2; ++pr) {
+                const double dqdr = pr == 0 ? dqdr12[i] : dqdr13[i];
+                const double* r = rv[pr];
+                const double rjk = pr == 0 ? rr1 : rr2;
+                for (int t = 0; t < 6; ++t) red[2 + t] -= vt * dqdr * r[sa[t]] * r[sb[t]] / rjk;
+                red[1] += vt * rjk * dqdr;
+            }
+            double dc[6];
+            for (int t = 0; t < 6; ++t)
+                dc[t] = dcosdr12 * rv[0][sa[t]] * rv[0][sb[t]] / rr1 + dcosdr13 * rv[1][sa[t]] * rv[1][sb[t]] / rr2 +
+                        dcosdr23 * r3v[sa[t]] * r3v[sb[t]] / rr3;
+            for (int t = 0; t < 6; ++t) red[2 + t] -= vt * dqdcos[i] * dc[t];
+            red[1] += vt * (dc[0] + dc[3] + dc[5]) * dqdcos[i];
+        }
+    }
+    block_sum_store(red, blk + 8 * blockIdx.x);
+}
+
+// Site forces and the per-real-atom outputs of the heat-current decomposition, with the
+// M-site redistribution (dlpoly_cmd.f:2583-2606, heatcurrent.f90:302-555).  One thread
+// per water: fout (3 per site; M forces moved to O, H1, H2 when distribute != 0),
+// eout (energy ew1p + ew2p + ew3p + polar), sout (ew1p stress), wout (force-matrix
+// contraction) for O, H1, H2.
+__global__ void k_el_combine(ElecDev P, int distribute, const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, const double* __restrict__ fk, const double* __restrict__ f2,
+                             const double* __restrict__ f3, const double* __restrict__ e1, const double* __restrict__ e2,
+                             const double* __restrict__ e3, const double* __restrict__ ep,
+                             const double* __restrict__ s1, const double* __restrict__ w2,
+                             const double* __restrict__ w3, const double* __restrict__ fmsum,
+                             const double* __restrict__ fmx, double* __restrict__ fout, double* __restrict__ eout,
+                             double* __restrict__ sout, double* __restrict__ wout)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= P.nw) return;
+    const int m = P.nreal + w;
+    const double wt[3] = {1.0 - P.gamma, P.gamma / 2.0, P.gamma / 2.0};
+    double fm[3];
+    for (int q = 0; q < 3; ++q) fm[q] = fk[3 * m + q] + f2[3 * m + q] + f3[3 * m + q];
+    const double em = e1[m] + e2[m] + e3[m] + ep[m];
+    for (int t = 0; t < 3; ++t) {
+        const int r = 3 * w + t;
+        for (int q = 0; q < 3; ++q) {
+            const double f = fk[3 * r + q] + f2[3 * r + q] + f3[3 * r + q];
+            fout[3 * r + q] = distribute ? f + wt[t] * fm[q] : f;
+        }
+        eout[r] = (e1[r] + e2[r] + e3[r] + ep[r]) + wt[t] * em;
+        for (int q = 0; q < 9; ++q) sout[9 * r + q] = s1[9 * r + q] + wt[t] * s1[9 * m + q];
+        const double xr[3] = {x[r], y[r], z[r]};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                wout[9 * r + 3 * a + b] =
+                    w2[9 * r + 3 * a + b] + w3[9 * r + 3 * a + b] + wt[t] * (fmsum[3 * m + a] * xr[b] - fmx[9 * m + 3 * a + b]);
+    }
+    for (int q = 0; q < 3; ++q) fout[3 * m + q] = distribute ? 0.0 : fm[q];
+}
+
+// Fixed-order sums of block partials: out[q] = sum_b blk[b * stride + q], q < nv.
+__global__ __launch_bounds__(64) void k_el_sum_blocks(const double* __restrict__ blk, int nb, int stride, int nv,
+                                                      double* __restrict__ out)
+{
+    for (int q = 0; q < nv; ++q) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 64) s += blk[(long)b * stride + q];
+        s = wsum(s);
+        if (threadIdx.x == 0) out[q] = s;
+    }
+}
+
+// Zero a double array (graph-safe replacement for a memset node).
+__global__ void k_el_zero(double* __restrict__ p, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ============================================================ host launchers ======
+
+namespace {
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+}
+
+int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s)
+{
+    const int nwb = ceil_div(P.nw, 64);
+    hipLaunchKernelGGL(k_el_sites, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.dg);
+    hipLaunchKernelGGL(k_el_nbr, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.nbrA, E.cntA, E.nbrP, E.cntP, E.err);
+    return (int)hipGetLastError();
+}
+
+// after the neighbour counts were checked by the host
+int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hipStream_t s)
+{
+    const int nwb = ceil_div(P.nw, 64);
+    hipLaunchKernelGGL(k_el_prep, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tab, E.nbrA, E.cntA, E.nbrP,
+                       E.cntP, E.tens, E.efk);
+    hipLaunchKernelGGL(k_el_intra_prep, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tin, E.efk);
+    hipLaunchKernelGGL(k_el_phase, dim3(ceil_div(P.n, 64)), dim3(64), 0, s, P, E.x, E.y, E.z, rcell_dev, E.phc, E.phs,
+                       E.pht);
+    // charge structure factors and the iloop = 1 reciprocal fields
+    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div(P.nrun, 64), E.schunks), dim3(64), 0, s, P, E.runs, E.runk,
+                       E.pht, E.chge, E.dip, E.Spart, E.state);
+    hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
+                       E.Sq, E.kc, E.state);
+    hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 64), P.kchunks), dim3(64), 0, s, P, E.runs, E.runk, E.phc,
+                       E.phs, E.kc, E.Fpart, E.state);
+    hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(P.n, 256)), dim3(256), 0, s, P, E.Fpart, E.dip, E.efdd,
+                       E.efdc, E.state);
+    return (int)hipGetLastError();
+}
+
+int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t s)
+{
+    if (!first) {
+        // reciprocal dipole field from the current dipoles (the first iteration's came
+        // with the charge field)
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.nrun, 64), E.schunks), dim3(64), 0, s, P, E.runs, E.runk,
+                           E.pht, E.chge, E.dip, E.Spart, E.state);
+        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv,
+                           E.S, E.Sq, E.kc, E.state);
+        hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 64), P.kchunks), dim3(64), 0, s, P, E.runs,
+                           E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
+        hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, E.dip,
+                           E.efdd, E.efdc, E.state);
+    }
+    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
+    const int nb = ceil_div(P.nrow, 256);
+    hipLaunchKernelGGL(k_el_sor, dim3(nb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.blk, E.state);
+    hipLaunchKernelGGL(k_el_converge, dim3(1), dim3(64), 0, s, P, nb, E.blk, E.state);
+    return (int)hipGetLastError();
+}
+
+int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, hipStream_t s)
+{
+    const int nsb = ceil_div(P.n, 256), nwb = ceil_div(P.nw, 64), nkb = ceil_div(P.K, 256);
+    double* b_pol = E.blk;                       // [nsb]
+    double* b_ks = b_pol + nsb;                  // [nkb][8]
+    double* b_q2 = b_ks + 8L * nkb;              // [nsb][2]
+    double* b_p2 = b_q2 + 2L * nsb;              // [n][8]
+    double* b_p3 = b_p2 + 8L * P.n;              // [nwb][8]
+    double* b_qd = b_p3 + 8L * nwb;              // [nwb][8]
+    hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
+    // structure factors of the converged dipoles
+    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.nrun, 64), E.schunks), dim3(64), 0, s, P, E.runs, E.runk,
+                       E.pht, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
+    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(nkb), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
+                       (const double*)E.zero_state);
+    hipLaunchKernelGGL(k_el_kscal, dim3(nkb), dim3(256), 0, s, P, E.kv, E.S, E.Sq, E.kf, b_ks);
+    hipLaunchKernelGGL(k_el_kfinal, dim3(ceil_div(P.n, 64), P.kchunks), dim3(64), 0, s, P, E.runs, E.runk, E.phc,
+                       E.phs, E.kv, E.kf, E.chge, E.dip, E.Fpart);
+    hipLaunchKernelGGL(k_el_kfinal_reduce, dim3(nsb), dim3(256), 0, s, P, E.Fpart, E.chge, E.fk, E.vk, E.vdck, E.vs,
+                       E.e1, E.s1, b_q2);
+    hipLaunchKernelGGL(k_el_pairs, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.dip, E.tab, E.nbrA, E.cntA,
+                       g34, gln34, E.f2, E.e2, E.vr, E.vdcr, E.w2, E.fmsum, E.fmx, b_p2);
+    hipLaunchKernelGGL(k_el_intra_final, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.dip, E.f3, E.e3, E.vc,
+                       E.w3, b_p3);
+    hipLaunchKernelGGL(k_el_vesp, dim3(nsb), dim3(256), 0, s, P.n, E.vk, E.vr, E.vs, E.vc, E.vdck, E.vdcr, E.vesp);
+    hipLaunchKernelGGL(k_el_qd, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.dg, E.vesp, E.f3, E.w3, b_qd);
+    hipLaunchKernelGGL(k_el_combine, dim3(nwb), dim3(64), 0, s, P, distribute, E.x, E.y, E.z, E.fk, E.f2, E.f3, E.e1,
+                       E.e2, E.e3, E.ep, E.s1, E.w2, E.w3, E.fmsum, E.fmx, E.fsite, E.eout, E.sout, E.wout);
+    // scalars: [0] eng_polar, [1..8] k sums (engc, engdum, omg6), [9..10] sum q^2,
+    // [11..18] pairs (energy, virial, stress6), [19..26] excluded pairs, [27..34] qd
+    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_pol, nsb, 1, 1, E.scal);
+    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_ks, nkb, 8, 8, E.scal + 1);
+    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_q2, nsb, 2, 2, E.scal + 9);
+    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_p2, P.n, 8, 8, E.scal + 11);
+    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_p3, nwb, 8, 8, E.scal + 19);
+    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_qd, nwb, 8, 8, E.scal + 27);
+    return (int)hipGetLastError();
+}
+
+void el_launch_zero(double* p, long n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_el_zero, dim3(ceil_div(n, 256)), dim3(256), 0, s, p, n);
+}
+
+__global__ void k_el_add(double* __restrict__ dst, const double* __restrict__ src, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += src[i];
+}
+
+// dst[i] += src[i], i < n
+void el_launch_add(double* dst, const double* src, long n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_el_add, dim3(ceil_div(n, 256)), dim3(256), 0, s, dst, src, n);
+}

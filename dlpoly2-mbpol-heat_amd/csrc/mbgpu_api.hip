@@ -98,6 +98,7 @@ struct mbgpu_ctx {
     // heat-flux scratch
     double *dv = nullptr, *dek = nullptr, *dep = nullptr, *dw = nullptr, *dsew = nullptr, *dout = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    ElecState el;                     // polarizable electrostatics
 };
 
 namespace {
@@ -282,6 +283,22 @@ int leaf_eval(int nmon, const double* const* w, double* E, double* gout)
     return 0;
 }
 
+// Add the electrostatics' per-atom terms (when the last electrostatics call completed) to
+// the heat-flux extras: an extra already in a context scratch buffer gets the term added in
+// place; a missing extra is replaced by the term itself.
+void with_elec_terms(mbgpu_ctx* c, const double*& ep, const double*& we, const double*& se)
+{
+    if (!c->el.ready) return;
+    const int nr = 3 * c->B.nw;
+    const ElecBuffers& E = c->el.E;
+    if (ep) el_launch_add(const_cast<double*>(ep), E.eout, nr, c->stream);
+    else ep = E.eout;
+    if (we) el_launch_add(const_cast<double*>(we), E.wout, 9L * nr, c->stream);
+    else we = E.wout;
+    if (se) el_launch_add(const_cast<double*>(se), E.sout, 9L * nr, c->stream);
+    else se = E.sout;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI =======
@@ -330,6 +347,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(B.cs2); dfree(B.cs3);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
+    elec_release(c->el);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -430,9 +448,11 @@ int mbgpu_heat_flux(mbgpu_ctx* c, const double* vxx, const double* vyy, const do
     if (stress_ew1p)
         HIPCHK(hipMemcpyAsync(c->dsew, stress_ew1p, sizeof(double) * 9 * nr, hipMemcpyHostToDevice, c->stream));
     if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
-    int rc = mb_launch_heat_flux(nr, v, v + nr, v + 2 * nr, c->dek, c->B.atomic_energy,
-                                 e_pot_extra ? c->dep : nullptr, c->B.heat_w, w_extra ? c->dw : nullptr,
-                                 stress_ew1p ? c->dsew : nullptr, c->B.blk, c->B.nblk_cap, c->dout, c->stream);
+    const double *ep = e_pot_extra ? c->dep : nullptr, *we = w_extra ? c->dw : nullptr,
+                 *se = stress_ew1p ? c->dsew : nullptr;
+    with_elec_terms(c, ep, we, se);
+    int rc = mb_launch_heat_flux(nr, v, v + nr, v + 2 * nr, c->dek, c->B.atomic_energy, ep, c->B.heat_w, we, se,
+                                 c->B.blk, c->B.nblk_cap, c->dout, c->stream);
     if (rc) return fail(rc, "heat-flux kernels failed");
     double out[8];
     HIPCHK(hipMemcpyAsync(out, c->dout, sizeof out, hipMemcpyDeviceToHost, c->stream));
@@ -452,8 +472,19 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const 
     if (!c) return fail(-1, "null context");
     const int nr = 3 * c->B.nw;
     if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
-    int rc = mb_launch_heat_flux(nr, vx, vy, vz, e_kin, c->B.atomic_energy, e_pot_extra, c->B.heat_w, w_extra,
-                                 stress_ew1p, c->B.blk, c->B.nblk_cap, c->dout, c->stream);
+    const double *ep = e_pot_extra, *we = w_extra, *se = stress_ew1p;
+    if (c->el.ready) {
+        // the electrostatics' terms are added to copies of the caller's extras
+        if (ep) HIPCHK(hipMemcpyAsync(c->dep, ep, sizeof(double) * nr, hipMemcpyDeviceToDevice, c->stream));
+        if (we) HIPCHK(hipMemcpyAsync(c->dw, we, sizeof(double) * 9 * nr, hipMemcpyDeviceToDevice, c->stream));
+        if (se) HIPCHK(hipMemcpyAsync(c->dsew, se, sizeof(double) * 9 * nr, hipMemcpyDeviceToDevice, c->stream));
+        ep = ep ? c->dep : nullptr;
+        we = we ? c->dw : nullptr;
+        se = se ? c->dsew : nullptr;
+    }
+    with_elec_terms(c, ep, we, se);
+    int rc = mb_launch_heat_flux(nr, vx, vy, vz, e_kin, c->B.atomic_energy, ep, c->B.heat_w, we, se, c->B.blk,
+                                 c->B.nblk_cap, c->dout, c->stream);
     if (rc) return fail(rc, "heat-flux kernels failed");
     double out[8];
     HIPCHK(hipMemcpyAsync(out, c->dout, sizeof out, hipMemcpyDeviceToHost, c->stream));
@@ -463,6 +494,117 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const 
         jpot[a] = out[3 + a];
         jtot[a] = out[a] + out[3 + a];
     }
+    return 0;
+}
+
+int mbgpu_polar_electrostatics_dev(mbgpu_ctx* c, int imcon, const double cell[9], const mbgpu_elec_params* p,
+                                   double* x_dev, double* y_dev, double* z_dev, double* dip_dev, double* f_dev,
+                                   double* engcpe, double* vircpe, double stress[9], int* niter)
+{
+    if (!c) return fail(-1, "null context");
+    const int nw = c->B.nw, n = 4 * nw;
+    if (c->B.natms < n) return fail(-1, "the context holds %d sites; the electrostatics need 4*nw = %d", c->B.natms, n);
+    CellGeom g;
+    if (!make_geom(imcon, cell, g))
+        return fail(-4, "imcon = %d is not supported by images (reference stops in fixme)", imcon);
+    if (!p) return fail(-1, "null electrostatics parameters");
+    std::string err;
+    c->el.ready = false;
+    if (elec_configure(c->el, nw, g, cell, p, err)) return fail(-1, "%s", err.c_str());
+    ElecBuffers& E = c->el.E;
+    HIPCHK(hipMemcpyAsync(E.x, x_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(E.y, y_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(E.z, z_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(E.dip, dip_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, c->stream, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    const int nr = 3 * nw;
+    HIPCHK(hipMemcpyAsync(x_dev + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(y_dev + nr, E.y + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(z_dev + nr, E.z + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dip_dev, E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+    if (f_dev) el_launch_add(f_dev, E.fsite, 3L * n, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double* o = c->el.out;
+    if (engcpe) *engcpe += o[4];
+    if (vircpe) *vircpe += o[5];
+    if (stress)
+        for (int i = 0; i < 9; ++i) stress[i] += o[10 + i];
+    if (niter) *niter = (int)o[8];
+    return 0;
+}
+
+int mbgpu_polar_electrostatics(mbgpu_ctx* c, int imcon, const double cell[9], const mbgpu_elec_params* p,
+                               double* xxx, double* yyy, double* zzz, double* dipx, double* dipy, double* dipz,
+                               double* chge, double* fxx, double* fyy, double* fzz, double* engcpe, double* vircpe,
+                               double stress[9], int* niter)
+{
+    if (!c) return fail(-1, "null context");
+    const int nw = c->B.nw, n = 4 * nw;
+    if (c->B.natms < n) return fail(-1, "the context holds %d sites; the electrostatics need 4*nw = %d", c->B.natms, n);
+    if (!xxx || !yyy || !zzz || !dipx || !dipy || !dipz) return fail(-1, "null position or dipole array");
+    CellGeom g;
+    if (!make_geom(imcon, cell, g))
+        return fail(-4, "imcon = %d is not supported by images (reference stops in fixme)", imcon);
+    if (!p) return fail(-1, "null electrostatics parameters");
+    std::string err;
+    c->el.ready = false;
+    if (elec_configure(c->el, nw, g, cell, p, err)) return fail(-1, "%s", err.c_str());
+    ElecBuffers& E = c->el.E;
+    std::vector<double> dip((size_t)3 * n);
+    for (int i = 0; i < n; ++i) {
+        dip[3 * i] = dipx[i];
+        dip[3 * i + 1] = dipy[i];
+        dip[3 * i + 2] = dipz[i];
+    }
+    HIPCHK(hipMemcpyAsync(E.x, xxx, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(E.y, yyy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(E.z, zzz, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(E.dip, dip.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, c->stream, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    const int nr = 3 * nw;
+    std::vector<double> f((size_t)3 * n);
+    HIPCHK(hipMemcpyAsync(xxx + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(yyy + nr, E.y + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(zzz + nr, E.z + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(dip.data(), E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(f.data(), E.fsite, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    if (chge) HIPCHK(hipMemcpyAsync(chge, E.chge, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) {
+        dipx[i] = dip[3 * i];
+        dipy[i] = dip[3 * i + 1];
+        dipz[i] = dip[3 * i + 2];
+        if (fxx) fxx[i] += f[3 * i];
+        if (fyy) fyy[i] += f[3 * i + 1];
+        if (fzz) fzz[i] += f[3 * i + 2];
+    }
+    const double* o = c->el.out;
+    if (engcpe) *engcpe += o[4];
+    if (vircpe) *vircpe += o[5];
+    if (stress)
+        for (int i = 0; i < 9; ++i) stress[i] += o[10 + i];
+    if (niter) *niter = (int)o[8];
+    return 0;
+}
+
+int mbgpu_get_elec_details(mbgpu_ctx* c, double* scal, double* chge, double* vesp, double* e_real, double* w_real,
+                           double* s_real, double* f_site)
+{
+    if (!c) return fail(-1, "null context");
+    if (!c->el.ready) return fail(-1, "no completed electrostatics evaluation in this context");
+    const ElecBuffers& E = c->el.E;
+    const int nw = c->B.nw, n = 4 * nw, nr = 3 * nw;
+    if (scal)
+        for (int i = 0; i < 20; ++i) scal[i] = c->el.out[i];
+    if (chge) HIPCHK(hipMemcpyAsync(chge, E.chge, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (vesp) HIPCHK(hipMemcpyAsync(vesp, E.vesp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (e_real) HIPCHK(hipMemcpyAsync(e_real, E.eout, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+    if (w_real) HIPCHK(hipMemcpyAsync(w_real, E.wout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->stream));
+    if (s_real) HIPCHK(hipMemcpyAsync(s_real, E.sout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->stream));
+    if (f_site) HIPCHK(hipMemcpyAsync(f_site, E.fsite, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

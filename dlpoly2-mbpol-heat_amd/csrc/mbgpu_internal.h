@@ -2,6 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
+#include "../../include/mbgpu.h"
+
 #include "mbgpu_dev.h"
 
 // Launch wrappers for the generated polynomial kernels (pip2b.hip / pip3b.hip).
@@ -53,3 +57,85 @@ struct MbBuffers {
     // coefficient streams
     double *cs2 = nullptr, *cs3 = nullptr;
 };
+
+// ------------------------------------------------------------------ electrostatics ----
+
+// Kernel-argument block of the polarizable electrostatics (passed by value).
+struct ElecDev {
+    int nw, n, nreal, nrow;        // waters, sites (4nw), real atoms (3nw), rows carrying dipoles
+    CellGeom g;
+    double rcut, alpha, gamma, q_o;
+    double polr[3], polr2[3];      // O, H, M
+    double athole, athole12, athole13, ascc, ascd;
+    double toler, omega;
+    double drewd;                  // B-function table spacing
+    int compat_qd;                 // qdforce M column as the reference (n_water + imol)
+    int capA, capP;                // neighbour-row capacities (all partners / polarizable partners)
+    int K, nrun, kchunks;          // k vectors, runs of consecutive n, k chunks of the field pass
+    int km[3], kstride;            // kmax per axis, phase-table stride (max kmax + 1)
+    double scale, scal1, rvolm, self_dd;   // 4 rvolm r4pie0, 2 rvolm r4pie0, 2 pi/V, 4 r4pie0 a^3/(3 sqrt(pi))
+};
+
+// Device buffers of the electrostatics (owned by the context).
+struct ElecBuffers {
+    int n = 0, capA = 0, capP = 0, K = 0, nrun = 0, kchunks = 0, schunks = 0, ntab = 0, kstride = 0;
+    double *x = nullptr, *y = nullptr, *z = nullptr;          // [n] site positions
+    double *chge = nullptr, *polr = nullptr, *polr2 = nullptr; // [n]
+    double* dg = nullptr;                                      // [nw][6] DMS derivatives
+    double* tab = nullptr;                                     // [(ntab+1)*4] B-function table
+    int *nbrA = nullptr, *cntA = nullptr;                      // [n][capA] partners within rcut
+    int *nbrP = nullptr, *cntP = nullptr;                      // [n][capP] polarizable partners
+    double* tens = nullptr;                                    // [6][n*capP] dipole tensors
+    double* tin = nullptr;                                     // [nw][6 pairs][6] intramolecular tensors
+    double *dip = nullptr, *efk = nullptr, *emu = nullptr, *efdc = nullptr, *efdd = nullptr;  // [3n]
+    double* state = nullptr;                                   // [8] solver scalars
+    double* kv = nullptr;                                      // [K][8] kx ky kz akk akv akv2 . .
+    int* runs = nullptr;                                       // [nrun][4] l, m, n0, count; k0 in runk
+    int* runk = nullptr;                                       // [nrun] index of the run's first k
+    double *phc = nullptr, *phs = nullptr;                     // [3][kstride][n] cos / sin of l*2pi*s_a
+    double* pht = nullptr;                                     // [n][3][kstride][2] (site-major copy)
+    double *S = nullptr, *Spart = nullptr;                     // [K][8], [schunks][K][8]
+    double* Sq = nullptr;                                      // [K][2] charge structure factors
+    double *kc = nullptr, *kf = nullptr;                       // [K][16] per-k constants of the passes
+    double* zero_state = nullptr;                              // [8] zeros (solver state of final passes)
+    double* Fpart = nullptr;                                   // [kchunks][16][n] site partials
+    double *vk = nullptr, *vr = nullptr, *vs = nullptr, *vc = nullptr, *vdck = nullptr, *vdcr = nullptr,
+           *vesp = nullptr;                                    // [n]
+    double *fk = nullptr, *f2 = nullptr, *f3 = nullptr, *fsite = nullptr;  // [3n]
+    double *e1 = nullptr, *e2 = nullptr, *e3 = nullptr, *ep = nullptr;     // [n]
+    double *s1 = nullptr, *w2 = nullptr, *w3 = nullptr;        // [9n]
+    double *fmsum = nullptr, *fmx = nullptr;                   // [3n], [9n] M-row sums of ewald2p
+    double *eout = nullptr, *wout = nullptr, *sout = nullptr;  // [nreal], [9 nreal] x2: per real atom
+    double* blk = nullptr;                                     // [nblk][16] block partials
+    int nblk = 0;
+    double* scal = nullptr;                                    // [32] reduced scalars
+    int* err = nullptr;                                        // [4] device error flags
+};
+
+int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s);
+int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hipStream_t s);
+int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t s);
+int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, hipStream_t s);
+void el_launch_zero(double* p, long n, hipStream_t s);
+void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
+
+// Host-side state of the electrostatics of one context.
+struct ElecState {
+    ElecBuffers E;
+    ElecDev P{};
+    bool ready = false;                 // the last evaluation completed
+    int nw = 0;
+    double tab_alpha = -1.0, tab_rcut = -1.0;           // key of the B-function table
+    double k_cell[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};     // key of the k-vector list
+    int k_kmax[3] = {0, 0, 0};
+    double k_alpha = -1.0, k_rcut = -1.0;
+    double* rcell = nullptr;            // [9] device copy of the inverse cell
+    double scal[48];                    // host copy of the reduced scalars
+    double out[20];                     // eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe, vir_cpe,
+                                        // vir_ew1p, vir_qd, iterations, rms change, stress[9], pairs
+};
+
+int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, const mbgpu_elec_params* p,
+                   std::string& err);
+int elec_evaluate(ElecState& S, int max_iter, int distribute, hipStream_t s, std::string& err);
+void elec_release(ElecState& S);

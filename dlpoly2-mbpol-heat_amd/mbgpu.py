@@ -48,7 +48,39 @@ _SIGS = {
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp,
                                            _dp, _dp]),
     "mbgpu_time_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "mbgpu_polar_electrostatics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, _dp, _dp,
+                                                  _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "mbgpu_polar_electrostatics_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp, _ip]),
+    "mbgpu_get_elec_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
 }
+
+
+class ElecParams(ctypes.Structure):
+    """mbgpu_elec_params (include/mbgpu.h)."""
+    _fields_ = [("rcut", ctypes.c_double), ("alpha", ctypes.c_double), ("kmax", ctypes.c_int * 3),
+                ("toler", ctypes.c_double), ("omega", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("gamma", ctypes.c_double), ("q_o", ctypes.c_double), ("polr", ctypes.c_double * 3),
+                ("polr2", ctypes.c_double * 3), ("athole", ctypes.c_double), ("athole12", ctypes.c_double),
+                ("athole13", ctypes.c_double), ("ascc", ctypes.c_double), ("ascd", ctypes.c_double),
+                ("compat_qd_column", ctypes.c_int), ("distribute_msite", ctypes.c_int)]
+
+    @classmethod
+    def from_params(cls, p, distribute_msite=1):
+        """From an elec_params.ElecParams (or any object with the same attributes)."""
+        e = cls()
+        e.rcut, e.alpha = p.rcut, p.alpha
+        for i in range(3):
+            e.kmax[i] = p.kmax[i]
+            e.polr[i] = p.polr[i]
+            e.polr2[i] = p.polr2[i]
+        e.toler, e.omega, e.max_iter = p.toler, p.omega, p.max_iter
+        e.gamma, e.q_o = p.gamma, p.q_o
+        e.athole, e.athole12, e.athole13, e.ascc, e.ascd = p.athole, p.athole12, p.athole13, p.ascc, p.ascd
+        e.compat_qd_column = p.compat_qd_column
+        e.distribute_msite = distribute_msite
+        return e
 
 
 class MbgpuError(RuntimeError):
@@ -217,3 +249,54 @@ class Context:
         ms, k3 = ctypes.c_double(), ctypes.c_double()
         _check(load_library().mbgpu_time_forces(self._h, reps, ctypes.byref(ms), ctypes.byref(k3)), "time_forces")
         return ms.value, k3.value
+
+    # ------------------------------------------------------------ electrostatics ----
+    def polar_electrostatics(self, imcon, cell, params, xxx, yyy, zzz, dipx, dipy, dipz, fxx=None, fyy=None,
+                             fzz=None, distribute_msite=1):
+        """Polarizable Ewald electrostatics (forces.f:268-404 and the routines it calls):
+        xxx..zzz (natms >= 4*nw; M entries overwritten), dipx..dipz (in/out) and fxx..fzz
+        (+=) are modified in place.  Returns dict(eng, vir, stress, niter, chge)."""
+        n = 4 * self.nw
+        arrs = [xxx, yyy, zzz, dipx, dipy, dipz] + [a for a in (fxx, fyy, fzz) if a is not None]
+        for a in arrs:
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.size < n:
+                raise ValueError("position, dipole and force arrays must be contiguous float64 of >= 4*nw values")
+        cell = _f64(cell, 9)
+        p = params if isinstance(params, ElecParams) else ElecParams.from_params(params, distribute_msite)
+        chge = np.zeros(n)
+        e, v, it = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
+        stress = np.zeros(9)
+        rc = load_library().mbgpu_polar_electrostatics(self._h, imcon, _p(cell), ctypes.byref(p), _p(xxx), _p(yyy),
+                                                       _p(zzz), _p(dipx), _p(dipy), _p(dipz), _p(chge), _p(fxx),
+                                                       _p(fyy), _p(fzz), ctypes.byref(e), ctypes.byref(v),
+                                                       _p(stress), ctypes.byref(it))
+        _check(rc, "mbgpu_polar_electrostatics")
+        return dict(eng=e.value, vir=v.value, stress=stress, niter=it.value, chge=chge)
+
+    def polar_electrostatics_dev(self, imcon, cell, params, x_ptr, y_ptr, z_ptr, dip_ptr, f_ptr=None,
+                                 distribute_msite=1):
+        """Device-pointer variant: dip_ptr holds 3 interleaved values per site (in/out)."""
+        cell = _f64(cell, 9)
+        p = params if isinstance(params, ElecParams) else ElecParams.from_params(params, distribute_msite)
+        e, v, it = ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int(0)
+        stress = np.zeros(9)
+        rc = load_library().mbgpu_polar_electrostatics_dev(self._h, imcon, _p(cell), ctypes.byref(p), x_ptr, y_ptr,
+                                                           z_ptr, dip_ptr, f_ptr, ctypes.byref(e), ctypes.byref(v),
+                                                           _p(stress), ctypes.byref(it))
+        _check(rc, "mbgpu_polar_electrostatics_dev")
+        return e.value, v.value, stress, it.value
+
+    def elec_details(self):
+        """Results of the last electrostatics call (see mbgpu_get_elec_details)."""
+        n, nr = 4 * self.nw, 3 * self.nw
+        scal = np.zeros(20)
+        chge, vesp = np.zeros(n), np.zeros(n)
+        e, w, s, f = np.zeros(nr), np.zeros(9 * nr), np.zeros(9 * nr), np.zeros(3 * n)
+        _check(load_library().mbgpu_get_elec_details(self._h, _p(scal), _p(chge), _p(vesp), _p(e), _p(w), _p(s),
+                                                     _p(f)), "mbgpu_get_elec_details")
+        names = ["eng_ew1p", "eng_ew2p", "eng_ew3p", "eng_polar", "eng", "vir", "vir_ew1p", "vir_qd", "niter",
+                 "last_delta"]
+        out = {k: scal[i] for i, k in enumerate(names)}
+        out.update(stress=scal[10:19].copy(), chge=chge, vesp=vesp, atomic_energy=e, heat_w=w.reshape(nr, 3, 3),
+                   s_ew1p=s.reshape(nr, 9), f_sites=f.reshape(n, 3))
+        return out

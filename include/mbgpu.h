@@ -113,6 +113,73 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* ctx, const double* vx_dev, const double* vy_d
                         const double* e_kin_dev, const double* e_pot_extra_dev, const double* w_extra_dev,
                         const double* stress_ew1p_dev, double jkin[3], double jpot[3], double jtot[3]);
 
+/* ------------------------------------------------------------------------------------
+ * Polarizable electrostatics (MB-pol / TTM4-F induced dipoles with Ewald sums).
+ * ---------------------------------------------------------------------------------- */
+
+/* Parameters of the polarizable Ewald electrostatics.  The reference reads them from
+ * CONTROL (ewald sum, cut, dipole tolerance, sor_omega), the vmbpol preset
+ * (simdef_cmd.f:1036-1058: gamma, athole, athole12, athole13, ascc, ascd, nthole = 4) and
+ * FIELD (site polarizabilities, oxygen charge). */
+typedef struct {
+    double rcut;              /* real-space cutoff (A) */
+    double alpha;             /* Ewald convergence parameter (1/A) */
+    int kmax[3];              /* Ewald kmax1..3 */
+    double toler;             /* dipole tolerance, e*A (the reference converts from Debye) */
+    double omega;             /* sor_omega: weight of the old dipole in the SOR update */
+    int max_iter;             /* error after this many iterations (the reference: 22581) */
+    double gamma;             /* M-site position parameter (0.426706882) */
+    double q_o;               /* oxygen charge (0) */
+    double polr[3];           /* smearing polarizabilities of O, H, M (A^3) */
+    double polr2[3];          /* induction polarizabilities of O, H, M (A^3; M must be 0) */
+    double athole, athole12, athole13, ascc, ascd;
+    int compat_qd_column;     /* 1: the heat-current force-matrix column of the M-site
+                                 charge-derivative forces is n_water+imol as in
+                                 qdforce.f90:332-337; 0: the M site's own column */
+    int distribute_msite;     /* 1: move the M-site forces onto O, H1, H2 and zero them
+                                 (dlpoly_cmd.f:2583-2597); 0: return them on the M sites */
+} mbgpu_elec_params;
+
+/* Stands in for the polarizable-Ewald part of forces (forces.f:268-404 with rfield,
+ * ewald1p, iter, coulomb_polar; forces.f:585-610 ewald2p; forces.f:700-712 ewald3p;
+ * forces.f:874-892 with qdforce) together with the M-site construction and DMS charges
+ * that precede it (dlpoly_cmd.f:2041-2065, qdetermine.f90), single rank, Verlet lists.
+ * Site layout as ttm2list.f builds it: O, H1, H2 of water w at 3w, 3w+1, 3w+2 (0-based),
+ * M of water w at 3*nw + w; the context must have natms >= 4*nw.
+ *   xxx..zzz: positions; the M-site entries are overwritten with the constructed sites.
+ *   dipx..dipz: induced dipoles (4*nw), start values in (the reference keeps the previous
+ *     step's dipoles), converged dipoles out.
+ *   chge: receives the 4*nw site charges (may be NULL).
+ *   fxx..fzz: += forces (internal units) at the sites, M-site forces moved to O, H1, H2
+ *     when distribute_msite is set.
+ *   engcpe, vircpe, stress: += electrostatic energy, virial and stress.
+ *   niter: receives the number of solver iterations.
+ * Per-atom energies, reciprocal-space stresses and force-matrix contractions stay on the
+ * device and enter mbgpu_heat_flux automatically.  Errors: no convergence within
+ * max_iter (the reference stops, forces.f:351-359), unsupported imcon, capacity. */
+int mbgpu_polar_electrostatics(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_elec_params* p,
+                               double* xxx, double* yyy, double* zzz, double* dipx, double* dipy, double* dipz,
+                               double* chge, double* fxx, double* fyy, double* fzz, double* engcpe,
+                               double* vircpe, double stress[9], int* niter);
+
+/* Same with device pointers: x_dev..z_dev (4*nw, M entries written), dip_dev (3 per site,
+ * interleaved, in/out), f_dev (3 per site, interleaved, +=; may be NULL). */
+int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_elec_params* p,
+                                   double* x_dev, double* y_dev, double* z_dev, double* dip_dev, double* f_dev,
+                                   double* engcpe, double* vircpe, double stress[9], int* niter);
+
+/* Results of the last electrostatics call (host arrays, any may be NULL):
+ *   scal[20]: eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe, vir_cpe, vir_ew1p, vir_qd,
+ *             iterations, last rms dipole change, stress[9] (in scal[10..18]), reserved;
+ *   chge, vesp: 4*nw site charges and electrostatic potentials;
+ *   e_real: per-real-atom energy (ew1p + ew2p + ew3p + polar, M share included);
+ *   w_real: 9 per real atom, force-matrix contraction (M rows included);
+ *   s_real: 9 per real atom, per-atom reciprocal-space stress (M share included);
+ *   f_site: 3 per site, forces of the last call. */
+int mbgpu_get_elec_details(mbgpu_ctx* ctx, double* scal, double* chge, double* vesp, double* e_real,
+                           double* w_real, double* s_real, double* f_site);
+
+
 /* Timing helper: runs the MB-pol force evaluation `reps` times on device-resident inputs
  * already uploaded by a previous mbgpu_mbpol_forces(_dev) call and returns the mean
  * milliseconds per evaluation, measured with HIP events on the library's stream. */
