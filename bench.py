@@ -1,12 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark: MB-pol force evaluations per second at 512 waters on one MI355X.
 
-One step = one evaluation of the MB-pol 2-body + 3-body forces with the per-atom energy /
-heat-flux decomposition and the Green-Kubo heat flux, on the 512-water box (L = 24.84 A,
-8^3 lattice, seeded synthetic configuration), with positions, velocities and per-atom
-kinetic energies resident in HBM before the timed region.  The step includes the
-neighbour / dimer / trimer list builds, the polynomial kernels, the per-atom accumulation
-and the read-back of the energy, virial and stress scalars.
+One step = one full force evaluation of the 512-water box (L = 24.84 A, 8^3 lattice,
+seeded synthetic configuration) with positions, velocities and per-atom kinetic energies
+resident in HBM before the timed region:
+  * MB-pol 2-body + 3-body forces with the per-atom energy / force-matrix decomposition
+    (lists, polynomials, per-atom accumulation);
+  * the polarizable Ewald electrostatics (M sites, DMS charges, induced dipoles solved
+    from zero start values every step with the reference's SOR, omega = 0.3, tolerance
+    1e-12 D; reciprocal space kmax = 16, alpha = 0.3 1/A, rcut = 9 A), with their per-atom
+    energies, reciprocal stresses and force-matrix contractions;
+  * the Green-Kubo heat flux over both.
+The read-back of the energy, virial and stress scalars is inside the step.
 
 Prints one JSON line (see DESIGN.md for the field definitions).
 """
@@ -43,10 +48,19 @@ def cpu_baseline(box):
                            ctypes.cast(ref.mbpol_3b_poly_, ctypes.c_void_p))
         kind = "reference"
     ek = 0.5 * box.mass * (box.vel ** 2).sum(1)
+    ep = importlib.import_module("dlpoly2-mbpol-heat_amd").elec_params.for_box(box.L)
+    nr = box.nreal
     n, t0 = 0, time.perf_counter()
     while True:
         r = oracle_lib.mbpol_forces(box, compat_inplace=1, lists=False)
-        oracle_lib.heat_flux_tensor(box, r["heat_w"], box.vel, ek, r["atomic_energy"])
+        el = oracle_lib.elec_forces(box, ep)
+        hw = r["heat_w"].copy()
+        hw[:nr] += el["heat_w"]
+        e = r["atomic_energy"].copy()
+        e[:nr] += el["atomic_energy"]
+        s = np.zeros((box.natms, 9))
+        s[:nr] = el["s_ew1p"]
+        oracle_lib.heat_flux_tensor(box, hw.reshape(-1), box.vel, ek, e, s.reshape(-1).copy())
         n += 1
         dt = time.perf_counter() - t0
         if dt > 10.0 or n >= 8:
@@ -55,8 +69,9 @@ def cpu_baseline(box):
     what = "reference C++ leaves (oracle/_ref) under the oracle's mbpol_forces loop" if kind == "reference" \
         else "oracle C restatement (extended-precision polynomial sums)"
     return {"value": n / dt, "unit": "force evals/s", "cores": 1, "kind": kind,
-            "sample": f"{n} full 512-water MB-pol 2B+3B force + decomposition + heat-flux evaluations "
-                      f"(all-pairs / all-triples scan as mbpol.f90), {what}, {dt:.1f} s"}
+            "sample": f"{n} full 512-water evaluation(s): MB-pol 2B+3B (all-pairs / all-triples scan as "
+                      f"mbpol.f90, {what}) + the oracle's polarizable Ewald electrostatics (same parameters, "
+                      f"SOR from zero dipoles) + decomposition + heat flux, {dt:.1f} s"}
 
 
 def main():
@@ -65,6 +80,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--solver", choices=["sor", "cg"], default="cg",
+                    help="induced-dipole solver: the reference's SOR or conjugate gradients to the same test")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is "
@@ -79,31 +96,51 @@ def main():
     torch.cuda.set_device(dev)
     box = pkg.waterbox.box_512()
     ctx = pkg.Context(box.nw, box.natms)
+    ep = pkg.elec_params.for_box(box.L)
+    eparams = pkg.ElecParams.from_params(ep, distribute_msite=1, solver=1 if args.solver == "cg" else 0)
     T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
     x, y, z = T(box.x), T(box.y), T(box.z)
     lst = T(box.listttm2, torch.int32)
+    dip = torch.zeros(3 * box.natms, dtype=torch.float64, device=dev)
+    fsite = torch.zeros(3 * box.natms, dtype=torch.float64, device=dev)
     vx, vy, vz = T(box.vel[:box.nreal, 0]), T(box.vel[:box.nreal, 1]), T(box.vel[:box.nreal, 2])
     ek = T(0.5 * box.mass[:box.nreal] * (box.vel[:box.nreal] ** 2).sum(1))
     torch.cuda.synchronize()
 
-    def step():
+    parts = {"mbpol": 0.0, "elec": 0.0, "heat": 0.0}
+    iters = []
+
+    def step(timed=False):
+        t0 = time.perf_counter()
         e, v, s = ctx.mbpol_forces_dev(box.imcon, box.cell, lst.data_ptr(), x.data_ptr(), y.data_ptr(),
                                        z.data_ptr())
+        t1 = time.perf_counter()
+        dip.zero_()
+        ee, ve, se, it = ctx.polar_electrostatics_dev(box.imcon, box.cell, eparams, x.data_ptr(), y.data_ptr(),
+                                                      z.data_ptr(), dip.data_ptr(), fsite.data_ptr())
+        t2 = time.perf_counter()
         jk, jp, jt = ctx.heat_flux_dev(vx.data_ptr(), vy.data_ptr(), vz.data_ptr(), ek.data_ptr())
-        return e, jt
+        t3 = time.perf_counter()
+        if timed:
+            parts["mbpol"] += t1 - t0
+            parts["elec"] += t2 - t1
+            parts["heat"] += t3 - t2
+            iters.append(it)
+        return e + ee, jt
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        step(True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms = 1e3 * dt / args.steps
 
     d, t = ctx.lists()
     nd, nt = len(d), len(t)
+    kvec = int(ctx.elec_details()["k_vectors"])
     _, ms_k3 = ctx.time_forces(max(5, min(args.steps, 20)))
     achieved = nt * FLOP_TRIMER / (ms_k3 * 1e-3) / 1e12
     out = {
@@ -119,9 +156,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded 512-water lattice box, rigid-geometry perturbations, 300 K velocities)",
-        "config": {"workload": "MB-pol 2B+3B forces + per-atom energy/heat-flux decomposition + Green-Kubo "
-                               "heat flux, FP64, 512 H2O L=24.84 A (electrostatics not yet in the step)",
-                   "waters": box.nw, "dimers": nd, "trimers": nt, "parallelism": "single GPU"},
+        "config": {"workload": "MB-pol 2B+3B + polarizable Ewald electrostatics (DMS charges, SOR induced "
+                               "dipoles from zero, kmax 16, rcut 9 A) + per-atom energy/heat-flux decomposition "
+                               "+ Green-Kubo heat flux, FP64, 512 H2O L=24.84 A",
+                   "waters": box.nw, "dimers": nd, "trimers": nt, "k_vectors": kvec, "dipole_iterations":
+                   int(np.median(iters)) if iters else None, "dipole_solver": args.solver, "parallelism": "single GPU",
+                   "ms_mbpol": 1e3 * parts["mbpol"] / args.steps, "ms_electrostatics": 1e3 * parts["elec"] / args.steps,
+                   "ms_heat_flux": 1e3 * parts["heat"] / args.steps},
         "roofline": {"bound": "valu-fp64", "kernel": "k_pip3b<double>", "achieved": achieved, "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
                      "kernel_ms": ms_k3, "flop_per_launch": nt * FLOP_TRIMER},

@@ -79,8 +79,9 @@ void free_buffers(ElecBuffers& E)
 {
     dfree(E.x); dfree(E.y); dfree(E.z); dfree(E.chge); dfree(E.polr); dfree(E.polr2); dfree(E.dg); dfree(E.tab);
     dfree(E.nbrA); dfree(E.cntA); dfree(E.nbrP); dfree(E.cntP); dfree(E.tens); dfree(E.tin);
-    dfree(E.dip); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd); dfree(E.state); dfree(E.zero_state);
-    dfree(E.kv); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs); dfree(E.pht);
+    dfree(E.dip); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd);
+    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.state); dfree(E.zero_state);
+    dfree(E.kv); dfree(E.klmn); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs);
     dfree(E.S); dfree(E.Spart); dfree(E.Sq); dfree(E.kc); dfree(E.kf); dfree(E.Fpart);
     dfree(E.vk); dfree(E.vr); dfree(E.vs); dfree(E.vc); dfree(E.vdck); dfree(E.vdcr); dfree(E.vesp);
     dfree(E.fk); dfree(E.f2); dfree(E.f3); dfree(E.fsite); dfree(E.e1); dfree(E.e2); dfree(E.e3); dfree(E.ep);
@@ -100,6 +101,8 @@ int alloc_sites(ElecState& S, int nw, std::string& err)
     ECHK(dalloc(&E.cntA, n)); ECHK(dalloc(&E.cntP, n));
     ECHK(dalloc(&E.dip, (size_t)3 * n)); ECHK(dalloc(&E.efk, (size_t)3 * n)); ECHK(dalloc(&E.emu, (size_t)3 * n));
     ECHK(dalloc(&E.efdc, (size_t)3 * n)); ECHK(dalloc(&E.efdd, (size_t)3 * n));
+    ECHK(dalloc(&E.cg_p, (size_t)3 * n)); ECHK(dalloc(&E.cg_q, (size_t)3 * n));
+    ECHK(dalloc(&E.cg_r, (size_t)3 * n)); ECHK(dalloc(&E.cg_z, (size_t)3 * n));
     ECHK(dalloc(&E.state, 8)); ECHK(dalloc(&E.zero_state, 8));
     ECHK(dalloc(&E.vk, n)); ECHK(dalloc(&E.vr, n)); ECHK(dalloc(&E.vs, n)); ECHK(dalloc(&E.vc, n));
     ECHK(dalloc(&E.vdck, n)); ECHK(dalloc(&E.vdcr, n)); ECHK(dalloc(&E.vesp, n));
@@ -201,7 +204,11 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     const double ralph = -0.25 / (alpha * alpha);
 
     std::vector<double> kv;
-    std::vector<int> runs, runk;
+    std::vector<int> runs, runk, klmn;
+    if (kmax[0] > 127 || kmax[1] > 127 || kmax[2] > 127) {
+        err = "kmax above 127 is not supported";
+        return -1;
+    }
     int K = 0;
     int mmin = 0, nmin = 1;
     for (int ll = 0; ll <= kmax[0]; ++ll) {
@@ -225,6 +232,7 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
                 const double akv2 = 2.0 * (rrksq - ralph);
                 const double rec[8] = {k0, k1, k2, akk, akv, akv2, 0.0, 0.0};
                 kv.insert(kv.end(), rec, rec + 8);
+                klmn.push_back((ll << 16) | ((mm + 128) << 8) | (nn + 128));
                 if (run_open < 0 || runs[4 * run_open + 3] == 8) {
                     run_open = (int)runk.size();
                     const int r4[4] = {ll, mm, nn, 0};
@@ -239,6 +247,7 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
         mmin = -kmax[1];
     }
     dfree(E.kv);
+    dfree(E.klmn);
     dfree(E.runs);
     dfree(E.runk);
     dfree(E.S);
@@ -252,14 +261,22 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     ECHK(dalloc(&E.runs, runs.size()));
     ECHK(dalloc(&E.runk, runk.size()));
     ECHK(hipMemcpy(E.kv, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice));
+    ECHK(dalloc(&E.klmn, klmn.size()));
+    ECHK(hipMemcpy(E.klmn, klmn.data(), sizeof(int) * klmn.size(), hipMemcpyHostToDevice));
     ECHK(hipMemcpy(E.runs, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice));
     ECHK(hipMemcpy(E.runk, runk.data(), sizeof(int) * runk.size(), hipMemcpyHostToDevice));
-    E.schunks = (E.n + 63) / 64;
+    // site chunks of the structure-factor passes: about 2048 wavefronts (4 per block of 256 k)
+    const int rblocks = (K + 255) / 256;
+    const int nrow = 3 * S.nw;
+    E.schunks = std::max(1, std::min((E.n + 15) / 16, 2048 / (4 * rblocks)));
+    E.spc = (E.n + E.schunks - 1) / E.schunks;
+    E.dchunks = std::max(1, std::min((nrow + 15) / 16, 2048 / (4 * rblocks)));
+    E.dspc = (nrow + E.dchunks - 1) / E.dchunks;
     ECHK(dalloc(&E.S, (size_t)8 * K));
     ECHK(dalloc(&E.Sq, (size_t)2 * K));
     ECHK(dalloc(&E.kc, (size_t)16 * K));
     ECHK(dalloc(&E.kf, (size_t)16 * K));
-    ECHK(dalloc(&E.Spart, (size_t)E.schunks * K * 8));
+    ECHK(dalloc(&E.Spart, (size_t)std::max(E.schunks, E.dchunks) * K * 8));
     ECHK(hipMemset(E.S, 0, sizeof(double) * 8 * K));
     ECHK(hipMemset(E.kc, 0, sizeof(double) * 16 * K));
     // phase tables
@@ -267,14 +284,20 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     if (ks != E.kstride) {
         dfree(E.phc);
         dfree(E.phs);
-        dfree(E.pht);
         E.kstride = ks;
         ECHK(dalloc(&E.phc, (size_t)3 * ks * E.n));
         ECHK(dalloc(&E.phs, (size_t)3 * ks * E.n));
-        ECHK(dalloc(&E.pht, (size_t)E.n * 3 * ks * 2));
     }
-    // k split of the site-parallel passes: about 1024 wavefronts over the sites
-    E.kchunks = std::max(1, std::min(E.nrun, 1024 / std::max(1, (E.n + 63) / 64)));
+    // k split of the site-parallel passes: about 2048 wavefronts over the sites
+    E.kchunks = std::max(1, std::min(E.nrun, 4096 / std::max(1, (E.n + 63) / 64)));
+    E.rpc = (E.nrun + E.kchunks - 1) / E.kchunks;
+    E.kchunks = (E.nrun + E.rpc - 1) / E.rpc;
+    E.kmaxch = 0;
+    for (int c = 0; c < E.kchunks; ++c) {
+        const int r0 = c * E.rpc, r1 = std::min(E.nrun, r0 + E.rpc);
+        const int k0 = runk[r0], k1 = r1 < E.nrun ? runk[r1] : K;
+        E.kmaxch = std::max(E.kmaxch, k1 - k0);
+    }
     dfree(E.Fpart);
     ECHK(dalloc(&E.Fpart, (size_t)E.kchunks * 16 * E.n));
     // block partials of the final passes
@@ -366,6 +389,8 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
     P.nrun = S.E.nrun;
     P.kchunks = S.E.kchunks;
     P.kstride = S.E.kstride;
+    P.rpc = S.E.rpc;
+    P.kmaxch = S.E.kmaxch;
     P.scale = 4.0 * P.rvolm * R4PIE0;
     P.scal1 = 2.0 * P.rvolm * R4PIE0;
     P.self_dd = 4.0 * R4PIE0 * P.alpha * P.alpha * P.alpha / SQRPI / 3.0;
@@ -373,7 +398,7 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
 }
 
 // One evaluation with positions (real atoms) in E.x/y/z and start dipoles in E.dip.
-int elec_evaluate(ElecState& S, int max_iter, int distribute, hipStream_t s, std::string& err)
+int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStream_t s, std::string& err)
 {
     ElecBuffers& E = S.E;
     ElecDev& P = S.P;
@@ -409,10 +434,17 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, hipStream_t s, std
     }
     // solver loop in batches; a converged state turns the rest of a batch into no-ops
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int launched = 0, batch = 12;
+    int launched = 0, batch = solver == 1 ? 8 : 12;
+    if (solver == 1) {
+        rc = el_launch_cg_init(E, P, s);
+        if (rc) {
+            err = std::string("electrostatics solver kernels failed: ") + hipGetErrorString((hipError_t)rc);
+            return rc;
+        }
+    }
     for (;;) {
         for (int b = 0; b < batch; ++b) {
-            rc = el_launch_iteration(E, P, launched == 0, s);
+            rc = solver == 1 ? el_launch_cg_iteration(E, P, s) : el_launch_iteration(E, P, launched == 0, s);
             ++launched;
             if (rc) {
                 err = std::string("electrostatics iteration kernels failed: ") + hipGetErrorString((hipError_t)rc);
@@ -433,10 +465,10 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, hipStream_t s, std
             err = buf;
             return -3;
         }
-        batch = 8;
+        batch = solver == 1 ? 4 : 8;
     }
     const double g34 = exp(gammln_host(0.75)), gln34 = gammln_host(0.75);
-    rc = el_launch_final(E, P, distribute, g34, gln34, s);
+    rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, s);
     if (rc) {
         err = std::string("electrostatics final kernels failed: ") + hipGetErrorString((hipError_t)rc);
         return rc;
@@ -487,7 +519,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, hipStream_t s, std
     o[8] = st[0];
     o[9] = st[1];
     for (int i = 0; i < 9; ++i) o[10 + i] = st9[i];
-    o[19] = 0.0;
+    o[19] = (double)E.K;
     S.ready = true;
     return 0;
 }

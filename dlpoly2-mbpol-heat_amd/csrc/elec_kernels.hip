@@ -425,11 +425,11 @@ __global__ void k_el_intra_prep(ElecDev P, const double* __restrict__ x, const d
 }
 
 // Reciprocal-space phase tables of each site: cos / sin (l * 2 pi * s_a), s = rcell . r,
-// l = 0..kmax_a, by the reference's recurrence (ewald1p.f:196-241, 262-282).  Two layouts:
-// phc/phs [a][l][n] for site-parallel kernels and pht [n][a][l][2] for k-parallel kernels.
+// l = 0..kmax_a, by the reference's recurrence (ewald1p.f:196-241, 262-282), layout
+// phc/phs [a][l][n].
 __global__ void k_el_phase(ElecDev P, const double* __restrict__ x, const double* __restrict__ y,
                            const double* __restrict__ z, const double* __restrict__ rcell,
-                           double* __restrict__ phc, double* __restrict__ phs, double* __restrict__ pht)
+                           double* __restrict__ phc, double* __restrict__ phs)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= P.n) return;
@@ -449,8 +449,6 @@ __global__ void k_el_phase(ElecDev P, const double* __restrict__ x, const double
             }
             phc[((long)a * ks + l) * P.n + j] = c;
             phs[((long)a * ks + l) * P.n + j] = s;
-            pht[(((long)j * 3 + a) * ks + l) * 2] = c;
-            pht[(((long)j * 3 + a) * ks + l) * 2 + 1] = s;
         }
     }
 }
@@ -464,86 +462,117 @@ namespace {
 __device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c, double em_s, double en_c, double en_s,
                                           int m, int n0, double& ckr, double& skr)
 {
-    double clm, slm;
-    if (m >= 0) {
-        clm = el_c * em_c - el_s * em_s;
-        slm = el_s * em_c + em_s * el_c;
-    } else {
-        clm = el_c * em_c + el_s * em_s;
-        slm = el_s * em_c - em_s * el_c;
-    }
-    if (n0 >= 0) {
-        ckr = clm * en_c - slm * en_s;
-        skr = slm * en_c + clm * en_s;
-    } else {
-        ckr = clm * en_c + slm * en_s;
-        skr = slm * en_c - clm * en_s;
-    }
+    // negative m or n use the conjugate table entry; the selects keep the wavefront
+    // convergent and give the same products as the reference's branches
+    const double sm = m >= 0 ? em_s : -em_s;
+    const double sn = n0 >= 0 ? en_s : -en_s;
+    const double clm = el_c * em_c - el_s * sm;
+    const double slm = el_s * em_c + sm * el_c;
+    ckr = clm * en_c - slm * sn;
+    skr = slm * en_c + clm * sn;
 }
 
 }  // namespace
 
-// Structure factors over a chunk of sites, k-parallel (one thread per run of <= EL_RUN
-// consecutive k).  Writes Spart[chunk][k][8] = (sum q ckr, sum q skr, sum mu ckr (3),
-// sum mu skr (3)) with the dipole sums over polarizable sites only (ewald1p.f:342-438).
+// Structure factors over a chunk of sites, k-parallel (ewald1p.f:342-438): one thread
+// per k vector; a block of 256 consecutive k stages the phase tables of its site chunk
+// in LDS, EL_TILE sites at a time (rows l = lmin..lmax of a*, every |m| row of b*, every
+// |n| row of c*), and forms exp(i k.r_j) with the reference's sign rules.  Partials
+// Spart[chunk][k][8] = (sum q ckr, sum q skr, sum mu ckr (3), sum mu skr (3)), the dipole
+// sums over polarizable sites only.  klmn[k] packs l << 16 | (m + 128) << 8 | (n + 128).
+#define EL_TILE 16
+#define EL_TROW (EL_TILE + 1)   // padded LDS row: rows of different n fall on different banks
+__device__ __forceinline__ void unpack_lmn(int v, int& l, int& m, int& n)
+{
+    l = v >> 16;
+    m = ((v >> 8) & 255) - 128;
+    n = (v & 255) - 128;
+}
+
 template <bool CHG>
-__global__ __launch_bounds__(64) void k_el_sfac(ElecDev P, const int* __restrict__ runs,
-                                                const int* __restrict__ runk, const double* __restrict__ pht,
-                                                const double* __restrict__ chge, const double* __restrict__ dip,
-                                                double* __restrict__ Spart, const double* __restrict__ state)
+__global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* __restrict__ klmn,
+                                                 const double* __restrict__ phc, const double* __restrict__ phs,
+                                                 const double* __restrict__ chge, const double* __restrict__ dip,
+                                                 double* __restrict__ Spart, const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
-    const int r = blockIdx.x * 64 + threadIdx.x;
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int kb = blockIdx.x * 256;
+    const int k = kb + tid;
     const int chunk = blockIdx.y;
-    if (r >= P.nrun) return;
-    const int l = runs[4 * r], m = runs[4 * r + 1], n0 = runs[4 * r + 2], cnt = runs[4 * r + 3];
-    const int am = abs(m), an0 = abs(n0);
-    const int j0 = chunk * EL_SCHUNK, j1 = min(P.n, j0 + EL_SCHUNK);
+    const int ns = CHG ? P.n : P.nrow;
+    const int j0 = chunk * spc, j1 = min(ns, j0 + spc);
+    int lmin, lmax, dm, dn;
+    unpack_lmn(klmn[kb], lmin, dm, dn);
+    unpack_lmn(klmn[min(P.K, kb + 256) - 1], lmax, dm, dn);
+    const int nl = lmax - lmin + 1, nm = P.km[1] + 1, nn = P.km[2] + 1;
+    const int nrows = nl + nm + nn;
+    double* tc = lds;
+    double* tsn = lds + nrows * EL_TROW;
+    double* tw = tsn + nrows * EL_TROW;
     const int ks = P.kstride;
-    double acc[EL_RUN][8];
-#pragma unroll
-    for (int t = 0; t < EL_RUN; ++t)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc[t][q] = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        const bool pol = site_polr2(P, j) > 1.e-6;
-        if (!CHG && !pol) continue;
-        const double* pj = pht + (long)j * 3 * ks * 2;
-        const double* pa = pj;
-        const double* pb = pj + ks * 2;
-        const double* pc = pj + 2 * ks * 2;
-        double ckr, skr;
-        run_phase(pa[2 * l], pa[2 * l + 1], pb[2 * am], pb[2 * am + 1], pc[2 * an0], pc[2 * an0 + 1], m, n0, ckr, skr);
-        const double c1 = pc[2], s1 = pc[3];
-        const double q = chge[j];
-        const double mx = dip[3 * j], my = dip[3 * j + 1], mz = dip[3 * j + 2];
-#pragma unroll
-        for (int t = 0; t < EL_RUN; ++t) {
-            if (t < cnt) {
+    int l = lmin, m = 0, n = 1;
+    if (k < P.K) unpack_lmn(klmn[k], l, m, n);
+    const int rowA = (l - lmin) * EL_TROW, rowB = (nl + abs(m)) * EL_TROW, rowC = (nl + nm + abs(n)) * EL_TROW;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t0 = j0; t0 < j1; t0 += EL_TILE) {
+        const int tn = min(EL_TILE, j1 - t0);
+        __syncthreads();
+        for (int idx = tid; idx < nrows * EL_TILE; idx += 256) {
+            const int row = idx / EL_TILE, sidx = idx - row * EL_TILE;
+            if (sidx < tn) {
+                int a, ll;
+                if (row < nl) {
+                    a = 0;
+                    ll = lmin + row;
+                } else if (row < nl + nm) {
+                    a = 1;
+                    ll = row - nl;
+                } else {
+                    a = 2;
+                    ll = row - nl - nm;
+                }
+                const long o = ((long)a * ks + ll) * P.n + t0 + sidx;
+                tc[row * EL_TROW + sidx] = phc[o];
+                tsn[row * EL_TROW + sidx] = phs[o];
+            }
+        }
+        if (tid < tn) {
+            const int j = t0 + tid;
+            tw[4 * tid] = chge[j];
+            tw[4 * tid + 1] = dip[3 * j];
+            tw[4 * tid + 2] = dip[3 * j + 1];
+            tw[4 * tid + 3] = dip[3 * j + 2];
+        }
+        __syncthreads();
+        if (k < P.K) {
+            for (int sj = 0; sj < tn; ++sj) {
+                const bool pol = site_polr2(P, t0 + sj) > 1.e-6;
+                if (!CHG && !pol) continue;
+                double ckr, skr;
+                run_phase(tc[rowA + sj], tsn[rowA + sj], tc[rowB + sj], tsn[rowB + sj], tc[rowC + sj],
+                          tsn[rowC + sj], m, n, ckr, skr);
                 if (CHG) {
-                    acc[t][0] += q * ckr;
-                    acc[t][1] += q * skr;
+                    const double q = tw[4 * sj];
+                    acc[0] += q * ckr;
+                    acc[1] += q * skr;
                 }
                 if (pol) {
-                    acc[t][2] += ckr * mx;
-                    acc[t][3] += ckr * my;
-                    acc[t][4] += ckr * mz;
-                    acc[t][5] += skr * mx;
-                    acc[t][6] += skr * my;
-                    acc[t][7] += skr * mz;
+                    const double mx = tw[4 * sj + 1], my = tw[4 * sj + 2], mz = tw[4 * sj + 3];
+                    acc[2] += ckr * mx;
+                    acc[3] += ckr * my;
+                    acc[4] += ckr * mz;
+                    acc[5] += skr * mx;
+                    acc[6] += skr * my;
+                    acc[7] += skr * mz;
                 }
-                const double cn = ckr * c1 - skr * s1;
-                skr = skr * c1 + ckr * s1;
-                ckr = cn;
             }
         }
     }
-    const int k0 = runk[r];
+    if (k >= P.K) return;
 #pragma unroll
-    for (int t = 0; t < EL_RUN; ++t)
-        if (t < cnt)
-#pragma unroll
-            for (int q = 0; q < 8; ++q) Spart[((long)chunk * P.K + k0 + t) * 8 + q] = acc[t][q];
+    for (int q = 0; q < 8; ++q) Spart[((long)chunk * P.K + k) * 8 + q] = acc[q];
 }
 
 // Sum the chunk partials in chunk order and derive the per-k constants of the field
@@ -588,76 +617,122 @@ __global__ void k_el_sreduce(ElecDev P, int schunks, const double* __restrict__ 
     }
 }
 
-// Reciprocal-space fields over a chunk of runs, site-parallel: efdd_a += c_a skr + d_a ckr
-// (dipole field) and with CHG also efdc_a += e_a skr + f_a ckr (charge field).  Partials
-// Fpart[chunk][6][n].
+// Reciprocal-space fields over a chunk of runs, site-parallel (ewald1p.f:749-769):
+// efdd_a += c_a skr + d_a ckr (dipole field), with CHG also efdc_a += e_a skr + f_a ckr
+// (charge field).  A block of 256 sites stages the chunk's runs and per-k constants in
+// LDS.  Partials Fpart[chunk][6][n].
 template <bool CHG>
-__global__ __launch_bounds__(64) void k_el_kfield(ElecDev P, const int* __restrict__ runs,
-                                                  const int* __restrict__ runk, const double* __restrict__ phc,
-                                                  const double* __restrict__ phs, const double* __restrict__ kc,
-                                                  double* __restrict__ Fpart, const double* __restrict__ state)
+__global__ __launch_bounds__(256) void k_el_kfield(ElecDev P, const int* __restrict__ runs,
+                                                   const int* __restrict__ runk, const double* __restrict__ phc,
+                                                   const double* __restrict__ phs, const double* __restrict__ kc,
+                                                   double* __restrict__ Fpart, const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    extern __shared__ double lds[];
+    constexpr int NC = CHG ? 12 : 6;
+    const int tid = threadIdx.x;
     const int chunk = blockIdx.y;
-    const int rpc = (P.nrun + P.kchunks - 1) / P.kchunks;
-    const int r0 = chunk * rpc, r1 = min(P.nrun, r0 + rpc);
+    const int r0 = chunk * P.rpc, r1 = min(P.nrun, r0 + P.rpc);
+    const int nr = max(0, r1 - r0);
+    double* sk = lds;
+    int* sr = reinterpret_cast<int*>(lds + (long)P.kmaxch * 12);
+    const int kbase = nr ? runk[r0] : 0;
+    const int nk = nr ? ((r1 < P.nrun ? runk[r1] : P.K) - kbase) : 0;
+    for (int idx = tid; idx < nk * NC; idx += 256) {
+        const int kk = idx / NC, q = idx - kk * NC;
+        sk[idx] = kc[16L * (kbase + kk) + q];
+    }
+    for (int idx = tid; idx < nr * 4; idx += 256) sr[idx] = runs[4 * r0 + idx];
+    __syncthreads();
+    // two sites per thread share every constant read: j and j + 256 of a 512-site block
     const int nrow = CHG ? P.n : P.nrow;
-    if (j >= nrow) return;
+    const int ja = blockIdx.x * 512 + tid, jb = ja + 256;
+    if (ja >= nrow) return;
+    const bool hb = jb < nrow;
+    const int jB = hb ? jb : ja;
     const int ks = P.kstride;
-    double fd[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
-    const double c1 = phc[((long)2 * ks + 1) * P.n + j], s1 = phs[((long)2 * ks + 1) * P.n + j];
-    for (int r = r0; r < r1; ++r) {
-        const int l = runs[4 * r], m = runs[4 * r + 1], n0 = runs[4 * r + 2], cnt = runs[4 * r + 3];
-        const int am = abs(m), an0 = abs(n0);
-        double ckr, skr;
-        run_phase(phc[(long)l * P.n + j], phs[(long)l * P.n + j], phc[((long)ks + am) * P.n + j],
-                  phs[((long)ks + am) * P.n + j], phc[((long)2 * ks + an0) * P.n + j],
-                  phs[((long)2 * ks + an0) * P.n + j], m, n0, ckr, skr);
-        const int k0 = runk[r];
-        for (int t = 0; t < cnt; ++t) {
-            const double* c = kc + 16 * (k0 + t);
-            fd[0] += c[0] * skr + c[3] * ckr;
-            fd[1] += c[1] * skr + c[4] * ckr;
-            fd[2] += c[2] * skr + c[5] * ckr;
-            if (CHG) {
-                fc[0] += c[6] * skr + c[9] * ckr;
-                fc[1] += c[7] * skr + c[10] * ckr;
-                fc[2] += c[8] * skr + c[11] * ckr;
+    const long o1 = ((long)2 * ks + 1) * P.n;
+    const double ca1 = phc[o1 + ja], sa1 = phs[o1 + ja], cb1 = phc[o1 + jB], sb1 = phs[o1 + jB];
+    double fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0}, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+    int kk = 0;
+    for (int r = 0; r < nr; ++r) {
+        const int l = sr[4 * r], m = sr[4 * r + 1], n0 = sr[4 * r + 2], cnt = sr[4 * r + 3];
+        const long oa = (long)l * P.n, ob = ((long)ks + abs(m)) * P.n, oc = ((long)2 * ks + abs(n0)) * P.n;
+        double cA, sA, cB, sB;
+        run_phase(phc[oa + ja], phs[oa + ja], phc[ob + ja], phs[ob + ja], phc[oc + ja], phs[oc + ja], m, n0, cA, sA);
+        run_phase(phc[oa + jB], phs[oa + jB], phc[ob + jB], phs[ob + jB], phc[oc + jB], phs[oc + jB], m, n0, cB, sB);
+#pragma unroll
+        for (int t = 0; t < EL_RUN; ++t) {
+            if (t < cnt) {
+                const double* c = sk + NC * (kk + t);
+                const double c0 = c[0], c1v = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5];
+                fa[0] += c0 * sA + c3 * cA;
+                fa[1] += c1v * sA + c4 * cA;
+                fa[2] += c2 * sA + c5 * cA;
+                fb[0] += c0 * sB + c3 * cB;
+                fb[1] += c1v * sB + c4 * cB;
+                fb[2] += c2 * sB + c5 * cB;
+                if (CHG) {
+                    const double c6 = c[6], c7 = c[7], c8 = c[8], c9 = c[9], c10 = c[10], c11 = c[11];
+                    ga[0] += c6 * sA + c9 * cA;
+                    ga[1] += c7 * sA + c10 * cA;
+                    ga[2] += c8 * sA + c11 * cA;
+                    gb[0] += c6 * sB + c9 * cB;
+                    gb[1] += c7 * sB + c10 * cB;
+                    gb[2] += c8 * sB + c11 * cB;
+                }
+                const double na = cA * ca1 - sA * sa1;
+                sA = sA * ca1 + cA * sa1;
+                cA = na;
+                const double nb = cB * cb1 - sB * sb1;
+                sB = sB * cb1 + cB * sb1;
+                cB = nb;
             }
-            const double cn = ckr * c1 - skr * s1;
-            skr = skr * c1 + ckr * s1;
-            ckr = cn;
         }
+        kk += cnt;
     }
     double* out = Fpart + (long)chunk * 6 * P.n;
-    for (int q = 0; q < 3; ++q) out[(long)q * P.n + j] = fd[q];
-    if (CHG)
-        for (int q = 0; q < 3; ++q) out[(long)(3 + q) * P.n + j] = fc[q];
+    for (int q = 0; q < 3; ++q) {
+        out[(long)q * P.n + ja] = fa[q];
+        if (hb) out[(long)q * P.n + jb] = fb[q];
+        if (CHG) {
+            out[(long)(3 + q) * P.n + ja] = ga[q];
+            if (hb) out[(long)(3 + q) * P.n + jb] = gb[q];
+        }
+    }
 }
 
 // Sum the field partials in chunk order and scale (ewald1p.f:941-976): efdd =
-// scale * sum + self term; with CHG also efdc = scale * sum (iloop = 1).
+// scale * sum + self term; with CHG also efdc = scale * sum (iloop = 1).  One thread per
+// (component, site).
 template <bool CHG>
-__global__ void k_el_kfield_reduce(ElecDev P, const double* __restrict__ Fpart, const double* __restrict__ dip,
-                                   double* __restrict__ efdd, double* __restrict__ efdc, const double* __restrict__ state)
+__global__ __launch_bounds__(256) void k_el_kfield_reduce(ElecDev P, const double* __restrict__ Fpart,
+                                                          const double* __restrict__ dip, double* __restrict__ efdd,
+                                                          double* __restrict__ efdc, const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int nrow = CHG ? P.n : P.nrow;
-    if (j >= nrow) return;
-    double fd[3] = {0, 0, 0}, fc[3] = {0, 0, 0};
-    for (int c = 0; c < P.kchunks; ++c) {
-        const double* in = Fpart + (long)c * 6 * P.n;
-        for (int q = 0; q < 3; ++q) {
-            fd[q] += in[(long)q * P.n + j];
-            if (CHG) fc[q] += in[(long)(3 + q) * P.n + j];
-        }
+    const int nq = CHG ? 6 : 3;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * nrow) return;
+    const int q = t / nrow, j = t - q * nrow;
+    const double* in = Fpart + (long)q * P.n + j;
+    const long cs = 6L * P.n;
+    double s = 0.0;
+    int c = 0;
+    for (; c + 8 <= P.kchunks; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = in[(c + u) * cs];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
     }
-    const bool pol = site_polr2(P, j) > 1.e-6;
-    for (int q = 0; q < 3; ++q) {
-        efdd[3 * j + q] = (pol ? P.scale * fd[q] : 0.0) + P.self_dd * dip[3 * j + q];
-        if (CHG) efdc[3 * j + q] = P.scale * fc[q];
+    for (; c < P.kchunks; ++c) s += in[c * cs];
+    if (q < 3) {
+        const bool pol = site_polr2(P, j) > 1.e-6;
+        efdd[3 * j + q] = (pol ? P.scale * s : 0.0) + P.self_dd * dip[3 * j + q];
+    } else {
+        efdc[3 * j + q - 3] = P.scale * s;
     }
 }
 
@@ -759,6 +834,136 @@ __global__ __launch_bounds__(64) void k_el_converge(ElecDev P, int nb, const dou
     }
 }
 
+
+// ==================================================== conjugate-gradient solver ======
+// The induced dipoles solve A mu = E0 with A = r4pie0 / polr2 - T (T: the dipole-field
+// operator, emu + efdd; E0 = efk + efdc).  A is symmetric positive definite for Thole-
+// damped dipoles, so preconditioned conjugate gradients (Jacobi, z = polr2 r / r4pie0)
+// reach the SOR fixed point of iter.f in far fewer iterations.  The stopping test is the
+// reference's: the RMS of the step an SOR update would take from the current dipoles,
+// (1 - omega) * rms(z), against toler (iter.f:95-101).  state = [iterations, rms step,
+// done, r.z, alpha, beta].
+
+// r = E0 + T mu0 - r4pie0 mu0 / polr2, z = polr2 r / r4pie0, p = z; partials (r.z, |z|^2, ndip)
+__global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __restrict__ efk,
+                                                 const double* __restrict__ efdc, const double* __restrict__ emu,
+                                                 const double* __restrict__ efdd, const double* __restrict__ dip,
+                                                 double* __restrict__ r, double* __restrict__ z,
+                                                 double* __restrict__ pv, double* __restrict__ blk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < P.nrow) {
+        const double p2 = site_polr2(P, i);
+        const bool on = p2 > P.toler;
+        for (int q = 0; q < 3; ++q) {
+            const int o = 3 * i + q;
+            const double rr = on ? (efk[o] + efdc[o] + emu[o] + efdd[o]) - R4PIE0 * dip[o] / p2 : 0.0;
+            const double zz = on ? p2 * rr / R4PIE0 : 0.0;
+            r[o] = rr;
+            z[o] = zz;
+            pv[o] = zz;
+            v[0] += rr * zz;
+            v[1] += zz * zz;
+        }
+        v[2] = on ? 1.0 : 0.0;
+    }
+    block_sum_store(v, blk + 3 * blockIdx.x);
+}
+
+// q = A p (from T p in emu / efdd); partials of p.q
+__global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __restrict__ pv,
+                                                  const double* __restrict__ tp1, const double* __restrict__ tp2,
+                                                  double* __restrict__ qv, double* __restrict__ blk,
+                                                  const double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < P.nrow) {
+        const double p2 = site_polr2(P, i);
+        const bool on = p2 > P.toler;
+        for (int q = 0; q < 3; ++q) {
+            const int o = 3 * i + q;
+            const double qq = on ? R4PIE0 * pv[o] / p2 - (tp1[o] + tp2[o]) : 0.0;
+            qv[o] = qq;
+            v[0] += pv[o] * qq;
+        }
+    }
+    block_sum_store(v, blk + blockIdx.x);
+}
+
+// mu += alpha p, r -= alpha q, z = polr2 r / r4pie0; partials (r.z, |z|^2)
+__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, const double* __restrict__ pv,
+                                                   const double* __restrict__ qv, double* __restrict__ dip,
+                                                   double* __restrict__ r, double* __restrict__ z,
+                                                   double* __restrict__ blk, const double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double alpha = state[4];
+    double v[2] = {0.0, 0.0};
+    if (i < P.nrow) {
+        const double p2 = site_polr2(P, i);
+        if (p2 > P.toler) {
+            for (int q = 0; q < 3; ++q) {
+                const int o = 3 * i + q;
+                dip[o] += alpha * pv[o];
+                const double rr = r[o] - alpha * qv[o];
+                const double zz = p2 * rr / R4PIE0;
+                r[o] = rr;
+                z[o] = zz;
+                v[0] += rr * zz;
+                v[1] += zz * zz;
+            }
+        }
+    }
+    block_sum_store(v, blk + 2 * blockIdx.x);
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(256) void k_cg_direction(ElecDev P, const double* __restrict__ z,
+                                                      double* __restrict__ pv, const double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < 3 * P.nrow) pv[o] = z[o] + state[5] * pv[o];
+}
+
+// Scalar steps of the solver from block partials (single wavefront, fixed order).
+//   mode 0 (after k_cg_init):   r.z, convergence test, ndip -> state[6]
+//   mode 1 (after k_cg_apply):  alpha = r.z / p.q
+//   mode 2 (after k_cg_update): beta = r.z(new) / r.z, iteration count, convergence test
+__global__ __launch_bounds__(64) void k_cg_scalar(ElecDev P, int mode, int nb, const double* __restrict__ blk,
+                                                  double* __restrict__ state)
+{
+    if (mode != 0 && state[2] != 0.0) return;
+    const int stride = mode == 0 ? 3 : (mode == 1 ? 1 : 2);
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 64) {
+        a += blk[stride * k];
+        if (stride > 1) b += blk[stride * k + 1];
+        if (stride > 2) c += blk[stride * k + 2];
+    }
+    a = wsum(a);
+    b = wsum(b);
+    c = wsum(c);
+    if (threadIdx.x != 0) return;
+    if (mode == 1) {
+        state[4] = state[3] / a;
+        return;
+    }
+    if (mode == 0) state[6] = c == 0.0 ? 1.0 : c;
+    const double rms = (1.0 - P.omega) * sqrt(b / state[6]);
+    if (mode == 2) {
+        state[5] = a / state[3];
+        state[0] += 1.0;
+    }
+    state[3] = a;
+    state[1] = rms;
+    if (rms <= P.toler) state[2] = 1.0;
+}
+
 // ================================================================ final passes ======
 
 // Polarization energy per site (coulomb_polar.f:58-103).
@@ -832,18 +1037,38 @@ __global__ __launch_bounds__(256) void k_el_kscal(ElecDev P, const double* __res
 
 // Final reciprocal pass per site over a chunk of runs (ewald1p.f:550-693, 771-825):
 // forces, ESP parts vesp_k / vesp_dc_k, per-atom energy and per-atom stress (scaled by
-// akk, without the final prefactors).  Partials Fpart[chunk][16][n]:
-// f(3), vk, vdck, e, s(9).
-__global__ __launch_bounds__(64) void k_el_kfinal(ElecDev P, const int* __restrict__ runs,
-                                                  const int* __restrict__ runk, const double* __restrict__ phc,
-                                                  const double* __restrict__ phs, const double* __restrict__ kv,
-                                                  const double* __restrict__ kf, const double* __restrict__ chge,
-                                                  const double* __restrict__ dip, double* __restrict__ Fpart)
+// akk, without the final prefactors).  The chunk's runs and per-k constants (k, akk,
+// akv2, Qc, Qs, summu, k.Mc, k.Ms, Mc, Ms) are staged in LDS.  Partials
+// Fpart[chunk][16][n]: f(3), vk, vdck, e, s(9).
+#define EL_KF 17
+__global__ __launch_bounds__(256) void k_el_kfinal(ElecDev P, const int* __restrict__ runs,
+                                                   const int* __restrict__ runk, const double* __restrict__ phc,
+                                                   const double* __restrict__ phs, const double* __restrict__ kv,
+                                                   const double* __restrict__ kf, const double* __restrict__ chge,
+                                                   const double* __restrict__ dip, double* __restrict__ Fpart)
 {
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * 256 + tid;
     const int chunk = blockIdx.y;
-    const int rpc = (P.nrun + P.kchunks - 1) / P.kchunks;
-    const int r0 = chunk * rpc, r1 = min(P.nrun, r0 + rpc);
+    const int r0 = chunk * P.rpc, r1 = min(P.nrun, r0 + P.rpc);
+    const int nr = max(0, r1 - r0);
+    double* sk = lds;
+    int* sr = reinterpret_cast<int*>(lds + (long)P.kmaxch * EL_KF);
+    const int kbase = nr ? runk[r0] : 0;
+    const int nk = nr ? ((r1 < P.nrun ? runk[r1] : P.K) - kbase) : 0;
+    for (int idx = tid; idx < nk * EL_KF; idx += 256) {
+        const int kk = idx / EL_KF, q = idx - kk * EL_KF;
+        const long k = kbase + kk;
+        double v;
+        if (q < 4) v = kv[8 * k + q];                 // kx ky kz akk
+        else if (q == 4) v = kv[8 * k + 5];           // akv2
+        else if (q < 11) v = kf[16 * k + (q - 5)];    // Qc Qs summu1 summu2 k.Mc k.Ms
+        else v = kf[16 * k + 8 + (q - 11)];           // Mc(3) Ms(3)
+        sk[idx] = v;
+    }
+    for (int idx = tid; idx < nr * 4; idx += 256) sr[idx] = runs[4 * r0 + idx];
+    __syncthreads();
     if (j >= P.n) return;
     const int ks = P.kstride;
     const bool pol = site_polr2(P, j) > 1.e-6;
@@ -853,36 +1078,34 @@ __global__ __launch_bounds__(64) void k_el_kfinal(ElecDev P, const int* __restri
 #pragma unroll
     for (int t = 0; t < 15; ++t) acc[t] = 0.0;
     const double c1 = phc[((long)2 * ks + 1) * P.n + j], s1 = phs[((long)2 * ks + 1) * P.n + j];
-    for (int r = r0; r < r1; ++r) {
-        const int l = runs[4 * r], m = runs[4 * r + 1], n0 = runs[4 * r + 2], cnt = runs[4 * r + 3];
+    int kk = 0;
+    for (int r = 0; r < nr; ++r) {
+        const int l = sr[4 * r], m = sr[4 * r + 1], n0 = sr[4 * r + 2], cnt = sr[4 * r + 3];
         const int am = abs(m), an0 = abs(n0);
         double ckr, skr;
         run_phase(phc[(long)l * P.n + j], phs[(long)l * P.n + j], phc[((long)ks + am) * P.n + j],
                   phs[((long)ks + am) * P.n + j], phc[((long)2 * ks + an0) * P.n + j],
                   phs[((long)2 * ks + an0) * P.n + j], m, n0, ckr, skr);
-        const int k0 = runk[r];
         for (int t = 0; t < cnt; ++t) {
-            const int k = k0 + t;
-            const double kk[3] = {kv[8 * k], kv[8 * k + 1], kv[8 * k + 2]};
-            const double akk = kv[8 * k + 3], akv2 = kv[8 * k + 5];
-            const double* f = kf + 16 * k;
-            const double Qc = f[0], Qs = f[1], summu1 = f[2], summu2 = f[3];
-            const double skdmuckrs = f[4], skdmuskrs = f[5];
-            const double* Mc = f + 8;
-            const double* Ms = f + 11;
-            const double djmuk = pol ? (mu[0] * kk[0] + mu[1] * kk[1] + mu[2] * kk[2]) : 0.0;
+            const double* f = sk + EL_KF * (kk + t);
+            const double kk3[3] = {f[0], f[1], f[2]};
+            const double akk = f[3], akv2 = f[4];
+            const double Qc = f[5], Qs = f[6], summu1 = f[7], summu2 = f[8];
+            const double dpkMc = f[9], dpkMs = f[10];
+            const double* Mc = f + 11;
+            const double* Ms = f + 14;
+            const double djmuk = pol ? (mu[0] * kk3[0] + mu[1] * kk3[1] + mu[2] * kk3[2]) : 0.0;
             const double ssmu = summu1 * (-skr * q - ckr * djmuk) + summu2 * (-skr * djmuk + ckr * q);
-            acc[0] -= kk[0] * akk * ssmu;
-            acc[1] -= kk[1] * akk * ssmu;
-            acc[2] -= kk[2] * akk * ssmu;
+            acc[0] -= kk3[0] * akk * ssmu;
+            acc[1] -= kk3[1] * akk * ssmu;
+            acc[2] -= kk3[2] * akk * ssmu;
             acc[3] += akk * (ckr * Qc + skr * Qs);
-            acc[4] += akk * (skr * skdmuckrs - ckr * skdmuskrs);
+            acc[4] += akk * (skr * dpkMc - ckr * dpkMs);
             const double Cqi = q * ckr, Sqi = q * skr;
             const double CMi[3] = {mu[0] * ckr, mu[1] * ckr, mu[2] * ckr};
             const double SMi[3] = {mu[0] * skr, mu[1] * skr, mu[2] * skr};
-            const double dpkCMi = kk[0] * CMi[0] + kk[1] * CMi[1] + kk[2] * CMi[2];
-            const double dpkSMi = kk[0] * SMi[0] + kk[1] * SMi[1] + kk[2] * SMi[2];
-            const double dpkMc = skdmuckrs, dpkMs = skdmuskrs;
+            const double dpkCMi = kk3[0] * CMi[0] + kk3[1] * CMi[1] + kk3[2] * CMi[2];
+            const double dpkSMi = kk3[0] * SMi[0] + kk3[1] * SMi[1] + kk3[2] * SMi[2];
             const double cc = Cqi * Qc + Sqi * Qs;
             const double cd = Sqi * dpkMc - Cqi * dpkMs - dpkSMi * Qc + dpkCMi * Qs;
             const double dd = dpkCMi * dpkMc + dpkSMi * dpkMs;
@@ -891,18 +1114,19 @@ __global__ __launch_bounds__(64) void k_el_kfinal(ElecDev P, const int* __restri
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
-                    const double B = (a == b ? 1.0 : 0.0) - akv2 * kk[a] * kk[b];
+                    const double B = (a == b ? 1.0 : 0.0) - akv2 * kk3[a] * kk3[b];
                     const double sCC = B * cc;
-                    const double sCD = B * cd + Sqi * kk[a] * Mc[b] - Cqi * kk[a] * Ms[b] - kk[a] * SMi[b] * Qc +
-                                       kk[a] * CMi[b] * Qs;
-                    const double sDD = B * dd + kk[a] * CMi[b] * dpkMc + kk[a] * SMi[b] * dpkMs +
-                                       dpkCMi * kk[a] * Mc[b] + dpkSMi * kk[a] * Ms[b];
+                    const double sCD = B * cd + Sqi * kk3[a] * Mc[b] - Cqi * kk3[a] * Ms[b] - kk3[a] * SMi[b] * Qc +
+                                       kk3[a] * CMi[b] * Qs;
+                    const double sDD = B * dd + kk3[a] * CMi[b] * dpkMc + kk3[a] * SMi[b] * dpkMs +
+                                       dpkCMi * kk3[a] * Mc[b] + dpkSMi * kk3[a] * Ms[b];
                     acc[6 + 3 * a + b] += akk * (sCC + sCD + sDD);
                 }
             const double cn = ckr * c1 - skr * s1;
             skr = skr * c1 + ckr * s1;
             ckr = cn;
         }
+        kk += cnt;
     }
     double* out = Fpart + (long)chunk * 16 * P.n;
 #pragma unroll
@@ -1390,6 +1614,13 @@ __global__ void k_el_zero(double* __restrict__ p, long n)
 
 namespace {
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+// dynamic LDS of the reciprocal-space passes (bytes)
+inline size_t sfac_lds(const ElecDev& P)
+{
+    return sizeof(double) * (2 * EL_TROW * (size_t)(P.km[0] + P.km[1] + P.km[2] + 3) + 4 * EL_TILE);
+}
+inline size_t kfield_lds(const ElecDev& P) { return sizeof(double) * ((size_t)P.kmaxch * 12 + 2 * (size_t)P.rpc + 2); }
+inline size_t kfinal_lds(const ElecDev& P) { return sizeof(double) * ((size_t)P.kmaxch * EL_KF + 2 * (size_t)P.rpc + 2); }
 }
 
 int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s)
@@ -1407,16 +1638,16 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hi
     hipLaunchKernelGGL(k_el_prep, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tab, E.nbrA, E.cntA, E.nbrP,
                        E.cntP, E.tens, E.efk);
     hipLaunchKernelGGL(k_el_intra_prep, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tin, E.efk);
-    hipLaunchKernelGGL(k_el_phase, dim3(ceil_div(P.n, 64)), dim3(64), 0, s, P, E.x, E.y, E.z, rcell_dev, E.phc, E.phs,
-                       E.pht);
+    hipLaunchKernelGGL(k_el_phase, dim3(ceil_div(P.n, 64)), dim3(64), 0, s, P, E.x, E.y, E.z, rcell_dev, E.phc,
+                       E.phs);
     // charge structure factors and the iloop = 1 reciprocal fields
-    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div(P.nrun, 64), E.schunks), dim3(64), 0, s, P, E.runs, E.runk,
-                       E.pht, E.chge, E.dip, E.Spart, E.state);
+    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div(P.K, 256), E.schunks), dim3(256), sfac_lds(P), s, P, E.spc,
+                       E.klmn, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
     hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
-    hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 64), P.kchunks), dim3(64), 0, s, P, E.runs, E.runk, E.phc,
-                       E.phs, E.kc, E.Fpart, E.state);
-    hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(P.n, 256)), dim3(256), 0, s, P, E.Fpart, E.dip, E.efdd,
+    hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 512), P.kchunks), dim3(256), kfield_lds(P), s, P, E.runs,
+                       E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
+    hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(6L * P.n, 256)), dim3(256), 0, s, P, E.Fpart, E.dip, E.efdd,
                        E.efdc, E.state);
     return (int)hipGetLastError();
 }
@@ -1426,13 +1657,13 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
     if (!first) {
         // reciprocal dipole field from the current dipoles (the first iteration's came
         // with the charge field)
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.nrun, 64), E.schunks), dim3(64), 0, s, P, E.runs, E.runk,
-                           E.pht, E.chge, E.dip, E.Spart, E.state);
-        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv,
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.K, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+                           E.dspc, E.klmn, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
+        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
                            E.S, E.Sq, E.kc, E.state);
-        hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 64), P.kchunks), dim3(64), 0, s, P, E.runs,
-                           E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
-        hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, E.dip,
+        hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
+                           E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
+        hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, E.dip,
                            E.efdd, E.efdc, E.state);
     }
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
@@ -1442,8 +1673,48 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
     return (int)hipGetLastError();
 }
 
-int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, hipStream_t s)
+// T v into emu / efdd (real-space rows and reciprocal space) for the vector v
+static void launch_tfield(ElecBuffers& E, const ElecDev& P, const double* v, const double* state, hipStream_t s)
 {
+    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.K, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
+                       E.klmn, E.phc, E.phs, E.chge, v, E.Spart, state);
+    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
+                       E.Sq, E.kc, state);
+    hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
+                       E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, state);
+    hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, v, E.efdd,
+                       E.efdc, state);
+    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, v, E.emu, state);
+}
+
+// conjugate-gradient start: residual of the start dipoles (their fields came with the
+// charge fields of the preparation, plus the real-space dipole field here)
+int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
+    const int nb = ceil_div(P.nrow, 256);
+    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(256), 0, s, P, E.efk, E.efdc, E.emu, E.efdd, E.dip, E.cg_r, E.cg_z,
+                       E.cg_p, E.blk);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(64), 0, s, P, 0, nb, E.blk, E.state);
+    return (int)hipGetLastError();
+}
+
+int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s)
+{
+    const int nb = ceil_div(P.nrow, 256);
+    launch_tfield(E, P, E.cg_p, E.state, s);
+    hipLaunchKernelGGL(k_cg_apply, dim3(nb), dim3(256), 0, s, P, E.cg_p, E.emu, E.efdd, E.cg_q, E.blk, E.state);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(64), 0, s, P, 1, nb, E.blk, E.state);
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, P, E.cg_p, E.cg_q, E.dip, E.cg_r, E.cg_z, E.blk, E.state);
+    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(64), 0, s, P, 2, nb, E.blk, E.state);
+    hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.cg_z, E.cg_p, E.state);
+    return (int)hipGetLastError();
+}
+
+int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
+                    hipStream_t s)
+{
+    if (refresh) launch_tfield(E, P, E.dip, E.zero_state, s);   // fields of the final dipoles
     const int nsb = ceil_div(P.n, 256), nwb = ceil_div(P.nw, 64), nkb = ceil_div(P.K, 256);
     double* b_pol = E.blk;                       // [nsb]
     double* b_ks = b_pol + nsb;                  // [nkb][8]
@@ -1452,14 +1723,16 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     double* b_p3 = b_p2 + 8L * P.n;              // [nwb][8]
     double* b_qd = b_p3 + 8L * nwb;              // [nwb][8]
     hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
-    // structure factors of the converged dipoles
-    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.nrun, 64), E.schunks), dim3(64), 0, s, P, E.runs, E.runk,
-                       E.pht, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
-    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(nkb), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
-                       (const double*)E.zero_state);
+    // structure factors of the converged dipoles (already computed with the refreshed fields)
+    if (!refresh) {
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.K, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+                           E.dspc, E.klmn, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
+        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(nkb), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
+                           (const double*)E.zero_state);
+    }
     hipLaunchKernelGGL(k_el_kscal, dim3(nkb), dim3(256), 0, s, P, E.kv, E.S, E.Sq, E.kf, b_ks);
-    hipLaunchKernelGGL(k_el_kfinal, dim3(ceil_div(P.n, 64), P.kchunks), dim3(64), 0, s, P, E.runs, E.runk, E.phc,
-                       E.phs, E.kv, E.kf, E.chge, E.dip, E.Fpart);
+    hipLaunchKernelGGL(k_el_kfinal, dim3(ceil_div(P.n, 256), P.kchunks), dim3(256), kfinal_lds(P), s, P, E.runs,
+                       E.runk, E.phc, E.phs, E.kv, E.kf, E.chge, E.dip, E.Fpart);
     hipLaunchKernelGGL(k_el_kfinal_reduce, dim3(nsb), dim3(256), 0, s, P, E.Fpart, E.chge, E.fk, E.vk, E.vdck, E.vs,
                        E.e1, E.s1, b_q2);
     hipLaunchKernelGGL(k_el_pairs, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.dip, E.tab, E.nbrA, E.cntA,

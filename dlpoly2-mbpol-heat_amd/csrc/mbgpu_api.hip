@@ -516,7 +516,7 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* c, int imcon, const double cell[9]
     HIPCHK(hipMemcpyAsync(E.y, y_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(E.z, z_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(E.dip, dip_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
-    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, c->stream, err);
+    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, p->solver, c->stream, err);
     if (rc) return fail(rc, "%s", err.c_str());
     const int nr = 3 * nw;
     HIPCHK(hipMemcpyAsync(x_dev + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
@@ -561,7 +561,7 @@ int mbgpu_polar_electrostatics(mbgpu_ctx* c, int imcon, const double cell[9], co
     HIPCHK(hipMemcpyAsync(E.y, yyy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(E.z, zzz, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(E.dip, dip.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, c->stream, err);
+    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, p->solver, c->stream, err);
     if (rc) return fail(rc, "%s", err.c_str());
     const int nr = 3 * nw;
     std::vector<double> f((size_t)3 * n);

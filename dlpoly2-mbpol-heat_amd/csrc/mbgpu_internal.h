@@ -71,7 +71,8 @@ struct ElecDev {
     double drewd;                  // B-function table spacing
     int compat_qd;                 // qdforce M column as the reference (n_water + imol)
     int capA, capP;                // neighbour-row capacities (all partners / polarizable partners)
-    int K, nrun, kchunks;          // k vectors, runs of consecutive n, k chunks of the field pass
+    int K, nrun, kchunks;          // k vectors, runs of consecutive n, k chunks of the site-parallel passes
+    int rpc, kmaxch;               // runs per k chunk, most k vectors in one chunk
     int km[3], kstride;            // kmax per axis, phase-table stride (max kmax + 1)
     double scale, scal1, rvolm, self_dd;   // 4 rvolm r4pie0, 2 rvolm r4pie0, 2 pi/V, 4 r4pie0 a^3/(3 sqrt(pi))
 };
@@ -79,6 +80,7 @@ struct ElecDev {
 // Device buffers of the electrostatics (owned by the context).
 struct ElecBuffers {
     int n = 0, capA = 0, capP = 0, K = 0, nrun = 0, kchunks = 0, schunks = 0, ntab = 0, kstride = 0;
+    int spc = 0, dchunks = 0, dspc = 0, rpc = 0, kmaxch = 0;   // site chunks of the structure-factor passes
     double *x = nullptr, *y = nullptr, *z = nullptr;          // [n] site positions
     double *chge = nullptr, *polr = nullptr, *polr2 = nullptr; // [n]
     double* dg = nullptr;                                      // [nw][6] DMS derivatives
@@ -88,12 +90,13 @@ struct ElecBuffers {
     double* tens = nullptr;                                    // [6][n*capP] dipole tensors
     double* tin = nullptr;                                     // [nw][6 pairs][6] intramolecular tensors
     double *dip = nullptr, *efk = nullptr, *emu = nullptr, *efdc = nullptr, *efdd = nullptr;  // [3n]
+    double *cg_p = nullptr, *cg_q = nullptr, *cg_r = nullptr, *cg_z = nullptr;                // [3n] solver vectors
     double* state = nullptr;                                   // [8] solver scalars
     double* kv = nullptr;                                      // [K][8] kx ky kz akk akv akv2 . .
     int* runs = nullptr;                                       // [nrun][4] l, m, n0, count; k0 in runk
     int* runk = nullptr;                                       // [nrun] index of the run's first k
+    int* klmn = nullptr;                                       // [K] l << 16 | (m+128) << 8 | (n+128)
     double *phc = nullptr, *phs = nullptr;                     // [3][kstride][n] cos / sin of l*2pi*s_a
-    double* pht = nullptr;                                     // [n][3][kstride][2] (site-major copy)
     double *S = nullptr, *Spart = nullptr;                     // [K][8], [schunks][K][8]
     double* Sq = nullptr;                                      // [K][2] charge structure factors
     double *kc = nullptr, *kf = nullptr;                       // [K][16] per-k constants of the passes
@@ -115,7 +118,10 @@ struct ElecBuffers {
 int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s);
 int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hipStream_t s);
 int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t s);
-int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, hipStream_t s);
+int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
+                    hipStream_t s);
+int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s);
+int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s);
 void el_launch_zero(double* p, long n, hipStream_t s);
 void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
 
@@ -137,5 +143,5 @@ struct ElecState {
 
 int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, const mbgpu_elec_params* p,
                    std::string& err);
-int elec_evaluate(ElecState& S, int max_iter, int distribute, hipStream_t s, std::string& err);
+int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStream_t s, std::string& err);
 void elec_release(ElecState& S);

@@ -38,6 +38,7 @@ class ElecParams:
     ascc: float = 0.4
     ascd: float = 0.4
     compat_qd_column: int = 1
+    solver: int = 0                          # 0 SOR (iter.f), 1 conjugate gradients (library only)
     extra: dict = field(default_factory=dict)
 
 

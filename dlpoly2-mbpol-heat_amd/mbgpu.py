@@ -64,10 +64,10 @@ class ElecParams(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("q_o", ctypes.c_double), ("polr", ctypes.c_double * 3),
                 ("polr2", ctypes.c_double * 3), ("athole", ctypes.c_double), ("athole12", ctypes.c_double),
                 ("athole13", ctypes.c_double), ("ascc", ctypes.c_double), ("ascd", ctypes.c_double),
-                ("compat_qd_column", ctypes.c_int), ("distribute_msite", ctypes.c_int)]
+                ("compat_qd_column", ctypes.c_int), ("distribute_msite", ctypes.c_int), ("solver", ctypes.c_int)]
 
     @classmethod
-    def from_params(cls, p, distribute_msite=1):
+    def from_params(cls, p, distribute_msite=1, solver=None):
         """From an elec_params.ElecParams (or any object with the same attributes)."""
         e = cls()
         e.rcut, e.alpha = p.rcut, p.alpha
@@ -80,6 +80,7 @@ class ElecParams(ctypes.Structure):
         e.athole, e.athole12, e.athole13, e.ascc, e.ascd = p.athole, p.athole12, p.athole13, p.ascc, p.ascd
         e.compat_qd_column = p.compat_qd_column
         e.distribute_msite = distribute_msite
+        e.solver = getattr(p, "solver", 0) if solver is None else solver
         return e
 
 
@@ -297,6 +298,6 @@ class Context:
         names = ["eng_ew1p", "eng_ew2p", "eng_ew3p", "eng_polar", "eng", "vir", "vir_ew1p", "vir_qd", "niter",
                  "last_delta"]
         out = {k: scal[i] for i, k in enumerate(names)}
-        out.update(stress=scal[10:19].copy(), chge=chge, vesp=vesp, atomic_energy=e, heat_w=w.reshape(nr, 3, 3),
+        out.update(stress=scal[10:19].copy(), k_vectors=scal[19], chge=chge, vesp=vesp, atomic_energy=e, heat_w=w.reshape(nr, 3, 3),
                    s_ew1p=s.reshape(nr, 9), f_sites=f.reshape(n, 3))
         return out

@@ -138,6 +138,11 @@ typedef struct {
                                  qdforce.f90:332-337; 0: the M site's own column */
     int distribute_msite;     /* 1: move the M-site forces onto O, H1, H2 and zero them
                                  (dlpoly_cmd.f:2583-2597); 0: return them on the M sites */
+    int solver;               /* 0: SOR exactly as iter.f (omega, toler as above);
+                                 1: preconditioned conjugate gradients on the same linear
+                                 system, stopped by the same test (the RMS of the step an
+                                 SOR update would take is <= toler); same fixed point,
+                                 fewer iterations.  No counterpart in the reference. */
 } mbgpu_elec_params;
 
 /* Stands in for the polarizable-Ewald part of forces (forces.f:268-404 with rfield,
@@ -170,7 +175,7 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[
 
 /* Results of the last electrostatics call (host arrays, any may be NULL):
  *   scal[20]: eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe, vir_cpe, vir_ew1p, vir_qd,
- *             iterations, last rms dipole change, stress[9] (in scal[10..18]), reserved;
+ *             iterations, last rms dipole change, stress[9] (in scal[10..18]), k vectors;
  *   chge, vesp: 4*nw site charges and electrostatic potentials;
  *   e_real: per-real-atom energy (ew1p + ew2p + ew3p + polar, M share included);
  *   w_real: 9 per real atom, force-matrix contraction (M rows included);

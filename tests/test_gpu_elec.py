@@ -144,3 +144,26 @@ def test_parameter_errors(pkg, waterbox, ep):
     small = pkg.Context(box.nw, 3 * box.nw)
     with pytest.raises(pkg.MbgpuError, match="4\\*nw"):
         small.polar_electrostatics(1, box.cell, ep.ElecParams(), *a)
+
+
+@pytest.mark.parametrize("which", ["small", "box64", "box512"])
+def test_conjugate_gradient_solver_matches_sor_oracle(pkg, waterbox, ep, which, ora512):
+    """The library's conjugate-gradient option reaches the reference's SOR fixed point:
+    every output within 1e-10 of the oracle's SOR result, in fewer iterations."""
+    import dataclasses
+    if which == "small":
+        box = waterbox.make_waterbox(16, 12.42, 4, seed=7)
+        p = ep.ElecParams(rcut=6.2, alpha=0.72, kmax=(16, 16, 16), max_iter=2000)
+        r = o.elec_forces(box, p)
+    elif which == "box64":
+        box = waterbox.box_64()
+        p = ep.for_box(box.L)
+        p.max_iter = 2000
+        r = o.elec_forces(box, p)
+    else:
+        box = waterbox.box_512()
+        p, r = ora512
+    _, g = run_gpu(pkg, box, dataclasses.replace(p, solver=1))
+    assert g["niter"] < r["niter"] / 2
+    g["niter"] = r["niter"]
+    compare(g, r)
