@@ -205,8 +205,8 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
 
     std::vector<double> kv;
     std::vector<int> runs, runk, klmn;
-    if (kmax[0] > 127 || kmax[1] > 127 || kmax[2] > 127) {
-        err = "kmax above 127 is not supported";
+    if (kmax[0] + kmax[1] + kmax[2] + 3 > 80) {
+        err = "kmax1 + kmax2 + kmax3 above 77 is not supported";
         return -1;
     }
     int K = 0;
@@ -265,12 +265,13 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     ECHK(hipMemcpy(E.klmn, klmn.data(), sizeof(int) * klmn.size(), hipMemcpyHostToDevice));
     ECHK(hipMemcpy(E.runs, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice));
     ECHK(hipMemcpy(E.runk, runk.data(), sizeof(int) * runk.size(), hipMemcpyHostToDevice));
-    // site chunks of the structure-factor passes: about 2048 wavefronts (4 per block of 256 k)
-    const int rblocks = (K + 255) / 256;
+    // site chunks of the structure-factor passes: up to 16 (their partials are re-read
+    // by the chunk sum), about 1024 wavefronts (4 per block of 128 runs)
+    const int rblocks = (2 * E.nrun + 255) / 256;
     const int nrow = 3 * S.nw;
-    E.schunks = std::max(1, std::min((E.n + 15) / 16, 2048 / (4 * rblocks)));
+    E.schunks = std::max(1, std::min(16, std::min((E.n + 15) / 16, 1024 / (4 * rblocks))));
     E.spc = (E.n + E.schunks - 1) / E.schunks;
-    E.dchunks = std::max(1, std::min((nrow + 15) / 16, 2048 / (4 * rblocks)));
+    E.dchunks = std::max(1, std::min(16, std::min((nrow + 15) / 16, 1024 / (4 * rblocks))));
     E.dspc = (nrow + E.dchunks - 1) / E.dchunks;
     ECHK(dalloc(&E.S, (size_t)8 * K));
     ECHK(dalloc(&E.Sq, (size_t)2 * K));

@@ -474,54 +474,68 @@ __device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c,
 
 }  // namespace
 
-// Structure factors over a chunk of sites, k-parallel (ewald1p.f:342-438): one thread
-// per k vector; a block of 256 consecutive k stages the phase tables of its site chunk
-// in LDS, EL_TILE sites at a time (rows l = lmin..lmax of a*, every |m| row of b*, every
-// |n| row of c*), and forms exp(i k.r_j) with the reference's sign rules.  Partials
-// Spart[chunk][k][8] = (sum q ckr, sum q skr, sum mu ckr (3), sum mu skr (3)), the dipole
-// sums over polarizable sites only.  klmn[k] packs l << 16 | (m + 128) << 8 | (n + 128).
+// Structure factors over a chunk of sites, k-parallel (ewald1p.f:342-438): each thread
+// owns half a run (up to EL_HALF consecutive k at fixed l, m) and steps exp(i k.r_j)
+// along n by a complex recurrence; a block stages the phase tables of its site chunk in
+// LDS, EL_TILE sites at a time (rows l = lmin..lmax of a*, every |m| row of b*, every |n|
+// row of c*, padded against bank conflicts).  Partials Spart[chunk][k][8] =
+// (sum q ckr, sum q skr, sum mu ckr (3), sum mu skr (3)), the dipole sums over
+// polarizable sites only.
 #define EL_TILE 16
-#define EL_TROW (EL_TILE + 1)   // padded LDS row: rows of different n fall on different banks
-__device__ __forceinline__ void unpack_lmn(int v, int& l, int& m, int& n)
-{
-    l = v >> 16;
-    m = ((v >> 8) & 255) - 128;
-    n = (v & 255) - 128;
-}
-
+#define EL_TROW (EL_TILE + 1)
+#define EL_HALF (EL_RUN / 2)
 template <bool CHG>
-__global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* __restrict__ klmn,
-                                                 const double* __restrict__ phc, const double* __restrict__ phs,
-                                                 const double* __restrict__ chge, const double* __restrict__ dip,
-                                                 double* __restrict__ Spart, const double* __restrict__ state)
+__global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* __restrict__ runs,
+                                                 const int* __restrict__ runk, const double* __restrict__ phc,
+                                                 const double* __restrict__ phs, const double* __restrict__ chge,
+                                                 const double* __restrict__ dip, double* __restrict__ Spart,
+                                                 const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
-    const int kb = blockIdx.x * 256;
-    const int k = kb + tid;
+    const int g = blockIdx.x * 256 + tid;
+    const int r = g >> 1, half = g & 1;
     const int chunk = blockIdx.y;
     const int ns = CHG ? P.n : P.nrow;
     const int j0 = chunk * spc, j1 = min(ns, j0 + spc);
-    int lmin, lmax, dm, dn;
-    unpack_lmn(klmn[kb], lmin, dm, dn);
-    unpack_lmn(klmn[min(P.K, kb + 256) - 1], lmax, dm, dn);
+    const int rfirst = (blockIdx.x * 256) >> 1;
+    const int rlast = min(P.nrun - 1, (blockIdx.x * 256 + 255) >> 1);
+    const int lmin = runs[4 * rfirst], lmax = runs[4 * rlast];
     const int nl = lmax - lmin + 1, nm = P.km[1] + 1, nn = P.km[2] + 1;
     const int nrows = nl + nm + nn;
     double* tc = lds;
     double* tsn = lds + nrows * EL_TROW;
     double* tw = tsn + nrows * EL_TROW;
     const int ks = P.kstride;
-    int l = lmin, m = 0, n = 1;
-    if (k < P.K) unpack_lmn(klmn[k], l, m, n);
-    const int rowA = (l - lmin) * EL_TROW, rowB = (nl + abs(m)) * EL_TROW, rowC = (nl + nm + abs(n)) * EL_TROW;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int t0 = j0; t0 < j1; t0 += EL_TILE) {
+    int l = lmin, m = 0, nst = 1, cnt = 0;
+    if (r < P.nrun) {
+        l = runs[4 * r];
+        m = runs[4 * r + 1];
+        nst = runs[4 * r + 2] + EL_HALF * half;
+        cnt = min(EL_HALF, max(0, runs[4 * r + 3] - EL_HALF * half));
+        if (cnt == 0) nst = runs[4 * r + 2];
+    }
+    const int rowA = (l - lmin) * EL_TROW, rowB = (nl + abs(m)) * EL_TROW, rowC = (nl + nm + abs(nst)) * EL_TROW;
+    const int rowC1 = (nl + nm + 1) * EL_TROW;
+    double acc[EL_HALF][8];
+#pragma unroll
+    for (int t = 0; t < EL_HALF; ++t)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[t][q] = 0.0;
+    // tiles of EL_TILE sites, double-buffered: the next tile's table entries are loaded
+    // into registers while the current tile is consumed from LDS
+    constexpr int PF = 5;                       // staged entries per thread (nrows <= 80)
+    double pc[PF], ps[PF], pw = 0.0;
+    auto fetch = [&](int t0) {
         const int tn = min(EL_TILE, j1 - t0);
-        __syncthreads();
-        for (int idx = tid; idx < nrows * EL_TILE; idx += 256) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int idx = tid + 256 * u;
             const int row = idx / EL_TILE, sidx = idx - row * EL_TILE;
-            if (sidx < tn) {
+            pc[u] = 0.0;
+            ps[u] = 0.0;
+            if (row < nrows && sidx < tn) {
                 int a, ll;
                 if (row < nl) {
                     a = 0;
@@ -534,45 +548,71 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
                     ll = row - nl - nm;
                 }
                 const long o = ((long)a * ks + ll) * P.n + t0 + sidx;
-                tc[row * EL_TROW + sidx] = phc[o];
-                tsn[row * EL_TROW + sidx] = phs[o];
+                pc[u] = phc[o];
+                ps[u] = phs[o];
             }
         }
-        if (tid < tn) {
-            const int j = t0 + tid;
-            tw[4 * tid] = chge[j];
-            tw[4 * tid + 1] = dip[3 * j];
-            tw[4 * tid + 2] = dip[3 * j + 1];
-            tw[4 * tid + 3] = dip[3 * j + 2];
+        if (tid < 4 * EL_TILE) {
+            const int sj = tid >> 2, w = tid & 3;
+            const int j = t0 + sj;
+            pw = sj < tn ? (w == 0 ? chge[j] : dip[3 * j + w - 1]) : 0.0;
         }
+    };
+    if (j0 < j1) fetch(j0);
+    for (int t0 = j0; t0 < j1; t0 += EL_TILE) {
+        const int tn = min(EL_TILE, j1 - t0);
         __syncthreads();
-        if (k < P.K) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int idx = tid + 256 * u;
+            const int row = idx / EL_TILE, sidx = idx - row * EL_TILE;
+            if (row < nrows) {
+                tc[row * EL_TROW + sidx] = pc[u];
+                tsn[row * EL_TROW + sidx] = ps[u];
+            }
+        }
+        if (tid < 4 * EL_TILE) tw[tid] = pw;
+        __syncthreads();
+        if (t0 + EL_TILE < j1) fetch(t0 + EL_TILE);
+        if (cnt > 0) {
             for (int sj = 0; sj < tn; ++sj) {
                 const bool pol = site_polr2(P, t0 + sj) > 1.e-6;
                 if (!CHG && !pol) continue;
                 double ckr, skr;
                 run_phase(tc[rowA + sj], tsn[rowA + sj], tc[rowB + sj], tsn[rowB + sj], tc[rowC + sj],
-                          tsn[rowC + sj], m, n, ckr, skr);
-                if (CHG) {
-                    const double q = tw[4 * sj];
-                    acc[0] += q * ckr;
-                    acc[1] += q * skr;
-                }
-                if (pol) {
-                    const double mx = tw[4 * sj + 1], my = tw[4 * sj + 2], mz = tw[4 * sj + 3];
-                    acc[2] += ckr * mx;
-                    acc[3] += ckr * my;
-                    acc[4] += ckr * mz;
-                    acc[5] += skr * mx;
-                    acc[6] += skr * my;
-                    acc[7] += skr * mz;
+                          tsn[rowC + sj], m, nst, ckr, skr);
+                const double c1 = tc[rowC1 + sj], s1 = tsn[rowC1 + sj];
+                const double q = tw[4 * sj], mx = tw[4 * sj + 1], my = tw[4 * sj + 2], mz = tw[4 * sj + 3];
+#pragma unroll
+                for (int t = 0; t < EL_HALF; ++t) {
+                    if (t < cnt) {
+                        if (CHG) {
+                            acc[t][0] += q * ckr;
+                            acc[t][1] += q * skr;
+                        }
+                        if (pol) {
+                            acc[t][2] += ckr * mx;
+                            acc[t][3] += ckr * my;
+                            acc[t][4] += ckr * mz;
+                            acc[t][5] += skr * mx;
+                            acc[t][6] += skr * my;
+                            acc[t][7] += skr * mz;
+                        }
+                        const double cn = ckr * c1 - skr * s1;
+                        skr = skr * c1 + ckr * s1;
+                        ckr = cn;
+                    }
                 }
             }
         }
     }
-    if (k >= P.K) return;
+    if (cnt == 0) return;
+    const int k0 = runk[r] + EL_HALF * half;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) Spart[((long)chunk * P.K + k) * 8 + q] = acc[q];
+    for (int t = 0; t < EL_HALF; ++t)
+        if (t < cnt)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Spart[((long)chunk * P.K + k0 + t) * 8 + q] = acc[t][q];
 }
 
 // Sum the chunk partials in chunk order and derive the per-k constants of the field
@@ -1641,8 +1681,8 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hi
     hipLaunchKernelGGL(k_el_phase, dim3(ceil_div(P.n, 64)), dim3(64), 0, s, P, E.x, E.y, E.z, rcell_dev, E.phc,
                        E.phs);
     // charge structure factors and the iloop = 1 reciprocal fields
-    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div(P.K, 256), E.schunks), dim3(256), sfac_lds(P), s, P, E.spc,
-                       E.klmn, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
+    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div(2L * P.nrun, 256), E.schunks), dim3(256), sfac_lds(P), s, P, E.spc,
+                       E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
     hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
     hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 512), P.kchunks), dim3(256), kfield_lds(P), s, P, E.runs,
@@ -1657,8 +1697,8 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
     if (!first) {
         // reciprocal dipole field from the current dipoles (the first iteration's came
         // with the charge field)
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.K, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
-                           E.dspc, E.klmn, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
         hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
                            E.S, E.Sq, E.kc, E.state);
         hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
@@ -1676,8 +1716,8 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
 // T v into emu / efdd (real-space rows and reciprocal space) for the vector v
 static void launch_tfield(ElecBuffers& E, const ElecDev& P, const double* v, const double* state, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.K, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
-                       E.klmn, E.phc, E.phs, E.chge, v, E.Spart, state);
+    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
+                       E.runs, E.runk, E.phc, E.phs, E.chge, v, E.Spart, state);
     hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, state);
     hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
@@ -1725,8 +1765,8 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
     // structure factors of the converged dipoles (already computed with the refreshed fields)
     if (!refresh) {
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(P.K, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
-                           E.dspc, E.klmn, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
         hipLaunchKernelGGL(k_el_sreduce<false>, dim3(nkb), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
                            (const double*)E.zero_state);
     }
