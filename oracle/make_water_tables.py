@@ -10,7 +10,8 @@ the reference is compiled or run) and writes them as plain number tables under
                  (qdetermine.f90:153-181);
   ps_table.txt   245 rows "i1 i2 i3 c5z cbasis ccore crest": the Partridge-Schwenke
                  potential expansion of module ps_type_pes (module.f90:65-455), used by
-                 ps_type_couple (ps_type_couple.f90:15-274).
+                 ps_type_couple (ps_type_couple.f90:15-274);
+  ps_mass.txt    9 rows "i1 i2 i3 cmass0": its mass-correction terms (module.f90:456-463).
 
 The numbers are the model itself, so they must equal the reference's; only the file format
 is this package's.  Usage:  python oracle/make_water_tables.py [/root/reference/SOURCE]
@@ -31,7 +32,7 @@ def module_text(src, name):
 
 
 def array(text, name, n, kind):
-    m = re.search(r"::\s*%s\(1:%d\)\s*=\s*\(/(.*?)/\)" % (name, n), text, re.S | re.I)
+    m = re.search(r"::\s*%s\((?:1:)?%d\)\s*=\s*\(/(.*?)/\)" % (name, n), text, re.S | re.I)
     if not m:
         raise SystemExit(f"array {name}(1:{n}) not found")
     body = re.sub(r"!.*", "", m.group(1)).replace("&", " ")
@@ -61,7 +62,12 @@ def main():
     with open(os.path.join(DATA, "ps_table.txt"), "w") as f:
         for row in zip(j1, j2, j3, *cols):
             f.write("%d %d %d %r %r %r %r\n" % row)
-    print("wrote", os.path.join(DATA, "dms_table.txt"), "and ps_table.txt")
+    m1, m2, m3 = (array(pes, f"idxm{k}", 9, "i") for k in (1, 2, 3))
+    cm = array(pes, "cmass0", 9, "f")
+    with open(os.path.join(DATA, "ps_mass.txt"), "w") as f:
+        for row in zip(m1, m2, m3, cm):
+            f.write("%d %d %d %r\n" % row)
+    print("wrote", os.path.join(DATA, "dms_table.txt"), ", ps_table.txt and ps_mass.txt")
 
 
 if __name__ == "__main__":

@@ -149,6 +149,49 @@ int ora_elec_forces(ora_elec_args *a);
 /* M-site redistribution of forces (3 per site), energies and 9-component stresses. */
 void ora_msite_distribute(int nw, double gamma, double *f, double *e, double *s9);
 
+/* ---- monomer, dispersion, long-range tail (water_oracle.c) -------------------- */
+
+/* PS potential tables (ps_table.txt, ps_mass.txt); init_ps_type_pes processing. */
+int ora_ps_load(const char *table, const char *mass);
+/* the processed c5z[245] and cmass[9] (internal units) */
+void ora_ps_coefficients(double *c5z, double *cmass);
+/* ps_type_couple for one water: bond lengths, unit O->H vectors, angle -> energy,
+ * forces on H1 (fdum[0..2]) and H2 (fdum[3..5]), virial */
+void ora_ps_couple(double roh1, const double *u1, double roh2, const double *u2, double theta,
+                   double *pterm, double *fdum, double *vterm);
+/* damping function f_n(x) = 1 - exp(-x) sum_{k<=n} x^k/k! (tang-toennies.f90) */
+double ora_tang_toennies(int n, double x);
+
+typedef struct {
+    /* inputs */
+    int imcon;
+    double cell[9];
+    int nw;                    /* waters; O,H1,H2 of water w at 3w..3w+2, M at 3nw+w */
+    const double *x, *y, *z;   /* 4*nw positions */
+    double rcut;               /* largest cutoff: sets the table grid (parset.f:210-215) */
+    double rvdw;               /* dispersion cutoff */
+    double c6[3], a6[3], c8[3], a8[3];   /* TT dispersion per pair type OO, OH, HH
+                                            (internal energy units, 1/A) */
+    int bond_key[2];           /* bndfrc key of the O-H bonds [0] and the H-H bond [1]:
+                                  0 none, 1 harm, 2 mors, 7 expo */
+    double bond_prm[2][3];     /* their prmbnd(1..3), energies in internal units */
+    int ps_monomer, dispersion, long_range;   /* switches of the three terms */
+    /* outputs: arrays accumulate, scalars are set */
+    double *f;                 /* 3*4nw interleaved forces */
+    double *atomic_energy;     /* optional 3nw */
+    double *heat_w;            /* optional 3nw*9 W_i[a][b] */
+    double *force_matrix;      /* optional dense [k][j][i] over 4nw sites */
+    double eng_bnd, vir_bnd;   /* monomer bonds */
+    double eng_ang, vir_ang;   /* PS monomer coupling terms */
+    double eng_srp, vir_srp;   /* dispersion pairs */
+    double elrc, virlrc;       /* long-range correction */
+    double stress[9];          /* bonds + PS + dispersion (accumulates) */
+} ora_water_args;
+
+/* Monomer bonds (bndfrc.f), PS monomer coupling (angfrc.f keya = 9), dispersion
+ * (srfrce2.f) and lrcorrect for a box of MB-pol waters.  Returns 0, -1 (imcon), -3 (PS tables not loaded). */
+int ora_water_terms(ora_water_args *a);
+
 /* ---- heat current (heat_oracle.c) ------------------------------------------ */
 
 /* compute_heat_flux2 (heatcurrent.f90:703-728) from the dense force matrix, layout

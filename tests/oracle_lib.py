@@ -271,3 +271,93 @@ def elec_forces(box, p, dip0=None, dense=False, xyz=None, cell=None, imcon=None,
     if dense:
         out["force_matrix"] = fm
     return out
+
+
+class WaterArgs(ctypes.Structure):
+    _fields_ = [("imcon", ctypes.c_int), ("cell", ctypes.c_double * 9), ("nw", ctypes.c_int),
+                ("x", _dp), ("y", _dp), ("z", _dp), ("rcut", ctypes.c_double), ("rvdw", ctypes.c_double),
+                ("c6", ctypes.c_double * 3), ("a6", ctypes.c_double * 3), ("c8", ctypes.c_double * 3),
+                ("a8", ctypes.c_double * 3), ("bond_key", ctypes.c_int * 2), ("bond_prm", (ctypes.c_double * 3) * 2),
+                ("ps_monomer", ctypes.c_int), ("dispersion", ctypes.c_int), ("long_range", ctypes.c_int),
+                ("f", _dp), ("atomic_energy", _dp), ("heat_w", _dp),
+                ("force_matrix", _dp), ("eng_bnd", ctypes.c_double), ("vir_bnd", ctypes.c_double),
+                ("eng_ang", ctypes.c_double), ("vir_ang", ctypes.c_double),
+                ("eng_srp", ctypes.c_double), ("vir_srp", ctypes.c_double), ("elrc", ctypes.c_double),
+                ("virlrc", ctypes.c_double), ("stress", ctypes.c_double * 9)]
+
+
+_ps_loaded = False
+
+
+def water_lib():
+    global _ps_loaded
+    lib = oracle()
+    if not _ps_loaded:
+        lib.ora_tang_toennies.restype = ctypes.c_double
+        lib.ora_tang_toennies.argtypes = [ctypes.c_int, ctypes.c_double]
+        lib.ora_ps_couple.argtypes = [ctypes.c_double, _dp, ctypes.c_double, _dp, ctypes.c_double, _dp, _dp, _dp]
+        lib.ora_ps_coefficients.argtypes = [_dp, _dp]
+        if lib.ora_ps_load(os.path.join(DATA, "ps_table.txt").encode(),
+                           os.path.join(DATA, "ps_mass.txt").encode()) != 0:
+            raise RuntimeError("oracle could not load the PS tables")
+        _ps_loaded = True
+    return lib
+
+
+def tang_toennies(n, x):
+    return water_lib().ora_tang_toennies(n, x)
+
+
+def ps_coefficients():
+    lib = water_lib()
+    c5z, cm = np.zeros(245), np.zeros(9)
+    lib.ora_ps_coefficients(P(c5z), P(cm))
+    return c5z, cm
+
+
+def ps_couple(roh1, u1, roh2, u2, theta):
+    lib = water_lib()
+    pt, vt, fd = ctypes.c_double(), ctypes.c_double(), np.zeros(6)
+    lib.ora_ps_couple(roh1, P(np.ascontiguousarray(u1, float)), roh2, P(np.ascontiguousarray(u2, float)), theta,
+                      ctypes.byref(pt), P(fd), ctypes.byref(vt))
+    return pt.value, fd.reshape(2, 3), vt.value
+
+
+def water_terms(box, p, xyz=None, cell=None, imcon=None, dense=False):
+    """PS monomer + TT dispersion + long-range correction of the oracle on a WaterBox with
+    WaterParams p (terms switched off in p are dropped by zeroing their inputs)."""
+    lib = water_lib()
+    nw = box.nw
+    n = 4 * nw
+    a = WaterArgs()
+    a.imcon = box.imcon if imcon is None else imcon
+    c = box.cell if cell is None else cell
+    for q in range(9):
+        a.cell[q] = c[q]
+    a.nw = nw
+    r = site_xyz(box) if xyz is None else np.array(xyz, float)
+    x, y, z = (np.ascontiguousarray(r[:, k]) for k in range(3))
+    a.x, a.y, a.z = P(x), P(y), P(z)
+    a.rcut, a.rvdw = p.rcut, p.rvdw
+    c6, a6, c8, a8 = p.internal()
+    for q in range(3):
+        a.c6[q], a.a6[q], a.c8[q], a.a8[q] = c6[q], a6[q], c8[q], a8[q]
+    for t, (key, prm) in enumerate(p.bonds_internal()):
+        a.bond_key[t] = key
+        for q in range(3):
+            a.bond_prm[t][q] = prm[q]
+    a.ps_monomer, a.dispersion, a.long_range = p.ps_monomer, p.dispersion, p.long_range
+    f = np.zeros(3 * n)
+    ae = np.zeros(3 * nw)
+    hw = np.zeros(9 * 3 * nw)
+    fm = np.zeros(3 * n * n) if dense else None
+    a.f, a.atomic_energy, a.heat_w, a.force_matrix = P(f), P(ae), P(hw), P(fm)
+    rc = lib.ora_water_terms(ctypes.byref(a))
+    if rc != 0:
+        raise RuntimeError(f"oracle water_terms returned {rc}")
+    out = dict(f=f.reshape(n, 3)[:3 * nw].copy(), atomic_energy=ae, heat_w=hw.reshape(3 * nw, 3, 3),
+               eng_bnd=a.eng_bnd, vir_bnd=a.vir_bnd, eng_ang=a.eng_ang, vir_ang=a.vir_ang, eng_srp=a.eng_srp, vir_srp=a.vir_srp, elrc=a.elrc,
+               virlrc=a.virlrc, stress=np.array(a.stress[:]), xyz=np.stack([x, y, z], 1))
+    if dense:
+        out["force_matrix"] = fm
+    return out
