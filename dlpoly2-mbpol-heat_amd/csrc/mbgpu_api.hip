@@ -99,6 +99,7 @@ struct mbgpu_ctx {
     double *dv = nullptr, *dek = nullptr, *dep = nullptr, *dw = nullptr, *dsew = nullptr, *dout = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     ElecState el;                     // polarizable electrostatics
+    WaterState wt;                    // monomer, dispersion and long-range terms
 };
 
 namespace {
@@ -283,20 +284,46 @@ int leaf_eval(int nmon, const double* const* w, double* E, double* gout)
     return 0;
 }
 
-// Add the electrostatics' per-atom terms (when the last electrostatics call completed) to
-// the heat-flux extras: an extra already in a context scratch buffer gets the term added in
-// place; a missing extra is replaced by the term itself.
-void with_elec_terms(mbgpu_ctx* c, const double*& ep, const double*& we, const double*& se)
+// Sum the per-atom terms of the model parts whose last evaluation in this context completed
+// (the electrostatics' energies, contractions W and reciprocal stresses, the water terms'
+// energies and W) into the heat-flux extras.  `x` is either NULL or already a context
+// scratch buffer (then the terms are added in place); a NULL extra becomes the single
+// term itself, or the scratch buffer holding the sum of several.
+void add_terms(mbgpu_ctx* c, const double*& x, const double* const* terms, int nterms, double* scratch, long n)
 {
-    if (!c->el.ready) return;
+    int k = 0;
+    if (!x) {
+        if (nterms == 0) return;
+        if (nterms == 1) {
+            x = terms[0];
+            return;
+        }
+        (void)hipMemcpyAsync(scratch, terms[0], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream);
+        x = scratch;
+        k = 1;
+    }
+    for (; k < nterms; ++k) el_launch_add(const_cast<double*>(x), terms[k], n, c->stream);
+}
+
+bool have_model_terms(const mbgpu_ctx* c) { return c->el.ready || c->wt.ready; }
+
+void with_model_terms(mbgpu_ctx* c, const double*& ep, const double*& we, const double*& se)
+{
     const int nr = 3 * c->B.nw;
-    const ElecBuffers& E = c->el.E;
-    if (ep) el_launch_add(const_cast<double*>(ep), E.eout, nr, c->stream);
-    else ep = E.eout;
-    if (we) el_launch_add(const_cast<double*>(we), E.wout, 9L * nr, c->stream);
-    else we = E.wout;
-    if (se) el_launch_add(const_cast<double*>(se), E.sout, 9L * nr, c->stream);
-    else se = E.sout;
+    const double *es[2], *ws[2], *ss[1];
+    int ne = 0, ns = 0;
+    if (c->el.ready) {
+        es[ne] = c->el.E.eout;
+        ws[ne++] = c->el.E.wout;
+        ss[ns++] = c->el.E.sout;
+    }
+    if (c->wt.ready) {
+        es[ne] = c->wt.B.eout;
+        ws[ne++] = c->wt.B.wout;
+    }
+    add_terms(c, ep, es, ne, c->dep, nr);
+    add_terms(c, we, ws, ne, c->dw, 9L * nr);
+    add_terms(c, se, ss, ns, c->dsew, 9L * nr);
 }
 
 }  // namespace
@@ -348,6 +375,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
     elec_release(c->el);
+    water_release(c->wt);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
@@ -450,7 +478,7 @@ int mbgpu_heat_flux(mbgpu_ctx* c, const double* vxx, const double* vyy, const do
     if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
     const double *ep = e_pot_extra ? c->dep : nullptr, *we = w_extra ? c->dw : nullptr,
                  *se = stress_ew1p ? c->dsew : nullptr;
-    with_elec_terms(c, ep, we, se);
+    with_model_terms(c, ep, we, se);
     int rc = mb_launch_heat_flux(nr, v, v + nr, v + 2 * nr, c->dek, c->B.atomic_energy, ep, c->B.heat_w, we, se,
                                  c->B.blk, c->B.nblk_cap, c->dout, c->stream);
     if (rc) return fail(rc, "heat-flux kernels failed");
@@ -473,8 +501,8 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const 
     const int nr = 3 * c->B.nw;
     if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
     const double *ep = e_pot_extra, *we = w_extra, *se = stress_ew1p;
-    if (c->el.ready) {
-        // the electrostatics' terms are added to copies of the caller's extras
+    if (have_model_terms(c)) {
+        // the model terms are added to copies of the caller's extras
         if (ep) HIPCHK(hipMemcpyAsync(c->dep, ep, sizeof(double) * nr, hipMemcpyDeviceToDevice, c->stream));
         if (we) HIPCHK(hipMemcpyAsync(c->dw, we, sizeof(double) * 9 * nr, hipMemcpyDeviceToDevice, c->stream));
         if (se) HIPCHK(hipMemcpyAsync(c->dsew, se, sizeof(double) * 9 * nr, hipMemcpyDeviceToDevice, c->stream));
@@ -482,7 +510,7 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const 
         we = we ? c->dw : nullptr;
         se = se ? c->dsew : nullptr;
     }
-    with_elec_terms(c, ep, we, se);
+    with_model_terms(c, ep, we, se);
     int rc = mb_launch_heat_flux(nr, vx, vy, vz, e_kin, c->B.atomic_energy, ep, c->B.heat_w, we, se, c->B.blk,
                                  c->B.nblk_cap, c->dout, c->stream);
     if (rc) return fail(rc, "heat-flux kernels failed");
@@ -604,6 +632,84 @@ int mbgpu_get_elec_details(mbgpu_ctx* c, double* scal, double* chge, double* ves
     if (w_real) HIPCHK(hipMemcpyAsync(w_real, E.wout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->stream));
     if (s_real) HIPCHK(hipMemcpyAsync(s_real, E.sout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->stream));
     if (f_site) HIPCHK(hipMemcpyAsync(f_site, E.fsite, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_water_terms_dev(mbgpu_ctx* c, int imcon, const double cell[9], const mbgpu_water_params* p,
+                          const double* x_dev, const double* y_dev, const double* z_dev, double* fx_dev,
+                          double* fy_dev, double* fz_dev, double eng[4], double vir[4], double stress[9])
+{
+    if (!c) return fail(-1, "null context");
+    if (!p) return fail(-1, "null water-term parameters");
+    if (!x_dev || !y_dev || !z_dev) return fail(-1, "null position array");
+    CellGeom g;
+    if (!make_geom(imcon, cell, g))
+        return fail(-4, "imcon = %d is not supported by images (reference stops in fixme)", imcon);
+    std::string err;
+    c->wt.ready = false;
+    if (water_configure(c->wt, c->B.nw, g, cell, p, c->stream, err)) return fail(-1, "%s", err.c_str());
+    if (water_evaluate(c->wt, x_dev, y_dev, z_dev, fx_dev, fy_dev, fz_dev, c->stream, err))
+        return fail(-1, "%s", err.c_str());
+    const double* o = c->wt.out;
+    if (eng) {
+        eng[0] = o[0];
+        eng[1] = o[2];
+        eng[2] = o[4];
+        eng[3] = o[6];
+    }
+    if (vir) {
+        vir[0] = o[1];
+        vir[1] = o[3];
+        vir[2] = o[5];
+        vir[3] = o[7];
+    }
+    if (stress)
+        for (int i = 0; i < 9; ++i) stress[i] += c->wt.stress[i];
+    return 0;
+}
+
+int mbgpu_water_terms(mbgpu_ctx* c, int imcon, const double cell[9], const mbgpu_water_params* p,
+                      const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
+                      double* fzz, double eng[4], double vir[4], double stress[9])
+{
+    if (!c) return fail(-1, "null context");
+    if (!xxx || !yyy || !zzz) return fail(-1, "null position array");
+    const int nr = 3 * c->B.nw;
+    HIPCHK(hipMemcpyAsync(c->hx, xxx, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hy, yyy, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hz, zzz, sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    int rc = mbgpu_water_terms_dev(c, imcon, cell, p, c->hx, c->hy, c->hz, nullptr, nullptr, nullptr, eng, vir,
+                                   stress);
+    if (rc) return rc;
+    if (fxx || fyy || fzz) {
+        std::vector<double> f((size_t)3 * nr);
+        HIPCHK(hipMemcpyAsync(f.data(), c->wt.B.fw, sizeof(double) * 3 * nr, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < nr; ++i) {
+            if (fxx) fxx[i] += f[i];
+            if (fyy) fyy[i] += f[nr + i];
+            if (fzz) fzz[i] += f[2 * nr + i];
+        }
+    }
+    return 0;
+}
+
+int mbgpu_get_water_details(mbgpu_ctx* c, double* e_real, double* w_real, double* f_real)
+{
+    if (!c) return fail(-1, "null context");
+    if (!c->wt.ready) return fail(-1, "no completed water-terms evaluation in this context");
+    const WaterBuffers& W = c->wt.B;
+    const int nr = 3 * c->B.nw;
+    if (e_real) HIPCHK(hipMemcpyAsync(e_real, W.eout, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+    if (w_real) HIPCHK(hipMemcpyAsync(w_real, W.wout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->stream));
+    if (f_real) {
+        std::vector<double> f((size_t)3 * nr);
+        HIPCHK(hipMemcpyAsync(f.data(), W.fw, sizeof(double) * 3 * nr, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < nr; ++i)
+            for (int k = 0; k < 3; ++k) f_real[3 * i + k] = f[(size_t)k * nr + i];
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

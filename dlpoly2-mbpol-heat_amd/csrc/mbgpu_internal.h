@@ -145,3 +145,48 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
                    std::string& err);
 int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStream_t s, std::string& err);
 void elec_release(ElecState& S);
+
+// ------------------------------------------------------------- monomer and dispersion ----
+
+// Kernel-argument block of the water terms (passed by value).
+struct WaterDev {
+    int nw, nr;                    // waters, real atoms (3nw)
+    CellGeom g;
+    int bond_key[2];               // O-H, H-H bond keys (0 none, 1 harm, 2 mors, 7 expo)
+    double bond_prm[2][3];
+    int ps_monomer, dispersion;    // switches
+    int tmask;                     // bit t: dispersion table of pair type t is non-zero
+    int ngrid;                     // table row stride (mxgrid + 3)
+    double rc2, rdr;               // rvdw^2, 1/dlrpot
+};
+
+// Device buffers of the water terms (owned by the context).
+struct WaterBuffers {
+    double* mono = nullptr;        // [nr][13] per-atom monomer terms: f3, e, W9
+    double* wsc = nullptr;         // [nw][10] per-water scalars
+    double* asc = nullptr;         // [nr][8] per-atom dispersion scalars
+    double* fw = nullptr;          // [3][nr] forces of the water terms
+    double *eout = nullptr, *wout = nullptr;   // [nr], [nr][9] per-atom energy and W
+    double *vvv = nullptr, *ggg = nullptr;     // [3][ngrid] dispersion tables
+    double *psc = nullptr, *pcm = nullptr;     // processed PS coefficients [245], [9]
+    int *pidx = nullptr, *pidxm = nullptr;     // packed exponents i1 | i2 << 8 | i3 << 16
+    double* scal = nullptr;        // [24] reduced scalars
+    int ngrid = 0;
+};
+
+struct WaterState {
+    WaterBuffers B;
+    WaterDev P{};
+    bool ready = false;            // the last evaluation completed
+    int nw = 0;
+    double tab_key[14];            // rcut, rvdw, c6, a6, c8, a8 of the tables
+    bool have_tab = false;
+    double out[8];                 // eng_bnd, vir_bnd, eng_ang, vir_ang, eng_srp, vir_srp, elrc, virlrc
+    double stress[9];              // bonds + PS + dispersion
+};
+
+int water_configure(WaterState& S, int nw, const CellGeom& g, const double* cell, const mbgpu_water_params* p,
+                    hipStream_t s, std::string& err);
+int water_evaluate(WaterState& S, const double* x, const double* y, const double* z, double* fx, double* fy,
+                   double* fz, hipStream_t s, std::string& err);
+void water_release(WaterState& S);

@@ -54,6 +54,12 @@ _SIGS = {
                                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                       ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp, _ip]),
     "mbgpu_get_elec_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbgpu_water_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, _dp, _dp, _dp, _dp,
+                                         _dp, _dp, _dp, _dp, _dp]),
+    "mbgpu_water_terms_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, _dp, _dp, _dp]),
+    "mbgpu_get_water_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
 }
 
 
@@ -82,6 +88,29 @@ class ElecParams(ctypes.Structure):
         e.distribute_msite = distribute_msite
         e.solver = getattr(p, "solver", 0) if solver is None else solver
         return e
+
+
+class WaterParams(ctypes.Structure):
+    """mbgpu_water_params (include/mbgpu.h)."""
+    _fields_ = [("rcut", ctypes.c_double), ("rvdw", ctypes.c_double), ("c6", ctypes.c_double * 3),
+                ("a6", ctypes.c_double * 3), ("c8", ctypes.c_double * 3), ("a8", ctypes.c_double * 3),
+                ("bond_key", ctypes.c_int * 2), ("bond_prm", (ctypes.c_double * 3) * 2),
+                ("ps_monomer", ctypes.c_int), ("dispersion", ctypes.c_int), ("long_range", ctypes.c_int)]
+
+    @classmethod
+    def from_params(cls, p):
+        """From a water_params.WaterParams (energies converted to the internal unit)."""
+        w = cls()
+        w.rcut, w.rvdw = p.rcut, p.rvdw
+        c6, a6, c8, a8 = p.internal()
+        for i in range(3):
+            w.c6[i], w.a6[i], w.c8[i], w.a8[i] = c6[i], a6[i], c8[i], a8[i]
+        for t, (key, prm) in enumerate(p.bonds_internal()):
+            w.bond_key[t] = key
+            for q in range(3):
+                w.bond_prm[t][q] = prm[q]
+        w.ps_monomer, w.dispersion, w.long_range = p.ps_monomer, p.dispersion, p.long_range
+        return w
 
 
 class MbgpuError(RuntimeError):
@@ -286,6 +315,41 @@ class Context:
                                                            _p(stress), ctypes.byref(it))
         _check(rc, "mbgpu_polar_electrostatics_dev")
         return e.value, v.value, stress, it.value
+
+    # --------------------------------------------- monomer, dispersion, long range ----
+    def water_terms(self, imcon, cell, params, xxx, yyy, zzz, fxx=None, fyy=None, fzz=None):
+        """Monomer bonds, PS coupling terms, dispersion and long-range correction
+        (bndfrc, angfrc, srfrce2, lrcorrect): fxx..fzz (+=) modified in place.
+        Returns dict(eng_bnd, eng_ang, eng_srp, elrc, vir_*, virlrc, stress)."""
+        nr = 3 * self.nw
+        for a in [xxx, yyy, zzz] + [a for a in (fxx, fyy, fzz) if a is not None]:
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.size < nr:
+                raise ValueError("position and force arrays must be contiguous float64 of >= 3*nw values")
+        cell = _f64(cell, 9)
+        p = params if isinstance(params, WaterParams) else WaterParams.from_params(params)
+        eng, vir, stress = np.zeros(4), np.zeros(4), np.zeros(9)
+        rc = load_library().mbgpu_water_terms(self._h, imcon, _p(cell), ctypes.byref(p), _p(xxx), _p(yyy), _p(zzz),
+                                              _p(fxx), _p(fyy), _p(fzz), _p(eng), _p(vir), _p(stress))
+        _check(rc, "mbgpu_water_terms")
+        return dict(eng_bnd=eng[0], eng_ang=eng[1], eng_srp=eng[2], elrc=eng[3], vir_bnd=vir[0], vir_ang=vir[1],
+                    vir_srp=vir[2], virlrc=vir[3], stress=stress)
+
+    def water_terms_dev(self, imcon, cell, params, x_ptr, y_ptr, z_ptr, fx_ptr=None, fy_ptr=None, fz_ptr=None):
+        """Device-pointer variant; returns (eng[4], vir[4], stress[9])."""
+        cell = _f64(cell, 9)
+        p = params if isinstance(params, WaterParams) else WaterParams.from_params(params)
+        eng, vir, stress = np.zeros(4), np.zeros(4), np.zeros(9)
+        rc = load_library().mbgpu_water_terms_dev(self._h, imcon, _p(cell), ctypes.byref(p), x_ptr, y_ptr, z_ptr,
+                                                  fx_ptr, fy_ptr, fz_ptr, _p(eng), _p(vir), _p(stress))
+        _check(rc, "mbgpu_water_terms_dev")
+        return eng, vir, stress
+
+    def water_details(self):
+        """Per-atom energy, W and forces of the last water-terms call."""
+        nr = 3 * self.nw
+        e, w, f = np.zeros(nr), np.zeros(9 * nr), np.zeros(3 * nr)
+        _check(load_library().mbgpu_get_water_details(self._h, _p(e), _p(w), _p(f)), "mbgpu_get_water_details")
+        return dict(atomic_energy=e, heat_w=w.reshape(nr, 3, 3), f=f.reshape(nr, 3))
 
     def elec_details(self):
         """Results of the last electrostatics call (see mbgpu_get_elec_details)."""

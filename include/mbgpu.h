@@ -184,6 +184,50 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[
 int mbgpu_get_elec_details(mbgpu_ctx* ctx, double* scal, double* chge, double* vesp, double* e_real,
                            double* w_real, double* s_real, double* f_site);
 
+/* Parameters of the monomer, dispersion and long-range terms (FIELD inputs in the
+ * reference; energies in the internal unit, 10 J/mol). */
+typedef struct {
+    double rcut;          /* largest cutoff of the run: sets the dispersion table grid,
+                             max(1000, int(rcut/1e-4 + 0.5) + 4) points (parset.f:210-215) */
+    double rvdw;          /* dispersion cutoff, table spacing rvdw/(grid - 4) (forgen.f:97) */
+    double c6[3], a6[3], c8[3], a8[3];   /* "tt68" vdw parameters (sysdef_cmd.f:1505-1510,
+                                            1598-1602) of the O-O, O-H and H-H pairs */
+    int bond_key[2];      /* bndfrc key of the two O-H bonds [0] and the H-H bond [1]:
+                             0 none, 1 harm, 2 mors, 7 expo (sysdef_cmd.f:615-644) */
+    double bond_prm[2][3];   /* their prmbnd(1..3) */
+    int ps_monomer;       /* 1: Partridge-Schwenke coupling terms (angle key 9) */
+    int dispersion;       /* 1: pair dispersion within rvdw */
+    int long_range;       /* 1: long-range dispersion correction */
+} mbgpu_water_params;
+
+/* Stands in for the intramolecular and dispersion terms of a force evaluation of MB-pol
+ * water: bndfrc (dlpoly_cmd.f:2422; bndfrc.f:136-330) for the monomer bonds, angfrc with
+ * the Partridge-Schwenke branch (dlpoly_cmd.f:2430; angfrc.f:160-545 -> ps_type_couple,
+ * ps_type_couple.f90:15-274), the dispersion part of the pair loop (forces.f:549-556 ->
+ * srfrce2, srfrce2.f:60-281, with the forgen tables, forgen.f:90-100, 262-269, 338-357)
+ * and lrcorrect (dlpoly_cmd.f:2388; lrcorrect.f:100-209).  Same site layout as
+ * mbgpu_polar_electrostatics; only the 3*nw real atoms take part (the M sites have no
+ * dispersion).  Dispersion pairs are all pairs of atoms of different waters within rvdw.
+ *   fxx..fzz: += forces (internal units) on the real atoms.
+ *   eng[4]  : set to engbnd, engang, engsrp (dispersion pairs) and elrc;
+ *   vir[4]  : set to virbnd, virang, virsrp and virlrc;
+ *   stress  : += bond, PS and dispersion stress (the caller adds -virlrc/3 to the diagonal,
+ *             as dlpoly_cmd.f:2572-2574 does).
+ * Per-atom energies (bond halves, pterm/3, dispersion halves; the long-range correction
+ * has no per-atom share, its per-atom update is disabled in lrcorrect.f:215-222) and the
+ * force-matrix contractions stay on the device and enter mbgpu_heat_flux automatically. */
+int mbgpu_water_terms(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_water_params* p,
+                      const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
+                      double* fzz, double eng[4], double vir[4], double stress[9]);
+
+/* Same with device pointers (x_dev..z_dev at least 3*nw long; f*_dev +=, may be NULL). */
+int mbgpu_water_terms_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_water_params* p,
+                          const double* x_dev, const double* y_dev, const double* z_dev, double* fx_dev,
+                          double* fy_dev, double* fz_dev, double eng[4], double vir[4], double stress[9]);
+
+/* Results of the last water-terms call (host arrays of 3*nw, 9*3*nw and 3*3*nw entries,
+ * any may be NULL): per-atom energy, force-matrix contraction W_i[a][b], forces. */
+int mbgpu_get_water_details(mbgpu_ctx* ctx, double* e_real, double* w_real, double* f_real);
 
 /* Timing helper: runs the MB-pol force evaluation `reps` times on device-resident inputs
  * already uploaded by a previous mbgpu_mbpol_forces(_dev) call and returns the mean
