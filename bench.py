@@ -7,10 +7,13 @@ resident in HBM before the timed region:
   * MB-pol 2-body + 3-body forces with the per-atom energy / force-matrix decomposition
     (lists, polynomials, per-atom accumulation);
   * the polarizable Ewald electrostatics (M sites, DMS charges, induced dipoles solved
-    from zero start values every step with the reference's SOR, omega = 0.3, tolerance
-    1e-12 D; reciprocal space kmax = 16, alpha = 0.3 1/A, rcut = 9 A), with their per-atom
-    energies, reciprocal stresses and force-matrix contractions;
-  * the Green-Kubo heat flux over both.
+    from zero start values every step to the reference's stopping test, tolerance 1e-12 D,
+    by conjugate gradients (default) or the reference's SOR with omega = 0.3; reciprocal
+    space kmax = 16, alpha = 0.3 1/A, rcut = 9 A), with their per-atom energies,
+    reciprocal stresses and force-matrix contractions;
+  * the Partridge-Schwenke monomer (bonds + coupling terms), the TT-damped dispersion
+    within 9 A and the long-range dispersion correction, with their per-atom terms;
+  * the Green-Kubo heat flux over all of them.
 The read-back of the energy, virial and stress scalars is inside the step.
 
 Prints one JSON line (see DESIGN.md for the field definitions).
@@ -29,7 +32,11 @@ sys.path.insert(0, REPO)
 METRIC = "force evals/s @512 H2O, mixed vs FP64 (same GPU, ΔF≤1e-5) + p99 serve latency"
 FLOP_DIMER = 31600      # algorithmic FLOP per 2B dimer (value + gradient)
 FLOP_TRIMER = 113100    # algorithmic FLOP per 3B trimer (value + gradient)
-PEAK_FP64 = 78.6        # TFLOP/s, MI355X vector / matrix FP64
+PEAK_FP64 = 78.6        # TFLOP/s, MI355X FP64 (vector and matrix peaks are equal)
+# HBM bytes per launch of k_el_sfac<false> at this workload, from rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE (KiB per dispatch) scaled by the same counters on a plain 512 MiB copy (FETCH x 2,
+# WRITE x 1); profiles/k_el_sfac_bench512_traffic.md
+SFAC_TRAFFIC_BYTES = (2863.1 * 2.0 + 10040.5 * 1.0) * 1024
 
 
 def cpu_baseline(box):
@@ -48,16 +55,19 @@ def cpu_baseline(box):
                            ctypes.cast(ref.mbpol_3b_poly_, ctypes.c_void_p))
         kind = "reference"
     ek = 0.5 * box.mass * (box.vel ** 2).sum(1)
-    ep = importlib.import_module("dlpoly2-mbpol-heat_amd").elec_params.for_box(box.L)
+    pkg = importlib.import_module("dlpoly2-mbpol-heat_amd")
+    ep = pkg.elec_params.for_box(box.L)
+    wp = pkg.water_params.for_box(box.L)
     nr = box.nreal
     n, t0 = 0, time.perf_counter()
     while True:
         r = oracle_lib.mbpol_forces(box, compat_inplace=1, lists=False)
         el = oracle_lib.elec_forces(box, ep)
+        wt = oracle_lib.water_terms(box, wp)
         hw = r["heat_w"].copy()
-        hw[:nr] += el["heat_w"]
+        hw[:nr] += el["heat_w"] + wt["heat_w"]
         e = r["atomic_energy"].copy()
-        e[:nr] += el["atomic_energy"]
+        e[:nr] += el["atomic_energy"] + wt["atomic_energy"]
         s = np.zeros((box.natms, 9))
         s[:nr] = el["s_ew1p"]
         oracle_lib.heat_flux_tensor(box, hw.reshape(-1), box.vel, ek, e, s.reshape(-1).copy())
@@ -71,7 +81,8 @@ def cpu_baseline(box):
     return {"value": n / dt, "unit": "force evals/s", "cores": 1, "kind": kind,
             "sample": f"{n} full 512-water evaluation(s): MB-pol 2B+3B (all-pairs / all-triples scan as "
                       f"mbpol.f90, {what}) + the oracle's polarizable Ewald electrostatics (same parameters, "
-                      f"SOR from zero dipoles) + decomposition + heat flux, {dt:.1f} s"}
+                      f"SOR from zero dipoles) + PS monomer, dispersion and long-range tail + decomposition + heat "
+                      f"flux, {dt:.1f} s"}
 
 
 def main():
@@ -82,7 +93,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--solver", choices=["sor", "cg"], default="cg",
                     help="induced-dipole solver: the reference's SOR or conjugate gradients to the same test")
+    ap.add_argument("--copy-calibration", type=int, metavar="MIB", default=0,
+                    help="only run a plain device copy of MIB MiB five times (scale reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is "
               "described in DESIGN.md but not built)", file=sys.stderr)
@@ -98,16 +122,18 @@ def main():
     ctx = pkg.Context(box.nw, box.natms)
     ep = pkg.elec_params.for_box(box.L)
     eparams = pkg.ElecParams.from_params(ep, distribute_msite=1, solver=1 if args.solver == "cg" else 0)
+    wparams = pkg.WaterParams.from_params(pkg.water_params.for_box(box.L))
     T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
     x, y, z = T(box.x), T(box.y), T(box.z)
     lst = T(box.listttm2, torch.int32)
     dip = torch.zeros(3 * box.natms, dtype=torch.float64, device=dev)
     fsite = torch.zeros(3 * box.natms, dtype=torch.float64, device=dev)
+    fw = torch.zeros(3, box.natms, dtype=torch.float64, device=dev)
     vx, vy, vz = T(box.vel[:box.nreal, 0]), T(box.vel[:box.nreal, 1]), T(box.vel[:box.nreal, 2])
     ek = T(0.5 * box.mass[:box.nreal] * (box.vel[:box.nreal] ** 2).sum(1))
     torch.cuda.synchronize()
 
-    parts = {"mbpol": 0.0, "elec": 0.0, "heat": 0.0}
+    parts = {"mbpol": 0.0, "elec": 0.0, "water": 0.0, "heat": 0.0}
     iters = []
 
     def step(timed=False):
@@ -115,18 +141,23 @@ def main():
         e, v, s = ctx.mbpol_forces_dev(box.imcon, box.cell, lst.data_ptr(), x.data_ptr(), y.data_ptr(),
                                        z.data_ptr())
         t1 = time.perf_counter()
-        dip.zero_()
+        dip.zero_()                               # zero start dipoles (torch's stream) ...
+        torch.cuda.current_stream().synchronize()  # ... complete before the library's stream reads them
         ee, ve, se, it = ctx.polar_electrostatics_dev(box.imcon, box.cell, eparams, x.data_ptr(), y.data_ptr(),
                                                       z.data_ptr(), dip.data_ptr(), fsite.data_ptr())
         t2 = time.perf_counter()
-        jk, jp, jt = ctx.heat_flux_dev(vx.data_ptr(), vy.data_ptr(), vz.data_ptr(), ek.data_ptr())
+        ew, vw, sw = ctx.water_terms_dev(box.imcon, box.cell, wparams, x.data_ptr(), y.data_ptr(), z.data_ptr(),
+                                         fw[0].data_ptr(), fw[1].data_ptr(), fw[2].data_ptr())
         t3 = time.perf_counter()
+        jk, jp, jt = ctx.heat_flux_dev(vx.data_ptr(), vy.data_ptr(), vz.data_ptr(), ek.data_ptr())
+        t4 = time.perf_counter()
         if timed:
             parts["mbpol"] += t1 - t0
             parts["elec"] += t2 - t1
-            parts["heat"] += t3 - t2
+            parts["water"] += t3 - t2
+            parts["heat"] += t4 - t3
             iters.append(it)
-        return e + ee, jt
+        return e + ee + ew.sum(), jt
 
     for _ in range(args.warmup):
         step()
@@ -141,8 +172,12 @@ def main():
     d, t = ctx.lists()
     nd, nt = len(d), len(t)
     kvec = int(ctx.elec_details()["k_vectors"])
-    _, ms_k3 = ctx.time_forces(max(5, min(args.steps, 20)))
-    achieved = nt * FLOP_TRIMER / (ms_k3 * 1e-3) / 1e12
+    reps = max(5, min(args.steps, 20))
+    _, ms_k3 = ctx.time_forces(reps)
+    ms_sf, flop_sf = ctx.time_elec(reps)
+    tf_k3 = nt * FLOP_TRIMER / (ms_k3 * 1e-3) / 1e12
+    achieved = flop_sf / (ms_sf * 1e-3) / 1e12
+    n_sfac = (int(np.median(iters)) + 2) if iters else None
     out = {
         "metric": METRIC,
         "value": args.steps / dt,
@@ -156,16 +191,23 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded 512-water lattice box, rigid-geometry perturbations, 300 K velocities)",
-        "config": {"workload": "MB-pol 2B+3B + polarizable Ewald electrostatics (DMS charges, SOR induced "
-                               "dipoles from zero, kmax 16, rcut 9 A) + per-atom energy/heat-flux decomposition "
-                               "+ Green-Kubo heat flux, FP64, 512 H2O L=24.84 A",
+        "config": {"workload": "MB-pol 2B+3B + polarizable Ewald electrostatics (DMS charges, induced dipoles "
+                               "from zero to 1e-12 D, kmax 16, rcut 9 A) + PS monomer + TT dispersion and "
+                               "long-range tail + per-atom energy/heat-flux decomposition + Green-Kubo heat flux, "
+                               "FP64, 512 H2O L=24.84 A",
                    "waters": box.nw, "dimers": nd, "trimers": nt, "k_vectors": kvec, "dipole_iterations":
                    int(np.median(iters)) if iters else None, "dipole_solver": args.solver, "parallelism": "single GPU",
                    "ms_mbpol": 1e3 * parts["mbpol"] / args.steps, "ms_electrostatics": 1e3 * parts["elec"] / args.steps,
-                   "ms_heat_flux": 1e3 * parts["heat"] / args.steps},
-        "roofline": {"bound": "valu-fp64", "kernel": "k_pip3b<double>", "achieved": achieved, "peak": PEAK_FP64,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
-                     "kernel_ms": ms_k3, "flop_per_launch": nt * FLOP_TRIMER},
+                   "ms_water_terms": 1e3 * parts["water"] / args.steps,
+                   "ms_heat_flux": 1e3 * parts["heat"] / args.steps,
+                   "k_pip3b_ms": ms_k3, "k_pip3b_tflops": tf_k3, "k_pip3b_frac": tf_k3 / PEAK_FP64,
+                   "sfac_launches_per_step": n_sfac},
+        "roofline": {"bound": "mfma", "engine": "fp64 vector ALU (no dense contraction; MI355X FP64 vector and "
+                                                "matrix peaks are both 78.6 TFLOP/s)",
+                     "kernel": "k_el_sfac<false> (dipole structure factors, launched once per solver iteration)",
+                     "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": SFAC_TRAFFIC_BYTES, "traffic_unit": "bytes per launch (rocprofv3 PMC)",
+                     "kernel_ms": ms_sf, "flop_per_launch": flop_sf},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(box)

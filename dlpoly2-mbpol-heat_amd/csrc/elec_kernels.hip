@@ -1751,6 +1751,19 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s)
     return (int)hipGetLastError();
 }
 
+// `reps` launches of the dipole structure-factor pass over the current dipoles, bracketed
+// by events on the stream (timing only: the partials it writes are rewritten by every
+// evaluation before use)
+int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEvent_t b, hipStream_t s)
+{
+    (void)hipEventRecord(a, s);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.zero_state);
+    (void)hipEventRecord(b, s);
+    return (int)hipGetLastError();
+}
+
 int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
                     hipStream_t s)
 {

@@ -743,6 +743,24 @@ int mbgpu_time_forces(mbgpu_ctx* c, int reps, double* ms_per_eval, double* ms_pi
     return 0;
 }
 
+int mbgpu_time_elec(mbgpu_ctx* c, int reps, double* ms_sfac, double* flop_sfac)
+{
+    if (!c || !c->el.ready) return fail(-1, "mbgpu_time_elec: run mbgpu_polar_electrostatics first");
+    if (reps < 1) return fail(-1, "reps must be positive");
+    ElecState& S = c->el;
+    int rc = el_time_sfac(S.E, S.P, reps, c->ev0, c->ev1, c->stream);
+    if (rc) return fail(rc, "timing run failed");
+    float ms = 0.f;
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    // polarizable sites: the O and H of every water, plus the M sites if they carry polr2
+    const int npol = 2 * S.nw * (S.P.polr2[1] > 1.e-6) + S.nw * (S.P.polr2[0] > 1.e-6) +
+                     S.nw * (S.P.polr2[2] > 1.e-6);
+    if (ms_sfac) *ms_sfac = ms / reps;
+    if (flop_sfac) *flop_sfac = 18.0 * S.P.K * npol;
+    return 0;
+}
+
 // ------------------------------------------------------------------------- leaves ----
 void mbpol_2b_cutoff_(double* r) { *r = MBGPU_R2F; }
 void mbpol_3b_cutoff_(double* r) { *r = MBGPU_R3F; }

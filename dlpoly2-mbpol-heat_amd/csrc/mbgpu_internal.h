@@ -122,6 +122,7 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
                     hipStream_t s);
 int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s);
 int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s);
+int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEvent_t b, hipStream_t s);
 void el_launch_zero(double* p, long n, hipStream_t s);
 void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
 

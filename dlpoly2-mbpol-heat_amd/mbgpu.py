@@ -48,6 +48,7 @@ _SIGS = {
                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp,
                                            _dp, _dp]),
     "mbgpu_time_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
+    "mbgpu_time_elec": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp]),
     "mbgpu_polar_electrostatics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, _dp, _dp,
                                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
     "mbgpu_polar_electrostatics_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p,
@@ -279,6 +280,12 @@ class Context:
         ms, k3 = ctypes.c_double(), ctypes.c_double()
         _check(load_library().mbgpu_time_forces(self._h, reps, ctypes.byref(ms), ctypes.byref(k3)), "time_forces")
         return ms.value, k3.value
+
+    def time_elec(self, reps):
+        """(ms per launch, FLOP per launch) of the dipole structure-factor kernel."""
+        ms, fl = ctypes.c_double(), ctypes.c_double()
+        _check(load_library().mbgpu_time_elec(self._h, reps, ctypes.byref(ms), ctypes.byref(fl)), "time_elec")
+        return ms.value, fl.value
 
     # ------------------------------------------------------------ electrostatics ----
     def polar_electrostatics(self, imcon, cell, params, xxx, yyy, zzz, dipx, dipy, dipz, fxx=None, fyy=None,

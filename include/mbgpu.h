@@ -234,6 +234,13 @@ int mbgpu_get_water_details(mbgpu_ctx* ctx, double* e_real, double* w_real, doub
  * milliseconds per evaluation, measured with HIP events on the library's stream. */
 int mbgpu_time_forces(mbgpu_ctx* ctx, int reps, double* ms_per_eval, double* ms_pip3b_kernel);
 
+/* Timing helper: `reps` launches of the reciprocal-space dipole structure-factor kernel on
+ * the dipoles of the last mbgpu_polar_electrostatics(_dev) call, timed with HIP events on
+ * the library's stream.  Returns the mean milliseconds per launch and the algorithmic
+ * FLOP of one launch: 18 per (k vector, polarizable site) -- six multiply-adds of the
+ * dipole components into Re/Im S(k) and the complex phase recurrence along n. */
+int mbgpu_time_elec(mbgpu_ctx* ctx, int reps, double* ms_sfac, double* flop_sfac);
+
 #ifdef __cplusplus
 }
 #endif
