@@ -184,6 +184,18 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
         HIPCHK(dalloc(&B.g3, (size_t)27 * B.tcap));
         HIPCHK(dalloc(&B.s3, (size_t)3 * B.tcap));
     }
+    if (B.mixed && (B.fcap2 != B.dcap || B.fcap3 != B.tcap)) {
+        dfree(B.xv2f);
+        dfree(B.part2f);
+        dfree(B.xv3f);
+        dfree(B.part3f);
+        HIPCHK(dalloc(&B.xv2f, (size_t)31 * B.dcap));
+        HIPCHK(dalloc(&B.part2f, (size_t)pip2b_nslices() * 32 * B.dcap));
+        HIPCHK(dalloc(&B.xv3f, (size_t)36 * B.tcap));
+        HIPCHK(dalloc(&B.part3f, (size_t)pip3b_nslices() * 37 * B.tcap));
+        B.fcap2 = B.dcap;
+        B.fcap3 = B.tcap;
+    }
     const int nblk = (int)((B.dcap + 127) / 128 + (B.tcap + 127) / 128 + (B.natms + 255) / 256 + 8);
     if (nblk > B.nblk_cap) {
         dfree(B.blk);
@@ -372,6 +384,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3); dfree(B.s3);
     dfree(B.is_water); dfree(B.nbr2l); dfree(B.cnt2l); dfree(B.kmask); dfree(B.rowoff); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
     dfree(B.cs2); dfree(B.cs3);
+    dfree(B.xv2f); dfree(B.part2f); dfree(B.cs2f); dfree(B.xv3f); dfree(B.part3f); dfree(B.cs3f);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
     elec_release(c->el);
@@ -382,6 +395,27 @@ void mbgpu_destroy(mbgpu_ctx* c)
     if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+int mbgpu_set_precision(mbgpu_ctx* c, int mode)
+{
+    if (!c) return fail(-1, "null context");
+    if (mode < 0 || mode > 3)
+        return fail(-1, "precision mode %d: bit 0 = FP32 2B polynomial, bit 1 = FP32 3B polynomial", mode);
+    MbBuffers& B = c->B;
+    if (mode != 0 && !B.cs2f) {
+        const int l2 = pip2b_stream_len(), l3 = pip3b_stream_len();
+        std::vector<float> f2(l2), f3(l3);
+        for (int i = 0; i < l2; ++i) f2[i] = (float)pip2b_stream_values()[i];
+        for (int i = 0; i < l3; ++i) f3[i] = (float)pip3b_stream_values()[i];
+        HIPCHK(dalloc(&B.cs2f, l2));
+        HIPCHK(dalloc(&B.cs3f, l3));
+        HIPCHK(hipMemcpy(B.cs2f, f2.data(), sizeof(float) * l2, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(B.cs3f, f3.data(), sizeof(float) * l3, hipMemcpyHostToDevice));
+    }
+    B.mixed = mode;
+    if (mode != 0 && B.dcap > 0 && ensure_clusters(c, B.dcap, B.tcap)) return -1;
+    return 0;
 }
 
 int mbgpu_mbpol_forces_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
@@ -543,14 +577,18 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* c, int imcon, const double cell[9]
     HIPCHK(hipMemcpyAsync(E.x, x_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(E.y, y_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(E.z, z_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(E.dip, dip_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+    if (dip_dev)
+        HIPCHK(hipMemcpyAsync(E.dip, dip_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+    else
+        el_launch_zero(E.dip, 3L * n, c->stream);   // start from zero dipoles
     int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, p->solver, c->stream, err);
     if (rc) return fail(rc, "%s", err.c_str());
     const int nr = 3 * nw;
     HIPCHK(hipMemcpyAsync(x_dev + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(y_dev + nr, E.y + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(z_dev + nr, E.z + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(dip_dev, E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+    if (dip_dev)
+        HIPCHK(hipMemcpyAsync(dip_dev, E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
     if (f_dev) el_launch_add(f_dev, E.fsite, 3L * n, c->stream);
     HIPCHK(hipStreamSynchronize(c->stream));
     const double* o = c->el.out;

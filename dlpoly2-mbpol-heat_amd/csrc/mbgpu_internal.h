@@ -12,6 +12,9 @@
 // xv: variables SoA [nvars][nc]; part: SoA [nslices][nvars+1][nc]; cs: coefficient stream.
 void launch_pip2b_f64(const double* xv, double* part, int nc, const double* cs, hipStream_t s);
 void launch_pip3b_f64(const double* xv, double* part, int nc, const double* cs, hipStream_t s);
+// FP32 instantiations of the same code (mixed-precision mode)
+void launch_pip2b_f32(const float* xv, float* part, int nc, const float* cs, hipStream_t s);
+void launch_pip3b_f32(const float* xv, float* part, int nc, const float* cs, hipStream_t s);
 int pip2b_nslices();
 int pip3b_nslices();
 int pip2b_stream_len();
@@ -56,6 +59,12 @@ struct MbBuffers {
     double* fout = nullptr;            // [3*natms] MB-pol forces of the last call
     // coefficient streams
     double *cs2 = nullptr, *cs3 = nullptr;
+    // mixed precision: FP32 variables, slice partials and coefficient streams for the
+    // polynomial kernels (the slice sums, chain rule and accumulation stay FP64)
+    int mixed = 0;
+    float *xv2f = nullptr, *part2f = nullptr, *cs2f = nullptr;
+    float *xv3f = nullptr, *part3f = nullptr, *cs3f = nullptr;
+    long fcap2 = 0, fcap3 = 0;     // cluster capacities of the FP32 buffers
 };
 
 // ------------------------------------------------------------------ electrostatics ----

@@ -420,8 +420,9 @@ __global__ void k_gather3(int nt, const int* __restrict__ trimers, const double*
     }
 }
 
-// 2B variables (mbpol-2b-poly.cpp:358-445): SoA [31][nd]
-__global__ void k_setup2(int nd, const double* __restrict__ clus, double* __restrict__ xv)
+// 2B variables (mbpol-2b-poly.cpp:358-445): SoA [31][nd]; xvf (may be NULL) receives the
+// same values rounded to FP32 for the mixed-precision polynomial
+__global__ void k_setup2(int nd, const double* __restrict__ clus, double* __restrict__ xv, float* __restrict__ xvf)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= nd) return;
@@ -436,7 +437,9 @@ __global__ void k_setup2(int nd, const double* __restrict__ clus, double* __rest
                      dz = s[3 * v.s1 + 2] - s[3 * v.s2 + 2];
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         const double e = exp(c2_k[v.k] * ((double)v.d0 - r));
-        xv[(long)n * nd + d] = v.kind ? e * (1.0 / r) : e;
+        const double val = v.kind ? e * (1.0 / r) : e;
+        xv[(long)n * nd + d] = val;
+        if (xvf) xvf[(long)n * nd + d] = (float)val;
     }
 }
 
@@ -564,8 +567,8 @@ __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* 
     block_reduce_store(red, 8, blk + 8 * blockIdx.x);
 }
 
-// 3B variables (mbpol-3b-poly.cpp:170-206): SoA [36][nt]
-__global__ void k_setup3(int nt, const double* __restrict__ clus, double* __restrict__ xv)
+// 3B variables (mbpol-3b-poly.cpp:170-206): SoA [36][nt]; xvf as in k_setup2
+__global__ void k_setup3(int nt, const double* __restrict__ clus, double* __restrict__ xv, float* __restrict__ xvf)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
@@ -578,7 +581,9 @@ __global__ void k_setup3(int nt, const double* __restrict__ clus, double* __rest
         const double dx = w[3 * v.a1] - w[3 * v.a2], dy = w[3 * v.a1 + 1] - w[3 * v.a2 + 1],
                      dz = w[3 * v.a1 + 2] - w[3 * v.a2 + 2];
         const double dd = sqrt(dx * dx + dy * dy + dz * dz);
-        xv[(long)n * nt + t] = exp(-c3_k[v.cls] * (dd - c3_r0[v.cls]));
+        const double val = exp(-c3_k[v.cls] * (dd - c3_r0[v.cls]));
+        xv[(long)n * nt + t] = val;
+        if (xvf) xvf[(long)n * nt + t] = (float)val;
     }
 }
 
@@ -840,6 +845,16 @@ __global__ void k_slice_sum(long n, int nsl, double* __restrict__ part)
     part[i] = s;
 }
 
+// FP64 sum of the nsl FP32 slice partials of every (variable, cluster), in slice order.
+__global__ void k_slice_sum_f(long n, int nsl, const float* __restrict__ pf, double* __restrict__ part)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = (double)pf[i];
+    for (int q = 1; q < nsl; ++q) s += (double)pf[q * n + i];
+    part[i] = s;
+}
+
 // Zero the per-atom outputs of the sites that are not water atoms (M sites).
 __global__ void k_zero_other_sites(int natms, const unsigned char* __restrict__ is_water, double* __restrict__ fout,
                                    double* __restrict__ ae, double* __restrict__ hw)
@@ -933,10 +948,17 @@ int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipSt
     if (nd <= 0) return 0;
     const int nb = (nd + 127) / 128;
     if (gather) hipLaunchKernelGGL(k_gather2, dim3(nb), dim3(128), 0, s, nd, B.dimers, B.mon, g, B.clus2);
-    hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2);
-    launch_pip2b_f64(B.xv2, B.part2, nd, B.cs2, s);
     const long n = 32L * nd;
-    hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2);
+    if (B.mixed & 1) {
+        hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2, B.xv2f);
+        launch_pip2b_f32(B.xv2f, B.part2f, nd, B.cs2f, s);
+        hipLaunchKernelGGL(k_slice_sum_f, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2f,
+                           B.part2);
+    } else {
+        hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2, nullptr);
+        launch_pip2b_f64(B.xv2, B.part2, nd, B.cs2, s);
+        hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2);
+    }
     hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, 1, B.clus2, B.xv2, B.part2, B.e2, B.g2, B.blk,
                        gather ? B.dimers : nullptr, B.list, B.x, B.y, B.z, B.s2);
     return (int)hipGetLastError();
@@ -949,10 +971,17 @@ int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gath
     const int nb = (nt + 127) / 128;
     const int nb2 = (nd + 127) / 128;
     if (gather) hipLaunchKernelGGL(k_gather3, dim3(nb), dim3(128), 0, s, nt, B.trimers, B.mon, g, B.clus3);
-    hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3);
-    launch_pip3b_f64(B.xv3, B.part3, nt, B.cs3, s);
     const long n = 37L * nt;
-    hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3);
+    if (B.mixed & 2) {
+        hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3, B.xv3f);
+        launch_pip3b_f32(B.xv3f, B.part3f, nt, B.cs3f, s);
+        hipLaunchKernelGGL(k_slice_sum_f, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3f,
+                           B.part3);
+    } else {
+        hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3, nullptr);
+        launch_pip3b_f64(B.xv3, B.part3, nt, B.cs3, s);
+        hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3);
+    }
     hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, 1, B.clus3, B.xv3, B.part3, B.e3, B.g3,
                        B.blk + 8L * nb2, gather ? B.trimers : nullptr, B.list, B.x, B.y, B.z, B.s3);
     return (int)hipGetLastError();

@@ -1,0 +1,119 @@
+"""Device-resident force evaluations of MB-pol water boxes, alone or several at once.
+
+A `Slot` owns one library context (its own HIP stream and device buffers) and the
+device-resident inputs of one configuration: positions (with the M sites), the water list,
+velocities and per-atom kinetic energies.  `Slot.evaluate()` runs one complete force
+evaluation in the order of the reference's force routine (dlpoly_cmd.f:2287-2504 with
+forces.f): MB-pol 2B/3B, the polarizable Ewald electrostatics with induced dipoles solved
+from zero, the monomer, dispersion and long-range terms, and the heat flux over all of
+them.  Forces stay on the device; the scalars come back to the host.
+
+`run_concurrently(slots, steps)` drives several slots from one host thread each, so that
+their kernels overlap on the GPU (each context launches on its own stream).  A context's
+results do not depend on what runs beside it: every kernel sums in a fixed order.
+"""
+import threading
+import time
+
+import numpy as np
+
+from . import elec_params as _ep
+from . import water_params as _wp
+from .mbgpu import Context, ElecParams, WaterParams
+
+ENGUNIT = 418.4
+
+
+class Slot:
+    def __init__(self, box, precision=0, solver=1, elec=None, water=None, device="cuda:0"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        self.box = box
+        self.ctx = Context(box.nw, box.natms)
+        self.ctx.set_precision(precision)
+        self.precision = precision
+        self.eparams = ElecParams.from_params(elec or _ep.for_box(box.L), distribute_msite=1, solver=solver)
+        self.wparams = WaterParams.from_params(water or _wp.for_box(box.L))
+        self.stream = torch.cuda.Stream(device=self.dev)
+        self._alloc(box)
+        self.load(box)
+
+    def _alloc(self, box):
+        t = self.torch
+        n, nr = box.natms, box.nreal
+        f64 = dict(dtype=t.float64, device=self.dev)
+        self.x, self.y, self.z = (t.zeros(n, **f64) for _ in range(3))
+        self.lst = t.zeros(nr, dtype=t.int32, device=self.dev)
+        self.v = t.zeros(3, nr, **f64)
+        self.ek = t.zeros(nr, **f64)
+
+    def load(self, box):
+        """Upload a configuration of the same size (positions, list, velocities)."""
+        t = self.torch
+        if box.nw != self.box.nw:
+            raise ValueError("a slot holds configurations of one size")
+        self.box = box
+        nr = box.nreal
+        with t.cuda.stream(self.stream):
+            for dst, src in ((self.x, box.x), (self.y, box.y), (self.z, box.z)):
+                dst.copy_(t.as_tensor(np.ascontiguousarray(src), dtype=t.float64))
+            self.lst.copy_(t.as_tensor(np.ascontiguousarray(box.listttm2), dtype=t.int32))
+            self.v.copy_(t.as_tensor(np.ascontiguousarray(box.vel[:nr].T), dtype=t.float64))
+            ek = 0.5 * box.mass[:nr] * (box.vel[:nr] ** 2).sum(1)
+            self.ek.copy_(t.as_tensor(ek, dtype=t.float64))
+        self.stream.synchronize()
+
+    def evaluate(self):
+        """One complete force evaluation; returns a dict of scalars (internal units).
+        Everything runs on the context's stream: the induced dipoles start from zero inside
+        the library and the forces stay in the context (see forces())."""
+        b, c = self.box, self.ctx
+        x, y, z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+        e_mb, v_mb, s_mb = c.mbpol_forces_dev(b.imcon, b.cell, self.lst.data_ptr(), x, y, z)
+        e_el, v_el, s_el, it = c.polar_electrostatics_dev(b.imcon, b.cell, self.eparams, x, y, z, None)
+        e_w, v_w, s_w = c.water_terms_dev(b.imcon, b.cell, self.wparams, x, y, z)
+        jk, jp, jt = c.heat_flux_dev(self.v[0].data_ptr(), self.v[1].data_ptr(), self.v[2].data_ptr(),
+                                     self.ek.data_ptr())
+        return dict(eng=e_mb + e_el + float(np.sum(e_w)), eng_mbpol=e_mb, eng_elec=e_el, eng_water=np.array(e_w),
+                    vir=v_mb + v_el + float(np.sum(v_w)), stress=np.array(s_mb) + s_el + s_w, dipole_iterations=it,
+                    jkin=np.array(jk), jpot=np.array(jp), jtot=np.array(jt))
+
+    def forces(self):
+        """Total forces of the last evaluation on the real atoms, (3*nw, 3), internal units."""
+        nr = self.box.nreal
+        _, _, fm = self.ctx.decomposition()
+        fe = self.ctx.elec_details()["f_sites"]
+        fw = self.ctx.water_details()["f"]
+        return fm[:nr] + fe[:nr] + fw
+
+
+def run_concurrently(slots, steps, warmup=0):
+    """Evaluate every slot `steps` times, one host thread per slot.  Returns (seconds,
+    per-slot lists of results); the timed region starts after `warmup` evaluations."""
+    import torch
+    for s in slots:
+        for _ in range(warmup):
+            s.evaluate()
+    torch.cuda.synchronize()
+    out = [[] for _ in slots]
+    errors = []
+
+    def loop(i, s):
+        try:
+            for _ in range(steps):
+                out[i].append(s.evaluate())
+        except Exception as e:          # surfaced after the join
+            errors.append(e)
+
+    threads = [threading.Thread(target=loop, args=(i, s)) for i, s in enumerate(slots)]
+    t0 = time.perf_counter()
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    return dt, out

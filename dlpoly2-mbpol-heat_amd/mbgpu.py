@@ -41,6 +41,7 @@ _SIGS = {
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp]),
     "mbgpu_get_decomposition": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "mbgpu_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mbgpu_get_lists": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _ip,
                                        ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "mbgpu_heat_flux": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -236,6 +237,10 @@ class Context:
                                                    _p(stress))
         _check(rc, "mbgpu_mbpol_forces_dev")
         return e.value, v.value, stress
+
+    def set_precision(self, mode):
+        """Bit mask: 0 = FP64; bit 0 / bit 1 = the 2B / 3B polynomial in FP32 (FP64 accumulation)."""
+        _check(load_library().mbgpu_set_precision(self._h, int(mode)), "mbgpu_set_precision")
 
     def decomposition(self):
         ae = np.zeros(self.natms)

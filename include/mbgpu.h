@@ -92,6 +92,13 @@ int mbgpu_mbpol_forces_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], int 
  *   forces: the MB-pol forces alone, 3*natms interleaved. */
 int mbgpu_get_decomposition(mbgpu_ctx* ctx, double* atomic_energy_mbpol, double* heat_w, double* forces);
 
+/* Precision of the MB-pol polynomials in this context, a bit mask: 0 = FP64 (default,
+ * the reference's arithmetic); bit 0 = the 2B polynomial, bit 1 = the 3B polynomial in
+ * FP32 (values and gradients; variables rounded from FP64), with the slice sums, switches,
+ * chain rule, per-atom accumulation, virial and stress in FP64.  No counterpart in the reference; the electrostatics, water terms and
+ * heat flux are FP64 in both modes. */
+int mbgpu_set_precision(mbgpu_ctx* ctx, int mode);
+
 /* Dimer / trimer sets of the last call (0-based water indices; dimers i<j, trimers
  * i<j<k, in the order evaluated).  *nd, *nt receive the counts; arrays may be NULL. */
 int mbgpu_get_lists(mbgpu_ctx* ctx, int* dimers, long dcap, long* nd, int* trimers, long tcap, long* nt);
@@ -168,7 +175,8 @@ int mbgpu_polar_electrostatics(mbgpu_ctx* ctx, int imcon, const double cell[9], 
                                double* vircpe, double stress[9], int* niter);
 
 /* Same with device pointers: x_dev..z_dev (4*nw, M entries written), dip_dev (3 per site,
- * interleaved, in/out), f_dev (3 per site, interleaved, +=; may be NULL). */
+ * interleaved, in/out; NULL = start from zero dipoles and keep the converged ones inside
+ * the context), f_dev (3 per site, interleaved, +=; may be NULL). */
 int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_elec_params* p,
                                    double* x_dev, double* y_dev, double* z_dev, double* dip_dev, double* f_dev,
                                    double* engcpe, double* vircpe, double stress[9], int* niter);
