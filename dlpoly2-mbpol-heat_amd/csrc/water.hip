@@ -35,7 +35,7 @@ constexpr double PS_HDMASS = 3670.483031;
 constexpr double PS_ENGCNV = 1.19627;              // module.f90:486
 constexpr double PS_F5Z = 0.99967788500000, PS_FBASIS = 0.15860145369897;
 constexpr double PS_FCORE = -1.6351695982132, PS_FREST = 1.0;
-constexpr double PS_EOFFSET = 2.0384848702093006e-6 * 2625.50 / 100.0;   // ps_type_couple.f90:239
+constexpr double PS_EOFFSET = 2.0384848702093006e-6 * 2625.50 / 100.0;   // ps_type_couple.f90:251
 constexpr double PI_D = 3.141592653589793;
 
 __device__ __forceinline__ double wsum(double v)

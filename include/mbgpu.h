@@ -222,7 +222,7 @@ typedef struct {
  *   stress  : += bond, PS and dispersion stress (the caller adds -virlrc/3 to the diagonal,
  *             as dlpoly_cmd.f:2572-2574 does).
  * Per-atom energies (bond halves, pterm/3, dispersion halves; the long-range correction
- * has no per-atom share, its per-atom update is disabled in lrcorrect.f:215-222) and the
+ * has no per-atom share, its per-atom update is commented out in lrcorrect.f:226-232) and the
  * force-matrix contractions stay on the device and enter mbgpu_heat_flux automatically. */
 int mbgpu_water_terms(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_water_params* p,
                       const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
