@@ -94,6 +94,46 @@ def cpu_baseline(box):
                       f"flux, {dt:.1f} s"}
 
 
+def serve_leg(pkg, args):
+    """Config-3 style serving mix through serve.Server; returns latency and throughput."""
+    import numpy as np
+    serve = importlib.import_module("dlpoly2-mbpol-heat_amd.serve")
+    wb, ep, wp = pkg.waterbox, pkg.elec_params, pkg.water_params
+    rng = np.random.default_rng(9)
+    base = wb.box_256()
+    beads = [wb.displaced(base, 0.05, seed=500 + i) for i in range(32)]
+    sizes = np.exp(rng.uniform(np.log(3), np.log(1024), 24)).astype(int)
+    clusters = [wb.make_cluster(int(n), seed=600 + i) for i, n in enumerate(sizes)]
+    jobs = []
+    for i in range(args.serve_requests):
+        if rng.random() < 0.5:
+            b = beads[i % len(beads)]
+            jobs.append((b, ep.for_box(b.L), wp.for_box(b.L), "bead"))
+        else:
+            c = clusters[int(rng.integers(len(clusters)))]
+            jobs.append((c, ep.for_cluster(c.L), wp.for_cluster(c.L), "cluster"))
+    srv = serve.Server(workers=max(1, args.batch), solver=1 if args.solver == "cg" else 0)
+    # warm the pool: one request of each size, untimed
+    warm = [srv.submit(*j) for j in {j[0].nw: j for j in jobs}.values()]
+    for r in warm:
+        r.done.wait()
+    reqs, wall = serve.run_poisson(srv, jobs, args.serve_rate, seed=3)
+    srv.close()
+    out = serve.latency_summary(reqs)
+    for tag in ("bead", "cluster"):
+        sub = [r for r in reqs if r.tag == tag]
+        s = serve.latency_summary(sub)
+        if s:
+            out[f"{tag}_p50_ms"], out[f"{tag}_p99_ms"] = s["p50_ms"], s["p99_ms"]
+    out.update(arrival_rate=args.serve_rate, wall_s=wall, evals_per_s=len(reqs) / wall, workers=max(1, args.batch),
+               slots_created=srv.slots_created, slots_evicted=srv.slots_evicted,
+               cluster_sizes=[int(n) for n in sizes],
+               what="32 PIMD beads x 256 H2O (L=19.72 A, N(0,0.05 A) displacements) and isolated clusters "
+                    "n in [3,1024] log-uniform (random sequential addition at 0.997 g/cm^3, in a periodic box "
+                    "of side 2R+12 A), half and half, Poisson arrivals; latency = arrival to forces on the host")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +144,9 @@ def main():
                     help="induced-dipole solver: the reference's SOR or conjugate gradients to the same test")
     ap.add_argument("--batch", type=int, default=16,
                     help="configurations evaluated concurrently in the batched legs (0 skips them)")
+    ap.add_argument("--serve-requests", type=int, default=240,
+                    help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, metavar="MIB", default=0,
                     help="only run a plain device copy of MIB MiB five times (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -216,6 +259,12 @@ def main():
         batched["mixed"]["dE_per_molecule"] = float((rmx["eng"] - r64["eng"]) / 418.4 / boxes[0].nw)
         batched["mixed"]["rel_dJ"] = float(np.abs(rmx["jtot"] - r64["jtot"]).max() / np.abs(r64["jtot"]).max())
         batched["mixed"]["rel_dvir"] = float(abs(rmx["vir"] - r64["vir"]) / abs(r64["vir"]))
+    # serving leg: the mixed stream of PIMD beads (256 waters, L = 19.72 A, one configuration
+    # displaced by N(0, 0.05 A) per bead) and isolated clusters (n log-uniform in [3, 1024]),
+    # Poisson arrivals, continuous batching over --batch workers
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_leg(pkg, args)
     # headline: the fastest leg whose forces are within 1e-5 kcal/mol/A RMS of FP64
     lead = ("fp64 single", single, ms, "f64")
     if "fp64" in batched and batched["fp64"]["evals_per_s"] > single:
@@ -227,6 +276,8 @@ def main():
     for name, b in batched.items():
         legs[f"{name}_batched_evals_per_s"] = b["evals_per_s"]
         legs[f"{name}_batched_speedup_vs_fp64_single"] = b["evals_per_s"] / single
+    if serving is not None:
+        legs["serving"] = serving
     if "mixed" in batched:
         m = batched["mixed"]
         legs["mixed_vs_fp64"] = {"rms_dF_kcal_mol_A": m["rms_dF"], "max_dF_kcal_mol_A": m["max_dF"],

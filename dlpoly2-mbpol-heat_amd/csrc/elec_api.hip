@@ -21,9 +21,7 @@ namespace {
 template <typename T>
 hipError_t dalloc(T** p, size_t n)
 {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    return hipMalloc((void**)p, n * sizeof(T));
+    return mbgpu_alloc(p, n);
 }
 
 template <typename T>

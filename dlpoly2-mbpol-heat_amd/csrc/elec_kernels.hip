@@ -813,6 +813,9 @@ __global__ __launch_bounds__(64) void k_el_emu(ElecDev P, const int* __restrict_
             for (int b = a + 1; b < 4; ++b, ++pr) {
                 if (a != si && b != si) continue;
                 const int j = (a == si) ? s[b] : s[a];
+                // a partner without an induced dipole (the M site) contributes nothing, and
+                // its entries of the vector are not part of the solve: skip it
+                if (!(site_polr2(P, j) > 1.e-6)) continue;
                 const double* dt = tin + 36 * w + 6 * pr;
                 const double mx = dip[3 * j] * R4PIE0, my = dip[3 * j + 1] * R4PIE0, mz = dip[3 * j + 2] * R4PIE0;
                 e[0] += dt[0] * mx + dt[1] * my + dt[2] * mz;

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -21,6 +22,22 @@ int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, d
 int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const double* vz, const double* ekin,
                         const double* epot, const double* epot2, const double* w, const double* w2,
                         const double* sew, double* blk, int nblk_cap, double* out, hipStream_t s);
+
+static std::atomic<long> g_alloc_serial{0};
+
+long mbgpu_alloc_serial() { return g_alloc_serial.fetch_add(1); }
+
+int mbgpu_poison_this(long serial)
+{
+    const char* v = getenv("MBGPU_POISON");
+    if (!v || !v[0] || v[0] == '0') return 0;
+    long a = 0, b = 0;
+    if (v[0] == '!' && sscanf(v + 1, "%ld:%ld", &a, &b) == 2) return (serial >= a && serial < b) ? 2 : 1;
+    if (sscanf(v, "%ld:%ld", &a, &b) == 2) return (serial >= a && serial < b) ? 1 : 0;
+    return 1;
+}
+
+extern "C" long mbgpu_debug_alloc_serial(void) { return g_alloc_serial.load(); }
 
 namespace {
 
@@ -46,9 +63,7 @@ int fail(int code, const char* fmt, ...)
 template <typename T>
 hipError_t dalloc(T** p, size_t n)
 {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    return hipMalloc((void**)p, n * sizeof(T));
+    return mbgpu_alloc(p, n);
 }
 
 template <typename T>
@@ -748,6 +763,16 @@ int mbgpu_get_water_details(mbgpu_ctx* c, double* e_real, double* w_real, double
         for (int i = 0; i < nr; ++i)
             for (int k = 0; k < 3; ++k) f_real[3 * i + k] = f[(size_t)k * nr + i];
     }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_get_dipoles(mbgpu_ctx* c, double* dip)
+{
+    if (!c) return fail(-1, "null context");
+    if (!c->el.ready) return fail(-1, "no completed electrostatics evaluation in this context");
+    if (dip)
+        HIPCHK(hipMemcpyAsync(dip, c->el.E.dip, sizeof(double) * 3 * 4 * c->B.nw, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }

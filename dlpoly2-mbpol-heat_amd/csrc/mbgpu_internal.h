@@ -2,11 +2,37 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <stdlib.h>
+
 #include <string>
+#include <type_traits>
 
 #include "../../include/mbgpu.h"
 
 #include "mbgpu_dev.h"
+
+// Device allocation of every buffer of the library.  With MBGPU_POISON=1 in the environment
+// each new floating-point buffer is filled with 0xff bytes (a NaN in every element), so that a
+// kernel that reads an element it did not write during the same evaluation yields NaN
+// instead of silently using whatever the memory held.  MBGPU_POISON=a:b poisons only the
+// allocations numbered a <= serial < b (mbgpu_debug_alloc_serial counts them) and
+// MBGPU_POISON=!a:b every allocation but those, which are zeroed: ways to find the buffer.
+long mbgpu_alloc_serial();     // allocations made so far (every buffer, in order)
+int mbgpu_poison_this(long serial);   // 0: leave, 1: fill with 0xff, 2: fill with zeros
+
+template <typename T>
+inline hipError_t mbgpu_alloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    const long serial = mbgpu_alloc_serial();
+    if constexpr (std::is_floating_point<T>::value) {
+        const int how = e == hipSuccess ? mbgpu_poison_this(serial) : 0;
+        if (how) e = hipMemset(*p, how == 1 ? 0xff : 0, n * sizeof(T));
+    }
+    return e;
+}
 
 // Launch wrappers for the generated polynomial kernels (pip2b.hip / pip3b.hip).
 // xv: variables SoA [nvars][nc]; part: SoA [nslices][nvars+1][nc]; cs: coefficient stream.

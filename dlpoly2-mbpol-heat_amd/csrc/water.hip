@@ -394,8 +394,7 @@ __global__ __launch_bounds__(1024) void k_wt_reduce(int nw, int nr, const double
 template <typename T>
 hipError_t dalloc(T** p, size_t n)
 {
-    *p = nullptr;
-    return hipMalloc((void**)p, (n ? n : 1) * sizeof(T));
+    return mbgpu_alloc(p, n);
 }
 
 template <typename T>

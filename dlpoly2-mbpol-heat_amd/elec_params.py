@@ -48,3 +48,14 @@ def for_box(L):
         k = 16 if L < 30 else 24
         return ElecParams(rcut=9.0, alpha=0.3, kmax=(k, k, k))
     return ElecParams(rcut=6.0, alpha=0.55, kmax=(8, 8, 8))
+
+
+def for_cluster(L):
+    """Ewald parameters for an isolated cluster in a cubic box of side L (waterbox.make_cluster):
+    rcut = min(9, L/2 - 0.05) A, alpha = 3.2 / rcut (erfc(3.2) = 6e-6 at the cutoff) and
+    kmax = ceil(1.3 alpha L) (exp(-(pi kmax / (alpha L))^2) < 1e-7), at most 25 per axis."""
+    import math
+    rcut = min(9.0, 0.5 * L - 0.05)
+    alpha = 3.2 / rcut
+    k = min(25, int(math.ceil(1.3 * alpha * L)))
+    return ElecParams(rcut=rcut, alpha=alpha, kmax=(k, k, k))

@@ -33,11 +33,17 @@ class Slot:
         self.ctx = Context(box.nw, box.natms)
         self.ctx.set_precision(precision)
         self.precision = precision
-        self.eparams = ElecParams.from_params(elec or _ep.for_box(box.L), distribute_msite=1, solver=solver)
-        self.wparams = WaterParams.from_params(water or _wp.for_box(box.L))
+        self.solver = solver
+        self.set_params(elec or _ep.for_box(box.L), water or _wp.for_box(box.L))
         self.stream = torch.cuda.Stream(device=self.dev)
         self._alloc(box)
         self.load(box)
+
+    def set_params(self, elec, water):
+        """Electrostatics (elec_params.ElecParams) and water-term (water_params.WaterParams)
+        parameters of the following evaluations."""
+        self.eparams = ElecParams.from_params(elec, distribute_msite=1, solver=self.solver)
+        self.wparams = WaterParams.from_params(water)
 
     def _alloc(self, box):
         t = self.torch
@@ -76,6 +82,7 @@ class Slot:
         jk, jp, jt = c.heat_flux_dev(self.v[0].data_ptr(), self.v[1].data_ptr(), self.v[2].data_ptr(),
                                      self.ek.data_ptr())
         return dict(eng=e_mb + e_el + float(np.sum(e_w)), eng_mbpol=e_mb, eng_elec=e_el, eng_water=np.array(e_w),
+                    vir_water=np.array(v_w),
                     vir=v_mb + v_el + float(np.sum(v_w)), stress=np.array(s_mb) + s_el + s_w, dipole_iterations=it,
                     jkin=np.array(jk), jpot=np.array(jp), jtot=np.array(jt))
 
@@ -86,6 +93,28 @@ class Slot:
         fe = self.ctx.elec_details()["f_sites"]
         fw = self.ctx.water_details()["f"]
         return fm[:nr] + fe[:nr] + fw
+
+
+    def water_dipoles(self):
+        """Total dipole of every water of the last evaluation (e*Angstrom): the site charges
+        times the site positions relative to the oxygen (minimum image) plus the induced
+        dipoles of the four sites."""
+        nw = self.box.nw
+        chge = self.ctx.elec_details()["chge"]
+        dip = self.ctx.dipoles()
+        self.torch.cuda.synchronize(self.dev)
+        xyz = np.stack([self.x.cpu().numpy(), self.y.cpu().numpy(), self.z.cpu().numpy()], 1)
+        cell = np.asarray(self.box.cell, dtype=float).reshape(3, 3)     # rows: cell vectors
+        inv = np.linalg.inv(cell)
+        out = np.zeros((nw, 3))
+        for w in range(nw):
+            sites = [3 * w, 3 * w + 1, 3 * w + 2, 3 * nw + w]
+            o = xyz[3 * w]
+            for s in sites:
+                d = xyz[s] - o
+                d = d - np.rint(d @ inv) @ cell
+                out[w] += chge[s] * d + dip[s]
+        return out
 
 
 def run_concurrently(slots, steps, warmup=0):

@@ -56,6 +56,7 @@ _SIGS = {
                                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                       ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp, _ip]),
     "mbgpu_get_elec_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mbgpu_get_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "mbgpu_water_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, _dp, _dp, _dp, _dp,
                                          _dp, _dp, _dp, _dp, _dp]),
     "mbgpu_water_terms_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, ctypes.c_void_p,
@@ -362,6 +363,12 @@ class Context:
         e, w, f = np.zeros(nr), np.zeros(9 * nr), np.zeros(3 * nr)
         _check(load_library().mbgpu_get_water_details(self._h, _p(e), _p(w), _p(f)), "mbgpu_get_water_details")
         return dict(atomic_energy=e, heat_w=w.reshape(nr, 3, 3), f=f.reshape(nr, 3))
+
+    def dipoles(self):
+        """Converged induced dipoles of the last electrostatics call, (4*nw, 3)."""
+        d = np.zeros(12 * self.nw)
+        _check(load_library().mbgpu_get_dipoles(self._h, _p(d)), "mbgpu_get_dipoles")
+        return d.reshape(4 * self.nw, 3)
 
     def elec_details(self):
         """Results of the last electrostatics call (see mbgpu_get_elec_details)."""

@@ -64,3 +64,9 @@ def for_box(L):
     """Cutoffs for a cubic box of side L (Angstrom): 9 A, or 6 A below 18 A."""
     r = 9.0 if L >= 18.0 else 6.0
     return WaterParams(rvdw=r, rcut=r)
+
+
+def for_cluster(L):
+    """Dispersion cutoff for an isolated cluster in a cubic box of side L: min(9, L/2 - 0.05)."""
+    r = min(9.0, 0.5 * L - 0.05)
+    return WaterParams(rvdw=r, rcut=r)

@@ -134,3 +134,66 @@ def box_256(seed=3):
 def box_2048(seed=4):
     """2048 of 2197 sites of a 13^3 lattice, L = 39.43 A."""
     return make_waterbox(2048, 39.43, 13, seed)
+
+
+RHO_WATER = 0.997 * 6.02214076e23 / 18.01528 * 1e-24   # molecules per A^3 at 0.997 g/cm^3
+
+
+def make_cluster(n, seed, margin=12.0, dmin=2.6, temperature=300.0):
+    """An isolated cluster of n waters built by random sequential addition: oxygens placed
+    uniformly in a sphere holding n molecules at 0.997 g/cm^3, rejected below an O-O distance
+    of dmin, each molecule a perturbed monomer with a random orientation as in
+    make_waterbox.  The cluster sits in a cubic periodic box of side 2R + margin (the
+    library's images need a periodic cell; with the margin no image is within the 2B/3B
+    cutoffs of another, and the Ewald sums of the box stand in for the open boundary)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    R = (3.0 * n / (4.0 * np.pi * RHO_WATER)) ** (1.0 / 3.0)
+    L = 2.0 * R + margin
+    centre = np.full(3, 0.5 * L)
+    oxy = []
+    tries = 0
+    while len(oxy) < n:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("random sequential addition did not place every molecule")
+        u = rng.uniform(-1.0, 1.0, 3)
+        if u @ u > 1.0:
+            continue
+        o = centre + R * u
+        if oxy and np.min(np.linalg.norm(np.asarray(oxy) - o, axis=1)) < dmin:
+            continue
+        oxy.append(o)
+    real = np.zeros((3 * n, 3))
+    for m, o in enumerate(oxy):
+        r1 = R_OH + rng.normal(0.0, 0.02)
+        r2 = R_OH + rng.normal(0.0, 0.02)
+        th = np.deg2rad(THETA_HOH + rng.normal(0.0, 3.0))
+        h1 = np.array([r1 * np.sin(th / 2), 0.0, r1 * np.cos(th / 2)])
+        h2 = np.array([-r2 * np.sin(th / 2), 0.0, r2 * np.cos(th / 2)])
+        Rm = _rotation(rng)
+        real[3 * m] = o
+        real[3 * m + 1] = o + Rm @ h1
+        real[3 * m + 2] = o + Rm @ h2
+    mass_real = np.tile([MASS_O, MASS_H, MASS_H], n)
+    v = rng.normal(size=(3 * n, 3)) * np.sqrt(BOLTZ * temperature / mass_real)[:, None]
+    v -= (mass_real[:, None] * v).sum(0) / mass_real.sum()
+    xyz = np.zeros((4 * n, 3))
+    xyz[:3 * n] = real
+    xyz[3 * n:] = m_sites(real, n, L)
+    vel = np.zeros((4 * n, 3))
+    vel[:3 * n] = v
+    mass = np.zeros(4 * n)
+    mass[:3 * n] = mass_real
+    return WaterBox(n, L, xyz, vel, mass, np.arange(1, 3 * n + 1, dtype=np.int32))
+
+
+def displaced(box, sigma, seed):
+    """A copy of box with every atom displaced by N(0, sigma) per axis (a PIMD bead of the
+    serving mix); the M sites are rebuilt."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nw = box.nw
+    real = box.xyz[:3 * nw] + rng.normal(0.0, sigma, (3 * nw, 3))
+    xyz = np.zeros_like(box.xyz)
+    xyz[:3 * nw] = real
+    xyz[3 * nw:] = m_sites(real, nw, box.L)
+    return WaterBox(nw, box.L, xyz, box.vel.copy(), box.mass.copy(), box.listttm2.copy())

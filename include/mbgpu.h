@@ -191,6 +191,9 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[
  *   f_site: 3 per site, forces of the last call. */
 int mbgpu_get_elec_details(mbgpu_ctx* ctx, double* scal, double* chge, double* vesp, double* e_real,
                            double* w_real, double* s_real, double* f_site);
+/* Converged induced dipoles of the last electrostatics call: 3 per site, interleaved, 4*nw
+ * sites (the reference's dipx/y/z after iter; e*Angstrom). */
+int mbgpu_get_dipoles(mbgpu_ctx* ctx, double* dip);
 
 /* Parameters of the monomer, dispersion and long-range terms (FIELD inputs in the
  * reference; energies in the internal unit, 10 J/mol). */

@@ -81,3 +81,30 @@ def test_fp32_polynomials_error_envelope(waterbox, engine):
     assert 0.0 < rms < 0.05
     assert abs(rb["eng_elec"] - ra["eng_elec"]) <= 1e-6 * abs(ra["eng_elec"])
     assert np.array_equal(rb["eng_water"], ra["eng_water"])
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_no_read_of_unwritten_memory(pkg, waterbox, engine, solver, monkeypatch):
+    """Every floating-point buffer filled with NaN at allocation (MBGPU_POISON=1): a kernel
+    that read an element it did not write in the same evaluation would turn the results
+    into NaN.  Results must equal those of a context on zero-filled memory, bit for bit
+    (regression: the conjugate-gradient direction's M-site entries were read once)."""
+    cases = [(waterbox.box_64(), pkg.elec_params.for_box(12.42), pkg.water_params.for_box(12.42))]
+    c = waterbox.make_cluster(40, 3)
+    cases.append((c, pkg.elec_params.for_cluster(c.L), pkg.water_params.for_cluster(c.L)))
+    for box, e, w in cases:
+        out = []
+        for poison in ("0", "1"):
+            monkeypatch.setenv("MBGPU_POISON", poison)
+            s = engine.Slot(box, solver=solver, elec=e, water=w)
+            r = s.evaluate()
+            out.append((r, s.forces(), s.ctx.elec_details(), s.ctx.water_details(), s.ctx.decomposition()))
+            del s
+        (r0, f0, d0, w0, m0), (r1, f1, d1, w1, m1) = out
+        assert r1["eng"] == r0["eng"] and np.array_equal(f1, f0) and np.array_equal(r1["jtot"], r0["jtot"])
+        for k in ("atomic_energy", "heat_w", "s_ew1p", "chge", "vesp"):
+            assert np.array_equal(d1[k], d0[k]), k
+        for k in ("atomic_energy", "heat_w"):
+            assert np.array_equal(w1[k], w0[k]), k
+        for a, b in zip(m1, m0):
+            assert np.array_equal(a, b)
