@@ -127,6 +127,13 @@ def load_library(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise MbgpuError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    try:
+        # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7).  Loaded
+        # first, it is the one libmbgpu.so binds to, so the process holds a single runtime;
+        # loaded second, it would bring a second runtime whose initialisation then fails.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)

@@ -49,6 +49,12 @@ def same(r, f, ra, fa):
     assert np.array_equal(f, fa)
 
 
+SERVE_SKIP = ("concurrent serving of mixed sizes is switched off: the serving leg of bench.py (16 workers, "
+              "256-water beads and clusters up to 953 waters, run after the batched legs) once ended in an "
+              "illegal memory access whose cause has not been found")
+
+
+@pytest.mark.skip(reason=SERVE_SKIP)
 def test_mix_equals_alone(mods):
     serve, engine, *_ = mods
     jobs = jobs_mix(mods)
@@ -67,6 +73,7 @@ def test_mix_equals_alone(mods):
     assert srv.slots_created <= 4 + 4           # at most one slot per size per worker
 
 
+@pytest.mark.skip(reason=SERVE_SKIP)
 def test_eviction_under_memory_budget(mods):
     serve, engine, *_ = mods
     jobs = jobs_mix(mods)
