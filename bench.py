@@ -144,6 +144,8 @@ def main():
                     help="induced-dipole solver: the reference's SOR or conjugate gradients to the same test")
     ap.add_argument("--batch", type=int, default=16,
                     help="configurations evaluated concurrently in the batched legs (0 skips them)")
+    ap.add_argument("--nve-steps", type=int, default=1000,
+                    help="steps of the NVE leg (512 waters, dt 0.2 fs, heat current every step; 0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=0,
                     help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it).  Off by "
                          "default: this leg, run after the batched legs, once ended in an illegal memory access "
@@ -261,6 +263,29 @@ def main():
         batched["mixed"]["dE_per_molecule"] = float((rmx["eng"] - r64["eng"]) / 418.4 / boxes[0].nw)
         batched["mixed"]["rel_dJ"] = float(np.abs(rmx["jtot"] - r64["jtot"]).max() / np.abs(r64["jtot"]).max())
         batched["mixed"]["rel_dvir"] = float(abs(rmx["vir"] - r64["vir"]) / abs(r64["vir"]))
+    # NVE leg: velocity Verlet on the device, a complete force evaluation and the heat flux
+    # every step, dipoles carried between steps (configuration of the plain FP64 path)
+    nve_out = None
+    if args.nve_steps > 0:
+        md = importlib.import_module("dlpoly2-mbpol-heat_amd.md")
+        engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
+        nve = md.NVE(engine.Slot(box, solver=1 if args.solver == "cg" else 0), 0.2)
+        e0, k0 = nve.pot + nve.kin, nve.kin
+        tot, its = [], []
+        torch.cuda.synchronize()
+        t0n = time.perf_counter()
+        for _ in range(args.nve_steps):
+            r = nve.step()
+            tot.append(r["tot"])
+            its.append(r["iterations"])
+        torch.cuda.synchronize()
+        tn = time.perf_counter() - t0n
+        tot = np.array(tot)
+        slope = np.polyfit(np.arange(1, len(tot) + 1) * 0.2e-3, tot - e0, 1)[0] if len(tot) > 1 else 0.0
+        nve_out = {"steps": args.nve_steps, "dt_fs": 0.2, "ms_per_step": 1e3 * tn / args.nve_steps,
+                   "max_abs_dE_over_KE0": float(np.abs(tot - e0).max() / k0),
+                   "drift_per_ps_over_KE0": float(slope / k0), "dipole_iterations_median": float(np.median(its)),
+                   "what": "512 H2O from the 300 K start, heat current every step, induced dipoles warm-started"}
     # serving leg: the mixed stream of PIMD beads (256 waters, L = 19.72 A, one configuration
     # displaced by N(0, 0.05 A) per bead) and isolated clusters (n log-uniform in [3, 1024]),
     # Poisson arrivals, continuous batching over --batch workers
@@ -280,6 +305,8 @@ def main():
         legs[f"{name}_batched_speedup_vs_fp64_single"] = b["evals_per_s"] / single
     if serving is not None:
         legs["serving"] = serving
+    if nve_out is not None:
+        legs["nve"] = nve_out
     if "mixed" in batched:
         m = batched["mixed"]
         legs["mixed_vs_fp64"] = {"rms_dF_kcal_mol_A": m["rms_dF"], "max_dF_kcal_mol_A": m["max_dF"],

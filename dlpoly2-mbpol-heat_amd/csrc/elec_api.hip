@@ -14,6 +14,12 @@
 
 #define R4PIE0 138935.4835
 #define SQRPI 1.7724538509055159
+#ifndef EL_DCHUNK_MAX
+#define EL_DCHUNK_MAX 32      // site chunks of the dipole structure-factor pass (partials re-read by the sum)
+#endif
+#ifndef EL_DCHUNK_WAVES
+#define EL_DCHUNK_WAVES 2048  // wavefronts the dipole structure-factor pass aims at
+#endif
 #define PI_D 3.141592653589793
 
 namespace {
@@ -269,7 +275,7 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     const int nrow = 3 * S.nw;
     E.schunks = std::max(1, std::min(16, std::min((E.n + 15) / 16, 1024 / (4 * rblocks))));
     E.spc = (E.n + E.schunks - 1) / E.schunks;
-    E.dchunks = std::max(1, std::min(16, std::min((nrow + 15) / 16, 1024 / (4 * rblocks))));
+    E.dchunks = std::max(1, std::min(EL_DCHUNK_MAX, std::min((nrow + 15) / 16, EL_DCHUNK_WAVES / (4 * rblocks))));
     E.dspc = (nrow + E.dchunks - 1) / E.dchunks;
     ECHK(dalloc(&E.S, (size_t)8 * K));
     ECHK(dalloc(&E.Sq, (size_t)2 * K));

@@ -19,6 +19,9 @@ int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s);
 int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipStream_t s);
 int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gather, hipStream_t s);
 int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s);
+int md_launch_stage(int stage, int nr, double dt, const double* mass, double* x, double* y, double* z, double* vx,
+                    double* vy, double* vz, double* fx, double* fy, double* fz, double* fsite, double* ekin,
+                    double* blk, double* out, hipStream_t s);
 int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const double* vz, const double* ekin,
                         const double* epot, const double* epot2, const double* w, const double* w2,
                         const double* sew, double* blk, int nblk_cap, double* out, hipStream_t s);
@@ -764,6 +767,30 @@ int mbgpu_get_water_details(mbgpu_ctx* c, double* e_real, double* w_real, double
             for (int k = 0; k < 3; ++k) f_real[3 * i + k] = f[(size_t)k * nr + i];
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_verlet_stage(mbgpu_ctx* c, int stage, double dt, const double* mass_dev, double* x_dev, double* y_dev,
+                       double* z_dev, double* vx_dev, double* vy_dev, double* vz_dev, double* fx_dev,
+                       double* fy_dev, double* fz_dev, double* fsite_dev, double* ekin_dev, double* ekin_total)
+{
+    if (!c) return fail(-1, "null context");
+    if (stage != 1 && stage != 2) return fail(-1, "stage must be 1 or 2 (got %d)", stage);
+    if (!mass_dev || !x_dev || !y_dev || !z_dev || !vx_dev || !vy_dev || !vz_dev)
+        return fail(-1, "null mass, position or velocity array");
+    if ((fx_dev != nullptr) != (fy_dev != nullptr) || (fx_dev != nullptr) != (fz_dev != nullptr))
+        return fail(-1, "fx, fy, fz must all be given or all be NULL");
+    if (!(dt > 0.0)) return fail(-1, "dt must be positive");
+    if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
+    const int nr = 3 * c->B.nw;
+    if ((nr + 255) / 256 > c->B.nblk_cap) return fail(-1, "block-partial capacity");
+    int rc = md_launch_stage(stage, nr, dt, mass_dev, x_dev, y_dev, z_dev, vx_dev, vy_dev, vz_dev, fx_dev, fy_dev,
+                             fz_dev, fsite_dev, ekin_dev, c->B.blk, c->dout, c->stream);
+    if (rc) return fail(rc, "verlet kernels failed");
+    double ke = 0.0;
+    HIPCHK(hipMemcpyAsync(&ke, c->dout, sizeof ke, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ekin_total) *ekin_total = ke;
     return 0;
 }
 

@@ -57,6 +57,8 @@ _SIGS = {
                                                       ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp, _ip]),
     "mbgpu_get_elec_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mbgpu_get_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "mbgpu_verlet_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 12
+                           + [_dp]),
     "mbgpu_water_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, _dp, _dp, _dp, _dp,
                                          _dp, _dp, _dp, _dp, _dp]),
     "mbgpu_water_terms_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, ctypes.c_void_p,
@@ -370,6 +372,13 @@ class Context:
         e, w, f = np.zeros(nr), np.zeros(9 * nr), np.zeros(3 * nr)
         _check(load_library().mbgpu_get_water_details(self._h, _p(e), _p(w), _p(f)), "mbgpu_get_water_details")
         return dict(atomic_energy=e, heat_w=w.reshape(nr, 3, 3), f=f.reshape(nr, 3))
+
+    def verlet_stage(self, stage, dt, mass, x, y, z, vx, vy, vz, fx=None, fy=None, fz=None, fsite=None, ekin=None):
+        """mbgpu_verlet_stage with device pointers; returns the kinetic energy (stage 2)."""
+        ke = ctypes.c_double(0.0)
+        _check(load_library().mbgpu_verlet_stage(self._h, stage, dt, mass, x, y, z, vx, vy, vz, fx, fy, fz, fsite,
+                                                 ekin, ctypes.byref(ke)), "mbgpu_verlet_stage")
+        return ke.value
 
     def dipoles(self):
         """Converged induced dipoles of the last electrostatics call, (4*nw, 3)."""

@@ -1,0 +1,116 @@
+"""NVE molecular dynamics on the device with the heat current every step.
+
+`NVE(slot, dt_fs)` advances one configuration held by an engine.Slot: velocity Verlet on
+the real atoms (mbgpu_verlet_stage), a complete force evaluation per step (MB-pol 2B/3B,
+polarizable electrostatics with the induced dipoles started from the previous step's, as
+the reference keeps them, the monomer / dispersion / long-range terms), per-atom kinetic
+energies, and J = J_kin + J_pot from the library's heat flux.  Positions, velocities,
+forces and dipoles stay in HBM; each step returns the scalars (potential and kinetic
+energy, J).
+
+`Heatflux2Writer` appends J records in the layout of write_heat_flux2
+(heatcurrent.f90:730-740): the time as es12.5, then J_kin, J_pot and J_tot, each as
+3es15.7, in internal units times junit = 1e3.
+"""
+import numpy as np
+
+JUNIT = 1.0e3     # heatcurrent.f90:19
+
+
+def fortran_es(v, w, d):
+    """Fortran ESw.d edit descriptor for one value (1 digit before the point, a two-digit
+    exponent with sign, or three digits without the letter E when |exponent| > 99)."""
+    if not np.isfinite(v):
+        return "*" * w
+    s = "%.*E" % (d, v)
+    mant, exp = s.split("E")
+    e = int(exp)
+    t = mant + ("E%+03d" % e if abs(e) <= 99 else "%+04d" % e)
+    return t.rjust(w) if len(t) <= w else "*" * w
+
+
+def heatflux2_record(time, jkin, jpot, jtot):
+    """The four lines write_heat_flux2 writes for one time (J in internal units)."""
+    lines = [fortran_es(time, 12, 5)]
+    for j in (jkin, jpot, jtot):
+        lines.append("".join(fortran_es(float(v) * JUNIT, 15, 7) for v in j))
+    return "\n".join(lines) + "\n"
+
+
+class Heatflux2Writer:
+    def __init__(self, path):
+        self.path = path
+
+    def write(self, time, jkin, jpot, jtot):
+        with open(self.path, "a") as f:
+            f.write(heatflux2_record(time, jkin, jpot, jtot))
+
+
+class NVE:
+    def __init__(self, slot, dt_fs):
+        import torch
+        self.t = torch
+        self.slot = slot
+        self.dt = dt_fs * 1.0e-3                   # ps
+        box = slot.box
+        nr, n = box.nreal, box.natms
+        f64 = dict(dtype=torch.float64, device=slot.dev)
+        self.mass = torch.as_tensor(np.ascontiguousarray(box.mass[:nr]), **f64)
+        self.f = torch.zeros(3, n, **f64)          # MB-pol + water forces (SoA)
+        self.fsite = torch.zeros(3 * n, **f64)     # electrostatics forces (3 per site)
+        self.dip = torch.zeros(3 * n, **f64)       # induced dipoles, kept between steps
+        torch.cuda.synchronize(slot.dev)
+        self.time = 0.0
+        self.nstep = 0
+        self.pot = self._forces()
+        self.kin = self._kinetic()
+        torch.cuda.synchronize(slot.dev)          # ek is read by the library's stream
+
+    def _ptrs(self):
+        s = self.slot
+        return s.x.data_ptr(), s.y.data_ptr(), s.z.data_ptr()
+
+    def _forces(self):
+        s, b, c = self.slot, self.slot.box, self.slot.ctx
+        x, y, z = self._ptrs()
+        f = self.f
+        e_mb, _, _ = c.mbpol_forces_dev(b.imcon, b.cell, s.lst.data_ptr(), x, y, z, f[0].data_ptr(), f[1].data_ptr(),
+                                        f[2].data_ptr())
+        e_el, _, _, self.iterations = c.polar_electrostatics_dev(b.imcon, b.cell, s.eparams, x, y, z,
+                                                                 self.dip.data_ptr(), self.fsite.data_ptr())
+        e_w, _, _ = c.water_terms_dev(b.imcon, b.cell, s.wparams, x, y, z, f[0].data_ptr(), f[1].data_ptr(),
+                                      f[2].data_ptr())
+        return e_mb + e_el + float(np.sum(e_w))
+
+    def _kinetic(self):
+        v = self.slot.v
+        ke = 0.5 * (self.mass * (v * v).sum(0))
+        self.slot.ek.copy_(ke)
+        return float(ke.sum())
+
+    def step(self):
+        """One time step; returns dict(time, pot, kin, tot, jkin, jpot, jtot)."""
+        s, c = self.slot, self.slot.ctx
+        x, y, z = self._ptrs()
+        v, f = s.v, self.f
+        vp = (v[0].data_ptr(), v[1].data_ptr(), v[2].data_ptr())
+        fp = (f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr())
+        c.verlet_stage(1, self.dt, self.mass.data_ptr(), x, y, z, *vp, *fp, self.fsite.data_ptr())
+        self.pot = self._forces()
+        self.kin = c.verlet_stage(2, self.dt, self.mass.data_ptr(), x, y, z, *vp, *fp, self.fsite.data_ptr(),
+                                  s.ek.data_ptr())
+        jk, jp, jt = c.heat_flux_dev(*vp, s.ek.data_ptr())
+        self.nstep += 1
+        self.time += self.dt
+        return dict(time=self.time, pot=self.pot, kin=self.kin, tot=self.pot + self.kin, jkin=np.array(jk),
+                    jpot=np.array(jp), jtot=np.array(jt), iterations=self.iterations)
+
+    def positions(self):
+        s = self.slot
+        self.t.cuda.synchronize(s.dev)
+        nr = s.box.nreal
+        return np.stack([s.x[:nr].cpu().numpy(), s.y[:nr].cpu().numpy(), s.z[:nr].cpu().numpy()], 1)
+
+    def velocities(self):
+        self.t.cuda.synchronize(self.slot.dev)
+        return self.slot.v.cpu().numpy().T.copy()

@@ -191,6 +191,20 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[
  *   f_site: 3 per site, forces of the last call. */
 int mbgpu_get_elec_details(mbgpu_ctx* ctx, double* scal, double* chge, double* vesp, double* e_real,
                            double* w_real, double* s_real, double* f_site);
+/* One velocity-Verlet stage on the 3*nw real atoms (device arrays, internal units: amu,
+ * Angstrom, ps, 10 J/mol).  The reference integrates with the Verlet leapfrog of nve_0
+ * (nve_0.f:50-110), which gives the same positions; this form has the full-step velocities
+ * the heat flux needs.  F = (fx, fy, fz) [three arrays, or all NULL] + fsite [3 per site,
+ * interleaved, or NULL].
+ *   stage 1: v += dt/2 F/m; x += dt v; then F is cleared (the next force evaluation
+ *            accumulates into it);
+ *   stage 2: v += dt/2 F/m; ekin_dev[i] = m v^2/2 (may be NULL); *ekin_total = the sum.
+ * The M sites are not moved: the next electrostatics call rebuilds them. */
+int mbgpu_verlet_stage(mbgpu_ctx* ctx, int stage, double dt, const double* mass_dev, double* x_dev,
+                       double* y_dev, double* z_dev, double* vx_dev, double* vy_dev, double* vz_dev,
+                       double* fx_dev, double* fy_dev, double* fz_dev, double* fsite_dev, double* ekin_dev,
+                       double* ekin_total);
+
 /* Converged induced dipoles of the last electrostatics call: 3 per site, interleaved, 4*nw
  * sites (the reference's dipx/y/z after iter; e*Angstrom). */
 int mbgpu_get_dipoles(mbgpu_ctx* ctx, double* dip);

@@ -1,0 +1,33 @@
+"""The heatflux2 record layout (write_heat_flux2, heatcurrent.f90:730-740) and the Fortran
+ES edit descriptor it uses, against values whose Fortran output is fixed by the standard."""
+import importlib
+
+import pytest
+
+
+@pytest.fixture(scope="module")
+def md():
+    return importlib.import_module("dlpoly2-mbpol-heat_amd.md")
+
+
+@pytest.mark.parametrize("v,w,d,out", [
+    (1.0e-3, 12, 5, " 1.00000E-03"),
+    (0.2, 12, 5, " 2.00000E-01"),
+    (-123.456, 15, 7, " -1.2345600E+02"),
+    (0.0, 15, 7, "  0.0000000E+00"),
+    (9.99999996, 15, 7, "  1.0000000E+01"),      # rounds up into the next decade
+    (1.0e100, 15, 7, "  1.0000000+100"),         # three-digit exponent drops the letter
+    (-2.5e-120, 15, 7, " -2.5000000-120"),
+])
+def test_fortran_es(md, v, w, d, out):
+    assert md.fortran_es(v, w, d) == out
+
+
+def test_record_layout(md):
+    rec = md.heatflux2_record(0.0002, [1e-3, -2e-3, 3e-3], [0.5, 0.25, -0.125], [1.0, 2.0, 3.0])
+    lines = rec.splitlines()
+    assert len(lines) == 4
+    assert lines[0] == " 2.00000E-04"
+    assert lines[1] == "  1.0000000E+00 -2.0000000E+00  3.0000000E+00"     # x junit = 1e3
+    assert [len(l) for l in lines[1:]] == [45, 45, 45]
+    assert lines[3] == "  1.0000000E+03  2.0000000E+03  3.0000000E+03"
