@@ -84,7 +84,7 @@ void free_buffers(ElecBuffers& E)
     dfree(E.x); dfree(E.y); dfree(E.z); dfree(E.chge); dfree(E.polr); dfree(E.polr2); dfree(E.dg); dfree(E.tab);
     dfree(E.nbrA); dfree(E.cntA); dfree(E.nbrP); dfree(E.cntP); dfree(E.tens); dfree(E.tin);
     dfree(E.dip); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd);
-    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.state); dfree(E.zero_state);
+    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.ctr); dfree(E.state); dfree(E.zero_state);
     dfree(E.kv); dfree(E.klmn); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs);
     dfree(E.S); dfree(E.Spart); dfree(E.Sq); dfree(E.kc); dfree(E.kf); dfree(E.Fpart);
     dfree(E.vk); dfree(E.vr); dfree(E.vs); dfree(E.vc); dfree(E.vdck); dfree(E.vdcr); dfree(E.vesp);
@@ -107,6 +107,9 @@ int alloc_sites(ElecState& S, int nw, std::string& err)
     ECHK(dalloc(&E.efdc, (size_t)3 * n)); ECHK(dalloc(&E.efdd, (size_t)3 * n));
     ECHK(dalloc(&E.cg_p, (size_t)3 * n)); ECHK(dalloc(&E.cg_q, (size_t)3 * n));
     ECHK(dalloc(&E.cg_r, (size_t)3 * n)); ECHK(dalloc(&E.cg_z, (size_t)3 * n));
+    ECHK(dalloc(&E.cg_minv, (size_t)45 * nw));
+    ECHK(dalloc(&E.ctr, 4));
+    ECHK(hipMemset(E.ctr, 0, 4 * sizeof(unsigned int)));
     ECHK(dalloc(&E.state, 8)); ECHK(dalloc(&E.zero_state, 8));
     ECHK(dalloc(&E.vk, n)); ECHK(dalloc(&E.vr, n)); ECHK(dalloc(&E.vs, n)); ECHK(dalloc(&E.vc, n));
     ECHK(dalloc(&E.vdck, n)); ECHK(dalloc(&E.vdcr, n)); ECHK(dalloc(&E.vesp, n));

@@ -57,6 +57,33 @@ __device__ void block_sum_store(double (&v)[NV], double* out)
     }
 }
 
+// Called by every thread of a block after block_sum_store: the block that arrives last
+// (ticket from a per-kernel counter) sums the nb blocks' partials in block order on its
+// first wavefront, resets the counter for the next launch and returns true on thread 0
+// with the totals.  The fences order each block's partial stores before its ticket and the
+// last block's loads after it, across the L2s of all dies.
+template <int NV>
+__device__ bool last_block_sum(unsigned int* __restrict__ ctr, const double* blk, int nb, double (&tot)[NV])
+{
+    __shared__ unsigned int ticket;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) ticket = atomicAdd(ctr, 1u);
+    __syncthreads();
+    if (ticket != (unsigned int)(nb - 1) || threadIdx.x >= 64) return false;
+    __threadfence();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 64)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tot[q] += blk[NV * k + q];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = wsum(tot[q]);
+    if (threadIdx.x != 0) return false;
+    atomicExch(ctr, 0u);
+    return true;
+}
+
 __device__ __forceinline__ int water_of(const ElecDev& P, int i) { return i < P.nreal ? i / 3 : i - P.nreal; }
 
 __device__ __forceinline__ double site_polr(const ElecDev& P, int i)
@@ -881,117 +908,20 @@ __global__ __launch_bounds__(64) void k_el_converge(ElecDev P, int nb, const dou
 // ==================================================== conjugate-gradient solver ======
 // The induced dipoles solve A mu = E0 with A = r4pie0 / polr2 - T (T: the dipole-field
 // operator, emu + efdd; E0 = efk + efdc).  A is symmetric positive definite for Thole-
-// damped dipoles, so preconditioned conjugate gradients (Jacobi, z = polr2 r / r4pie0)
-// reach the SOR fixed point of iter.f in far fewer iterations.  The stopping test is the
-// reference's: the RMS of the step an SOR update would take from the current dipoles,
-// (1 - omega) * rms(z), against toler (iter.f:95-101).  state = [iterations, rms step,
-// done, r.z, alpha, beta].
+// damped dipoles, so preconditioned conjugate gradients reach the SOR fixed point of
+// iter.f in far fewer iterations.  The preconditioner is block-Jacobi over waters: the
+// 9 x 9 block of each water's O, H1, H2 dipoles, which holds the strong intramolecular
+// couplings (half the iterations of a diagonal preconditioner at liquid density).  The
+// stopping test is the reference's: the RMS of the step an SOR update would take from the
+// current dipoles, (1 - omega) * rms(polr2 r / r4pie0), against toler (iter.f:95-101).
+// state = [iterations, rms step, done, r.z, alpha, beta, dipole count].
 
-// r = E0 + T mu0 - r4pie0 mu0 / polr2, z = polr2 r / r4pie0, p = z; partials (r.z, |z|^2, ndip)
-__global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __restrict__ efk,
-                                                 const double* __restrict__ efdc, const double* __restrict__ emu,
-                                                 const double* __restrict__ efdd, const double* __restrict__ dip,
-                                                 double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ pv, double* __restrict__ blk)
+// Scalar steps of the solver from the summed partials (a, b, c):
+//   mode 0 (start):        r.z = a, rms test on b, dipole count c -> state[6]
+//   mode 1 (after T p):    alpha = r.z / p.q (a = p.q)
+//   mode 2 (after update): beta = r.z(new) / r.z = a / state[3], iteration count, rms test on b
+__device__ void cg_scalars(const ElecDev& P, int mode, double a, double b, double c, double* __restrict__ state)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[3] = {0.0, 0.0, 0.0};
-    if (i < P.nrow) {
-        const double p2 = site_polr2(P, i);
-        const bool on = p2 > P.toler;
-        for (int q = 0; q < 3; ++q) {
-            const int o = 3 * i + q;
-            const double rr = on ? (efk[o] + efdc[o] + emu[o] + efdd[o]) - R4PIE0 * dip[o] / p2 : 0.0;
-            const double zz = on ? p2 * rr / R4PIE0 : 0.0;
-            r[o] = rr;
-            z[o] = zz;
-            pv[o] = zz;
-            v[0] += rr * zz;
-            v[1] += zz * zz;
-        }
-        v[2] = on ? 1.0 : 0.0;
-    }
-    block_sum_store(v, blk + 3 * blockIdx.x);
-}
-
-// q = A p (from T p in emu / efdd); partials of p.q
-__global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __restrict__ pv,
-                                                  const double* __restrict__ tp1, const double* __restrict__ tp2,
-                                                  double* __restrict__ qv, double* __restrict__ blk,
-                                                  const double* __restrict__ state)
-{
-    if (state[2] != 0.0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[1] = {0.0};
-    if (i < P.nrow) {
-        const double p2 = site_polr2(P, i);
-        const bool on = p2 > P.toler;
-        for (int q = 0; q < 3; ++q) {
-            const int o = 3 * i + q;
-            const double qq = on ? R4PIE0 * pv[o] / p2 - (tp1[o] + tp2[o]) : 0.0;
-            qv[o] = qq;
-            v[0] += pv[o] * qq;
-        }
-    }
-    block_sum_store(v, blk + blockIdx.x);
-}
-
-// mu += alpha p, r -= alpha q, z = polr2 r / r4pie0; partials (r.z, |z|^2)
-__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, const double* __restrict__ pv,
-                                                   const double* __restrict__ qv, double* __restrict__ dip,
-                                                   double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ blk, const double* __restrict__ state)
-{
-    if (state[2] != 0.0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const double alpha = state[4];
-    double v[2] = {0.0, 0.0};
-    if (i < P.nrow) {
-        const double p2 = site_polr2(P, i);
-        if (p2 > P.toler) {
-            for (int q = 0; q < 3; ++q) {
-                const int o = 3 * i + q;
-                dip[o] += alpha * pv[o];
-                const double rr = r[o] - alpha * qv[o];
-                const double zz = p2 * rr / R4PIE0;
-                r[o] = rr;
-                z[o] = zz;
-                v[0] += rr * zz;
-                v[1] += zz * zz;
-            }
-        }
-    }
-    block_sum_store(v, blk + 2 * blockIdx.x);
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(256) void k_cg_direction(ElecDev P, const double* __restrict__ z,
-                                                      double* __restrict__ pv, const double* __restrict__ state)
-{
-    if (state[2] != 0.0) return;
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o < 3 * P.nrow) pv[o] = z[o] + state[5] * pv[o];
-}
-
-// Scalar steps of the solver from block partials (single wavefront, fixed order).
-//   mode 0 (after k_cg_init):   r.z, convergence test, ndip -> state[6]
-//   mode 1 (after k_cg_apply):  alpha = r.z / p.q
-//   mode 2 (after k_cg_update): beta = r.z(new) / r.z, iteration count, convergence test
-__global__ __launch_bounds__(64) void k_cg_scalar(ElecDev P, int mode, int nb, const double* __restrict__ blk,
-                                                  double* __restrict__ state)
-{
-    if (mode != 0 && state[2] != 0.0) return;
-    const int stride = mode == 0 ? 3 : (mode == 1 ? 1 : 2);
-    double a = 0.0, b = 0.0, c = 0.0;
-    for (int k = threadIdx.x; k < nb; k += 64) {
-        a += blk[stride * k];
-        if (stride > 1) b += blk[stride * k + 1];
-        if (stride > 2) c += blk[stride * k + 2];
-    }
-    a = wsum(a);
-    b = wsum(b);
-    c = wsum(c);
-    if (threadIdx.x != 0) return;
     if (mode == 1) {
         state[4] = state[3] / a;
         return;
@@ -1005,6 +935,256 @@ __global__ __launch_bounds__(64) void k_cg_scalar(ElecDev P, int mode, int nb, c
     state[3] = a;
     state[1] = rms;
     if (rms <= P.toler) state[2] = 1.0;
+}
+
+// packed index of (i, j) in a symmetric 9 x 9 matrix stored as its lower triangle
+__device__ __forceinline__ constexpr int tri9(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+// Per water: the preconditioner block M = diag(r4pie0 / polr2) - r4pie0 T_damped over the
+// intramolecular pairs (the Thole-damped dipole tensor of rfield.f:419-520 without its Ewald
+// split: M only has to be close to A's block, it does not move the fixed point), inverted
+// by Cholesky and stored packed (45 values per water).  Rows of sites without a dipole are
+// decoupled unit rows.  A block that is not positive definite falls back to the diagonal.
+__global__ __launch_bounds__(256) void k_cg_block(ElecDev P, const double* __restrict__ x,
+                                                  const double* __restrict__ y, const double* __restrict__ z,
+                                                  double* __restrict__ minv)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= P.nw) return;
+    double a[45];
+    bool on[3];
+#pragma unroll
+    for (int q = 0; q < 45; ++q) a[q] = 0.0;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const double p2 = site_polr2(P, 3 * w + s);
+        on[s] = p2 > P.toler;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[tri9(3 * s + c, 3 * s + c)] = on[s] ? R4PIE0 / p2 : 1.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int t = s + 1; t < 3; ++t) {
+            const int i = 3 * w + s, j = 3 * w + t;
+            if (!(on[s] && on[t] && site_polr2(P, i) > 1.e-6 && site_polr2(P, j) > 1.e-6)) continue;
+            double d[3];
+            const double r2 = sep(P, x, y, z, i, j, d);
+            const double rr = sqrt(r2);
+            const double r3i = 1.0 / (rr * r2), r5i = r3i / r2;
+            const double bb = pow(site_polr(P, i) * site_polr(P, j), 1.0 / 6.0);
+            const double u = rr / bb;
+            const double arob4 = (s == 0 ? P.athole12 : P.athole13) * (u * u) * (u * u);
+            const double dum1 = exp(-arob4);
+            const double dum2 = (1.0 + (4.0 / 3.0) * arob4) * dum1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int e = 0; e < 3; ++e)
+                    a[tri9(3 * t + e, 3 * s + c)] =
+                        -R4PIE0 * (3.0 * d[c] * d[e] * r5i * (1.0 - dum2) - (c == e ? r3i * (1.0 - dum1) : 0.0));
+        }
+    // Cholesky M = L L^T in place
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double v = a[tri9(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) v -= a[tri9(i, k)] * a[tri9(j, k)];
+            if (i == j) {
+                ok = ok && v > 0.0;
+                a[tri9(i, i)] = sqrt(fmax(v, 1.e-300));
+            } else {
+                a[tri9(i, j)] = v / a[tri9(j, j)];
+            }
+        }
+    // X = L^-1 (lower), then M^-1 = X^T X
+    double X[45];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        X[tri9(i, i)] = 1.0 / a[tri9(i, i)];
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = j; k < i; ++k) v += a[tri9(i, k)] * X[tri9(k, j)];
+            X[tri9(i, j)] = -v / a[tri9(i, i)];
+        }
+    }
+    double* m = minv + 45L * w;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = i; k < 9; ++k) v += X[tri9(k, i)] * X[tri9(k, j)];
+            ok = ok && isfinite(v);
+            a[tri9(i, j)] = v;
+        }
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            const int s = i / 3;
+            const double jac = (i == j && on[s]) ? site_polr2(P, 3 * w + s) / R4PIE0 : 0.0;
+            m[tri9(i, j)] = ok ? a[tri9(i, j)] : jac;
+        }
+}
+
+// z = M^-1 r for one water (rows of sites without a dipole stay zero)
+__device__ __forceinline__ void cg_precondition(const double* __restrict__ m, const double (&r)[9],
+                                                const bool (&on)[3], double (&z)[9])
+{
+    double mm[45];
+#pragma unroll
+    for (int q = 0; q < 45; ++q) mm[q] = m[q];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) v += mm[tri9(i, j)] * r[j];
+        z[i] = on[i / 3] ? v : 0.0;
+    }
+}
+
+// r = E0 + T mu0 - r4pie0 mu0 / polr2, z = M^-1 r, p = z; one thread per water; partials
+// (r.z, |polr2 r / r4pie0|^2, dipole count)
+__global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __restrict__ efk,
+                                                 const double* __restrict__ efdc, const double* __restrict__ emu,
+                                                 const double* __restrict__ efdd, const double* __restrict__ dip,
+                                                 const double* __restrict__ minv, double* __restrict__ r,
+                                                 double* __restrict__ z, double* __restrict__ pv,
+                                                 double* __restrict__ blk, unsigned int* __restrict__ ctr,
+                                                 double* __restrict__ state)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (w < P.nw) {
+        double rr[9], zz[9];
+        bool on[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const double p2 = site_polr2(P, 3 * w + s);
+            on[s] = p2 > P.toler;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int o = 9 * w + 3 * s + c;
+                rr[3 * s + c] = on[s] ? (efk[o] + efdc[o] + emu[o] + efdd[o]) - R4PIE0 * dip[o] / p2 : 0.0;
+                const double zs = on[s] ? p2 * rr[3 * s + c] / R4PIE0 : 0.0;
+                v[1] += zs * zs;
+            }
+            v[2] += on[s] ? 1.0 : 0.0;
+        }
+        cg_precondition(minv + 45L * w, rr, on, zz);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            r[9 * w + q] = rr[q];
+            z[9 * w + q] = zz[q];
+            pv[9 * w + q] = zz[q];
+            v[0] += rr[q] * zz[q];
+        }
+    }
+    block_sum_store(v, blk + 3 * blockIdx.x);
+    double t[3];
+    if (last_block_sum(ctr, blk, gridDim.x, t)) cg_scalars(P, 0, t[0], t[1], t[2], state);
+}
+
+// The reciprocal dipole field of p from the chunk partials, summed in chunk order
+// (efdd = scale * sum + self term, as k_el_kfield_reduce), then q = A p =
+// r4pie0 p / polr2 - (emu + efdd) and the partials of p.q; the last block sets alpha.
+// One thread per (component, site).
+__global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __restrict__ Fpart,
+                                                  const double* __restrict__ pv, const double* __restrict__ emu,
+                                                  double* __restrict__ efdd, double* __restrict__ qv,
+                                                  double* __restrict__ blk, unsigned int* __restrict__ ctr,
+                                                  double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (t < 3 * P.nrow) {
+        const int q = t / P.nrow, j = t - q * P.nrow;
+        const double* in = Fpart + (long)q * P.n + j;
+        const long cs = 6L * P.n;
+        double sum = 0.0;
+        int c = 0;
+        for (; c + 8 <= P.kchunks; c += 8) {
+            double u[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) u[k] = in[(c + k) * cs];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sum += u[k];
+        }
+        for (; c < P.kchunks; ++c) sum += in[c * cs];
+        const double p2 = site_polr2(P, j);
+        const int o = 3 * j + q;
+        const double fd = (p2 > 1.e-6 ? P.scale * sum : 0.0) + P.self_dd * pv[o];
+        efdd[o] = fd;
+        const double qq = p2 > P.toler ? R4PIE0 * pv[o] / p2 - (emu[o] + fd) : 0.0;
+        qv[o] = qq;
+        v[0] = pv[o] * qq;
+    }
+    block_sum_store(v, blk + blockIdx.x);
+    double tot[1];
+    if (last_block_sum(ctr, blk, gridDim.x, tot)) cg_scalars(P, 1, tot[0], 0.0, 0.0, state);
+}
+
+// mu += alpha p, r -= alpha q, z = M^-1 r; one thread per water; partials
+// (r.z, |polr2 r / r4pie0|^2)
+__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, const double* __restrict__ pv,
+                                                   const double* __restrict__ qv, const double* __restrict__ minv,
+                                                   double* __restrict__ dip, double* __restrict__ r,
+                                                   double* __restrict__ z, double* __restrict__ blk,
+                                                   unsigned int* __restrict__ ctr, double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const double alpha = state[4];
+    double v[2] = {0.0, 0.0};
+    if (w < P.nw) {
+        double rr[9], zz[9];
+        bool on[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const double p2 = site_polr2(P, 3 * w + s);
+            on[s] = p2 > P.toler;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int o = 9 * w + 3 * s + c;
+                rr[3 * s + c] = 0.0;
+                if (on[s]) {
+                    dip[o] += alpha * pv[o];
+                    rr[3 * s + c] = r[o] - alpha * qv[o];
+                    const double zs = p2 * rr[3 * s + c] / R4PIE0;
+                    v[1] += zs * zs;
+                }
+            }
+        }
+        cg_precondition(minv + 45L * w, rr, on, zz);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            if (on[q / 3]) {
+                r[9 * w + q] = rr[q];
+                z[9 * w + q] = zz[q];
+                v[0] += rr[q] * zz[q];
+            }
+        }
+    }
+    block_sum_store(v, blk + 2 * blockIdx.x);
+    double t[2];
+    if (last_block_sum(ctr, blk, gridDim.x, t)) cg_scalars(P, 2, t[0], t[1], 0.0, state);
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(256) void k_cg_direction(ElecDev P, const double* __restrict__ z,
+                                                      double* __restrict__ pv, const double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < 3 * P.nrow) pv[o] = z[o] + state[5] * pv[o];
 }
 
 // ================================================================ final passes ======
@@ -1717,6 +1897,7 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
 }
 
 // T v into emu / efdd (real-space rows and reciprocal space) for the vector v
+// T v into emu / efdd (real-space rows and reciprocal space) for the vector v
 static void launch_tfield(ElecBuffers& E, const ElecDev& P, const double* v, const double* state, hipStream_t s)
 {
     hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
@@ -1734,22 +1915,30 @@ static void launch_tfield(ElecBuffers& E, const ElecDev& P, const double* v, con
 // charge fields of the preparation, plus the real-space dipole field here)
 int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
 {
+    const int nbw = ceil_div(P.nw, 256);
+    hipLaunchKernelGGL(k_cg_block, dim3(nbw), dim3(256), 0, s, P, E.x, E.y, E.z, E.cg_minv);
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
-    const int nb = ceil_div(P.nrow, 256);
-    hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(256), 0, s, P, E.efk, E.efdc, E.emu, E.efdd, E.dip, E.cg_r, E.cg_z,
-                       E.cg_p, E.blk);
-    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(64), 0, s, P, 0, nb, E.blk, E.state);
+    hipLaunchKernelGGL(k_cg_init, dim3(nbw), dim3(256), 0, s, P, E.efk, E.efdc, E.emu, E.efdd, E.dip, E.cg_minv, E.cg_r,
+                       E.cg_z, E.cg_p, E.blk, E.ctr, E.state);
     return (int)hipGetLastError();
 }
 
+// One iteration: T p (real-space rows, structure factors, reciprocal field), q = A p and
+// alpha, the update of mu, r, z with beta and the stopping test, the new direction.
 int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s)
 {
-    const int nb = ceil_div(P.nrow, 256);
-    launch_tfield(E, P, E.cg_p, E.state, s);
-    hipLaunchKernelGGL(k_cg_apply, dim3(nb), dim3(256), 0, s, P, E.cg_p, E.emu, E.efdd, E.cg_q, E.blk, E.state);
-    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(64), 0, s, P, 1, nb, E.blk, E.state);
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, P, E.cg_p, E.cg_q, E.dip, E.cg_r, E.cg_z, E.blk, E.state);
-    hipLaunchKernelGGL(k_cg_scalar, dim3(1), dim3(64), 0, s, P, 2, nb, E.blk, E.state);
+    const double* p = E.cg_p;
+    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
+    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
+                       E.runs, E.runk, E.phc, E.phs, E.chge, p, E.Spart, E.state);
+    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
+                       E.Sq, E.kc, E.state);
+    hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
+                       E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
+    hipLaunchKernelGGL(k_cg_apply, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd,
+                       E.cg_q, E.blk, E.ctr + 1, E.state);
+    hipLaunchKernelGGL(k_cg_update, dim3(ceil_div(P.nw, 256)), dim3(256), 0, s, P, E.cg_p, E.cg_q, E.cg_minv, E.dip,
+                       E.cg_r, E.cg_z, E.blk, E.ctr + 2, E.state);
     hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.cg_z, E.cg_p, E.state);
     return (int)hipGetLastError();
 }

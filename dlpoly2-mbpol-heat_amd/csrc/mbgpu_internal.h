@@ -126,6 +126,8 @@ struct ElecBuffers {
     double* tin = nullptr;                                     // [nw][6 pairs][6] intramolecular tensors
     double *dip = nullptr, *efk = nullptr, *emu = nullptr, *efdc = nullptr, *efdd = nullptr;  // [3n]
     double *cg_p = nullptr, *cg_q = nullptr, *cg_r = nullptr, *cg_z = nullptr;                // [3n] solver vectors
+    double* cg_minv = nullptr;                                 // [nw][45] inverse preconditioner blocks
+    unsigned int* ctr = nullptr;                               // [4] arrival counters of the solver kernels
     double* state = nullptr;                                   // [8] solver scalars
     double* kv = nullptr;                                      // [K][8] kx ky kz akk akv akv2 . .
     int* runs = nullptr;                                       // [nrun][4] l, m, n0, count; k0 in runk

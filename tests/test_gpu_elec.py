@@ -149,7 +149,8 @@ def test_parameter_errors(pkg, waterbox, ep):
 @pytest.mark.parametrize("which", ["small", "box64", "box512"])
 def test_conjugate_gradient_solver_matches_sor_oracle(pkg, waterbox, ep, which, ora512):
     """The library's conjugate-gradient option reaches the reference's SOR fixed point:
-    every output within 1e-10 of the oracle's SOR result, in fewer iterations."""
+    every output within 1e-10 of the oracle's SOR result, in fewer iterations (per-water
+    block preconditioner: 14 iterations at 512 waters, where a diagonal one takes 29)."""
     import dataclasses
     if which == "small":
         box = waterbox.make_waterbox(16, 12.42, 4, seed=7)
@@ -164,6 +165,9 @@ def test_conjugate_gradient_solver_matches_sor_oracle(pkg, waterbox, ep, which, 
         box = waterbox.box_512()
         p, r = ora512
     _, g = run_gpu(pkg, box, dataclasses.replace(p, solver=1))
+    print(f"{which}: CG {g['niter']} iterations, SOR {r['niter']}", flush=True)
     assert g["niter"] < r["niter"] / 2
+    if which == "box512":
+        assert g["niter"] <= 16
     g["niter"] = r["niter"]
     compare(g, r)
