@@ -84,7 +84,7 @@ void free_buffers(ElecBuffers& E)
     dfree(E.x); dfree(E.y); dfree(E.z); dfree(E.chge); dfree(E.polr); dfree(E.polr2); dfree(E.dg); dfree(E.tab);
     dfree(E.nbrA); dfree(E.cntA); dfree(E.nbrP); dfree(E.cntP); dfree(E.tens); dfree(E.tin);
     dfree(E.dip); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd);
-    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.ctr); dfree(E.state); dfree(E.zero_state);
+    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.ctr); dfree(E.ksum); dfree(E.state); dfree(E.zero_state);
     dfree(E.kv); dfree(E.klmn); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs);
     dfree(E.S); dfree(E.Spart); dfree(E.Sq); dfree(E.kc); dfree(E.kf); dfree(E.Fpart);
     dfree(E.vk); dfree(E.vr); dfree(E.vs); dfree(E.vc); dfree(E.vdck); dfree(E.vdcr); dfree(E.vesp);
@@ -109,6 +109,7 @@ int alloc_sites(ElecState& S, int nw, std::string& err)
     ECHK(dalloc(&E.cg_r, (size_t)3 * n)); ECHK(dalloc(&E.cg_z, (size_t)3 * n));
     ECHK(dalloc(&E.cg_minv, (size_t)45 * nw));
     ECHK(dalloc(&E.ctr, 4));
+    ECHK(dalloc(&E.ksum, (size_t)16 * n));
     ECHK(hipMemset(E.ctr, 0, 4 * sizeof(unsigned int)));
     ECHK(dalloc(&E.state, 8)); ECHK(dalloc(&E.zero_state, 8));
     ECHK(dalloc(&E.vk, n)); ECHK(dalloc(&E.vr, n)); ECHK(dalloc(&E.vs, n)); ECHK(dalloc(&E.vc, n));
@@ -441,8 +442,11 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         return rc;
     }
     // solver loop in batches; a converged state turns the rest of a batch into no-ops
+    // the first batch covers the last evaluation's count and two more (one host round trip
+    // in steady state; an iteration past convergence costs a few microseconds of no-op
+    // launches, a second round trip tens)
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int launched = 0, batch = solver == 1 ? 8 : 12;
+    int launched = 0, batch = S.last_iters > 0 ? std::min(S.last_iters + 2, 64) : (solver == 1 ? 8 : 12);
     if (solver == 1) {
         rc = el_launch_cg_init(E, P, s);
         if (rc) {
@@ -461,7 +465,10 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         }
         ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
         ECHK(hipStreamSynchronize(s));
-        if (st[2] != 0.0) break;
+        if (st[2] != 0.0) {
+            S.last_iters = (int)st[0];
+            break;
+        }
         if (!(st[1] == st[1])) {
             err = "induced dipoles became NaN";
             return -3;

@@ -57,31 +57,71 @@ __device__ void block_sum_store(double (&v)[NV], double* out)
     }
 }
 
-// Called by every thread of a block after block_sum_store: the block that arrives last
+// Called by every thread of a block after block_sum_store.  The block that arrives last
 // (ticket from a per-kernel counter) sums the nb blocks' partials in block order on its
-// first wavefront, resets the counter for the next launch and returns true on thread 0
-// with the totals.  The fences order each block's partial stores before its ticket and the
-// last block's loads after it, across the L2s of all dies.
+// first wavefront, resets the counter for the next launch and returns true on all of its
+// threads, with the totals in tot; the other blocks return false.  The fences order each
+// block's partial stores before its ticket and the last block's loads after it, across the
+// L2s of all dies.
 template <int NV>
 __device__ bool last_block_sum(unsigned int* __restrict__ ctr, const double* blk, int nb, double (&tot)[NV])
 {
     __shared__ unsigned int ticket;
+    __shared__ double st[NV];
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) ticket = atomicAdd(ctr, 1u);
     __syncthreads();
-    if (ticket != (unsigned int)(nb - 1) || threadIdx.x >= 64) return false;
+    if (ticket != (unsigned int)(nb - 1)) return false;
     __threadfence();
+    if (threadIdx.x < 64) {
+        double t[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
-    for (int k = threadIdx.x; k < nb; k += 64)
+        for (int q = 0; q < NV; ++q) t[q] = 0.0;
+        for (int k = threadIdx.x; k < nb; k += 64)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) tot[q] += blk[NV * k + q];
+            for (int q = 0; q < NV; ++q) t[q] += blk[NV * k + q];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] = wsum(tot[q]);
-    if (threadIdx.x != 0) return false;
-    atomicExch(ctr, 0u);
+        for (int q = 0; q < NV; ++q) t[q] = wsum(t[q]);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) st[q] = t[q];
+            atomicExch(ctr, 0u);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = st[q];
     return true;
+}
+
+// Sum over chunks of in[c * cs + off] (c = 0..nc-1) for the 64 elements of a 256-thread
+// block, one element per lane: wave w adds the chunks [w nc / 4, (w + 1) nc / 4) in order
+// (coalesced loads of 64 consecutive elements), and the four wave sums are added in wave
+// order.  Every thread of the block must call it; the sum is returned on threads 0..63.
+__device__ double chunk_sum64(const double* __restrict__ in, long cs, int nc, long off, bool valid)
+{
+    __shared__ double part[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c0 = (nc * wv) / 4, c1 = (nc * (wv + 1)) / 4;
+    double sum = 0.0;
+    if (valid) {
+        const double* p = in + off;
+        int c = c0;
+        for (; c + 4 <= c1; c += 4) {
+            const double u0 = p[c * cs], u1 = p[(c + 1) * cs], u2 = p[(c + 2) * cs], u3 = p[(c + 3) * cs];
+            sum += u0;
+            sum += u1;
+            sum += u2;
+            sum += u3;
+        }
+        for (; c < c1; ++c) sum += p[c * cs];
+    }
+    part[wv][lane] = sum;
+    __syncthreads();
+    const double t = wv == 0 ? ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane] : 0.0;
+    __syncthreads();
+    return t;
 }
 
 __device__ __forceinline__ int water_of(const ElecDev& P, int i) { return i < P.nreal ? i / 3 : i - P.nreal; }
@@ -642,22 +682,27 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
             for (int q = 0; q < 8; ++q) Spart[((long)chunk * P.K + k0 + t) * 8 + q] = acc[t][q];
 }
 
-// Sum the chunk partials in chunk order and derive the per-k constants of the field
-// passes: kc[k][0..5] = (-k_a akk k.Ms, -k_a akk k.Mc) for the dipole field (ewald1p.f:759-767).
-// With CHG the charge sums are stored in Sq[k][2] and kc[k][6..11] receives the charge-
-// field constants (k_a akk ckcs, -k_a akk ckss) (ewald1p.f:749-757).
+// Sum the chunk partials and derive the per-k constants of the field passes: kc[k][0..5] =
+// (-k_a akk k.Ms, -k_a akk k.Mc) for the dipole field (ewald1p.f:759-767).  With CHG the
+// charge sums are stored in Sq[k][2] and kc[k][6..11] receives the charge-field constants
+// (k_a akk ckcs, -k_a akk ckss) (ewald1p.f:749-757).  A block sums 64 (k, component)
+// elements (8 k vectors); lanes 8k'..8k'+7 of the first wave hold the 8 sums of one k.
 template <bool CHG>
-__global__ void k_el_sreduce(ElecDev P, int schunks, const double* __restrict__ Spart, const double* __restrict__ kv,
-                             double* __restrict__ S, double* __restrict__ Sq, double* __restrict__ kc,
-                             const double* __restrict__ state)
+__global__ __launch_bounds__(256) void k_el_sreduce(ElecDev P, int schunks, const double* __restrict__ Spart,
+                                                    const double* __restrict__ kv, double* __restrict__ S,
+                                                    double* __restrict__ Sq, double* __restrict__ kc,
+                                                    const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.K) return;
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < schunks; ++c)
+    const long e = (long)blockIdx.x * 64 + (threadIdx.x & 63);
+    const double v = chunk_sum64(Spart, 8L * P.K, schunks, e, e < 8L * P.K);
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x, base = lane & ~7;
+    double s[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) s[q] += Spart[((long)c * P.K + k) * 8 + q];
+    for (int q = 0; q < 8; ++q) s[q] = __shfl(v, base + q, 64);
+    const int k = (int)(e >> 3);
+    if ((lane & 7) != 0 || k >= P.K) return;
     if (CHG) {
         Sq[2 * k] = s[0];
         Sq[2 * k + 1] = s[1];
@@ -769,9 +814,9 @@ __global__ __launch_bounds__(256) void k_el_kfield(ElecDev P, const int* __restr
     }
 }
 
-// Sum the field partials in chunk order and scale (ewald1p.f:941-976): efdd =
-// scale * sum + self term; with CHG also efdc = scale * sum (iloop = 1).  One thread per
-// (component, site).
+// Sum the field partials over the k chunks and scale (ewald1p.f:941-976): efdd =
+// scale * sum + self term; with CHG also efdc = scale * sum (iloop = 1).  Element
+// (component q, site j) = q * nrow + j, 64 per block.
 template <bool CHG>
 __global__ __launch_bounds__(256) void k_el_kfield_reduce(ElecDev P, const double* __restrict__ Fpart,
                                                           const double* __restrict__ dip, double* __restrict__ efdd,
@@ -780,26 +825,16 @@ __global__ __launch_bounds__(256) void k_el_kfield_reduce(ElecDev P, const doubl
     if (!CHG && state[2] != 0.0) return;
     const int nrow = CHG ? P.n : P.nrow;
     const int nq = CHG ? 6 : 3;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nq * nrow) return;
-    const int q = t / nrow, j = t - q * nrow;
-    const double* in = Fpart + (long)q * P.n + j;
-    const long cs = 6L * P.n;
-    double s = 0.0;
-    int c = 0;
-    for (; c + 8 <= P.kchunks; c += 8) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = in[(c + u) * cs];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
-    }
-    for (; c < P.kchunks; ++c) s += in[c * cs];
+    const int t = blockIdx.x * 64 + (threadIdx.x & 63);
+    const bool valid = t < nq * nrow;
+    const int q = valid ? t / nrow : 0, j = valid ? t - q * nrow : 0;
+    const double sum = chunk_sum64(Fpart, 6L * P.n, P.kchunks, (long)q * P.n + j, valid);
+    if (threadIdx.x >= 64 || !valid) return;
     if (q < 3) {
         const bool pol = site_polr2(P, j) > 1.e-6;
-        efdd[3 * j + q] = (pol ? P.scale * s : 0.0) + P.self_dd * dip[3 * j + q];
+        efdd[3 * j + q] = (pol ? P.scale * sum : 0.0) + P.self_dd * dip[3 * j + q];
     } else {
-        efdc[3 * j + q - 3] = P.scale * s;
+        efdc[3 * j + q - 3] = P.scale * sum;
     }
 }
 
@@ -1089,13 +1124,13 @@ __global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __rest
     }
     block_sum_store(v, blk + 3 * blockIdx.x);
     double t[3];
-    if (last_block_sum(ctr, blk, gridDim.x, t)) cg_scalars(P, 0, t[0], t[1], t[2], state);
+    if (last_block_sum(ctr, blk, gridDim.x, t) && threadIdx.x == 0) cg_scalars(P, 0, t[0], t[1], t[2], state);
 }
 
-// The reciprocal dipole field of p from the chunk partials, summed in chunk order
-// (efdd = scale * sum + self term, as k_el_kfield_reduce), then q = A p =
-// r4pie0 p / polr2 - (emu + efdd) and the partials of p.q; the last block sets alpha.
-// One thread per (component, site).
+// The reciprocal dipole field of p from the chunk partials (efdd = scale * sum + self
+// term, as k_el_kfield_reduce), then q = A p = r4pie0 p / polr2 - (emu + efdd) and the
+// partials of p.q; the last block sets alpha.  Element (component, site) as in
+// k_el_kfield_reduce, 64 per block.
 __global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __restrict__ Fpart,
                                                   const double* __restrict__ pv, const double* __restrict__ emu,
                                                   double* __restrict__ efdd, double* __restrict__ qv,
@@ -1103,22 +1138,12 @@ __global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __res
                                                   double* __restrict__ state)
 {
     if (state[2] != 0.0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * 64 + (threadIdx.x & 63);
+    const bool valid = t < 3 * P.nrow;
+    const int q = valid ? t / P.nrow : 0, j = valid ? t - q * P.nrow : 0;
+    const double sum = chunk_sum64(Fpart, 6L * P.n, P.kchunks, (long)q * P.n + j, valid);
     double v[1] = {0.0};
-    if (t < 3 * P.nrow) {
-        const int q = t / P.nrow, j = t - q * P.nrow;
-        const double* in = Fpart + (long)q * P.n + j;
-        const long cs = 6L * P.n;
-        double sum = 0.0;
-        int c = 0;
-        for (; c + 8 <= P.kchunks; c += 8) {
-            double u[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) u[k] = in[(c + k) * cs];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) sum += u[k];
-        }
-        for (; c < P.kchunks; ++c) sum += in[c * cs];
+    if (threadIdx.x < 64 && valid) {
         const double p2 = site_polr2(P, j);
         const int o = 3 * j + q;
         const double fd = (p2 > 1.e-6 ? P.scale * sum : 0.0) + P.self_dd * pv[o];
@@ -1129,63 +1154,60 @@ __global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __res
     }
     block_sum_store(v, blk + blockIdx.x);
     double tot[1];
-    if (last_block_sum(ctr, blk, gridDim.x, tot)) cg_scalars(P, 1, tot[0], 0.0, 0.0, state);
+    if (last_block_sum(ctr, blk, gridDim.x, tot) && threadIdx.x == 0) cg_scalars(P, 1, tot[0], 0.0, 0.0, state);
 }
 
-// mu += alpha p, r -= alpha q, z = M^-1 r; one thread per water; partials
-// (r.z, |polr2 r / r4pie0|^2)
-__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, const double* __restrict__ pv,
-                                                   const double* __restrict__ qv, const double* __restrict__ minv,
-                                                   double* __restrict__ dip, double* __restrict__ r,
-                                                   double* __restrict__ z, double* __restrict__ blk,
-                                                   unsigned int* __restrict__ ctr, double* __restrict__ state)
+// mu += alpha p, r -= alpha q, z = M^-1 r, partials (r.z, |polr2 r / r4pie0|^2); one
+// thread per (water, row), CG_WPB waters per block, the new r of a water shared through
+// LDS.  The last block sets beta, the iteration count and the stopping test, and then
+// forms the next direction p = z + beta p over all rows.
+#define CG_WPB 28
+__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, double* __restrict__ pv, const double* __restrict__ qv,
+                                                   const double* __restrict__ minv, double* __restrict__ dip,
+                                                   double* __restrict__ r, double* __restrict__ z,
+                                                   double* __restrict__ blk, unsigned int* __restrict__ ctr,
+                                                   double* __restrict__ state)
 {
     if (state[2] != 0.0) return;
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double rs[CG_WPB * 9];
+    __shared__ double beta_s;
+    const int tid = threadIdx.x;
+    const int wl = tid / 9, i = tid - 9 * wl;
+    const int w = blockIdx.x * CG_WPB + wl;
+    const bool act = wl < CG_WPB && w < P.nw;
+    const int o = 9 * w + i;
+    const double p2 = act ? site_polr2(P, 3 * w + i / 3) : 0.0;
+    const bool on = act && p2 > P.toler;
     const double alpha = state[4];
-    double v[2] = {0.0, 0.0};
-    if (w < P.nw) {
-        double rr[9], zz[9];
-        bool on[3];
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const double p2 = site_polr2(P, 3 * w + s);
-            on[s] = p2 > P.toler;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int o = 9 * w + 3 * s + c;
-                rr[3 * s + c] = 0.0;
-                if (on[s]) {
-                    dip[o] += alpha * pv[o];
-                    rr[3 * s + c] = r[o] - alpha * qv[o];
-                    const double zs = p2 * rr[3 * s + c] / R4PIE0;
-                    v[1] += zs * zs;
-                }
-            }
-        }
-        cg_precondition(minv + 45L * w, rr, on, zz);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            if (on[q / 3]) {
-                r[9 * w + q] = rr[q];
-                z[9 * w + q] = zz[q];
-                v[0] += rr[q] * zz[q];
-            }
-        }
+    double rr = 0.0, zs = 0.0;
+    if (on) {
+        dip[o] += alpha * pv[o];
+        rr = r[o] - alpha * qv[o];
+        zs = p2 * rr / R4PIE0;
     }
+    if (wl < CG_WPB) rs[tid] = rr;
+    __syncthreads();
+    double zz = 0.0;
+    if (on) {
+        const double* m = minv + 45L * w;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) zz += m[tri9(i, j)] * rs[9 * wl + j];
+        r[o] = rr;
+        z[o] = zz;
+    }
+    double v[2] = {rr * zz, zs * zs};
     block_sum_store(v, blk + 2 * blockIdx.x);
     double t[2];
-    if (last_block_sum(ctr, blk, gridDim.x, t)) cg_scalars(P, 2, t[0], t[1], 0.0, state);
+    if (!last_block_sum(ctr, blk, gridDim.x, t)) return;
+    if (tid == 0) {
+        cg_scalars(P, 2, t[0], t[1], 0.0, state);
+        beta_s = state[5];
+    }
+    __syncthreads();
+    const double beta = beta_s;
+    for (int e = tid; e < 3 * P.nrow; e += 256) pv[e] = z[e] + beta * pv[e];
 }
 
-// p = z + beta p
-__global__ __launch_bounds__(256) void k_cg_direction(ElecDev P, const double* __restrict__ z,
-                                                      double* __restrict__ pv, const double* __restrict__ state)
-{
-    if (state[2] != 0.0) return;
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o < 3 * P.nrow) pv[o] = z[o] + state[5] * pv[o];
-}
 
 // ================================================================ final passes ======
 
@@ -1356,10 +1378,21 @@ __global__ __launch_bounds__(256) void k_el_kfinal(ElecDev P, const int* __restr
     for (int t = 0; t < 15; ++t) out[(long)t * P.n + j] = acc[t];
 }
 
-// Sum the final-pass partials in chunk order and apply the prefactors (ewald1p.f:941-1001):
-// fk = scale * f, vesp_k, vesp_dc_k = scale * ..., e_ew1p = scal1 * e + self energy,
-// s_ew1p = -scal1 * s, vesp_s.  Block partials [nb][2]: sum of q^2 (self energy).
-__global__ __launch_bounds__(256) void k_el_kfinal_reduce(ElecDev P, const double* __restrict__ Fpart,
+// Sums of the final pass's site partials over the k chunks: ksum[t][j] for the 15 site
+// quantities t of k_el_kfinal, 64 elements (t, j) per block.
+__global__ __launch_bounds__(256) void k_el_ksum(ElecDev P, const double* __restrict__ Fpart,
+                                                 double* __restrict__ ksum)
+{
+    const long e = (long)blockIdx.x * 64 + (threadIdx.x & 63);
+    const bool valid = e < 15L * P.n;
+    const double v = chunk_sum64(Fpart, 16L * P.n, P.kchunks, e, valid);
+    if (threadIdx.x < 64 && valid) ksum[e] = v;
+}
+
+// Per-site results of the final reciprocal pass from the chunk sums, with the prefactors
+// (ewald1p.f:941-1001): fk = scale * f, vesp_k, vesp_dc_k = scale * ..., e_ew1p = scal1 * e +
+// self energy, s_ew1p = -scal1 * s, vesp_s.  Block partials [nb][2]: sum of q^2.
+__global__ __launch_bounds__(256) void k_el_kfinal_reduce(ElecDev P, const double* __restrict__ ksum,
                                                           const double* __restrict__ chge, double* __restrict__ fk,
                                                           double* __restrict__ vk, double* __restrict__ vdck,
                                                           double* __restrict__ vs, double* __restrict__ e1,
@@ -1369,11 +1402,7 @@ __global__ __launch_bounds__(256) void k_el_kfinal_reduce(ElecDev P, const doubl
     double v[2] = {0.0, 0.0};
     if (j < P.n) {
         double a[15];
-        for (int t = 0; t < 15; ++t) a[t] = 0.0;
-        for (int c = 0; c < P.kchunks; ++c) {
-            const double* in = Fpart + (long)c * 16 * P.n;
-            for (int t = 0; t < 15; ++t) a[t] += in[(long)t * P.n + j];
-        }
+        for (int t = 0; t < 15; ++t) a[t] = ksum[(long)t * P.n + j];
         const double q = chge[j];
         for (int t = 0; t < 3; ++t) fk[3 * j + t] = P.scale * a[t];
         vk[j] = a[3] * P.scale;
@@ -1814,15 +1843,21 @@ __global__ void k_el_combine(ElecDev P, int distribute, const double* __restrict
     for (int q = 0; q < 3; ++q) fout[3 * m + q] = distribute ? 0.0 : fm[q];
 }
 
-// Fixed-order sums of block partials: out[q] = sum_b blk[b * stride + q], q < nv.
-__global__ __launch_bounds__(64) void k_el_sum_blocks(const double* __restrict__ blk, int nb, int stride, int nv,
-                                                      double* __restrict__ out)
+// Fixed-order sums of block partials, one set per block of one wavefront (lane-strided, then across
+// the wavefront): set b has nb[b] blocks of stride[b] values, of which nv[b] are summed into
+// out[off[b] ..].
+struct SumSets {
+    const double* blk[6];
+    int nb[6], stride[6], nv[6], off[6];
+};
+__global__ __launch_bounds__(64) void k_el_sum_sets(SumSets S, double* __restrict__ out)
 {
-    for (int q = 0; q < nv; ++q) {
+    const int b = blockIdx.x;
+    for (int q = 0; q < S.nv[b]; ++q) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += 64) s += blk[(long)b * stride + q];
+        for (int k = threadIdx.x; k < S.nb[b]; k += 64) s += S.blk[b][(long)k * S.stride[b] + q];
         s = wsum(s);
-        if (threadIdx.x == 0) out[q] = s;
+        if (threadIdx.x == 0) out[S.off[b] + q] = s;
     }
 }
 
@@ -1866,11 +1901,11 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hi
     // charge structure factors and the iloop = 1 reciprocal fields
     hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div(2L * P.nrun, 256), E.schunks), dim3(256), sfac_lds(P), s, P, E.spc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
-    hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
+    hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
     hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 512), P.kchunks), dim3(256), kfield_lds(P), s, P, E.runs,
                        E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
-    hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(6L * P.n, 256)), dim3(256), 0, s, P, E.Fpart, E.dip, E.efdd,
+    hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(6L * P.n, 64)), dim3(256), 0, s, P, E.Fpart, E.dip, E.efdd,
                        E.efdc, E.state);
     return (int)hipGetLastError();
 }
@@ -1882,11 +1917,11 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
         // with the charge field)
         hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
-        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
+        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
                            E.S, E.Sq, E.kc, E.state);
         hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
                            E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
-        hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, E.dip,
+        hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, P, E.Fpart, E.dip,
                            E.efdd, E.efdc, E.state);
     }
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
@@ -1902,11 +1937,11 @@ static void launch_tfield(ElecBuffers& E, const ElecDev& P, const double* v, con
 {
     hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, v, E.Spart, state);
-    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
+    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, state);
     hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
                        E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, state);
-    hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, v, E.efdd,
+    hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, P, E.Fpart, v, E.efdd,
                        E.efdc, state);
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, v, E.emu, state);
 }
@@ -1924,22 +1959,21 @@ int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
 }
 
 // One iteration: T p (real-space rows, structure factors, reciprocal field), q = A p and
-// alpha, the update of mu, r, z with beta and the stopping test, the new direction.
+// alpha, then the update of mu, r, z with beta, the stopping test and the new direction.
 int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s)
 {
     const double* p = E.cg_p;
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
     hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, p, E.Spart, E.state);
-    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(P.K, 256)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
+    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
     hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
                        E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
-    hipLaunchKernelGGL(k_cg_apply, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd,
+    hipLaunchKernelGGL(k_cg_apply, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd,
                        E.cg_q, E.blk, E.ctr + 1, E.state);
-    hipLaunchKernelGGL(k_cg_update, dim3(ceil_div(P.nw, 256)), dim3(256), 0, s, P, E.cg_p, E.cg_q, E.cg_minv, E.dip,
+    hipLaunchKernelGGL(k_cg_update, dim3(ceil_div(P.nw, CG_WPB)), dim3(256), 0, s, P, E.cg_p, E.cg_q, E.cg_minv, E.dip,
                        E.cg_r, E.cg_z, E.blk, E.ctr + 2, E.state);
-    hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, E.cg_z, E.cg_p, E.state);
     return (int)hipGetLastError();
 }
 
@@ -1972,13 +2006,14 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     if (!refresh) {
         hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
-        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(nkb), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
+        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
                            (const double*)E.zero_state);
     }
     hipLaunchKernelGGL(k_el_kscal, dim3(nkb), dim3(256), 0, s, P, E.kv, E.S, E.Sq, E.kf, b_ks);
     hipLaunchKernelGGL(k_el_kfinal, dim3(ceil_div(P.n, 256), P.kchunks), dim3(256), kfinal_lds(P), s, P, E.runs,
                        E.runk, E.phc, E.phs, E.kv, E.kf, E.chge, E.dip, E.Fpart);
-    hipLaunchKernelGGL(k_el_kfinal_reduce, dim3(nsb), dim3(256), 0, s, P, E.Fpart, E.chge, E.fk, E.vk, E.vdck, E.vs,
+    hipLaunchKernelGGL(k_el_ksum, dim3(ceil_div(15L * P.n, 64)), dim3(256), 0, s, P, E.Fpart, E.ksum);
+    hipLaunchKernelGGL(k_el_kfinal_reduce, dim3(nsb), dim3(256), 0, s, P, E.ksum, E.chge, E.fk, E.vk, E.vdck, E.vs,
                        E.e1, E.s1, b_q2);
     hipLaunchKernelGGL(k_el_pairs, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.dip, E.tab, E.nbrA, E.cntA,
                        g34, gln34, E.f2, E.e2, E.vr, E.vdcr, E.w2, E.fmsum, E.fmx, b_p2);
@@ -1990,12 +2025,17 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
                        E.e2, E.e3, E.ep, E.s1, E.w2, E.w3, E.fmsum, E.fmx, E.fsite, E.eout, E.sout, E.wout);
     // scalars: [0] eng_polar, [1..8] k sums (engc, engdum, omg6), [9..10] sum q^2,
     // [11..18] pairs (energy, virial, stress6), [19..26] excluded pairs, [27..34] qd
-    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_pol, nsb, 1, 1, E.scal);
-    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_ks, nkb, 8, 8, E.scal + 1);
-    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_q2, nsb, 2, 2, E.scal + 9);
-    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_p2, P.n, 8, 8, E.scal + 11);
-    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_p3, nwb, 8, 8, E.scal + 19);
-    hipLaunchKernelGGL(k_el_sum_blocks, dim3(1), dim3(64), 0, s, b_qd, nwb, 8, 8, E.scal + 27);
+    SumSets ss{};
+    const double* sb[6] = {b_pol, b_ks, b_q2, b_p2, b_p3, b_qd};
+    const int snb[6] = {nsb, nkb, nsb, P.n, nwb, nwb}, sst[6] = {1, 8, 2, 8, 8, 8}, soff[6] = {0, 1, 9, 11, 19, 27};
+    for (int q = 0; q < 6; ++q) {
+        ss.blk[q] = sb[q];
+        ss.nb[q] = snb[q];
+        ss.stride[q] = sst[q];
+        ss.nv[q] = sst[q];
+        ss.off[q] = soff[q];
+    }
+    hipLaunchKernelGGL(k_el_sum_sets, dim3(6), dim3(64), 0, s, ss, E.scal);
     return (int)hipGetLastError();
 }
 

@@ -128,6 +128,7 @@ struct ElecBuffers {
     double *cg_p = nullptr, *cg_q = nullptr, *cg_r = nullptr, *cg_z = nullptr;                // [3n] solver vectors
     double* cg_minv = nullptr;                                 // [nw][45] inverse preconditioner blocks
     unsigned int* ctr = nullptr;                               // [4] arrival counters of the solver kernels
+    double* ksum = nullptr;                                    // [16][n] chunk sums of the final pass
     double* state = nullptr;                                   // [8] solver scalars
     double* kv = nullptr;                                      // [K][8] kx ky kz akk akv akv2 . .
     int* runs = nullptr;                                       // [nrun][4] l, m, n0, count; k0 in runk
@@ -168,6 +169,7 @@ struct ElecState {
     ElecBuffers E;
     ElecDev P{};
     bool ready = false;                 // the last evaluation completed
+    int last_iters = 0;                 // solver iterations of the last evaluation
     int nw = 0;
     double tab_alpha = -1.0, tab_rcut = -1.0;           // key of the B-function table
     double k_cell[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};     // key of the k-vector list
