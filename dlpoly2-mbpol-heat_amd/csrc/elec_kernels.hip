@@ -1318,10 +1318,15 @@ __global__ __launch_bounds__(256) void k_el_kfinal(ElecDev P, const int* __restr
     const int ks = P.kstride;
     const bool pol = site_polr2(P, j) > 1.e-6;
     const double q = chge[j];
-    const double mu[3] = {dip[3 * j], dip[3 * j + 1], dip[3 * j + 2]};
-    double acc[15];
-#pragma unroll
-    for (int t = 0; t < 15; ++t) acc[t] = 0.0;
+    const double mu[3] = {pol ? dip[3 * j] : 0.0, pol ? dip[3 * j + 1] : 0.0, pol ? dip[3 * j + 2] : 0.0};
+    // Per k, with km = k.mu, the stress term of ewald1p.f:771-825,
+    //   akk [B_ab (cc + cd + dd) + k_a X_b],  B_ab = delta_ab - akv2 k_a k_b,
+    //   X_b = (s q + c km) Mc_b - (c q - s km) Ms_b + (c (Qs + k.Mc) + s (k.Ms - Qc)) mu_b,
+    // is accumulated as delta_ab D - G_ab + Pm_ab + mu_b R_a with D = sum akk T, G_ab =
+    // sum akk akv2 T k_a k_b, Pm_ab = sum k_a akk (..) and R_a = sum akk (..) k_a
+    // (T = cc + cd + dd); the site energy sum akk q (c Qc + s Qs) is q times acc3.
+    double f3[3] = {0.0, 0.0, 0.0}, vk_ = 0.0, vd_ = 0.0, D = 0.0;
+    double G[6] = {0, 0, 0, 0, 0, 0}, Pm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, R[3] = {0, 0, 0};
     const double c1 = phc[((long)2 * ks + 1) * P.n + j], s1 = phs[((long)2 * ks + 1) * P.n + j];
     int kk = 0;
     for (int r = 0; r < nr; ++r) {
@@ -1333,46 +1338,65 @@ __global__ __launch_bounds__(256) void k_el_kfinal(ElecDev P, const int* __restr
                   phs[((long)2 * ks + an0) * P.n + j], m, n0, ckr, skr);
         for (int t = 0; t < cnt; ++t) {
             const double* f = sk + EL_KF * (kk + t);
-            const double kk3[3] = {f[0], f[1], f[2]};
+            const double kx = f[0], ky = f[1], kz = f[2];
             const double akk = f[3], akv2 = f[4];
             const double Qc = f[5], Qs = f[6], summu1 = f[7], summu2 = f[8];
             const double dpkMc = f[9], dpkMs = f[10];
-            const double* Mc = f + 11;
-            const double* Ms = f + 14;
-            const double djmuk = pol ? (mu[0] * kk3[0] + mu[1] * kk3[1] + mu[2] * kk3[2]) : 0.0;
-            const double ssmu = summu1 * (-skr * q - ckr * djmuk) + summu2 * (-skr * djmuk + ckr * q);
-            acc[0] -= kk3[0] * akk * ssmu;
-            acc[1] -= kk3[1] * akk * ssmu;
-            acc[2] -= kk3[2] * akk * ssmu;
-            acc[3] += akk * (ckr * Qc + skr * Qs);
-            acc[4] += akk * (skr * dpkMc - ckr * dpkMs);
-            const double Cqi = q * ckr, Sqi = q * skr;
-            const double CMi[3] = {mu[0] * ckr, mu[1] * ckr, mu[2] * ckr};
-            const double SMi[3] = {mu[0] * skr, mu[1] * skr, mu[2] * skr};
-            const double dpkCMi = kk3[0] * CMi[0] + kk3[1] * CMi[1] + kk3[2] * CMi[2];
-            const double dpkSMi = kk3[0] * SMi[0] + kk3[1] * SMi[1] + kk3[2] * SMi[2];
-            const double cc = Cqi * Qc + Sqi * Qs;
-            const double cd = Sqi * dpkMc - Cqi * dpkMs - dpkSMi * Qc + dpkCMi * Qs;
-            const double dd = dpkCMi * dpkMc + dpkSMi * dpkMs;
-            acc[5] += akk * cc;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    const double B = (a == b ? 1.0 : 0.0) - akv2 * kk3[a] * kk3[b];
-                    const double sCC = B * cc;
-                    const double sCD = B * cd + Sqi * kk3[a] * Mc[b] - Cqi * kk3[a] * Ms[b] - kk3[a] * SMi[b] * Qc +
-                                       kk3[a] * CMi[b] * Qs;
-                    const double sDD = B * dd + kk3[a] * CMi[b] * dpkMc + kk3[a] * SMi[b] * dpkMs +
-                                       dpkCMi * kk3[a] * Mc[b] + dpkSMi * kk3[a] * Ms[b];
-                    acc[6 + 3 * a + b] += akk * (sCC + sCD + sDD);
-                }
+            const double km = mu[0] * kx + mu[1] * ky + mu[2] * kz;
+            const double ssmu = summu1 * (-skr * q - ckr * km) + summu2 * (-skr * km + ckr * q);
+            const double wf = akk * ssmu;
+            f3[0] -= kx * wf;
+            f3[1] -= ky * wf;
+            f3[2] -= kz * wf;
+            const double cq = ckr * Qc + skr * Qs;
+            vk_ += akk * cq;
+            vd_ += akk * (skr * dpkMc - ckr * dpkMs);
+            const double T = q * cq + q * (skr * dpkMc - ckr * dpkMs) + km * (ckr * Qs - skr * Qc) +
+                             km * (ckr * dpkMc + skr * dpkMs);
+            const double aT = akk * T;
+            D += aT;
+            const double u = aT * akv2;
+            const double ux = u * kx, uy = u * ky, uz = u * kz;
+            G[0] += ux * kx;
+            G[1] += ux * ky;
+            G[2] += ux * kz;
+            G[3] += uy * ky;
+            G[4] += uy * kz;
+            G[5] += uz * kz;
+            const double b1 = akk * (skr * q + ckr * km), b2 = akk * (ckr * q - skr * km);
+            const double Y0 = b1 * f[11] - b2 * f[14], Y1 = b1 * f[12] - b2 * f[15], Y2 = b1 * f[13] - b2 * f[16];
+            Pm[0] += kx * Y0;
+            Pm[1] += kx * Y1;
+            Pm[2] += kx * Y2;
+            Pm[3] += ky * Y0;
+            Pm[4] += ky * Y1;
+            Pm[5] += ky * Y2;
+            Pm[6] += kz * Y0;
+            Pm[7] += kz * Y1;
+            Pm[8] += kz * Y2;
+            const double a3 = akk * (ckr * (Qs + dpkMc) + skr * (dpkMs - Qc));
+            R[0] += a3 * kx;
+            R[1] += a3 * ky;
+            R[2] += a3 * kz;
             const double cn = ckr * c1 - skr * s1;
             skr = skr * c1 + ckr * s1;
             ckr = cn;
         }
         kk += cnt;
     }
+    double acc[15];
+    acc[0] = f3[0];
+    acc[1] = f3[1];
+    acc[2] = f3[2];
+    acc[3] = vk_;
+    acc[4] = vd_;
+    acc[5] = q * vk_;
+    const int gi[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            acc[6 + 3 * a + b] = (a == b ? D : 0.0) - G[gi[a][b]] + Pm[3 * a + b] + mu[b] * R[a];
     double* out = Fpart + (long)chunk * 16 * P.n;
 #pragma unroll
     for (int t = 0; t < 15; ++t) out[(long)t * P.n + j] = acc[t];
