@@ -571,9 +571,10 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
     const int lmin = runs[4 * rfirst], lmax = runs[4 * rlast];
     const int nl = lmax - lmin + 1, nm = P.km[1] + 1, nn = P.km[2] + 1;
     const int nrows = nl + nm + nn;
-    double* tc = lds;
-    double* tsn = lds + nrows * EL_TROW;
-    double* tw = tsn + nrows * EL_TROW;
+    // (cos, sin) pairs of the tile's table rows, and (q, mu) of its sites, as double2 so
+    // that each entry is one 16-byte LDS read (dynamic LDS starts 16-byte aligned)
+    double2* tcs = reinterpret_cast<double2*>(lds);
+    double2* tw = tcs + nrows * EL_TROW;
     const int ks = P.kstride;
     int l = lmin, m = 0, nst = 1, cnt = 0;
     if (r < P.nrun) {
@@ -620,9 +621,14 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
             }
         }
         if (tid < 4 * EL_TILE) {
+            // a site without an induced dipole carries mu = 0, so no test in the sums
             const int sj = tid >> 2, w = tid & 3;
             const int j = t0 + sj;
-            pw = sj < tn ? (w == 0 ? chge[j] : dip[3 * j + w - 1]) : 0.0;
+            pw = 0.0;
+            if (sj < tn) {
+                if (w == 0) pw = chge[j];
+                else if (site_polr2(P, j) > 1.e-6) pw = dip[3 * j + w - 1];
+            }
         }
     };
     if (j0 < j1) fetch(j0);
@@ -633,42 +639,35 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
         for (int u = 0; u < PF; ++u) {
             const int idx = tid + 256 * u;
             const int row = idx / EL_TILE, sidx = idx - row * EL_TILE;
-            if (row < nrows) {
-                tc[row * EL_TROW + sidx] = pc[u];
-                tsn[row * EL_TROW + sidx] = ps[u];
-            }
+            if (row < nrows) tcs[row * EL_TROW + sidx] = make_double2(pc[u], ps[u]);
         }
-        if (tid < 4 * EL_TILE) tw[tid] = pw;
+        if (tid < 4 * EL_TILE) reinterpret_cast<double*>(tw)[tid] = pw;
         __syncthreads();
         if (t0 + EL_TILE < j1) fetch(t0 + EL_TILE);
         if (cnt > 0) {
+            // every thread steps EL_HALF k; terms past the end of its run are never stored
+#pragma unroll 2
             for (int sj = 0; sj < tn; ++sj) {
-                const bool pol = site_polr2(P, t0 + sj) > 1.e-6;
-                if (!CHG && !pol) continue;
+                const double2 A = tcs[rowA + sj], B = tcs[rowB + sj], C = tcs[rowC + sj], C1 = tcs[rowC1 + sj];
+                const double2 w01 = tw[2 * sj], w23 = tw[2 * sj + 1];
                 double ckr, skr;
-                run_phase(tc[rowA + sj], tsn[rowA + sj], tc[rowB + sj], tsn[rowB + sj], tc[rowC + sj],
-                          tsn[rowC + sj], m, nst, ckr, skr);
-                const double c1 = tc[rowC1 + sj], s1 = tsn[rowC1 + sj];
-                const double q = tw[4 * sj], mx = tw[4 * sj + 1], my = tw[4 * sj + 2], mz = tw[4 * sj + 3];
+                run_phase(A.x, A.y, B.x, B.y, C.x, C.y, m, nst, ckr, skr);
+                const double q = w01.x, mx = w01.y, my = w23.x, mz = w23.y;
 #pragma unroll
                 for (int t = 0; t < EL_HALF; ++t) {
-                    if (t < cnt) {
-                        if (CHG) {
-                            acc[t][0] += q * ckr;
-                            acc[t][1] += q * skr;
-                        }
-                        if (pol) {
-                            acc[t][2] += ckr * mx;
-                            acc[t][3] += ckr * my;
-                            acc[t][4] += ckr * mz;
-                            acc[t][5] += skr * mx;
-                            acc[t][6] += skr * my;
-                            acc[t][7] += skr * mz;
-                        }
-                        const double cn = ckr * c1 - skr * s1;
-                        skr = skr * c1 + ckr * s1;
-                        ckr = cn;
+                    if (CHG) {
+                        acc[t][0] += q * ckr;
+                        acc[t][1] += q * skr;
                     }
+                    acc[t][2] += ckr * mx;
+                    acc[t][3] += ckr * my;
+                    acc[t][4] += ckr * mz;
+                    acc[t][5] += skr * mx;
+                    acc[t][6] += skr * my;
+                    acc[t][7] += skr * mz;
+                    const double cn = ckr * C1.x - skr * C1.y;
+                    skr = skr * C1.x + ckr * C1.y;
+                    ckr = cn;
                 }
             }
         }
