@@ -17,6 +17,9 @@
 #ifndef EL_DCHUNK_MAX
 #define EL_DCHUNK_MAX 32      // site chunks of the dipole structure-factor pass (partials re-read by the sum)
 #endif
+#ifndef EL_KCHUNK_WAVES
+#define EL_KCHUNK_WAVES 4096  // site-block x k-chunk product the reciprocal field passes aim at
+#endif
 #ifndef EL_DCHUNK_WAVES
 #define EL_DCHUNK_WAVES 2048  // wavefronts the dipole structure-factor pass aims at
 #endif
@@ -298,7 +301,7 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
         ECHK(dalloc(&E.phs, (size_t)3 * ks * E.n));
     }
     // k split of the site-parallel passes: about 2048 wavefronts over the sites
-    E.kchunks = std::max(1, std::min(E.nrun, 4096 / std::max(1, (E.n + 63) / 64)));
+    E.kchunks = std::max(1, std::min(E.nrun, EL_KCHUNK_WAVES / std::max(1, (E.n + 63) / 64)));
     E.rpc = (E.nrun + E.kchunks - 1) / E.kchunks;
     E.kchunks = (E.nrun + E.rpc - 1) / E.rpc;
     E.kmaxch = 0;

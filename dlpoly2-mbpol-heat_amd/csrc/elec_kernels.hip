@@ -775,22 +775,35 @@ __global__ __launch_bounds__(256) void k_el_kfield(ElecDev P, const int* __restr
 #pragma unroll
         for (int t = 0; t < EL_RUN; ++t) {
             if (t < cnt) {
+                // one fused multiply-add per product into the accumulators
                 const double* c = sk + NC * (kk + t);
                 const double c0 = c[0], c1v = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5];
-                fa[0] += c0 * sA + c3 * cA;
-                fa[1] += c1v * sA + c4 * cA;
-                fa[2] += c2 * sA + c5 * cA;
-                fb[0] += c0 * sB + c3 * cB;
-                fb[1] += c1v * sB + c4 * cB;
-                fb[2] += c2 * sB + c5 * cB;
+                fa[0] += c0 * sA;
+                fa[0] += c3 * cA;
+                fa[1] += c1v * sA;
+                fa[1] += c4 * cA;
+                fa[2] += c2 * sA;
+                fa[2] += c5 * cA;
+                fb[0] += c0 * sB;
+                fb[0] += c3 * cB;
+                fb[1] += c1v * sB;
+                fb[1] += c4 * cB;
+                fb[2] += c2 * sB;
+                fb[2] += c5 * cB;
                 if (CHG) {
                     const double c6 = c[6], c7 = c[7], c8 = c[8], c9 = c[9], c10 = c[10], c11 = c[11];
-                    ga[0] += c6 * sA + c9 * cA;
-                    ga[1] += c7 * sA + c10 * cA;
-                    ga[2] += c8 * sA + c11 * cA;
-                    gb[0] += c6 * sB + c9 * cB;
-                    gb[1] += c7 * sB + c10 * cB;
-                    gb[2] += c8 * sB + c11 * cB;
+                    ga[0] += c6 * sA;
+                    ga[0] += c9 * cA;
+                    ga[1] += c7 * sA;
+                    ga[1] += c10 * cA;
+                    ga[2] += c8 * sA;
+                    ga[2] += c11 * cA;
+                    gb[0] += c6 * sB;
+                    gb[0] += c9 * cB;
+                    gb[1] += c7 * sB;
+                    gb[1] += c10 * cB;
+                    gb[2] += c8 * sB;
+                    gb[2] += c11 * cB;
                 }
                 const double na = cA * ca1 - sA * sa1;
                 sA = sA * ca1 + cA * sa1;
