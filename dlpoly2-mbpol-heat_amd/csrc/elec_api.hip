@@ -87,7 +87,7 @@ void free_buffers(ElecBuffers& E)
     dfree(E.x); dfree(E.y); dfree(E.z); dfree(E.chge); dfree(E.polr); dfree(E.polr2); dfree(E.dg); dfree(E.tab);
     dfree(E.nbrA); dfree(E.cntA); dfree(E.nbrP); dfree(E.cntP); dfree(E.tens); dfree(E.tin);
     dfree(E.dip); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd);
-    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.ctr); dfree(E.ksum); dfree(E.state); dfree(E.zero_state);
+    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.cgblk); dfree(E.ksum); dfree(E.state); dfree(E.zero_state);
     dfree(E.kv); dfree(E.klmn); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs);
     dfree(E.S); dfree(E.Spart); dfree(E.Sq); dfree(E.kc); dfree(E.kf); dfree(E.Fpart);
     dfree(E.vk); dfree(E.vr); dfree(E.vs); dfree(E.vc); dfree(E.vdck); dfree(E.vdcr); dfree(E.vesp);
@@ -111,10 +111,9 @@ int alloc_sites(ElecState& S, int nw, std::string& err)
     ECHK(dalloc(&E.cg_p, (size_t)3 * n)); ECHK(dalloc(&E.cg_q, (size_t)3 * n));
     ECHK(dalloc(&E.cg_r, (size_t)3 * n)); ECHK(dalloc(&E.cg_z, (size_t)3 * n));
     ECHK(dalloc(&E.cg_minv, (size_t)45 * nw));
-    ECHK(dalloc(&E.ctr, 4));
+    ECHK(dalloc(&E.cgblk, (size_t)(9 * nw + 63) / 64 + 2 * ((nw + 27) / 28) + 3 * ((nw + 255) / 256) + 64));
     ECHK(dalloc(&E.ksum, (size_t)16 * n));
-    ECHK(hipMemset(E.ctr, 0, 4 * sizeof(unsigned int)));
-    ECHK(dalloc(&E.state, 8)); ECHK(dalloc(&E.zero_state, 8));
+    ECHK(dalloc(&E.state, 16)); ECHK(dalloc(&E.zero_state, 8));
     ECHK(dalloc(&E.vk, n)); ECHK(dalloc(&E.vr, n)); ECHK(dalloc(&E.vs, n)); ECHK(dalloc(&E.vc, n));
     ECHK(dalloc(&E.vdck, n)); ECHK(dalloc(&E.vdcr, n)); ECHK(dalloc(&E.vesp, n));
     ECHK(dalloc(&E.fk, (size_t)3 * n)); ECHK(dalloc(&E.f2, (size_t)3 * n)); ECHK(dalloc(&E.f3, (size_t)3 * n));
@@ -435,7 +434,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         P.capA = E.capA;
         P.capP = E.capP;
     }
-    ECHK(hipMemsetAsync(E.state, 0, 8 * sizeof(double), s));
+    ECHK(hipMemsetAsync(E.state, 0, 16 * sizeof(double), s));
     ECHK(hipMemsetAsync(E.emu, 0, sizeof(double) * 3 * P.n, s));
     // M sites carry no induced dipole (polr2 = 0): clear any start value
     ECHK(hipMemsetAsync(E.dip + 3 * P.nreal, 0, sizeof(double) * 3 * (P.n - P.nreal), s));
@@ -459,7 +458,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     }
     for (;;) {
         for (int b = 0; b < batch; ++b) {
-            rc = solver == 1 ? el_launch_cg_iteration(E, P, s) : el_launch_iteration(E, P, launched == 0, s);
+            rc = solver == 1 ? el_launch_cg_iteration(E, P, launched, s) : el_launch_iteration(E, P, launched == 0, s);
             ++launched;
             if (rc) {
                 err = std::string("electrostatics iteration kernels failed: ") + hipGetErrorString((hipError_t)rc);

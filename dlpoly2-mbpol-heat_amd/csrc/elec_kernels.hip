@@ -57,44 +57,6 @@ __device__ void block_sum_store(double (&v)[NV], double* out)
     }
 }
 
-// Called by every thread of a block after block_sum_store.  The block that arrives last
-// (ticket from a per-kernel counter) sums the nb blocks' partials in block order on its
-// first wavefront, resets the counter for the next launch and returns true on all of its
-// threads, with the totals in tot; the other blocks return false.  The fences order each
-// block's partial stores before its ticket and the last block's loads after it, across the
-// L2s of all dies.
-template <int NV>
-__device__ bool last_block_sum(unsigned int* __restrict__ ctr, const double* blk, int nb, double (&tot)[NV])
-{
-    __shared__ unsigned int ticket;
-    __shared__ double st[NV];
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) ticket = atomicAdd(ctr, 1u);
-    __syncthreads();
-    if (ticket != (unsigned int)(nb - 1)) return false;
-    __threadfence();
-    if (threadIdx.x < 64) {
-        double t[NV];
-#pragma unroll
-        for (int q = 0; q < NV; ++q) t[q] = 0.0;
-        for (int k = threadIdx.x; k < nb; k += 64)
-#pragma unroll
-            for (int q = 0; q < NV; ++q) t[q] += blk[NV * k + q];
-#pragma unroll
-        for (int q = 0; q < NV; ++q) t[q] = wsum(t[q]);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int q = 0; q < NV; ++q) st[q] = t[q];
-            atomicExch(ctr, 0u);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] = st[q];
-    return true;
-}
-
 // Sum over chunks of in[c * cs + off] (c = 0..nc-1) for the 64 elements of a 256-thread
 // block, one element per lane: wave w adds the chunks [w nc / 4, (w + 1) nc / 4) in order
 // (coalesced loads of 64 consecutive elements), and the four wave sums are added in wave
@@ -961,7 +923,11 @@ __global__ __launch_bounds__(64) void k_el_converge(ElecDev P, int nb, const dou
 // couplings (half the iterations of a diagonal preconditioner at liquid density).  The
 // stopping test is the reference's: the RMS of the step an SOR update would take from the
 // current dipoles, (1 - omega) * rms(polr2 r / r4pie0), against toler (iter.f:95-101).
-// state = [iterations, rms step, done, r.z, alpha, beta, dipole count].
+// state = [iterations, rms step, done, r.z, alpha, beta, dipole count, -, r.z by iteration
+// parity (two slots)].  The scalar steps need no grid-wide synchronization: every block of
+// the kernel that needs alpha or beta sums the previous kernel's block partials itself, in
+// the same fixed order, and the current r.z sits in the slot of the iteration's parity
+// (passed by the host), which that kernel only reads; its block 0 writes the other slot.
 
 // Scalar steps of the solver from the summed partials (a, b, c):
 //   mode 0 (start):        r.z = a, rms test on b, dipole count c -> state[6]
@@ -1104,8 +1070,7 @@ __global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __rest
                                                  const double* __restrict__ efdd, const double* __restrict__ dip,
                                                  const double* __restrict__ minv, double* __restrict__ r,
                                                  double* __restrict__ z, double* __restrict__ pv,
-                                                 double* __restrict__ blk, unsigned int* __restrict__ ctr,
-                                                 double* __restrict__ state)
+                                                 double* __restrict__ blk)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -1135,19 +1100,53 @@ __global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __rest
         }
     }
     block_sum_store(v, blk + 3 * blockIdx.x);
-    double t[3];
-    if (last_block_sum(ctr, blk, gridDim.x, t) && threadIdx.x == 0) cg_scalars(P, 0, t[0], t[1], t[2], state);
+}
+
+// Sum of nb block partials (value q of stride nv) on the first wavefront, in block order;
+// the result is returned to every thread of the block.
+template <int NV>
+__device__ double block_partials_sum(const double* __restrict__ blk, int nb, int q)
+{
+    __shared__ double res;
+    if (threadIdx.x < 64) {
+        double a = 0.0;
+        for (int k = threadIdx.x; k < nb; k += 64) a += blk[NV * k + q];
+        a = wsum(a);
+        if (threadIdx.x == 0) res = a;
+    }
+    __syncthreads();
+    const double r = res;
+    __syncthreads();
+    return r;
+}
+
+// Start of the solver from k_cg_init's partials: r.z, the stopping test, the dipole count;
+// r.z also into the parity-0 slot (single wavefront)
+__global__ __launch_bounds__(64) void k_cg_start(ElecDev P, int nb, const double* __restrict__ blk,
+                                                 double* __restrict__ state)
+{
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 64) {
+        a += blk[3 * k];
+        b += blk[3 * k + 1];
+        c += blk[3 * k + 2];
+    }
+    a = wsum(a);
+    b = wsum(b);
+    c = wsum(c);
+    if (threadIdx.x != 0) return;
+    cg_scalars(P, 0, a, b, c, state);
+    state[8] = a;
 }
 
 // The reciprocal dipole field of p from the chunk partials (efdd = scale * sum + self
 // term, as k_el_kfield_reduce), then q = A p = r4pie0 p / polr2 - (emu + efdd) and the
-// partials of p.q; the last block sets alpha.  Element (component, site) as in
+// partials of p.q.  Element (component, site) as in
 // k_el_kfield_reduce, 64 per block.
 __global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __restrict__ Fpart,
                                                   const double* __restrict__ pv, const double* __restrict__ emu,
                                                   double* __restrict__ efdd, double* __restrict__ qv,
-                                                  double* __restrict__ blk, unsigned int* __restrict__ ctr,
-                                                  double* __restrict__ state)
+                                                  double* __restrict__ blk, const double* __restrict__ state)
 {
     if (state[2] != 0.0) return;
     const int t = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -1165,32 +1164,29 @@ __global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __res
         v[0] = pv[o] * qq;
     }
     block_sum_store(v, blk + blockIdx.x);
-    double tot[1];
-    if (last_block_sum(ctr, blk, gridDim.x, tot) && threadIdx.x == 0) cg_scalars(P, 1, tot[0], 0.0, 0.0, state);
 }
 
 // mu += alpha p, r -= alpha q, z = M^-1 r, partials (r.z, |polr2 r / r4pie0|^2); one
 // thread per (water, row), CG_WPB waters per block, the new r of a water shared through
-// LDS.  The last block sets beta, the iteration count and the stopping test, and then
-// forms the next direction p = z + beta p over all rows.
+// LDS.  alpha = r.z / p.q, with p.q summed from k_cg_apply's nbq partials by every block.
 #define CG_WPB 28
-__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, double* __restrict__ pv, const double* __restrict__ qv,
+__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int it, int nbq, const double* __restrict__ blkq,
+                                                   const double* __restrict__ pv, const double* __restrict__ qv,
                                                    const double* __restrict__ minv, double* __restrict__ dip,
                                                    double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ blk, unsigned int* __restrict__ ctr,
-                                                   double* __restrict__ state)
+                                                   double* __restrict__ blk, double* __restrict__ state)
 {
     if (state[2] != 0.0) return;
     __shared__ double rs[CG_WPB * 9];
-    __shared__ double beta_s;
     const int tid = threadIdx.x;
+    const double alpha = state[8 + (it & 1)] / block_partials_sum<1>(blkq, nbq, 0);
+    if (blockIdx.x == 0 && tid == 0) state[4] = alpha;
     const int wl = tid / 9, i = tid - 9 * wl;
     const int w = blockIdx.x * CG_WPB + wl;
     const bool act = wl < CG_WPB && w < P.nw;
     const int o = 9 * w + i;
     const double p2 = act ? site_polr2(P, 3 * w + i / 3) : 0.0;
     const bool on = act && p2 > P.toler;
-    const double alpha = state[4];
     double rr = 0.0, zs = 0.0;
     if (on) {
         dip[o] += alpha * pv[o];
@@ -1209,15 +1205,30 @@ __global__ __launch_bounds__(256) void k_cg_update(ElecDev P, double* __restrict
     }
     double v[2] = {rr * zz, zs * zs};
     block_sum_store(v, blk + 2 * blockIdx.x);
-    double t[2];
-    if (!last_block_sum(ctr, blk, gridDim.x, t)) return;
-    if (tid == 0) {
-        cg_scalars(P, 2, t[0], t[1], 0.0, state);
-        beta_s = state[5];
-    }
-    __syncthreads();
-    const double beta = beta_s;
-    for (int e = tid; e < 3 * P.nrow; e += 256) pv[e] = z[e] + beta * pv[e];
+}
+
+// p = z + beta p with beta = r.z(new) / r.z, r.z(new) summed from k_cg_update's nbu
+// partials by every block; block 0 records the iteration: the new r.z into the other
+// parity slot, beta, the iteration count and the stopping test.
+__global__ __launch_bounds__(256) void k_cg_direction(ElecDev P, int it, int nbu, const double* __restrict__ blku,
+                                                      const double* __restrict__ z, double* __restrict__ pv,
+                                                      double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const double rz = block_partials_sum<2>(blku, nbu, 0);
+    const double beta = rz / state[8 + (it & 1)];
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o < 3 * P.nrow) pv[o] = z[o] + beta * pv[o];
+    if (blockIdx.x != 0) return;
+    const double zz = block_partials_sum<2>(blku, nbu, 1);
+    if (threadIdx.x != 0) return;
+    const double rms = (1.0 - P.omega) * sqrt(zz / state[6]);
+    state[8 + ((it + 1) & 1)] = rz;
+    state[5] = beta;
+    state[3] = rz;
+    state[0] += 1.0;
+    state[1] = rms;
+    if (rms <= P.toler) state[2] = 1.0;
 }
 
 
@@ -1982,6 +1993,10 @@ static void launch_tfield(ElecBuffers& E, const ElecDev& P, const double* v, con
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, v, E.emu, state);
 }
 
+// Block partials of the solver kernels in E.cgblk: k_cg_apply's at the start, then
+// k_cg_update's (disjoint, since every update block reads all of k_cg_apply's).
+static inline int cg_nbq(const ElecDev& P) { return ceil_div(3 * P.nrow, 64); }
+
 // conjugate-gradient start: residual of the start dipoles (their fields came with the
 // charge fields of the preparation, plus the real-space dipole field here)
 int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
@@ -1990,15 +2005,20 @@ int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
     hipLaunchKernelGGL(k_cg_block, dim3(nbw), dim3(256), 0, s, P, E.x, E.y, E.z, E.cg_minv);
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
     hipLaunchKernelGGL(k_cg_init, dim3(nbw), dim3(256), 0, s, P, E.efk, E.efdc, E.emu, E.efdd, E.dip, E.cg_minv, E.cg_r,
-                       E.cg_z, E.cg_p, E.blk, E.ctr, E.state);
+                       E.cg_z, E.cg_p, E.cgblk);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s, P, nbw, E.cgblk, E.state);
     return (int)hipGetLastError();
 }
 
-// One iteration: T p (real-space rows, structure factors, reciprocal field), q = A p and
-// alpha, then the update of mu, r, z with beta, the stopping test and the new direction.
-int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s)
+// Iteration `it` (0-based): T p (real-space rows, structure factors, reciprocal field),
+// q = A p, the update of mu, r, z with alpha, then the new direction with beta and the
+// stopping test.
+int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t s)
 {
     const double* p = E.cg_p;
+    const int nbq = cg_nbq(P), nbu = ceil_div(P.nw, CG_WPB);
+    double* blkq = E.cgblk;
+    double* blku = E.cgblk + nbq;
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
     hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, p, E.Spart, E.state);
@@ -2006,10 +2026,11 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s)
                        E.Sq, E.kc, E.state);
     hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
                        E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
-    hipLaunchKernelGGL(k_cg_apply, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd,
-                       E.cg_q, E.blk, E.ctr + 1, E.state);
-    hipLaunchKernelGGL(k_cg_update, dim3(ceil_div(P.nw, CG_WPB)), dim3(256), 0, s, P, E.cg_p, E.cg_q, E.cg_minv, E.dip,
-                       E.cg_r, E.cg_z, E.blk, E.ctr + 2, E.state);
+    hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
+    hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, P, it, nbq, blkq, E.cg_p, E.cg_q, E.cg_minv, E.dip,
+                       E.cg_r, E.cg_z, blku, E.state);
+    hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, it, nbu, blku, E.cg_z,
+                       E.cg_p, E.state);
     return (int)hipGetLastError();
 }
 

@@ -127,9 +127,9 @@ struct ElecBuffers {
     double *dip = nullptr, *efk = nullptr, *emu = nullptr, *efdc = nullptr, *efdd = nullptr;  // [3n]
     double *cg_p = nullptr, *cg_q = nullptr, *cg_r = nullptr, *cg_z = nullptr;                // [3n] solver vectors
     double* cg_minv = nullptr;                                 // [nw][45] inverse preconditioner blocks
-    unsigned int* ctr = nullptr;                               // [4] arrival counters of the solver kernels
+    double* cgblk = nullptr;                                   // block partials of the solver kernels
     double* ksum = nullptr;                                    // [16][n] chunk sums of the final pass
-    double* state = nullptr;                                   // [8] solver scalars
+    double* state = nullptr;                                   // [16] solver scalars
     double* kv = nullptr;                                      // [K][8] kx ky kz akk akv akv2 . .
     int* runs = nullptr;                                       // [nrun][4] l, m, n0, count; k0 in runk
     int* runk = nullptr;                                       // [nrun] index of the run's first k
@@ -159,7 +159,7 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
 int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
                     hipStream_t s);
 int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s);
-int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, hipStream_t s);
+int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t s);
 int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEvent_t b, hipStream_t s);
 void el_launch_zero(double* p, long n, hipStream_t s);
 void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
