@@ -838,28 +838,26 @@ __global__ __launch_bounds__(64) void k_el_emu(ElecDev P, const int* __restrict_
         e[1] += t1 * mx + t3 * my + t4 * mz;
         e[2] += t2 * mx + t4 * my + t5 * mz;
     }
-    for (int q = 0; q < 3; ++q) e[q] = wsum(e[q]);
-    if (lane == 0) {
-        // excluded partners of the same water (pair order (O,H1) (O,H2) (O,M) (H1,H2) (H1,M) (H2,M))
-        const int w = water_of(P, i);
-        const int si = i < P.nreal ? i - 3 * w : 3;
-        const int s[4] = {3 * w, 3 * w + 1, 3 * w + 2, P.nreal + w};
-        int pr = 0;
-        for (int a = 0; a < 4; ++a)
-            for (int b = a + 1; b < 4; ++b, ++pr) {
-                if (a != si && b != si) continue;
-                const int j = (a == si) ? s[b] : s[a];
-                // a partner without an induced dipole (the M site) contributes nothing, and
-                // its entries of the vector are not part of the solve: skip it
-                if (!(site_polr2(P, j) > 1.e-6)) continue;
-                const double* dt = tin + 36 * w + 6 * pr;
-                const double mx = dip[3 * j] * R4PIE0, my = dip[3 * j + 1] * R4PIE0, mz = dip[3 * j + 2] * R4PIE0;
-                e[0] += dt[0] * mx + dt[1] * my + dt[2] * mz;
-                e[1] += dt[1] * mx + dt[3] * my + dt[4] * mz;
-                e[2] += dt[2] * mx + dt[4] * my + dt[5] * mz;
-            }
-        for (int q = 0; q < 3; ++q) emu[3 * i + q] = e[q];
+    // excluded partners of the same water on lanes 0 and 1: the other two real sites (the M
+    // site carries no induced dipole and its entries are not part of the solve); pair order
+    // of tin: (O,H1) (O,H2) (O,M) (H1,H2) (H1,M) (H2,M)
+    if (lane < 2) {
+        const int w = i / 3, si = i - 3 * w;
+        const int pt = lane < si ? lane : lane + 1;
+        const int a = min(si, pt), b = max(si, pt);
+        const int pr = a == 0 ? b - 1 : 3;
+        const int j = 3 * w + pt;
+        if (site_polr2(P, j) > 1.e-6) {
+            const double* dt = tin + 36 * w + 6 * pr;
+            const double mx = dip[3 * j] * R4PIE0, my = dip[3 * j + 1] * R4PIE0, mz = dip[3 * j + 2] * R4PIE0;
+            e[0] += dt[0] * mx + dt[1] * my + dt[2] * mz;
+            e[1] += dt[1] * mx + dt[3] * my + dt[4] * mz;
+            e[2] += dt[2] * mx + dt[4] * my + dt[5] * mz;
+        }
     }
+    for (int q = 0; q < 3; ++q) e[q] = wsum(e[q]);
+    if (lane == 0)
+        for (int q = 0; q < 3; ++q) emu[3 * i + q] = e[q];
 }
 
 // SOR update (iter.f:58-89) and per-block sums of |dmu|^2 and of the dipole count.
@@ -1890,22 +1888,26 @@ __global__ void k_el_combine(ElecDev P, int distribute, const double* __restrict
     for (int q = 0; q < 3; ++q) fout[3 * m + q] = distribute ? 0.0 : fm[q];
 }
 
-// Fixed-order sums of block partials, one set per block of one wavefront (lane-strided, then across
-// the wavefront): set b has nb[b] blocks of stride[b] values, of which nv[b] are summed into
-// out[off[b] ..].
+// Fixed-order sums of block partials: set b has nb[b] blocks of stride[b] values, of which
+// nv[b] are summed into out[off[b] ..].  One block of 256 threads per (set, value), listed
+// in sv[]: thread t adds blocks t, t + 256, ..., then the wavefront sums and the four wave
+// sums are added in order.
 struct SumSets {
     const double* blk[6];
     int nb[6], stride[6], nv[6], off[6];
+    int nsv;
+    unsigned char sv[64][2];   // (set, value) of each block
 };
-__global__ __launch_bounds__(64) void k_el_sum_sets(SumSets S, double* __restrict__ out)
+__global__ __launch_bounds__(256) void k_el_sum_sets(SumSets S, double* __restrict__ out)
 {
-    const int b = blockIdx.x;
-    for (int q = 0; q < S.nv[b]; ++q) {
-        double s = 0.0;
-        for (int k = threadIdx.x; k < S.nb[b]; k += 64) s += S.blk[b][(long)k * S.stride[b] + q];
-        s = wsum(s);
-        if (threadIdx.x == 0) out[S.off[b] + q] = s;
-    }
+    __shared__ double ws[4];
+    const int b = S.sv[blockIdx.x][0], q = S.sv[blockIdx.x][1];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < S.nb[b]; k += 256) s += S.blk[b][(long)k * S.stride[b] + q];
+    s = wsum(s);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[S.off[b] + q] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 // Zero a double array (graph-safe replacement for a memset node).
@@ -2085,14 +2087,20 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     SumSets ss{};
     const double* sb[6] = {b_pol, b_ks, b_q2, b_p2, b_p3, b_qd};
     const int snb[6] = {nsb, nkb, nsb, P.n, nwb, nwb}, sst[6] = {1, 8, 2, 8, 8, 8}, soff[6] = {0, 1, 9, 11, 19, 27};
+    ss.nsv = 0;
     for (int q = 0; q < 6; ++q) {
         ss.blk[q] = sb[q];
         ss.nb[q] = snb[q];
         ss.stride[q] = sst[q];
         ss.nv[q] = sst[q];
         ss.off[q] = soff[q];
+        for (int v = 0; v < sst[q]; ++v) {
+            ss.sv[ss.nsv][0] = (unsigned char)q;
+            ss.sv[ss.nsv][1] = (unsigned char)v;
+            ++ss.nsv;
+        }
     }
-    hipLaunchKernelGGL(k_el_sum_sets, dim3(6), dim3(64), 0, s, ss, E.scal);
+    hipLaunchKernelGGL(k_el_sum_sets, dim3(ss.nsv), dim3(256), 0, s, ss, E.scal);
     return (int)hipGetLastError();
 }
 

@@ -25,9 +25,9 @@ Output: pip{2,3}b_gen.inc with
         const T* __restrict__ C);
   xv:   variables, SoA [nvars][nc]   (cluster c reads xv[v*nc + c])
   part: partials, SoA [nslices][nvars+1][nc] (slot 0 = E, slot 1+v = dE/dx_v)
-  C:    the coefficients in the order the code consumes them (each slot is read once,
-        so no coefficient stays live across the function); pip{N}b_stream_basis[]
-        maps every slot to its basis index in the fitted coefficient vector a[].
+  C:    the coefficients as the code addresses them (class Stream: by default the fitted
+        coefficient vector a[] itself); pip{N}b_stream_basis[] maps every slot of C to
+        its basis index in a[].
 and a dispatcher pip{N}b_slice(s, ...).
 """
 import collections
@@ -79,28 +79,43 @@ def cost(n):
 
 
 class Stream:
-    """Coefficients in the order the generated code consumes them."""
+    """Coefficient addressing of the generated code.
 
-    def __init__(self):
-        self.basis = []
+    "basis" (default): the code reads C[basis index] from the fitted coefficient vector
+    itself (1163 doubles for the 3B PIP, 9 KB), which the scalar data cache of a CU holds
+    whole, however many wavefronts run the slice at different points of its code.
+    "stream": every read gets its own slot in consumption order (29 000 slots, 230 KB for
+    the 3B PIP), read once and in sequence."""
+
+    def __init__(self, mode, nbasis):
+        self.mode = mode
+        self.basis = list(range(nbasis)) if mode == "basis" else []
 
     def take(self, b):
+        if self.mode == "basis":
+            return b
         self.basis.append(b)
         return len(self.basis) - 1
+
+    def ref(self, k):
+        # in "basis" mode a coefficient is read at every use through an opaque copy of the
+        # base pointer (pip_coef), so that the compiler neither merges the reads of one
+        # coefficient nor keeps it live between its uses
+        return f"pip_coef(C, {k})" if self.mode == "basis" else f"C[{k}]"
 
 
 def emit_edge(out, st, node, var, P, V, depth, ind):
     """Edge from a parent with prefix product P and value accumulator V to `node` via x[var]."""
     if not node.ch:
         k = st.take(node.coef)
-        out.append(f"{ind}{{ const T a = C[{k}]; {V} = fma(a, x[{var}], {V}); "
+        out.append(f"{ind}{{ const T a = {st.ref(k)}; {V} = fma(a, x[{var}], {V}); "
                    f"g[{var}] = fma(a, {P}, g[{var}]); }}")
         return
     p, v = f"p{depth}", f"v{depth}"
     out.append(ind + "{")
     i2 = ind + "  "
     out.append(f"{i2}const T {p} = {P} * x[{var}];")
-    out.append(f"{i2}T {v} = " + (f"C[{st.take(node.coef)}];" if node.coef is not None else "T(0);"))
+    out.append(f"{i2}T {v} = " + (f"{st.ref(st.take(node.coef))};" if node.coef is not None else "T(0);"))
     for cv in sorted(node.ch):
         emit_edge(out, st, node.ch[cv], cv, p, v, depth + 1, i2)
     out.append(f"{i2}{V} = fma(x[{var}], {v}, {V});")
@@ -138,8 +153,21 @@ def generate(which, nslices):
     root = build_trie(mono)
     units = make_units(root)
     bins, loads = partition(units, nslices)
-    st = Stream()
+    with open(os.path.join(DATA, f"poly{which}b_coeffs.txt")) as f:
+        coef = [float(t) for t in f.read().split()]
+    st = Stream(os.environ.get("MBGPU_PIP_COEF", "basis"), len(coef))
     lines = [f"// generated by gen_pip.py from data/poly{which}b_monomials.txt -- do not edit",
+             "#ifndef PIP_COEF_DEFINED",
+             "#define PIP_COEF_DEFINED",
+             "template <typename T>",
+             "__device__ __forceinline__ T pip_coef(const T* C, int i)",
+             "{",
+             "  // constant address space: a uniform read of read-only data (scalar load)",
+             "  const __attribute__((address_space(4))) T* K = (const __attribute__((address_space(4))) T*)C;",
+             "  asm volatile(\"\" : \"+s\"(K));",
+             "  return K[i];",
+             "}",
+             "#endif",
              f"// {len(mono)} monomials, trie cost {cost(root)} fma/mul, {nslices} slices",
              f"#define PIP{which}B_NVARS {nv}",
              f"#define PIP{which}B_NSLICES {nslices}",
@@ -160,7 +188,7 @@ def generate(which, nslices):
             nk = root.ch[k]
             lines.append("  {")
             own = None in js
-            lines.append(f"    T v1 = " + (f"C[{st.take(nk.coef)}];" if own else "T(0);"))
+            lines.append(f"    T v1 = " + (f"{st.ref(st.take(nk.coef))};" if own else "T(0);"))
             for j in js:
                 if j is None:
                     continue
@@ -190,8 +218,6 @@ def generate(which, nslices):
     for i in range(0, len(st.basis), 16):
         lines.append("  " + ", ".join(str(b) for b in st.basis[i:i + 16]) + ",")
     lines.append("};")
-    with open(os.path.join(DATA, f"poly{which}b_coeffs.txt")) as f:
-        coef = [float(t) for t in f.read().split()]
     lines.append(f"// the coefficient stream itself: a[pip{which}b_stream_basis[i]]")
     lines.append(f"static const double pip{which}b_stream_coef[{len(st.basis)}] = {{")
     for i in range(0, len(st.basis), 4):
