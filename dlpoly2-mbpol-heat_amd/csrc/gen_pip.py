@@ -87,8 +87,9 @@ class Stream:
     "stream": every read gets its own slot in consumption order (29 000 slots, 230 KB for
     the 3B PIP), read once and in sequence."""
 
-    def __init__(self, mode, nbasis):
+    def __init__(self, mode, nbasis, grouped=False):
         self.mode = mode
+        self.grouped = grouped and mode == "basis"
         self.basis = list(range(nbasis)) if mode == "basis" else []
 
     def take(self, b):
@@ -104,20 +105,30 @@ class Stream:
         return f"pip_coef(C, {k})" if self.mode == "basis" else f"C[{k}]"
 
 
-def emit_edge(out, st, node, var, P, V, depth, ind):
-    """Edge from a parent with prefix product P and value accumulator V to `node` via x[var]."""
+def emit_edge(out, st, node, var, P, V, depth, ind, K=None):
+    """Edge from a parent with prefix product P and value accumulator V to `node` via x[var].
+    K: name of the enclosing block's coefficient pointer (grouped reads), or None."""
     if not node.ch:
         k = st.take(node.coef)
-        out.append(f"{ind}{{ const T a = {st.ref(k)}; {V} = fma(a, x[{var}], {V}); "
+        a = f"{K}[{k}]" if K else st.ref(k)
+        out.append(f"{ind}{{ const T a = {a}; {V} = fma(a, x[{var}], {V}); "
                    f"g[{var}] = fma(a, {P}, g[{var}]); }}")
         return
     p, v = f"p{depth}", f"v{depth}"
     out.append(ind + "{")
     i2 = ind + "  "
+    Kn = None
+    if st.grouped:
+        Kn = f"K{depth}"
+        out.append(f"{i2}const pip_kptr<T> {Kn} = pip_base(C);")
     out.append(f"{i2}const T {p} = {P} * x[{var}];")
-    out.append(f"{i2}T {v} = " + (f"{st.ref(st.take(node.coef))};" if node.coef is not None else "T(0);"))
+    if node.coef is not None:
+        k = st.take(node.coef)
+        out.append(f"{i2}T {v} = " + (f"{Kn}[{k}];" if Kn else f"{st.ref(k)};"))
+    else:
+        out.append(f"{i2}T {v} = T(0);")
     for cv in sorted(node.ch):
-        emit_edge(out, st, node.ch[cv], cv, p, v, depth + 1, i2)
+        emit_edge(out, st, node.ch[cv], cv, p, v, depth + 1, i2, Kn)
     out.append(f"{i2}{V} = fma(x[{var}], {v}, {V});")
     out.append(f"{i2}g[{var}] = fma({P}, {v}, g[{var}]);")
     out.append(ind + "}")
@@ -155,10 +166,19 @@ def generate(which, nslices):
     bins, loads = partition(units, nslices)
     with open(os.path.join(DATA, f"poly{which}b_coeffs.txt")) as f:
         coef = [float(t) for t in f.read().split()]
-    st = Stream(os.environ.get("MBGPU_PIP_COEF", "basis"), len(coef))
+    st = Stream(os.environ.get("MBGPU_PIP_COEF", "basis"), len(coef), os.environ.get("MBGPU_PIP_GROUP", "1") == "1")
     lines = [f"// generated by gen_pip.py from data/poly{which}b_monomials.txt -- do not edit",
              "#ifndef PIP_COEF_DEFINED",
              "#define PIP_COEF_DEFINED",
+             "template <typename T>",
+             "using pip_kptr = const __attribute__((address_space(4))) T*;",
+             "template <typename T>",
+             "__device__ __forceinline__ pip_kptr<T> pip_base(const T* C)",
+             "{",
+             "  pip_kptr<T> K = (pip_kptr<T>)C;",
+             "  asm volatile(\"\" : \"+s\"(K));",
+             "  return K;",
+             "}",
              "template <typename T>",
              "__device__ __forceinline__ T pip_coef(const T* C, int i)",
              "{",
@@ -188,11 +208,19 @@ def generate(which, nslices):
             nk = root.ch[k]
             lines.append("  {")
             own = None in js
-            lines.append(f"    T v1 = " + (f"{st.ref(st.take(nk.coef))};" if own else "T(0);"))
+            K1 = None
+            if st.grouped:
+                K1 = "K1"
+                lines.append("    const pip_kptr<T> K1 = pip_base(C);")
+            if own:
+                kk = st.take(nk.coef)
+                lines.append(f"    T v1 = " + (f"K1[{kk}];" if K1 else f"{st.ref(kk)};"))
+            else:
+                lines.append("    T v1 = T(0);")
             for j in js:
                 if j is None:
                     continue
-                emit_edge(lines, st, nk.ch[j], j, f"x[{k}]", "v1", 2, "    ")
+                emit_edge(lines, st, nk.ch[j], j, f"x[{k}]", "v1", 2, "    ", K1)
             lines.append(f"    e = fma(x[{k}], v1, e);")
             lines.append(f"    g[{k}] += v1;")
             lines.append("  }")
