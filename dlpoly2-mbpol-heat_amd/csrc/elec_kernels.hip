@@ -510,7 +510,9 @@ __device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c,
 // row of c*, padded against bank conflicts).  Partials Spart[chunk][k][8] =
 // (sum q ckr, sum q skr, sum mu ckr (3), sum mu skr (3)), the dipole sums over
 // polarizable sites only.
+#ifndef EL_TILE
 #define EL_TILE 16
+#endif
 #define EL_TROW (EL_TILE + 1)
 #define EL_HALF (EL_RUN / 2)
 template <bool CHG>
@@ -555,7 +557,7 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
         for (int q = 0; q < 8; ++q) acc[t][q] = 0.0;
     // tiles of EL_TILE sites, double-buffered: the next tile's table entries are loaded
     // into registers while the current tile is consumed from LDS
-    constexpr int PF = 5;                       // staged entries per thread (nrows <= 80)
+    constexpr int PF = (80 * EL_TILE + 255) / 256;   // staged entries per thread (nrows <= 80)
     double pc[PF], ps[PF], pw = 0.0;
     auto fetch = [&](int t0) {
         const int tn = min(EL_TILE, j1 - t0);
