@@ -271,20 +271,28 @@ def main():
         engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
         nve = md.NVE(engine.Slot(box, solver=1 if args.solver == "cg" else 0), 0.2)
         e0, k0 = nve.pot + nve.kin, nve.kin
-        tot, its = [], []
+        tot, its, jser = [], [], []
         torch.cuda.synchronize()
         t0n = time.perf_counter()
         for _ in range(args.nve_steps):
             r = nve.step()
             tot.append(r["tot"])
             its.append(r["iterations"])
+            jser.append(r["jtot"])
         torch.cuda.synchronize()
         tn = time.perf_counter() - t0n
+        # autocorrelation of the run's J_tot series on the device (its cost; a 0.2 ps run gives
+        # no converged conductivity)
+        ntau = max(1, min(200, args.nve_steps // 2))
+        t0g = time.perf_counter()
+        gk = md.green_kubo(nve.slot.ctx, np.array(jser), 0.2e-3, box.L ** 3, 300.0, ntau)
+        tg = time.perf_counter() - t0g
         tot = np.array(tot)
         slope = np.polyfit(np.arange(1, len(tot) + 1) * 0.2e-3, tot - e0, 1)[0] if len(tot) > 1 else 0.0
         nve_out = {"steps": args.nve_steps, "dt_fs": 0.2, "ms_per_step": 1e3 * tn / args.nve_steps,
                    "max_abs_dE_over_KE0": float(np.abs(tot - e0).max() / k0),
                    "drift_per_ps_over_KE0": float(slope / k0), "dipole_iterations_median": float(np.median(its)),
+                   "heat_acf": {"lags": ntau, "ms": 1e3 * tg, "what": "autocorrelation of the run's J_tot on the device"},
                    "what": "512 H2O from the 300 K start, heat current every step, induced dipoles warm-started"}
     # serving leg: the mixed stream of PIMD beads (256 waters, L = 19.72 A, one configuration
     # displaced by N(0, 0.05 A) per bead) and isolated clusters (n log-uniform in [3, 1024]),

@@ -22,6 +22,7 @@ int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, d
 int md_launch_stage(int stage, int nr, double dt, const double* mass, double* x, double* y, double* z, double* vx,
                     double* vy, double* vz, double* fx, double* fy, double* fz, double* fsite, double* ekin,
                     double* blk, double* out, hipStream_t s);
+int md_launch_hcacf(long nt, int ntau, const double* j, double* acf, hipStream_t s);
 int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const double* vz, const double* ekin,
                         const double* epot, const double* epot2, const double* w, const double* w2,
                         const double* sew, double* blk, int nblk_cap, double* out, hipStream_t s);
@@ -792,6 +793,39 @@ int mbgpu_verlet_stage(mbgpu_ctx* c, int stage, double dt, const double* mass_de
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ekin_total) *ekin_total = ke;
     return 0;
+}
+
+int mbgpu_heat_acf_dev(mbgpu_ctx* c, const double* j_dev, long nt, int ntau, double* acf_dev)
+{
+    if (!c) return fail(-1, "null context");
+    if (!j_dev || !acf_dev) return fail(-1, "null current or autocorrelation array");
+    if (nt < 1 || ntau < 1 || ntau > nt) return fail(-1, "need 1 <= ntau <= nt (got nt %ld, ntau %d)", nt, ntau);
+    int rc = md_launch_hcacf(nt, ntau, j_dev, acf_dev, c->stream);
+    if (rc) return fail(rc, "autocorrelation kernel failed");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_heat_acf(mbgpu_ctx* c, const double* j, long nt, int ntau, double* acf)
+{
+    if (!c) return fail(-1, "null context");
+    if (!j || !acf) return fail(-1, "null current or autocorrelation array");
+    if (nt < 1 || ntau < 1 || ntau > nt) return fail(-1, "need 1 <= ntau <= nt (got nt %ld, ntau %d)", nt, ntau);
+    double *jd = nullptr, *ad = nullptr;
+    if (mbgpu_alloc(&jd, (size_t)3 * nt) != hipSuccess || mbgpu_alloc(&ad, (size_t)ntau) != hipSuccess) {
+        if (jd) (void)hipFree(jd);
+        return fail(-1, "cannot allocate %ld bytes for the current series", (long)(sizeof(double) * (3 * nt + ntau)));
+    }
+    int rc = 0;
+    if (hipMemcpyAsync(jd, j, sizeof(double) * 3 * nt, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(-1, "copy of the current series failed");
+    if (!rc) rc = mbgpu_heat_acf_dev(c, jd, nt, ntau, ad);
+    if (!rc && (hipMemcpyAsync(acf, ad, sizeof(double) * ntau, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = fail(-1, "copy of the autocorrelation failed");
+    (void)hipFree(jd);
+    (void)hipFree(ad);
+    return rc;
 }
 
 int mbgpu_get_dipoles(mbgpu_ctx* c, double* dip)

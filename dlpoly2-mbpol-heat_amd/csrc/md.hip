@@ -74,7 +74,34 @@ __global__ __launch_bounds__(64) void k_vv_sum(int nb, const double* __restrict_
     if (threadIdx.x == 0) out[0] = s;
 }
 
+// Heat-current autocorrelation C(tau) = (1 / (nt - tau)) sum_{t < nt - tau} J(t) . J(t + tau)
+// for J stored [nt][3]: one block per lag, the threads stride over t; thread sums, then the
+// wavefront sums and the four wave sums in order (the result does not depend on timing).
+__global__ __launch_bounds__(256) void k_hcacf(long nt, const double* __restrict__ j, double* __restrict__ acf)
+{
+    __shared__ double red[4];
+    const long tau = blockIdx.x;
+    const long n = nt - tau;
+    double s = 0.0;
+    for (long t = threadIdx.x; t < n; t += 256) {
+        const double* a = j + 3 * t;
+        const double* b = j + 3 * (t + tau);
+        s += a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    }
+    s = md_wsum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) acf[tau] = (((red[0] + red[1]) + red[2]) + red[3]) / (double)n;
+}
+
 }  // namespace
+
+// lags 0..ntau-1 of a series of nt >= ntau vectors
+int md_launch_hcacf(long nt, int ntau, const double* j, double* acf, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hcacf, dim3(ntau), dim3(256), 0, s, nt, j, acf);
+    return (int)hipGetLastError();
+}
 
 // nothing outside the nr real atoms is touched; blk needs ceil(nr/256) doubles
 int md_launch_stage(int stage, int nr, double dt, const double* mass, double* x, double* y, double* z, double* vx,

@@ -57,6 +57,9 @@ _SIGS = {
                                                       ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp, _ip]),
     "mbgpu_get_elec_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mbgpu_get_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "mbgpu_heat_acf": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_long, ctypes.c_int, _dp]),
+    "mbgpu_heat_acf_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
+                                          ctypes.c_void_p]),
     "mbgpu_verlet_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 12
                            + [_dp]),
     "mbgpu_water_terms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_void_p, _dp, _dp, _dp, _dp,
@@ -379,6 +382,17 @@ class Context:
         _check(load_library().mbgpu_verlet_stage(self._h, stage, dt, mass, x, y, z, vx, vy, vz, fx, fy, fz, fsite,
                                                  ekin, ctypes.byref(ke)), "mbgpu_verlet_stage")
         return ke.value
+
+    def heat_acf(self, j, ntau):
+        """Autocorrelation of a current series j (nt, 3) for lags 0..ntau-1 (mbgpu_heat_acf)."""
+        jj = np.ascontiguousarray(j, dtype=np.float64).reshape(-1, 3)
+        acf = np.zeros(int(ntau))
+        _check(load_library().mbgpu_heat_acf(self._h, _p(jj), len(jj), int(ntau), _p(acf)), "mbgpu_heat_acf")
+        return acf
+
+    def heat_acf_dev(self, j_ptr, nt, ntau, acf_ptr):
+        """mbgpu_heat_acf_dev with device pointers ([nt][3] in, [ntau] out)."""
+        _check(load_library().mbgpu_heat_acf_dev(self._h, j_ptr, int(nt), int(ntau), acf_ptr), "mbgpu_heat_acf_dev")
 
     def dipoles(self):
         """Converged induced dipoles of the last electrostatics call, (4*nw, 3)."""

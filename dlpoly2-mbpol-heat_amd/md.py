@@ -11,10 +11,22 @@ energy, J).
 `Heatflux2Writer` appends J records in the layout of write_heat_flux2
 (heatcurrent.f90:730-740): the time as es12.5, then J_kin, J_pot and J_tot, each as
 3es15.7, in internal units times junit = 1e3.
+
+`green_kubo` turns a J series into the heat-current autocorrelation (on the device,
+mbgpu_heat_acf) and the running Green-Kubo conductivity
+kappa(t) = 1 / (3 V k_B T^2) * int_0^t <J(0) . J(s)> ds (trapezoid rule), the analysis the
+reference leaves to post-processing of heatflux2.  The helper integrates whatever current it
+is given: in water (several atomic species) J must first be freed of its correlation with
+the non-conserved mass currents (multi-component Green-Kubo / cepstral analysis), and the
+series must be long (tens of ps) for the integral to converge.
 """
 import numpy as np
 
 JUNIT = 1.0e3     # heatcurrent.f90:19
+# SI value of the internal units: energy 10 J/mol per particle, velocity Angstrom/ps
+KB_SI = 1.380649e-23
+E_INT_SI = 10.0 / 6.02214076e23          # J
+V_INT_SI = 1.0e-10 / 1.0e-12             # m/s
 
 
 def fortran_es(v, w, d):
@@ -114,3 +126,21 @@ class NVE:
     def velocities(self):
         self.t.cuda.synchronize(self.slot.dev)
         return self.slot.v.cpu().numpy().T.copy()
+
+
+def kappa_from_acf(acf, dt_ps, volume_A3, temperature_K):
+    """Running Green-Kubo integral in W/(m K) of an autocorrelation of J in internal units
+    ((10 J/mol) Angstrom/ps per particle), sampled every dt_ps: kappa[k] = integral over
+    [0, k dt] by the trapezoid rule."""
+    c = np.asarray(acf, dtype=np.float64) * (E_INT_SI * V_INT_SI) ** 2
+    dt = dt_ps * 1.0e-12
+    integral = np.concatenate([[0.0], np.cumsum(0.5 * (c[1:] + c[:-1]) * dt)])
+    return integral / (3.0 * volume_A3 * 1.0e-30 * KB_SI * temperature_K ** 2)
+
+
+def green_kubo(ctx, jseries, dt_ps, volume_A3, temperature_K, ntau):
+    """Autocorrelation of J (nt, 3) on the device for lags 0..ntau-1 and the running
+    conductivity; returns dict(time_ps, acf, kappa_W_mK)."""
+    acf = ctx.heat_acf(jseries, ntau)
+    return dict(time_ps=dt_ps * np.arange(ntau), acf=acf,
+                kappa_W_mK=kappa_from_acf(acf, dt_ps, volume_A3, temperature_K))

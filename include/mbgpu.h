@@ -205,6 +205,15 @@ int mbgpu_verlet_stage(mbgpu_ctx* ctx, int stage, double dt, const double* mass_
                        double* fx_dev, double* fy_dev, double* fz_dev, double* fsite_dev, double* ekin_dev,
                        double* ekin_total);
 
+/* Heat-current autocorrelation for Green-Kubo conductivities: acf[tau] = (1/(nt - tau)) *
+ * sum_{t < nt - tau} J(t) . J(t + tau) for tau = 0 .. ntau-1, J stored [nt][3] (for example
+ * the J_tot of every step of a run, in the units the caller keeps them).  The reference
+ * leaves this to post-processing of its heatflux2 file (heatcurrent.f90:730-740); here it
+ * runs on the device.  The _dev form takes device arrays; the host form copies through
+ * temporary device buffers.  1 <= ntau <= nt. */
+int mbgpu_heat_acf(mbgpu_ctx* ctx, const double* j, long nt, int ntau, double* acf);
+int mbgpu_heat_acf_dev(mbgpu_ctx* ctx, const double* j_dev, long nt, int ntau, double* acf_dev);
+
 /* Converged induced dipoles of the last electrostatics call: 3 per site, interleaved, 4*nw
  * sites (the reference's dipx/y/z after iter; e*Angstrom). */
 int mbgpu_get_dipoles(mbgpu_ctx* ctx, double* dip);

@@ -31,3 +31,17 @@ def test_record_layout(md):
     assert lines[1] == "  1.0000000E+00 -2.0000000E+00  3.0000000E+00"     # x junit = 1e3
     assert [len(l) for l in lines[1:]] == [45, 45, 45]
     assert lines[3] == "  1.0000000E+03  2.0000000E+03  3.0000000E+03"
+
+
+def test_green_kubo_integral_of_an_exponential(md):
+    """kappa_from_acf on C(t) = C0 exp(-t / tc): the running integral tends to C0 tc, and the
+    unit chain (10 J/mol per particle, Angstrom/ps, Angstrom^3, K) gives the SI value
+    computed here independently."""
+    import numpy as np
+    c0, tc, dt = 2.5e4, 0.05, 1.0e-4           # internal units^2, ps, ps
+    t = dt * np.arange(20000)
+    k = md.kappa_from_acf(c0 * np.exp(-t / tc), dt, 15000.0, 300.0)
+    assert k[0] == 0.0 and np.all(np.diff(k) >= 0) and k[1] > 0
+    j_si = (10.0 / 6.02214076e23) * 100.0       # J m/s per internal unit of J
+    expect = c0 * j_si ** 2 * tc * 1e-12 / (3 * 15000.0e-30 * 1.380649e-23 * 300.0 ** 2)
+    assert abs(k[-1] - expect) <= 1e-6 * expect
