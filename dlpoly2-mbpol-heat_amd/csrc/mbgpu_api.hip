@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <mutex>
 #include <string>
@@ -23,6 +24,9 @@ int md_launch_stage(int stage, int nr, double dt, const double* mass, double* x,
                     double* vy, double* vz, double* fx, double* fy, double* fz, double* fsite, double* ekin,
                     double* blk, double* out, hipStream_t s);
 int md_launch_hcacf(long nt, int ntau, const double* j, double* acf, hipStream_t s);
+int ring_launch_transform(int P, long ncoord, const double* M, const double* in, double* out, hipStream_t s);
+int ring_launch_spring(int P, int natom, double coeff, double fac3, const double* mass, const double* x,
+                       const double* xc, double* f, double* blk, double* out, hipStream_t s);
 int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const double* vz, const double* ekin,
                         const double* epot, const double* epot2, const double* w, const double* w2,
                         const double* sew, double* blk, int nblk_cap, double* out, hipStream_t s);
@@ -119,6 +123,9 @@ struct mbgpu_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     ElecState el;                     // polarizable electrostatics
     WaterState wt;                    // monomer, dispersion and long-range terms
+    // ring polymer: bead-index matrix of the last transform, spring block partials
+    double *ring_m = nullptr, *ring_blk = nullptr, *ring_out = nullptr;
+    long ring_m_cap = 0, ring_blk_cap = 0;
 };
 
 namespace {
@@ -406,6 +413,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(B.xv2f); dfree(B.part2f); dfree(B.cs2f); dfree(B.xv3f); dfree(B.part3f); dfree(B.cs3f);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
+    dfree(c->ring_m); dfree(c->ring_blk); dfree(c->ring_out);
     elec_release(c->el);
     water_release(c->wt);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -826,6 +834,110 @@ int mbgpu_heat_acf(mbgpu_ctx* c, const double* j, long nt, int ntau, double* acf
     (void)hipFree(jd);
     (void)hipFree(ad);
     return rc;
+}
+
+int mbgpu_ring_matrices(int nbead, int langevin, double* nmode_to_cart, double* cart_to_nmode,
+                        double* tcart_to_tnmode, double* lambda)
+{
+    if (nbead < 2 || nbead % 2 || nbead > MBGPU_RING_MAX_BEADS)
+        return fail(-1, "nbead must be even, 2 <= nbead <= %d (got %d)", MBGPU_RING_MAX_BEADS, nbead);
+    if (!nmode_to_cart || !cart_to_nmode || !tcart_to_tnmode || !lambda) return fail(-1, "null output array");
+    const int P = nbead;
+    const double pi = 3.141592653589793;
+    std::vector<double> C((size_t)P * P, 0.0);
+    auto at = [&](int n1, int n2) -> double& { return C[(size_t)(n1 - 1) * P + (n2 - 1)]; };
+    for (int n1 = 1; n1 <= P; ++n1) at(n1, 1) = 1.0;
+    for (int n1 = 1; n1 <= P / 2; ++n1) {
+        at(2 * n1, P) = -1.0;
+        at(2 * n1 - 1, P) = 1.0;
+    }
+    for (int n1 = 1; n1 <= P; ++n1) {
+        const double t = 2.0 * pi * (double)(n1 - 1) / (double)P;
+        for (int n2 = 1; n2 <= P / 2 - 1; ++n2) {
+            at(n1, 2 * n2) = std::sqrt(2.0) * std::cos(t * (double)n2);
+            at(n1, 2 * n2 + 1) = -std::sqrt(2.0) * std::sin(t * (double)n2);
+        }
+    }
+    double scale = 1.0 / (double)P;
+    if (langevin) {
+        for (int n1 = 1; n1 <= P; ++n1) {
+            const double f1 = std::sqrt(1.0 / P), f2 = std::sqrt(2.0 / P);
+            at(n1, 1) = f1;
+            at(n1, P / 2 + 1) = (n1 % 2) ? -f1 : f1;
+            const double t = 2.0 * pi * (double)n1 / (double)P;
+            for (int n2 = 2; n2 <= P / 2; ++n2) {
+                at(n1, n2) = f2 * std::cos(t * (double)(n2 - 1));
+                at(n1, n2 + P / 2) = f2 * std::sin(t * ((double)n2 + (double)P / 2.0 - 1.0));
+            }
+        }
+        scale = 1.0;
+    }
+    for (int n1 = 1; n1 <= P; ++n1)
+        for (int n2 = 1; n2 <= P; ++n2) {
+            nmode_to_cart[(n1 - 1) * P + (n2 - 1)] = at(n1, n2);
+            cart_to_nmode[(n2 - 1) * P + (n1 - 1)] = at(n1, n2) * scale;
+            tcart_to_tnmode[(n2 - 1) * P + (n1 - 1)] = at(n1, n2);
+        }
+    if (langevin) {
+        for (int n = 0; n < P; ++n) lambda[n] = 1.0;
+    } else {
+        lambda[0] = 0.0;
+        for (int n = 1; n <= P / 2 - 1; ++n) {
+            const double t = 2.0 * (1.0 - std::cos(2.0 * pi * (double)n / (double)P));
+            lambda[2 * n - 1] = t * (double)P;
+            lambda[2 * n] = t * (double)P;
+        }
+        lambda[P - 1] = 4.0 * (double)P;
+    }
+    return 0;
+}
+
+int mbgpu_ring_transform_dev(mbgpu_ctx* c, int nbead, long ncoord, const double* M, const double* in_dev,
+                             double* out_dev)
+{
+    if (!c) return fail(-1, "null context");
+    if (nbead < 1 || nbead > MBGPU_RING_MAX_BEADS) return fail(-1, "nbead out of range (got %d)", nbead);
+    if (ncoord < 1) return fail(-1, "ncoord must be positive");
+    if (!M || !in_dev || !out_dev) return fail(-1, "null matrix or bead array");
+    if (in_dev == out_dev) return fail(-1, "the transform cannot work in place");
+    const long need = (long)nbead * nbead;
+    if (c->ring_m_cap < need) {
+        dfree(c->ring_m);
+        HIPCHK(dalloc(&c->ring_m, (size_t)need));
+        c->ring_m_cap = need;
+    }
+    HIPCHK(hipMemcpyAsync(c->ring_m, M, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+    int rc = ring_launch_transform(nbead, ncoord, c->ring_m, in_dev, out_dev, c->stream);
+    if (rc) return fail(rc, "ring transform kernel failed");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_ring_spring_dev(mbgpu_ctx* c, int nbead, int natom, double coeff, double fac3, double force_scale,
+                          const double* mass_dev, const double* x_dev, const double* xc_dev, double* f_dev,
+                          double* epot_spring, double* epot_deriv)
+{
+    if (!c) return fail(-1, "null context");
+    if (nbead < 1 || nbead > MBGPU_RING_MAX_BEADS) return fail(-1, "nbead out of range (got %d)", nbead);
+    if (natom < 1) return fail(-1, "natom must be positive");
+    if (!mass_dev || !x_dev || !xc_dev || !f_dev) return fail(-1, "null mass, position, centroid or force array");
+    if (!(force_scale != 0.0)) return fail(-1, "force_scale must be nonzero");
+    const long need = 2L * nbead * ((natom + 255) / 256);
+    if (c->ring_blk_cap < need) {
+        dfree(c->ring_blk);
+        HIPCHK(dalloc(&c->ring_blk, (size_t)need));
+        c->ring_blk_cap = need;
+    }
+    if (!c->ring_out) HIPCHK(dalloc(&c->ring_out, 2));
+    int rc = ring_launch_spring(nbead, natom, coeff, fac3, mass_dev, x_dev, xc_dev, f_dev, c->ring_blk, c->ring_out,
+                                c->stream);
+    if (rc) return fail(rc, "ring spring kernels failed");
+    double o[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(o, c->ring_out, sizeof o, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (epot_spring) *epot_spring = 0.5 * coeff * o[0];
+    if (epot_deriv) *epot_deriv = 0.5 * o[1] / force_scale;
+    return 0;
 }
 
 int mbgpu_get_dipoles(mbgpu_ctx* c, double* dip)

@@ -58,6 +58,13 @@ _SIGS = {
     "mbgpu_get_elec_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mbgpu_get_dipoles": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "mbgpu_heat_acf": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_long, ctypes.c_int, _dp]),
+    "mbgpu_ring_matrices": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
+    "mbgpu_ring_transform_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, _dp, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
+    "mbgpu_ring_spring_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double)]),
     "mbgpu_heat_acf_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
                                           ctypes.c_void_p]),
     "mbgpu_verlet_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 12
@@ -201,6 +208,16 @@ def mbpol_3b_poly(w1, w2, w3, g1=None, g2=None, g3=None):
 
 
 # ----------------------------------------------------------------------- context ----
+def ring_matrices(nbead, langevin=0):
+    """Normal-mode matrices of transformation_matrix (mbgpu_ring_matrices): (nmode_to_cart,
+    cart_to_nmode, tcart_to_tnmode, lambda), each matrix [k][b] = M(k+1, b+1)."""
+    P = int(nbead)
+    a, b, c, lam = np.zeros(P * P), np.zeros(P * P), np.zeros(P * P), np.zeros(P)
+    _check(load_library().mbgpu_ring_matrices(P, int(langevin), _p(a), _p(b), _p(c), _p(lam)),
+           "mbgpu_ring_matrices")
+    return a.reshape(P, P), b.reshape(P, P), c.reshape(P, P), lam
+
+
 class Context:
     """One device context for a system of nw waters and natms sites."""
 
@@ -393,6 +410,22 @@ class Context:
     def heat_acf_dev(self, j_ptr, nt, ntau, acf_ptr):
         """mbgpu_heat_acf_dev with device pointers ([nt][3] in, [ntau] out)."""
         _check(load_library().mbgpu_heat_acf_dev(self._h, j_ptr, int(nt), int(ntau), acf_ptr), "mbgpu_heat_acf_dev")
+
+    def ring_transform_dev(self, nbead, ncoord, M, in_ptr, out_ptr):
+        """mbgpu_ring_transform_dev: out[k] = sum_b M[k][b] in[b] over bead-major device arrays."""
+        m = np.ascontiguousarray(M, dtype=np.float64).reshape(-1)
+        if m.size != nbead * nbead:
+            raise MbgpuError("the bead matrix must be nbead x nbead")
+        _check(load_library().mbgpu_ring_transform_dev(self._h, int(nbead), int(ncoord), _p(m), in_ptr, out_ptr),
+               "mbgpu_ring_transform_dev")
+
+    def ring_spring_dev(self, nbead, natom, coeff, fac3, force_scale, mass_ptr, x_ptr, xc_ptr, f_ptr):
+        """mbgpu_ring_spring_dev; returns (Epot_spring, Epot_deriv)."""
+        es, ed = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(load_library().mbgpu_ring_spring_dev(self._h, int(nbead), int(natom), float(coeff), float(fac3),
+                                                    float(force_scale), mass_ptr, x_ptr, xc_ptr, f_ptr,
+                                                    ctypes.byref(es), ctypes.byref(ed)), "mbgpu_ring_spring_dev")
+        return es.value, ed.value
 
     def dipoles(self):
         """Converged induced dipoles of the last electrostatics call, (4*nw, 3)."""

@@ -214,6 +214,32 @@ int mbgpu_verlet_stage(mbgpu_ctx* ctx, int stage, double dt, const double* mass_
 int mbgpu_heat_acf(mbgpu_ctx* ctx, const double* j, long nt, int ntau, double* acf);
 int mbgpu_heat_acf_dev(mbgpu_ctx* ctx, const double* j_dev, long nt, int ntau, double* acf_dev);
 
+/* ---- ring polymer (in-process beads) ----------------------------------------------
+ * Beads are stored bead-major, a bead's coordinates as [3][natom] (its xxx, yyy, zzz
+ * blocks).  Matrices are nbead x nbead, row-major, element [k][b] = M(k+1, b+1) of the
+ * reference's Fortran arrays. */
+#define MBGPU_RING_MAX_BEADS 256
+
+/* Normal-mode matrices and eigenvalues of transformation_matrix (centroid_matrix.f:3-135):
+ * langevin = 0 gives the reference's default ordering (cart_to_nmode = C^T / nbead),
+ * langevin = 1 the orthonormal ordering of its Langevin branch.  Host computation; nbead
+ * even, 2 .. MBGPU_RING_MAX_BEADS. */
+int mbgpu_ring_matrices(int nbead, int langevin, double* nmode_to_cart, double* cart_to_nmode,
+                        double* tcart_to_tnmode, double* lambda);
+/* out[k][c] = sum_b M[k][b] in[b][c] over the ncoord coordinates of every bead: the
+ * position / velocity / force transforms of centroid_matrix.f:140-476 (M = cart_to_nmode,
+ * tcart_to_tnmode or nmode_to_cart; M on the host, bead arrays on the device). */
+int mbgpu_ring_transform_dev(mbgpu_ctx* ctx, int nbead, long ncoord, const double* M, const double* in_dev,
+                             double* out_dev);
+/* Spring forces of centroid_spring_force.f:41-157 for all beads at once: with
+ * coeff = nbead / (hbar beta)^2, the centroid xc ([3][natom], mode-1 coordinates times
+ * fac1), f_k -= coeff m (2 x_k - x_{k-1} - x_{k+1}) fac3; *epot_spring = coeff/2 sum m
+ * |x_k - x_{k-1}|^2 and *epot_deriv = 1/2 sum -(x_k - xc) . f_k / force_scale with f the
+ * physical forces on entry.  Device arrays: mass [natom], x and f [nbead][3][natom]. */
+int mbgpu_ring_spring_dev(mbgpu_ctx* ctx, int nbead, int natom, double coeff, double fac3, double force_scale,
+                          const double* mass_dev, const double* x_dev, const double* xc_dev, double* f_dev,
+                          double* epot_spring, double* epot_deriv);
+
 /* Converged induced dipoles of the last electrostatics call: 3 per site, interleaved, 4*nw
  * sites (the reference's dipx/y/z after iter; e*Angstrom). */
 int mbgpu_get_dipoles(mbgpu_ctx* ctx, double* dip);

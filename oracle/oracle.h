@@ -211,6 +211,16 @@ void ora_heat_flux_tensor(int nreal, const double *heat_w,
                           const double *stress_ew1p,
                           double jkin[3], double jpot[3], double jtot[3]);
 
+/* ---- ring polymer (ring_oracle.c) ------------------------------------------------- */
+
+/* transformation_matrix (centroid_matrix.f:3-135); P x P row-major, [k][b] = M(k+1, b+1) */
+void ora_ring_matrices(int P, int langevin, double *n2c, double *c2n, double *tc2n, double *lambda);
+/* out[k] = sum_b M[k][b] in[b] per coordinate (centroid_matrix.f:140-476) */
+void ora_ring_transform(int P, int ncoord, const double *M, const double *in, double *out);
+/* centroid_spring_force.f:41-157 over all beads; x, f bead-major [P][3][natom] */
+void ora_ring_spring(int P, int natom, double coeff, double fac3, double force_scale, const double *mass,
+                     const double *x, const double *xc, double *f, double *epot_spring, double *epot_deriv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -361,3 +361,43 @@ def water_terms(box, p, xyz=None, cell=None, imcon=None, dense=False):
     if dense:
         out["force_matrix"] = fm
     return out
+
+
+# ---------------------------------------------------------------- ring polymer ----
+
+def ring_lib():
+    lib = oracle()
+    lib.ora_ring_matrices.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+    lib.ora_ring_transform.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+    lib.ora_ring_spring.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]
+    return lib
+
+
+def ring_matrices(nb, langevin=0):
+    lib = ring_lib()
+    a, b, c, lam = np.zeros(nb * nb), np.zeros(nb * nb), np.zeros(nb * nb), np.zeros(nb)
+    lib.ora_ring_matrices(nb, langevin, P(a), P(b), P(c), P(lam))
+    return a.reshape(nb, nb), b.reshape(nb, nb), c.reshape(nb, nb), lam
+
+
+def ring_transform(M, x):
+    """x: (P, ncoord) -> (P, ncoord)"""
+    lib = ring_lib()
+    x = np.ascontiguousarray(x, float)
+    out = np.zeros_like(x)
+    lib.ora_ring_transform(x.shape[0], x.shape[1], P(np.ascontiguousarray(M, float)), P(x), P(out))
+    return out
+
+
+def ring_spring(x, xc, f, mass, coeff, fac3=1.0, force_scale=1.0):
+    """x, f: (P, 3, natom); xc: (3, natom).  Returns (f_new, Epot_spring, Epot_deriv)."""
+    lib = ring_lib()
+    x = np.ascontiguousarray(x, float)
+    f = np.ascontiguousarray(f, float).copy()
+    nb, _, n = x.shape
+    es, ed = ctypes.c_double(), ctypes.c_double()
+    lib.ora_ring_spring(nb, n, coeff, fac3, force_scale, P(np.ascontiguousarray(mass, float)), P(x),
+                        P(np.ascontiguousarray(xc, float)), P(f), ctypes.byref(es), ctypes.byref(ed))
+    return f, es.value, ed.value
