@@ -146,6 +146,8 @@ def main():
                     help="configurations evaluated concurrently in the batched legs (0 skips them)")
     ap.add_argument("--nve-steps", type=int, default=1000,
                     help="steps of the NVE leg (512 waters, dt 0.2 fs, heat current every step; 0 skips it)")
+    ap.add_argument("--pimd-steps", type=int, default=20,
+                    help="ring-polymer steps (8 beads x 256 H2O, 0.2 fs) of the PIMD leg; 0 skips it")
     ap.add_argument("--serve-requests", type=int, default=0,
                     help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it).  Off by "
                          "default: this leg, run after the batched legs, once ended in an illegal memory access "
@@ -294,6 +296,27 @@ def main():
                    "drift_per_ps_over_KE0": float(slope / k0), "dipole_iterations_median": float(np.median(its)),
                    "heat_acf": {"lags": ntau, "ms": 1e3 * tg, "what": "autocorrelation of the run's J_tot on the device"},
                    "what": "512 H2O from the 300 K start, heat current every step, induced dipoles warm-started"}
+    # PIMD leg: a ring polymer of 8 beads of 256 waters (300 K), every bead evaluated with the
+    # complete force field on its own context (overlapping), springs and normal modes on the
+    # device, velocity Verlet of the ring-polymer Hamiltonian
+    pimd_out = None
+    if args.pimd_steps > 0:
+        ring = importlib.import_module("dlpoly2-mbpol-heat_amd.ring")
+        base = pkg.waterbox.box_256()
+        beads = [pkg.waterbox.displaced(base, 0.02, seed=700 + k) for k in range(8)]
+        rp = ring.RingPolymer(beads, 300.0, solver=1 if args.solver == "cg" else 0)
+        h0, k0 = rp.energy(), rp.kinetic()
+        torch.cuda.synchronize()
+        t0p = time.perf_counter()
+        hs = [rp.step(0.2) for _ in range(args.pimd_steps)]
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0p
+        rp.close()
+        pimd_out = {"beads": 8, "waters": 256, "steps": args.pimd_steps, "dt_fs": 0.2,
+                    "ms_per_step": 1e3 * tp / args.pimd_steps,
+                    "bead_evals_per_s": 8 * args.pimd_steps / tp,
+                    "max_abs_dH_over_K0": float(np.max(np.abs(np.array(hs) - h0)) / k0),
+                    "what": "ring polymer at 300 K, normal modes and springs on the device, beads overlapping"}
     # serving leg: the mixed stream of PIMD beads (256 waters, L = 19.72 A, one configuration
     # displaced by N(0, 0.05 A) per bead) and isolated clusters (n log-uniform in [3, 1024]),
     # Poisson arrivals, continuous batching over --batch workers
@@ -315,6 +338,8 @@ def main():
         legs["serving"] = serving
     if nve_out is not None:
         legs["nve"] = nve_out
+    if pimd_out is not None:
+        legs["pimd"] = pimd_out
     if "mixed" in batched:
         m = batched["mixed"]
         legs["mixed_vs_fp64"] = {"rms_dF_kcal_mol_A": m["rms_dF"], "max_dF_kcal_mol_A": m["max_dF"],

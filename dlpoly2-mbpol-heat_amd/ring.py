@@ -18,6 +18,8 @@ normal-mode masses or barostat, which stay out of scope); `energy()` returns H.
 
 Units are the reference's internal ones; hbar and k_B as centroid_module.f90:19-21.
 """
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import elec_params as _ep
@@ -76,6 +78,7 @@ class RingPolymer:
             self.V[k, :, :self.nr].copy_(torch.as_tensor(np.ascontiguousarray(b.vel[:self.nr].T), **f64))
         torch.cuda.synchronize(self.dev)
         self.pot = None
+        self._pool = None
         self.forces()
 
     # ---------------------------------------------------------------- forces ----
@@ -98,11 +101,12 @@ class RingPolymer:
         self.F.zero_()
         self.fsite.zero_()
         t.cuda.synchronize(self.dev)
-        pots, its = [], []
-        for k in range(self.P):
-            e, it = self._bead_forces(k)
-            pots.append(e)
-            its.append(it)
+        # the beads are independent configurations: one host thread each, so that their
+        # kernels overlap on the GPU (the library calls release the interpreter lock)
+        if self._pool is None:
+            self._pool = ThreadPoolExecutor(max_workers=self.P)
+        res = list(self._pool.map(self._bead_forces, range(self.P)))
+        pots, its = [r[0] for r in res], [r[1] for r in res]
         # electrostatics forces (3 per site, already moved off the M sites) onto the atoms
         self.F[:, :, :self.nr] += self.fsite.view(self.P, self.natms, 3)[:, :self.nr, :].transpose(1, 2)
         t.cuda.synchronize(self.dev)
@@ -142,3 +146,8 @@ class RingPolymer:
                                v[0].data_ptr(), v[1].data_ptr(), v[2].data_ptr(), f[0].data_ptr(),
                                f[1].data_ptr(), f[2].data_ptr(), None, self.ek[k].data_ptr())
         return self.energy()
+
+    def close(self):
+        if self._pool is not None:
+            self._pool.shutdown()
+            self._pool = None
