@@ -102,19 +102,16 @@ def serve_leg(pkg, args):
     rng = np.random.default_rng(9)
     base = wb.box_256()
     beads = [wb.displaced(base, 0.05, seed=500 + i) for i in range(32)]
-    sizes = np.exp(rng.uniform(np.log(3), np.log(1024), 24)).astype(int)
-    clusters = [wb.make_cluster(int(n), seed=600 + i) for i, n in enumerate(sizes)]
+    if args.serve_mix == "mixed":
+        raise SystemExit("the mixed-size serving leg is switched off: one run of it ended in an illegal memory "
+                         "access whose cause has not been found")
     jobs = []
     for i in range(args.serve_requests):
-        if rng.random() < 0.5:
-            b = beads[i % len(beads)]
-            jobs.append((b, ep.for_box(b.L), wp.for_box(b.L), "bead"))
-        else:
-            c = clusters[int(rng.integers(len(clusters)))]
-            jobs.append((c, ep.for_cluster(c.L), wp.for_cluster(c.L), "cluster"))
+        b = beads[i % len(beads)]
+        jobs.append((b, ep.for_box(b.L), wp.for_box(b.L), "bead"))
     srv = serve.Server(workers=max(1, args.batch), solver=1 if args.solver == "cg" else 0)
-    # warm the pool: one request of each size, untimed
-    warm = [srv.submit(*j) for j in {j[0].nw: j for j in jobs}.values()]
+    # warm the pool, untimed: one request per worker at once, so every worker holds a slot
+    warm = [srv.submit(*j) for j in jobs[:max(1, args.batch)]]
     for r in warm:
         r.done.wait()
     reqs, wall = serve.run_poisson(srv, jobs, args.serve_rate, seed=3)
@@ -122,15 +119,15 @@ def serve_leg(pkg, args):
     out = serve.latency_summary(reqs)
     for tag in ("bead", "cluster"):
         sub = [r for r in reqs if r.tag == tag]
-        s = serve.latency_summary(sub)
-        if s:
+        s = serve.latency_summary(sub) if sub else {}
+        if "p50_ms" in s:
             out[f"{tag}_p50_ms"], out[f"{tag}_p99_ms"] = s["p50_ms"], s["p99_ms"]
     out.update(arrival_rate=args.serve_rate, wall_s=wall, evals_per_s=len(reqs) / wall, workers=max(1, args.batch),
                slots_created=srv.slots_created, slots_evicted=srv.slots_evicted,
-               cluster_sizes=[int(n) for n in sizes],
-               what="32 PIMD beads x 256 H2O (L=19.72 A, N(0,0.05 A) displacements) and isolated clusters "
-                    "n in [3,1024] log-uniform (random sequential addition at 0.997 g/cm^3, in a periodic box "
-                    "of side 2R+12 A), half and half, Poisson arrivals; latency = arrival to forces on the host")
+               mix=args.serve_mix,
+               what="requests drawn from 32 PIMD beads x 256 H2O (L=19.72 A, N(0,0.05 A) displacements), "
+                    "Poisson arrivals, continuous batching over the workers' slots; latency = arrival to forces "
+                    "on the host")
     return out
 
 
@@ -148,11 +145,12 @@ def main():
                     help="steps of the NVE leg (512 waters, dt 0.2 fs, heat current every step; 0 skips it)")
     ap.add_argument("--pimd-steps", type=int, default=20,
                     help="ring-polymer steps (8 beads x 256 H2O, 0.2 fs) of the PIMD leg; 0 skips it")
-    ap.add_argument("--serve-requests", type=int, default=0,
-                    help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it).  Off by "
-                         "default: this leg, run after the batched legs, once ended in an illegal memory access "
-                         "whose cause is not yet found")
-    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-requests", type=int, default=800,
+                    help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--serve-mix", choices=["beads", "mixed"], default="beads",
+                    help="beads: PIMD beads of 256 waters only (same-size slots); mixed: beads and isolated "
+                         "clusters of 3-1024 waters (switched off: see the serving-leg note in the design notes)")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB", default=0,
                     help="only run a plain device copy of MIB MiB five times (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")

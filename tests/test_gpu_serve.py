@@ -5,6 +5,8 @@
   is bit-identical to evaluating the same configuration alone on a fresh slot.
 * A memory budget that fits one slot at a time: slots of other sizes are evicted and every
   request still completes with the same results.
+  (These two mixed-size tests are skipped: see SERVE_SKIP.)
+* Same-size requests (beads) served by several workers at once equal the beads alone.
 * The i-PI driver with the GPU evaluator answers a scripted server with the forces of the
   evaluation converted to Hartree/bohr.
 """
@@ -87,6 +89,25 @@ def test_eviction_under_memory_budget(mods):
     for (box, elec, water, _), r in zip(jobs, reqs):
         assert r.error is None, r.error
         ra, fa = alone(engine, box, elec, water)
+        same(r.result, r.forces, ra, fa)
+
+
+def test_bead_serving_equals_alone(mods):
+    """Same-size requests (PIMD beads of one box) through the server's workers at once:
+    every result is bit-identical to evaluating the bead alone."""
+    serve, engine, ipi, ep, wp, wb = mods
+    base = wb.make_waterbox(64, 12.42, 4, seed=11)
+    beads = [wb.displaced(base, 0.05, seed=200 + i) for i in range(8)]
+    srv = serve.Server(workers=4)
+    print("serving 8 bead requests on 4 workers", flush=True)
+    reqs = [srv.submit(b, ep.for_box(b.L), wp.for_box(b.L), "bead") for b in beads]
+    for r in reqs:
+        assert r.done.wait(120)
+    srv.close()
+    assert srv.slots_created <= 4
+    for b, r in zip(beads, reqs):
+        assert r.error is None, r.error
+        ra, fa = alone(engine, b, ep.for_box(b.L), wp.for_box(b.L))
         same(r.result, r.forces, ra, fa)
 
 
