@@ -950,6 +950,28 @@ __device__ void cg_scalars(const ElecDev& P, int mode, double a, double b, doubl
     if (rms <= P.toler) state[2] = 1.0;
 }
 
+// S(mu) of the start dipoles (the dipole part of the preparation's structure factors)
+__global__ __launch_bounds__(256) void k_cg_smu_init(ElecDev P, const double* __restrict__ S, double* __restrict__ Smu)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e < 6L * P.K) Smu[e] = S[8 * (e / 6) + 2 + e % 6];
+}
+
+// Fields and structure factors of the converged dipoles for the final passes, from the
+// quantities the solver carried along with mu: emu = T mu (real and reciprocal parts
+// together), efdd = 0, S = S(mu)
+__global__ __launch_bounds__(256) void k_cg_final_fields(ElecDev P, const double* __restrict__ tmu,
+                                                         const double* __restrict__ Smu, double* __restrict__ emu,
+                                                         double* __restrict__ efdd, double* __restrict__ S)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e < 3L * P.nrow) {
+        emu[e] = tmu[e];
+        efdd[e] = 0.0;
+    }
+    if (e < 6L * P.K) S[8 * (e / 6) + 2 + e % 6] = Smu[e];
+}
+
 // packed index of (i, j) in a symmetric 9 x 9 matrix stored as its lower triangle
 __device__ __forceinline__ constexpr int tri9(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
@@ -1070,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __rest
                                                  const double* __restrict__ efdd, const double* __restrict__ dip,
                                                  const double* __restrict__ minv, double* __restrict__ r,
                                                  double* __restrict__ z, double* __restrict__ pv,
-                                                 double* __restrict__ blk)
+                                                 double* __restrict__ tmu, double* __restrict__ blk)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
@@ -1084,6 +1106,7 @@ __global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __rest
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int o = 9 * w + 3 * s + c;
+                tmu[o] = emu[o] + efdd[o];
                 rr[3 * s + c] = on[s] ? (efk[o] + efdc[o] + emu[o] + efdd[o]) - R4PIE0 * dip[o] / p2 : 0.0;
                 const double zs = on[s] ? p2 * rr[3 * s + c] / R4PIE0 : 0.0;
                 v[1] += zs * zs;
@@ -1174,7 +1197,10 @@ __global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int it, int nbq, c
                                                    const double* __restrict__ pv, const double* __restrict__ qv,
                                                    const double* __restrict__ minv, double* __restrict__ dip,
                                                    double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ blk, double* __restrict__ state)
+                                                   const double* __restrict__ emu, const double* __restrict__ efdd,
+                                                   double* __restrict__ tmu, const double* __restrict__ Sp,
+                                                   double* __restrict__ Smu, double* __restrict__ blk,
+                                                   double* __restrict__ state)
 {
     if (state[2] != 0.0) return;
     __shared__ double rs[CG_WPB * 9];
@@ -1190,9 +1216,13 @@ __global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int it, int nbq, c
     double rr = 0.0, zs = 0.0;
     if (on) {
         dip[o] += alpha * pv[o];
+        tmu[o] += alpha * (emu[o] + efdd[o]);      // T mu follows mu: T p is this iteration's field
         rr = r[o] - alpha * qv[o];
         zs = p2 * rr / R4PIE0;
     }
+    // the dipole structure factors follow mu likewise (S(p) from this iteration's sreduce)
+    for (long e = (long)blockIdx.x * 256 + tid; e < 6L * P.K; e += (long)gridDim.x * 256)
+        Smu[e] += alpha * Sp[8 * (e / 6) + 2 + e % 6];
     if (wl < CG_WPB) rs[tid] = rr;
     __syncthreads();
     double zz = 0.0;
@@ -1983,20 +2013,6 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
 }
 
 // T v into emu / efdd (real-space rows and reciprocal space) for the vector v
-// T v into emu / efdd (real-space rows and reciprocal space) for the vector v
-static void launch_tfield(ElecBuffers& E, const ElecDev& P, const double* v, const double* state, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
-                       E.runs, E.runk, E.phc, E.phs, E.chge, v, E.Spart, state);
-    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
-                       E.Sq, E.kc, state);
-    hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
-                       E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, state);
-    hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, P, E.Fpart, v, E.efdd,
-                       E.efdc, state);
-    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, v, E.emu, state);
-}
-
 // Block partials of the solver kernels in E.cgblk: k_cg_apply's at the start, then
 // k_cg_update's (disjoint, since every update block reads all of k_cg_apply's).
 static inline int cg_nbq(const ElecDev& P) { return ceil_div(3 * P.nrow, 64); }
@@ -2009,7 +2025,8 @@ int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
     hipLaunchKernelGGL(k_cg_block, dim3(nbw), dim3(256), 0, s, P, E.x, E.y, E.z, E.cg_minv);
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
     hipLaunchKernelGGL(k_cg_init, dim3(nbw), dim3(256), 0, s, P, E.efk, E.efdc, E.emu, E.efdd, E.dip, E.cg_minv, E.cg_r,
-                       E.cg_z, E.cg_p, E.cgblk);
+                       E.cg_z, E.cg_p, E.cg_tmu, E.cgblk);
+    hipLaunchKernelGGL(k_cg_smu_init, dim3(ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, E.S, E.cg_smu);
     hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s, P, nbw, E.cgblk, E.state);
     return (int)hipGetLastError();
 }
@@ -2032,7 +2049,7 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t
                        E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
     hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
     hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, P, it, nbq, blkq, E.cg_p, E.cg_q, E.cg_minv, E.dip,
-                       E.cg_r, E.cg_z, blku, E.state);
+                       E.cg_r, E.cg_z, E.emu, E.efdd, E.cg_tmu, E.S, E.cg_smu, blku, E.state);
     hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, it, nbu, blku, E.cg_z,
                        E.cg_p, E.state);
     return (int)hipGetLastError();
@@ -2054,7 +2071,11 @@ int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEv
 int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
                     hipStream_t s)
 {
-    if (refresh) launch_tfield(E, P, E.dip, E.zero_state, s);   // fields of the final dipoles
+    // fields and structure factors of the final dipoles: carried along by the conjugate-
+    // gradient solver; the SOR path keeps those of its last iteration, as the reference does
+    if (refresh)
+        hipLaunchKernelGGL(k_cg_final_fields, dim3(ceil_div(std::max(3L * P.nrow, 6L * P.K), 256)), dim3(256), 0, s, P,
+                           E.cg_tmu, E.cg_smu, E.emu, E.efdd, E.S);
     const int nsb = ceil_div(P.n, 256), nwb = ceil_div(P.nw, 64), nkb = ceil_div(P.K, 256);
     double* b_pol = E.blk;                       // [nsb]
     double* b_ks = b_pol + nsb;                  // [nkb][8]

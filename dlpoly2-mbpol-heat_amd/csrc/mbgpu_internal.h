@@ -127,6 +127,7 @@ struct ElecBuffers {
     double *dip = nullptr, *efk = nullptr, *emu = nullptr, *efdc = nullptr, *efdd = nullptr;  // [3n]
     double *cg_p = nullptr, *cg_q = nullptr, *cg_r = nullptr, *cg_z = nullptr;                // [3n] solver vectors
     double* cg_minv = nullptr;                                 // [nw][45] inverse preconditioner blocks
+    double *cg_tmu = nullptr, *cg_smu = nullptr;               // [3n] T mu, [K][6] S(mu) carried by the solver
     double* cgblk = nullptr;                                   // block partials of the solver kernels
     double* ksum = nullptr;                                    // [16][n] chunk sums of the final pass
     double* state = nullptr;                                   // [16] solver scalars
