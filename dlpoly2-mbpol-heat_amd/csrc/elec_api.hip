@@ -11,6 +11,7 @@
 
 #include "../../include/mbgpu.h"
 #include "mbgpu_internal.h"
+#include <cstdlib>
 
 #define R4PIE0 138935.4835
 #define SQRPI 1.7724538509055159
@@ -86,7 +87,7 @@ void free_buffers(ElecBuffers& E)
 {
     dfree(E.x); dfree(E.y); dfree(E.z); dfree(E.chge); dfree(E.polr); dfree(E.polr2); dfree(E.dg); dfree(E.tab);
     dfree(E.nbrA); dfree(E.cntA); dfree(E.nbrP); dfree(E.cntP); dfree(E.tens); dfree(E.tin);
-    dfree(E.dip); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd);
+    dfree(E.dip); dfree(E.dip0); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd);
     dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.cg_tmu); dfree(E.cg_smu); dfree(E.cgblk); dfree(E.ksum); dfree(E.state); dfree(E.zero_state);
     dfree(E.kv); dfree(E.klmn); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs);
     dfree(E.S); dfree(E.Spart); dfree(E.Sq); dfree(E.kc); dfree(E.kf); dfree(E.Fpart);
@@ -107,6 +108,7 @@ int alloc_sites(ElecState& S, int nw, std::string& err)
     ECHK(dalloc(&E.chge, n)); ECHK(dalloc(&E.dg, (size_t)6 * nw)); ECHK(dalloc(&E.tin, (size_t)36 * nw));
     ECHK(dalloc(&E.cntA, n)); ECHK(dalloc(&E.cntP, n));
     ECHK(dalloc(&E.dip, (size_t)3 * n)); ECHK(dalloc(&E.efk, (size_t)3 * n)); ECHK(dalloc(&E.emu, (size_t)3 * n));
+    ECHK(dalloc(&E.dip0, (size_t)3 * n));
     ECHK(dalloc(&E.efdc, (size_t)3 * n)); ECHK(dalloc(&E.efdd, (size_t)3 * n));
     ECHK(dalloc(&E.cg_p, (size_t)3 * n)); ECHK(dalloc(&E.cg_q, (size_t)3 * n));
     ECHK(dalloc(&E.cg_r, (size_t)3 * n)); ECHK(dalloc(&E.cg_z, (size_t)3 * n));
@@ -394,6 +396,9 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
         // rows: sites within rcut at a density of up to 0.15 sites per A^3, rounded to 64
         const double vol = 4.0 / 3.0 * PI_D * p->rcut * p->rcut * p->rcut;
         int cap = (int)(0.15 * vol) + 64;
+        // MBGPU_ELEC_ROWS=c starts from rows of c entries instead (a test aid: it makes the
+        // row-growth path run on ordinary configurations)
+        if (const char* ev = getenv("MBGPU_ELEC_ROWS")) cap = std::max(1, atoi(ev));
         cap = std::min(((cap + 63) / 64) * 64, ((P.n + 63) / 64) * 64);
         if (alloc_rows(S, cap, cap, err)) return -1;
     }
@@ -417,17 +422,78 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     ElecBuffers& E = S.E;
     ElecDev& P = S.P;
     S.ready = false;
+    // The neighbour rows' capacity check is read back with the solver's first convergence
+    // check instead of after the row build (one host round trip less): the rows are built
+    // with clamped counts, so an overflowed attempt runs in bounds and is simply repeated
+    // with larger rows, from the same start dipoles (kept in dip0).
+    ECHK(hipMemcpyAsync(E.dip0, E.dip, sizeof(double) * 3 * P.n, hipMemcpyDeviceToDevice, s));
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int attempt = 0;; ++attempt) {
+        if (attempt) ECHK(hipMemcpyAsync(E.dip, E.dip0, sizeof(double) * 3 * P.n, hipMemcpyDeviceToDevice, s));
         ECHK(hipMemsetAsync(E.err, 0, 4 * sizeof(int), s));
         int rc = el_launch_setup(E, P, s);
         if (rc) {
             err = std::string("electrostatics set-up kernels failed: ") + hipGetErrorString((hipError_t)rc);
             return rc;
         }
-        int herr = 0;
-        ECHK(hipMemcpyAsync(&herr, E.err, sizeof(int), hipMemcpyDeviceToHost, s));
-        ECHK(hipStreamSynchronize(s));
-        if (!herr) break;
+        ECHK(hipMemsetAsync(E.state, 0, 16 * sizeof(double), s));
+        ECHK(hipMemsetAsync(E.emu, 0, sizeof(double) * 3 * P.n, s));
+        // M sites carry no induced dipole (polr2 = 0): clear any start value
+        ECHK(hipMemsetAsync(E.dip + 3 * P.nreal, 0, sizeof(double) * 3 * (P.n - P.nreal), s));
+        rc = el_launch_prep(E, P, S.rcell, s);
+        if (rc) {
+            err = std::string("electrostatics preparation kernels failed: ") + hipGetErrorString((hipError_t)rc);
+            return rc;
+        }
+        // solver loop in batches; a converged state turns the rest of a batch into no-ops.
+        // The first batch covers the last evaluation's count and two more (one host round
+        // trip in steady state; an iteration past convergence costs a few microseconds of
+        // no-op launches, a second round trip tens)
+        int launched = 0, batch = S.last_iters > 0 ? std::min(S.last_iters + 2, 64) : (solver == 1 ? 8 : 12);
+        if (solver == 1) {
+            rc = el_launch_cg_init(E, P, s);
+            if (rc) {
+                err = std::string("electrostatics solver kernels failed: ") + hipGetErrorString((hipError_t)rc);
+                return rc;
+            }
+        }
+        bool overflow = false;
+        for (;;) {
+            for (int b = 0; b < batch; ++b) {
+                rc = solver == 1 ? el_launch_cg_iteration(E, P, launched, s)
+                                 : el_launch_iteration(E, P, launched == 0, s);
+                ++launched;
+                if (rc) {
+                    err = std::string("electrostatics iteration kernels failed: ") + hipGetErrorString((hipError_t)rc);
+                    return rc;
+                }
+            }
+            int herr = 0;
+            ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
+            ECHK(hipMemcpyAsync(&herr, E.err, sizeof(int), hipMemcpyDeviceToHost, s));
+            ECHK(hipStreamSynchronize(s));
+            if (herr) {
+                overflow = true;
+                break;
+            }
+            if (st[2] != 0.0) {
+                S.last_iters = (int)st[0];
+                break;
+            }
+            if (!(st[1] == st[1])) {
+                err = "induced dipoles became NaN";
+                return -3;
+            }
+            if (st[0] >= (double)max_iter) {
+                char buf[160];
+                snprintf(buf, sizeof buf, "induced dipoles did not converge in %d iterations (rms change %.3e)",
+                         (int)st[0], st[1]);
+                err = buf;
+                return -3;
+            }
+            batch = solver == 1 ? 4 : 8;
+        }
+        if (!overflow) break;
         const int full = ((P.n + 63) / 64) * 64;
         if (attempt >= 2 || E.capA >= full) {
             err = "electrostatics neighbour rows exceed their capacity";
@@ -437,56 +503,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         P.capA = E.capA;
         P.capP = E.capP;
     }
-    ECHK(hipMemsetAsync(E.state, 0, 16 * sizeof(double), s));
-    ECHK(hipMemsetAsync(E.emu, 0, sizeof(double) * 3 * P.n, s));
-    // M sites carry no induced dipole (polr2 = 0): clear any start value
-    ECHK(hipMemsetAsync(E.dip + 3 * P.nreal, 0, sizeof(double) * 3 * (P.n - P.nreal), s));
-    int rc = el_launch_prep(E, P, S.rcell, s);
-    if (rc) {
-        err = std::string("electrostatics preparation kernels failed: ") + hipGetErrorString((hipError_t)rc);
-        return rc;
-    }
-    // solver loop in batches; a converged state turns the rest of a batch into no-ops
-    // the first batch covers the last evaluation's count and two more (one host round trip
-    // in steady state; an iteration past convergence costs a few microseconds of no-op
-    // launches, a second round trip tens)
-    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int launched = 0, batch = S.last_iters > 0 ? std::min(S.last_iters + 2, 64) : (solver == 1 ? 8 : 12);
-    if (solver == 1) {
-        rc = el_launch_cg_init(E, P, s);
-        if (rc) {
-            err = std::string("electrostatics solver kernels failed: ") + hipGetErrorString((hipError_t)rc);
-            return rc;
-        }
-    }
-    for (;;) {
-        for (int b = 0; b < batch; ++b) {
-            rc = solver == 1 ? el_launch_cg_iteration(E, P, launched, s) : el_launch_iteration(E, P, launched == 0, s);
-            ++launched;
-            if (rc) {
-                err = std::string("electrostatics iteration kernels failed: ") + hipGetErrorString((hipError_t)rc);
-                return rc;
-            }
-        }
-        ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
-        ECHK(hipStreamSynchronize(s));
-        if (st[2] != 0.0) {
-            S.last_iters = (int)st[0];
-            break;
-        }
-        if (!(st[1] == st[1])) {
-            err = "induced dipoles became NaN";
-            return -3;
-        }
-        if (st[0] >= (double)max_iter) {
-            char buf[160];
-            snprintf(buf, sizeof buf, "induced dipoles did not converge in %d iterations (rms change %.3e)",
-                     (int)st[0], st[1]);
-            err = buf;
-            return -3;
-        }
-        batch = solver == 1 ? 4 : 8;
-    }
+    int rc = 0;
     const double g34 = exp(gammln_host(0.75)), gln34 = gammln_host(0.75);
     rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, s);
     if (rc) {

@@ -125,6 +125,7 @@ struct ElecBuffers {
     double* tens = nullptr;                                    // [6][n*capP] dipole tensors
     double* tin = nullptr;                                     // [nw][6 pairs][6] intramolecular tensors
     double *dip = nullptr, *efk = nullptr, *emu = nullptr, *efdc = nullptr, *efdd = nullptr;  // [3n]
+    double* dip0 = nullptr;                                    // [3n] start dipoles of the evaluation
     double *cg_p = nullptr, *cg_q = nullptr, *cg_r = nullptr, *cg_z = nullptr;                // [3n] solver vectors
     double* cg_minv = nullptr;                                 // [nw][45] inverse preconditioner blocks
     double *cg_tmu = nullptr, *cg_smu = nullptr;               // [3n] T mu, [K][6] S(mu) carried by the solver

@@ -5,6 +5,8 @@
 * Several slots driven concurrently from host threads (their kernels overlap on the GPU):
   every result is bit-identical to the same slot evaluated alone.
 * The FP32-polynomial mode stays within its measured error envelope against FP64.
+* Electrostatics neighbour rows that start too small (MBGPU_ELEC_ROWS): the evaluation
+  grows them and repeats from the same start dipoles, bit-identical to a normal context.
 """
 import importlib
 
@@ -108,3 +110,17 @@ def test_no_read_of_unwritten_memory(pkg, waterbox, engine, solver, monkeypatch)
             assert np.array_equal(w1[k], w0[k]), k
         for a, b in zip(m1, m0):
             assert np.array_equal(a, b)
+
+
+def test_neighbour_rows_grow_and_repeat(pkg, waterbox, engine, monkeypatch):
+    box = waterbox.box_64()
+    monkeypatch.setenv("MBGPU_ELEC_ROWS", "64")     # sites within 6 A at liquid density: ~120
+    small = engine.Slot(box)
+    print("evaluation with 64-entry neighbour rows (grown to 128, then repeated)", flush=True)
+    r1 = small.evaluate()
+    monkeypatch.delenv("MBGPU_ELEC_ROWS")
+    ref = engine.Slot(box)
+    r0 = ref.evaluate()
+    assert r1["eng"] == r0["eng"] and np.array_equal(r1["jtot"], r0["jtot"])
+    assert np.array_equal(small.forces(), ref.forces())
+    assert r1["dipole_iterations"] == r0["dipole_iterations"]
