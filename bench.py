@@ -25,7 +25,11 @@ Legs (all timed over --steps steps after --warmup):
     virial and heat-flux error against the FP64 leg on the same configuration.
 `value` is the fastest leg whose RMS force error against FP64 is <= 1e-5 kcal/mol/A.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+The legs after them: NVE on the device (with the heat-current autocorrelation), a ring
+polymer of 8 beads of 256 waters, and same-size serving with p50/p99 latency.  Prints one
+JSON line: metric, value, unit, timing and configuration, a `roofline` object for the
+dominant kernel (achieved vs peak FLOP/s, HBM traffic per launch) and a `cpu_baseline`
+object (the CPU path timed on this host).
 """
 import argparse
 import ctypes
@@ -166,8 +170,8 @@ def main():
         print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is "
-              "described in DESIGN.md but not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every run of this library "
+              "uses a single GPU; how the path would shard is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
     if args.batch > 4:
