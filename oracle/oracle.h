@@ -1,7 +1,7 @@
 /*
  * oracle.h -- CPU restatement of the reference's MB-pol force / heat-current path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only the
+ * TEST INFRASTRUCTURE ONLY.  The HIP path is tested against this library: only the
  * tests, the smoke check and the CPU-baseline leg of the benchmark load it.  The
  * product library (libmbgpu.so) never links or calls it.
  *
