@@ -18,6 +18,9 @@
 #ifndef EL_DCHUNK_MAX
 #define EL_DCHUNK_MAX 32      // site chunks of the dipole structure-factor pass (partials re-read by the sum)
 #endif
+#ifndef EL_SPLIT
+#define EL_SPLIT 2  // threads per k run in the structure-factor pass (elec_kernels.hip)
+#endif
 #ifndef EL_KCHUNK_WAVES
 #define EL_KCHUNK_WAVES 4096  // site-block x k-chunk product the reciprocal field passes aim at
 #endif
@@ -280,7 +283,7 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     ECHK(hipMemcpy(E.runk, runk.data(), sizeof(int) * runk.size(), hipMemcpyHostToDevice));
     // site chunks of the structure-factor passes: up to 16 (their partials are re-read
     // by the chunk sum), about 1024 wavefronts (4 per block of 128 runs)
-    const int rblocks = (2 * E.nrun + 255) / 256;
+    const int rblocks = (EL_SPLIT * E.nrun + 255) / 256;
     const int nrow = 3 * S.nw;
     E.schunks = std::max(1, std::min(16, std::min((E.n + 15) / 16, 1024 / (4 * rblocks))));
     E.spc = (E.n + E.schunks - 1) / E.schunks;

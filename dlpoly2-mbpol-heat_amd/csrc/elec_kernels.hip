@@ -514,7 +514,10 @@ __device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c,
 #define EL_TILE 16
 #endif
 #define EL_TROW (EL_TILE + 1)
-#define EL_HALF (EL_RUN / 2)
+#ifndef EL_SPLIT
+#define EL_SPLIT 2                    // threads per k run in the structure-factor pass
+#endif
+#define EL_HALF (EL_RUN / EL_SPLIT)
 template <bool CHG>
 __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* __restrict__ runs,
                                                  const int* __restrict__ runk, const double* __restrict__ phc,
@@ -526,12 +529,12 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
     extern __shared__ double lds[];
     const int tid = threadIdx.x;
     const int g = blockIdx.x * 256 + tid;
-    const int r = g >> 1, half = g & 1;
+    const int r = g / EL_SPLIT, half = g % EL_SPLIT;
     const int chunk = blockIdx.y;
     const int ns = CHG ? P.n : P.nrow;
     const int j0 = chunk * spc, j1 = min(ns, j0 + spc);
-    const int rfirst = (blockIdx.x * 256) >> 1;
-    const int rlast = min(P.nrun - 1, (blockIdx.x * 256 + 255) >> 1);
+    const int rfirst = (blockIdx.x * 256) / EL_SPLIT;
+    const int rlast = min(P.nrun - 1, (blockIdx.x * 256 + 255) / EL_SPLIT);
     const int lmin = runs[4 * rfirst], lmax = runs[4 * rlast];
     const int nl = lmax - lmin + 1, nm = P.km[1] + 1, nn = P.km[2] + 1;
     const int nrows = nl + nm + nn;
@@ -1980,7 +1983,7 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hi
     hipLaunchKernelGGL(k_el_phase, dim3(ceil_div(P.n, 64)), dim3(64), 0, s, P, E.x, E.y, E.z, rcell_dev, E.phc,
                        E.phs);
     // charge structure factors and the iloop = 1 reciprocal fields
-    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div(2L * P.nrun, 256), E.schunks), dim3(256), sfac_lds(P), s, P, E.spc,
+    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.schunks), dim3(256), sfac_lds(P), s, P, E.spc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
     hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
@@ -1996,7 +1999,7 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
     if (!first) {
         // reciprocal dipole field from the current dipoles (the first iteration's came
         // with the charge field)
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
         hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
                            E.S, E.Sq, E.kc, E.state);
@@ -2041,7 +2044,7 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t
     double* blkq = E.cgblk;
     double* blku = E.cgblk + nbq;
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
-    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
+    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, p, E.Spart, E.state);
     hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
@@ -2062,7 +2065,7 @@ int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEv
 {
     (void)hipEventRecord(a, s);
     for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.zero_state);
     (void)hipEventRecord(b, s);
     return (int)hipGetLastError();
@@ -2086,7 +2089,7 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
     // structure factors of the converged dipoles (already computed with the refreshed fields)
     if (!refresh) {
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div(2L * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
         hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
                            (const double*)E.zero_state);
