@@ -1195,6 +1195,7 @@ __global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __res
 // mu += alpha p, r -= alpha q, z = M^-1 r, partials (r.z, |polr2 r / r4pie0|^2); one
 // thread per (water, row), CG_WPB waters per block, the new r of a water shared through
 // LDS.  alpha = r.z / p.q, with p.q summed from k_cg_apply's nbq partials by every block.
+// The blocks after the waters' carry S(mu) along (S(mu) += alpha S(p)).
 #define CG_WPB 28
 __global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int it, int nbq, const double* __restrict__ blkq,
                                                    const double* __restrict__ pv, const double* __restrict__ qv,
@@ -1210,6 +1211,14 @@ __global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int it, int nbq, c
     const int tid = threadIdx.x;
     const double alpha = state[8 + (it & 1)] / block_partials_sum<1>(blkq, nbq, 0);
     if (blockIdx.x == 0 && tid == 0) state[4] = alpha;
+    const int nbw = (P.nw + CG_WPB - 1) / CG_WPB;
+    if ((int)blockIdx.x >= nbw) {
+        // blocks past the waters: the dipole structure factors follow mu (S(p) from this
+        // iteration's sreduce), 256 entries per block
+        const long e = (long)(blockIdx.x - nbw) * 256 + tid;
+        if (e < 6L * P.K) Smu[e] += alpha * Sp[8 * (e / 6) + 2 + e % 6];
+        return;
+    }
     const int wl = tid / 9, i = tid - 9 * wl;
     const int w = blockIdx.x * CG_WPB + wl;
     const bool act = wl < CG_WPB && w < P.nw;
@@ -1223,9 +1232,6 @@ __global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int it, int nbq, c
         rr = r[o] - alpha * qv[o];
         zs = p2 * rr / R4PIE0;
     }
-    // the dipole structure factors follow mu likewise (S(p) from this iteration's sreduce)
-    for (long e = (long)blockIdx.x * 256 + tid; e < 6L * P.K; e += (long)gridDim.x * 256)
-        Smu[e] += alpha * Sp[8 * (e / 6) + 2 + e % 6];
     if (wl < CG_WPB) rs[tid] = rr;
     __syncthreads();
     double zz = 0.0;
@@ -2051,7 +2057,8 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t
     hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
                        E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
     hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
-    hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, P, it, nbq, blkq, E.cg_p, E.cg_q, E.cg_minv, E.dip,
+    hipLaunchKernelGGL(k_cg_update, dim3(nbu + ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, it, nbq, blkq, E.cg_p,
+                       E.cg_q, E.cg_minv, E.dip,
                        E.cg_r, E.cg_z, E.emu, E.efdd, E.cg_tmu, E.S, E.cg_smu, blku, E.state);
     hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, it, nbu, blku, E.cg_z,
                        E.cg_p, E.state);
