@@ -1695,16 +1695,50 @@ __global__ __launch_bounds__(64) void k_el_intra_final(ElecDev P, const double* 
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (w < P.nw) {
+        // the water's four sites (O, H1, H2, M); every sum of the water's rows is kept in
+        // registers (slots 0..3, fully unrolled) and stored once
         const int s[4] = {3 * w, 3 * w + 1, 3 * w + 2, P.nreal + w};
+        double xs[4][3];
+#pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const int i = s[a];
-            for (int q = 0; q < 3; ++q) f3[3 * i + q] = 0.0;
-            e3[i] = 0.0;
-            vc[i] = 0.0;
-            if (i < P.nreal)
-                for (int t = 0; t < 9; ++t) w3[9 * i + t] = 0.0;
+            xs[a][0] = x[s[a]];
+            xs[a][1] = y[s[a]];
+            xs[a][2] = z[s[a]];
         }
+        double fl[4][3], el[4], vl[4], wl[3][9];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            fl[a][0] = fl[a][1] = fl[a][2] = 0.0;
+            el[a] = vl[a] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int t = 0; t < 9; ++t) wl[a][t] = 0.0;
+        const double wt[3] = {1.0 - P.gamma, 0.5 * P.gamma, 0.5 * P.gamma};
+        // force-matrix contraction of the pair force F on slot ai from slot bi (fm_local)
+        auto fml = [&](int ai, int bi, const double (&F)[3]) {
+            if (bi == 3) return;
+            if (ai < 3) {
+                if (ai == bi) return;
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) wl[ai][3 * p + q] += F[p] * (xs[ai][q] - xs[bi][q]);
+                return;
+            }
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                if (t == bi) continue;
+#pragma unroll
+                for (int p = 0; p < 3; ++p)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) wl[t][3 * p + q] += wt[t] * F[p] * (xs[t][q] - xs[bi][q]);
+            }
+        };
+#pragma unroll
         for (int a = 0; a < 4; ++a)
+#pragma unroll
             for (int b = a + 1; b < 4; ++b) {
                 const int i = s[a], j = s[b];
                 double d[3];
@@ -1718,8 +1752,8 @@ __global__ __launch_bounds__(64) void k_el_intra_final(ElecDev P, const double* 
                 const double qi = chge[i], qj = chge[j];
                 const double cicj = qi * qj;
                 red[0] += cicj * bbb[0] * R4PIE0;
-                e3[i] += 0.5 * cicj * bbb[0] * R4PIE0;
-                e3[j] += 0.5 * cicj * bbb[0] * R4PIE0;
+                el[a] += 0.5 * cicj * bbb[0] * R4PIE0;
+                el[b] += 0.5 * cicj * bbb[0] * R4PIE0;
                 const double espdum = bbb[0] * R4PIE0;
                 const double espdcdum = bbb[1] * R4PIE0;
                 double ffac = cicj * bbb[1] * R4PIE0;
@@ -1761,13 +1795,14 @@ __global__ __launch_bounds__(64) void k_el_intra_final(ElecDev P, const double* 
                 ffac += facmu;
                 const double fv[3] = {ffac * d[0] + dfac[0], ffac * d[1] + dfac[1], ffac * d[2] + dfac[2]};
                 const double virdum = dfac[0] * d[0] + dfac[1] * d[1] + dfac[2] * d[2];
+#pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    f3[3 * i + q] += fv[q];
-                    f3[3 * j + q] -= fv[q];
+                    fl[a][q] += fv[q];
+                    fl[b][q] -= fv[q];
                 }
                 const double mf[3] = {-fv[0], -fv[1], -fv[2]};
-                fm_local(P, x, y, z, i, j, fv, w3);
-                fm_local(P, x, y, z, j, i, mf, w3);
+                fml(a, b, fv);
+                fml(b, a, mf);
                 red[1] += -ffac * rsq - virdum;
                 red[2] += d[0] * fv[0];
                 red[3] += d[0] * fv[1];
@@ -1775,11 +1810,23 @@ __global__ __launch_bounds__(64) void k_el_intra_final(ElecDev P, const double* 
                 red[5] += d[1] * fv[1];
                 red[6] += d[1] * fv[2];
                 red[7] += d[2] * fv[2];
-                vc[i] += qj * espdum;
-                vc[j] += qi * espdum;
-                vc[i] += rdmuj * espdcdum;
-                vc[j] -= rdmui * espdcdum;
+                vl[a] += qj * espdum;
+                vl[b] += qi * espdum;
+                vl[a] += rdmuj * espdcdum;
+                vl[b] -= rdmui * espdcdum;
             }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int i = s[a];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) f3[3 * i + q] = fl[a][q];
+            e3[i] = el[a];
+            vc[i] = vl[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int t = 0; t < 9; ++t) w3[9L * s[a] + t] = wl[a][t];
     }
     block_sum_store(red, blk + 8 * blockIdx.x);
 }
