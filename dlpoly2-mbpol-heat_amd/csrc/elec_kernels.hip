@@ -733,12 +733,24 @@ __global__ __launch_bounds__(256) void k_el_kfield(ElecDev P, const int* __restr
     const double ca1 = phc[o1 + ja], sa1 = phs[o1 + ja], cb1 = phc[o1 + jB], sb1 = phs[o1 + jB];
     double fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0}, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
     int kk = 0;
-    for (int r = 0; r < nr; ++r) {
-        const int l = sr[4 * r], m = sr[4 * r + 1], n0 = sr[4 * r + 2], cnt = sr[4 * r + 3];
+    // the phase-table entries of a run are loaded one run ahead (double-buffered in
+    // registers), so their latency overlaps the previous run's arithmetic
+    double qa[6], qb[6];
+    auto load_run = [&](int r) {
+        const int l = sr[4 * r], m = sr[4 * r + 1], n0 = sr[4 * r + 2];
         const long oa = (long)l * P.n, ob = ((long)ks + abs(m)) * P.n, oc = ((long)2 * ks + abs(n0)) * P.n;
+        qa[0] = phc[oa + ja]; qa[1] = phs[oa + ja]; qa[2] = phc[ob + ja];
+        qa[3] = phs[ob + ja]; qa[4] = phc[oc + ja]; qa[5] = phs[oc + ja];
+        qb[0] = phc[oa + jB]; qb[1] = phs[oa + jB]; qb[2] = phc[ob + jB];
+        qb[3] = phs[ob + jB]; qb[4] = phc[oc + jB]; qb[5] = phs[oc + jB];
+    };
+    if (nr > 0) load_run(0);
+    for (int r = 0; r < nr; ++r) {
+        const int m = sr[4 * r + 1], n0 = sr[4 * r + 2], cnt = sr[4 * r + 3];
         double cA, sA, cB, sB;
-        run_phase(phc[oa + ja], phs[oa + ja], phc[ob + ja], phs[ob + ja], phc[oc + ja], phs[oc + ja], m, n0, cA, sA);
-        run_phase(phc[oa + jB], phs[oa + jB], phc[ob + jB], phs[ob + jB], phc[oc + jB], phs[oc + jB], m, n0, cB, sB);
+        run_phase(qa[0], qa[1], qa[2], qa[3], qa[4], qa[5], m, n0, cA, sA);
+        run_phase(qb[0], qb[1], qb[2], qb[3], qb[4], qb[5], m, n0, cB, sB);
+        if (r + 1 < nr) load_run(r + 1);
 #pragma unroll
         for (int t = 0; t < EL_RUN; ++t) {
             if (t < cnt) {
