@@ -111,6 +111,10 @@ bool make_geom(int imcon, const double* cell, CellGeom& g)
 struct mbgpu_ctx {
     MbBuffers B;
     hipStream_t stream = nullptr;
+    // the electrostatics run on a stream of their own, so that a caller may evaluate them
+    // from one host thread while the MB-pol and water terms run from another (the parts
+    // share no buffers; the heat flux, called after both, reads their per-atom results)
+    hipStream_t el_stream = nullptr;
     int device = 0;
     // host-side staging for the host-pointer entry points
     double *hx = nullptr, *hy = nullptr, *hz = nullptr;   // device copies of positions
@@ -386,7 +390,7 @@ mbgpu_ctx* mbgpu_create(int nw, int natms)
     c->B.nw = nw;
     c->B.natms = natms;
     (void)hipGetDevice(&c->device);
-    if (hipStreamCreate(&c->stream) != hipSuccess || alloc_fixed(c) ||
+    if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->el_stream) != hipSuccess || alloc_fixed(c) ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
         std::string msg = g_err;
@@ -403,6 +407,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->el_stream) (void)hipStreamSynchronize(c->el_stream);
     MbBuffers& B = c->B;
     dfree(B.mon); dfree(B.nbr2); dfree(B.cnt2); dfree(B.off2); dfree(B.nbr3); dfree(B.cnt3);
     dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
@@ -421,6 +426,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->el_stream) (void)hipStreamDestroy(c->el_stream);
     delete c;
 }
 
@@ -601,23 +607,23 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* c, int imcon, const double cell[9]
     c->el.ready = false;
     if (elec_configure(c->el, nw, g, cell, p, err)) return fail(-1, "%s", err.c_str());
     ElecBuffers& E = c->el.E;
-    HIPCHK(hipMemcpyAsync(E.x, x_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(E.y, y_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(E.z, z_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(E.x, x_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->el_stream));
+    HIPCHK(hipMemcpyAsync(E.y, y_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->el_stream));
+    HIPCHK(hipMemcpyAsync(E.z, z_dev, sizeof(double) * n, hipMemcpyDeviceToDevice, c->el_stream));
     if (dip_dev)
-        HIPCHK(hipMemcpyAsync(E.dip, dip_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(E.dip, dip_dev, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->el_stream));
     else
-        el_launch_zero(E.dip, 3L * n, c->stream);   // start from zero dipoles
-    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, p->solver, c->stream, err);
+        el_launch_zero(E.dip, 3L * n, c->el_stream);   // start from zero dipoles
+    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, p->solver, c->el_stream, err);
     if (rc) return fail(rc, "%s", err.c_str());
     const int nr = 3 * nw;
-    HIPCHK(hipMemcpyAsync(x_dev + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(y_dev + nr, E.y + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(z_dev + nr, E.z + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(x_dev + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->el_stream));
+    HIPCHK(hipMemcpyAsync(y_dev + nr, E.y + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->el_stream));
+    HIPCHK(hipMemcpyAsync(z_dev + nr, E.z + nr, sizeof(double) * nw, hipMemcpyDeviceToDevice, c->el_stream));
     if (dip_dev)
-        HIPCHK(hipMemcpyAsync(dip_dev, E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
-    if (f_dev) el_launch_add(f_dev, E.fsite, 3L * n, c->stream);
-    HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpyAsync(dip_dev, E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->el_stream));
+    if (f_dev) el_launch_add(f_dev, E.fsite, 3L * n, c->el_stream);
+    HIPCHK(hipStreamSynchronize(c->el_stream));
     const double* o = c->el.out;
     if (engcpe) *engcpe += o[4];
     if (vircpe) *vircpe += o[5];
@@ -650,21 +656,21 @@ int mbgpu_polar_electrostatics(mbgpu_ctx* c, int imcon, const double cell[9], co
         dip[3 * i + 1] = dipy[i];
         dip[3 * i + 2] = dipz[i];
     }
-    HIPCHK(hipMemcpyAsync(E.x, xxx, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(E.y, yyy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(E.z, zzz, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(E.dip, dip.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, p->solver, c->stream, err);
+    HIPCHK(hipMemcpyAsync(E.x, xxx, sizeof(double) * n, hipMemcpyHostToDevice, c->el_stream));
+    HIPCHK(hipMemcpyAsync(E.y, yyy, sizeof(double) * n, hipMemcpyHostToDevice, c->el_stream));
+    HIPCHK(hipMemcpyAsync(E.z, zzz, sizeof(double) * n, hipMemcpyHostToDevice, c->el_stream));
+    HIPCHK(hipMemcpyAsync(E.dip, dip.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->el_stream));
+    int rc = elec_evaluate(c->el, p->max_iter, p->distribute_msite, p->solver, c->el_stream, err);
     if (rc) return fail(rc, "%s", err.c_str());
     const int nr = 3 * nw;
     std::vector<double> f((size_t)3 * n);
-    HIPCHK(hipMemcpyAsync(xxx + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(yyy + nr, E.y + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(zzz + nr, E.z + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(dip.data(), E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(f.data(), E.fsite, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
-    if (chge) HIPCHK(hipMemcpyAsync(chge, E.chge, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(xxx + nr, E.x + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->el_stream));
+    HIPCHK(hipMemcpyAsync(yyy + nr, E.y + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->el_stream));
+    HIPCHK(hipMemcpyAsync(zzz + nr, E.z + nr, sizeof(double) * nw, hipMemcpyDeviceToHost, c->el_stream));
+    HIPCHK(hipMemcpyAsync(dip.data(), E.dip, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->el_stream));
+    HIPCHK(hipMemcpyAsync(f.data(), E.fsite, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->el_stream));
+    if (chge) HIPCHK(hipMemcpyAsync(chge, E.chge, sizeof(double) * n, hipMemcpyDeviceToHost, c->el_stream));
+    HIPCHK(hipStreamSynchronize(c->el_stream));
     for (int i = 0; i < n; ++i) {
         dipx[i] = dip[3 * i];
         dipy[i] = dip[3 * i + 1];
@@ -691,13 +697,13 @@ int mbgpu_get_elec_details(mbgpu_ctx* c, double* scal, double* chge, double* ves
     const int nw = c->B.nw, n = 4 * nw, nr = 3 * nw;
     if (scal)
         for (int i = 0; i < 20; ++i) scal[i] = c->el.out[i];
-    if (chge) HIPCHK(hipMemcpyAsync(chge, E.chge, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    if (vesp) HIPCHK(hipMemcpyAsync(vesp, E.vesp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    if (e_real) HIPCHK(hipMemcpyAsync(e_real, E.eout, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
-    if (w_real) HIPCHK(hipMemcpyAsync(w_real, E.wout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->stream));
-    if (s_real) HIPCHK(hipMemcpyAsync(s_real, E.sout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->stream));
-    if (f_site) HIPCHK(hipMemcpyAsync(f_site, E.fsite, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (chge) HIPCHK(hipMemcpyAsync(chge, E.chge, sizeof(double) * n, hipMemcpyDeviceToHost, c->el_stream));
+    if (vesp) HIPCHK(hipMemcpyAsync(vesp, E.vesp, sizeof(double) * n, hipMemcpyDeviceToHost, c->el_stream));
+    if (e_real) HIPCHK(hipMemcpyAsync(e_real, E.eout, sizeof(double) * nr, hipMemcpyDeviceToHost, c->el_stream));
+    if (w_real) HIPCHK(hipMemcpyAsync(w_real, E.wout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->el_stream));
+    if (s_real) HIPCHK(hipMemcpyAsync(s_real, E.sout, sizeof(double) * 9 * nr, hipMemcpyDeviceToHost, c->el_stream));
+    if (f_site) HIPCHK(hipMemcpyAsync(f_site, E.fsite, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->el_stream));
+    HIPCHK(hipStreamSynchronize(c->el_stream));
     return 0;
 }
 
@@ -945,8 +951,8 @@ int mbgpu_get_dipoles(mbgpu_ctx* c, double* dip)
     if (!c) return fail(-1, "null context");
     if (!c->el.ready) return fail(-1, "no completed electrostatics evaluation in this context");
     if (dip)
-        HIPCHK(hipMemcpyAsync(dip, c->el.E.dip, sizeof(double) * 3 * 4 * c->B.nw, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpyAsync(dip, c->el.E.dip, sizeof(double) * 3 * 4 * c->B.nw, hipMemcpyDeviceToHost, c->el_stream));
+    HIPCHK(hipStreamSynchronize(c->el_stream));
     return 0;
 }
 
@@ -984,7 +990,7 @@ int mbgpu_time_elec(mbgpu_ctx* c, int reps, double* ms_sfac, double* flop_sfac)
     if (!c || !c->el.ready) return fail(-1, "mbgpu_time_elec: run mbgpu_polar_electrostatics first");
     if (reps < 1) return fail(-1, "reps must be positive");
     ElecState& S = c->el;
-    int rc = el_time_sfac(S.E, S.P, reps, c->ev0, c->ev1, c->stream);
+    int rc = el_time_sfac(S.E, S.P, reps, c->ev0, c->ev1, c->el_stream);
     if (rc) return fail(rc, "timing run failed");
     float ms = 0.f;
     HIPCHK(hipEventSynchronize(c->ev1));

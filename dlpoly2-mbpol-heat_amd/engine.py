@@ -14,6 +14,7 @@ results do not depend on what runs beside it: every kernel sums in a fixed order
 """
 import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -25,7 +26,7 @@ ENGUNIT = 418.4
 
 
 class Slot:
-    def __init__(self, box, precision=0, solver=1, elec=None, water=None, device="cuda:0"):
+    def __init__(self, box, precision=0, solver=1, elec=None, water=None, device="cuda:0", overlap=True):
         import torch
         self.torch = torch
         self.dev = torch.device(device)
@@ -34,6 +35,8 @@ class Slot:
         self.ctx.set_precision(precision)
         self.precision = precision
         self.solver = solver
+        self.overlap = bool(overlap)
+        self._helper = None
         self.set_params(elec or _ep.for_box(box.L), water or _wp.for_box(box.L))
         self.stream = torch.cuda.Stream(device=self.dev)
         self._alloc(box)
@@ -72,19 +75,45 @@ class Slot:
 
     def evaluate(self):
         """One complete force evaluation; returns a dict of scalars (internal units).
-        Everything runs on the context's stream: the induced dipoles start from zero inside
-        the library and the forces stay in the context (see forces())."""
+        The induced dipoles start from zero inside the library and the forces stay in the
+        context (see forces()).  With `overlap` set, the electrostatics (the context's second
+        stream) run from a helper thread while this thread runs MB-pol and the water terms;
+        the parts share no buffers and the heat flux starts after both have returned."""
         b, c = self.box, self.ctx
         x, y, z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
-        e_mb, v_mb, s_mb = c.mbpol_forces_dev(b.imcon, b.cell, self.lst.data_ptr(), x, y, z)
-        e_el, v_el, s_el, it = c.polar_electrostatics_dev(b.imcon, b.cell, self.eparams, x, y, z, None)
-        e_w, v_w, s_w = c.water_terms_dev(b.imcon, b.cell, self.wparams, x, y, z)
+        (e_el, v_el, s_el, it), ((e_mb, v_mb, s_mb), (e_w, v_w, s_w)) = self.split(
+            lambda: c.polar_electrostatics_dev(b.imcon, b.cell, self.eparams, x, y, z, None),
+            lambda: (c.mbpol_forces_dev(b.imcon, b.cell, self.lst.data_ptr(), x, y, z),
+                     c.water_terms_dev(b.imcon, b.cell, self.wparams, x, y, z)))
         jk, jp, jt = c.heat_flux_dev(self.v[0].data_ptr(), self.v[1].data_ptr(), self.v[2].data_ptr(),
                                      self.ek.data_ptr())
         return dict(eng=e_mb + e_el + float(np.sum(e_w)), eng_mbpol=e_mb, eng_elec=e_el, eng_water=np.array(e_w),
                     vir_water=np.array(v_w),
                     vir=v_mb + v_el + float(np.sum(v_w)), stress=np.array(s_mb) + s_el + s_w, dipole_iterations=it,
                     jkin=np.array(jk), jpot=np.array(jp), jtot=np.array(jt))
+
+    def split(self, elec, rest):
+        """Returns (elec(), rest()).  With `overlap` set, elec() (the electrostatics, which
+        the library runs on the context's second stream) runs from a helper thread while
+        rest() (MB-pol and water terms on the first stream) runs from this one; every
+        library entry waits for its stream before it returns, so both parts are complete
+        here.  Without it, rest() then elec()."""
+        if not self.overlap:
+            r = rest()
+            return elec(), r
+        if self._helper is None:
+            self._helper = ThreadPoolExecutor(max_workers=1)
+        fut = self._helper.submit(elec)
+        try:
+            r = rest()
+        finally:
+            e = fut.result()
+        return e, r
+
+    def close(self):
+        if self._helper is not None:
+            self._helper.shutdown()
+            self._helper = None
 
     def forces(self):
         """Total forces of the last evaluation on the real atoms, (3*nw, 3), internal units."""

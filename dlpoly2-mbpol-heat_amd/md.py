@@ -86,12 +86,13 @@ class NVE:
         s, b, c = self.slot, self.slot.box, self.slot.ctx
         x, y, z = self._ptrs()
         f = self.f
-        e_mb, _, _ = c.mbpol_forces_dev(b.imcon, b.cell, s.lst.data_ptr(), x, y, z, f[0].data_ptr(), f[1].data_ptr(),
-                                        f[2].data_ptr())
-        e_el, _, _, self.iterations = c.polar_electrostatics_dev(b.imcon, b.cell, s.eparams, x, y, z,
-                                                                 self.dip.data_ptr(), self.fsite.data_ptr())
-        e_w, _, _ = c.water_terms_dev(b.imcon, b.cell, s.wparams, x, y, z, f[0].data_ptr(), f[1].data_ptr(),
-                                      f[2].data_ptr())
+        fp = (f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr())
+        # MB-pol and the water terms add into f; the electrostatics write fsite (own stream)
+        (e_el, _, _, self.iterations), ((e_mb, _, _), (e_w, _, _)) = s.split(
+            lambda: c.polar_electrostatics_dev(b.imcon, b.cell, s.eparams, x, y, z, self.dip.data_ptr(),
+                                               self.fsite.data_ptr()),
+            lambda: (c.mbpol_forces_dev(b.imcon, b.cell, s.lst.data_ptr(), x, y, z, *fp),
+                     c.water_terms_dev(b.imcon, b.cell, s.wparams, x, y, z, *fp)))
         return e_mb + e_el + float(np.sum(e_w))
 
     def _kinetic(self):

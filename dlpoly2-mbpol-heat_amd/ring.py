@@ -86,12 +86,12 @@ class RingPolymer:
         s, b, c = self.slots[k], self.slots[k].box, self.slots[k].ctx
         x, y, z = s.x.data_ptr(), s.y.data_ptr(), s.z.data_ptr()
         f = self.F[k]
-        e_mb, _, _ = c.mbpol_forces_dev(b.imcon, b.cell, s.lst.data_ptr(), x, y, z, f[0].data_ptr(),
-                                        f[1].data_ptr(), f[2].data_ptr())
-        e_el, _, _, its = c.polar_electrostatics_dev(b.imcon, b.cell, s.eparams, x, y, z, self.dip[k].data_ptr(),
-                                                     self.fsite[k].data_ptr())
-        e_w, _, _ = c.water_terms_dev(b.imcon, b.cell, s.wparams, x, y, z, f[0].data_ptr(), f[1].data_ptr(),
-                                      f[2].data_ptr())
+        fp = (f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr())
+        (e_el, _, _, its), ((e_mb, _, _), (e_w, _, _)) = s.split(
+            lambda: c.polar_electrostatics_dev(b.imcon, b.cell, s.eparams, x, y, z, self.dip[k].data_ptr(),
+                                               self.fsite[k].data_ptr()),
+            lambda: (c.mbpol_forces_dev(b.imcon, b.cell, s.lst.data_ptr(), x, y, z, *fp),
+                     c.water_terms_dev(b.imcon, b.cell, s.wparams, x, y, z, *fp)))
         return e_mb + e_el + float(np.sum(e_w)), its
 
     def forces(self):
@@ -151,3 +151,5 @@ class RingPolymer:
         if self._pool is not None:
             self._pool.shutdown()
             self._pool = None
+        for s in self.slots:
+            s.close()

@@ -4,6 +4,8 @@
   the oracle's MB-pol, electrostatics and water terms within 1e-10.
 * Several slots driven concurrently from host threads (their kernels overlap on the GPU):
   every result is bit-identical to the same slot evaluated alone.
+* The electrostatics on the context's second stream from a helper thread, beside MB-pol and
+  the water terms (Slot overlap), give bit-identical results to the three parts in order.
 * The FP32-polynomial mode stays within its measured error envelope against FP64.
 * Electrostatics neighbour rows that start too small (MBGPU_ELEC_ROWS): the evaluation
   grows them and repeats from the same start dipoles, bit-identical to a normal context.
@@ -69,6 +71,22 @@ def test_concurrent_slots_equal_alone(waterbox, engine):
             assert r["eng"] == r0["eng"] and r["vir"] == r0["vir"]
             assert np.array_equal(r["jtot"], r0["jtot"]) and np.array_equal(r["stress"], r0["stress"])
         assert np.array_equal(s.forces(), f0)
+
+
+def test_overlapped_parts_equal_ordered(waterbox, engine):
+    box = waterbox.make_waterbox(128, 15.65, 6, seed=7)
+    a, b = engine.Slot(box, overlap=False), engine.Slot(box, overlap=True)
+    print("ordered vs overlapped evaluation, 128 waters", flush=True)
+    r0 = a.evaluate()
+    f0 = a.forces()
+    for _ in range(3):
+        r = b.evaluate()
+        for key in ("eng", "vir", "dipole_iterations"):
+            assert r[key] == r0[key]
+        for key in ("stress", "jkin", "jpot", "jtot"):
+            assert np.array_equal(r[key], r0[key])
+        assert np.array_equal(b.forces(), f0)
+    b.close()
 
 
 def test_fp32_polynomials_error_envelope(waterbox, engine):
