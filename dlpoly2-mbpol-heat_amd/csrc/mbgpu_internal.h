@@ -232,3 +232,34 @@ int water_configure(WaterState& S, int nw, const CellGeom& g, const double* cell
 int water_evaluate(WaterState& S, const double* x, const double* y, const double* z, double* fx, double* fy,
                    double* fz, hipStream_t s, std::string& err);
 void water_release(WaterState& S);
+
+// ---- smooth particle-mesh Ewald, charges and dipoles (spme.hip) ----
+#define SPME_MAX_ORDER 12
+#define SPME_MAX_GRID 256
+struct SpmeGeom {
+    double rv[3][3];   // reciprocal vectors a*_a (rows), a*_a . a_b = delta_ab
+    double vol;
+    double alpha;
+    int K[3];
+    int p;
+};
+// one axis of the grid as lines for the transform passes (see k_sp_fft)
+struct SpmeAxis {
+    int K, se, ni, si, no, L;
+    long so;
+};
+struct SpmeBuffers {
+    unsigned long long* fixed = nullptr;   // spread density, 64-bit fixed point
+    double2* grid = nullptr;               // complex grid
+    double* bmod = nullptr;                // |b(m)|^2 of the three axes, concatenated
+    double2* tw = nullptr;                 // exp(-2 pi i t / K_a), t < K_a / 2, per axis
+    double* part = nullptr;                // energy block partials
+    double* out = nullptr;                 // energy
+    int* flag = nullptr;                   // input errors found on the device (2 ints)
+    long ng_cap = 0;
+    int cfg[4] = {0, 0, 0, 0};             // order and grid of bmod and tw
+    int twoff[3] = {0, 0, 0};
+    SpmeAxis axis[3] = {};
+};
+int spme_launch(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const double* y, const double* z,
+                const double* q, const double* mu, double* phi, double* ef, double* f, hipStream_t s);

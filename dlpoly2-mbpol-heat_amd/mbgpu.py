@@ -65,6 +65,9 @@ _SIGS = {
                                              ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double)]),
+    "mbgpu_spme_recip_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_double, ctypes.c_int] + [ctypes.c_void_p] * 8
+                             + [ctypes.POINTER(ctypes.c_double)]),
     "mbgpu_heat_acf_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
                                           ctypes.c_void_p]),
     "mbgpu_verlet_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 12
@@ -426,6 +429,18 @@ class Context:
                                                     float(force_scale), mass_ptr, x_ptr, xc_ptr, f_ptr,
                                                     ctypes.byref(es), ctypes.byref(ed)), "mbgpu_ring_spring_dev")
         return es.value, ed.value
+
+    def spme_recip_dev(self, cell, order, grid, alpha, n, x_ptr, y_ptr, z_ptr, q_ptr, mu_ptr=None, phi_ptr=None,
+                       efield_ptr=None, f_ptr=None):
+        """mbgpu_spme_recip_dev with device pointers; returns the reciprocal energy (Coulomb
+        constant 1).  mu, phi, E and F are [n][3] / [n] double arrays or None."""
+        c = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+        g = (ctypes.c_int * 3)(*[int(k) for k in grid])
+        u = ctypes.c_double(0.0)
+        _check(load_library().mbgpu_spme_recip_dev(self._h, _p(c), int(order), g, float(alpha), int(n), x_ptr, y_ptr,
+                                                   z_ptr, q_ptr, mu_ptr, phi_ptr, efield_ptr, f_ptr, ctypes.byref(u)),
+               "mbgpu_spme_recip_dev")
+        return u.value
 
     def dipoles(self):
         """Converged induced dipoles of the last electrostatics call, (4*nw, 3)."""

@@ -240,6 +240,24 @@ int mbgpu_ring_spring_dev(mbgpu_ctx* ctx, int nbead, int natom, double coeff, do
                           const double* mass_dev, const double* x_dev, const double* xc_dev, double* f_dev,
                           double* epot_spring, double* epot_deriv);
 
+/* ---- smooth particle-mesh Ewald reciprocal space ---------------------------------- */
+#define MBGPU_SPME_MAX_ORDER 12
+#define MBGPU_SPME_MAX_GRID 256
+/* Reciprocal-space Ewald energy, potential, field and force of point charges q[n] and point
+ * dipoles mu[3n] (NULL: none) by smooth particle-mesh Ewald: cardinal B-splines of even
+ * order 4..12 (bspgen.f:27-100), |b(m)|^2 moduli (bspcoe.f), spreading, 3-D FFT, influence
+ * function exp(-pi^2 m^2/alpha^2)/(pi V m^2) and gather as ewald_spme.f:1-362 and
+ * spme_for.f do for charges, here with the dipoles carried in the spread density, so that
+ * it converges to the reciprocal part of the induced-dipole Ewald sum (ewald1p.f).  grid:
+ * powers of two in [max(8, order), 256].  Coulomb constant 1 (the caller multiplies by
+ * r4pie0/epsq).  Outputs per site (device, each may be NULL): phi[n], the field E[3n] =
+ * -grad phi and the force F[3n] = q E + (mu . grad) E, each including the site's own image
+ * sum, as a reciprocal sum does; *energy = 1/2 sum (q phi - mu . E).  Deterministic: the
+ * spread density is summed in 64-bit fixed point (2^-44 e). */
+int mbgpu_spme_recip_dev(mbgpu_ctx* ctx, const double cell[9], int order, const int grid[3], double alpha, int n,
+                         const double* x_dev, const double* y_dev, const double* z_dev, const double* q_dev,
+                         const double* mu_dev, double* phi_dev, double* efield_dev, double* f_dev, double* energy);
+
 /* Converged induced dipoles of the last electrostatics call: 3 per site, interleaved, 4*nw
  * sites (the reference's dipx/y/z after iter; e*Angstrom). */
 int mbgpu_get_dipoles(mbgpu_ctx* ctx, double* dip);

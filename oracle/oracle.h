@@ -221,7 +221,21 @@ void ora_ring_transform(int P, int ncoord, const double *M, const double *in, do
 void ora_ring_spring(int P, int natom, double coeff, double fac3, double force_scale, const double *mass,
                      const double *x, const double *xc, double *f, double *epot_spring, double *epot_deriv);
 
+/* ---- smooth particle-mesh Ewald, charges and dipoles (spme_oracle.c) --------------- */
+/* M_p(w + t), M_p', M_p'' for t = 0..p-1 (bspgen.f:27-100); p even, 4..12 */
+void ora_bspline(int p, double w, double *a, double *d, double *d2);
+/* |b(m)|^2, m = 0..K-1 (bspcoe.f) */
+void ora_spme_bmod(int p, int K, double *bmod);
+/* SPME reciprocal energy, potential, field and force per site (Coulomb constant 1) */
+double ora_spme_recip(const double *cell, int p, const int *K, double alpha, int n, const double *x,
+                      const double *y, const double *z, const double *q, const double *mu, double *phi, double *E,
+                      double *F);
+/* the Ewald reciprocal sum over |m_a| <= kmax_a that SPME approximates */
+double ora_ewald_recip(const double *cell, const int *kmax, double alpha, int n, const double *x, const double *y,
+                       const double *z, const double *q, const double *mu, double *phi, double *E, double *F);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

@@ -401,3 +401,52 @@ def ring_spring(x, xc, f, mass, coeff, fac3=1.0, force_scale=1.0):
     lib.ora_ring_spring(nb, n, coeff, fac3, force_scale, P(np.ascontiguousarray(mass, float)), P(x),
                         P(np.ascontiguousarray(xc, float)), P(f), ctypes.byref(es), ctypes.byref(ed))
     return f, es.value, ed.value
+
+
+# ---------------------------------------------------------------- particle-mesh Ewald ----
+
+def spme_lib():
+    lib = oracle()
+    _ip3 = ctypes.POINTER(ctypes.c_int)
+    lib.ora_bspline.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+    lib.ora_spme_bmod.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+    lib.ora_spme_recip.restype = ctypes.c_double
+    lib.ora_spme_recip.argtypes = [_dp, ctypes.c_int, _ip3, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                   _dp, _dp, _dp]
+    lib.ora_ewald_recip.restype = ctypes.c_double
+    lib.ora_ewald_recip.argtypes = [_dp, _ip3, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                    _dp]
+    return lib
+
+
+def bspline(p, w):
+    """(M_p(w + t), M_p'(w + t), M_p''(w + t)) for t = 0..p-1."""
+    a, d, d2 = np.zeros(p), np.zeros(p), np.zeros(p)
+    spme_lib().ora_bspline(p, w, P(a), P(d), P(d2))
+    return a, d, d2
+
+
+def _recip_args(cell, xyz, q, mu):
+    cell = np.ascontiguousarray(cell, float).reshape(9)
+    xyz = np.asarray(xyz, float)
+    x, y, z = (np.ascontiguousarray(xyz[:, a]) for a in range(3))
+    q = np.ascontiguousarray(q, float)
+    mu = None if mu is None else np.ascontiguousarray(mu, float).reshape(-1)
+    n = len(q)
+    return cell, x, y, z, q, mu, n, np.zeros(n), np.zeros(3 * n), np.zeros(3 * n)
+
+
+def spme_recip(cell, order, grid, alpha, xyz, q, mu=None):
+    """SPME reciprocal space (Coulomb constant 1): dict(U, phi (n), E (n, 3), F (n, 3))."""
+    cell, x, y, z, q, mu, n, phi, E, F = _recip_args(cell, xyz, q, mu)
+    K = (ctypes.c_int * 3)(*grid)
+    U = spme_lib().ora_spme_recip(P(cell), order, K, alpha, n, P(x), P(y), P(z), P(q), P(mu), P(phi), P(E), P(F))
+    return dict(U=U, phi=phi, E=E.reshape(n, 3), F=F.reshape(n, 3))
+
+
+def ewald_recip(cell, kmax, alpha, xyz, q, mu=None):
+    """Ewald reciprocal sum over |m_a| <= kmax_a: dict(U, phi, E, F) as spme_recip."""
+    cell, x, y, z, q, mu, n, phi, E, F = _recip_args(cell, xyz, q, mu)
+    K = (ctypes.c_int * 3)(*kmax)
+    U = spme_lib().ora_ewald_recip(P(cell), K, alpha, n, P(x), P(y), P(z), P(q), P(mu), P(phi), P(E), P(F))
+    return dict(U=U, phi=phi, E=E.reshape(n, 3), F=F.reshape(n, 3))
