@@ -98,6 +98,48 @@ def cpu_baseline(box):
                       f"flux, {dt:.1f} s"}
 
 
+def spme_leg(pkg, torch, reps):
+    """Particle-mesh Ewald reciprocal space (charges and dipoles: energy, potential, field
+    and force) at 512 and 2048 waters, order 8 on a 64^3 grid, against the Ewald
+    structure-factor pass of the same box (k_el_sfac, measured with HIP events after one
+    complete electrostatics evaluation with the box's Ewald parameters)."""
+    import numpy as np
+    out = []
+    f64 = dict(dtype=torch.float64, device="cuda")
+    for nw, L, nlat in ((512, 24.84, 8), (2048, 39.43, 13)):
+        box = pkg.waterbox.make_waterbox(nw, L, nlat, seed=900 + nw)
+        ctx = pkg.Context(box.nw, box.natms)
+        n = box.natms
+        rng = np.random.default_rng(nw)
+        q = np.concatenate([np.tile([0.0, 0.6, 0.6], nw), -1.2 * np.ones(nw)]) + rng.normal(size=n) * 0.02
+        q -= q.mean()
+        mu = rng.normal(size=(n, 3)) * 0.05
+        x, y, z = (torch.as_tensor(np.ascontiguousarray(box.xyz[:, a]), **f64) for a in range(3))
+        qd, md = torch.as_tensor(q, **f64), torch.as_tensor(mu.reshape(-1), **f64)
+        phi, E, F = torch.zeros(n, **f64), torch.zeros(3 * n, **f64), torch.zeros(3 * n, **f64)
+        cell = np.diag([box.L] * 3)
+        torch.cuda.synchronize()
+        args = (cell, 8, (64, 64, 64), 0.3, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), qd.data_ptr(), md.data_ptr(),
+                phi.data_ptr(), E.data_ptr(), F.data_ptr())
+        for _ in range(3):
+            ctx.spme_recip_dev(*args)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.spme_recip_dev(*args)
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        ep = pkg.elec_params.for_box(box.L)
+        ctx.polar_electrostatics_dev(box.imcon, box.cell, pkg.ElecParams.from_params(ep, distribute_msite=1, solver=1),
+                                     x.data_ptr(), y.data_ptr(), z.data_ptr(), None)
+        ms_sf, _ = ctx.time_elec(max(5, reps // 2))
+        out.append({"waters": nw, "sites": n, "L": box.L, "order": 8, "grid": 64, "ms_per_call": ms,
+                    "ewald_kmax": list(ep.kmax), "ewald_k_vectors": int(ctx.elec_details()["k_vectors"]),
+                    "ewald_sfac_ms": ms_sf})
+    return {"sizes": out, "what": "mbgpu_spme_recip_dev per call (host-synchronous, spread + 3-D FFT + influence "
+                                  "+ inverse FFT + gather of phi, E and F) vs one launch of the Ewald structure-factor "
+                                  "pass k_el_sfac of the same box (the solver runs it and a field pass of about the "
+                                  "same time every iteration)"}
+
+
 def serve_leg(pkg, args):
     """Config-3 style serving mix through serve.Server; returns latency and throughput."""
     import numpy as np
@@ -151,6 +193,8 @@ def main():
                     help="ring-polymer steps (8 beads x 256 H2O, 0.2 fs) of the PIMD leg; 0 skips it")
     ap.add_argument("--serve-requests", type=int, default=800,
                     help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it)")
+    ap.add_argument("--spme-reps", type=int, default=20,
+                    help="calls per size of the particle-mesh Ewald leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--serve-mix", choices=["beads", "mixed"], default="beads",
                     help="beads: PIMD beads of 256 waters only (same-size slots); mixed: beads and isolated "
@@ -325,6 +369,7 @@ def main():
     serving = None
     if args.serve_requests > 0:
         serving = serve_leg(pkg, args)
+    spme_out = spme_leg(pkg, torch, args.spme_reps) if args.spme_reps > 0 else None
     # headline: the fastest leg whose forces are within 1e-5 kcal/mol/A RMS of FP64
     lead = ("fp64 single", single, ms, "f64")
     if "fp64" in batched and batched["fp64"]["evals_per_s"] > single:
@@ -342,6 +387,8 @@ def main():
         legs["nve"] = nve_out
     if pimd_out is not None:
         legs["pimd"] = pimd_out
+    if spme_out is not None:
+        legs["spme"] = spme_out
     if "mixed" in batched:
         m = batched["mixed"]
         legs["mixed_vs_fp64"] = {"rms_dF_kcal_mol_A": m["rms_dF"], "max_dF_kcal_mol_A": m["max_dF"],

@@ -420,7 +420,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
     dfree(c->ring_m); dfree(c->ring_blk); dfree(c->ring_out);
-    dfree(c->sp.fixed); dfree(c->sp.grid); dfree(c->sp.bmod); dfree(c->sp.tw); dfree(c->sp.part); dfree(c->sp.out);
+    dfree(c->sp.gi); dfree(c->sp.th); dfree(c->sp.cw); dfree(c->sp.grid); dfree(c->sp.bmod); dfree(c->sp.tw); dfree(c->sp.part); dfree(c->sp.out);
     dfree(c->sp.flag);
     elec_release(c->el);
     water_release(c->wt);
@@ -986,12 +986,20 @@ int mbgpu_spme_recip_dev(mbgpu_ctx* c, const double cell[9], int order, const in
     for (int a = 0; a < 3; ++a) G.K[a] = grid[a];
     SpmeBuffers& B = c->sp;
     const long ng = (long)grid[0] * grid[1] * grid[2];
+    if (B.n_cap < n) {
+        dfree(B.gi);
+        dfree(B.th);
+        dfree(B.cw);
+        B.n_cap = 0;
+        HIPCHK(dalloc(&B.gi, (size_t)n));
+        HIPCHK(dalloc(&B.th, (size_t)n * 9 * SPME_MAX_ORDER));
+        HIPCHK(dalloc(&B.cw, (size_t)n));
+        B.n_cap = n;
+    }
     if (B.ng_cap < ng) {
-        dfree(B.fixed);
         dfree(B.grid);
         dfree(B.part);
         B.ng_cap = 0;
-        HIPCHK(dalloc(&B.fixed, (size_t)ng));
         HIPCHK(dalloc(&B.grid, (size_t)ng));
         HIPCHK(dalloc(&B.part, (size_t)((ng + 255) / 256)));
         B.ng_cap = ng;

@@ -249,7 +249,10 @@ struct SpmeAxis {
     long so;
 };
 struct SpmeBuffers {
-    unsigned long long* fixed = nullptr;   // spread density, 64-bit fixed point
+    int4* gi = nullptr;                    // per site: grid index of t = 0 per axis, ok
+    double* th = nullptr;                  // per site and axis: M_p, M_p', M_p'' (SPME_MAX_ORDER each)
+    double4* cw = nullptr;                 // per site: q and the dipole weights K_a (a*_a . mu)
+    long n_cap = 0;
     double2* grid = nullptr;               // complex grid
     double* bmod = nullptr;                // |b(m)|^2 of the three axes, concatenated
     double2* tw = nullptr;                 // exp(-2 pi i t / K_a), t < K_a / 2, per axis

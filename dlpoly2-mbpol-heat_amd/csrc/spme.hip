@@ -8,12 +8,13 @@
 // the reciprocal part of the induced-dipole Ewald sum (ewald1p.f), to which it converges.
 //
 // Device steps (one stream, no host round trip):
-//   k_sp_zero       clear the fixed-point grid
-//   k_sp_spread     one thread per (site, t1, t2), p points along the third axis each;
-//                   contributions added as 64-bit fixed point (2^-44 e), so the grid does
-//                   not depend on the order the atomics land in
-//   k_sp_fft x3     radix-2 Stockham passes in LDS along each axis (the first pass reads
-//                   the fixed-point grid); L lines of one axis per workgroup of 128 threads
+//   k_sp_prep       per (site, axis): grid index, M_p, M_p', M_p'' and the dipole weight
+//   k_sp_tile       one workgroup per (plane, 16 rows) of the grid: the tile in LDS as
+//                   64-bit fixed point (2^-44 e), contributions of every site whose
+//                   support meets it added with LDS atomics (integer adds, so the grid
+//                   does not depend on their order), then written as complex values
+//   k_sp_fft x3     radix-2 Stockham passes in LDS along each axis; L lines of one axis
+//                   per workgroup of 128 threads
 //   k_sp_influence  Q^(m) *= B(m) C(m); U = 1/2 sum B C |Q^|^2 as block partials
 //   k_sp_fft x3     backward passes (unnormalised, conjugate twiddles)
 //   k_sp_gather     one wavefront per site: phi, grad phi and the Hessian of phi from the
@@ -89,65 +90,131 @@ __global__ __launch_bounds__(256) void k_sp_zero(unsigned long long* __restrict_
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < ng; i += (long)gridDim.x * 256) g[i] = 0ull;
 }
 
-// one block holds spb = 256 / p^2 sites; LDS: per (site, axis) the p splines and their
-// derivatives, the grid index of t = 0 and the weight K_a (a*_a . mu)
-__global__ __launch_bounds__(256) void k_sp_spread(SpmeGeom G, int n, const double* __restrict__ x,
-                                                   const double* __restrict__ y, const double* __restrict__ z,
-                                                   const double* __restrict__ q, const double* __restrict__ mu,
-                                                   unsigned long long* __restrict__ grid, int* __restrict__ flag)
+// Per (site, axis): the grid index of t = 0 (wrapped), M_p, M_p', M_p'' at the p points and
+// the dipole weight K_a (a*_a . mu); ok = 0 (and flag 1) for a position that cannot index
+// the grid.  Layout: gi[i] = (g1, g2, g3, ok); th[i][a][k][t] (k: M, M', M''); cw[i] =
+// (q, w1, w2, w3).
+__global__ __launch_bounds__(256) void k_sp_prep(SpmeGeom G, int n, const double* __restrict__ x,
+                                                 const double* __restrict__ y, const double* __restrict__ z,
+                                                 const double* __restrict__ q, const double* __restrict__ mu,
+                                                 int4* __restrict__ gi, double* __restrict__ th,
+                                                 double4* __restrict__ cw, int* __restrict__ flag)
 {
-    __shared__ double th[256 / 16][3][2][SPME_MAX_ORDER];
-    __shared__ double wv[256 / 16][3];
-    __shared__ int gs[256 / 16][3];
-    __shared__ int ok[256 / 16][3];
-    const int p = G.p, pp = p * p, spb = 256 / pp;
-    const int tid = threadIdx.x;
-    const int s0 = blockIdx.x * spb;
-    if (tid < 3 * spb) {
-        const int s = tid / 3, a = tid % 3, i = s0 + s;
+    __shared__ int okv[256];
+    __shared__ int gv[256];
+    __shared__ double wvv[256];
+    const int t = blockIdx.x * 255 + threadIdx.x;   // (site, axis) pairs, 255 = 3 * 85 per block
+    const int i = t / 3, a = t - 3 * (t / 3);
+    const int p = G.p;
+    const bool live = threadIdx.x < 255 && i < n;
+    int good = 0, g = 0;
+    double wa = 0.0;
+    if (live) {
         long g0 = 0;
         double w = 0.0;
-        const bool good = i < n && sp_frac(G, a, x[i], y[i], z[i], g0, w);
-        if (good) {
-            sp_splines<1>(p, w, th[s][a][0], th[s][a][1], nullptr);
-            gs[s][a] = sp_wrap(g0, G.K[a]);
-            wv[s][a] = mu ? G.K[a] * (G.rv[a][0] * mu[3 * i] + G.rv[a][1] * mu[3 * i + 1] + G.rv[a][2] * mu[3 * i + 2])
-                          : 0.0;
-        } else if (i < n) {
+        if (sp_frac(G, a, x[i], y[i], z[i], g0, w)) {
+            good = 1;
+            g = sp_wrap(g0, G.K[a]);
+            double* T = th + ((long)i * 3 + a) * 3 * SPME_MAX_ORDER;
+            sp_splines<2>(p, w, T, T + SPME_MAX_ORDER, T + 2 * SPME_MAX_ORDER);
+            if (mu) wa = G.K[a] * (G.rv[a][0] * mu[3 * i] + G.rv[a][1] * mu[3 * i + 1] + G.rv[a][2] * mu[3 * i + 2]);
+        } else {
             flag[0] = 1;
         }
-        ok[s][a] = good ? 1 : 0;
     }
+    okv[threadIdx.x] = good;
+    gv[threadIdx.x] = g;
+    wvv[threadIdx.x] = wa;
     __syncthreads();
-    if (tid >= spb * pp) return;
-    const int s = tid / pp, t1 = (tid / p) % p, t2 = tid % p, i = s0 + s;
-    if (i >= n || !(ok[s][0] & ok[s][1] & ok[s][2])) return;
-    const double qi = q[i];
-    const double a1 = th[s][0][0][t1], d1 = th[s][0][1][t1], a2 = th[s][1][0][t2], d2 = th[s][1][1][t2];
-    const double c12 = qi * a1 * a2 + wv[s][0] * d1 * a2 + wv[s][1] * a1 * d2;
-    const double w3 = wv[s][2] * a1 * a2;
-    const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2];
-    int g1 = gs[s][0] - t1, g2 = gs[s][1] - t2;
-    g1 += g1 < 0 ? K1 : 0;
-    g2 += g2 < 0 ? K2 : 0;
-    const long row = ((long)g1 * K2 + g2) * K3;
+    if (live && a == 0) {
+        const int b = threadIdx.x;
+        gi[i] = make_int4(gv[b], gv[b + 1], gv[b + 2], okv[b] & okv[b + 1] & okv[b + 2]);
+        cw[i] = make_double4(q[i], wvv[b], wvv[b + 1], wvv[b + 2]);
+    }
+}
+
+// Spread by tiles: a workgroup owns plane g1 and rows r0..r0+R-1 of the grid (all g3) in
+// LDS as 64-bit fixed point (2^-44 e), scans every site, lists those whose support meets
+// the tile and adds their contributions with LDS atomics (integer adds: the tile does not
+// depend on their order), then writes the tile as complex values.  Contribution at
+// (t1, t2, t3): q M1 M2 M3 + w1 M1' M2 M3 + w2 M1 M2' M3 + w3 M1 M2 M3'.
+constexpr int SP_ROWS = 16;
+__global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* __restrict__ gi,
+                                                 const double* __restrict__ th, const double4* __restrict__ cw,
+                                                 double2* __restrict__ grid, int* __restrict__ flag)
+{
+    __shared__ unsigned long long tile[SP_ROWS * SPME_MAX_GRID];
+    __shared__ int list[256], lt1[256];
+    __shared__ int cnt;
+    const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2], p = G.p;
+    const int R = min(SP_ROWS, K2), nrb = K2 / R;
+    const int g1 = blockIdx.x / nrb, r0 = (blockIdx.x - g1 * nrb) * R;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < R * K3; idx += 256) tile[idx] = 0ull;
     bool big = false;
-    for (int t3 = 0; t3 < p; ++t3) {
-        const double c = c12 * th[s][2][0][t3] + w3 * th[s][2][1][t3];
-        big |= !(fabs(c) < 1024.0);
-        int g3 = gs[s][2] - t3;
-        g3 += g3 < 0 ? K3 : 0;
-        atomicAdd(grid + row + g3, (unsigned long long)llrint(c * SP_FIX));
+    for (int base = 0; base < n; base += 256) {
+        if (tid == 0) cnt = 0;
+        __syncthreads();
+        const int i = base + tid;
+        if (i < n) {
+            const int4 g = gi[i];
+            int t1 = g.x - g1;
+            t1 += t1 < 0 ? K1 : 0;
+            if (g.w && t1 < p) {
+                int d = g.y - r0;
+                d += d < 0 ? K2 : 0;
+                bool hit = false;
+                for (int t2 = 0; t2 < p; ++t2) {
+                    int r = d - t2;
+                    r += r < 0 ? K2 : 0;
+                    hit |= r < R;
+                }
+                if (hit) {
+                    const int k = atomicAdd(&cnt, 1);
+                    list[k] = i;
+                    lt1[k] = t1;
+                }
+            }
+        }
+        __syncthreads();
+        const int m = cnt;
+        for (int wk = tid; wk < m * p; wk += 256) {
+            const int e = wk / p, t2 = wk - e * p;
+            const int i2 = list[e], t1 = lt1[e];
+            const int4 g = gi[i2];
+            int r = g.y - r0 - t2;
+            r += r < 0 ? K2 : 0;
+            r += r < 0 ? K2 : 0;
+            if (r >= R) continue;
+            const double* T = th + (long)i2 * 9 * SPME_MAX_ORDER;
+            const double4 c = cw[i2];
+            const double a1 = T[t1], d1 = T[SPME_MAX_ORDER + t1];
+            const double a2 = T[3 * SPME_MAX_ORDER + t2], d2 = T[4 * SPME_MAX_ORDER + t2];
+            const double c12 = c.x * a1 * a2 + c.y * d1 * a2 + c.z * a1 * d2;
+            const double w3 = c.w * a1 * a2;
+            const double* T3 = T + 6 * SPME_MAX_ORDER;
+            unsigned long long* row = tile + r * K3;
+            for (int t3 = 0; t3 < p; ++t3) {
+                const double v = c12 * T3[t3] + w3 * T3[SPME_MAX_ORDER + t3];
+                big |= !(fabs(v) < 1024.0);
+                int g3 = g.z - t3;
+                g3 += g3 < 0 ? K3 : 0;
+                atomicAdd(row + g3, (unsigned long long)llrint(v * SP_FIX));
+            }
+        }
+        __syncthreads();
     }
     if (big) flag[0] = 2;
+    for (int idx = tid; idx < R * K3; idx += 256) {
+        const int r = idx / K3, c3 = idx - r * K3;
+        grid[((long)g1 * K2 + r0 + r) * K3 + c3] = make_double2((double)(long long)tile[idx] * SP_UNFIX, 0.0);
+    }
 }
 
 // Radix-2 Stockham transform of L lines of length K along one axis (K a power of two).
 // Line l of block (o, ib) starts at o * so + (ib * L + l) * si, element e at + e * se.
-// FIX: the input is the fixed-point grid (first forward pass).
-template <bool FIX>
-__global__ __launch_bounds__(128) void k_sp_fft(double2* __restrict__ g, const unsigned long long* __restrict__ gi,
-                                                SpmeAxis A, const double2* __restrict__ tw, int inverse)
+__global__ __launch_bounds__(128) void k_sp_fft(double2* __restrict__ g, SpmeAxis A, const double2* __restrict__ tw,
+                                                int inverse)
 {
     __shared__ double2 buf[2][256];
     const int K = A.K, half = K >> 1, L = A.L;
@@ -164,15 +231,7 @@ __global__ __launch_bounds__(128) void k_sp_fft(double2* __restrict__ g, const u
             e = idx / L;
             l = idx - e * L;
         }
-        const long at = base + (long)l * A.si + (long)e * A.se;
-        double2 v;
-        if (FIX) {
-            v.x = (double)(long long)gi[at] * SP_UNFIX;
-            v.y = 0.0;
-        } else {
-            v = g[at];
-        }
-        buf[0][l * K + e] = v;
+        buf[0][l * K + e] = g[base + (long)l * A.si + (long)e * A.se];
     }
     __syncthreads();
     int src = 0;
@@ -251,49 +310,39 @@ __global__ __launch_bounds__(256) void k_sp_usum(int nb, const double* __restric
 }
 
 // one wavefront per site (4 per block): the lanes take the p^3 grid points in turn
-__global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const double* __restrict__ x,
-                                                   const double* __restrict__ y, const double* __restrict__ z,
-                                                   const double* __restrict__ q, const double* __restrict__ mu,
-                                                   const double2* __restrict__ g, double* __restrict__ phi,
-                                                   double* __restrict__ ef, double* __restrict__ f)
+__global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4* __restrict__ gi,
+                                                   const double* __restrict__ th, const double* __restrict__ q,
+                                                   const double* __restrict__ mu, const double2* __restrict__ g,
+                                                   double* __restrict__ phi, double* __restrict__ ef,
+                                                   double* __restrict__ f)
 {
-    __shared__ double th[4][3][3][SPME_MAX_ORDER];
-    __shared__ int gs[4][3];
-    __shared__ int ok[4][3];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + wv;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int4 gg = gi[i];
+    if (!gg.w) return;
     const int p = G.p;
-    if (lane < 3) {
-        long g0 = 0;
-        double w = 0.0;
-        const bool good = i < n && sp_frac(G, lane, x[i], y[i], z[i], g0, w);
-        if (good) {
-            sp_splines<2>(p, w, th[wv][lane][0], th[wv][lane][1], th[wv][lane][2]);
-            gs[wv][lane] = sp_wrap(g0, G.K[lane]);
-        }
-        ok[wv][lane] = good ? 1 : 0;
-    }
-    __syncthreads();
-    if (i >= n || !(ok[wv][0] & ok[wv][1] & ok[wv][2])) return;
     const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2];
+    const double* T = th + (long)i * 9 * SPME_MAX_ORDER;
+    constexpr int MO = SPME_MAX_ORDER;
     double s[10];
     for (int k = 0; k < 10; ++k) s[k] = 0.0;
     const int np3 = p * p * p;
     for (int fidx = lane; fidx < np3; fidx += 64) {
         const int t1 = fidx / (p * p), t2 = (fidx / p) % p, t3 = fidx % p;
-        int g1 = gs[wv][0] - t1, g2 = gs[wv][1] - t2, g3 = gs[wv][2] - t3;
+        int g1 = gg.x - t1, g2 = gg.y - t2, g3 = gg.z - t3;
         g1 += g1 < 0 ? K1 : 0;
         g2 += g2 < 0 ? K2 : 0;
         g3 += g3 < 0 ? K3 : 0;
         const double v = g[((long)g1 * K2 + g2) * K3 + g3].x;
-        const double A1 = th[wv][0][0][t1], D1 = th[wv][0][1][t1], E1 = th[wv][0][2][t1];
-        const double A2 = th[wv][1][0][t2], D2 = th[wv][1][1][t2], E2 = th[wv][1][2][t2];
-        const double A3 = th[wv][2][0][t3], D3 = th[wv][2][1][t3], E3 = th[wv][2][2][t3];
+        const double A1 = T[t1], D1 = T[MO + t1], E1 = T[2 * MO + t1];
+        const double A2 = T[3 * MO + t2], D2 = T[4 * MO + t2], E2 = T[5 * MO + t2];
+        const double A3 = T[6 * MO + t3], D3 = T[7 * MO + t3], E3 = T[8 * MO + t3];
         s[0] += v * A1 * A2 * A3;
         s[1] += v * D1 * A2 * A3;
         s[2] += v * A1 * D2 * A3;
         s[3] += v * A1 * A2 * D3;
-        s[4] += v * E1 * A2 * A3;   // uu 00
+        s[4] += v * E1 * A2 * A3;   // second derivatives in u: 00
         s[5] += v * A1 * E2 * A3;   // 11
         s[6] += v * A1 * A2 * E3;   // 22
         s[7] += v * D1 * D2 * A3;   // 01
@@ -332,16 +381,12 @@ __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const doub
 
 void sp_fft3(SpmeBuffers& B, const SpmeGeom& G, bool inverse, hipStream_t s)
 {
-    // axes in the order 2, 1, 0 forward (the first reads the fixed-point grid), 0, 1, 2 back
+    // axes in the order 2, 1, 0 forward, 0, 1, 2 back
     for (int k = 0; k < 3; ++k) {
         const int a = inverse ? k : 2 - k;
         const SpmeAxis& A = B.axis[a];
         const unsigned nblk = (unsigned)((long)A.no * (A.ni / A.L));
-        const double2* tw = B.tw + B.twoff[a];
-        if (!inverse && k == 0)
-            hipLaunchKernelGGL(k_sp_fft<true>, dim3(nblk), dim3(128), 0, s, B.grid, B.fixed, A, tw, 0);
-        else
-            hipLaunchKernelGGL(k_sp_fft<false>, dim3(nblk), dim3(128), 0, s, B.grid, B.fixed, A, tw, inverse ? 1 : 0);
+        hipLaunchKernelGGL(k_sp_fft, dim3(nblk), dim3(128), 0, s, B.grid, A, B.tw + B.twoff[a], inverse ? 1 : 0);
     }
 }
 
@@ -351,19 +396,19 @@ int spme_launch(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const
                 const double* q, const double* mu, double* phi, double* ef, double* f, hipStream_t s)
 {
     const long ng = (long)G.K[0] * G.K[1] * G.K[2];
-    const int zb = (int)std::min<long>(2048, (ng + 255) / 256);
-    hipLaunchKernelGGL(k_sp_zero, dim3(zb), dim3(256), 0, s, B.fixed, ng);
     hipLaunchKernelGGL(k_sp_zero, dim3(1), dim3(256), 0, s, reinterpret_cast<unsigned long long*>(B.flag), 1L);
-    const int spb = 256 / (G.p * G.p);
     if (n > 0)
-        hipLaunchKernelGGL(k_sp_spread, dim3((n + spb - 1) / spb), dim3(256), 0, s, G, n, x, y, z, q, mu, B.fixed,
-                           B.flag);
+        hipLaunchKernelGGL(k_sp_prep, dim3((unsigned)((3L * n + 254) / 255)), dim3(256), 0, s, G, n, x, y, z, q, mu,
+                           B.gi, B.th, B.cw, B.flag);
+    const int R = std::min(SP_ROWS, G.K[1]);
+    hipLaunchKernelGGL(k_sp_tile, dim3((unsigned)(G.K[0] * (G.K[1] / R))), dim3(256), 0, s, G, n, B.gi, B.th, B.cw,
+                       B.grid, B.flag);
     sp_fft3(B, G, false, s);
     const int nb = (int)((ng + 255) / 256);
     hipLaunchKernelGGL(k_sp_influence, dim3(nb), dim3(256), 0, s, G, B.grid, B.bmod, B.part);
     hipLaunchKernelGGL(k_sp_usum, dim3(1), dim3(256), 0, s, nb, B.part, B.out);
     sp_fft3(B, G, true, s);
     if (n > 0 && (phi || ef || f))
-        hipLaunchKernelGGL(k_sp_gather, dim3((n + 3) / 4), dim3(256), 0, s, G, n, x, y, z, q, mu, B.grid, phi, ef, f);
+        hipLaunchKernelGGL(k_sp_gather, dim3((n + 3) / 4), dim3(256), 0, s, G, n, B.gi, B.th, q, mu, B.grid, phi, ef, f);
     return (int)hipGetLastError();
 }
