@@ -193,6 +193,10 @@ def main():
                     help="ring-polymer steps (8 beads x 256 H2O, 0.2 fs) of the PIMD leg; 0 skips it")
     ap.add_argument("--serve-requests", type=int, default=800,
                     help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for the run (0: one per concurrent configuration, at most 16)")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="evaluate the electrostatics after MB-pol and the water terms instead of beside them")
     ap.add_argument("--spme-reps", type=int, default=20,
                     help="calls per size of the particle-mesh Ewald leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=400.0)
@@ -221,7 +225,7 @@ def main():
     if args.batch > 4:
         # one hardware queue per concurrently evaluated configuration (HIP's default is 4);
         # must be set before the HIP runtime starts
-        os.environ["GPU_MAX_HW_QUEUES"] = str(min(16, args.batch))
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues or min(16, args.batch))
     import numpy as np
     import torch
 
@@ -290,6 +294,7 @@ def main():
     n_sfac = (int(np.median(iters)) + 2) if iters else None
     single = args.steps / dt
 
+
     # batched legs: B independent 512-water configurations (different seeds), one context,
     # stream and host thread each, evaluated concurrently; FP64, then FP32 polynomials
     batched = {}
@@ -298,7 +303,7 @@ def main():
         solver = 1 if args.solver == "cg" else 0
         boxes = [pkg.waterbox.make_waterbox(512, 24.84, 8, 100 + i) for i in range(args.batch)]
         for name, prec in (("fp64", 0), ("mixed", 3)):
-            slots = [engine.Slot(b, precision=prec, solver=solver) for b in boxes]
+            slots = [engine.Slot(b, precision=prec, solver=solver, overlap=not args.no_overlap) for b in boxes]
             tb, res = engine.run_concurrently(slots, args.steps, warmup=args.warmup)
             batched[name] = dict(evals_per_s=args.batch * args.steps / tb, ms_per_step=1e3 * tb / args.steps,
                                  forces=slots[0].forces(), result=res[0][-1], nw=boxes[0].nw)
