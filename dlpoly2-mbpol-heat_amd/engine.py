@@ -49,13 +49,17 @@ class Slot:
         self.wparams = WaterParams.from_params(water)
 
     def _alloc(self, box):
+        # allocated and cleared on the slot's stream, where load() then writes them: a
+        # clear on another stream would not be ordered before the upload (torch's streams
+        # do not wait for the default stream)
         t = self.torch
         n, nr = box.natms, box.nreal
         f64 = dict(dtype=t.float64, device=self.dev)
-        self.x, self.y, self.z = (t.zeros(n, **f64) for _ in range(3))
-        self.lst = t.zeros(nr, dtype=t.int32, device=self.dev)
-        self.v = t.zeros(3, nr, **f64)
-        self.ek = t.zeros(nr, **f64)
+        with t.cuda.stream(self.stream):
+            self.x, self.y, self.z = (t.zeros(n, **f64) for _ in range(3))
+            self.lst = t.zeros(nr, dtype=t.int32, device=self.dev)
+            self.v = t.zeros(3, nr, **f64)
+            self.ek = t.zeros(nr, **f64)
 
     def load(self, box):
         """Upload a configuration of the same size (positions, list, velocities)."""
