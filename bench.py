@@ -49,7 +49,7 @@ PEAK_FP64 = 78.6        # TFLOP/s, MI355X FP64 (vector and matrix peaks are equa
 # HBM bytes per launch of k_el_sfac<false> at this workload, from rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE (KiB per dispatch) scaled by the same counters on a plain 512 MiB copy (FETCH x 2,
 # WRITE x 1); profiles/k_el_sfac_bench512_traffic.md
-SFAC_TRAFFIC_BYTES = (2921.5 * 2.0 + 20390.0 * 1.0) * 1024
+SFAC_TRAFFIC_BYTES = (2903.7 * 2.0 + 20376.6 * 1.0) * 1024
 
 
 def cpu_baseline(box):
