@@ -7,8 +7,8 @@ oracle's SPME and the Ewald reciprocal sum it approximates.
 * The 512-water geometry with DMS-sized charges and induced-dipole-sized dipoles, order 8
   on 64^3, against the Ewald sum at kmax 16: the SPME discretisation error, 1e-5.
 * Repeat calls bit-identical; positions shifted by lattice vectors give the same result
-  (1e-11); U = 1/2 sum (q phi - mu . E) at 8192 sites; charges alone; argument and input
-  errors reported.
+  (1e-11); U = 1/2 sum (q phi - mu . E) at 8192 sites; charges alone; no sites and one
+  site; argument and input errors reported.
 """
 import numpy as np
 import pytest
@@ -130,3 +130,10 @@ def test_argument_and_input_errors(pkg, dev):
     ok = dev.run(cell, 8, (16, 16, 16), 0.35, xyz, q, mu)
     r = o.spme_recip(cell, 8, (16, 16, 16), 0.35, xyz, q, mu)
     assert abs(ok["U"] - r["U"]) <= 1e-10 * abs(r["U"])
+    # no sites: zero energy; one site: the oracle's value
+    assert dev.ctx.spme_recip_dev(cell, 8, (16, 16, 16), 0.35, 0, None, None, None, None) == 0.0
+    one = dev.run(cell, 6, (16, 32, 16), 0.35, xyz[:1], [0.7], mu[:1])
+    r1 = o.spme_recip(cell, 6, (16, 32, 16), 0.35, xyz[:1], [0.7], mu[:1])
+    assert abs(one["U"] - r1["U"]) <= 1e-10 * abs(r1["U"])
+    # a lone site's reciprocal force is the mesh's small self-force: compare absolutely
+    assert np.abs(one["F"] - r1["F"]).max() <= 1e-10 * max(1.0, np.abs(r1["E"]).max())
