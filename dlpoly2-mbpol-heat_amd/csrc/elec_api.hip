@@ -24,6 +24,9 @@
 #ifndef EL_KCHUNK_WAVES
 #define EL_KCHUNK_WAVES 4096  // site-block x k-chunk product the reciprocal field passes aim at
 #endif
+#ifndef EL_KCHUNK_RUNS
+#define EL_KCHUNK_RUNS 32     // at most this many k runs per chunk of the site-parallel passes
+#endif
 #ifndef EL_DCHUNK_WAVES
 #define EL_DCHUNK_WAVES 2048  // wavefronts the dipole structure-factor pass aims at
 #endif
@@ -286,8 +289,11 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     const int rblocks = (EL_SPLIT * E.nrun + 255) / 256;
     const int nrow = 3 * S.nw;
     E.schunks = std::max(1, std::min(16, std::min((E.n + 15) / 16, 1024 / (4 * rblocks))));
-    E.spc = (E.n + E.schunks - 1) / E.schunks;
     E.dchunks = std::max(1, std::min(EL_DCHUNK_MAX, std::min((nrow + 15) / 16, EL_DCHUNK_WAVES / (4 * rblocks))));
+    // MBGPU_ELEC_WAVE_CHUNKS=1 keeps the wavefront rule alone for the k chunks below (a
+    // test aid that compares chunkings)
+    const bool wave_rules_only = getenv("MBGPU_ELEC_WAVE_CHUNKS") != nullptr;
+    E.spc = (E.n + E.schunks - 1) / E.schunks;
     E.dspc = (nrow + E.dchunks - 1) / E.dchunks;
     ECHK(dalloc(&E.S, (size_t)8 * K));
     dfree(E.cg_smu);
@@ -309,6 +315,10 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     }
     // k split of the site-parallel passes: about 2048 wavefronts over the sites
     E.kchunks = std::max(1, std::min(E.nrun, EL_KCHUNK_WAVES / std::max(1, (E.n + 63) / 64)));
+    // and at most EL_KCHUNK_RUNS runs per chunk: in large boxes the rule above leaves a few
+    // hundred workgroups with long k loops each
+    if (!wave_rules_only)
+        E.kchunks = std::max(E.kchunks, std::min(E.nrun, (E.nrun + EL_KCHUNK_RUNS - 1) / EL_KCHUNK_RUNS));
     E.rpc = (E.nrun + E.kchunks - 1) / E.kchunks;
     E.kchunks = (E.nrun + E.rpc - 1) / E.rpc;
     E.kmaxch = 0;

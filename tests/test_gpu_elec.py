@@ -171,3 +171,22 @@ def test_conjugate_gradient_solver_matches_sor_oracle(pkg, waterbox, ep, which, 
         assert g["niter"] <= 16
     g["niter"] = r["niter"]
     compare(g, r)
+
+
+def test_k_chunking_of_large_box(pkg, waterbox, ep, monkeypatch):
+    """1024 waters (kmax 24): the site-parallel reciprocal passes split k into chunks of at
+    most 32 runs; with that bound off (MBGPU_ELEC_WAVE_CHUNKS) the chunks follow the
+    wavefront target alone.  Only the order of the chunk sums differs:
+    every result within 1e-10, the iteration count within one."""
+    box = waterbox.make_waterbox(1024, 31.30, 11, seed=3)
+    p = ep.for_box(box.L)
+    _, a = run_gpu(pkg, box, p)
+    monkeypatch.setenv("MBGPU_ELEC_WAVE_CHUNKS", "1")
+    _, b = run_gpu(pkg, box, p)
+    monkeypatch.delenv("MBGPU_ELEC_WAVE_CHUNKS")
+    print(f"1024 waters: {a['niter']} and {b['niter']} iterations, energy {a['eng']:.15e} {b['eng']:.15e}",
+          flush=True)
+    assert abs(a["niter"] - b["niter"]) <= 1
+    assert abs(a["eng"] - b["eng"]) <= 1e-10 * abs(b["eng_ew1p"])
+    for k in ("dip", "f", "vesp", "atomic_energy", "s_ew1p", "heat_w"):
+        assert rel(a[k], b[k]) <= 1e-10, k
