@@ -1056,7 +1056,7 @@ int mbgpu_spme_recip_dev(mbgpu_ctx* c, const double cell[9], int order, const in
     B.axis[2] = SpmeAxis{K3, 1, K1 * K2, K3, 1, 0, 0L};
     B.axis[1] = SpmeAxis{K2, K3, K3, 1, K1, 0, (long)K2 * K3};
     B.axis[0] = SpmeAxis{K1, K2 * K3, K2 * K3, 1, 1, 0, 0L};
-    for (int a = 0; a < 3; ++a) B.axis[a].L = std::min(256 / B.axis[a].K, B.axis[a].ni);
+    for (int a = 0; a < 3; ++a) B.axis[a].L = std::max(1, std::min(SP_FFT_ELEMS / B.axis[a].K, B.axis[a].ni));
     int rc = spme_launch(B, G, n, x_dev, y_dev, z_dev, q_dev, mu_dev, phi_dev, efield_dev, f_dev, c->el_stream);
     if (rc) return fail(rc, "particle-mesh kernels failed");
     double u = 0.0;

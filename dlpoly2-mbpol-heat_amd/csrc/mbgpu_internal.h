@@ -236,6 +236,8 @@ void water_release(WaterState& S);
 // ---- smooth particle-mesh Ewald, charges and dipoles (spme.hip) ----
 #define SPME_MAX_ORDER 12
 #define SPME_MAX_GRID 256
+#define SP_FFT_THREADS 128     // threads of a transform workgroup
+#define SP_FFT_ELEMS 256       // L lines x K elements staged per transform workgroup
 struct SpmeGeom {
     double rv[3][3];   // reciprocal vectors a*_a (rows), a*_a . a_b = delta_ab
     double vol;

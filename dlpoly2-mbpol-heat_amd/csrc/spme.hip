@@ -14,7 +14,7 @@
 //                   support meets it added with LDS atomics (integer adds, so the grid
 //                   does not depend on their order), then written as complex values
 //   k_sp_fft x3     radix-2 Stockham passes in LDS along each axis; L lines of one axis
-//                   per workgroup of 128 threads
+//                   per workgroup of 128 threads (L K = 256 elements when the axis allows)
 //   k_sp_influence  Q^(m) *= B(m) C(m); U = 1/2 sum B C |Q^|^2 as block partials
 //   k_sp_fft x3     backward passes (unnormalised, conjugate twiddles)
 //   k_sp_gather     one wavefront per site: phi, grad phi and the Hessian of phi from the
@@ -213,16 +213,16 @@ __global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* 
 
 // Radix-2 Stockham transform of L lines of length K along one axis (K a power of two).
 // Line l of block (o, ib) starts at o * so + (ib * L + l) * si, element e at + e * se.
-__global__ __launch_bounds__(128) void k_sp_fft(double2* __restrict__ g, SpmeAxis A, const double2* __restrict__ tw,
+__global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_fft(double2* __restrict__ g, SpmeAxis A, const double2* __restrict__ tw,
                                                 int inverse)
 {
-    __shared__ double2 buf[2][256];
+    __shared__ double2 buf[2][SP_FFT_ELEMS > SPME_MAX_GRID ? SP_FFT_ELEMS : SPME_MAX_GRID];
     const int K = A.K, half = K >> 1, L = A.L;
     const int nib = A.ni / L;
     const int o = blockIdx.x / nib, ib = blockIdx.x - o * nib;
     const long base = (long)o * A.so + (long)ib * L * A.si;
     const int LK = L * K;
-    for (int idx = threadIdx.x; idx < LK; idx += 128) {
+    for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
         int l, e;
         if (A.se == 1) {
             l = idx / K;
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(128) void k_sp_fft(double2* __restrict__ g, SpmeAxi
     __syncthreads();
     int src = 0;
     for (int Ns = 1; Ns < K; Ns <<= 1) {
-        for (int b = threadIdx.x; b < L * half; b += 128) {
+        for (int b = threadIdx.x; b < L * half; b += SP_FFT_THREADS) {
             const int l = b / half, j = b - l * half;
             const double2* in = buf[src] + l * K;
             double2* out = buf[src ^ 1] + l * K;
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(128) void k_sp_fft(double2* __restrict__ g, SpmeAxi
         __syncthreads();
         src ^= 1;
     }
-    for (int idx = threadIdx.x; idx < LK; idx += 128) {
+    for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
         int l, e;
         if (A.se == 1) {
             l = idx / K;
@@ -386,7 +386,7 @@ void sp_fft3(SpmeBuffers& B, const SpmeGeom& G, bool inverse, hipStream_t s)
         const int a = inverse ? k : 2 - k;
         const SpmeAxis& A = B.axis[a];
         const unsigned nblk = (unsigned)((long)A.no * (A.ni / A.L));
-        hipLaunchKernelGGL(k_sp_fft, dim3(nblk), dim3(128), 0, s, B.grid, A, B.tw + B.twoff[a], inverse ? 1 : 0);
+        hipLaunchKernelGGL(k_sp_fft, dim3(nblk), dim3(SP_FFT_THREADS), 0, s, B.grid, A, B.tw + B.twoff[a], inverse ? 1 : 0);
     }
 }
 
