@@ -127,15 +127,26 @@ def spme_leg(pkg, torch, reps):
         for _ in range(reps):
             ctx.spme_recip_dev(*args)
         ms = 1e3 * (time.perf_counter() - t0) / reps
+        # the field of the dipoles alone, as a solver iteration needs it
+        q0 = torch.zeros_like(qd)
+        fargs = (cell, 8, (64, 64, 64), 0.3, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), q0.data_ptr(), md.data_ptr(),
+                 None, E.data_ptr(), None)
+        torch.cuda.synchronize()
+        ctx.spme_recip_dev(*fargs)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.spme_recip_dev(*fargs)
+        ms_field = 1e3 * (time.perf_counter() - t0) / reps
         ep = pkg.elec_params.for_box(box.L)
         ctx.polar_electrostatics_dev(box.imcon, box.cell, pkg.ElecParams.from_params(ep, distribute_msite=1, solver=1),
                                      x.data_ptr(), y.data_ptr(), z.data_ptr(), None)
         ms_sf, _ = ctx.time_elec(max(5, reps // 2))
         out.append({"waters": nw, "sites": n, "L": box.L, "order": 8, "grid": 64, "ms_per_call": ms,
+                    "ms_per_call_dipole_field": ms_field,
                     "ewald_kmax": list(ep.kmax), "ewald_k_vectors": int(ctx.elec_details()["k_vectors"]),
                     "ewald_sfac_ms": ms_sf})
     return {"sizes": out, "what": "mbgpu_spme_recip_dev per call (host-synchronous, spread + 3-D FFT + influence "
-                                  "+ inverse FFT + gather of phi, E and F) vs one launch of the Ewald structure-factor "
+                                  "+ inverse FFT + gather of phi, E and F; and of the dipoles' field alone) vs one launch of the Ewald structure-factor "
                                   "pass k_el_sfac of the same box (the solver runs it and a field pass of about the "
                                   "same time every iteration)"}
 

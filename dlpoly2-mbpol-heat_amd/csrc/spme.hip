@@ -17,8 +17,8 @@
 //                   per workgroup of 128 threads (L K = 256 elements when the axis allows)
 //   k_sp_influence  Q^(m) *= B(m) C(m); U = 1/2 sum B C |Q^|^2 as block partials
 //   k_sp_fft x3     backward passes (unnormalised, conjugate twiddles)
-//   k_sp_gather     one wavefront per site: phi, grad phi and the Hessian of phi from the
-//                   p^3 grid points, reduced over the lanes in a fixed order
+//   k_sp_gather     one wavefront per site: phi, grad phi and (for forces) the Hessian of
+//                   phi from the p^3 grid points, reduced over the lanes in a fixed order
 //   k_sp_usum       the energy partials in a fixed order
 // Coulomb constant 1 (energy e^2/A); the caller scales.
 #include "mbgpu_internal.h"
@@ -309,7 +309,9 @@ __global__ __launch_bounds__(256) void k_sp_usum(int nb, const double* __restric
     if (threadIdx.x == 0) out[0] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// one wavefront per site (4 per block): the lanes take the p^3 grid points in turn
+// one wavefront per site (4 per block): the lanes take the p^3 grid points in turn; HESS:
+// the second derivatives too (forces asked for)
+template <bool HESS>
 __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4* __restrict__ gi,
                                                    const double* __restrict__ th, const double* __restrict__ q,
                                                    const double* __restrict__ mu, const double2* __restrict__ g,
@@ -342,14 +344,16 @@ __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4
         s[1] += v * D1 * A2 * A3;
         s[2] += v * A1 * D2 * A3;
         s[3] += v * A1 * A2 * D3;
-        s[4] += v * E1 * A2 * A3;   // second derivatives in u: 00
-        s[5] += v * A1 * E2 * A3;   // 11
-        s[6] += v * A1 * A2 * E3;   // 22
-        s[7] += v * D1 * D2 * A3;   // 01
-        s[8] += v * D1 * A2 * D3;   // 02
-        s[9] += v * A1 * D2 * D3;   // 12
+        if (HESS) {
+            s[4] += v * E1 * A2 * A3;   // second derivatives in u: 00
+            s[5] += v * A1 * E2 * A3;   // 11
+            s[6] += v * A1 * A2 * E3;   // 22
+            s[7] += v * D1 * D2 * A3;   // 01
+            s[8] += v * D1 * A2 * D3;   // 02
+            s[9] += v * A1 * D2 * D3;   // 12
+        }
     }
-    for (int k = 0; k < 10; ++k) s[k] = sp_wsum(s[k]);
+    for (int k = 0; k < (HESS ? 10 : 4); ++k) s[k] = sp_wsum(s[k]);
     if (lane != 0) return;
     // d/dr_d = sum_a K_a a*_a[d] d/du_a
     double ga[3][3];
@@ -361,7 +365,7 @@ __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4
     if (phi) phi[i] = s[0];
     if (ef)
         for (int d = 0; d < 3; ++d) ef[3 * i + d] = -gr[d];
-    if (f) {
+    if (HESS && f) {
         const double qi = q[i];
         double m[3] = {0.0, 0.0, 0.0};
         if (mu)
@@ -408,7 +412,11 @@ int spme_launch(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const
     hipLaunchKernelGGL(k_sp_influence, dim3(nb), dim3(256), 0, s, G, B.grid, B.bmod, B.part);
     hipLaunchKernelGGL(k_sp_usum, dim3(1), dim3(256), 0, s, nb, B.part, B.out);
     sp_fft3(B, G, true, s);
-    if (n > 0 && (phi || ef || f))
-        hipLaunchKernelGGL(k_sp_gather, dim3((n + 3) / 4), dim3(256), 0, s, G, n, B.gi, B.th, q, mu, B.grid, phi, ef, f);
+    if (n > 0 && f)
+        hipLaunchKernelGGL(k_sp_gather<true>, dim3((n + 3) / 4), dim3(256), 0, s, G, n, B.gi, B.th, q, mu, B.grid, phi,
+                           ef, f);
+    else if (n > 0 && (phi || ef))
+        hipLaunchKernelGGL(k_sp_gather<false>, dim3((n + 3) / 4), dim3(256), 0, s, G, n, B.gi, B.th, q, mu, B.grid, phi,
+                           ef, f);
     return (int)hipGetLastError();
 }

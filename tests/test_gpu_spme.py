@@ -46,8 +46,9 @@ class Dev:
         phi, E, F = t.zeros(n, **f64), t.zeros(3 * n, **f64), t.zeros(3 * n, **f64)
         t.cuda.synchronize()
         ptr = lambda a: None if a is None else a.data_ptr()
+        outs = (phi, E, F) if outputs is True else (phi, E, None) if outputs == "field" else (None,) * 3
         U = self.ctx.spme_recip_dev(cell, order, grid, alpha, n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
-                                    qd.data_ptr(), ptr(md), *(map(ptr, (phi, E, F)) if outputs else (None,) * 3))
+                                    qd.data_ptr(), ptr(md), *map(ptr, outs))
         return dict(U=U, phi=phi.cpu().numpy(), E=E.cpu().numpy().reshape(n, 3), F=F.cpu().numpy().reshape(n, 3))
 
 
@@ -110,6 +111,10 @@ def test_repeat_shift_identity_and_charges_only(dev):
     assert abs(c["U"] - 0.5 * q @ c["phi"]) <= 1e-10 * abs(c["U"])
     e = dev.run(cell, 8, (64, 64, 64), 0.3, xyz, q, mu, outputs=False)
     assert e["U"] == a["U"]
+    # potential and field without forces (no second derivatives gathered): the same values
+    fo = dev.run(cell, 8, (64, 64, 64), 0.3, xyz, q, mu, outputs="field")
+    assert fo["U"] == a["U"] and np.array_equal(fo["phi"], a["phi"]) and np.array_equal(fo["E"], a["E"])
+    assert not fo["F"].any()
 
 
 def test_argument_and_input_errors(pkg, dev):
