@@ -159,16 +159,24 @@ def serve_leg(pkg, args):
     rng = np.random.default_rng(9)
     base = wb.box_256()
     beads = [wb.displaced(base, 0.05, seed=500 + i) for i in range(32)]
+    sizes, clusters = [], []
     if args.serve_mix == "mixed":
-        raise SystemExit("the mixed-size serving leg is switched off: one run of it ended in an illegal memory "
-                         "access whose cause has not been found")
+        sizes = [int(n) for n in np.exp(rng.uniform(np.log(3), np.log(1024), 24))]
+        clusters = [wb.make_cluster(n, seed=600 + i) for i, n in enumerate(sizes)]
     jobs = []
     for i in range(args.serve_requests):
-        b = beads[i % len(beads)]
-        jobs.append((b, ep.for_box(b.L), wp.for_box(b.L), "bead"))
+        if clusters and rng.random() < 0.5:
+            c = clusters[int(rng.integers(len(clusters)))]
+            jobs.append((c, ep.for_cluster(c.L), wp.for_cluster(c.L), "cluster"))
+        else:
+            b = beads[i % len(beads)]
+            jobs.append((b, ep.for_box(b.L), wp.for_box(b.L), "bead"))
     srv = serve.Server(workers=max(1, args.batch), solver=1 if args.solver == "cg" else 0)
-    # warm the pool, untimed: one request per worker at once, so every worker holds a slot
+    # warm the pool, untimed: one request per worker at once, so every worker holds a slot,
+    # then one request of every cluster size (slots of new sizes are still created and
+    # evicted during the timed run, as other workers use the same size at once)
     warm = [srv.submit(*j) for j in jobs[:max(1, args.batch)]]
+    warm += [srv.submit(c, ep.for_cluster(c.L), wp.for_cluster(c.L), "cluster") for c in clusters]
     for r in warm:
         r.done.wait()
     reqs, wall = serve.run_poisson(srv, jobs, args.serve_rate, seed=3)
@@ -181,10 +189,13 @@ def serve_leg(pkg, args):
             out[f"{tag}_p50_ms"], out[f"{tag}_p99_ms"] = s["p50_ms"], s["p99_ms"]
     out.update(arrival_rate=args.serve_rate, wall_s=wall, evals_per_s=len(reqs) / wall, workers=max(1, args.batch),
                slots_created=srv.slots_created, slots_evicted=srv.slots_evicted,
-               mix=args.serve_mix,
-               what="requests drawn from 32 PIMD beads x 256 H2O (L=19.72 A, N(0,0.05 A) displacements), "
-                    "Poisson arrivals, continuous batching over the workers' slots; latency = arrival to forces "
-                    "on the host")
+               mix=args.serve_mix, cluster_sizes=sizes,
+               what="requests drawn from 32 PIMD beads x 256 H2O (L=19.72 A, N(0,0.05 A) displacements)"
+                    + (" and, half of them, isolated clusters of n in [3,1024] waters, n log-uniform (random "
+                       "sequential addition at 0.997 g/cm^3, O-O >= 2.6 A, in a periodic box of side 2R+12 A)"
+                       if clusters else "")
+                    + ", Poisson arrivals, continuous batching over the workers' slots; latency = arrival to "
+                      "forces on the host")
     return out
 
 
@@ -202,7 +213,7 @@ def main():
                     help="steps of the NVE leg (512 waters, dt 0.2 fs, heat current every step; 0 skips it)")
     ap.add_argument("--pimd-steps", type=int, default=20,
                     help="ring-polymer steps (8 beads x 256 H2O, 0.2 fs) of the PIMD leg; 0 skips it")
-    ap.add_argument("--serve-requests", type=int, default=800,
+    ap.add_argument("--serve-requests", type=int, default=10000,
                     help="requests of the serving leg (Poisson arrivals at --serve-rate; 0 skips it)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for the run (0: one per concurrent configuration, at most 16)")
@@ -210,10 +221,10 @@ def main():
                     help="evaluate the electrostatics after MB-pol and the water terms instead of beside them")
     ap.add_argument("--spme-reps", type=int, default=20,
                     help="calls per size of the particle-mesh Ewald leg (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=400.0)
-    ap.add_argument("--serve-mix", choices=["beads", "mixed"], default="beads",
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-mix", choices=["beads", "mixed"], default="mixed",
                     help="beads: PIMD beads of 256 waters only (same-size slots); mixed: beads and isolated "
-                         "clusters of 3-1024 waters (switched off: see the serving-leg note in the design notes)")
+                         "clusters of 3-1024 waters, half and half")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB", default=0,
                     help="only run a plain device copy of MIB MiB five times (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")

@@ -7,5 +7,5 @@ The directory name is not a Python identifier, so import it with
 from . import waterbox  # noqa: F401
 from . import elec_params  # noqa: F401
 from . import water_params  # noqa: F401
-from .mbgpu import (Context, ElecParams, MbgpuError, WaterParams, load_library, mbpol_2b_cutoff, mbpol_2b_poly,  # noqa: F401
-                    mbpol_3b_cutoff, mbpol_3b_poly)
+from .mbgpu import (Context, ElecParams, MbgpuError, WaterParams, check_listttm2, load_library,  # noqa: F401
+                    mbpol_2b_cutoff, mbpol_2b_poly, mbpol_3b_cutoff, mbpol_3b_poly)

@@ -142,6 +142,7 @@ int alloc_fixed(mbgpu_ctx* c)
     B.cap2 = 128;
     B.cap3 = 64;
     HIPCHK(dalloc(&B.mon, (size_t)9 * nw));
+    HIPCHK(dalloc(&B.wbase, nw));
     HIPCHK(dalloc(&B.nbr2, (size_t)nw * B.cap2));
     HIPCHK(dalloc(&B.cnt2, nw));
     HIPCHK(dalloc(&B.off2, nw));
@@ -247,6 +248,9 @@ int build_lists(mbgpu_ctx* c)
         int cnt[4];
         HIPCHK(hipMemcpyAsync(cnt, B.counts, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        if (cnt[2] & 2)
+            return fail(-5, "listttm2: some water's entries are not three consecutive atom ids a, a+1, a+2 "
+                            "within 1..%d", B.natms);
         if (cnt[2]) return fail(-2, "neighbour list capacity exceeded (more than %d/%d O-O neighbours)", B.cap2,
                                 B.cap3);
         c->nd = cnt[0];
@@ -376,6 +380,14 @@ extern "C" {
 
 const char* mbgpu_last_error(void) { return g_err.c_str(); }
 
+int mbgpu_check_listttm2(int nttm2, const int* listttm2, int natms)
+{
+    if (nttm2 < 0 || nttm2 % 3 != 0 || (nttm2 > 0 && !listttm2)) return -1;
+    for (int w = 0; w < nttm2 / 3; ++w)
+        if (water_base(listttm2, w, natms) < 0) return w + 1;
+    return 0;
+}
+
 mbgpu_ctx* mbgpu_create(int nw, int natms)
 {
     if (nw < 1 || natms < 3 * nw) {
@@ -410,7 +422,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->el_stream) (void)hipStreamSynchronize(c->el_stream);
     MbBuffers& B = c->B;
-    dfree(B.mon); dfree(B.nbr2); dfree(B.cnt2); dfree(B.off2); dfree(B.nbr3); dfree(B.cnt3);
+    dfree(B.mon); dfree(B.wbase); dfree(B.nbr2); dfree(B.cnt2); dfree(B.off2); dfree(B.nbr3); dfree(B.cnt3);
     dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
     dfree(B.clus2); dfree(B.xv2); dfree(B.part2); dfree(B.e2); dfree(B.g2); dfree(B.s2);
     dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3); dfree(B.s3);
@@ -480,9 +492,10 @@ int mbgpu_mbpol_forces(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
     if (!c) return fail(-1, "null context");
     const int na = c->B.natms;
     if (nttm2 != 3 * c->B.nw) return fail(-1, "nttm2 = %d does not match the context (3*%d)", nttm2, c->B.nw);
-    for (int t = 0; t < nttm2; ++t)
-        if (listttm2[t] < 1 || listttm2[t] + 2 - (t % 3) > na)
-            return fail(-5, "listttm2(%d) = %d outside 1..%d", t + 1, listttm2[t], na);
+    if (!listttm2) return fail(-1, "null listttm2");
+    if (const int bad = mbgpu_check_listttm2(nttm2, listttm2, na))
+        return fail(-5, "listttm2(%d..%d) = %d %d %d: not three consecutive atom ids within 1..%d", 3 * bad - 2,
+                    3 * bad, listttm2[3 * bad - 3], listttm2[3 * bad - 2], listttm2[3 * bad - 1], na);
     HIPCHK(hipMemcpyAsync(c->hlist, listttm2, sizeof(int) * nttm2, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->hx, xxx, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->hy, yyy, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));

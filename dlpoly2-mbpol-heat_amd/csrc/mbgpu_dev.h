@@ -15,6 +15,18 @@
 // accumulator for any realistic number of contributions.
 #define MBGPU_FX_SCALE 4294967296.0     // 2^32
 
+// 0-based atom offset of water w's oxygen from listttm2 (1-based atom ids, O, H1, H2 per
+// water, read at mbpol.f90:184-188), or -1 when w's three entries are not the consecutive
+// ids a, a+1, a+2 with 1 <= a and a+2 <= natms.  The kernels address a water's atoms as
+// offset + 0..2, as the reference does (o1, o1+1, o1+2, mbpol.f90:229,526-544), so every
+// entry is checked by this one predicate, on the host for host arrays and on the device
+// for device arrays, before any atom array is indexed with it.
+__host__ __device__ inline int water_base(const int* list, int w, int natms)
+{
+    const int a = list[3 * w], b = list[3 * w + 1], c = list[3 * w + 2];
+    return (a >= 1 && a <= natms - 2 && b == a + 1 && c == a + 2) ? a - 1 : -1;
+}
+
 struct CellGeom {
     int imcon;          // 1 cubic, 2 orthorhombic, 3 general (mbpol.f90:1319-1357)
     double c[9];        // cell vectors, Fortran cell(1..9)

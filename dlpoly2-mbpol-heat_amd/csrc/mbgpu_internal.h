@@ -58,6 +58,7 @@ struct MbBuffers {
     // geometry
     double* mon = nullptr;         // [nw][9] imaged monomers
     int* list = nullptr;           // [3*nw] listttm2 (1-based atom ids)
+    int* wbase = nullptr;          // [nw] checked 0-based atom offset of each water's O (water_base)
     double *x = nullptr, *y = nullptr, *z = nullptr;   // [natms] raw positions
     // neighbour lists
     int *nbr2 = nullptr, *cnt2 = nullptr, *off2 = nullptr;   // partners j > i within 6.5 A

@@ -167,19 +167,28 @@ __global__ void k_zero_i(int* __restrict__ p, int n)
     if (i < n) p[i] = 0;
 }
 
-// image_monomers (mbpol.f90:1275-1301) on the waters gathered through listttm2
-__global__ void k_image_monomers(int nw, const int* __restrict__ list, const double* __restrict__ x,
+// image_monomers (mbpol.f90:1275-1301) on the waters gathered through listttm2.  Every
+// water's entries are checked (water_base) before an atom array is read with them: a bad
+// entry sets bit 1 of *err and the water reads atoms 0..2 instead, so that the evaluation
+// stays in bounds until the host sees the flag and stops with an error.  The checked
+// 0-based offsets go to wbase[nw] for the later kernels.
+__global__ void k_image_monomers(int nw, int natms, const int* __restrict__ list, const double* __restrict__ x,
                                  const double* __restrict__ y, const double* __restrict__ z, CellGeom g,
-                                 double* __restrict__ mon)
+                                 double* __restrict__ mon, int* __restrict__ wbase, int* __restrict__ err)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
+    int o = water_base(list, w, natms);
+    if (o < 0) {
+        atomicOr(err, 2);
+        o = 0;
+    }
+    wbase[w] = o;
     double m[9];
     for (int a = 0; a < 3; ++a) {
-        const int id = list[3 * w + a] - 1;
-        m[3 * a] = x[id];
-        m[3 * a + 1] = y[id];
-        m[3 * a + 2] = z[id];
+        m[3 * a] = x[o + a];
+        m[3 * a + 1] = y[o + a];
+        m[3 * a + 2] = z[o + a];
     }
     for (int h = 1; h < 3; ++h) {
         double dx = m[3 * h] - m[0], dy = m[3 * h + 1] - m[1], dz = m[3 * h + 2] - m[2];
@@ -467,14 +476,14 @@ __device__ __forceinline__ void cluster_virial(const double* w, const double* g,
 // block partials of energy, virial and stress.
 // sum of the raw (un-imaged) positions of the atoms of the NMON waters ids[0..NMON)
 template <int NMON>
-__device__ __forceinline__ void cluster_raw_sum(const int* ids, const int* __restrict__ list,
+__device__ __forceinline__ void cluster_raw_sum(const int* ids, const int* __restrict__ wbase,
                                                 const double* __restrict__ x, const double* __restrict__ y,
                                                 const double* __restrict__ z, double* S)
 {
     S[0] = S[1] = S[2] = 0.0;
 #pragma unroll
     for (int m = 0; m < NMON; ++m) {
-        const int o = list[3 * ids[m]] - 1;
+        const int o = wbase[ids[m]];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             S[0] += x[o + a];
@@ -488,7 +497,7 @@ __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* 
                                                  const double* __restrict__ xv, const double* __restrict__ part,
                                                  double* __restrict__ erec, double* __restrict__ grec,
                                                  double* __restrict__ blk, const int* __restrict__ dimers,
-                                                 const int* __restrict__ list, const double* __restrict__ x,
+                                                 const int* __restrict__ wbase, const double* __restrict__ x,
                                                  const double* __restrict__ y, const double* __restrict__ z,
                                                  double* __restrict__ srec)
 {
@@ -496,7 +505,7 @@ __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* 
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (d < nd && dimers) {
         double S[3];
-        cluster_raw_sum<2>(dimers + 2L * d, list, x, y, z, S);
+        cluster_raw_sum<2>(dimers + 2L * d, wbase, x, y, z, S);
         srec[d] = S[0];
         srec[(long)nd + d] = S[1];
         srec[2L * nd + d] = S[2];
@@ -593,7 +602,7 @@ __global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* 
                                                  const double* __restrict__ xv, const double* __restrict__ part,
                                                  double* __restrict__ erec, double* __restrict__ grec,
                                                  double* __restrict__ blk, const int* __restrict__ trimers,
-                                                 const int* __restrict__ list, const double* __restrict__ x,
+                                                 const int* __restrict__ wbase, const double* __restrict__ x,
                                                  const double* __restrict__ y, const double* __restrict__ z,
                                                  double* __restrict__ srec)
 {
@@ -601,7 +610,7 @@ __global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* 
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (t < nt && trimers) {
         double S[3];
-        cluster_raw_sum<3>(trimers + 3L * t, list, x, y, z, S);
+        cluster_raw_sum<3>(trimers + 3L * t, wbase, x, y, z, S);
         srec[t] = S[0];
         srec[(long)nt + t] = S[1];
         srec[2L * nt + t] = S[2];
@@ -696,7 +705,7 @@ __device__ __forceinline__ int find_rank(const int* __restrict__ L, int n, int v
 //            compute_heat_flux2 contracts force_matrix with un-imaged positions.
 struct GatherArgs {
     int nw, nd, nt, cap2;
-    const int *list, *dimers, *trimers;
+    const int *wbase, *dimers, *trimers;
     const int *nbr2, *cnt2, *off2, *nbr2l, *cnt2l;
     const int *nbr3, *cnt3, *toff, *tcnt, *rowoff;
     const unsigned long long* kmask;
@@ -740,7 +749,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather_water(GatherArgs A)
         }
     }
     __syncthreads();
-    const int o = A.list[3 * w] - 1;
+    const int o = A.wbase[w];
     double xa[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -866,11 +875,11 @@ __global__ void k_zero_other_sites(int natms, const unsigned char* __restrict__ 
     for (int k = 0; k < 9; ++k) hw[9L * i + k] = 0.0;
 }
 
-__global__ void k_mark_water(int nw, const int* __restrict__ list, unsigned char* __restrict__ is_water)
+__global__ void k_mark_water(int nw, const int* __restrict__ wbase, unsigned char* __restrict__ is_water)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
-    const int o = list[3 * w] - 1;
+    const int o = wbase[w];
     is_water[o] = is_water[o + 1] = is_water[o + 2] = 1;
 }
 
@@ -926,8 +935,8 @@ int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s)
 {
     const int nw = B.nw;
     hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, B.counts, 4);
-    hipLaunchKernelGGL(k_image_monomers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.list, B.x, B.y, B.z, g,
-                       B.mon);
+    hipLaunchKernelGGL(k_image_monomers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.natms, B.list, B.x, B.y,
+                       B.z, g, B.mon, B.wbase, B.counts + 2);
     hipLaunchKernelGGL(k_neighbours, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap2, B.nbr2, B.cnt2, B.nbr2l,
                        B.cnt2l, 64, B.nbr3, B.cnt3, B.counts + 2);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.cnt2, nw, B.off2, B.counts + 0);
@@ -960,7 +969,7 @@ int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipSt
         hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2);
     }
     hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, 1, B.clus2, B.xv2, B.part2, B.e2, B.g2, B.blk,
-                       gather ? B.dimers : nullptr, B.list, B.x, B.y, B.z, B.s2);
+                       gather ? B.dimers : nullptr, B.wbase, B.x, B.y, B.z, B.s2);
     return (int)hipGetLastError();
 }
 
@@ -983,7 +992,7 @@ int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gath
         hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3);
     }
     hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, 1, B.clus3, B.xv3, B.part3, B.e3, B.g3,
-                       B.blk + 8L * nb2, gather ? B.trimers : nullptr, B.list, B.x, B.y, B.z, B.s3);
+                       B.blk + 8L * nb2, gather ? B.trimers : nullptr, B.wbase, B.x, B.y, B.z, B.s3);
     return (int)hipGetLastError();
 }
 
@@ -996,12 +1005,12 @@ int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, d
     if (nb2 + nb3 > B.nblk_cap) return -1;
     hipLaunchKernelGGL(k_zero_i, dim3((na + 1023) / 1024), dim3(256), 0, s, reinterpret_cast<int*>(B.is_water),
                        (na + 3) / 4);
-    hipLaunchKernelGGL(k_mark_water, dim3((nw + 255) / 256), dim3(256), 0, s, nw, B.list, B.is_water);
+    hipLaunchKernelGGL(k_mark_water, dim3((nw + 255) / 256), dim3(256), 0, s, nw, B.wbase, B.is_water);
     hipLaunchKernelGGL(k_zero_other_sites, dim3((na + 255) / 256), dim3(256), 0, s, na, B.is_water, B.fout,
                        B.atomic_energy, B.heat_w);
     GatherArgs A;
     A.nw = nw; A.nd = nd; A.nt = nt; A.cap2 = B.cap2;
-    A.list = B.list; A.dimers = B.dimers; A.trimers = B.trimers;
+    A.wbase = B.wbase; A.dimers = B.dimers; A.trimers = B.trimers;
     A.nbr2 = B.nbr2; A.cnt2 = B.cnt2; A.off2 = B.off2; A.nbr2l = B.nbr2l; A.cnt2l = B.cnt2l;
     A.nbr3 = B.nbr3; A.cnt3 = B.cnt3; A.toff = B.toff; A.tcnt = B.tcnt; A.rowoff = B.rowoff; A.kmask = B.kmask;
     A.e2 = B.e2; A.g2 = B.g2; A.s2 = B.s2; A.e3 = B.e3; A.g3 = B.g3; A.s3 = B.s3;

@@ -35,6 +35,7 @@ _SIGS = {
     "mbgpu_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int]),
     "mbgpu_destroy": (None, [ctypes.c_void_p]),
     "mbgpu_last_error": (ctypes.c_char_p, []),
+    "mbgpu_check_listttm2": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int]),
     "mbgpu_mbpol_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
                                           _dp, _dp, _dp, _dp, _dp, _dp]),
     "mbgpu_mbpol_forces_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_void_p,
@@ -211,6 +212,14 @@ def mbpol_3b_poly(w1, w2, w3, g1=None, g2=None, g3=None):
 
 
 # ----------------------------------------------------------------------- context ----
+def check_listttm2(listttm2, natms):
+    """mbgpu_check_listttm2: 0 when every water's entries are consecutive atom ids a, a+1,
+    a+2 within 1..natms, else the 1-based index of the first bad water (-1: size not a
+    multiple of 3).  Runs on the host; the device entries apply the same predicate."""
+    a = np.ascontiguousarray(listttm2, dtype=np.int32)
+    return load_library().mbgpu_check_listttm2(a.size, a.ctypes.data_as(_ip), natms)
+
+
 def ring_matrices(nbead, langevin=0):
     """Normal-mode matrices of transformation_matrix (mbgpu_ring_matrices): (nmode_to_cart,
     cart_to_nmode, tcart_to_tnmode, lambda), each matrix [k][b] = M(k+1, b+1)."""

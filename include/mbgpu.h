@@ -78,11 +78,22 @@ int mbgpu_mbpol_forces(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm
                        const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
                        double* fzz, double* engsrp, double* virsrp, double stress[9]);
 
-/* Same with device pointers (inputs resident in HBM; f{x,y,z} device arrays or NULL). */
+/* Same with device pointers (inputs resident in HBM; f{x,y,z} device arrays or NULL).
+ * The device list is checked on the device before any atom array is read with it; a bad
+ * entry returns -5 and no forces are added. */
 int mbgpu_mbpol_forces_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm2,
                            const int* listttm2_dev, const double* x_dev, const double* y_dev,
                            const double* z_dev, double* fx_dev, double* fy_dev, double* fz_dev,
                            double* engsrp, double* virsrp, double stress[9]);
+
+/* The check both mbgpu_mbpol_forces entries apply to listttm2 (host array here, no GPU
+ * needed): every water's three entries must be consecutive atom ids a, a+1, a+2 with
+ * 1 <= a and a+2 <= natms, as DL_POLY's FIELD layout gives them (the reference reads
+ * listttm2 at mbpol.f90:184-188 and addresses a water's atoms as o1, o1+1, o1+2 with
+ * o1 = listttm2(3i-2), mbpol.f90:229,526-544, without checking either).  Returns 0 when the
+ * list is valid, the 1-based index of the first bad water otherwise, -1 for a bad nttm2
+ * (negative or not a multiple of 3) or a NULL list.  No counterpart in the reference. */
+int mbgpu_check_listttm2(int nttm2, const int* listttm2, int natms);
 
 /* Per-atom results of the last mbgpu_mbpol_forces call (host arrays of natms, natms and
  * 9*natms, any may be NULL):

@@ -161,6 +161,34 @@ def test_bad_list_is_an_error(pkg, box64):
         ctx.mbpol_forces(1, box64.cell, 3 * box64.nw, lst, box64.x, box64.y, box64.z, f[0], f[1], f[2])
 
 
+def test_bad_device_list_is_an_error(pkg, box64):
+    """A device listttm2 is checked on the device before an atom array is indexed with it
+    (the list of zeros is what a late clear of a serving slot's list left behind); the
+    context evaluates a valid list unchanged afterwards."""
+    import torch
+    dev = torch.device("cuda:0")
+    T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    x, y, z = T(box64.x), T(box64.y), T(box64.z)
+    ctx = pkg.Context(box64.nw, box64.natms)
+    good = T(box64.listttm2, torch.int32)
+    torch.cuda.synchronize()
+    e0, v0, s0 = ctx.mbpol_forces_dev(box64.imcon, box64.cell, good.data_ptr(), x.data_ptr(), y.data_ptr(),
+                                      z.data_ptr())
+    swapped = box64.listttm2.copy()
+    swapped[[4, 5]] = swapped[[5, 4]]
+    beyond = box64.listttm2.copy()
+    beyond[-3:] = [box64.natms - 1, box64.natms, box64.natms + 1]
+    for name, bad in (("zeros", np.zeros_like(box64.listttm2)), ("swapped", swapped), ("beyond", beyond)):
+        print("bad device list:", name, flush=True)
+        lst = T(bad, torch.int32)
+        torch.cuda.synchronize()
+        with pytest.raises(pkg.MbgpuError, match="listttm2"):
+            ctx.mbpol_forces_dev(box64.imcon, box64.cell, lst.data_ptr(), x.data_ptr(), y.data_ptr(), z.data_ptr())
+    e1, v1, s1 = ctx.mbpol_forces_dev(box64.imcon, box64.cell, good.data_ptr(), x.data_ptr(), y.data_ptr(),
+                                      z.data_ptr())
+    assert e1 == e0 and v1 == v0 and np.array_equal(s1, s0)
+
+
 def test_2048_size_independent_properties(pkg, waterbox):
     box = waterbox.box_2048()
     ctx, g = gpu_forces(pkg, box)

@@ -5,7 +5,9 @@
   is bit-identical to evaluating the same configuration alone on a fresh slot.
 * A memory budget that fits one slot at a time: slots of other sizes are evicted and every
   request still completes with the same results.
-  (These two mixed-size tests are skipped: see SERVE_SKIP.)
+  (Slots of new sizes are created while other slots run: the test of the stream order of a
+  slot's first upload, and of the water-list check that keeps a bad list in bounds, is
+  test_slot_inputs.py on the CPU.)
 * Same-size requests (beads) served by several workers at once equal the beads alone.
 * The i-PI driver with the GPU evaluator answers a scripted server with the forces of the
   evaluation converted to Hartree/bohr.
@@ -51,12 +53,6 @@ def same(r, f, ra, fa):
     assert np.array_equal(f, fa)
 
 
-SERVE_SKIP = ("concurrent serving of mixed sizes is switched off: the serving leg of bench.py (16 workers, "
-              "256-water beads and clusters up to 953 waters, run after the batched legs) once ended in an "
-              "illegal memory access whose cause has not been found")
-
-
-@pytest.mark.skip(reason=SERVE_SKIP)
 def test_mix_equals_alone(mods):
     serve, engine, *_ = mods
     jobs = jobs_mix(mods)
@@ -75,12 +71,12 @@ def test_mix_equals_alone(mods):
     assert srv.slots_created <= 4 + 4           # at most one slot per size per worker
 
 
-@pytest.mark.skip(reason=SERVE_SKIP)
 def test_eviction_under_memory_budget(mods):
     serve, engine, *_ = mods
     jobs = jobs_mix(mods)
     budget = serve.slot_bytes(64) + serve.MB        # one 64-water slot, or a few small ones
     srv = serve.Server(workers=2, memory_budget=budget)
+    print("serving 8 mixed requests on 2 workers under a one-slot budget", flush=True)
     reqs = [srv.submit(*j) for j in jobs]
     for r in reqs:
         assert r.done.wait(120)
@@ -93,11 +89,13 @@ def test_eviction_under_memory_budget(mods):
 
 
 def test_bead_serving_equals_alone(mods):
-    """Same-size requests (PIMD beads of one box) through the server's workers at once:
-    every result is bit-identical to evaluating the bead alone."""
+    """Same-size requests through the server's workers at once: PIMD beads of the serving
+    workload's stream A (256 waters, L = 19.72 A, each bead the base configuration with
+    N(0, 0.05 A) per-atom displacements).  Every result is bit-identical to evaluating the
+    bead alone."""
     serve, engine, ipi, ep, wp, wb = mods
-    base = wb.make_waterbox(64, 12.42, 4, seed=11)
-    beads = [wb.displaced(base, 0.05, seed=200 + i) for i in range(8)]
+    base = wb.box_256()
+    beads = [wb.displaced(base, 0.05, seed=500 + i) for i in range(8)]
     srv = serve.Server(workers=4)
     print("serving 8 bead requests on 4 workers", flush=True)
     reqs = [srv.submit(b, ep.for_box(b.L), wp.for_box(b.L), "bead") for b in beads]
