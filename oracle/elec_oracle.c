@@ -217,7 +217,7 @@ typedef struct {
 /* qdetermine.f90:118-259 for one water: p1, p2 and their derivatives; the new charge
  * assignment (lttmfold = .false., lttm3 = .false. for MB-pol, simdef_cmd.f vmbpol). */
 static void dms_water(int imcon, const double *cell, const double *o, const double *h1, const double *h2,
-                      double gamma, double *qh1, double *qh2, dms_grad_t *G)
+                      double gamma, double *qh1, double *qh2, dms_grad_t *G, double *pout)
 {
     double r1v[3] = {h1[0] - o[0], h1[1] - o[1], h1[2] - o[2]};
     double r2v[3] = {h2[0] - o[0], h2[1] - o[1], h2[2] - o[2]};
@@ -275,16 +275,35 @@ static void dms_water(int imcon, const double *cell, const double *o, const doub
     *qh1 = p1 + factor * (p1 + p2);
     *qh2 = p2 + factor * (p1 + p2);
     if (G) *G = g;
+    if (pout) {
+        pout[0] = p1;
+        pout[1] = p2;
+    }
 }
 
 int ora_dms_charges(int imcon, const double *cell, const double *w9, double gamma, double *q3)
 {
     if (!g_dms_loaded) return -1;
     double qh1, qh2;
-    dms_water(imcon, cell, w9, w9 + 3, w9 + 6, gamma, &qh1, &qh2, NULL);
+    dms_water(imcon, cell, w9, w9 + 3, w9 + 6, gamma, &qh1, &qh2, NULL, NULL);
     q3[0] = qh1;
     q3[1] = qh2;
     q3[2] = -(qh1 + qh2);
+    return 0;
+}
+
+int ora_dms_p(int imcon, const double *cell, const double *w9, double *p8)
+{
+    if (!g_dms_loaded) return -1;
+    double qh1, qh2;
+    dms_grad_t g;
+    dms_water(imcon, cell, w9, w9 + 3, w9 + 6, 0.0, &qh1, &qh2, &g, p8);
+    p8[2] = g.dp1dr1;
+    p8[3] = g.dp1dr2;
+    p8[4] = g.dp1dcos;
+    p8[5] = g.dp2dr1;
+    p8[6] = g.dp2dr2;
+    p8[7] = g.dp2dcos;
     return 0;
 }
 
@@ -1076,7 +1095,7 @@ int ora_elec_forces(ora_elec_args *A)
         const double h1[3] = {C.x[io + 1], C.y[io + 1], C.z[io + 1]};
         const double h2[3] = {C.x[io + 2], C.y[io + 2], C.z[io + 2]};
         double qh1, qh2;
-        dms_water(A->imcon, A->cell, o, h1, h2, A->gamma, &qh1, &qh2, &C.dg[w]);
+        dms_water(A->imcon, A->cell, o, h1, h2, A->gamma, &qh1, &qh2, &C.dg[w], NULL);
         C.chge[io] = A->q_o;
         C.chge[io + 1] = qh1;
         C.chge[io + 2] = qh2;

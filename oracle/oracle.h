@@ -100,6 +100,10 @@ void ora_erfc_table(double alpha, double rcut, double *table);
 int ora_dms_load(const char *path);
 /* DMS charges of one water (w9: O, H1, H2 xyz) -> q(H1), q(H2), q(M) (qdetermine.f90). */
 int ora_dms_charges(int imcon, const double *cell, const double *w9, double gamma, double *q3);
+/* PS dipole charges of one water (qdetermine.f90:130-236), bonds by minimum image in cell:
+ * p8 = p1, p2, dp1/dr1, dp1/dr2,
+ * dp1/dcos, dp2/dr1, dp2/dr2, dp2/dcos (r in Angstrom, cos of the HOH angle). */
+int ora_dms_p(int imcon, const double *cell, const double *w9, double *p8);
 
 typedef struct {
     /* inputs */

@@ -209,6 +209,15 @@ def dms_charges(w9, gamma=0.426706882, imcon=1, cell=None):
     return q
 
 
+def dms_p(w9, imcon=1, cell=None):
+    """p1, p2 and (dp1/dr1, dp1/dr2, dp1/dcos), (dp2/dr1, dp2/dr2, dp2/dcos) of one water."""
+    lib = elec_lib()
+    cell = np.asarray(cell if cell is not None else [100.0, 0, 0, 0, 100.0, 0, 0, 0, 100.0], float)
+    p = np.zeros(8)
+    assert lib.ora_dms_p(imcon, P(cell), P(np.ascontiguousarray(w9, float)), P(p)) == 0
+    return p[0], p[1], p[2:5].copy(), p[5:8].copy()
+
+
 def site_xyz(box):
     """4*nw site positions (O, H1, H2 per water, then M sites), interleaved (n, 3)."""
     return np.ascontiguousarray(box.xyz[:4 * box.nw], float).copy()

@@ -4,7 +4,9 @@ The reference's Fortran cannot be built here and ships no fixtures for this path
 restatement is pinned by what the mathematics fixes:
   * the special functions against scipy / the Numerical Recipes error bounds;
   * the B-function table against the closed forms of the Ewald real-space kernels;
-  * the DMS charges: neutrality, H1<->H2 exchange, rotation invariance;
+  * the DMS charges: against the reference's own qdetermine compiled from its sources
+    (golden vectors), its molecular-dipole identity, the gas-phase dipole at the PS
+    equilibrium and finite differences; neutrality, H1<->H2 exchange, rotation invariance;
   * the Ewald split: the converged total energy does not depend on alpha (while each
     of the reciprocal, real-space and excluded-pair parts changes by ~10 %);
   * forces = -dE/dx by central differences (the converged dipoles make the energy
@@ -19,6 +21,7 @@ consistency, the alpha independence or one of the sums.
 """
 import importlib
 import math
+import os
 
 import numpy as np
 import pytest
@@ -95,6 +98,84 @@ def test_erfc_table_closed_forms():
         for n in (1, 2, 3):
             fd = -(B(n - 1, r0 + h) - B(n - 1, r0 - h)) / (2 * h) / r0
             assert abs(fd - B(n, r0)) <= 1e-7 * abs(B(n, r0))
+
+
+@pytest.fixture(scope="module")
+def dms_ref():
+    """Outputs of the reference's own qdetermine (tests/golden/make_dms_golden.py)."""
+    return dict(np.load(os.path.join(o.REPO, "tests", "golden", "dms_reference.npz")))
+
+
+def _waters(d, s):
+    nw = d["xyz"].shape[1] // 4
+    return nw, [d["xyz"][s][3 * i:3 * i + 3].reshape(-1) for i in range(nw)]
+
+
+def test_dms_matches_the_reference_qdetermine(dms_ref):
+    """Charges and dp/d(r1, r2, cos) of the oracle against qdetermine.f90 itself, compiled
+    from the reference's sources: 9 sets of 24 waters (O-H 0.75-1.35 A, HOH 70-150 deg,
+    bonds across the boundary of cubic, orthorhombic and parallelepiped cells, three gamma)."""
+    d = dms_ref
+    for s in range(len(d["xyz"])):
+        nw, ws = _waters(d, s)
+        imcon, cell, gamma = int(d["imcon"][s]), d["cell"][s], float(d["gamma"][s])
+        assert np.all(d["chge"][s][0:3 * nw:3] == 0.0)
+        for i, w in enumerate(ws):
+            q = o.dms_charges(w, gamma=gamma, imcon=imcon, cell=cell)
+            ref = np.array([d["chge"][s][3 * i + 1], d["chge"][s][3 * i + 2], d["chge"][s][3 * nw + i]])
+            assert np.abs(q - ref).max() <= 1e-14, (s, i)
+            _, _, g1, g2 = o.dms_p(w, imcon=imcon, cell=cell)
+            assert np.abs(np.concatenate([g1, g2]) - d["dp"][s][i]).max() <= 1e-13 * np.abs(d["dp"][s][i]).max()
+
+
+def test_dms_molecular_dipole_is_the_ps_dipole(dms_ref):
+    """qdetermine's own check (qdetermine.f90:262-305): the charges on H1, H2 and the M
+    site, M = O + gamma/2 (r1 + r2), carry the molecular dipole p1 r1 + p2 r2 of the PS
+    surface for any gamma.  Computed from the oracle's charges and compared with the
+    reference's chkdip; a wrong assignment factor gamma / (2 (1 - gamma)) fails it."""
+    d = dms_ref
+    for s in range(len(d["xyz"])):
+        nw, ws = _waters(d, s)
+        imcon, cell, gamma = int(d["imcon"][s]), d["cell"][s], float(d["gamma"][s])
+        cv = cell.reshape(3, 3)
+        inv = np.linalg.inv(cv)
+        for i, w in enumerate(ws):
+            q = o.dms_charges(w, gamma=gamma, imcon=imcon, cell=cell)
+            O = w[:3]
+            r = [w[3 * k:3 * k + 3] - O for k in (1, 2)]
+            r = [x - np.round(x @ inv) @ cv for x in r]
+            m = 0.5 * gamma * (r[0] + r[1])
+            dip = q[0] * r[0] + q[1] * r[1] + q[2] * m
+            assert np.abs(dip - d["chkdip"][s][i]).max() <= 1e-14, (s, i)
+
+
+def test_dms_dipole_at_the_ps_equilibrium():
+    """At the PS reference geometry (r_e = 0.958649 A, theta_e = 104.3475 deg, module.f90:
+    480-481) the DMS dipole is within 1 % of water's gas-phase dipole, 1.855 D.  The charge
+    term adms (r1^bdms + r2^bdms)(c0 + c1 P1 + c2 P2) carries nearly all of it there, so a
+    wrong adms, c0, c1, c2 or coef(1) moves it by more than that."""
+    th = np.radians(104.3475)
+    w = np.array([0, 0, 0, 0.958649, 0, 0, 0.958649 * np.cos(th), 0.958649 * np.sin(th), 0.0])
+    p1, p2, _, _ = o.dms_p(w)
+    mu = np.linalg.norm(p1 * w[3:6] + p2 * w[6:9]) * 4.803239          # e A -> Debye
+    assert abs(mu - 1.855) <= 0.01 * 1.855
+    assert p1 == p2
+
+
+@pytest.mark.parametrize("r1, r2, deg", [(0.9572, 0.9572, 104.52), (0.88, 1.07, 95.0), (1.2, 0.93, 128.0)])
+def test_dms_derivatives_finite_difference(r1, r2, deg):
+    def geom(a, b, c):
+        return np.array([0, 0, 0, a, 0, 0, b * c, b * np.sqrt(1 - c * c), 0.0])
+
+    c0 = np.cos(np.radians(deg))
+    p1, p2, g1, g2 = o.dms_p(geom(r1, r2, c0))
+    h = 1e-6
+    for k, (dr1, dr2, dc) in enumerate(((h, 0, 0), (0, h, 0), (0, 0, h))):
+        pp = o.dms_p(geom(r1 + dr1, r2 + dr2, c0 + dc))
+        pm = o.dms_p(geom(r1 - dr1, r2 - dr2, c0 - dc))
+        for j, g in ((0, g1), (1, g2)):
+            fd = (pp[j] - pm[j]) / (2 * h)
+            assert abs(fd - g[k]) <= 1e-7 * max(1.0, abs(g[k])), (k, j)
 
 
 def test_dms_charges():
