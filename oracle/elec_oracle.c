@@ -945,6 +945,7 @@ static void qdforce(ctx_t *C)
     const double gamma = A->gamma;
     const double *vesp = A->vesp;
     double strs[6] = {0, 0, 0, 0, 0, 0};
+    double qdc[6] = {0, 0, 0, 0, 0, 0};   /* third * vtmp * dqdcos * dcostmp over O, H1, H2 */
     double virqdf = 0.0;
     for (int w = 0; w < C->nw; ++w) {
         const int io = 3 * w, ih1 = io + 1, ih2 = io + 2, im = C->nreal + w;
@@ -1041,8 +1042,13 @@ static void qdforce(ctx_t *C)
                 dc[t] = dcosdr12 * rv[0][sa[t]] * rv[0][sb[t]] / rr1 + dcosdr13 * rv[1][sa[t]] * rv[1][sb[t]] / rr2 +
                         dcosdr23 * r3v[sa[t]] * r3v[sb[t]] / rr3;
             for (int t = 0; t < 6; ++t) strs[t] -= vt * dqdcos[i] * dc[t];
+            for (int t = 0; t < 6; ++t) qdc[t] += vt * dqdcos[i] * dc[t];
             virqdf += vt * (dc[0] + dc[3] + dc[5]) * dqdcos[i];
         }
+    }
+    {
+        const int q9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int q = 0; q < 9; ++q) A->stress_qd_atoms[q] = qdc[q9[q]];
     }
     A->stress[0] += strs[0];
     A->stress[1] += strs[1];
@@ -1054,6 +1060,37 @@ static void qdforce(ctx_t *C)
     A->stress[7] += strs[4];
     A->stress[8] += strs[5];
     A->vir_qd = virqdf;
+}
+
+/* qdforce alone for given site potentials (a test aid, to compare with the reference's own
+ * qdforce): A->vesp in, A->f, A->stress, A->force_matrix, A->heat_w accumulated,
+ * A->vir_qd and A->stress_qd_atoms set.  Positions as given (the M sites are not rebuilt). */
+int ora_qdforce(ora_elec_args *A)
+{
+    if (A->imcon < 1 || A->imcon > 3) return -1;
+    if (!g_dms_loaded) return -3;
+    ctx_t C;
+    memset(&C, 0, sizeof C);
+    C.A = A;
+    C.nw = A->nw;
+    C.nreal = 3 * A->nw;
+    C.n = 4 * A->nw;
+    C.x = A->x;
+    C.y = A->y;
+    C.z = A->z;
+    C.dg = calloc(C.nw, sizeof(dms_grad_t));
+    if (!C.dg) return -4;
+    for (int w = 0; w < C.nw; ++w) {
+        const int io = 3 * w;
+        const double o[3] = {C.x[io], C.y[io], C.z[io]};
+        const double h1[3] = {C.x[io + 1], C.y[io + 1], C.z[io + 1]};
+        const double h2[3] = {C.x[io + 2], C.y[io + 2], C.z[io + 2]};
+        double qh1, qh2;
+        dms_water(A->imcon, A->cell, o, h1, h2, A->gamma, &qh1, &qh2, &C.dg[w], NULL);
+    }
+    qdforce(&C);
+    free(C.dg);
+    return 0;
 }
 
 /* ------------------------------------------------------------------- driver ---- */

@@ -143,12 +143,20 @@ typedef struct {
     int niter;                 /* solver iterations */
     double last_delta;         /* RMS dipole change of the last iteration */
     double volm;
+    double stress_qd_atoms[9]; /* sum over atoms of the per-atom qd stresses the reference
+                                  records when HEAT_STRESS is not defined: the angular terms
+                                  of update_stress_qd (qdforce.f90:504-531), set */
 } ora_elec_args;
 
 /* The polarizable-Ewald part of forces (forces.f:268-404,585-610,700-712,874-892) with
  * qdetermine and the M sites.  Returns 0, -1 (imcon), -2 (no convergence within
  * max_iter: the reference stops), -3 (DMS table not loaded). */
 int ora_elec_forces(ora_elec_args *a);
+
+/* qdforce alone (qdforce.f90:13-569) for the site potentials in a->vesp (a test aid):
+ * accumulates a->f, a->stress, a->force_matrix, a->heat_w; sets a->vir_qd and
+ * a->stress_qd_atoms.  Positions are used as given (M sites not rebuilt). */
+int ora_qdforce(ora_elec_args *a);
 
 /* M-site redistribution of forces (3 per site), energies and 9-component stresses. */
 void ora_msite_distribute(int nw, double gamma, double *f, double *e, double *s9);

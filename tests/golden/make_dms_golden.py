@@ -1,9 +1,10 @@
 #!/usr/bin/env python3
-"""Write golden vectors of the DMS charges from the reference's own qdetermine.
+"""Write golden vectors of the DMS charges and their forces from the reference's own code.
 
 Run once, in a container that holds the reference tree: ``make -C oracle ref`` compiles
-qdetermine.f90 with the modules and helpers it needs (module.f90, multibead.f90, error.f,
-invert.f, images.f) where they lie, with amdflang, plus the harness oracle/ref_dms.f90,
+qdetermine.f90 and qdforce.f90 with the modules and helpers they need (module.f90,
+multibead.f90, error.f, invert.f, images.f, gdsum.f, heatcurrent.f90) where they lie, with
+amdflang in the reference's serial mode, plus the harness oracle/ref_dms.f90,
 into oracle/_ref/libdms_ref.so.  Output: tests/golden/dms_reference.npz, inputs and the
 reference's outputs only:
 
@@ -15,6 +16,15 @@ reference's outputs only:
   dp     (nset, nw, 6)    dp1/dr1, dp1/dr2, dp1/dcos, dp2/dr1, dp2/dr2, dp2/dcos
   moldip (nset, nw, 3)    the reference's molecular dipole of the assigned charges
   chkdip (nset, nw, 3)    its PS dipole p1 r1 + p2 r2 (qdetermine.f90:262-305)
+
+and the reference's qdforce (qdforce.f90, built with HEAT_CURRENT and without HEAT_STRESS as
+the reference's Makefile sets it) for 9 sets of 8 waters with random site potentials:
+
+  qd_xyz (nset, 32, 3), qd_cell (nset, 9), qd_imcon, qd_gamma, qd_vesp (nset, 32)
+  qd_f      (nset, 32, 3)    charge-derivative forces at the sites (M not redistributed)
+  qd_stress (nset, 9), qd_vir (nset,)
+  qd_sqa    (nset, 9)        sum over atoms of atomic_stress_qd
+  qd_fm     (nset, 3, 32, 32) force_matrix(i, j, k) stored [k][j][i]
 
 Geometries: waters with O-H lengths in [0.75, 1.35] A and HOH angles in [70, 150] deg,
 random orientations, placed so that some hydrogens sit across the cell boundary (their
@@ -94,8 +104,26 @@ def main():
             for k, v in (("xyz", xyz), ("cell", c), ("imcon", imcon), ("gamma", gamma), ("chge", chge),
                          ("dp", dp), ("moldip", md), ("chkdip", cd)):
                 out[k].append(v)
+    lib.ref_qd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double] + [_dp] * 7
+    nq = 8
+    qd = {k: [] for k in ("qd_xyz", "qd_cell", "qd_imcon", "qd_gamma", "qd_vesp", "qd_f", "qd_stress", "qd_vir",
+                          "qd_sqa", "qd_fm")}
+    for gamma in (0.426706882, 0.2, 0.65):
+        for imcon, cell in cells:
+            xyz = np.ascontiguousarray(water_set(rng, nq, cell, gamma))
+            c = np.asarray(cell, float)
+            n = 4 * nq
+            vesp = rng.normal(size=n) * 50.0
+            f, st, sqa, fm = np.zeros((n, 3)), np.zeros(9), np.zeros(9), np.zeros((3, n, n))
+            vir = ctypes.c_double()
+            lib.ref_qd(nq, imcon, P(c), gamma, P(xyz), P(vesp), P(f), P(st), ctypes.byref(vir), P(sqa), P(fm))
+            for k, v in (("qd_xyz", xyz), ("qd_cell", c), ("qd_imcon", imcon), ("qd_gamma", gamma),
+                         ("qd_vesp", vesp), ("qd_f", f), ("qd_stress", st), ("qd_vir", vir.value), ("qd_sqa", sqa),
+                         ("qd_fm", fm)):
+                qd[k].append(v)
+    out.update(qd)
     np.savez_compressed(os.path.join(HERE, "dms_reference.npz"), **{k: np.array(v) for k, v in out.items()})
-    print("wrote", len(out["xyz"]), "sets of", nw, "waters")
+    print("wrote", len(out["xyz"]), "sets of", nw, "waters and", len(qd["qd_f"]), "qdforce sets of", nq)
 
 
 if __name__ == "__main__":

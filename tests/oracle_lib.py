@@ -167,7 +167,8 @@ class ElecArgs(ctypes.Structure):
                 ("stress_ew1p", ctypes.c_double * 9),
                 ("e_ew1p", _dp), ("e_ew2p", _dp), ("e_ew3p", _dp), ("e_polar", _dp), ("s_ew1p", _dp),
                 ("heat_w", _dp), ("force_matrix", _dp), ("npairs", ctypes.c_long), ("niter", ctypes.c_int),
-                ("last_delta", ctypes.c_double), ("volm", ctypes.c_double)]
+                ("last_delta", ctypes.c_double), ("volm", ctypes.c_double),
+                ("stress_qd_atoms", ctypes.c_double * 9)]
 
 
 _dms_loaded = False
@@ -216,6 +217,28 @@ def dms_p(w9, imcon=1, cell=None):
     p = np.zeros(8)
     assert lib.ora_dms_p(imcon, P(cell), P(np.ascontiguousarray(w9, float)), P(p)) == 0
     return p[0], p[1], p[2:5].copy(), p[5:8].copy()
+
+
+def qdforce(xyz, vesp, gamma, imcon, cell, compat_qd_column=1):
+    """The oracle's qdforce alone for given sites (4*nw, M sites as given) and potentials:
+    (forces (n, 3), stress[9], virial, sum of the angular per-atom stresses [9], force
+    matrix [3][n][n] with the M rows moved onto O, H1, H2)."""
+    lib = elec_lib()
+    xyz = np.asarray(xyz, float)
+    n = xyz.shape[0]
+    a = ElecArgs()
+    a.imcon = imcon
+    for q in range(9):
+        a.cell[q] = cell[q]
+    a.nw = n // 4
+    x, y, z = (np.ascontiguousarray(xyz[:, k]) for k in range(3))
+    a.x, a.y, a.z = P(x), P(y), P(z)
+    a.gamma, a.compat_qd_column = gamma, compat_qd_column
+    v = np.ascontiguousarray(vesp, float)
+    f, fm = np.zeros(3 * n), np.zeros(3 * n * n)
+    a.vesp, a.f, a.force_matrix = P(v), P(f), P(fm)
+    assert lib.ora_qdforce(ctypes.byref(a)) == 0
+    return (f.reshape(n, 3), np.array(a.stress[:]), a.vir_qd, np.array(a.stress_qd_atoms[:]), fm.reshape(3, n, n))
 
 
 def site_xyz(box):
@@ -267,7 +290,8 @@ def elec_forces(box, p, dip0=None, dense=False, xyz=None, cell=None, imcon=None,
                eng=a.eng_cpe, vir=a.vir_cpe, vir_ew1p=a.vir_ew1p, vir_qd=a.vir_qd, stress=np.array(a.stress[:]),
                stress_ew1p=np.array(a.stress_ew1p[:]),
                e_sites=np.stack([e1, e2, e3, ep]), s_ew1p_sites=s1.reshape(n, 9).copy(), heat_w=hw.reshape(3 * nw, 3, 3),
-               npairs=a.npairs, niter=a.niter, last_delta=a.last_delta, xyz=np.stack([x, y, z], 1))
+               npairs=a.npairs, niter=a.niter, last_delta=a.last_delta, xyz=np.stack([x, y, z], 1),
+               stress_qd_atoms=np.array(a.stress_qd_atoms[:]))
     out["f_ew1p_sites"] = fk.reshape(n, 3).copy()
     if distribute:
         e = (e1 + e2 + e3 + ep).copy()

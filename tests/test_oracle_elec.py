@@ -128,6 +128,36 @@ def test_dms_matches_the_reference_qdetermine(dms_ref):
             assert np.abs(np.concatenate([g1, g2]) - d["dp"][s][i]).max() <= 1e-13 * np.abs(d["dp"][s][i]).max()
 
 
+def test_qdforce_matches_the_reference_qdforce(dms_ref):
+    """Forces, stress, virial, per-atom (angular) stress sums and force-matrix entries of the
+    charge derivatives against qdforce.f90 itself (compiled from the reference's sources with
+    HEAT_CURRENT, without HEAT_STRESS), for random site potentials: 9 sets of 8 waters in
+    three cells with three gamma.  The force-matrix columns include the reference's M-site
+    column n_water + imol (qdforce.f90:332-337); the reference's M rows are moved onto O, H1
+    and H2 with 1 - gamma, gamma/2, gamma/2 here (distribute_forces, heatcurrent.f90:541-555),
+    as the oracle stores them."""
+    d = dms_ref
+    for s in range(len(d["qd_f"])):
+        gamma, imcon, cell = float(d["qd_gamma"][s]), int(d["qd_imcon"][s]), d["qd_cell"][s]
+        f, st, vir, sqa, fm = o.qdforce(d["qd_xyz"][s], d["qd_vesp"][s], gamma, imcon, cell)
+        fr = d["qd_f"][s]
+        assert np.abs(f - fr).max() <= 1e-12 * np.abs(fr).max(), s
+        assert np.abs(st - d["qd_stress"][s]).max() <= 1e-12 * np.abs(d["qd_stress"][s]).max(), s
+        assert abs(vir - d["qd_vir"][s]) <= 1e-12 * abs(d["qd_vir"][s]), s
+        assert np.abs(sqa - d["qd_sqa"][s]).max() <= 1e-12 * np.abs(d["qd_sqa"][s]).max(), s
+        n = fr.shape[0]
+        nw = n // 4
+        ref = d["qd_fm"][s].copy()
+        for w in range(nw):
+            m = 3 * nw + w
+            for t, wt in ((0, 1 - gamma), (1, gamma / 2), (2, gamma / 2)):
+                ref[:, :, 3 * w + t] += wt * ref[:, :, m]
+            ref[:, :, m] = 0.0
+        assert np.abs(fm - ref).max() <= 1e-12 * np.abs(ref).max(), s
+        # the quirk is exercised: the M-site derivatives land in columns nw..2nw-1
+        assert np.abs(d["qd_fm"][s][:, nw:2 * nw, :]).max() > 0
+
+
 def test_dms_molecular_dipole_is_the_ps_dipole(dms_ref):
     """qdetermine's own check (qdetermine.f90:262-305): the charges on H1, H2 and the M
     site, M = O + gamma/2 (r1 + r2), carry the molecular dipole p1 r1 + p2 r2 of the PS
