@@ -332,7 +332,7 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     // block partials of the final passes
     const int nsb = (E.n + 255) / 256, nkb = (K + 255) / 256, nwb = (S.nw + 63) / 64;
     dfree(E.blk);
-    E.nblk = nsb + 8 * nkb + 2 * nsb + 8 * E.n + 16 * nwb + 2 * nsb + 64;
+    E.nblk = nsb + 8 * nkb + 2 * nsb + 8 * E.n + 24 * nwb + 2 * nsb + 64;
     ECHK(dalloc(&E.blk, (size_t)E.nblk));
     ECHK(hipMemcpy(S.rcell, rc, sizeof(double) * 9, hipMemcpyHostToDevice));
     S.P.rvolm = twopi / volm;
@@ -523,7 +523,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         err = std::string("electrostatics final kernels failed: ") + hipGetErrorString((hipError_t)rc);
         return rc;
     }
-    ECHK(hipMemcpyAsync(S.scal, E.scal, sizeof(double) * 35, hipMemcpyDeviceToHost, s));
+    ECHK(hipMemcpyAsync(S.scal, E.scal, sizeof(double) * 41, hipMemcpyDeviceToHost, s));
     ECHK(hipStreamSynchronize(s));
     const double* q = S.scal;
     const double engsic = -R4PIE0 * P.alpha * q[9] / SQRPI;

@@ -1860,7 +1860,9 @@ __global__ __launch_bounds__(64) void k_el_qd(ElecDev P, const double* __restric
                                               double* __restrict__ w3, double* __restrict__ blk)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // [0..7] energy (unused), virial, stress6; [8..13] the angular terms of the per-atom qd
+    // stresses (qdforce.f90:504-531, the ones recorded without HEAT_STRESS), summed
+    double red[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (w < P.nw) {
         const int io = 3 * w, ih1 = io + 1, ih2 = io + 2, im = P.nreal + w;
         const int site[4] = {io, ih1, ih2, im};
@@ -1944,10 +1946,11 @@ __global__ __launch_bounds__(64) void k_el_qd(ElecDev P, const double* __restric
                 dc[t] = dcosdr12 * rv[0][sa[t]] * rv[0][sb[t]] / rr1 + dcosdr13 * rv[1][sa[t]] * rv[1][sb[t]] / rr2 +
                         dcosdr23 * r3v[sa[t]] * r3v[sb[t]] / rr3;
             for (int t = 0; t < 6; ++t) red[2 + t] -= vt * dqdcos[i] * dc[t];
+            for (int t = 0; t < 6; ++t) red[8 + t] += vt * dqdcos[i] * dc[t];
             red[1] += vt * (dc[0] + dc[3] + dc[5]) * dqdcos[i];
         }
     }
-    block_sum_store(red, blk + 8 * blockIdx.x);
+    block_sum_store(red, blk + 16 * blockIdx.x);
 }
 
 // Site forces and the per-real-atom outputs of the heat-current decomposition, with the
@@ -1993,8 +1996,8 @@ __global__ void k_el_combine(ElecDev P, int distribute, const double* __restrict
 // in sv[]: thread t adds blocks t, t + 256, ..., then the wavefront sums and the four wave
 // sums are added in order.
 struct SumSets {
-    const double* blk[6];
-    int nb[6], stride[6], nv[6], off[6];
+    const double* blk[7];
+    int nb[7], stride[7], nv[7], off[7];
     int nsv;
     unsigned char sv[64][2];   // (set, value) of each block
 };
@@ -2151,7 +2154,7 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     double* b_q2 = b_ks + 8L * nkb;              // [nsb][2]
     double* b_p2 = b_q2 + 2L * nsb;              // [n][8]
     double* b_p3 = b_p2 + 8L * P.n;              // [nwb][8]
-    double* b_qd = b_p3 + 8L * nwb;              // [nwb][8]
+    double* b_qd = b_p3 + 8L * nwb;              // [nwb][16]
     hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
     // structure factors of the converged dipoles (already computed with the refreshed fields)
     if (!refresh) {
@@ -2175,18 +2178,20 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     hipLaunchKernelGGL(k_el_combine, dim3(nwb), dim3(64), 0, s, P, distribute, E.x, E.y, E.z, E.fk, E.f2, E.f3, E.e1,
                        E.e2, E.e3, E.ep, E.s1, E.w2, E.w3, E.fmsum, E.fmx, E.fsite, E.eout, E.sout, E.wout);
     // scalars: [0] eng_polar, [1..8] k sums (engc, engdum, omg6), [9..10] sum q^2,
-    // [11..18] pairs (energy, virial, stress6), [19..26] excluded pairs, [27..34] qd
+    // [11..18] pairs (energy, virial, stress6), [19..26] excluded pairs, [27..34] qd,
+    // [35..40] angular per-atom qd stresses
     SumSets ss{};
-    const double* sb[6] = {b_pol, b_ks, b_q2, b_p2, b_p3, b_qd};
-    const int snb[6] = {nsb, nkb, nsb, P.n, nwb, nwb}, sst[6] = {1, 8, 2, 8, 8, 8}, soff[6] = {0, 1, 9, 11, 19, 27};
+    const double* sb[7] = {b_pol, b_ks, b_q2, b_p2, b_p3, b_qd, b_qd + 8};
+    const int snb[7] = {nsb, nkb, nsb, P.n, nwb, nwb, nwb}, sst[7] = {1, 8, 2, 8, 8, 16, 16};
+    const int snv[7] = {1, 8, 2, 8, 8, 8, 6}, soff[7] = {0, 1, 9, 11, 19, 27, 35};
     ss.nsv = 0;
-    for (int q = 0; q < 6; ++q) {
+    for (int q = 0; q < 7; ++q) {
         ss.blk[q] = sb[q];
         ss.nb[q] = snb[q];
         ss.stride[q] = sst[q];
-        ss.nv[q] = sst[q];
+        ss.nv[q] = snv[q];
         ss.off[q] = soff[q];
-        for (int v = 0; v < sst[q]; ++v) {
+        for (int v = 0; v < snv[q]; ++v) {
             ss.sv[ss.nsv][0] = (unsigned char)q;
             ss.sv[ss.nsv][1] = (unsigned char)v;
             ++ss.nsv;

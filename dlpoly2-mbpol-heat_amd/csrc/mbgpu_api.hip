@@ -122,6 +122,7 @@ struct mbgpu_ctx {
     long nd = 0, nt = 0;
     CellGeom geom{};
     bool have_inputs = false;
+    bool mb_ready = false;            // the last MB-pol evaluation completed
     // heat-flux scratch
     double *dv = nullptr, *dek = nullptr, *dep = nullptr, *dw = nullptr, *dsew = nullptr, *dout = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
@@ -480,9 +481,12 @@ int mbgpu_mbpol_forces_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nt
     B.y = const_cast<double*>(y_dev);
     B.z = const_cast<double*>(z_dev);
     c->have_inputs = true;
+    c->mb_ready = false;
     if (build_lists(c)) return -1;
     if (run_clusters(c, fx_dev, fy_dev, fz_dev)) return -1;
-    return finish_scalars(c, engsrp, virsrp, stress);
+    if (finish_scalars(c, engsrp, virsrp, stress)) return -1;
+    c->mb_ready = true;
+    return 0;
 }
 
 int mbgpu_mbpol_forces(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2,
@@ -798,6 +802,32 @@ int mbgpu_get_water_details(mbgpu_ctx* c, double* e_real, double* w_real, double
             for (int k = 0; k < 3; ++k) f_real[3 * i + k] = f[(size_t)k * nr + i];
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mbgpu_heat_check(mbgpu_ctx* c, const double* e_kin_dev, double out[12])
+{
+    if (!c) return fail(-1, "null context");
+    if (!out) return fail(-1, "null output");
+    const int na = c->B.natms, nr = 3 * c->B.nw;
+    for (int k = 0; k < 12; ++k) out[k] = 0.0;
+    std::vector<double> buf((size_t)std::max(na, 4 * c->B.nw));
+    auto sum_dev = [&](const double* p, int n, double& acc) -> int {
+        HIPCHK(hipMemcpyAsync(buf.data(), p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < n; ++i) acc += buf[i];
+        return 0;
+    };
+    if (e_kin_dev && sum_dev(e_kin_dev, nr, out[0])) return -1;
+    if (c->mb_ready && sum_dev(c->B.atomic_energy, na, out[1])) return -1;
+    if (c->el.ready) {
+        if (sum_dev(c->el.E.eout, nr, out[1])) return -1;
+        if (sum_dev(c->el.E.e1, 4 * c->B.nw, out[2])) return -1;
+        const double* q = c->el.scal + 35;   // xx, xy, xz, yy, yz, zz
+        const double s9[9] = {q[0], q[1], q[2], q[1], q[3], q[4], q[2], q[4], q[5]};
+        for (int k = 0; k < 9; ++k) out[3 + k] = s9[k];
+    }
+    if (c->wt.ready && sum_dev(c->wt.B.eout, nr, out[1])) return -1;
     return 0;
 }
 

@@ -79,6 +79,7 @@ _SIGS = {
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, _dp, _dp, _dp]),
     "mbgpu_get_water_details": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "mbgpu_heat_check": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp]),
 }
 
 
@@ -404,6 +405,13 @@ class Context:
         e, w, f = np.zeros(nr), np.zeros(9 * nr), np.zeros(3 * nr)
         _check(load_library().mbgpu_get_water_details(self._h, _p(e), _p(w), _p(f)), "mbgpu_get_water_details")
         return dict(atomic_energy=e, heat_w=w.reshape(nr, 3, 3), f=f.reshape(nr, 3))
+
+    def heat_check(self, ekin_ptr=None):
+        """write_check's invariants (heat_check.f90:13-49): (sum e_kin, sum e_pot, sum e_ew1p,
+        sum of the per-atom stresses minus the reciprocal ones as a 3x3), internal units."""
+        out = np.zeros(12)
+        _check(load_library().mbgpu_heat_check(self._h, ekin_ptr, _p(out)), "mbgpu_heat_check")
+        return out[0], out[1], out[2], out[3:].reshape(3, 3)
 
     def verlet_stage(self, stage, dt, mass, x, y, z, vx, vy, vz, fx=None, fy=None, fz=None, fsite=None, ekin=None):
         """mbgpu_verlet_stage with device pointers; returns the kinetic energy (stage 2)."""

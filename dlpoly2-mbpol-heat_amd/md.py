@@ -58,6 +58,27 @@ class Heatflux2Writer:
             f.write(heatflux2_record(time, jkin, jpot, jtot))
 
 
+def check_record(time, tot_kin, tot_pot, stress9, engunit=418.4, strsunit=1.0):
+    """One record of write_check's CHECK_ENERGY_STRESS file (heat_check.f90:33-35): the time
+    (es12.5), total kinetic and potential energy / engunit (2es15.7) and the 9 components of
+    total minus reciprocal-space per-atom stress * strsunit (9es15.7).  The inputs are those
+    mbgpu_heat_check returns (internal units)."""
+    s9 = [float(v) * strsunit for v in np.asarray(stress9).reshape(-1)]
+    return (fortran_es(time, 12, 5) + "\n" + fortran_es(tot_kin / engunit, 15, 7) + fortran_es(tot_pot / engunit, 15, 7)
+            + "\n" + "".join(fortran_es(v, 15, 7) for v in s9) + "\n")
+
+
+class CheckWriter:
+    """Appends check_record lines to a CHECK_ENERGY_STRESS file."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def write(self, time, tot_kin, tot_pot, stress9, engunit=418.4, strsunit=1.0):
+        with open(self.path, "a") as f:
+            f.write(check_record(time, tot_kin, tot_pot, stress9, engunit, strsunit))
+
+
 class NVE:
     def __init__(self, slot, dt_fs):
         import torch

@@ -215,6 +215,13 @@ module mbgpu
       integer(c_int) :: mbgpu_get_water_details
     end function
 
+    function mbgpu_heat_check(ctx, e_kin_dev, out) bind(c, name="mbgpu_heat_check")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, e_kin_dev
+      real(c_double) :: out(12)
+      integer(c_int) :: mbgpu_heat_check
+    end function
+
     ! ---- integrator, Green-Kubo, ring polymer, particle-mesh Ewald ----
     function mbgpu_verlet_stage(ctx, stage, dt, mass_dev, x_dev, y_dev, z_dev, vx_dev, vy_dev, vz_dev, &
                                 fx_dev, fy_dev, fz_dev, fsite_dev, ekin_dev, ekin_total) &

@@ -202,6 +202,23 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[
  *   f_site: 3 per site, forces of the last call. */
 int mbgpu_get_elec_details(mbgpu_ctx* ctx, double* scal, double* chge, double* vesp, double* e_real,
                            double* w_real, double* s_real, double* f_site);
+/* The invariants write_check prints to CHECK_ENERGY_STRESS (heat_check.f90:13-49, called
+ * under HEAT_CHECK), from the per-atom arrays of the last evaluations in this context
+ * (MB-pol, electrostatics and water terms, each included when its last call completed):
+ *   out[0]     sum of e_kin_dev over the 3*nw real atoms (total_kinetic_energy; e_kin_dev
+ *              may be NULL, then 0);
+ *   out[1]     sum of the per-atom potential energies (total_potential_energy,
+ *              potential_energy_per_atom, heatcurrent.f90:282-290);
+ *   out[2]     sum of the per-atom reciprocal-space energies (total_energy_ew1p);
+ *   out[3..11] sum of the per-atom stresses minus those of the reciprocal space
+ *              (total_stress - total_stress_ew1p), row-major 3x3.  With the reference's
+ *              build flags (HEAT_CURRENT and HEAT_CHECK without HEAT_STRESS) the only other
+ *              per-atom stresses recorded are the angular terms of the DMS charge
+ *              derivatives (qdforce.f90:504-531), so this is their sum.
+ * Internal units; the caller divides by engunit and multiplies by strsunit as write_check
+ * does.  No kernels: device-to-host copies of the per-atom arrays, summed on the host. */
+int mbgpu_heat_check(mbgpu_ctx* ctx, const double* e_kin_dev, double out[12]);
+
 /* One velocity-Verlet stage on the 3*nw real atoms (device arrays, internal units: amu,
  * Angstrom, ps, 10 J/mol).  The reference integrates with the Verlet leapfrog of nve_0
  * (nve_0.f:50-110), which gives the same positions; this form has the full-step velocities

@@ -55,6 +55,15 @@ def test_slot_matches_oracle_sum(pkg, waterbox, engine):
     s[:nr] = re["s_ew1p"]
     _, _, jt = o.heat_flux_tensor(box, hw.reshape(-1), vel, ek, e, s.reshape(-1).copy())
     assert np.abs(r["jtot"] - jt).max() <= 1e-10 * np.abs(jt).max()
+    # write_check's invariants (heat_check.f90:13-49) from the same per-atom arrays
+    tk, tp, te1, s9 = slot.ctx.heat_check(slot.ek.data_ptr())
+    assert abs(tk - ek[:nr].sum()) <= 1e-12 * ek[:nr].sum()
+    assert abs(tp - e.sum()) <= 1e-10 * scale
+    e1 = re["e_sites"][0].sum()
+    assert abs(te1 - e1) <= 1e-10 * abs(e1)
+    sq = re["stress_qd_atoms"].reshape(3, 3)
+    assert np.abs(s9 - sq).max() <= 1e-10 * np.abs(sq).max()
+    assert np.abs(sq).max() > 0.0
 
 
 def test_concurrent_slots_equal_alone(waterbox, engine):

@@ -33,6 +33,17 @@ def test_record_layout(md):
     assert lines[3] == "  1.0000000E+03  2.0000000E+03  3.0000000E+03"
 
 
+def test_check_record_layout(md):
+    """write_check's CHECK_ENERGY_STRESS record (heat_check.f90:33-35): es12.5, 2es15.7, 9es15.7."""
+    rec = md.check_record(0.0004, 418.4 * 2.0, -418.4 * 30.5, [[1.5, 0, -2], [0, 3, 0], [-2, 0, 4.25]], strsunit=2.0)
+    lines = rec.splitlines()
+    assert lines[0] == " 4.00000E-04"
+    assert lines[1] == "  2.0000000E+00 -3.0500000E+01"
+    assert lines[2].split() == ["3.0000000E+00", "0.0000000E+00", "-4.0000000E+00", "0.0000000E+00",
+                                "6.0000000E+00", "0.0000000E+00", "-4.0000000E+00", "0.0000000E+00", "8.5000000E+00"]
+    assert len(lines[2]) == 9 * 15
+
+
 def test_green_kubo_integral_of_an_exponential(md):
     """kappa_from_acf on C(t) = C0 exp(-t / tc): the running integral tends to C0 tc, and the
     unit chain (10 J/mol per particle, Angstrom/ps, Angstrom^3, K) gives the SI value
