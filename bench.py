@@ -32,7 +32,6 @@ dominant kernel (achieved vs peak FLOP/s, HBM traffic per launch) and a `cpu_bas
 object (the CPU path timed on this host).
 """
 import argparse
-import ctypes
 import importlib
 import json
 import os
@@ -46,29 +45,25 @@ METRIC = "force evals/s @512 H2O, mixed vs FP64 (same GPU, ΔF≤1e-5) + p99 ser
 FLOP_DIMER = 31600      # algorithmic FLOP per 2B dimer (value + gradient)
 FLOP_TRIMER = 113100    # algorithmic FLOP per 3B trimer (value + gradient)
 PEAK_FP64 = 78.6        # TFLOP/s, MI355X FP64 (vector and matrix peaks are equal)
+FLOP_SFAC = 12         # FLOP per (k vector, site) of the Ewald structure factors: the six multiply-adds of
+                       # mu_a cos(k.r) and mu_a sin(k.r) into S(k); the phase recurrence is not counted
 # HBM bytes per launch of k_el_sfac<false> at this workload, from rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE (KiB per dispatch) scaled by the same counters on a plain 512 MiB copy (FETCH x 2,
 # WRITE x 1); profiles/k_el_sfac_bench512_traffic.md
 SFAC_TRAFFIC_BYTES = (2903.7 * 2.0 + 20376.6 * 1.0) * 1024
 
 
-def cpu_baseline(box):
-    """The CPU restatement of mbpol_forces + compute_heat_flux2 on this host, one core.
-    When the reference's own compiled leaves are present (oracle/_ref) the polynomial
-    leaves are the reference's; the driver loop is the oracle's restatement."""
+def _cpu_eval(seed_and_budget):
+    """One or more complete 512-water evaluations by the oracle's C restatement on this
+    process's core; returns (evaluations, seconds)."""
+    seed, budget = seed_and_budget
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import numpy as np
     import oracle_lib
 
-    lib = oracle_lib.oracle()
-    ref = oracle_lib.reference_leaves()
-    kind = "port"
-    if ref is not None:
-        lib.ora_set_leaves(ctypes.cast(ref.mbpol_2b_poly_, ctypes.c_void_p),
-                           ctypes.cast(ref.mbpol_3b_poly_, ctypes.c_void_p))
-        kind = "reference"
-    ek = 0.5 * box.mass * (box.vel ** 2).sum(1)
     pkg = importlib.import_module("dlpoly2-mbpol-heat_amd")
+    box = pkg.waterbox.make_waterbox(512, 24.84, 8, seed)
+    ek = 0.5 * box.mass * (box.vel ** 2).sum(1)
     ep = pkg.elec_params.for_box(box.L)
     wp = pkg.water_params.for_box(box.L)
     nr = box.nreal
@@ -81,21 +76,50 @@ def cpu_baseline(box):
         hw[:nr] += el["heat_w"] + wt["heat_w"]
         e = r["atomic_energy"].copy()
         e[:nr] += el["atomic_energy"] + wt["atomic_energy"]
-        s = np.zeros((box.natms, 9))
-        s[:nr] = el["s_ew1p"]
-        oracle_lib.heat_flux_tensor(box, hw.reshape(-1), box.vel, ek, e, s.reshape(-1).copy())
+        sv = np.zeros((box.natms, 9))
+        sv[:nr] = el["s_ew1p"]
+        oracle_lib.heat_flux_tensor(box, hw.reshape(-1), box.vel, ek, e, sv.reshape(-1).copy())
         n += 1
         dt = time.perf_counter() - t0
-        if dt > 10.0 or n >= 8:
-            break
-    lib.ora_set_leaves(None, None)
-    what = "reference C++ leaves (oracle/_ref) under the oracle's mbpol_forces loop" if kind == "reference" \
-        else "oracle C restatement (extended-precision polynomial sums)"
-    return {"value": n / dt, "unit": "force evals/s", "cores": 1, "kind": kind,
-            "sample": f"{n} full 512-water evaluation(s): MB-pol 2B+3B (all-pairs / all-triples scan as "
-                      f"mbpol.f90, {what}) + the oracle's polarizable Ewald electrostatics (same parameters, "
-                      f"SOR from zero dipoles) + PS monomer, dispersion and long-range tail + decomposition + heat "
-                      f"flux, {dt:.1f} s"}
+        if dt > budget:
+            return n, dt
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cores):
+    """The oracle's C restatement of the whole evaluation (MB-pol all-pairs / all-triples
+    scan as mbpol.f90, Ewald electrostatics with SOR from zero dipoles as forces.f, water
+    terms, dense decomposition and heat flux), FP64, on this host's cores: one evaluation
+    on one core, then one evaluation per core on `cores` worker processes at once (each a
+    different configuration, as the GPU's batched leg).  Run before the GPU is touched, so
+    that the workers are plain forks."""
+    import multiprocessing as mp
+    n1, t1 = _cpu_eval((2, 0.0))
+    out = {"value": n1 / t1, "unit": "force evals/s", "cores": 1, "kind": "port",
+           "cpu_model": cpu_model(), "sample": f"{n1} complete 512-water evaluation(s) on one core, {t1:.1f} s"}
+    if cores > 1:
+        t0 = time.perf_counter()
+        with mp.get_context("fork").Pool(cores) as pool:
+            res = pool.map(_cpu_eval, [(100 + i, 0.0) for i in range(cores)])
+        wall = time.perf_counter() - t0
+        nall = sum(r[0] for r in res)
+        out["all_cores"] = {"value": nall / wall, "unit": "force evals/s", "cores": cores,
+                            "sample": f"{nall} complete 512-water evaluations, one per worker process at once, "
+                                      f"{wall:.1f} s wall"}
+    out["what"] = ("the oracle's C restatement of the reference's algorithms (MB-pol all-pairs / all-triples scan, "
+                   "plain Ewald with SOR from zero dipoles, water terms, dense force-matrix heat flux), FP64; the "
+                   "reference itself is not run on this host")
+    return out
 
 
 def spme_leg(pkg, torch, reps):
@@ -244,6 +268,12 @@ def main():
               "uses a single GPU; how the path would shard is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    cpu = None
+    if not args.no_cpu:
+        # before the GPU is touched: the all-core leg forks worker processes
+        ncpu = min(16, len(os.sched_getaffinity(0)))
+        print(f"cpu baseline: oracle restatement, 1 core then {ncpu} worker processes", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(ncpu)
     if args.batch > 4:
         # one hardware queue per concurrently evaluated configuration (HIP's default is 4);
         # must be set before the HIP runtime starts
@@ -310,8 +340,10 @@ def main():
     kvec = int(ctx.elec_details()["k_vectors"])
     reps = max(5, min(args.steps, 20))
     _, ms_k3 = ctx.time_forces(reps)
-    ms_sf, flop_sf = ctx.time_elec(reps)
+    ms_sf, flop_sf_lib = ctx.time_elec(reps)
     tf_k3 = nt * FLOP_TRIMER / (ms_k3 * 1e-3) / 1e12
+    # algorithmic work of one launch: FLOP_SFAC per (k vector, polarizable site)
+    flop_sf = FLOP_SFAC * kvec * box.nreal
     achieved = flop_sf / (ms_sf * 1e-3) / 1e12
     n_sfac = (int(np.median(iters)) + 2) if iters else None
     single = args.steps / dt
@@ -448,15 +480,19 @@ def main():
                    "single_ms_heat_flux": 1e3 * parts["heat"] / args.steps,
                    "k_pip3b_ms": ms_k3, "k_pip3b_tflops": tf_k3, "k_pip3b_frac": tf_k3 / PEAK_FP64,
                    "sfac_launches_per_step": n_sfac},
-        "roofline": {"bound": "mfma", "engine": "fp64 vector ALU (no dense contraction; MI355X FP64 vector and "
-                                                "matrix peaks are both 78.6 TFLOP/s)",
+        "roofline": {"bound": "fp64-valu", "engine": "FP64 vector ALU (no dense contraction, so no MFMA; peak "
+                                                     "78.6 TFLOP/s)",
                      "kernel": "k_el_sfac<false> (dipole structure factors, launched once per solver iteration)",
                      "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": SFAC_TRAFFIC_BYTES, "traffic_unit": "bytes per launch (rocprofv3 PMC)",
-                     "kernel_ms": ms_sf, "flop_per_launch": flop_sf},
+                     "traffic": SFAC_TRAFFIC_BYTES,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this workload, scaled by a "
+                                       "plain copy (profiles/k_el_sfac_bench512_traffic.md), bytes per launch",
+                     "kernel_ms": ms_sf, "flop_per_launch": flop_sf,
+                     "flop_per_unit": FLOP_SFAC, "units_per_launch": f"{kvec} k vectors x {box.nreal} sites",
+                     "frac_with_recurrence": flop_sf_lib / (ms_sf * 1e-3) / 1e12 / PEAK_FP64},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(box)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 
