@@ -35,6 +35,7 @@ import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -101,8 +102,8 @@ def cpu_baseline(cores):
     scan as mbpol.f90, Ewald electrostatics with SOR from zero dipoles as forces.f, water
     terms, dense decomposition and heat flux), FP64, on this host's cores: one evaluation
     on one core, then one evaluation per core on `cores` worker processes at once (each a
-    different configuration, as the GPU's batched leg).  Run before the GPU is touched, so
-    that the workers are plain forks."""
+    different configuration, as the GPU's batched leg).  Runs in a child process of the
+    benchmark that never touches the GPU, so that its workers are plain forks."""
     import multiprocessing as mp
     n1, t1 = _cpu_eval((2, 0.0))
     out = {"value": n1 / t1, "unit": "force evals/s", "cores": 1, "kind": "port",
@@ -229,6 +230,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-only", type=int, default=0, metavar="CORES",
+                    help="only run the CPU baseline on CORES worker processes and print it (used by the benchmark "
+                         "itself, in a child process)")
     ap.add_argument("--solver", choices=["sor", "cg"], default="cg",
                     help="induced-dipole solver: the reference's SOR or conjugate gradients to the same test")
     ap.add_argument("--batch", type=int, default=16,
@@ -268,12 +272,10 @@ def main():
               "uses a single GPU; how the path would shard is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
-    cpu = None
-    if not args.no_cpu:
-        # before the GPU is touched: the all-core leg forks worker processes
-        ncpu = min(16, len(os.sched_getaffinity(0)))
-        print(f"cpu baseline: oracle restatement, 1 core then {ncpu} worker processes", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(ncpu)
+    if args.cpu_only:
+        # child process of a benchmark run: the CPU baseline alone, never touching the GPU
+        print(json.dumps(cpu_baseline(args.cpu_only)), flush=True)
+        return
     if args.batch > 4:
         # one hardware queue per concurrently evaluated configuration (HIP's default is 4);
         # must be set before the HIP runtime starts
@@ -491,8 +493,16 @@ def main():
                      "flop_per_unit": FLOP_SFAC, "units_per_launch": f"{kvec} k vectors x {box.nreal} sites",
                      "frac_with_recurrence": flop_sf_lib / (ms_sf * 1e-3) / 1e12 / PEAK_FP64},
     }
-    if cpu is not None:
-        out["cpu_baseline"] = cpu
+    if not args.no_cpu:
+        # in a child process that never touches the GPU (its all-core leg forks workers)
+        ncpu = min(16, len(os.sched_getaffinity(0)))
+        print(f"cpu baseline: oracle restatement, 1 core then {ncpu} worker processes", file=sys.stderr, flush=True)
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-only", str(ncpu)], capture_output=True,
+                           text=True, timeout=900)
+        lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+        if r.returncode != 0 or not lines:
+            raise RuntimeError(f"cpu baseline failed ({r.returncode}): {r.stderr[-2000:]}")
+        out["cpu_baseline"] = json.loads(lines[-1])
     print(json.dumps(out), flush=True)
 
 

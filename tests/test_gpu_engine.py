@@ -6,7 +6,8 @@
   every result is bit-identical to the same slot evaluated alone.
 * The electrostatics on the context's second stream from a helper thread, beside MB-pol and
   the water terms (Slot overlap), give bit-identical results to the three parts in order.
-* The FP32-polynomial mode stays within its measured error envelope against FP64.
+* The FP32-polynomial mode against the mixed-precision bar (expected to fail: see
+  FP32_BAR_REASON), and its FP64 parts unchanged.
 * Electrostatics neighbour rows that start too small (MBGPU_ELEC_ROWS): the evaluation
   grows them and repeats from the same start dipoles, bit-identical to a normal context.
 """
@@ -98,18 +99,70 @@ def test_overlapped_parts_equal_ordered(waterbox, engine):
     b.close()
 
 
-def test_fp32_polynomials_error_envelope(waterbox, engine):
-    """FP32 polynomials: RMS force error 1e-2 kcal/mol/A class at 512 waters (the polynomial
-    conditioning makes 1e-5 unreachable in FP32); the FP64 parts are untouched."""
+def test_2048_whole_path_size_independent(pkg, waterbox, engine):
+    """The large-box workload at full size: 2048 waters on a 13^3 lattice (L = 39.43 A),
+    Ewald kmax 24 (33 530 k vectors), rcut 9 A, every term and the heat flux, checked by
+    what holds at any size: zero net force, per-atom energies summing to the energy (the
+    long-range tail has no per-atom share), virial = -trace(stress) (the tail's -virlrc/3
+    per diagonal is the caller's), heat flux linear in the velocities, the dipole solver
+    converged, and a second evaluation bit-identical."""
+    import torch
+    box = waterbox.box_2048()
+    slot = engine.Slot(box)
+    print("2048 waters, whole path", flush=True)
+    r = slot.evaluate()
+    assert int(slot.ctx.elec_details()["k_vectors"]) == 33530
+    f = slot.forces()
+    assert np.abs(f.sum(0)).max() <= 1e-9 * np.abs(f).max()
+    tk, tp, te1, _ = slot.ctx.heat_check(slot.ek.data_ptr())
+    ew = r["eng_water"]
+    assert abs(tp - (r["eng"] - ew[3])) <= 1e-10 * abs(r["eng_elec"])
+    vw = r["vir_water"]
+    tr = r["stress"][0] + r["stress"][4] + r["stress"][8]
+    assert abs(tr + (r["vir"] - vw[3])) <= 1e-10 * abs(r["vir"])
+    assert 0 < r["dipole_iterations"] < 200
+    # J is linear in v for fixed per-atom kinetic energies
+    v2 = 2.0 * slot.v
+    torch.cuda.synchronize()
+    jk, jp, jt = slot.ctx.heat_flux_dev(v2[0].data_ptr(), v2[1].data_ptr(), v2[2].data_ptr(), slot.ek.data_ptr())
+    assert np.abs(np.array(jt) - 2.0 * r["jtot"]).max() <= 1e-13 * np.abs(r["jtot"]).max()
+    r2 = slot.evaluate()
+    assert r2["eng"] == r["eng"] and np.array_equal(r2["jtot"], r["jtot"]) and np.array_equal(r2["stress"], r["stress"])
+    assert np.array_equal(slot.forces(), f)
+
+
+FP32_BAR_REASON = (
+    "FP32 polynomials miss the mixed-precision bar (RMS dF <= 1e-5 kcal/mol/A, |dJ|/|J| <= 1e-6) by about "
+    "10^3: the terms of the 2B and 3B polynomials cancel by about 10^6 (sum |terms| / |E|), so FP32 rounding "
+    "of the terms is amplified past the bar; re-centring the polynomials with the constant and linear terms "
+    "in FP64 gains less than 3x (oracle/precision_study.py)")
+
+
+@pytest.fixture(scope="module")
+def fp32_vs_fp64(waterbox, engine):
     box = waterbox.box_512()
     a, b = engine.Slot(box, precision=0), engine.Slot(box, precision=3)
     ra, rb = a.evaluate(), b.evaluate()
-    df = (b.forces() - a.forces()) / 418.4
+    return ra, rb, a.forces(), b.forces()
+
+
+@pytest.mark.xfail(strict=True, reason=FP32_BAR_REASON)
+def test_fp32_polynomials_meet_the_mixed_precision_bar(fp32_vs_fp64):
+    ra, rb, fa, fb = fp32_vs_fp64
+    df = (fb - fa) / 418.4
     rms = float(np.sqrt((df ** 2).mean()))
-    print(f"fp32 polynomials: rms dF {rms:.3e} kcal/mol/A", flush=True)
-    assert 0.0 < rms < 0.05
-    assert abs(rb["eng_elec"] - ra["eng_elec"]) <= 1e-6 * abs(ra["eng_elec"])
+    rel_dj = float(np.abs(rb["jtot"] - ra["jtot"]).max() / np.abs(ra["jtot"]).max())
+    print(f"fp32 polynomials: rms dF {rms:.3e} kcal/mol/A, rel dJ {rel_dj:.2e}", flush=True)
+    assert rms <= 1e-5 and rel_dj <= 1e-6
+
+
+def test_fp32_polynomials_leave_the_fp64_parts(fp32_vs_fp64):
+    """The FP32 mode changes only the polynomials: the electrostatics and the water terms,
+    which do not depend on them, are bit-identical; the MB-pol energy moves."""
+    ra, rb, fa, fb = fp32_vs_fp64
+    assert rb["eng_elec"] == ra["eng_elec"]
     assert np.array_equal(rb["eng_water"], ra["eng_water"])
+    assert rb["eng_mbpol"] != ra["eng_mbpol"]
 
 
 @pytest.mark.parametrize("solver", [0, 1])
