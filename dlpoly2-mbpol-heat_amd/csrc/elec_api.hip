@@ -27,8 +27,20 @@
 #ifndef EL_KCHUNK_RUNS
 #define EL_KCHUNK_RUNS 32     // at most this many k runs per chunk of the site-parallel passes
 #endif
+#ifndef EL_FCHUNK_WAVES
+#define EL_FCHUNK_WAVES 3072  // wavefronts the reciprocal field passes aim at
+#endif
+#ifndef EL_FCHUNK_RUNS
+#define EL_FCHUNK_RUNS 48     // at most this many k runs per chunk of the field passes
+#endif
+#ifndef EL_KG
+#define EL_KG 4               // run groups (wavefronts) per field block (elec_kernels.hip)
+#endif
 #ifndef EL_DCHUNK_WAVES
-#define EL_DCHUNK_WAVES 2048  // wavefronts the dipole structure-factor pass aims at
+#define EL_DCHUNK_WAVES 3072  // wavefronts the structure-factor passes aim at
+#endif
+#ifndef EL_SG
+#define EL_SG 4               // site groups (wavefronts) per structure-factor block (elec_kernels.hip)
 #endif
 #define PI_D 3.141592653589793
 
@@ -284,15 +296,18 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     ECHK(hipMemcpy(E.klmn, klmn.data(), sizeof(int) * klmn.size(), hipMemcpyHostToDevice));
     ECHK(hipMemcpy(E.runs, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice));
     ECHK(hipMemcpy(E.runk, runk.data(), sizeof(int) * runk.size(), hipMemcpyHostToDevice));
-    // site chunks of the structure-factor passes: up to 16 (their partials are re-read
-    // by the chunk sum), about 1024 wavefronts (4 per block of 128 runs)
-    const int rblocks = (EL_SPLIT * E.nrun + 255) / 256;
+    // site chunks of the structure-factor passes (their partials are re-read by the chunk
+    // sum): about EL_DCHUNK_WAVES wavefronts, EL_SG per block of 64 half-runs, and at least
+    // 64 sites per chunk
+    const int rblocks = (EL_SPLIT * E.nrun + 63) / 64;
     const int nrow = 3 * S.nw;
-    E.schunks = std::max(1, std::min(16, std::min((E.n + 15) / 16, 1024 / (4 * rblocks))));
-    E.dchunks = std::max(1, std::min(EL_DCHUNK_MAX, std::min((nrow + 15) / 16, EL_DCHUNK_WAVES / (4 * rblocks))));
-    // MBGPU_ELEC_WAVE_CHUNKS=1 keeps the wavefront rule alone for the k chunks below (a
-    // test aid that compares chunkings)
-    const bool wave_rules_only = getenv("MBGPU_ELEC_WAVE_CHUNKS") != nullptr;
+    E.schunks = std::max(1, std::min(16, std::min((E.n + 63) / 64, EL_DCHUNK_WAVES / (EL_SG * rblocks))));
+    E.dchunks = std::max(1, std::min(EL_DCHUNK_MAX, std::min((nrow + 63) / 64, EL_DCHUNK_WAVES / (EL_SG * rblocks))));
+    // MBGPU_ELEC_WAVE_CHUNKS=1 (any nonzero integer) keeps the wavefront rule alone for the k
+    // chunks of the final pass below (a test aid that compares chunkings); unset, empty or 0
+    // leaves the bound on
+    const char* wave_env = getenv("MBGPU_ELEC_WAVE_CHUNKS");
+    const bool wave_rules_only = wave_env && atoi(wave_env) != 0;
     E.spc = (E.n + E.schunks - 1) / E.schunks;
     E.dspc = (nrow + E.dchunks - 1) / E.dchunks;
     ECHK(dalloc(&E.S, (size_t)8 * K));
@@ -301,7 +316,7 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     ECHK(dalloc(&E.Sq, (size_t)2 * K));
     ECHK(dalloc(&E.kc, (size_t)16 * K));
     ECHK(dalloc(&E.kf, (size_t)16 * K));
-    ECHK(dalloc(&E.Spart, (size_t)std::max(E.schunks, E.dchunks) * K * 8));
+    ECHK(dalloc(&E.Spart, (size_t)std::max(8 * E.schunks, 6 * E.dchunks) * K));
     ECHK(hipMemset(E.S, 0, sizeof(double) * 8 * K));
     ECHK(hipMemset(E.kc, 0, sizeof(double) * 16 * K));
     // phase tables
@@ -327,8 +342,23 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
         const int k0 = runk[r0], k1 = r1 < E.nrun ? runk[r1] : K;
         E.kmaxch = std::max(E.kmaxch, k1 - k0);
     }
+    // k chunks of the field passes: about EL_FCHUNK_WAVES wavefronts (EL_KG per block of
+    // 128 sites), at most EL_FCHUNK_RUNS runs per chunk (their constants are staged in LDS)
+    {
+        const int sblocks = (nrow + 127) / 128;
+        E.fchunks = std::max(1, std::min(E.nrun, EL_FCHUNK_WAVES / (EL_KG * sblocks)));
+        E.fchunks = std::max(E.fchunks, std::min(E.nrun, (E.nrun + EL_FCHUNK_RUNS - 1) / EL_FCHUNK_RUNS));
+        E.frpc = (E.nrun + E.fchunks - 1) / E.fchunks;
+        E.fchunks = (E.nrun + E.frpc - 1) / E.frpc;
+        E.fkmax = 0;
+        for (int c = 0; c < E.fchunks; ++c) {
+            const int r0 = c * E.frpc, r1 = std::min(E.nrun, r0 + E.frpc);
+            const int k0 = runk[r0], k1 = r1 < E.nrun ? runk[r1] : K;
+            E.fkmax = std::max(E.fkmax, k1 - k0);
+        }
+    }
     dfree(E.Fpart);
-    ECHK(dalloc(&E.Fpart, (size_t)E.kchunks * 16 * E.n));
+    ECHK(dalloc(&E.Fpart, std::max((size_t)E.kchunks * 16, (size_t)E.fchunks * 6) * E.n));
     // block partials of the final passes
     const int nsb = (E.n + 255) / 256, nkb = (K + 255) / 256, nwb = (S.nw + 63) / 64;
     dfree(E.blk);
@@ -423,6 +453,9 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
     P.kstride = S.E.kstride;
     P.rpc = S.E.rpc;
     P.kmaxch = S.E.kmaxch;
+    P.fchunks = S.E.fchunks;
+    P.frpc = S.E.frpc;
+    P.fkmax = S.E.fkmax;
     P.scale = 4.0 * P.rvolm * R4PIE0;
     P.scal1 = 2.0 * P.rvolm * R4PIE0;
     P.self_dd = 4.0 * R4PIE0 * P.alpha * P.alpha * P.alpha / SQRPI / 3.0;

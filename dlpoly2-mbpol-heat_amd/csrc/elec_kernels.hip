@@ -503,12 +503,15 @@ __device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c,
 
 }  // namespace
 
-// Structure factors over a chunk of sites, k-parallel (ewald1p.f:342-438): each thread
-// owns half a run (up to EL_HALF consecutive k at fixed l, m) and steps exp(i k.r_j)
-// along n by a complex recurrence; a block stages the phase tables of its site chunk in
-// LDS, EL_TILE sites at a time (rows l = lmin..lmax of a*, every |m| row of b*, every |n|
-// row of c*, padded against bank conflicts).  Partials Spart[chunk][k][8] =
-// (sum q ckr, sum q skr, sum mu ckr (3), sum mu skr (3)), the dipole sums over
+// Structure factors over a chunk of sites, k-parallel (ewald1p.f:342-438).  A block owns
+// 64 half-runs (lane = half-run: up to EL_HALF consecutive k at fixed l, m, stepping
+// exp(i k.r_j) along n by a complex recurrence) and EL_SG site groups (one wavefront
+// each): the block stages the phase tables of its site chunk in LDS, EL_TILE sites at a
+// time (rows l = lmin..lmax of a*, every |m| row of b*, every |n| row of c*, padded against
+// bank conflicts), wavefront w takes the tile's sites w, w + EL_SG, ..., and the EL_SG
+// partial sums of every k are added in LDS in group order before one partial per chunk is
+// written: Spart[chunk][k][NV], NV = 8 with charges (sum q ckr, sum q skr, sum mu ckr (3),
+// sum mu skr (3)), NV = 6 for the dipoles alone (the last six).  The dipole sums run over
 // polarizable sites only.
 #ifndef EL_TILE
 #define EL_TILE 16
@@ -517,24 +520,30 @@ __device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c,
 #ifndef EL_SPLIT
 #define EL_SPLIT 2                    // threads per k run in the structure-factor pass
 #endif
+#ifndef EL_SG
+#define EL_SG 4                       // site groups (wavefronts) per structure-factor block
+#endif
 #define EL_HALF (EL_RUN / EL_SPLIT)
 template <bool CHG>
-__global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* __restrict__ runs,
-                                                 const int* __restrict__ runk, const double* __restrict__ phc,
-                                                 const double* __restrict__ phs, const double* __restrict__ chge,
-                                                 const double* __restrict__ dip, double* __restrict__ Spart,
-                                                 const double* __restrict__ state)
+__global__ __launch_bounds__(64 * EL_SG) void k_el_sfac(ElecDev P, int spc, const int* __restrict__ runs,
+                                                        const int* __restrict__ runk, const double* __restrict__ phc,
+                                                        const double* __restrict__ phs, const double* __restrict__ chge,
+                                                        const double* __restrict__ dip, double* __restrict__ Spart,
+                                                        const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
     extern __shared__ double lds[];
-    const int tid = threadIdx.x;
-    const int g = blockIdx.x * 256 + tid;
+    constexpr int NT = 64 * EL_SG;
+    constexpr int NV = CHG ? 8 : 6;
+    constexpr int Q0 = CHG ? 0 : 2;          // first accumulator component written
+    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
+    const int g = blockIdx.x * 64 + lane;
     const int r = g / EL_SPLIT, half = g % EL_SPLIT;
     const int chunk = blockIdx.y;
     const int ns = CHG ? P.n : P.nrow;
     const int j0 = chunk * spc, j1 = min(ns, j0 + spc);
-    const int rfirst = (blockIdx.x * 256) / EL_SPLIT;
-    const int rlast = min(P.nrun - 1, (blockIdx.x * 256 + 255) / EL_SPLIT);
+    const int rfirst = (blockIdx.x * 64) / EL_SPLIT;
+    const int rlast = min(P.nrun - 1, (blockIdx.x * 64 + 63) / EL_SPLIT);
     const int lmin = runs[4 * rfirst], lmax = runs[4 * rlast];
     const int nl = lmax - lmin + 1, nm = P.km[1] + 1, nn = P.km[2] + 1;
     const int nrows = nl + nm + nn;
@@ -560,13 +569,13 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
         for (int q = 0; q < 8; ++q) acc[t][q] = 0.0;
     // tiles of EL_TILE sites, double-buffered: the next tile's table entries are loaded
     // into registers while the current tile is consumed from LDS
-    constexpr int PF = (80 * EL_TILE + 255) / 256;   // staged entries per thread (nrows <= 80)
+    constexpr int PF = (80 * EL_TILE + NT - 1) / NT;   // staged entries per thread (nrows <= 80)
     double pc[PF], ps[PF], pw = 0.0;
     auto fetch = [&](int t0) {
         const int tn = min(EL_TILE, j1 - t0);
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int idx = tid + 256 * u;
+            const int idx = tid + NT * u;
             const int row = idx / EL_TILE, sidx = idx - row * EL_TILE;
             pc[u] = 0.0;
             ps[u] = 0.0;
@@ -604,7 +613,7 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int idx = tid + 256 * u;
+            const int idx = tid + NT * u;
             const int row = idx / EL_TILE, sidx = idx - row * EL_TILE;
             if (row < nrows) tcs[row * EL_TROW + sidx] = make_double2(pc[u], ps[u]);
         }
@@ -614,7 +623,7 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
         if (cnt > 0) {
             // every thread steps EL_HALF k; terms past the end of its run are never stored
 #pragma unroll 2
-            for (int sj = 0; sj < tn; ++sj) {
+            for (int sj = grp; sj < tn; sj += EL_SG) {
                 const double2 A = tcs[rowA + sj], B = tcs[rowB + sj], C = tcs[rowC + sj], C1 = tcs[rowC1 + sj];
                 const double2 w01 = tw[2 * sj], w23 = tw[2 * sj + 1];
                 double ckr, skr;
@@ -639,13 +648,31 @@ __global__ __launch_bounds__(256) void k_el_sfac(ElecDev P, int spc, const int* 
             }
         }
     }
-    if (cnt == 0) return;
+    // the site groups' sums, added in group order through LDS (the tiles are done with it);
+    // every thread reaches every barrier
+    double* red = lds;                                   // [EL_SG - 1][64][NV]
+#pragma unroll
+    for (int t = 0; t < EL_HALF; ++t) {
+        __syncthreads();
+        if (grp > 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) red[((grp - 1) * 64 + lane) * NV + q] = acc[t][Q0 + q];
+        __syncthreads();
+        if (grp == 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                double v = acc[t][Q0 + q];
+                for (int w = 1; w < EL_SG; ++w) v += red[((w - 1) * 64 + lane) * NV + q];
+                acc[t][Q0 + q] = v;
+            }
+    }
+    if (grp != 0 || cnt == 0) return;
     const int k0 = runk[r] + EL_HALF * half;
 #pragma unroll
     for (int t = 0; t < EL_HALF; ++t)
         if (t < cnt)
 #pragma unroll
-            for (int q = 0; q < 8; ++q) Spart[((long)chunk * P.K + k0 + t) * 8 + q] = acc[t][q];
+            for (int q = 0; q < NV; ++q) Spart[((long)chunk * P.K + k0 + t) * NV + q] = acc[t][Q0 + q];
 }
 
 // Sum the chunk partials and derive the per-k constants of the field passes: kc[k][0..5] =
@@ -660,15 +687,19 @@ __global__ __launch_bounds__(256) void k_el_sreduce(ElecDev P, int schunks, cons
                                                     const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
-    const long e = (long)blockIdx.x * 64 + (threadIdx.x & 63);
-    const double v = chunk_sum64(Spart, 8L * P.K, schunks, e, e < 8L * P.K);
+    // NV values per k in the partials (see k_el_sfac); a block sums EPB consecutive
+    // elements, a whole number of k
+    constexpr int NV = CHG ? 8 : 6, EPB = CHG ? 64 : 60, Q0 = CHG ? 0 : 2;
+    const int lane = threadIdx.x & 63;
+    const long e = (long)blockIdx.x * EPB + lane;
+    const double v = chunk_sum64(Spart, (long)NV * P.K, schunks, e, lane < EPB && e < (long)NV * P.K);
     if (threadIdx.x >= 64) return;
-    const int lane = threadIdx.x, base = lane & ~7;
-    double s[8];
+    const int base = lane - lane % NV;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) s[q] = __shfl(v, base + q, 64);
-    const int k = (int)(e >> 3);
-    if ((lane & 7) != 0 || k >= P.K) return;
+    for (int q = 0; q < NV; ++q) s[Q0 + q] = __shfl(v, min(base + q, 63), 64);
+    const int k = (int)(e / NV);
+    if (lane % NV != 0 || lane >= EPB || k >= P.K) return;
     if (CHG) {
         Sq[2 * k] = s[0];
         Sq[2 * k + 1] = s[1];
@@ -697,60 +728,68 @@ __global__ __launch_bounds__(256) void k_el_sreduce(ElecDev P, int schunks, cons
 
 // Reciprocal-space fields over a chunk of runs, site-parallel (ewald1p.f:749-769):
 // efdd_a += c_a skr + d_a ckr (dipole field), with CHG also efdc_a += e_a skr + f_a ckr
-// (charge field).  A block of 256 sites stages the chunk's runs and per-k constants in
-// LDS.  Partials Fpart[chunk][6][n].
+// (charge field).  A block covers 128 sites (two per lane: j and j + 64) and EL_KG run
+// groups (one wavefront each, runs g, g + EL_KG, ... of the chunk); it stages the chunk's
+// runs and per-k constants in LDS, and the run groups' sums are added in LDS in group
+// order before one partial per chunk is written: Fpart[chunk][6][n] (components 0..2 the
+// dipole field, 3..5 the charge field).
+#ifndef EL_KG
+#define EL_KG 4                       // run groups (wavefronts) per field block
+#endif
 template <bool CHG>
-__global__ __launch_bounds__(256) void k_el_kfield(ElecDev P, const int* __restrict__ runs,
-                                                   const int* __restrict__ runk, const double* __restrict__ phc,
-                                                   const double* __restrict__ phs, const double* __restrict__ kc,
-                                                   double* __restrict__ Fpart, const double* __restrict__ state)
+__global__ __launch_bounds__(64 * EL_KG) void k_el_kfield(ElecDev P, const int* __restrict__ runs,
+                                                          const int* __restrict__ runk, const double* __restrict__ phc,
+                                                          const double* __restrict__ phs, const double* __restrict__ kc,
+                                                          double* __restrict__ Fpart, const double* __restrict__ state)
 {
     if (!CHG && state[2] != 0.0) return;
     extern __shared__ double lds[];
     constexpr int NC = CHG ? 12 : 6;
-    const int tid = threadIdx.x;
+    constexpr int NT = 64 * EL_KG;
+    const int tid = threadIdx.x, lane = tid & 63, grp = tid >> 6;
     const int chunk = blockIdx.y;
-    const int r0 = chunk * P.rpc, r1 = min(P.nrun, r0 + P.rpc);
+    const int r0 = chunk * P.frpc, r1 = min(P.nrun, r0 + P.frpc);
     const int nr = max(0, r1 - r0);
-    double* sk = lds;
-    int* sr = reinterpret_cast<int*>(lds + (long)P.kmaxch * 12);
+    double* sk = lds;                                          // [fkmax][NC]
+    int* sr = reinterpret_cast<int*>(lds + (long)P.fkmax * NC); // [frpc][4] runs
+    int* so = sr + 4 * P.frpc;                                  // [frpc] first k of each run - kbase
     const int kbase = nr ? runk[r0] : 0;
     const int nk = nr ? ((r1 < P.nrun ? runk[r1] : P.K) - kbase) : 0;
-    for (int idx = tid; idx < nk * NC; idx += 256) {
+    for (int idx = tid; idx < nk * NC; idx += NT) {
         const int kk = idx / NC, q = idx - kk * NC;
         sk[idx] = kc[16L * (kbase + kk) + q];
     }
-    for (int idx = tid; idx < nr * 4; idx += 256) sr[idx] = runs[4 * r0 + idx];
+    for (int idx = tid; idx < nr * 4; idx += NT) sr[idx] = runs[4 * r0 + idx];
+    for (int idx = tid; idx < nr; idx += NT) so[idx] = runk[r0 + idx] - kbase;
     __syncthreads();
-    // two sites per thread share every constant read: j and j + 256 of a 512-site block
+    // two sites per lane share every constant read; lanes past the last site compute on a
+    // clamped site and store nothing (every thread reaches the barriers below)
     const int nrow = CHG ? P.n : P.nrow;
-    const int ja = blockIdx.x * 512 + tid, jb = ja + 256;
-    if (ja >= nrow) return;
-    const bool hb = jb < nrow;
-    const int jB = hb ? jb : ja;
+    const int ja = blockIdx.x * 128 + lane, jb = ja + 64;
+    const bool ha = ja < nrow, hb = jb < nrow;
+    const int jA = ha ? ja : 0, jB = hb ? jb : jA;
     const int ks = P.kstride;
     const long o1 = ((long)2 * ks + 1) * P.n;
-    const double ca1 = phc[o1 + ja], sa1 = phs[o1 + ja], cb1 = phc[o1 + jB], sb1 = phs[o1 + jB];
+    const double ca1 = phc[o1 + jA], sa1 = phs[o1 + jA], cb1 = phc[o1 + jB], sb1 = phs[o1 + jB];
     double fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0}, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
-    int kk = 0;
     // the phase-table entries of a run are loaded one run ahead (double-buffered in
     // registers), so their latency overlaps the previous run's arithmetic
     double qa[6], qb[6];
     auto load_run = [&](int r) {
         const int l = sr[4 * r], m = sr[4 * r + 1], n0 = sr[4 * r + 2];
         const long oa = (long)l * P.n, ob = ((long)ks + abs(m)) * P.n, oc = ((long)2 * ks + abs(n0)) * P.n;
-        qa[0] = phc[oa + ja]; qa[1] = phs[oa + ja]; qa[2] = phc[ob + ja];
-        qa[3] = phs[ob + ja]; qa[4] = phc[oc + ja]; qa[5] = phs[oc + ja];
+        qa[0] = phc[oa + jA]; qa[1] = phs[oa + jA]; qa[2] = phc[ob + jA];
+        qa[3] = phs[ob + jA]; qa[4] = phc[oc + jA]; qa[5] = phs[oc + jA];
         qb[0] = phc[oa + jB]; qb[1] = phs[oa + jB]; qb[2] = phc[ob + jB];
         qb[3] = phs[ob + jB]; qb[4] = phc[oc + jB]; qb[5] = phs[oc + jB];
     };
-    if (nr > 0) load_run(0);
-    for (int r = 0; r < nr; ++r) {
-        const int m = sr[4 * r + 1], n0 = sr[4 * r + 2], cnt = sr[4 * r + 3];
+    if (grp < nr) load_run(grp);
+    for (int r = grp; r < nr; r += EL_KG) {
+        const int m = sr[4 * r + 1], n0 = sr[4 * r + 2], cnt = sr[4 * r + 3], kk = so[r];
         double cA, sA, cB, sB;
         run_phase(qa[0], qa[1], qa[2], qa[3], qa[4], qa[5], m, n0, cA, sA);
         run_phase(qb[0], qb[1], qb[2], qb[3], qb[4], qb[5], m, n0, cB, sB);
-        if (r + 1 < nr) load_run(r + 1);
+        if (r + EL_KG < nr) load_run(r + EL_KG);
 #pragma unroll
         for (int t = 0; t < EL_RUN; ++t) {
             if (t < cnt) {
@@ -792,15 +831,36 @@ __global__ __launch_bounds__(256) void k_el_kfield(ElecDev P, const int* __restr
                 cB = nb;
             }
         }
-        kk += cnt;
     }
+    // the run groups' sums, added in group order through LDS
+    constexpr int NQ = CHG ? 12 : 6;                           // values per lane
+    double v[12];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        v[q] = fa[q];
+        v[3 + q] = fb[q];
+        if (CHG) {
+            v[6 + q] = ga[q];
+            v[9 + q] = gb[q];
+        }
+    }
+    double* red = lds;                                         // [EL_KG - 1][64][NQ]
+    __syncthreads();
+    if (grp > 0)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) red[((grp - 1) * 64 + lane) * NQ + q] = v[q];
+    __syncthreads();
+    if (grp != 0) return;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        for (int w = 1; w < EL_KG; ++w) v[q] += red[((w - 1) * 64 + lane) * NQ + q];
     double* out = Fpart + (long)chunk * 6 * P.n;
     for (int q = 0; q < 3; ++q) {
-        out[(long)q * P.n + ja] = fa[q];
-        if (hb) out[(long)q * P.n + jb] = fb[q];
+        if (ha) out[(long)q * P.n + ja] = v[q];
+        if (hb) out[(long)q * P.n + jb] = v[3 + q];
         if (CHG) {
-            out[(long)(3 + q) * P.n + ja] = ga[q];
-            if (hb) out[(long)(3 + q) * P.n + jb] = gb[q];
+            if (ha) out[(long)(3 + q) * P.n + ja] = v[6 + q];
+            if (hb) out[(long)(3 + q) * P.n + jb] = v[9 + q];
         }
     }
 }
@@ -819,7 +879,7 @@ __global__ __launch_bounds__(256) void k_el_kfield_reduce(ElecDev P, const doubl
     const int t = blockIdx.x * 64 + (threadIdx.x & 63);
     const bool valid = t < nq * nrow;
     const int q = valid ? t / nrow : 0, j = valid ? t - q * nrow : 0;
-    const double sum = chunk_sum64(Fpart, 6L * P.n, P.kchunks, (long)q * P.n + j, valid);
+    const double sum = chunk_sum64(Fpart, 6L * P.n, P.fchunks, (long)q * P.n + j, valid);
     if (threadIdx.x >= 64 || !valid) return;
     if (q < 3) {
         const bool pol = site_polr2(P, j) > 1.e-6;
@@ -1190,7 +1250,7 @@ __global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __res
     const int t = blockIdx.x * 64 + (threadIdx.x & 63);
     const bool valid = t < 3 * P.nrow;
     const int q = valid ? t / P.nrow : 0, j = valid ? t - q * P.nrow : 0;
-    const double sum = chunk_sum64(Fpart, 6L * P.n, P.kchunks, (long)q * P.n + j, valid);
+    const double sum = chunk_sum64(Fpart, 6L * P.n, P.fchunks, (long)q * P.n + j, valid);
     double v[1] = {0.0};
     if (threadIdx.x < 64 && valid) {
         const double p2 = site_polr2(P, j);
@@ -2027,9 +2087,18 @@ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 // dynamic LDS of the reciprocal-space passes (bytes)
 inline size_t sfac_lds(const ElecDev& P)
 {
-    return sizeof(double) * (2 * EL_TROW * (size_t)(P.km[0] + P.km[1] + P.km[2] + 3) + 4 * EL_TILE);
+    // the tile of table rows and site weights, reused afterwards for the site-group sums
+    const size_t tile = 2 * EL_TROW * (size_t)(P.km[0] + P.km[1] + P.km[2] + 3) + 4 * EL_TILE;
+    const size_t red = (size_t)(EL_SG - 1) * 64 * 8;
+    return sizeof(double) * (tile > red ? tile : red);
 }
-inline size_t kfield_lds(const ElecDev& P) { return sizeof(double) * ((size_t)P.kmaxch * 12 + 2 * (size_t)P.rpc + 2); }
+inline size_t kfield_lds(const ElecDev& P)
+{
+    // per-k constants (12), runs (4 ints) and run offsets (1 int), then the run-group sums
+    const size_t stage = (size_t)P.fkmax * 12 + (5 * (size_t)P.frpc + 1) / 2 + 1;
+    const size_t red = (size_t)(EL_KG - 1) * 64 * 12;
+    return sizeof(double) * (stage > red ? stage : red);
+}
 inline size_t kfinal_lds(const ElecDev& P) { return sizeof(double) * ((size_t)P.kmaxch * EL_KF + 2 * (size_t)P.rpc + 2); }
 }
 
@@ -2051,11 +2120,11 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hi
     hipLaunchKernelGGL(k_el_phase, dim3(ceil_div(P.n, 64)), dim3(64), 0, s, P, E.x, E.y, E.z, rcell_dev, E.phc,
                        E.phs);
     // charge structure factors and the iloop = 1 reciprocal fields
-    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.schunks), dim3(256), sfac_lds(P), s, P, E.spc,
+    hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.schunks), dim3(64 * EL_SG), sfac_lds(P), s, P, E.spc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
     hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
-    hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 512), P.kchunks), dim3(256), kfield_lds(P), s, P, E.runs,
+    hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 128), P.fchunks), dim3(64 * EL_KG), kfield_lds(P), s, P, E.runs,
                        E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
     hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(6L * P.n, 64)), dim3(256), 0, s, P, E.Fpart, E.dip, E.efdd,
                        E.efdc, E.state);
@@ -2067,11 +2136,11 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t
     if (!first) {
         // reciprocal dipole field from the current dipoles (the first iteration's came
         // with the charge field)
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
-        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
+        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
                            E.S, E.Sq, E.kc, E.state);
-        hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
+        hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 128), P.fchunks), dim3(64 * EL_KG), kfield_lds(P), s, P,
                            E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
         hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, P, E.Fpart, E.dip,
                            E.efdd, E.efdc, E.state);
@@ -2112,11 +2181,11 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t
     double* blkq = E.cgblk;
     double* blku = E.cgblk + nbq;
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
-    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P, E.dspc,
+    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P, E.dspc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, p, E.Spart, E.state);
-    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
+    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
-    hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 512), P.kchunks), dim3(256), kfield_lds(P), s, P,
+    hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 128), P.fchunks), dim3(64 * EL_KG), kfield_lds(P), s, P,
                        E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
     hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
     hipLaunchKernelGGL(k_cg_update, dim3(nbu + ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, it, nbq, blkq, E.cg_p,
@@ -2134,7 +2203,7 @@ int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEv
 {
     (void)hipEventRecord(a, s);
     for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.zero_state);
     (void)hipEventRecord(b, s);
     return (int)hipGetLastError();
@@ -2158,9 +2227,9 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
     // structure factors of the converged dipoles (already computed with the refreshed fields)
     if (!refresh) {
-        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 256), E.dchunks), dim3(256), sfac_lds(P), s, P,
+        hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
-        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
+        hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
                            (const double*)E.zero_state);
     }
     hipLaunchKernelGGL(k_el_kscal, dim3(nkb), dim3(256), 0, s, P, E.kv, E.S, E.Sq, E.kf, b_ks);

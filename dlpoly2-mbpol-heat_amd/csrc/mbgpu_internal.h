@@ -108,7 +108,8 @@ struct ElecDev {
     int compat_qd;                 // qdforce M column as the reference (n_water + imol)
     int capA, capP;                // neighbour-row capacities (all partners / polarizable partners)
     int K, nrun, kchunks;          // k vectors, runs of consecutive n, k chunks of the site-parallel passes
-    int rpc, kmaxch;               // runs per k chunk, most k vectors in one chunk
+    int rpc, kmaxch;               // runs per k chunk, most k vectors in one chunk (final pass)
+    int fchunks, frpc, fkmax;      // the same for the field passes
     int km[3], kstride;            // kmax per axis, phase-table stride (max kmax + 1)
     double scale, scal1, rvolm, self_dd;   // 4 rvolm r4pie0, 2 rvolm r4pie0, 2 pi/V, 4 r4pie0 a^3/(3 sqrt(pi))
 };
@@ -117,6 +118,7 @@ struct ElecDev {
 struct ElecBuffers {
     int n = 0, capA = 0, capP = 0, K = 0, nrun = 0, kchunks = 0, schunks = 0, ntab = 0, kstride = 0;
     int spc = 0, dchunks = 0, dspc = 0, rpc = 0, kmaxch = 0;   // site chunks of the structure-factor passes
+    int fchunks = 0, frpc = 0, fkmax = 0;                      // k chunks of the field passes
     double *x = nullptr, *y = nullptr, *z = nullptr;          // [n] site positions
     double *chge = nullptr, *polr = nullptr, *polr2 = nullptr; // [n]
     double* dg = nullptr;                                      // [nw][6] DMS derivatives

@@ -174,10 +174,10 @@ def test_conjugate_gradient_solver_matches_sor_oracle(pkg, waterbox, ep, which, 
 
 
 def test_k_chunking_of_large_box(pkg, waterbox, ep, monkeypatch):
-    """1024 waters (kmax 24): the site-parallel reciprocal passes split k into chunks of at
-    most 32 runs; with that bound off (MBGPU_ELEC_WAVE_CHUNKS) the chunks follow the
-    wavefront target alone.  Only the order of the chunk sums differs:
-    every result within 1e-10, the iteration count within one."""
+    """1024 waters (kmax 24): the final reciprocal pass splits k into chunks of at most 32
+    runs; with that bound off (MBGPU_ELEC_WAVE_CHUNKS=1) the chunks follow the wavefront
+    target alone.  Only the order of the chunk sums differs: every result within 1e-10, the
+    iteration count within one."""
     box = waterbox.make_waterbox(1024, 31.30, 11, seed=3)
     p = ep.for_box(box.L)
     _, a = run_gpu(pkg, box, p)
