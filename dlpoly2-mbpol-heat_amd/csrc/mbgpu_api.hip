@@ -376,6 +376,48 @@ void with_model_terms(mbgpu_ctx* c, const double*& ep, const double*& we, const 
 
 }  // namespace
 
+// ------------------------------------------------------- no-allocation scratch ----
+// Scratch of mbgpu_ring_spring_dev_noalloc: the block partials of the spring kernels and
+// their two sums.  A file-scope device array, so each GPU the library runs on holds its own
+// copy (256 KiB) until the process ends.  Host code never uses the array's name as a device
+// address: its device address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and kept.  Every call on one GPU shares that GPU's copy, so calls that
+// overlap on one GPU race on it.
+static __device__ __align__(256) double scratch[MBGPU_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this GPU's scratch (the context's device must be current)
+double* noalloc_scratch(mbgpu_ctx* c)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        fail(-1, "no current HIP device");
+        return nullptr;
+    }
+    if (dev != c->device) {
+        fail(-1, "the context belongs to device %d but device %d is current", c->device, dev);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fail((int)e, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
 // ====================================================================== C ABI =======
 extern "C" {
 
@@ -986,6 +1028,34 @@ int mbgpu_ring_spring_dev(mbgpu_ctx* c, int nbead, int natom, double coeff, doub
     if (rc) return fail(rc, "ring spring kernels failed");
     double o[2] = {0.0, 0.0};
     HIPCHK(hipMemcpyAsync(o, c->ring_out, sizeof o, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (epot_spring) *epot_spring = 0.5 * coeff * o[0];
+    if (epot_deriv) *epot_deriv = 0.5 * o[1] / force_scale;
+    return 0;
+}
+
+int mbgpu_ring_spring_dev_noalloc(mbgpu_ctx* c, int nbead, int natom, double coeff, double fac3, double force_scale,
+                                  const double* mass_dev, const double* x_dev, const double* xc_dev, double* f_dev,
+                                  double* epot_spring, double* epot_deriv)
+{
+    if (!c) return fail(-1, "null context");
+    if (nbead < 1 || nbead > MBGPU_RING_MAX_BEADS) return fail(-1, "nbead out of range (got %d)", nbead);
+    if (natom < 1) return fail(-1, "natom must be positive");
+    if (!mass_dev || !x_dev || !xc_dev || !f_dev) return fail(-1, "null mass, position, centroid or force array");
+    if (!(force_scale != 0.0)) return fail(-1, "force_scale must be nonzero");
+    const long need = 2L * nbead * ((natom + 255) / 256) + 2;
+    if (need > MBGPU_NOALLOC_SCRATCH_DOUBLES)
+        return fail(-6, "nbead %d x natom %d needs %ld doubles of scratch; the no-allocation entry holds %d "
+                        "(2 per bead per 256 atoms, plus 2)", nbead, natom, need, MBGPU_NOALLOC_SCRATCH_DOUBLES);
+    double* blk = noalloc_scratch(c);
+    if (!blk) return -1;
+    // every block partial and both sums are written by the kernels of this call: nothing in
+    // the scratch needs clearing
+    double* out = blk + (need - 2);
+    int rc = ring_launch_spring(nbead, natom, coeff, fac3, mass_dev, x_dev, xc_dev, f_dev, blk, out, c->stream);
+    if (rc) return fail(rc, "ring spring kernels failed");
+    double o[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(o, out, sizeof o, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (epot_spring) *epot_spring = 0.5 * coeff * o[0];
     if (epot_deriv) *epot_deriv = 0.5 * o[1] / force_scale;

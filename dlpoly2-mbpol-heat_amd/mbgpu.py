@@ -66,6 +66,11 @@ _SIGS = {
                                              ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double)]),
+    "mbgpu_ring_spring_dev_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                     ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.POINTER(ctypes.c_double),
+                                                     ctypes.POINTER(ctypes.c_double)]),
     "mbgpu_spme_recip_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_double, ctypes.c_int] + [ctypes.c_void_p] * 8
                              + [ctypes.POINTER(ctypes.c_double)]),
@@ -445,6 +450,16 @@ class Context:
         _check(load_library().mbgpu_ring_spring_dev(self._h, int(nbead), int(natom), float(coeff), float(fac3),
                                                     float(force_scale), mass_ptr, x_ptr, xc_ptr, f_ptr,
                                                     ctypes.byref(es), ctypes.byref(ed)), "mbgpu_ring_spring_dev")
+        return es.value, ed.value
+
+    def ring_spring_dev_noalloc(self, nbead, natom, coeff, fac3, force_scale, mass_ptr, x_ptr, xc_ptr, f_ptr):
+        """mbgpu_ring_spring_dev_noalloc (no allocation, fixed per-GPU scratch); returns
+        (Epot_spring, Epot_deriv)."""
+        es, ed = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(load_library().mbgpu_ring_spring_dev_noalloc(self._h, int(nbead), int(natom), float(coeff), float(fac3),
+                                                            float(force_scale), mass_ptr, x_ptr, xc_ptr, f_ptr,
+                                                            ctypes.byref(es), ctypes.byref(ed)),
+               "mbgpu_ring_spring_dev_noalloc")
         return es.value, ed.value
 
     def spme_recip_dev(self, cell, order, grid, alpha, n, x_ptr, y_ptr, z_ptr, q_ptr, mu_ptr=None, phi_ptr=None,

@@ -51,6 +51,7 @@ module mbgpu
 
   integer, parameter :: mbgpu_ring_max_beads = 256
   integer, parameter :: mbgpu_spme_max_order = 12, mbgpu_spme_max_grid = 256
+  integer, parameter :: mbgpu_noalloc_scratch_doubles = 32768
 
   interface
     ! ---- leaves (reference C ABI) ----
@@ -273,6 +274,16 @@ module mbgpu
       real(c_double), value :: coeff, fac3, force_scale
       real(c_double) :: epot_spring, epot_deriv
       integer(c_int) :: mbgpu_ring_spring_dev
+    end function
+    function mbgpu_ring_spring_dev_noalloc(ctx, nbead, natom, coeff, fac3, force_scale, mass_dev, x_dev, xc_dev, &
+                                           f_dev, epot_spring, epot_deriv) &
+        bind(c, name="mbgpu_ring_spring_dev_noalloc")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, mass_dev, x_dev, xc_dev, f_dev
+      integer(c_int), value :: nbead, natom
+      real(c_double), value :: coeff, fac3, force_scale
+      real(c_double) :: epot_spring, epot_deriv
+      integer(c_int) :: mbgpu_ring_spring_dev_noalloc
     end function
     function mbgpu_spme_recip_dev(ctx, cell, order, grid, alpha, n, x_dev, y_dev, z_dev, q_dev, mu_dev, &
                                   phi_dev, efield_dev, f_dev, energy) bind(c, name="mbgpu_spme_recip_dev")

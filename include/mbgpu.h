@@ -268,6 +268,19 @@ int mbgpu_ring_spring_dev(mbgpu_ctx* ctx, int nbead, int natom, double coeff, do
                           const double* mass_dev, const double* x_dev, const double* xc_dev, double* f_dev,
                           double* epot_spring, double* epot_deriv);
 
+/* mbgpu_ring_spring_dev without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+ * mbgpu_ring_spring_dev; its scratch (the kernels' block partials) is a fixed device array
+ * of MBGPU_NOALLOC_SCRATCH_DOUBLES doubles per GPU, held until the process ends, instead
+ * of context buffers grown on demand.  It needs 2 * nbead * ceil(natom / 256) + 2 doubles;
+ * larger calls return -6.  All calls on one GPU share that GPU's scratch: calls that
+ * overlap on one GPU (from several threads or contexts) race. */
+#define MBGPU_NOALLOC_SCRATCH_DOUBLES 32768
+int mbgpu_ring_spring_dev_noalloc(mbgpu_ctx* ctx, int nbead, int natom, double coeff, double fac3,
+                                  double force_scale, const double* mass_dev, const double* x_dev,
+                                  const double* xc_dev, double* f_dev, double* epot_spring, double* epot_deriv);
+
 /* ---- smooth particle-mesh Ewald reciprocal space ---------------------------------- */
 #define MBGPU_SPME_MAX_ORDER 12
 #define MBGPU_SPME_MAX_GRID 256
