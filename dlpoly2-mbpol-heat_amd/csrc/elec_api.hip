@@ -360,7 +360,9 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     dfree(E.Fpart);
     ECHK(dalloc(&E.Fpart, std::max((size_t)E.kchunks * 16, (size_t)E.fchunks * 6) * E.n));
     // block partials of the final passes
-    const int nsb = (E.n + 255) / 256, nkb = (K + 255) / 256, nwb = (S.nw + 63) / 64;
+    // (the particle-mesh final pass writes a partial per 4 sites where the k sum writes one
+    // per 256 k)
+    const int nsb = (E.n + 255) / 256, nkb = std::max((K + 255) / 256, (E.n + 3) / 4), nwb = (S.nw + 63) / 64;
     dfree(E.blk);
     E.nblk = nsb + 8 * nkb + 2 * nsb + 8 * E.n + 24 * nwb + 2 * nsb + 64;
     ECHK(dalloc(&E.blk, (size_t)E.nblk));
@@ -378,6 +380,13 @@ void elec_release(ElecState& S)
 {
     free_buffers(S.E);
     dfree(S.rcell);
+    if (S.sp) {
+        spme_free(*S.sp);
+        delete S.sp;
+        S.sp = nullptr;
+    }
+    delete S.spg;
+    S.spg = nullptr;
     S.ready = false;
 }
 
@@ -435,6 +444,15 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
     for (int i = 0; i < 9; ++i) same_k = same_k && S.k_cell[i] == cell[i];
     for (int i = 0; i < 3; ++i) same_k = same_k && S.k_kmax[i] == p->kmax[i];
     if (!same_k && build_klist(S, cell, p->kmax, p->alpha, err)) return -1;
+    if (S.recip) {
+        // particle mesh: eleven grids (the final pass's), splines of every site
+        if (!S.sp) S.sp = new SpmeBuffers();
+        if (!S.spg) S.spg = new SpmeGeom();
+        if (spme_plan(*S.sp, *S.spg, cell, S.sp_order, S.sp_grid, p->alpha, 4 * nw, SP_MAX_GRIDS, err)) {
+            err = "particle mesh: " + err;
+            return -1;
+        }
+    }
     if (S.E.capA == 0) {
         // rows: sites within rcut at a density of up to 0.15 sites per A^3, rounded to 64
         const double vol = 4.0 / 3.0 * PI_D * p->rcut * p->rcut * p->rcut;
@@ -468,6 +486,8 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     ElecBuffers& E = S.E;
     ElecDev& P = S.P;
     S.ready = false;
+    ElecSpme spx{S.sp, S.spg, S.spg ? S.spg->vol / (8.0 * PI_D) : 0.0};
+    const ElecSpme* sp = S.recip ? &spx : nullptr;
     // The neighbour rows' capacity check is read back with the solver's first convergence
     // check instead of after the row build (one host round trip less): the rows are built
     // with clamped counts, so an overflowed attempt runs in bounds and is simply repeated
@@ -486,7 +506,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         ECHK(hipMemsetAsync(E.emu, 0, sizeof(double) * 3 * P.n, s));
         // M sites carry no induced dipole (polr2 = 0): clear any start value
         ECHK(hipMemsetAsync(E.dip + 3 * P.nreal, 0, sizeof(double) * 3 * (P.n - P.nreal), s));
-        rc = el_launch_prep(E, P, S.rcell, s);
+        rc = el_launch_prep(E, P, S.rcell, sp, s);
         if (rc) {
             err = std::string("electrostatics preparation kernels failed: ") + hipGetErrorString((hipError_t)rc);
             return rc;
@@ -497,7 +517,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         // no-op launches, a second round trip tens)
         int launched = 0, batch = S.last_iters > 0 ? std::min(S.last_iters + 2, 64) : (solver == 1 ? 8 : 12);
         if (solver == 1) {
-            rc = el_launch_cg_init(E, P, s);
+            rc = el_launch_cg_init(E, P, sp, s);
             if (rc) {
                 err = std::string("electrostatics solver kernels failed: ") + hipGetErrorString((hipError_t)rc);
                 return rc;
@@ -506,8 +526,8 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         bool overflow = false;
         for (;;) {
             for (int b = 0; b < batch; ++b) {
-                rc = solver == 1 ? el_launch_cg_iteration(E, P, launched, s)
-                                 : el_launch_iteration(E, P, launched == 0, s);
+                rc = solver == 1 ? el_launch_cg_iteration(E, P, launched, sp, s)
+                                 : el_launch_iteration(E, P, launched == 0, sp, s);
                 ++launched;
                 if (rc) {
                     err = std::string("electrostatics iteration kernels failed: ") + hipGetErrorString((hipError_t)rc);
@@ -551,13 +571,23 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     }
     int rc = 0;
     const double g34 = exp(gammln_host(0.75)), gln34 = gammln_host(0.75);
-    rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, s);
+    rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, sp, s);
     if (rc) {
         err = std::string("electrostatics final kernels failed: ") + hipGetErrorString((hipError_t)rc);
         return rc;
     }
     ECHK(hipMemcpyAsync(S.scal, E.scal, sizeof(double) * 41, hipMemcpyDeviceToHost, s));
+    int spflag[2] = {0, 0};
+    if (sp) ECHK(hipMemcpyAsync(spflag, S.sp->flag, sizeof spflag, hipMemcpyDeviceToHost, s));
     ECHK(hipStreamSynchronize(s));
+    if (spflag[0] == 1) {
+        err = "particle mesh: a site position is not finite or too far from the cell to index the grid";
+        return -1;
+    }
+    if (spflag[0] == 2) {
+        err = "particle mesh: a charge or dipole is too large for the fixed-point grid";
+        return -1;
+    }
     const double* q = S.scal;
     const double engsic = -R4PIE0 * P.alpha * q[9] / SQRPI;
     const double eng1 = P.scal1 * q[1] + engsic;

@@ -2111,12 +2111,57 @@ int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s)
 }
 
 // after the neighbour counts were checked by the host
-int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hipStream_t s)
+// The reciprocal field of the dipoles mu at the first n sites (particle mesh) into
+// Fpart[c * P.n + j] as a k-sum partial (times kap), for the chunk-sum readers with one chunk
+static void el_spme_dipole_field(ElecBuffers& E, const ElecSpme& sp, int n, const double* mu, const double* state,
+                                 hipStream_t s)
+{
+    SpSpec spec{};
+    spec.ngr = 1;
+    spec.mu_on[0] = 1;
+    spec.comp[0] = -1;
+    double2* g = sp.B->grid;
+    sp_launch_spread(*sp.B, *sp.G, n, nullptr, mu, spec, g, state, s);
+    sp_launch_fft(*sp.B, *sp.G, g, 1, false, state, s);
+    sp_launch_influence(*sp.B, *sp.G, g, 1, false, state, s);
+    sp_launch_fft(*sp.B, *sp.G, g, 1, true, state, s);
+    sp_launch_field(*sp.B, *sp.G, n, g, 1, sp.kap, E.Fpart, E.n, state, s);
+}
+
+static inline ElecDev one_chunk(const ElecDev& P)
+{
+    ElecDev Q = P;
+    Q.fchunks = 1;
+    Q.K = 0;
+    return Q;
+}
+
+int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, const ElecSpme* sp, hipStream_t s)
 {
     const int nwb = ceil_div(P.nw, 64);
     hipLaunchKernelGGL(k_el_prep, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tab, E.nbrA, E.cntA, E.nbrP,
                        E.cntP, E.tens, E.efk);
     hipLaunchKernelGGL(k_el_intra_prep, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tin, E.efk);
+    if (sp) {
+        // particle mesh: splines of every site, then the fields of the start dipoles (grid 0)
+        // and of the charges (grid 1) at every site, in the layout of the k-sum field partials
+        sp_launch_prep(*sp->B, *sp->G, P.n, E.x, E.y, E.z, s);
+        SpSpec spec{};
+        spec.ngr = 2;
+        spec.mu_on[0] = 1;
+        spec.comp[0] = -1;
+        spec.q_on[1] = 1;
+        spec.comp[1] = -1;
+        double2* g = sp->B->grid;
+        sp_launch_spread(*sp->B, *sp->G, P.n, E.chge, E.dip, spec, g, nullptr, s);
+        sp_launch_fft(*sp->B, *sp->G, g, 2, false, nullptr, s);
+        sp_launch_influence(*sp->B, *sp->G, g, 2, false, nullptr, s);
+        sp_launch_fft(*sp->B, *sp->G, g, 2, true, nullptr, s);
+        sp_launch_field(*sp->B, *sp->G, P.n, g, 2, sp->kap, E.Fpart, P.n, nullptr, s);
+        hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(6L * P.n, 64)), dim3(256), 0, s, one_chunk(P),
+                           E.Fpart, E.dip, E.efdd, E.efdc, E.state);
+        return (int)hipGetLastError();
+    }
     hipLaunchKernelGGL(k_el_phase, dim3(ceil_div(P.n, 64)), dim3(64), 0, s, P, E.x, E.y, E.z, rcell_dev, E.phc,
                        E.phs);
     // charge structure factors and the iloop = 1 reciprocal fields
@@ -2131,9 +2176,13 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hi
     return (int)hipGetLastError();
 }
 
-int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t s)
+int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, const ElecSpme* sp, hipStream_t s)
 {
-    if (!first) {
+    if (!first && sp) {
+        el_spme_dipole_field(E, *sp, P.nrow, E.dip, E.state, s);
+        hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, one_chunk(P),
+                           E.Fpart, E.dip, E.efdd, E.efdc, E.state);
+    } else if (!first) {
         // reciprocal dipole field from the current dipoles (the first iteration's came
         // with the charge field)
         hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
@@ -2159,14 +2208,14 @@ static inline int cg_nbq(const ElecDev& P) { return ceil_div(3 * P.nrow, 64); }
 
 // conjugate-gradient start: residual of the start dipoles (their fields came with the
 // charge fields of the preparation, plus the real-space dipole field here)
-int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
+int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, const ElecSpme* sp, hipStream_t s)
 {
     const int nbw = ceil_div(P.nw, 256);
     hipLaunchKernelGGL(k_cg_block, dim3(nbw), dim3(256), 0, s, P, E.x, E.y, E.z, E.cg_minv);
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
     hipLaunchKernelGGL(k_cg_init, dim3(nbw), dim3(256), 0, s, P, E.efk, E.efdc, E.emu, E.efdd, E.dip, E.cg_minv, E.cg_r,
                        E.cg_z, E.cg_p, E.cg_tmu, E.cgblk);
-    hipLaunchKernelGGL(k_cg_smu_init, dim3(ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, E.S, E.cg_smu);
+    if (!sp) hipLaunchKernelGGL(k_cg_smu_init, dim3(ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, E.S, E.cg_smu);
     hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s, P, nbw, E.cgblk, E.state);
     return (int)hipGetLastError();
 }
@@ -2174,13 +2223,24 @@ int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s)
 // Iteration `it` (0-based): T p (real-space rows, structure factors, reciprocal field),
 // q = A p, the update of mu, r, z with alpha, then the new direction with beta and the
 // stopping test.
-int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t s)
+int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, const ElecSpme* sp, hipStream_t s)
 {
     const double* p = E.cg_p;
     const int nbq = cg_nbq(P), nbu = ceil_div(P.nw, CG_WPB);
     double* blkq = E.cgblk;
     double* blku = E.cgblk + nbq;
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
+    if (sp) {
+        // particle mesh: the reciprocal field of p, one chunk; no structure factors to carry
+        el_spme_dipole_field(E, *sp, P.nrow, p, E.state, s);
+        const ElecDev Q = one_chunk(P);
+        hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, Q, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
+        hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, Q, it, nbq, blkq, E.cg_p, E.cg_q, E.cg_minv, E.dip,
+                           E.cg_r, E.cg_z, E.emu, E.efdd, E.cg_tmu, E.S, E.cg_smu, blku, E.state);
+        hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, it, nbu, blku, E.cg_z,
+                           E.cg_p, E.state);
+        return (int)hipGetLastError();
+    }
     hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P, E.dspc,
                        E.runs, E.runk, E.phc, E.phs, E.chge, p, E.Spart, E.state);
     hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
@@ -2210,14 +2270,18 @@ int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEv
 }
 
 int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
-                    hipStream_t s)
+                    const ElecSpme* sp, hipStream_t s)
 {
     // fields and structure factors of the final dipoles: carried along by the conjugate-
     // gradient solver; the SOR path keeps those of its last iteration, as the reference does
-    if (refresh)
-        hipLaunchKernelGGL(k_cg_final_fields, dim3(ceil_div(std::max(3L * P.nrow, 6L * P.K), 256)), dim3(256), 0, s, P,
+    // (the particle mesh carries no structure factors)
+    if (refresh) {
+        const ElecDev Q = sp ? one_chunk(P) : P;
+        hipLaunchKernelGGL(k_cg_final_fields, dim3(ceil_div(std::max(3L * Q.nrow, 6L * Q.K), 256)), dim3(256), 0, s, Q,
                            E.cg_tmu, E.cg_smu, E.emu, E.efdd, E.S);
-    const int nsb = ceil_div(P.n, 256), nwb = ceil_div(P.nw, 64), nkb = ceil_div(P.K, 256);
+    }
+    // k-sum: one partial per 256 k; particle mesh: one per 4 sites (k_sp_elec_final's blocks)
+    const int nsb = ceil_div(P.n, 256), nwb = ceil_div(P.nw, 64), nkb = sp ? ceil_div(P.n, 4) : ceil_div(P.K, 256);
     double* b_pol = E.blk;                       // [nsb]
     double* b_ks = b_pol + nsb;                  // [nkb][8]
     double* b_q2 = b_ks + 8L * nkb;              // [nsb][2]
@@ -2225,17 +2289,39 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     double* b_p3 = b_p2 + 8L * P.n;              // [nwb][8]
     double* b_qd = b_p3 + 8L * nwb;              // [nwb][16]
     hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
-    // structure factors of the converged dipoles (already computed with the refreshed fields)
-    if (!refresh) {
+    if (sp) {
+        // particle mesh: grids 0..4 hold q + mu, q, and mu_x, mu_y, mu_z as charges; the
+        // stress kernels of grid 0 go to grids 5..10; every per-site quantity of the k-sum
+        // site pass then comes from one gather (k_sp_elec_final), in the k-sum pass's units
+        SpSpec spec{};
+        spec.ngr = 5;
+        spec.q_on[0] = 1;
+        spec.mu_on[0] = 1;
+        spec.q_on[1] = 1;
+        spec.comp[0] = spec.comp[1] = -1;
+        for (int b = 0; b < 3; ++b) spec.comp[2 + b] = (signed char)b;
+        SpmeBuffers& B = *sp->B;
+        const SpmeGeom& G = *sp->G;
+        double2* g = B.grid;
+        sp_launch_spread(B, G, P.n, E.chge, E.dip, spec, g, nullptr, s);
+        sp_launch_fft(B, G, g, 5, false, nullptr, s);
+        sp_launch_influence_stress(B, G, g, g + 5 * B.ng_cap, s);
+        sp_launch_influence(B, G, g + B.ng_cap, 4, false, nullptr, s);
+        sp_launch_fft(B, G, g, 11, true, nullptr, s);
+        sp_launch_elec_final(B, G, P.n, E.chge, E.dip, g, sp->kap, E.ksum, b_ks, s);
+    } else if (!refresh) {
+        // structure factors of the converged dipoles (already computed with the refreshed fields)
         hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
                            E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
         hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
                            (const double*)E.zero_state);
     }
-    hipLaunchKernelGGL(k_el_kscal, dim3(nkb), dim3(256), 0, s, P, E.kv, E.S, E.Sq, E.kf, b_ks);
-    hipLaunchKernelGGL(k_el_kfinal, dim3(ceil_div(P.n, 256), P.kchunks), dim3(256), kfinal_lds(P), s, P, E.runs,
-                       E.runk, E.phc, E.phs, E.kv, E.kf, E.chge, E.dip, E.Fpart);
-    hipLaunchKernelGGL(k_el_ksum, dim3(ceil_div(15L * P.n, 64)), dim3(256), 0, s, P, E.Fpart, E.ksum);
+    if (!sp) {
+        hipLaunchKernelGGL(k_el_kscal, dim3(nkb), dim3(256), 0, s, P, E.kv, E.S, E.Sq, E.kf, b_ks);
+        hipLaunchKernelGGL(k_el_kfinal, dim3(ceil_div(P.n, 256), P.kchunks), dim3(256), kfinal_lds(P), s, P, E.runs,
+                           E.runk, E.phc, E.phs, E.kv, E.kf, E.chge, E.dip, E.Fpart);
+        hipLaunchKernelGGL(k_el_ksum, dim3(ceil_div(15L * P.n, 64)), dim3(256), 0, s, P, E.Fpart, E.ksum);
+    }
     hipLaunchKernelGGL(k_el_kfinal_reduce, dim3(nsb), dim3(256), 0, s, P, E.ksum, E.chge, E.fk, E.vk, E.vdck, E.vs,
                        E.e1, E.s1, b_q2);
     hipLaunchKernelGGL(k_el_pairs, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.dip, E.tab, E.nbrA, E.cntA,

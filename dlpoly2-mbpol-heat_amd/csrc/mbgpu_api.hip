@@ -475,8 +475,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
     dfree(c->ring_m); dfree(c->ring_blk); dfree(c->ring_out);
-    dfree(c->sp.gi); dfree(c->sp.th); dfree(c->sp.cw); dfree(c->sp.grid); dfree(c->sp.bmod); dfree(c->sp.tw); dfree(c->sp.part); dfree(c->sp.out);
-    dfree(c->sp.flag);
+    spme_free(c->sp);
     elec_release(c->el);
     water_release(c->wt);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -486,6 +485,34 @@ void mbgpu_destroy(mbgpu_ctx* c)
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->el_stream) (void)hipStreamDestroy(c->el_stream);
     delete c;
+}
+
+int mbgpu_set_elec_reciprocal(mbgpu_ctx* c, int mode, int order, const int grid[3])
+{
+    if (!c) return fail(-1, "null context");
+    if (mode == 0) {
+        c->el.recip = 0;
+        c->el.ready = false;
+        return 0;
+    }
+    if (mode != 1) return fail(-1, "reciprocal mode %d: 0 = Ewald sum over k, 1 = particle mesh", mode);
+    if (!grid) return fail(-1, "null grid");
+    if (order < 4 || order > SPME_MAX_ORDER || (order & 1))
+        return fail(-1, "order must be even, 4..%d (got %d)", SPME_MAX_ORDER, order);
+    for (int a = 0; a < 3; ++a) {
+        int K = grid[a];
+        const int K0 = K;
+        for (int r : {2, 3, 5})
+            while (K > 0 && K % r == 0) K /= r;
+        if (K0 < 8 || K0 > SPME_MAX_GRID || K != 1 || K0 < order)
+            return fail(-1, "grid[%d] = %d: each grid size must be 2^a 3^b 5^c in [max(8, order), %d]", a, K0,
+                        SPME_MAX_GRID);
+    }
+    c->el.recip = 1;
+    c->el.sp_order = order;
+    for (int a = 0; a < 3; ++a) c->el.sp_grid[a] = grid[a];
+    c->el.ready = false;
+    return 0;
 }
 
 int mbgpu_set_precision(mbgpu_ctx* c, int mode)
@@ -1067,109 +1094,12 @@ int mbgpu_spme_recip_dev(mbgpu_ctx* c, const double cell[9], int order, const in
                          const double* mu_dev, double* phi_dev, double* efield_dev, double* f_dev, double* energy)
 {
     if (!c) return fail(-1, "null context");
-    if (!cell || !grid) return fail(-1, "null cell or grid");
-    if (order < 4 || order > SPME_MAX_ORDER || (order & 1))
-        return fail(-1, "order must be even, 4..%d (got %d)", SPME_MAX_ORDER, order);
-    for (int a = 0; a < 3; ++a) {
-        const int K = grid[a];
-        if (K < 8 || K > SPME_MAX_GRID || (K & (K - 1)) || K < order)
-            return fail(-1, "grid[%d] = %d: each grid size must be a power of two in [max(8, order), %d]", a, K,
-                        SPME_MAX_GRID);
-    }
-    if (!(alpha > 0.0)) return fail(-1, "alpha must be positive");
     if (n < 0) return fail(-1, "n must not be negative");
     if (n > 0 && (!x_dev || !y_dev || !z_dev || !q_dev)) return fail(-1, "null position or charge array");
     SpmeGeom G{};
-    {
-        const double *a = cell, *b = cell + 3, *cc = cell + 6;
-        const double bxc[3] = {b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]};
-        const double cxa[3] = {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]};
-        const double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-        const double v = a[0] * bxc[0] + a[1] * bxc[1] + a[2] * bxc[2];
-        if (!(fabs(v) > 0.0) || !std::isfinite(v)) return fail(-1, "the cell has no volume");
-        for (int d = 0; d < 3; ++d) {
-            G.rv[0][d] = bxc[d] / v;
-            G.rv[1][d] = cxa[d] / v;
-            G.rv[2][d] = axb[d] / v;
-        }
-        G.vol = fabs(v);
-    }
-    G.alpha = alpha;
-    G.p = order;
-    for (int a = 0; a < 3; ++a) G.K[a] = grid[a];
     SpmeBuffers& B = c->sp;
-    const long ng = (long)grid[0] * grid[1] * grid[2];
-    if (B.n_cap < n) {
-        dfree(B.gi);
-        dfree(B.th);
-        dfree(B.cw);
-        B.n_cap = 0;
-        HIPCHK(dalloc(&B.gi, (size_t)n));
-        HIPCHK(dalloc(&B.th, (size_t)n * 9 * SPME_MAX_ORDER));
-        HIPCHK(dalloc(&B.cw, (size_t)n));
-        B.n_cap = n;
-    }
-    if (B.ng_cap < ng) {
-        dfree(B.grid);
-        dfree(B.part);
-        B.ng_cap = 0;
-        HIPCHK(dalloc(&B.grid, (size_t)ng));
-        HIPCHK(dalloc(&B.part, (size_t)((ng + 255) / 256)));
-        B.ng_cap = ng;
-    }
-    if (!B.out) HIPCHK(dalloc(&B.out, 1));
-    if (!B.flag) HIPCHK(dalloc(&B.flag, 2));
-    if (B.cfg[0] != order || B.cfg[1] != grid[0] || B.cfg[2] != grid[1] || B.cfg[3] != grid[2]) {
-        // |b(m)|^2 = 1 / |sum_{k=0}^{p-2} M_p(k+1) exp(2 pi i m k / K)|^2 (bspcoe.f); M_p at
-        // the integers by the recursion of bspgen.f:27-100 at w = 0
-        std::vector<double> mp(order + 1, 0.0), nx(order + 1);
-        mp[1] = 1.0;   // M_2(1) = 1 (M_2(0) = M_2(2) = 0)
-        for (int k = 3; k <= order; ++k) {
-            for (int j = 0; j <= k; ++j)
-                nx[j] = ((double)j * (j < (int)mp.size() ? mp[j] : 0.0) + (double)(k - j) * (j > 0 ? mp[j - 1] : 0.0)) /
-                        (double)(k - 1);
-            for (int j = 0; j <= order; ++j) mp[j] = j <= k ? nx[j] : 0.0;
-        }
-        std::vector<double> bm(grid[0] + grid[1] + grid[2]);
-        std::vector<double2> tw;
-        int off = 0;
-        for (int a = 0; a < 3; ++a) {
-            const int K = grid[a];
-            for (int m = 0; m < K; ++m) {
-                double re = 0.0, im = 0.0;
-                for (int k = 0; k <= order - 2; ++k) {
-                    const double ang = 2.0 * M_PI * (double)(((long)m * k) % K) / (double)K;
-                    re += mp[k + 1] * std::cos(ang);
-                    im += mp[k + 1] * std::sin(ang);
-                }
-                bm[off + m] = 1.0 / (re * re + im * im);
-            }
-            off += K;
-            B.twoff[a] = (int)tw.size();
-            for (int t = 0; t < K / 2; ++t) {
-                const double ang = 2.0 * M_PI * (double)t / (double)K;
-                tw.push_back(make_double2(std::cos(ang), -std::sin(ang)));
-            }
-        }
-        dfree(B.bmod);
-        dfree(B.tw);
-        B.cfg[0] = 0;
-        HIPCHK(dalloc(&B.bmod, bm.size()));
-        HIPCHK(dalloc(&B.tw, tw.size()));
-        HIPCHK(hipMemcpy(B.bmod, bm.data(), sizeof(double) * bm.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(B.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
-        B.cfg[0] = order;
-        B.cfg[1] = grid[0];
-        B.cfg[2] = grid[1];
-        B.cfg[3] = grid[2];
-    }
-    // transform lines: axis 2 contiguous lines, axes 1 and 0 strided with neighbouring
-    // lines side by side
-    const int K1 = grid[0], K2 = grid[1], K3 = grid[2];
-    B.axis[2] = SpmeAxis{K3, 1, K1 * K2, K3, 1, 0, 0L};
-    B.axis[1] = SpmeAxis{K2, K3, K3, 1, K1, 0, (long)K2 * K3};
-    B.axis[0] = SpmeAxis{K1, K2 * K3, K2 * K3, 1, 1, 0, 0L};
-    for (int a = 0; a < 3; ++a) B.axis[a].L = std::max(1, std::min(SP_FFT_ELEMS / B.axis[a].K, B.axis[a].ni));
+    std::string err;
+    if (spme_plan(B, G, cell, order, grid, alpha, n, 1, err)) return fail(-1, "%s", err.c_str());
     int rc = spme_launch(B, G, n, x_dev, y_dev, z_dev, q_dev, mu_dev, phi_dev, efield_dev, f_dev, c->el_stream);
     if (rc) return fail(rc, "particle-mesh kernels failed");
     double u = 0.0;

@@ -158,13 +158,21 @@ struct ElecBuffers {
     int* err = nullptr;                                        // [4] device error flags
 };
 
+struct SpmeBuffers;
+struct SpmeGeom;
+// the particle-mesh reciprocal space of the electrostatics (null: the Ewald sum over k)
+struct ElecSpme {
+    SpmeBuffers* B;
+    const SpmeGeom* G;
+    double kap;          // V / (8 pi): from the full-space mesh sums to the half-space k sums
+};
 int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s);
-int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, hipStream_t s);
-int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, hipStream_t s);
+int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, const ElecSpme* sp, hipStream_t s);
+int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, const ElecSpme* sp, hipStream_t s);
 int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
-                    hipStream_t s);
-int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, hipStream_t s);
-int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, hipStream_t s);
+                    const ElecSpme* sp, hipStream_t s);
+int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, const ElecSpme* sp, hipStream_t s);
+int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, const ElecSpme* sp, hipStream_t s);
 int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEvent_t b, hipStream_t s);
 void el_launch_zero(double* p, long n, hipStream_t s);
 void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
@@ -173,6 +181,10 @@ void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
 struct ElecState {
     ElecBuffers E;
     ElecDev P{};
+    int recip = 0;                      // 0 Ewald sum over k, 1 particle mesh (order, grid)
+    int sp_order = 0, sp_grid[3] = {0, 0, 0};
+    SpmeBuffers* sp = nullptr;          // particle-mesh buffers and geometry (recip = 1)
+    SpmeGeom* spg = nullptr;
     bool ready = false;                 // the last evaluation completed
     int last_iters = 0;                 // solver iterations of the last evaluation
     int nw = 0;
@@ -247,27 +259,69 @@ struct SpmeGeom {
     double alpha;
     int K[3];
     int p;
+    int rows;          // grid rows per spreading tile (a divisor of K[1], at most 16)
 };
-// one axis of the grid as lines for the transform passes (see k_sp_fft)
+// one axis of the grid as lines for the transform passes (see k_sp_fft): K elements per
+// line (K = 2^a 3^b 5^c), element stride se, ni lines of stride si in each of no outer
+// blocks of stride so, L lines per workgroup (L divides ni); the radices of the passes
+#define SPME_MAX_FACTORS 8
 struct SpmeAxis {
     int K, se, ni, si, no, L;
     long so;
+    int nf, fac[SPME_MAX_FACTORS];
 };
 struct SpmeBuffers {
     int4* gi = nullptr;                    // per site: grid index of t = 0 per axis, ok
     double* th = nullptr;                  // per site and axis: M_p, M_p', M_p'' (SPME_MAX_ORDER each)
-    double4* cw = nullptr;                 // per site: q and the dipole weights K_a (a*_a . mu)
     long n_cap = 0;
-    double2* grid = nullptr;               // complex grid
+    double2* grid = nullptr;               // complex grids, ng_cap points apart
     double* bmod = nullptr;                // |b(m)|^2 of the three axes, concatenated
-    double2* tw = nullptr;                 // exp(-2 pi i t / K_a), t < K_a / 2, per axis
-    double* part = nullptr;                // energy block partials
-    double* out = nullptr;                 // energy
+    double2* tw = nullptr;                 // exp(-2 pi i t / K_a), t < K_a, per axis
+    double* part = nullptr;                // energy block partials, [grid][block]
+    double* out = nullptr;                 // energies, one per grid
     int* flag = nullptr;                   // input errors found on the device (2 ints)
-    long ng_cap = 0;
+    long ng_cap = 0;                       // points per grid the allocation holds
+    int ngr_cap = 0;                       // grids the allocation holds
     int cfg[4] = {0, 0, 0, 0};             // order and grid of bmod and tw
     int twoff[3] = {0, 0, 0};
     SpmeAxis axis[3] = {};
 };
+// The density spread onto grid g: q[i] (charge) when q_on[g], the dipole mu_i when mu_on[g],
+// and the component comp[g] of mu_i as a charge when comp[g] >= 0.
+#define SP_MAX_GRIDS 11
+struct SpSpec {
+    int ngr;
+    signed char q_on[SP_MAX_GRIDS], mu_on[SP_MAX_GRIDS], comp[SP_MAX_GRIDS];
+};
+// Geometry, tables, transform plans and buffers for n sites and ngr grids; 0 or -1 with err.
+int spme_plan(SpmeBuffers& B, SpmeGeom& G, const double cell[9], int order, const int grid[3], double alpha, int n,
+              int ngr, std::string& err);
+void spme_free(SpmeBuffers& B);
+inline long spme_points(const SpmeGeom& G) { return (long)G.K[0] * G.K[1] * G.K[2]; }
+// Device steps on stream s; `state` (may be null) is the electrostatics solver state, whose
+// done flag (state[2] != 0) turns the steps into no-ops.
+void sp_launch_prep(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const double* y, const double* z,
+                    hipStream_t s);
+void sp_launch_spread(SpmeBuffers& B, const SpmeGeom& G, int n, const double* q, const double* mu, const SpSpec& spec,
+                      double2* grids, const double* state, hipStream_t s);
+void sp_launch_fft(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool inverse, const double* state,
+                   hipStream_t s);
+// in place on ngr grids: Q^ *= B C; with part, the energy partials of each grid
+void sp_launch_influence(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool energy,
+                         const double* state, hipStream_t s);
+// energies of the first ngr grids from their partials into B.out[0..ngr)
+void sp_launch_usum(SpmeBuffers& B, const SpmeGeom& G, int ngr, hipStream_t s);
+// from the transformed grid src: src *= B C in place, dst[t] = B C akv2 k_a k_b src (t: xx
+// xy xz yy yz zz, akv2 = 2 (1 / k^2 + 1 / (4 alpha^2)); dst grids ng_cap points apart)
+void sp_launch_influence_stress(SpmeBuffers& B, const SpmeGeom& G, double2* src, double2* dst,
+                                hipStream_t s);
+// E = -grad phi of grid y at the first n sites into out[(3 y + c) * ostride + j] * fac
+void sp_launch_field(SpmeBuffers& B, const SpmeGeom& G, int n, const double2* grids, int ngr, double fac,
+                     double* out, long ostride, const double* state, hipStream_t s);
+// every per-site reciprocal quantity of the polarizable electrostatics from the eleven
+// transformed grids of the final pass, as the k-sum site pass's partial sums (times kap):
+// ks[t * n + j] (t < 15) and per-block partials blk[8 * b + v] (see k_sp_elec_final)
+void sp_launch_elec_final(SpmeBuffers& B, const SpmeGeom& G, int n, const double* q, const double* mu,
+                          const double2* grids, double kap, double* ks, double* blk, hipStream_t s);
 int spme_launch(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const double* y, const double* z,
                 const double* q, const double* mu, double* phi, double* ef, double* f, hipStream_t s);

@@ -7,20 +7,33 @@
 // same with the dipoles carried in the spread density (mu . grad of the spline product),
 // the reciprocal part of the induced-dipole Ewald sum (ewald1p.f), to which it converges.
 //
-// Device steps (one stream, no host round trip):
-//   k_sp_prep       per (site, axis): grid index, M_p, M_p', M_p'' and the dipole weight
-//   k_sp_tile       one workgroup per (plane, 16 rows) of the grid: the tile in LDS as
-//                   64-bit fixed point (2^-44 e), contributions of every site whose
-//                   support meets it added with LDS atomics (integer adds, so the grid
-//                   does not depend on their order), then written as complex values
-//   k_sp_fft x3     radix-2 Stockham passes in LDS along each axis; L lines of one axis
-//                   per workgroup of 128 threads (L K = 256 elements when the axis allows)
+// Device steps (one stream, no host round trip), each over one or several grids:
+//   k_sp_prep       per (site, axis): grid index, M_p, M_p', M_p'' (positions only: the
+//                   solver's repeated dipole fields reuse them)
+//   k_sp_tile       one workgroup per (plane, R rows) of the grid: the tiles of up to five
+//                   densities in LDS as 64-bit fixed point (2^-44 e), contributions of every
+//                   site whose support meets them added with LDS atomics (integer adds, so
+//                   the grid does not depend on their order), then written as complex values
+//   k_sp_fft x3     mixed-radix (2, 3, 5) Stockham passes in LDS along each axis; L lines
+//                   of one axis per workgroup of 128 threads (L K <= 256 elements)
 //   k_sp_influence  Q^(m) *= B(m) C(m); U = 1/2 sum B C |Q^|^2 as block partials
+//                   (k_sp_influence_stress: also the six kernels B C akv2 k_a k_b that the
+//                   per-site stress needs)
 //   k_sp_fft x3     backward passes (unnormalised, conjugate twiddles)
 //   k_sp_gather     one wavefront per site: phi, grad phi and (for forces) the Hessian of
 //                   phi from the p^3 grid points, reduced over the lanes in a fixed order
+//                   (k_sp_field: the field alone; k_sp_elec_final: every per-site
+//                   reciprocal quantity of the polarizable electrostatics)
 //   k_sp_usum       the energy partials in a fixed order
 // Coulomb constant 1 (energy e^2/A); the caller scales.
+#include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
 #include "mbgpu_internal.h"
 
 namespace {
@@ -85,30 +98,26 @@ __device__ __forceinline__ bool sp_frac(const SpmeGeom& G, int a, double x, doub
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_sp_zero(unsigned long long* __restrict__ g, long ng)
+__global__ __launch_bounds__(64) void k_sp_clear_flag(int* __restrict__ flag)
 {
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < ng; i += (long)gridDim.x * 256) g[i] = 0ull;
+    if (threadIdx.x < 2) flag[threadIdx.x] = 0;
 }
 
-// Per (site, axis): the grid index of t = 0 (wrapped), M_p, M_p', M_p'' at the p points and
-// the dipole weight K_a (a*_a . mu); ok = 0 (and flag 1) for a position that cannot index
-// the grid.  Layout: gi[i] = (g1, g2, g3, ok); th[i][a][k][t] (k: M, M', M''); cw[i] =
-// (q, w1, w2, w3).
+// Per (site, axis): the grid index of t = 0 (wrapped) and M_p, M_p', M_p'' at the p points;
+// ok = 0 (and flag 1) for a position that cannot index the grid.  Layout: gi[i] = (g1, g2,
+// g3, ok); th[i][a][k][t] (k: M, M', M'').
 __global__ __launch_bounds__(256) void k_sp_prep(SpmeGeom G, int n, const double* __restrict__ x,
                                                  const double* __restrict__ y, const double* __restrict__ z,
-                                                 const double* __restrict__ q, const double* __restrict__ mu,
                                                  int4* __restrict__ gi, double* __restrict__ th,
-                                                 double4* __restrict__ cw, int* __restrict__ flag)
+                                                 int* __restrict__ flag)
 {
     __shared__ int okv[256];
     __shared__ int gv[256];
-    __shared__ double wvv[256];
     const int t = blockIdx.x * 255 + threadIdx.x;   // (site, axis) pairs, 255 = 3 * 85 per block
     const int i = t / 3, a = t - 3 * (t / 3);
     const int p = G.p;
     const bool live = threadIdx.x < 255 && i < n;
     int good = 0, g = 0;
-    double wa = 0.0;
     if (live) {
         long g0 = 0;
         double w = 0.0;
@@ -117,40 +126,42 @@ __global__ __launch_bounds__(256) void k_sp_prep(SpmeGeom G, int n, const double
             g = sp_wrap(g0, G.K[a]);
             double* T = th + ((long)i * 3 + a) * 3 * SPME_MAX_ORDER;
             sp_splines<2>(p, w, T, T + SPME_MAX_ORDER, T + 2 * SPME_MAX_ORDER);
-            if (mu) wa = G.K[a] * (G.rv[a][0] * mu[3 * i] + G.rv[a][1] * mu[3 * i + 1] + G.rv[a][2] * mu[3 * i + 2]);
         } else {
             flag[0] = 1;
         }
     }
     okv[threadIdx.x] = good;
     gv[threadIdx.x] = g;
-    wvv[threadIdx.x] = wa;
     __syncthreads();
     if (live && a == 0) {
         const int b = threadIdx.x;
         gi[i] = make_int4(gv[b], gv[b + 1], gv[b + 2], okv[b] & okv[b + 1] & okv[b + 2]);
-        cw[i] = make_double4(q[i], wvv[b], wvv[b + 1], wvv[b + 2]);
     }
 }
 
-// Spread by tiles: a workgroup owns plane g1 and rows r0..r0+R-1 of the grid (all g3) in
-// LDS as 64-bit fixed point (2^-44 e), scans every site, lists those whose support meets
-// the tile and adds their contributions with LDS atomics (integer adds: the tile does not
-// depend on their order), then writes the tile as complex values.  Contribution at
-// (t1, t2, t3): q M1 M2 M3 + w1 M1' M2 M3 + w2 M1 M2' M3 + w3 M1 M2 M3'.
-constexpr int SP_ROWS = 16;
+// Spread by tiles: a workgroup owns plane g1 and rows r0..r0+R-1 of the grid (all g3) of
+// each of the spec.ngr densities in LDS as 64-bit fixed point (2^-44 e), scans every site,
+// lists those whose support meets the tile and adds their contributions with LDS atomics
+// (integer adds: the tile does not depend on their order), then writes the tiles as complex
+// values to grids + g * gs.  Contribution of a site with charge c and dipole weights w_a =
+// K_a (a*_a . mu) at (t1, t2, t3): c M1 M2 M3 + w1 M1' M2 M3 + w2 M1 M2' M3 + w3 M1 M2 M3'.
+#define SP_MAX_SPREAD 5
 __global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* __restrict__ gi,
-                                                 const double* __restrict__ th, const double4* __restrict__ cw,
-                                                 double2* __restrict__ grid, int* __restrict__ flag)
+                                                 const double* __restrict__ th, const double* __restrict__ q,
+                                                 const double* __restrict__ mu, SpSpec spec,
+                                                 double2* __restrict__ grids, long gs, int* __restrict__ flag,
+                                                 const double* __restrict__ state)
 {
-    __shared__ unsigned long long tile[SP_ROWS * SPME_MAX_GRID];
+    if (state && state[2] != 0.0) return;
+    extern __shared__ unsigned long long tile[];   // [ngr][R][K3]
     __shared__ int list[256], lt1[256];
     __shared__ int cnt;
     const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2], p = G.p;
-    const int R = min(SP_ROWS, K2), nrb = K2 / R;
+    const int R = G.rows, nrb = K2 / R, ngr = spec.ngr;
+    const int TS = R * K3;
     const int g1 = blockIdx.x / nrb, r0 = (blockIdx.x - g1 * nrb) * R;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < R * K3; idx += 256) tile[idx] = 0ull;
+    for (int idx = tid; idx < ngr * TS; idx += 256) tile[idx] = 0ull;
     bool big = false;
     for (int base = 0; base < n; base += 256) {
         if (tid == 0) cnt = 0;
@@ -187,37 +198,96 @@ __global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* 
             r += r < 0 ? K2 : 0;
             if (r >= R) continue;
             const double* T = th + (long)i2 * 9 * SPME_MAX_ORDER;
-            const double4 c = cw[i2];
             const double a1 = T[t1], d1 = T[SPME_MAX_ORDER + t1];
             const double a2 = T[3 * SPME_MAX_ORDER + t2], d2 = T[4 * SPME_MAX_ORDER + t2];
-            const double c12 = c.x * a1 * a2 + c.y * d1 * a2 + c.z * a1 * d2;
-            const double w3 = c.w * a1 * a2;
             const double* T3 = T + 6 * SPME_MAX_ORDER;
-            unsigned long long* row = tile + r * K3;
-            for (int t3 = 0; t3 < p; ++t3) {
-                const double v = c12 * T3[t3] + w3 * T3[SPME_MAX_ORDER + t3];
-                big |= !(fabs(v) < 1024.0);
-                int g3 = g.z - t3;
-                g3 += g3 < 0 ? K3 : 0;
-                atomicAdd(row + g3, (unsigned long long)llrint(v * SP_FIX));
+            const double qi = q ? q[i2] : 0.0;
+            double mv[3] = {0.0, 0.0, 0.0};
+            if (mu)
+                for (int c = 0; c < 3; ++c) mv[c] = mu[3 * i2 + c];
+            double wa[3];
+            for (int a = 0; a < 3; ++a) wa[a] = G.K[a] * (G.rv[a][0] * mv[0] + G.rv[a][1] * mv[1] + G.rv[a][2] * mv[2]);
+            for (int gr = 0; gr < ngr; ++gr) {
+                const double c = (spec.q_on[gr] ? qi : 0.0) + (spec.comp[gr] >= 0 ? mv[spec.comp[gr]] : 0.0);
+                const double mo = spec.mu_on[gr] ? 1.0 : 0.0;
+                const double c12 = c * a1 * a2 + mo * (wa[0] * d1 * a2 + wa[1] * a1 * d2);
+                const double w3 = mo * wa[2] * a1 * a2;
+                unsigned long long* row = tile + gr * TS + r * K3;
+                for (int t3 = 0; t3 < p; ++t3) {
+                    const double v = c12 * T3[t3] + w3 * T3[SPME_MAX_ORDER + t3];
+                    big |= !(fabs(v) < 1024.0);
+                    int g3 = g.z - t3;
+                    g3 += g3 < 0 ? K3 : 0;
+                    atomicAdd(row + g3, (unsigned long long)llrint(v * SP_FIX));
+                }
             }
         }
         __syncthreads();
     }
     if (big) flag[0] = 2;
-    for (int idx = tid; idx < R * K3; idx += 256) {
-        const int r = idx / K3, c3 = idx - r * K3;
-        grid[((long)g1 * K2 + r0 + r) * K3 + c3] = make_double2((double)(long long)tile[idx] * SP_UNFIX, 0.0);
+    for (int idx = tid; idx < ngr * TS; idx += 256) {
+        const int gr = idx / TS, rem = idx - gr * TS;
+        const int r = rem / K3, c3 = rem - r * K3;
+        grids[gr * gs + ((long)g1 * K2 + r0 + r) * K3 + c3] =
+            make_double2((double)(long long)tile[idx] * SP_UNFIX, 0.0);
     }
 }
 
-// Radix-2 Stockham transform of L lines of length K along one axis (K a power of two).
+// Mixed-radix Stockham transform of L lines of length K = 2^a 3^b 5^c along one axis.
 // Line l of block (o, ib) starts at o * so + (ib * L + l) * si, element e at + e * se.
-__global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_fft(double2* __restrict__ g, SpmeAxis A, const double2* __restrict__ tw,
-                                                int inverse)
+// Pass f with radix R after Ns = (product of the earlier radices): butterfly j < K/R reads
+// elements j + q K/R (q < R), multiplies element q by w^(q k) with k = j mod Ns and
+// w = exp(-2 pi i / (Ns R)) (conjugated backward), takes the length-R DFT and writes it to
+// (j / Ns) Ns R + k + q Ns.  tw holds exp(-2 pi i t / K), t < K.
+__device__ __forceinline__ double2 sp_cmul(double2 a, double2 b)
 {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// v <- DFT_R(v) with exp(sgn 2 pi i q m / R), sgn = -1 forward, +1 backward
+__device__ __forceinline__ void sp_dft_small(int R, double sgn, double2* v)
+{
+    if (R == 2) {
+        const double2 a = v[0], b = v[1];
+        v[0] = make_double2(a.x + b.x, a.y + b.y);
+        v[1] = make_double2(a.x - b.x, a.y - b.y);
+    } else if (R == 3) {
+        const double h = 0.86602540378443864676;   // sin(2 pi / 3)
+        const double2 t1 = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
+        const double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+        const double2 t3 = make_double2(sgn * h * (v[1].x - v[2].x), sgn * h * (v[1].y - v[2].y));
+        v[0] = make_double2(v[0].x + t1.x, v[0].y + t1.y);
+        // y1 = t2 + i t3, y2 = t2 - i t3
+        v[1] = make_double2(t2.x - t3.y, t2.y + t3.x);
+        v[2] = make_double2(t2.x + t3.y, t2.y - t3.x);
+    } else {   // R == 5
+        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;   // cos(2 pi/5), cos(4 pi/5)
+        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;    // sin(2 pi/5), sin(4 pi/5)
+        const double2 t1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
+        const double2 t2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
+        const double2 t3 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
+        const double2 t4 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
+        const double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+        const double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+        const double2 b1 = make_double2(sgn * (s1 * t3.x + s2 * t4.x), sgn * (s1 * t3.y + s2 * t4.y));
+        const double2 b2 = make_double2(sgn * (s2 * t3.x - s1 * t4.x), sgn * (s2 * t3.y - s1 * t4.y));
+        v[0] = make_double2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+        v[1] = make_double2(a1.x - b1.y, a1.y + b1.x);   // a1 + i b1
+        v[4] = make_double2(a1.x + b1.y, a1.y - b1.x);   // a1 - i b1
+        v[2] = make_double2(a2.x - b2.y, a2.y + b2.x);
+        v[3] = make_double2(a2.x + b2.y, a2.y - b2.x);
+    }
+}
+
+// blockIdx.y: the grid (grids gs points apart)
+__global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_fft(double2* __restrict__ grids, long gs, SpmeAxis A,
+                                                           const double2* __restrict__ tw, int inverse,
+                                                           const double* __restrict__ state)
+{
+    if (state && state[2] != 0.0) return;
     __shared__ double2 buf[2][SP_FFT_ELEMS > SPME_MAX_GRID ? SP_FFT_ELEMS : SPME_MAX_GRID];
-    const int K = A.K, half = K >> 1, L = A.L;
+    double2* __restrict__ g = grids + blockIdx.y * gs;
+    const int K = A.K, L = A.L;
     const int nib = A.ni / L;
     const int o = blockIdx.x / nib, ib = blockIdx.x - o * nib;
     const long base = (long)o * A.so + (long)ib * L * A.si;
@@ -234,23 +304,31 @@ __global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_fft(double2* __restrict__
         buf[0][l * K + e] = g[base + (long)l * A.si + (long)e * A.se];
     }
     __syncthreads();
-    int src = 0;
-    for (int Ns = 1; Ns < K; Ns <<= 1) {
-        for (int b = threadIdx.x; b < L * half; b += SP_FFT_THREADS) {
-            const int l = b / half, j = b - l * half;
+    const double sgn = inverse ? 1.0 : -1.0;
+    int src = 0, Ns = 1;
+    for (int f = 0; f < A.nf; ++f) {
+        const int R = A.fac[f], KR = K / R, ts = K / (Ns * R);
+        for (int b = threadIdx.x; b < L * KR; b += SP_FFT_THREADS) {
+            const int l = b / KR, j = b - l * KR;
             const double2* in = buf[src] + l * K;
             double2* out = buf[src ^ 1] + l * K;
-            const double2 v0 = in[j], v1 = in[j + half];
-            const int k = j & (Ns - 1);
-            double2 w = tw[k * (half / Ns)];
-            if (inverse) w.y = -w.y;
-            const double tr = v1.x * w.x - v1.y * w.y, ti = v1.x * w.y + v1.y * w.x;
-            const int d = ((j - k) << 1) + k;
-            out[d] = make_double2(v0.x + tr, v0.y + ti);
-            out[d + Ns] = make_double2(v0.x - tr, v0.y - ti);
+            const int k = j % Ns;
+            double2 v[5];
+            for (int q = 0; q < R; ++q) {
+                v[q] = in[j + q * KR];
+                if (q > 0 && k > 0) {
+                    double2 w = tw[q * k * ts];
+                    if (inverse) w.y = -w.y;
+                    v[q] = sp_cmul(v[q], w);
+                }
+            }
+            sp_dft_small(R, sgn, v);
+            const int d = (j - k) * R + k;
+            for (int q = 0; q < R; ++q) out[d + q * Ns] = v[q];
         }
         __syncthreads();
         src ^= 1;
+        Ns *= R;
     }
     for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
         int l, e;
@@ -265,52 +343,165 @@ __global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_fft(double2* __restrict__
     }
 }
 
-// Q^(m) *= B(m) C(m), C(m) = exp(-pi^2 m^2 / alpha^2) / (pi V m^2), m = sum_a m_a a*_a with
-// m_a folded into (-K_a/2, K_a/2]; block partials of 1/2 B C |Q^|^2
-__global__ __launch_bounds__(256) void k_sp_influence(SpmeGeom G, double2* __restrict__ g,
-                                                      const double* __restrict__ bmod, double* __restrict__ part)
+// B(m) C(m) at flat index i of the grid, C(m) = exp(-pi^2 m^2 / alpha^2) / (pi V m^2),
+// m = sum_a m_a a*_a with m_a folded into (-K_a/2, K_a/2]; the Cartesian m into mv and m^2
+// into msq (0 at m = 0)
+__device__ __forceinline__ double sp_bc(const SpmeGeom& G, const double* __restrict__ bmod, long i, double* mv,
+                                        double& msq)
 {
-    __shared__ double red[4];
     const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2];
-    const long ng = (long)K1 * K2 * K3;
+    const int m3 = (int)(i % K3), m2 = (int)((i / K3) % K2), m1 = (int)(i / ((long)K2 * K3));
+    const int a1 = m1 <= K1 / 2 ? m1 : m1 - K1, a2 = m2 <= K2 / 2 ? m2 : m2 - K2, a3 = m3 <= K3 / 2 ? m3 : m3 - K3;
+    msq = 0.0;
+    for (int d = 0; d < 3; ++d) mv[d] = 0.0;
+    if (!(a1 || a2 || a3)) return 0.0;
+    for (int d = 0; d < 3; ++d) mv[d] = a1 * G.rv[0][d] + a2 * G.rv[1][d] + a3 * G.rv[2][d];
+    msq = mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2];
+    return bmod[m1] * bmod[K1 + m2] * bmod[K1 + K2 + m3] * exp(-SP_PI * SP_PI * msq / (G.alpha * G.alpha)) /
+           (SP_PI * G.vol * msq);
+}
+
+// Q^(m) *= B(m) C(m) on grid blockIdx.y; with part, block partials of 1/2 B C |Q^|^2 into
+// part[blockIdx.y * gridDim.x + blockIdx.x]
+__global__ __launch_bounds__(256) void k_sp_influence(SpmeGeom G, double2* __restrict__ grids, long gs,
+                                                      const double* __restrict__ bmod, double* __restrict__ part,
+                                                      const double* __restrict__ state)
+{
+    if (state && state[2] != 0.0) return;
+    __shared__ double red[4];
+    double2* __restrict__ g = grids + blockIdx.y * gs;
+    const long ng = (long)G.K[0] * G.K[1] * G.K[2];
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     double e = 0.0;
     if (i < ng) {
-        const int m3 = (int)(i % K3), m2 = (int)((i / K3) % K2), m1 = (int)(i / ((long)K2 * K3));
-        const int a1 = m1 <= K1 / 2 ? m1 : m1 - K1, a2 = m2 <= K2 / 2 ? m2 : m2 - K2, a3 = m3 <= K3 / 2 ? m3 : m3 - K3;
-        double bc = 0.0;
-        if (a1 || a2 || a3) {
-            double mv[3];
-            for (int d = 0; d < 3; ++d) mv[d] = a1 * G.rv[0][d] + a2 * G.rv[1][d] + a3 * G.rv[2][d];
-            const double msq = mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2];
-            bc = bmod[m1] * bmod[K1 + m2] * bmod[K1 + K2 + m3] * exp(-SP_PI * SP_PI * msq / (G.alpha * G.alpha)) /
-                 (SP_PI * G.vol * msq);
-        }
+        double mv[3], msq;
+        const double bc = sp_bc(G, bmod, i, mv, msq);
         double2 v = g[i];
         e = 0.5 * bc * (v.x * v.x + v.y * v.y);
         v.x *= bc;
         v.y *= bc;
         g[i] = v;
     }
+    if (!part) return;
     e = sp_wsum(e);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) part[(long)blockIdx.y * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// src *= B C in place, and dst[t] = B C akv2 k_a k_b src (k = 2 pi m, akv2 = 2 (1 / k^2 +
+// 1 / (4 alpha^2)); t = xx xy xz yy yz zz), dst grids gs points apart
+__global__ __launch_bounds__(256) void k_sp_influence_stress(SpmeGeom G, double2* __restrict__ src,
+                                                             double2* __restrict__ dst, long gs,
+                                                             const double* __restrict__ bmod)
+{
+    const long ng = (long)G.K[0] * G.K[1] * G.K[2];
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ng) return;
+    double mv[3], msq;
+    const double bc = sp_bc(G, bmod, i, mv, msq);
+    const double2 v = src[i];
+    src[i] = make_double2(bc * v.x, bc * v.y);
+    const double k2 = 4.0 * SP_PI * SP_PI * msq;
+    const double f = msq > 0.0 ? bc * 2.0 * (1.0 / k2 + 0.25 / (G.alpha * G.alpha)) * 4.0 * SP_PI * SP_PI : 0.0;
+    const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+    for (int t = 0; t < 6; ++t) {
+        const double w = f * mv[ia[t]] * mv[ib[t]];
+        dst[t * gs + i] = make_double2(w * v.x, w * v.y);
+    }
+}
+
+// energy of grid blockIdx.x from its nb partials
 __global__ __launch_bounds__(256) void k_sp_usum(int nb, const double* __restrict__ part, double* __restrict__ out)
 {
     __shared__ double red[4];
+    const double* pp = part + (long)blockIdx.x * nb;
     double a = 0.0;
-    for (int k = threadIdx.x; k < nb; k += 256) a += part[k];
+    for (int k = threadIdx.x; k < nb; k += 256) a += pp[k];
     a = sp_wsum(a);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
-    if (threadIdx.x == 0) out[0] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) out[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// one wavefront per site (4 per block): the lanes take the p^3 grid points in turn; HESS:
-// the second derivatives too (forces asked for)
+// d/dr_d = sum_a K_a a*_a[d] d/du_a
+__device__ __forceinline__ void sp_ga(const SpmeGeom& G, double (&ga)[3][3])
+{
+    for (int a = 0; a < 3; ++a)
+        for (int d = 0; d < 3; ++d) ga[a][d] = G.K[a] * G.rv[a][d];
+}
+
+// Cartesian gradient from the u-derivatives s[0..2]
+__device__ __forceinline__ void sp_cart_grad(const double (&ga)[3][3], const double* s, double* gr)
+{
+    for (int d = 0; d < 3; ++d) gr[d] = ga[0][d] * s[0] + ga[1][d] * s[1] + ga[2][d] * s[2];
+}
+
+// Cartesian Hessian times m from the u-second derivatives h = (00, 11, 22, 01, 02, 12)
+__device__ __forceinline__ void sp_cart_hess_mul(const double (&ga)[3][3], const double* h, const double* m,
+                                                 double* out)
+{
+    const double hu[3][3] = {{h[0], h[3], h[4]}, {h[3], h[1], h[5]}, {h[4], h[5], h[2]}};
+    for (int d = 0; d < 3; ++d) {
+        double hm = 0.0;
+        for (int e = 0; e < 3; ++e) {
+            double hh = 0.0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) hh += ga[a][d] * ga[b][e] * hu[a][b];
+            hm += hh * m[e];
+        }
+        out[d] = hm;
+    }
+}
+
+// Sums over the p^3 grid points of one site (lanes in turn, then a fixed-order wavefront
+// reduction): NV = 1 the value; 4 value and u-gradient; 10 also the u-Hessian (00 11 22 01
+// 02 12); 9 u-gradient and u-Hessian without the value.  Grid points index g[...].x.
+template <int NV>
+__device__ __forceinline__ void sp_site_sums(const SpmeGeom& G, int4 gg, const double* __restrict__ T,
+                                             const double2* __restrict__ g, double* s)
+{
+    constexpr int MO = SPME_MAX_ORDER;
+    const int p = G.p, K1 = G.K[0], K2 = G.K[1], K3 = G.K[2];
+    const int lane = threadIdx.x & 63;
+    double a[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) a[k] = 0.0;
+    const int np3 = p * p * p;
+    for (int fidx = lane; fidx < np3; fidx += 64) {
+        const int t1 = fidx / (p * p), t2 = (fidx / p) % p, t3 = fidx % p;
+        int g1 = gg.x - t1, g2 = gg.y - t2, g3 = gg.z - t3;
+        g1 += g1 < 0 ? K1 : 0;
+        g2 += g2 < 0 ? K2 : 0;
+        g3 += g3 < 0 ? K3 : 0;
+        const double v = g[((long)g1 * K2 + g2) * K3 + g3].x;
+        const double A1 = T[t1], A2 = T[3 * MO + t2], A3 = T[6 * MO + t3];
+        if constexpr (NV == 1) {
+            a[0] += v * A1 * A2 * A3;
+        } else {
+            const double D1 = T[MO + t1], D2 = T[4 * MO + t2], D3 = T[7 * MO + t3];
+            constexpr int o = NV == 9 ? -1 : 0;
+            if constexpr (NV != 9) a[0] += v * A1 * A2 * A3;
+            a[o + 1] += v * D1 * A2 * A3;
+            a[o + 2] += v * A1 * D2 * A3;
+            a[o + 3] += v * A1 * A2 * D3;
+            if constexpr (NV >= 9) {
+                const double E1 = T[2 * MO + t1], E2 = T[5 * MO + t2], E3 = T[8 * MO + t3];
+                a[o + 4] += v * E1 * A2 * A3;
+                a[o + 5] += v * A1 * E2 * A3;
+                a[o + 6] += v * A1 * A2 * E3;
+                a[o + 7] += v * D1 * D2 * A3;
+                a[o + 8] += v * D1 * A2 * D3;
+                a[o + 9] += v * A1 * D2 * D3;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] = sp_wsum(a[k]);
+}
+
+// one wavefront per site (4 per block): phi, the field and (HESS, forces asked for) the
+// force -q grad phi - (grad grad phi) mu
 template <bool HESS>
 __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4* __restrict__ gi,
                                                    const double* __restrict__ th, const double* __restrict__ q,
@@ -323,45 +514,16 @@ __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4
     if (i >= n) return;
     const int4 gg = gi[i];
     if (!gg.w) return;
-    const int p = G.p;
-    const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2];
     const double* T = th + (long)i * 9 * SPME_MAX_ORDER;
-    constexpr int MO = SPME_MAX_ORDER;
     double s[10];
-    for (int k = 0; k < 10; ++k) s[k] = 0.0;
-    const int np3 = p * p * p;
-    for (int fidx = lane; fidx < np3; fidx += 64) {
-        const int t1 = fidx / (p * p), t2 = (fidx / p) % p, t3 = fidx % p;
-        int g1 = gg.x - t1, g2 = gg.y - t2, g3 = gg.z - t3;
-        g1 += g1 < 0 ? K1 : 0;
-        g2 += g2 < 0 ? K2 : 0;
-        g3 += g3 < 0 ? K3 : 0;
-        const double v = g[((long)g1 * K2 + g2) * K3 + g3].x;
-        const double A1 = T[t1], D1 = T[MO + t1], E1 = T[2 * MO + t1];
-        const double A2 = T[3 * MO + t2], D2 = T[4 * MO + t2], E2 = T[5 * MO + t2];
-        const double A3 = T[6 * MO + t3], D3 = T[7 * MO + t3], E3 = T[8 * MO + t3];
-        s[0] += v * A1 * A2 * A3;
-        s[1] += v * D1 * A2 * A3;
-        s[2] += v * A1 * D2 * A3;
-        s[3] += v * A1 * A2 * D3;
-        if (HESS) {
-            s[4] += v * E1 * A2 * A3;   // second derivatives in u: 00
-            s[5] += v * A1 * E2 * A3;   // 11
-            s[6] += v * A1 * A2 * E3;   // 22
-            s[7] += v * D1 * D2 * A3;   // 01
-            s[8] += v * D1 * A2 * D3;   // 02
-            s[9] += v * A1 * D2 * D3;   // 12
-        }
-    }
-    for (int k = 0; k < (HESS ? 10 : 4); ++k) s[k] = sp_wsum(s[k]);
+    if (HESS)
+        sp_site_sums<10>(G, gg, T, g, s);
+    else
+        sp_site_sums<4>(G, gg, T, g, s);
     if (lane != 0) return;
-    // d/dr_d = sum_a K_a a*_a[d] d/du_a
-    double ga[3][3];
-    for (int a = 0; a < 3; ++a)
-        for (int d = 0; d < 3; ++d) ga[a][d] = G.K[a] * G.rv[a][d];
-    const double hu[3][3] = {{s[4], s[7], s[8]}, {s[7], s[5], s[9]}, {s[8], s[9], s[6]}};
-    double gr[3];
-    for (int d = 0; d < 3; ++d) gr[d] = ga[0][d] * s[1] + ga[1][d] * s[2] + ga[2][d] * s[3];
+    double ga[3][3], gr[3];
+    sp_ga(G, ga);
+    sp_cart_grad(ga, s + 1, gr);
     if (phi) phi[i] = s[0];
     if (ef)
         for (int d = 0; d < 3; ++d) ef[3 * i + d] = -gr[d];
@@ -370,48 +532,205 @@ __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4
         double m[3] = {0.0, 0.0, 0.0};
         if (mu)
             for (int d = 0; d < 3; ++d) m[d] = mu[3 * i + d];
-        for (int d = 0; d < 3; ++d) {
-            double hm = 0.0;
-            for (int e = 0; e < 3; ++e) {
-                double h = 0.0;
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) h += ga[a][d] * ga[b][e] * hu[a][b];
-                hm += h * m[e];
-            }
-            f[3 * i + d] = -qi * gr[d] - hm;
-        }
+        double hm[3];
+        sp_cart_hess_mul(ga, s + 4, m, hm);
+        for (int d = 0; d < 3; ++d) f[3 * i + d] = -qi * gr[d] - hm[d];
     }
 }
 
-void sp_fft3(SpmeBuffers& B, const SpmeGeom& G, bool inverse, hipStream_t s)
+// E = -grad phi of grid blockIdx.y at site i, times fac, into out[(3 y + c) * os + i]
+// (zeros for a site that cannot index the grid); one wavefront per site, 4 per block
+__global__ __launch_bounds__(256) void k_sp_field(SpmeGeom G, int n, const int4* __restrict__ gi,
+                                                  const double* __restrict__ th, const double2* __restrict__ grids,
+                                                  long gs, double fac, double* __restrict__ out, long os,
+                                                  const double* __restrict__ state)
 {
+    if (state && state[2] != 0.0) return;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int4 gg = gi[i];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if (gg.w) sp_site_sums<4>(G, gg, th + (long)i * 9 * SPME_MAX_ORDER, grids + blockIdx.y * gs, s);
+    if (lane != 0) return;
+    double ga[3][3], gr[3];
+    sp_ga(G, ga);
+    sp_cart_grad(ga, s + 1, gr);
+    double* o = out + 3L * blockIdx.y * os + i;
+    for (int c = 0; c < 3; ++c) o[c * os] = -fac * gr[c];
+}
+
+// Every per-site reciprocal quantity of the polarizable electrostatics (the quantities the
+// Ewald k-sum's final site pass accumulates, ewald1p.f:550-693, 771-825) from the
+// transformed grids of one evaluation:
+//   grid 0      B C S^ of the density q + mu: phi, grad phi, grad grad phi
+//   grid 1      B C Q^ of the charges alone: phi_q
+//   grids 2..4  B C M_b^ of dipole component b as a charge density: grad and grad grad Lambda_b
+//   grids 5..10 B C akv2 k_a k_b S^ (xx xy xz yy yz zz): Psi_ab and grad Psi_ab
+// With kap the factor from these full-space convolutions to the half-space k-sums
+// (V / 8 pi), site j gets, as the k-sum pass's partial sums:
+//   [0..2]  kap (-q grad phi - (grad grad phi) mu)      the force
+//   [3]     kap phi_q,  [4] kap (phi - phi_q),  [5] q [3]
+//   [6 + 3a + b]  kap (delta_ab D - G_ab + P_ab + mu_b d_a phi), D = q phi + mu . grad phi,
+//           G_ab = q Psi_ab + mu . grad Psi_ab, P_ab = -(q d_a Lambda_b + (grad grad Lambda_b mu)_a)
+// into ks[t * n + j], and block partials [nb][8] of: q [3], kap D, and the six k-sum stress
+// components (xx yy zz xy xz yz) of the site, [6 + 3a + b] - delta_ab kap D.
+__global__ __launch_bounds__(256) void k_sp_elec_final(SpmeGeom G, int n, const int4* __restrict__ gi,
+                                                       const double* __restrict__ th,
+                                                       const double* __restrict__ q, const double* __restrict__ mu,
+                                                       const double2* __restrict__ grids, long gs, double kap,
+                                                       double* __restrict__ ks, double* __restrict__ blk)
+{
+    __shared__ double red[4][8];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wv;
+    double out[15], kd = 0.0;
+    for (int t = 0; t < 15; ++t) out[t] = 0.0;
+    if (i < n) {
+        const int4 gg = gi[i];
+        if (gg.w) {
+            const double* T = th + (long)i * 9 * SPME_MAX_ORDER;
+            double ga[3][3];
+            sp_ga(G, ga);
+            const double qi = q[i];
+            const double m[3] = {mu[3 * i], mu[3 * i + 1], mu[3 * i + 2]};
+            double s[10], gr[3], hm[3];
+            sp_site_sums<10>(G, gg, T, grids, s);
+            const double phi = s[0];
+            sp_cart_grad(ga, s + 1, gr);
+            sp_cart_hess_mul(ga, s + 4, m, hm);
+            for (int d = 0; d < 3; ++d) out[d] = kap * (-qi * gr[d] - hm[d]);
+            const double D = qi * phi + m[0] * gr[0] + m[1] * gr[1] + m[2] * gr[2];
+            double st[9];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) st[3 * a + b] = (a == b ? D : 0.0) + m[b] * gr[a];
+            sp_site_sums<1>(G, gg, T, grids + gs, s);
+            out[3] = kap * s[0];
+            out[4] = kap * (phi - s[0]);
+            out[5] = qi * out[3];
+            for (int b = 0; b < 3; ++b) {
+                double lg[3], lh[3];
+                sp_site_sums<9>(G, gg, T, grids + (2 + b) * gs, s);
+                sp_cart_grad(ga, s, lg);
+                sp_cart_hess_mul(ga, s + 3, m, lh);
+                for (int a = 0; a < 3; ++a) st[3 * a + b] -= qi * lg[a] + lh[a];
+            }
+            const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+            for (int t = 0; t < 6; ++t) {
+                double pg[3];
+                sp_site_sums<4>(G, gg, T, grids + (5 + t) * gs, s);
+                sp_cart_grad(ga, s + 1, pg);
+                const double Gt = qi * s[0] + m[0] * pg[0] + m[1] * pg[1] + m[2] * pg[2];
+                st[3 * ia[t] + ib[t]] -= Gt;
+                if (ia[t] != ib[t]) st[3 * ib[t] + ia[t]] -= Gt;
+            }
+            for (int t = 0; t < 9; ++t) out[6 + t] = kap * st[t];
+            kd = kap * D;
+        }
+        if (lane == 0)
+            for (int t = 0; t < 15; ++t) ks[(long)t * n + i] = out[t];
+    }
+    if (lane == 0) {
+        red[wv][0] = out[5];
+        red[wv][1] = kd;
+        red[wv][2] = out[6] - kd;
+        red[wv][3] = out[10] - kd;
+        red[wv][4] = out[14] - kd;
+        red[wv][5] = out[7];
+        red[wv][6] = out[8];
+        red[wv][7] = out[11];
+    }
+    __syncthreads();
+    if (threadIdx.x < 8)
+        blk[8L * blockIdx.x + threadIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------- launchers ----
+
+void sp_launch_prep(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const double* y, const double* z,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sp_clear_flag, dim3(1), dim3(64), 0, s, B.flag);
+    if (n > 0)
+        hipLaunchKernelGGL(k_sp_prep, dim3((unsigned)((3L * n + 254) / 255)), dim3(256), 0, s, G, n, x, y, z, B.gi,
+                           B.th, B.flag);
+}
+
+void sp_launch_spread(SpmeBuffers& B, const SpmeGeom& G, int n, const double* q, const double* mu, const SpSpec& spec,
+                      double2* grids, const double* state, hipStream_t s)
+{
+    const size_t lds = sizeof(unsigned long long) * (size_t)spec.ngr * G.rows * G.K[2];
+    hipLaunchKernelGGL(k_sp_tile, dim3((unsigned)(G.K[0] * (G.K[1] / G.rows))), dim3(256), lds, s, G, n, B.gi, B.th,
+                       q, mu, spec, grids, B.ng_cap, B.flag, state);
+}
+
+void sp_launch_fft(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool inverse, const double* state,
+                   hipStream_t s)
+{
+    (void)G;
     // axes in the order 2, 1, 0 forward, 0, 1, 2 back
     for (int k = 0; k < 3; ++k) {
         const int a = inverse ? k : 2 - k;
         const SpmeAxis& A = B.axis[a];
         const unsigned nblk = (unsigned)((long)A.no * (A.ni / A.L));
-        hipLaunchKernelGGL(k_sp_fft, dim3(nblk), dim3(SP_FFT_THREADS), 0, s, B.grid, A, B.tw + B.twoff[a], inverse ? 1 : 0);
+        hipLaunchKernelGGL(k_sp_fft, dim3(nblk, (unsigned)ngr), dim3(SP_FFT_THREADS), 0, s, grids, B.ng_cap, A,
+                           B.tw + B.twoff[a], inverse ? 1 : 0, state);
     }
 }
 
-}  // namespace
+void sp_launch_influence(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool energy,
+                         const double* state, hipStream_t s)
+{
+    const int nb = (int)((spme_points(G) + 255) / 256);
+    hipLaunchKernelGGL(k_sp_influence, dim3((unsigned)nb, (unsigned)ngr), dim3(256), 0, s, G, grids, B.ng_cap, B.bmod,
+                       energy ? B.part : nullptr, state);
+}
+
+void sp_launch_usum(SpmeBuffers& B, const SpmeGeom& G, int ngr, hipStream_t s)
+{
+    const int nb = (int)((spme_points(G) + 255) / 256);
+    hipLaunchKernelGGL(k_sp_usum, dim3((unsigned)ngr), dim3(256), 0, s, nb, B.part, B.out);
+}
+
+void sp_launch_influence_stress(SpmeBuffers& B, const SpmeGeom& G, double2* src, double2* dst, hipStream_t s)
+{
+    const int nb = (int)((spme_points(G) + 255) / 256);
+    hipLaunchKernelGGL(k_sp_influence_stress, dim3((unsigned)nb), dim3(256), 0, s, G, src, dst, B.ng_cap, B.bmod);
+}
+
+void sp_launch_field(SpmeBuffers& B, const SpmeGeom& G, int n, const double2* grids, int ngr, double fac,
+                     double* out, long ostride, const double* state, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_sp_field, dim3((unsigned)((n + 3) / 4), (unsigned)ngr), dim3(256), 0, s, G, n, B.gi, B.th,
+                           grids, B.ng_cap, fac, out, ostride, state);
+}
+
+void sp_launch_elec_final(SpmeBuffers& B, const SpmeGeom& G, int n, const double* q, const double* mu,
+                          const double2* grids, double kap, double* ks, double* blk, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_sp_elec_final, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, G, n, B.gi, B.th, q, mu,
+                           grids, B.ng_cap, kap, ks, blk);
+}
 
 int spme_launch(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const double* y, const double* z,
                 const double* q, const double* mu, double* phi, double* ef, double* f, hipStream_t s)
 {
-    const long ng = (long)G.K[0] * G.K[1] * G.K[2];
-    hipLaunchKernelGGL(k_sp_zero, dim3(1), dim3(256), 0, s, reinterpret_cast<unsigned long long*>(B.flag), 1L);
-    if (n > 0)
-        hipLaunchKernelGGL(k_sp_prep, dim3((unsigned)((3L * n + 254) / 255)), dim3(256), 0, s, G, n, x, y, z, q, mu,
-                           B.gi, B.th, B.cw, B.flag);
-    const int R = std::min(SP_ROWS, G.K[1]);
-    hipLaunchKernelGGL(k_sp_tile, dim3((unsigned)(G.K[0] * (G.K[1] / R))), dim3(256), 0, s, G, n, B.gi, B.th, B.cw,
-                       B.grid, B.flag);
-    sp_fft3(B, G, false, s);
-    const int nb = (int)((ng + 255) / 256);
-    hipLaunchKernelGGL(k_sp_influence, dim3(nb), dim3(256), 0, s, G, B.grid, B.bmod, B.part);
-    hipLaunchKernelGGL(k_sp_usum, dim3(1), dim3(256), 0, s, nb, B.part, B.out);
-    sp_fft3(B, G, true, s);
+    sp_launch_prep(B, G, n, x, y, z, s);
+    SpSpec spec{};
+    spec.ngr = 1;
+    spec.q_on[0] = 1;
+    spec.mu_on[0] = mu ? 1 : 0;
+    spec.comp[0] = -1;
+    sp_launch_spread(B, G, n, q, mu, spec, B.grid, nullptr, s);
+    sp_launch_fft(B, G, B.grid, 1, false, nullptr, s);
+    sp_launch_influence(B, G, B.grid, 1, true, nullptr, s);
+    sp_launch_usum(B, G, 1, s);
+    sp_launch_fft(B, G, B.grid, 1, true, nullptr, s);
     if (n > 0 && f)
         hipLaunchKernelGGL(k_sp_gather<true>, dim3((n + 3) / 4), dim3(256), 0, s, G, n, B.gi, B.th, q, mu, B.grid, phi,
                            ef, f);
@@ -419,4 +738,211 @@ int spme_launch(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const
         hipLaunchKernelGGL(k_sp_gather<false>, dim3((n + 3) / 4), dim3(256), 0, s, G, n, B.gi, B.th, q, mu, B.grid, phi,
                            ef, f);
     return (int)hipGetLastError();
+}
+
+// --------------------------------------------------------------------------- plan ----
+
+namespace {
+
+bool sp_smooth(int K)
+{
+    for (int r : {2, 3, 5})
+        while (K % r == 0) K /= r;
+    return K == 1;
+}
+
+int sp_largest_divisor(int n, int cap)
+{
+    for (int d = std::min(n, std::max(1, cap)); d > 1; --d)
+        if (n % d == 0) return d;
+    return 1;
+}
+
+// radices of the transform passes (the 2s, then the 3s, then the 5s) and L, the lines per
+// workgroup: the largest divisor of ni with L K <= SP_FFT_ELEMS
+void sp_axis_plan(SpmeAxis& A)
+{
+    int K = A.K, nf = 0;
+    for (int r : {2, 3, 5})
+        while (K % r == 0) {
+            A.fac[nf++] = r;
+            K /= r;
+        }
+    A.nf = nf;
+    A.L = sp_largest_divisor(A.ni, SP_FFT_ELEMS / A.K);
+}
+
+template <typename T>
+void sp_free(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+#define SPCHK(x)                                                                 \
+    do {                                                                         \
+        hipError_t e_ = (x);                                                     \
+        if (e_ != hipSuccess) {                                                  \
+            err = std::string(#x) + ": " + hipGetErrorString(e_);                \
+            return -1;                                                           \
+        }                                                                        \
+    } while (0)
+
+}  // namespace
+
+void spme_free(SpmeBuffers& B)
+{
+    sp_free(B.gi);
+    sp_free(B.th);
+    sp_free(B.grid);
+    sp_free(B.bmod);
+    sp_free(B.tw);
+    sp_free(B.part);
+    sp_free(B.out);
+    sp_free(B.flag);
+    B.n_cap = 0;
+    B.ng_cap = 0;
+    B.ngr_cap = 0;
+    B.cfg[0] = 0;
+}
+
+int spme_plan(SpmeBuffers& B, SpmeGeom& G, const double cell[9], int order, const int grid[3], double alpha, int n,
+              int ngr, std::string& err)
+{
+    char buf[200];
+    if (!cell || !grid) {
+        err = "null cell or grid";
+        return -1;
+    }
+    if (order < 4 || order > SPME_MAX_ORDER || (order & 1)) {
+        snprintf(buf, sizeof buf, "order must be even, 4..%d (got %d)", SPME_MAX_ORDER, order);
+        err = buf;
+        return -1;
+    }
+    for (int a = 0; a < 3; ++a) {
+        const int K = grid[a];
+        if (K < 8 || K > SPME_MAX_GRID || !sp_smooth(K) || K < order) {
+            snprintf(buf, sizeof buf, "grid[%d] = %d: each grid size must be 2^a 3^b 5^c in [max(8, order), %d]", a,
+                     K, SPME_MAX_GRID);
+            err = buf;
+            return -1;
+        }
+    }
+    if (!(alpha > 0.0)) {
+        err = "alpha must be positive";
+        return -1;
+    }
+    if (n < 0 || ngr < 1 || ngr > SP_MAX_GRIDS) {
+        err = "site or grid count out of range";
+        return -1;
+    }
+    G = SpmeGeom{};
+    {
+        const double *a = cell, *b = cell + 3, *cc = cell + 6;
+        const double bxc[3] = {b[1] * cc[2] - b[2] * cc[1], b[2] * cc[0] - b[0] * cc[2], b[0] * cc[1] - b[1] * cc[0]};
+        const double cxa[3] = {cc[1] * a[2] - cc[2] * a[1], cc[2] * a[0] - cc[0] * a[2], cc[0] * a[1] - cc[1] * a[0]};
+        const double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        const double v = a[0] * bxc[0] + a[1] * bxc[1] + a[2] * bxc[2];
+        if (!(fabs(v) > 0.0) || !std::isfinite(v)) {
+            err = "the cell has no volume";
+            return -1;
+        }
+        for (int d = 0; d < 3; ++d) {
+            G.rv[0][d] = bxc[d] / v;
+            G.rv[1][d] = cxa[d] / v;
+            G.rv[2][d] = axb[d] / v;
+        }
+        G.vol = fabs(v);
+    }
+    G.alpha = alpha;
+    G.p = order;
+    for (int a = 0; a < 3; ++a) G.K[a] = grid[a];
+    const int K1 = grid[0], K2 = grid[1], K3 = grid[2];
+    // tile rows: LDS of up to SP_MAX_SPREAD tiles within 60 000 bytes (the site lists take
+    // 2 KiB more of a workgroup's 64 KiB), at most 16 rows, and at least 512 workgroups
+    // where the grid allows
+    {
+        const int nsp = std::min(ngr, SP_MAX_SPREAD);
+        int pick = 0, smallest = 1;
+        for (int R = std::min(16, K2); R >= 1; --R) {
+            if (K2 % R || (long)nsp * R * K3 * 8 > 60000) continue;
+            smallest = R;
+            if (!pick && (long)K1 * (K2 / R) >= 512) pick = R;
+        }
+        G.rows = pick ? pick : smallest;
+    }
+    const long ng = (long)K1 * K2 * K3;
+    if (B.n_cap < n) {
+        sp_free(B.gi);
+        sp_free(B.th);
+        B.n_cap = 0;
+        SPCHK(mbgpu_alloc(&B.gi, (size_t)n));
+        SPCHK(mbgpu_alloc(&B.th, (size_t)n * 9 * SPME_MAX_ORDER));
+        B.n_cap = n;
+    }
+    if (B.ng_cap < ng || B.ngr_cap < ngr) {
+        const long cap = std::max(B.ng_cap, ng);
+        const int gcap = std::max(B.ngr_cap, ngr);
+        sp_free(B.grid);
+        sp_free(B.part);
+        B.ng_cap = 0;
+        B.ngr_cap = 0;
+        SPCHK(mbgpu_alloc(&B.grid, (size_t)cap * gcap));
+        SPCHK(mbgpu_alloc(&B.part, (size_t)((cap + 255) / 256) * gcap));
+        B.ng_cap = cap;
+        B.ngr_cap = gcap;
+    }
+    if (!B.out) SPCHK(mbgpu_alloc(&B.out, (size_t)SP_MAX_GRIDS));
+    if (!B.flag) SPCHK(mbgpu_alloc(&B.flag, 2));
+    if (B.cfg[0] != order || B.cfg[1] != K1 || B.cfg[2] != K2 || B.cfg[3] != K3) {
+        // |b(m)|^2 = 1 / |sum_{k=0}^{p-2} M_p(k+1) exp(2 pi i m k / K)|^2 (bspcoe.f); M_p at
+        // the integers by the recursion of bspgen.f:27-100 at w = 0
+        std::vector<double> mp(order + 1, 0.0), nx(order + 1);
+        mp[1] = 1.0;   // M_2(1) = 1 (M_2(0) = M_2(2) = 0)
+        for (int k = 3; k <= order; ++k) {
+            for (int j = 0; j <= k; ++j)
+                nx[j] = ((double)j * (j < (int)mp.size() ? mp[j] : 0.0) + (double)(k - j) * (j > 0 ? mp[j - 1] : 0.0)) /
+                        (double)(k - 1);
+            for (int j = 0; j <= order; ++j) mp[j] = j <= k ? nx[j] : 0.0;
+        }
+        std::vector<double> bm(K1 + K2 + K3);
+        std::vector<double2> tw;
+        int off = 0;
+        for (int a = 0; a < 3; ++a) {
+            const int K = grid[a];
+            for (int m = 0; m < K; ++m) {
+                double re = 0.0, im = 0.0;
+                for (int k = 0; k <= order - 2; ++k) {
+                    const double ang = 2.0 * M_PI * (double)(((long)m * k) % K) / (double)K;
+                    re += mp[k + 1] * std::cos(ang);
+                    im += mp[k + 1] * std::sin(ang);
+                }
+                bm[off + m] = 1.0 / (re * re + im * im);
+            }
+            off += K;
+            B.twoff[a] = (int)tw.size();
+            for (int t = 0; t < K; ++t) {
+                const double ang = 2.0 * M_PI * (double)t / (double)K;
+                tw.push_back(make_double2(std::cos(ang), -std::sin(ang)));
+            }
+        }
+        sp_free(B.bmod);
+        sp_free(B.tw);
+        B.cfg[0] = 0;
+        SPCHK(mbgpu_alloc(&B.bmod, bm.size()));
+        SPCHK(mbgpu_alloc(&B.tw, tw.size()));
+        SPCHK(hipMemcpy(B.bmod, bm.data(), sizeof(double) * bm.size(), hipMemcpyHostToDevice));
+        SPCHK(hipMemcpy(B.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+        B.cfg[0] = order;
+        B.cfg[1] = K1;
+        B.cfg[2] = K2;
+        B.cfg[3] = K3;
+    }
+    // transform lines: axis 2 contiguous lines, axes 1 and 0 strided with neighbouring
+    // lines side by side
+    B.axis[2] = SpmeAxis{K3, 1, K1 * K2, K3, 1, 0, 0L, 0, {}};
+    B.axis[1] = SpmeAxis{K2, K3, K3, 1, K1, 0, (long)K2 * K3, 0, {}};
+    B.axis[0] = SpmeAxis{K1, K2 * K3, K2 * K3, 1, 1, 0, 0L, 0, {}};
+    for (int a = 0; a < 3; ++a) sp_axis_plan(B.axis[a]);
+    return 0;
 }

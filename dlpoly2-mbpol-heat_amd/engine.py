@@ -26,13 +26,17 @@ ENGUNIT = 418.4
 
 
 class Slot:
-    def __init__(self, box, precision=0, solver=1, elec=None, water=None, device="cuda:0", overlap=True):
+    def __init__(self, box, precision=0, solver=1, elec=None, water=None, device="cuda:0", overlap=True,
+                 recip=None):
         import torch
         self.torch = torch
         self.dev = torch.device(device)
         self.box = box
         self.ctx = Context(box.nw, box.natms)
         self.ctx.set_precision(precision)
+        # recip: None (the Ewald sum over k) or ("spme", order, grid)
+        if recip is not None:
+            self.ctx.set_elec_reciprocal(recip[0], recip[1], recip[2])
         self.precision = precision
         self.solver = solver
         self.overlap = bool(overlap)

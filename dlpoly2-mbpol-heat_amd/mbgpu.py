@@ -43,6 +43,7 @@ _SIGS = {
                                               ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp]),
     "mbgpu_get_decomposition": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mbgpu_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mbgpu_set_elec_reciprocal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "mbgpu_get_lists": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _ip,
                                        ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "mbgpu_heat_flux": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -289,6 +290,14 @@ class Context:
     def set_precision(self, mode):
         """Bit mask: 0 = FP64; bit 0 / bit 1 = the 2B / 3B polynomial in FP32 (FP64 accumulation)."""
         _check(load_library().mbgpu_set_precision(self._h, int(mode)), "mbgpu_set_precision")
+
+    def set_elec_reciprocal(self, mode, order=6, grid=(32, 32, 32)):
+        """Reciprocal space of the electrostatics: mode 0 / "ewald" = the Ewald sum over k
+        (ewald1p.f); mode 1 / "spme" = particle-mesh Ewald of charges and dipoles with
+        B-splines of the given even order on the given grid (sizes 2^a 3^b 5^c)."""
+        m = {"ewald": 0, "spme": 1}.get(mode, mode)
+        g = (ctypes.c_int * 3)(*[int(k) for k in grid])
+        _check(load_library().mbgpu_set_elec_reciprocal(self._h, int(m), int(order), g), "mbgpu_set_elec_reciprocal")
 
     def decomposition(self):
         ae = np.zeros(self.natms)

@@ -127,6 +127,13 @@ module mbgpu
       integer(c_int), value :: mode
       integer(c_int) :: mbgpu_set_precision
     end function
+    function mbgpu_set_elec_reciprocal(ctx, mode, order, grid) bind(c, name="mbgpu_set_elec_reciprocal")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode, order
+      integer(c_int), intent(in) :: grid(3)
+      integer(c_int) :: mbgpu_set_elec_reciprocal
+    end function
     function mbgpu_get_lists(ctx, dimers, dcap, nd, trimers, tcap, nt) bind(c, name="mbgpu_get_lists")
       import :: c_int, c_long, c_ptr
       type(c_ptr), value :: ctx, dimers, trimers

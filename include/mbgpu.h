@@ -192,6 +192,18 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[
                                    double* x_dev, double* y_dev, double* z_dev, double* dip_dev, double* f_dev,
                                    double* engcpe, double* vircpe, double stress[9], int* niter);
 
+/* Reciprocal space of the electrostatics in this context: mode 0 = the Ewald sum over k
+ * (ewald1p.f, the reference's method for the polarizable model; default); mode 1 =
+ * smooth particle-mesh Ewald of charges and induced dipoles with cardinal B-splines of
+ * even order 4..12 on a grid[0] x grid[1] x grid[2] mesh (each size 2^a 3^b 5^c in
+ * [max(8, order), 256]), used for the solver's dipole fields, the reciprocal energy,
+ * forces, potentials, per-atom energies and per-atom stresses (the terms the heat current
+ * takes from ewald1p.f:550-693, 771-825), with the same alpha.  The reference's own SPME
+ * (ewald_spme.f, spme_for.f with bspgen.f, bspcoe.f, dlpfft3.f) handles point charges
+ * only; this carries the dipoles through the spread density.  Takes effect at the next
+ * electrostatics call.  No counterpart in the reference for the polarizable path. */
+int mbgpu_set_elec_reciprocal(mbgpu_ctx* ctx, int mode, int order, const int grid[3]);
+
 /* Results of the last electrostatics call (host arrays, any may be NULL):
  *   scal[20]: eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe, vir_cpe, vir_ew1p, vir_qd,
  *             iterations, last rms dipole change, stress[9] (in scal[10..18]), k vectors;

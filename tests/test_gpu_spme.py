@@ -2,7 +2,9 @@
 oracle's SPME and the Ewald reciprocal sum it approximates.
 
 * Same order and grid as the oracle: energy, potential, field and force within 1e-10
-  (the device sums the spread density in 64-bit fixed point, 2^-44 e per contribution).
+  (the device sums the spread density in 64-bit fixed point, 2^-44 e per contribution),
+  on grids of powers of two and of mixed radices 2, 3, 5 (48^3, 24 x 30 x 40, 45 x 20 x 36,
+  250 x 12 x 10); the oracle transforms by direct DFT sums.
 * Order 12 on a 64^3 grid against the Ewald sum over k: within 1e-9 (a triclinic cell).
 * The 512-water geometry with DMS-sized charges and induced-dipole-sized dipoles, order 8
   on 64^3, against the Ewald sum at kmax 16: the SPME discretisation error, 1e-5.
@@ -61,7 +63,8 @@ def rel(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max()
 
 
-@pytest.mark.parametrize("order,grid", [(4, (16, 16, 16)), (8, (32, 16, 64)), (12, (32, 32, 32))])
+@pytest.mark.parametrize("order,grid", [(4, (16, 16, 16)), (8, (32, 16, 64)), (12, (32, 32, 32)), (6, (48, 48, 48)),
+                                        (8, (24, 30, 40)), (10, (45, 20, 36)), (6, (250, 12, 10))])
 def test_matches_oracle_spme(dev, order, grid):
     cell, xyz, q, mu = system()
     print(f"spme order {order} grid {grid}", flush=True)
@@ -120,8 +123,9 @@ def test_repeat_shift_identity_and_charges_only(dev):
 def test_argument_and_input_errors(pkg, dev):
     cell, xyz, q, mu = system()
     for order, grid, what in ((5, (16, 16, 16), "order"), (14, (32, 32, 32), "order"),
-                              (8, (24, 16, 16), "power of two"), (8, (16, 16, 4), "power of two"),
-                              (8, (512, 16, 16), "power of two")):
+                              (8, (14, 16, 16), "3\\^b 5\\^c"), (8, (16, 16, 4), "3\\^b 5\\^c"),
+                              (8, (512, 16, 16), "3\\^b 5\\^c"), (8, (16, 21, 16), "3\\^b 5\\^c"),
+                              (12, (16, 10, 16), "3\\^b 5\\^c")):
         with pytest.raises(pkg.MbgpuError, match=what):
             dev.run(cell, order, grid, 0.35, xyz, q, mu)
     bad = xyz.copy()
