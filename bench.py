@@ -51,7 +51,7 @@ FLOP_SFAC = 12         # FLOP per (k vector, site) of the Ewald structure factor
 # HBM bytes per launch of k_el_sfac<false> at this workload, from rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE (KiB per dispatch) scaled by the same counters on a plain 512 MiB copy (FETCH x 2,
 # WRITE x 1); profiles/k_el_sfac_bench512_traffic.md
-SFAC_TRAFFIC_BYTES = (2903.7 * 2.0 + 20376.6 * 1.0) * 1024
+SFAC_TRAFFIC_BYTES = (4190.5 * 2.0 + 7486.5 * 1.0) * 1024
 
 
 def _cpu_eval(seed_and_budget):
@@ -123,57 +123,39 @@ def cpu_baseline(cores):
     return out
 
 
-def spme_leg(pkg, torch, reps):
-    """Particle-mesh Ewald reciprocal space (charges and dipoles: energy, potential, field
-    and force) at 512 and 2048 waters, order 8 on a 64^3 grid, against the Ewald
-    structure-factor pass of the same box (k_el_sfac, measured with HIP events after one
-    complete electrostatics evaluation with the box's Ewald parameters)."""
+def large_box_leg(pkg, torch, reps):
+    """The 2048-water box (L = 39.43 A, alpha 0.3 1/A, rcut 9 A): one complete force
+    evaluation with the heat flux (engine.Slot: MB-pol, electrostatics with the CG solver
+    from zero dipoles, water terms, J) with the Ewald sum over k at kmax 24 (33 530 k
+    vectors), and with the particle-mesh reciprocal space (order 6 on 48^3); the mesh's
+    error against the k-sum in forces, energy and heat current."""
     import numpy as np
-    out = []
-    f64 = dict(dtype=torch.float64, device="cuda")
-    for nw, L, nlat in ((512, 24.84, 8), (2048, 39.43, 13)):
-        box = pkg.waterbox.make_waterbox(nw, L, nlat, seed=900 + nw)
-        ctx = pkg.Context(box.nw, box.natms)
-        n = box.natms
-        rng = np.random.default_rng(nw)
-        q = np.concatenate([np.tile([0.0, 0.6, 0.6], nw), -1.2 * np.ones(nw)]) + rng.normal(size=n) * 0.02
-        q -= q.mean()
-        mu = rng.normal(size=(n, 3)) * 0.05
-        x, y, z = (torch.as_tensor(np.ascontiguousarray(box.xyz[:, a]), **f64) for a in range(3))
-        qd, md = torch.as_tensor(q, **f64), torch.as_tensor(mu.reshape(-1), **f64)
-        phi, E, F = torch.zeros(n, **f64), torch.zeros(3 * n, **f64), torch.zeros(3 * n, **f64)
-        cell = np.diag([box.L] * 3)
+    engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
+    box = pkg.waterbox.box_2048()
+    out, res = {}, {}
+    for name, recip in (("ewald_kmax24", None), ("spme_order6_48", ("spme", 6, (48, 48, 48)))):
+        slot = engine.Slot(box, recip=recip)
+        for _ in range(2):
+            slot.evaluate()
         torch.cuda.synchronize()
-        args = (cell, 8, (64, 64, 64), 0.3, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), qd.data_ptr(), md.data_ptr(),
-                phi.data_ptr(), E.data_ptr(), F.data_ptr())
-        for _ in range(3):
-            ctx.spme_recip_dev(*args)
         t0 = time.perf_counter()
         for _ in range(reps):
-            ctx.spme_recip_dev(*args)
+            r = slot.evaluate()
+        torch.cuda.synchronize()
         ms = 1e3 * (time.perf_counter() - t0) / reps
-        # the field of the dipoles alone, as a solver iteration needs it
-        q0 = torch.zeros_like(qd)
-        fargs = (cell, 8, (64, 64, 64), 0.3, n, x.data_ptr(), y.data_ptr(), z.data_ptr(), q0.data_ptr(), md.data_ptr(),
-                 None, E.data_ptr(), None)
-        torch.cuda.synchronize()
-        ctx.spme_recip_dev(*fargs)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            ctx.spme_recip_dev(*fargs)
-        ms_field = 1e3 * (time.perf_counter() - t0) / reps
-        ep = pkg.elec_params.for_box(box.L)
-        ctx.polar_electrostatics_dev(box.imcon, box.cell, pkg.ElecParams.from_params(ep, distribute_msite=1, solver=1),
-                                     x.data_ptr(), y.data_ptr(), z.data_ptr(), None)
-        ms_sf, _ = ctx.time_elec(max(5, reps // 2))
-        out.append({"waters": nw, "sites": n, "L": box.L, "order": 8, "grid": 64, "ms_per_call": ms,
-                    "ms_per_call_dipole_field": ms_field,
-                    "ewald_kmax": list(ep.kmax), "ewald_k_vectors": int(ctx.elec_details()["k_vectors"]),
-                    "ewald_sfac_ms": ms_sf})
-    return {"sizes": out, "what": "mbgpu_spme_recip_dev per call (host-synchronous, spread + 3-D FFT + influence "
-                                  "+ inverse FFT + gather of phi, E and F; and of the dipoles' field alone) vs one launch of the Ewald structure-factor "
-                                  "pass k_el_sfac of the same box (the solver runs it and a field pass of about the "
-                                  "same time every iteration)"}
+        res[name] = (r, slot.forces())
+        out[name] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "dipole_iterations": r["dipole_iterations"]}
+        del slot
+    (re_, fe), (rs, fs) = res["ewald_kmax24"], res["spme_order6_48"]
+    out["spme_vs_ewald"] = {
+        "rel_force": float(np.abs(fs - fe).max() / np.abs(fe).max()),
+        "rms_dF_kcal_mol_A": float(np.sqrt(((fs - fe) ** 2).mean()) / 418.4),
+        "rel_energy_of_elec": float(abs(rs["eng"] - re_["eng"]) / abs(re_["eng_elec"])),
+        "rel_J": float(np.abs(rs["jtot"] - re_["jtot"]).max() / np.abs(re_["jtot"]).max()),
+        "speedup": out["ewald_kmax24"]["ms_per_eval"] / out["spme_order6_48"]["ms_per_eval"]}
+    out["what"] = ("2048 H2O, complete evaluation with the heat flux, host-timed over " + str(reps) +
+                   " evaluations after 2 warm-up, reciprocal space by the k-sum and by the particle mesh")
+    return out
 
 
 def serve_leg(pkg, args):
@@ -247,8 +229,10 @@ def main():
                     help="GPU_MAX_HW_QUEUES for the run (0: one per concurrent configuration, at most 16)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="evaluate the electrostatics after MB-pol and the water terms instead of beside them")
-    ap.add_argument("--spme-reps", type=int, default=20,
-                    help="calls per size of the particle-mesh Ewald leg (0: skip)")
+    ap.add_argument("--only-large", action="store_true",
+                    help="run the 2048-water leg alone and print its record (for profiling)")
+    ap.add_argument("--large-reps", type=int, default=5,
+                    help="evaluations per reciprocal method of the 2048-water leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-mix", choices=["beads", "mixed"], default="mixed",
                     help="beads: PIMD beads of 256 waters only (same-size slots); mixed: beads and isolated "
@@ -286,6 +270,9 @@ def main():
     pkg = importlib.import_module("dlpoly2-mbpol-heat_amd")
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.only_large:
+        print(json.dumps({"large_box": large_box_leg(pkg, torch, max(1, args.large_reps))}), flush=True)
+        return
     box = pkg.waterbox.box_512()
     ctx = pkg.Context(box.nw, box.natms)
     ep = pkg.elec_params.for_box(box.L)
@@ -430,7 +417,7 @@ def main():
     serving = None
     if args.serve_requests > 0:
         serving = serve_leg(pkg, args)
-    spme_out = spme_leg(pkg, torch, args.spme_reps) if args.spme_reps > 0 else None
+    large_out = large_box_leg(pkg, torch, args.large_reps) if args.large_reps > 0 else None
     # headline: the fastest leg whose forces are within 1e-5 kcal/mol/A RMS of FP64
     lead = ("fp64 single", single, ms, "f64")
     if "fp64" in batched and batched["fp64"]["evals_per_s"] > single:
@@ -448,8 +435,8 @@ def main():
         legs["nve"] = nve_out
     if pimd_out is not None:
         legs["pimd"] = pimd_out
-    if spme_out is not None:
-        legs["spme"] = spme_out
+    if large_out is not None:
+        legs["large_box"] = large_out
     if "mixed" in batched:
         m = batched["mixed"]
         legs["mixed_vs_fp64"] = {"rms_dF_kcal_mol_A": m["rms_dF"], "max_dF_kcal_mol_A": m["max_dF"],
@@ -487,8 +474,9 @@ def main():
                      "kernel": "k_el_sfac<false> (dipole structure factors, launched once per solver iteration)",
                      "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                      "traffic": SFAC_TRAFFIC_BYTES,
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this workload, scaled by a "
-                                       "plain copy (profiles/k_el_sfac_bench512_traffic.md), bytes per launch",
+                     "traffic_source": "copied from profiles/k_el_sfac_bench512_traffic.md (rocprofv3 --pmc FETCH_SIZE "
+                                       "and WRITE_SIZE of this workload and build, scaled by a plain copy), bytes "
+                                       "per launch; not measured in this run",
                      "kernel_ms": ms_sf, "flop_per_launch": flop_sf,
                      "flop_per_unit": FLOP_SFAC, "units_per_launch": f"{kvec} k vectors x {box.nreal} sites",
                      "frac_with_recurrence": flop_sf_lib / (ms_sf * 1e-3) / 1e12 / PEAK_FP64},

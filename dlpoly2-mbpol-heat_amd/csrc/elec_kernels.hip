@@ -2121,10 +2121,8 @@ static void el_spme_dipole_field(ElecBuffers& E, const ElecSpme& sp, int n, cons
     spec.mu_on[0] = 1;
     spec.comp[0] = -1;
     double2* g = sp.B->grid;
-    sp_launch_spread(*sp.B, *sp.G, n, nullptr, mu, spec, g, state, s);
-    sp_launch_fft(*sp.B, *sp.G, g, 1, false, state, s);
-    sp_launch_influence(*sp.B, *sp.G, g, 1, false, state, s);
-    sp_launch_fft(*sp.B, *sp.G, g, 1, true, state, s);
+    sp_launch_spread(*sp.B, *sp.G, n, nullptr, mu, spec, g, true, state, s);
+    sp_launch_convolve(*sp.B, *sp.G, g, 1, true, state, s);
     sp_launch_field(*sp.B, *sp.G, n, g, 1, sp.kap, E.Fpart, E.n, state, s);
 }
 
@@ -2153,10 +2151,8 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, co
         spec.q_on[1] = 1;
         spec.comp[1] = -1;
         double2* g = sp->B->grid;
-        sp_launch_spread(*sp->B, *sp->G, P.n, E.chge, E.dip, spec, g, nullptr, s);
-        sp_launch_fft(*sp->B, *sp->G, g, 2, false, nullptr, s);
-        sp_launch_influence(*sp->B, *sp->G, g, 2, false, nullptr, s);
-        sp_launch_fft(*sp->B, *sp->G, g, 2, true, nullptr, s);
+        sp_launch_spread(*sp->B, *sp->G, P.n, E.chge, E.dip, spec, g, true, nullptr, s);
+        sp_launch_convolve(*sp->B, *sp->G, g, 2, true, nullptr, s);
         sp_launch_field(*sp->B, *sp->G, P.n, g, 2, sp->kap, E.Fpart, P.n, nullptr, s);
         hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(6L * P.n, 64)), dim3(256), 0, s, one_chunk(P),
                            E.Fpart, E.dip, E.efdd, E.efdc, E.state);
@@ -2303,11 +2299,11 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
         SpmeBuffers& B = *sp->B;
         const SpmeGeom& G = *sp->G;
         double2* g = B.grid;
-        sp_launch_spread(B, G, P.n, E.chge, E.dip, spec, g, nullptr, s);
-        sp_launch_fft(B, G, g, 5, false, nullptr, s);
+        sp_launch_spread(B, G, P.n, E.chge, E.dip, spec, g, true, nullptr, s);
+        sp_launch_fft(B, G, g, 5, false, nullptr, s, true);
         sp_launch_influence_stress(B, G, g, g + 5 * B.ng_cap, s);
         sp_launch_influence(B, G, g + B.ng_cap, 4, false, nullptr, s);
-        sp_launch_fft(B, G, g, 11, true, nullptr, s);
+        sp_launch_fft(B, G, g, 11, true, nullptr, s, false);
         sp_launch_elec_final(B, G, P.n, E.chge, E.dip, g, sp->kap, E.ksum, b_ks, s);
     } else if (!refresh) {
         // structure factors of the converged dipoles (already computed with the refreshed fields)

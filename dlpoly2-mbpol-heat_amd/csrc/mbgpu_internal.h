@@ -273,6 +273,8 @@ struct SpmeAxis {
 struct SpmeBuffers {
     int4* gi = nullptr;                    // per site: grid index of t = 0 per axis, ok
     double* th = nullptr;                  // per site and axis: M_p, M_p', M_p'' (SPME_MAX_ORDER each)
+    int* bin_off = nullptr;                // [K1 + 1] offsets of the grid planes' site lists
+    int* bin_site = nullptr;               // sites by grid plane
     long n_cap = 0;
     double2* grid = nullptr;               // complex grids, ng_cap points apart
     double* bmod = nullptr;                // |b(m)|^2 of the three axes, concatenated
@@ -302,10 +304,16 @@ inline long spme_points(const SpmeGeom& G) { return (long)G.K[0] * G.K[1] * G.K[
 // done flag (state[2] != 0) turns the steps into no-ops.
 void sp_launch_prep(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const double* y, const double* z,
                     hipStream_t s);
+// fwdz: the grids leave the spreading with their axis-2 forward pass done
 void sp_launch_spread(SpmeBuffers& B, const SpmeGeom& G, int n, const double* q, const double* mu, const SpSpec& spec,
-                      double2* grids, const double* state, hipStream_t s);
+                      double2* grids, bool fwdz, const double* state, hipStream_t s);
+// zdone: skip the axis-2 forward pass (the spreading did it)
 void sp_launch_fft(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool inverse, const double* state,
-                   hipStream_t s);
+                   hipStream_t s, bool zdone);
+// the whole convolution of ngr grids in place: forward transform, Q^ *= B C, backward
+// transform (no energy; the axis-0 passes and the influence function in one step)
+void sp_launch_convolve(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool zdone, const double* state,
+                        hipStream_t s);
 // in place on ngr grids: Q^ *= B C; with part, the energy partials of each grid
 void sp_launch_influence(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool energy,
                          const double* state, hipStream_t s);

@@ -54,22 +54,24 @@ __device__ __forceinline__ int sp_wrap(long g, int K)
     return (int)(r < 0 ? r + K : r);
 }
 
-// M_p(w + t) (and M_p', M_p'' when asked) for t = 0..p-1 by the recursion in the order
-// (bspgen.f:27-100): M_n(x) = [x M_{n-1}(x) + (n - x) M_{n-1}(x - 1)] / (n - 1) from
-// M_2(w) = w, M_2(w + 1) = 1 - w; M_n' = M_{n-1}(x) - M_{n-1}(x - 1).
-template <int NDER>
-__device__ void sp_splines(int p, double w, double* a, double* d, double* d2)
+template <int P, int NDER>
+__device__ __forceinline__ void sp_splines_p(double w, double* a, double* d, double* d2)
 {
-    double m[SPME_MAX_ORDER], m1[SPME_MAX_ORDER], m2[SPME_MAX_ORDER];
-    for (int t = 0; t < SPME_MAX_ORDER; ++t) m[t] = m1[t] = m2[t] = 0.0;
+    double m[P], m1[P], m2[P];
+#pragma unroll
+    for (int t = 0; t < P; ++t) m[t] = m1[t] = m2[t] = 0.0;
     m[0] = w;
     m[1] = 1.0 - w;
-    for (int n = 3; n <= p; ++n) {
-        if (n == p - 1)
-            for (int t = 0; t < SPME_MAX_ORDER; ++t) m2[t] = m[t];
-        if (n == p)
-            for (int t = 0; t < SPME_MAX_ORDER; ++t) m1[t] = m[t];
+#pragma unroll
+    for (int n = 3; n <= P; ++n) {
+        if (n == P - 1)
+#pragma unroll
+            for (int t = 0; t < P; ++t) m2[t] = m[t];
+        if (n == P)
+#pragma unroll
+            for (int t = 0; t < P; ++t) m1[t] = m[t];
         double prev = 0.0;
+#pragma unroll
         for (int t = 0; t < n; ++t) {
             const double x = w + t;
             const double cur = t < n - 1 ? m[t] : 0.0;
@@ -78,11 +80,28 @@ __device__ void sp_splines(int p, double w, double* a, double* d, double* d2)
             m[t] = nx;
         }
     }
-    for (int t = 0; t < p; ++t) {
+#pragma unroll
+    for (int t = 0; t < P; ++t) {
         a[t] = m[t];
-        if (NDER >= 1) d[t] = (t < p - 1 ? m1[t] : 0.0) - (t > 0 ? m1[t - 1] : 0.0);
+        if (NDER >= 1) d[t] = (t < P - 1 ? m1[t] : 0.0) - (t > 0 ? m1[t - 1] : 0.0);
         if (NDER >= 2)
-            d2[t] = (t < p - 2 ? m2[t] : 0.0) - 2.0 * (t > 0 ? m2[t - 1] : 0.0) + (t > 1 ? m2[t - 2] : 0.0);
+            d2[t] = (t < P - 2 ? m2[t] : 0.0) - 2.0 * (t > 0 ? m2[t - 1] : 0.0) + (t > 1 ? m2[t - 2] : 0.0);
+    }
+}
+
+// M_p(w + t) (and M_p', M_p'' when asked) for t = 0..p-1 by the recursion in the order
+// (bspgen.f:27-100): M_n(x) = [x M_{n-1}(x) + (n - x) M_{n-1}(x - 1)] / (n - 1) from
+// M_2(w) = w, M_2(w + 1) = 1 - w; M_n' = M_{n-1}(x) - M_{n-1}(x - 1).  Compiled per order
+// so that the recursion stays in registers.
+template <int NDER>
+__device__ void sp_splines(int p, double w, double* a, double* d, double* d2)
+{
+    switch (p) {
+    case 4: sp_splines_p<4, NDER>(w, a, d, d2); break;
+    case 6: sp_splines_p<6, NDER>(w, a, d, d2); break;
+    case 8: sp_splines_p<8, NDER>(w, a, d, d2); break;
+    case 10: sp_splines_p<10, NDER>(w, a, d, d2); break;
+    default: sp_splines_p<12, NDER>(w, a, d, d2); break;
     }
 }
 
@@ -97,6 +116,8 @@ __device__ __forceinline__ bool sp_frac(const SpmeGeom& G, int a, double x, doub
     w = u - fl;
     return true;
 }
+
+#define SP_MAX_SPREAD 5   // densities one spreading launch takes
 
 __global__ __launch_bounds__(64) void k_sp_clear_flag(int* __restrict__ flag)
 {
@@ -139,119 +160,38 @@ __global__ __launch_bounds__(256) void k_sp_prep(SpmeGeom G, int n, const double
     }
 }
 
-// Spread by tiles: a workgroup owns plane g1 and rows r0..r0+R-1 of the grid (all g3) of
-// each of the spec.ngr densities in LDS as 64-bit fixed point (2^-44 e), scans every site,
-// lists those whose support meets the tile and adds their contributions with LDS atomics
-// (integer adds: the tile does not depend on their order), then writes the tiles as complex
-// values to grids + g * gs.  Contribution of a site with charge c and dipole weights w_a =
-// K_a (a*_a . mu) at (t1, t2, t3): c M1 M2 M3 + w1 M1' M2 M3 + w2 M1 M2' M3 + w3 M1 M2 M3'.
-#define SP_MAX_SPREAD 5
-__global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* __restrict__ gi,
-                                                 const double* __restrict__ th, const double* __restrict__ q,
-                                                 const double* __restrict__ mu, SpSpec spec,
-                                                 double2* __restrict__ grids, long gs, int* __restrict__ flag,
-                                                 const double* __restrict__ state)
-{
-    if (state && state[2] != 0.0) return;
-    extern __shared__ unsigned long long tile[];   // [ngr][R][K3]
-    __shared__ int list[256], lt1[256];
-    __shared__ int cnt;
-    const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2], p = G.p;
-    const int R = G.rows, nrb = K2 / R, ngr = spec.ngr;
-    const int TS = R * K3;
-    const int g1 = blockIdx.x / nrb, r0 = (blockIdx.x - g1 * nrb) * R;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < ngr * TS; idx += 256) tile[idx] = 0ull;
-    bool big = false;
-    for (int base = 0; base < n; base += 256) {
-        if (tid == 0) cnt = 0;
-        __syncthreads();
-        const int i = base + tid;
-        if (i < n) {
-            const int4 g = gi[i];
-            int t1 = g.x - g1;
-            t1 += t1 < 0 ? K1 : 0;
-            if (g.w && t1 < p) {
-                int d = g.y - r0;
-                d += d < 0 ? K2 : 0;
-                bool hit = false;
-                for (int t2 = 0; t2 < p; ++t2) {
-                    int r = d - t2;
-                    r += r < 0 ? K2 : 0;
-                    hit |= r < R;
-                }
-                if (hit) {
-                    const int k = atomicAdd(&cnt, 1);
-                    list[k] = i;
-                    lt1[k] = t1;
-                }
-            }
-        }
-        __syncthreads();
-        const int m = cnt;
-        for (int wk = tid; wk < m * p; wk += 256) {
-            const int e = wk / p, t2 = wk - e * p;
-            const int i2 = list[e], t1 = lt1[e];
-            const int4 g = gi[i2];
-            int r = g.y - r0 - t2;
-            r += r < 0 ? K2 : 0;
-            r += r < 0 ? K2 : 0;
-            if (r >= R) continue;
-            const double* T = th + (long)i2 * 9 * SPME_MAX_ORDER;
-            const double a1 = T[t1], d1 = T[SPME_MAX_ORDER + t1];
-            const double a2 = T[3 * SPME_MAX_ORDER + t2], d2 = T[4 * SPME_MAX_ORDER + t2];
-            const double* T3 = T + 6 * SPME_MAX_ORDER;
-            const double qi = q ? q[i2] : 0.0;
-            double mv[3] = {0.0, 0.0, 0.0};
-            if (mu)
-                for (int c = 0; c < 3; ++c) mv[c] = mu[3 * i2 + c];
-            double wa[3];
-            for (int a = 0; a < 3; ++a) wa[a] = G.K[a] * (G.rv[a][0] * mv[0] + G.rv[a][1] * mv[1] + G.rv[a][2] * mv[2]);
-            for (int gr = 0; gr < ngr; ++gr) {
-                const double c = (spec.q_on[gr] ? qi : 0.0) + (spec.comp[gr] >= 0 ? mv[spec.comp[gr]] : 0.0);
-                const double mo = spec.mu_on[gr] ? 1.0 : 0.0;
-                const double c12 = c * a1 * a2 + mo * (wa[0] * d1 * a2 + wa[1] * a1 * d2);
-                const double w3 = mo * wa[2] * a1 * a2;
-                unsigned long long* row = tile + gr * TS + r * K3;
-                for (int t3 = 0; t3 < p; ++t3) {
-                    const double v = c12 * T3[t3] + w3 * T3[SPME_MAX_ORDER + t3];
-                    big |= !(fabs(v) < 1024.0);
-                    int g3 = g.z - t3;
-                    g3 += g3 < 0 ? K3 : 0;
-                    atomicAdd(row + g3, (unsigned long long)llrint(v * SP_FIX));
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (big) flag[0] = 2;
-    for (int idx = tid; idx < ngr * TS; idx += 256) {
-        const int gr = idx / TS, rem = idx - gr * TS;
-        const int r = rem / K3, c3 = rem - r * K3;
-        grids[gr * gs + ((long)g1 * K2 + r0 + r) * K3 + c3] =
-            make_double2((double)(long long)tile[idx] * SP_UNFIX, 0.0);
-    }
-}
-
 // Mixed-radix Stockham transform of L lines of length K = 2^a 3^b 5^c along one axis.
 // Line l of block (o, ib) starts at o * so + (ib * L + l) * si, element e at + e * se.
 // Pass f with radix R after Ns = (product of the earlier radices): butterfly j < K/R reads
 // elements j + q K/R (q < R), multiplies element q by w^(q k) with k = j mod Ns and
 // w = exp(-2 pi i / (Ns R)) (conjugated backward), takes the length-R DFT and writes it to
-// (j / Ns) Ns R + k + q Ns.  tw holds exp(-2 pi i t / K), t < K.
+// (j / Ns) Ns R + k + q Ns.  tw holds exp(-2 pi i t / K), t < K, staged in LDS with the
+// lines; the radices are 4 (pairs of 2s), 2, 3, 5, each pass compiled for its radix.
 __device__ __forceinline__ double2 sp_cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // v <- DFT_R(v) with exp(sgn 2 pi i q m / R), sgn = -1 forward, +1 backward
-__device__ __forceinline__ void sp_dft_small(int R, double sgn, double2* v)
+template <int R>
+__device__ __forceinline__ void sp_dft(double sgn, double2 (&v)[R])
 {
-    if (R == 2) {
+    if constexpr (R == 2) {
         const double2 a = v[0], b = v[1];
         v[0] = make_double2(a.x + b.x, a.y + b.y);
         v[1] = make_double2(a.x - b.x, a.y - b.y);
-    } else if (R == 3) {
+    } else if constexpr (R == 4) {
+        // y1 = (v0 - v2) + W (v1 - v3), y3 = (v0 - v2) - W (v1 - v3), W = sgn i
+        const double2 s02 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+        const double2 d02 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+        const double2 s13 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+        const double2 d13 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+        const double2 wd = make_double2(-sgn * d13.y, sgn * d13.x);
+        v[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+        v[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+        v[1] = make_double2(d02.x + wd.x, d02.y + wd.y);
+        v[3] = make_double2(d02.x - wd.x, d02.y - wd.y);
+    } else if constexpr (R == 3) {
         const double h = 0.86602540378443864676;   // sin(2 pi / 3)
         const double2 t1 = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
         const double2 t2 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
@@ -279,6 +219,74 @@ __device__ __forceinline__ void sp_dft_small(int R, double sgn, double2* v)
     }
 }
 
+// one pass of radix R over the L lines of in (K apart) into out; blockDim threads
+template <int R>
+__device__ __forceinline__ void sp_pass(const double2* __restrict__ in, double2* __restrict__ out, int K, int L,
+                                        int Ns, const double2* __restrict__ tws, bool inverse)
+{
+    const int KR = K / R, ts = K / (Ns * R);
+    const double sgn = inverse ? 1.0 : -1.0;
+    for (int b = threadIdx.x; b < L * KR; b += blockDim.x) {
+        const int l = b / KR, j = b - l * KR;
+        const int k = j % Ns;
+        const double2* li = in + l * K;
+        double2 v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            v[q] = li[j + q * KR];
+            if (q > 0 && k > 0) {
+                double2 w = tws[q * k * ts];
+                if (inverse) w.y = -w.y;
+                v[q] = sp_cmul(v[q], w);
+            }
+        }
+        sp_dft<R>(sgn, v);
+        double2* lo = out + l * K + (j - k) * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) lo[q * Ns] = v[q];
+    }
+}
+
+// every pass of axis A over L lines staged in b0 (b1 the other buffer); returns the
+// buffer that holds the result.  Ends with a barrier.
+__device__ double2* sp_fft_lines(double2* b0, double2* b1, int K, int L, int nf, const int* fac,
+                                 const double2* __restrict__ tws, bool inverse)
+{
+    double2* in = b0;
+    double2* out = b1;
+    int Ns = 1;
+    for (int f = 0; f < nf; ++f) {
+        const int R = fac[f];
+        if (R == 4)
+            sp_pass<4>(in, out, K, L, Ns, tws, inverse);
+        else if (R == 2)
+            sp_pass<2>(in, out, K, L, Ns, tws, inverse);
+        else if (R == 3)
+            sp_pass<3>(in, out, K, L, Ns, tws, inverse);
+        else
+            sp_pass<5>(in, out, K, L, Ns, tws, inverse);
+        __syncthreads();
+        double2* t = in;
+        in = out;
+        out = t;
+        Ns *= R;
+    }
+    return in;
+}
+
+// line l / element e of the staged index idx: lines side by side for strided axes (so
+// that neighbouring threads touch neighbouring lines), elements side by side otherwise
+__device__ __forceinline__ void sp_line_elem(const SpmeAxis& A, int idx, int& l, int& e)
+{
+    if (A.se == 1) {
+        l = idx / A.K;
+        e = idx - l * A.K;
+    } else {
+        e = idx / A.L;
+        l = idx - e * A.L;
+    }
+}
+
 // blockIdx.y: the grid (grids gs points apart)
 __global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_fft(double2* __restrict__ grids, long gs, SpmeAxis A,
                                                            const double2* __restrict__ tw, int inverse,
@@ -286,60 +294,203 @@ __global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_fft(double2* __restrict__
 {
     if (state && state[2] != 0.0) return;
     __shared__ double2 buf[2][SP_FFT_ELEMS > SPME_MAX_GRID ? SP_FFT_ELEMS : SPME_MAX_GRID];
+    __shared__ double2 tws[SPME_MAX_GRID];
     double2* __restrict__ g = grids + blockIdx.y * gs;
     const int K = A.K, L = A.L;
     const int nib = A.ni / L;
     const int o = blockIdx.x / nib, ib = blockIdx.x - o * nib;
     const long base = (long)o * A.so + (long)ib * L * A.si;
     const int LK = L * K;
+    for (int t = threadIdx.x; t < K; t += SP_FFT_THREADS) tws[t] = tw[t];
     for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
         int l, e;
-        if (A.se == 1) {
-            l = idx / K;
-            e = idx - l * K;
-        } else {
-            e = idx / L;
-            l = idx - e * L;
-        }
+        sp_line_elem(A, idx, l, e);
         buf[0][l * K + e] = g[base + (long)l * A.si + (long)e * A.se];
     }
     __syncthreads();
-    const double sgn = inverse ? 1.0 : -1.0;
-    int src = 0, Ns = 1;
-    for (int f = 0; f < A.nf; ++f) {
-        const int R = A.fac[f], KR = K / R, ts = K / (Ns * R);
-        for (int b = threadIdx.x; b < L * KR; b += SP_FFT_THREADS) {
-            const int l = b / KR, j = b - l * KR;
-            const double2* in = buf[src] + l * K;
-            double2* out = buf[src ^ 1] + l * K;
-            const int k = j % Ns;
-            double2 v[5];
-            for (int q = 0; q < R; ++q) {
-                v[q] = in[j + q * KR];
-                if (q > 0 && k > 0) {
-                    double2 w = tw[q * k * ts];
-                    if (inverse) w.y = -w.y;
-                    v[q] = sp_cmul(v[q], w);
-                }
-            }
-            sp_dft_small(R, sgn, v);
-            const int d = (j - k) * R + k;
-            for (int q = 0; q < R; ++q) out[d + q * Ns] = v[q];
-        }
-        __syncthreads();
-        src ^= 1;
-        Ns *= R;
-    }
+    const double2* res = sp_fft_lines(buf[0], buf[1], K, L, A.nf, A.fac, tws, inverse != 0);
     for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
         int l, e;
-        if (A.se == 1) {
-            l = idx / K;
-            e = idx - l * K;
-        } else {
-            e = idx / L;
-            l = idx - e * L;
+        sp_line_elem(A, idx, l, e);
+        g[base + (long)l * A.si + (long)e * A.se] = res[l * K + e];
+    }
+}
+
+// Sites by grid plane (gi.x), for the spreading tiles: bin_off[0..K1] offsets into
+// bin_site, which lists the sites of each plane (their order within a plane is whatever
+// the LDS counters give: the spread adds integers, so it does not depend on it).  Sites
+// that cannot index the grid are left out.  One workgroup.
+__global__ __launch_bounds__(1024) void k_sp_bin(int n, int K1, const int4* __restrict__ gi, int* __restrict__ bin_off,
+                                                 int* __restrict__ bin_site)
+{
+    __shared__ int cnt[SPME_MAX_GRID], pos[SPME_MAX_GRID];
+    const int tid = threadIdx.x;
+    for (int t = tid; t < K1; t += 1024) cnt[t] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += 1024) {
+        const int4 g = gi[i];
+        if (g.w) atomicAdd(&cnt[g.x], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int a = 0;
+        for (int t = 0; t < K1; ++t) {
+            pos[t] = a;
+            bin_off[t] = a;
+            a += cnt[t];
         }
-        g[base + (long)l * A.si + (long)e * A.se] = buf[src][l * K + e];
+        bin_off[K1] = a;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 1024) {
+        const int4 g = gi[i];
+        if (g.w) bin_site[atomicAdd(&pos[g.x], 1)] = i;
+    }
+}
+
+// Spread by tiles: a workgroup owns plane g1 and rows r0..r0+R-1 of the grid (all g3) of
+// each of the spec.ngr densities in LDS as 64-bit fixed point (2^-44 e), takes the sites
+// of planes g1..g1+p-1 (the only ones whose support reaches g1) from the plane bins, keeps
+// those whose rows meet the tile and adds their contributions with LDS atomics (integer
+// adds: the tile does not depend on their order).  Sites i >= n are skipped.  Contribution
+// of a site with charge c and dipole weights w_a = K_a (a*_a . mu) at (t1, t2, t3):
+// c M1 M2 M3 + w1 M1' M2 M3 + w2 M1 M2' M3 + w3 M1 M2 M3'.  The tiles are written as
+// complex values to grids + g * gs; with fwdz, after the forward transform of their rows
+// (the axis-2 pass), from tw (the axis's twiddles) and A (its radices).
+__global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* __restrict__ gi,
+                                                 const double* __restrict__ th, const int* __restrict__ bin_off,
+                                                 const int* __restrict__ bin_site, const double* __restrict__ q,
+                                                 const double* __restrict__ mu, SpSpec spec,
+                                                 double2* __restrict__ grids, long gs, int fwdz, SpmeAxis A,
+                                                 const double2* __restrict__ tw, int* __restrict__ flag,
+                                                 const double* __restrict__ state)
+{
+    if (state && state[2] != 0.0) return;
+    extern __shared__ unsigned long long tile[];   // [ngr][R][K3], then (fwdz) 2 x [R][K3] + [K3] complex
+    constexpr int CU = 4;                          // candidates per thread and round
+    __shared__ int list[256 * CU], lt1[256 * CU];
+    __shared__ int cnt, boff[SPME_MAX_ORDER + 1], bstart[SPME_MAX_ORDER], bcnt[SPME_MAX_ORDER];
+    const int K1 = G.K[0], K2 = G.K[1], K3 = G.K[2], p = G.p;
+    const int R = G.rows, nrb = K2 / R, ngr = spec.ngr;
+    const int TS = R * K3;
+    const int g1 = blockIdx.x / nrb, r0 = (blockIdx.x - g1 * nrb) * R;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < ngr * TS; idx += 256) tile[idx] = 0ull;
+    if (tid < p) {
+        // the p plane bins g1 + t1, read side by side, then as one candidate range [0, boff[p])
+        int b = g1 + tid;
+        b -= b >= K1 ? K1 : 0;
+        const int o0 = bin_off[b], o1 = bin_off[b + 1];
+        bstart[tid] = o0;
+        bcnt[tid] = o1 - o0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int a = 0;
+        for (int t1 = 0; t1 < p; ++t1) {
+            boff[t1] = a;
+            a += bcnt[t1];
+        }
+        boff[p] = a;
+    }
+    __syncthreads();
+    const int ncand = boff[p];
+    bool big = false;
+    for (int base = 0; base < ncand; base += 256 * CU) {
+        if (tid == 0) cnt = 0;
+        __syncthreads();
+        // CU candidates per thread, their loads issued together
+        int ci[CU], ct[CU];
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+            const int c = base + tid + 256 * u;
+            ci[u] = n;
+            ct[u] = 0;
+            if (c < ncand) {
+                int t1 = 0;
+                while (c >= boff[t1 + 1]) ++t1;
+                ct[u] = t1;
+                ci[u] = bin_site[bstart[t1] + (c - boff[t1])];
+            }
+        }
+        int gy[CU];
+#pragma unroll
+        for (int u = 0; u < CU; ++u) gy[u] = ci[u] < n ? gi[ci[u]].y : -1;
+#pragma unroll
+        for (int u = 0; u < CU; ++u) {
+            if (gy[u] < 0) continue;
+            int d = gy[u] - r0;
+            d += d < 0 ? K2 : 0;
+            bool hit = false;
+            for (int t2 = 0; t2 < p; ++t2) {
+                int r = d - t2;
+                r += r < 0 ? K2 : 0;
+                hit |= r < R;
+            }
+            if (hit) {
+                const int k = atomicAdd(&cnt, 1);
+                list[k] = ci[u];
+                lt1[k] = ct[u];
+            }
+        }
+        __syncthreads();
+        const int m = cnt;
+        for (int wk = tid; wk < m * p; wk += 256) {
+            const int e = wk / p, t2 = wk - e * p;
+            const int i2 = list[e], t1 = lt1[e];
+            const int4 g = gi[i2];
+            int r = g.y - r0 - t2;
+            r += r < 0 ? K2 : 0;
+            r += r < 0 ? K2 : 0;
+            if (r >= R) continue;
+            const double* T = th + (long)i2 * 9 * SPME_MAX_ORDER;
+            const double a1 = T[t1], d1 = T[SPME_MAX_ORDER + t1];
+            const double a2 = T[3 * SPME_MAX_ORDER + t2], d2 = T[4 * SPME_MAX_ORDER + t2];
+            const double* T3 = T + 6 * SPME_MAX_ORDER;
+            const double qi = q ? q[i2] : 0.0;
+            double mv[3] = {0.0, 0.0, 0.0};
+            if (mu)
+                for (int cc = 0; cc < 3; ++cc) mv[cc] = mu[3 * i2 + cc];
+            double wa[3];
+            for (int a = 0; a < 3; ++a) wa[a] = G.K[a] * (G.rv[a][0] * mv[0] + G.rv[a][1] * mv[1] + G.rv[a][2] * mv[2]);
+            for (int gr = 0; gr < ngr; ++gr) {
+                const double cg = (spec.q_on[gr] ? qi : 0.0) + (spec.comp[gr] >= 0 ? mv[spec.comp[gr]] : 0.0);
+                const double mo = spec.mu_on[gr] ? 1.0 : 0.0;
+                const double c12 = cg * a1 * a2 + mo * (wa[0] * d1 * a2 + wa[1] * a1 * d2);
+                const double w3 = mo * wa[2] * a1 * a2;
+                unsigned long long* row = tile + gr * TS + r * K3;
+                for (int t3 = 0; t3 < p; ++t3) {
+                    const double v = c12 * T3[t3] + w3 * T3[SPME_MAX_ORDER + t3];
+                    big |= !(fabs(v) < 1024.0);
+                    int g3 = g.z - t3;
+                    g3 += g3 < 0 ? K3 : 0;
+                    atomicAdd(row + g3, (unsigned long long)llrint(v * SP_FIX));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (big) flag[0] = 2;
+    if (!fwdz) {
+        for (int idx = tid; idx < ngr * TS; idx += 256) {
+            const int gr = idx / TS, rem = idx - gr * TS;
+            const int r = rem / K3, c3 = rem - r * K3;
+            grids[gr * gs + ((long)g1 * K2 + r0 + r) * K3 + c3] =
+                make_double2((double)(long long)tile[idx] * SP_UNFIX, 0.0);
+        }
+        return;
+    }
+    double2* b0 = reinterpret_cast<double2*>(tile + ngr * TS);
+    double2* b1 = b0 + TS;
+    double2* tws = b1 + TS;
+    for (int t = tid; t < K3; t += 256) tws[t] = tw[t];
+    for (int gr = 0; gr < ngr; ++gr) {
+        for (int idx = tid; idx < TS; idx += 256) b0[idx] = make_double2((double)(long long)tile[gr * TS + idx] * SP_UNFIX, 0.0);
+        __syncthreads();
+        const double2* res = sp_fft_lines(b0, b1, K3, R, A.nf, A.fac, tws, false);
+        double2* out = grids + gr * gs + ((long)g1 * K2 + r0) * K3;
+        for (int idx = tid; idx < TS; idx += 256) out[idx] = res[idx];
+        __syncthreads();
     }
 }
 
@@ -408,6 +559,45 @@ __global__ __launch_bounds__(256) void k_sp_influence_stress(SpmeGeom G, double2
     for (int t = 0; t < 6; ++t) {
         const double w = f * mv[ia[t]] * mv[ib[t]];
         dst[t * gs + i] = make_double2(w * v.x, w * v.y);
+    }
+}
+
+// Axis 0 of grid blockIdx.y as one step: the forward pass, Q^ *= B C, the backward pass
+// (the last forward and the first backward pass of the convolution, with the influence
+// function between them, without the energy)
+__global__ __launch_bounds__(SP_FFT_THREADS) void k_sp_conv0(SpmeGeom G, double2* __restrict__ grids, long gs,
+                                                             SpmeAxis A, const double2* __restrict__ tw,
+                                                             const double* __restrict__ bmod,
+                                                             const double* __restrict__ state)
+{
+    if (state && state[2] != 0.0) return;
+    __shared__ double2 buf[2][SP_FFT_ELEMS > SPME_MAX_GRID ? SP_FFT_ELEMS : SPME_MAX_GRID];
+    __shared__ double2 tws[SPME_MAX_GRID];
+    double2* __restrict__ g = grids + blockIdx.y * gs;
+    const int K = A.K, L = A.L;
+    const long base = (long)blockIdx.x * L;   // axis 0: lines are the K2 K3 columns, side by side
+    const int LK = L * K;
+    for (int t = threadIdx.x; t < K; t += SP_FFT_THREADS) tws[t] = tw[t];
+    for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
+        int l, e;
+        sp_line_elem(A, idx, l, e);
+        buf[0][l * K + e] = g[base + l + (long)e * A.se];
+    }
+    __syncthreads();
+    double2* res = sp_fft_lines(buf[0], buf[1], K, L, A.nf, A.fac, tws, false);
+    for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
+        const int l = idx / K, e = idx - l * K;
+        double mv[3], msq;
+        const double bc = sp_bc(G, bmod, (long)e * A.se + base + l, mv, msq);
+        double2 v = res[idx];
+        res[idx] = make_double2(bc * v.x, bc * v.y);
+    }
+    __syncthreads();
+    const double2* out = sp_fft_lines(res, res == buf[0] ? buf[1] : buf[0], K, L, A.nf, A.fac, tws, true);
+    for (int idx = threadIdx.x; idx < LK; idx += SP_FFT_THREADS) {
+        int l, e;
+        sp_line_elem(A, idx, l, e);
+        g[base + l + (long)e * A.se] = out[l * K + e];
     }
 }
 
@@ -657,27 +847,56 @@ void sp_launch_prep(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, c
     if (n > 0)
         hipLaunchKernelGGL(k_sp_prep, dim3((unsigned)((3L * n + 254) / 255)), dim3(256), 0, s, G, n, x, y, z, B.gi,
                            B.th, B.flag);
+    hipLaunchKernelGGL(k_sp_bin, dim3(1), dim3(1024), 0, s, n, G.K[0], B.gi, B.bin_off, B.bin_site);
+}
+
+size_t sp_spread_lds(const SpmeGeom& G, int ngr, bool fwdz)
+{
+    const size_t ts = (size_t)G.rows * G.K[2];
+    return sizeof(unsigned long long) * ngr * ts + (fwdz ? sizeof(double2) * (2 * ts + G.K[2]) : 0);
 }
 
 void sp_launch_spread(SpmeBuffers& B, const SpmeGeom& G, int n, const double* q, const double* mu, const SpSpec& spec,
-                      double2* grids, const double* state, hipStream_t s)
+                      double2* grids, bool fwdz, const double* state, hipStream_t s)
 {
-    const size_t lds = sizeof(unsigned long long) * (size_t)spec.ngr * G.rows * G.K[2];
-    hipLaunchKernelGGL(k_sp_tile, dim3((unsigned)(G.K[0] * (G.K[1] / G.rows))), dim3(256), lds, s, G, n, B.gi, B.th,
-                       q, mu, spec, grids, B.ng_cap, B.flag, state);
+    hipLaunchKernelGGL(k_sp_tile, dim3((unsigned)(G.K[0] * (G.K[1] / G.rows))), dim3(256),
+                       sp_spread_lds(G, spec.ngr, fwdz), s, G, n, B.gi, B.th, B.bin_off, B.bin_site, q, mu, spec,
+                       grids, B.ng_cap, fwdz ? 1 : 0, B.axis[2], B.tw + B.twoff[2], B.flag, state);
 }
 
 void sp_launch_fft(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool inverse, const double* state,
-                   hipStream_t s)
+                   hipStream_t s, bool zdone)
 {
     (void)G;
-    // axes in the order 2, 1, 0 forward, 0, 1, 2 back
+    // axes in the order 2, 1, 0 forward (2 skipped when the spreading did it), 0, 1, 2 back
     for (int k = 0; k < 3; ++k) {
         const int a = inverse ? k : 2 - k;
+        if (!inverse && a == 2 && zdone) continue;
         const SpmeAxis& A = B.axis[a];
         const unsigned nblk = (unsigned)((long)A.no * (A.ni / A.L));
         hipLaunchKernelGGL(k_sp_fft, dim3(nblk, (unsigned)ngr), dim3(SP_FFT_THREADS), 0, s, grids, B.ng_cap, A,
                            B.tw + B.twoff[a], inverse ? 1 : 0, state);
+    }
+}
+
+void sp_launch_convolve(SpmeBuffers& B, const SpmeGeom& G, double2* grids, int ngr, bool zdone, const double* state,
+                        hipStream_t s)
+{
+    // forward axes 2 (unless the spreading did it) and 1, axis 0 forward - influence -
+    // backward in one step, backward 1, 2
+    for (int a : {2, 1}) {
+        if (a == 2 && zdone) continue;
+        const SpmeAxis& A = B.axis[a];
+        hipLaunchKernelGGL(k_sp_fft, dim3((unsigned)((long)A.no * (A.ni / A.L)), (unsigned)ngr), dim3(SP_FFT_THREADS),
+                           0, s, grids, B.ng_cap, A, B.tw + B.twoff[a], 0, state);
+    }
+    const SpmeAxis& A0 = B.axis[0];
+    hipLaunchKernelGGL(k_sp_conv0, dim3((unsigned)(A0.ni / A0.L), (unsigned)ngr), dim3(SP_FFT_THREADS), 0, s, G, grids,
+                       B.ng_cap, A0, B.tw + B.twoff[0], B.bmod, state);
+    for (int a : {1, 2}) {
+        const SpmeAxis& A = B.axis[a];
+        hipLaunchKernelGGL(k_sp_fft, dim3((unsigned)((long)A.no * (A.ni / A.L)), (unsigned)ngr), dim3(SP_FFT_THREADS),
+                           0, s, grids, B.ng_cap, A, B.tw + B.twoff[a], 1, state);
     }
 }
 
@@ -726,11 +945,11 @@ int spme_launch(SpmeBuffers& B, const SpmeGeom& G, int n, const double* x, const
     spec.q_on[0] = 1;
     spec.mu_on[0] = mu ? 1 : 0;
     spec.comp[0] = -1;
-    sp_launch_spread(B, G, n, q, mu, spec, B.grid, nullptr, s);
-    sp_launch_fft(B, G, B.grid, 1, false, nullptr, s);
+    sp_launch_spread(B, G, n, q, mu, spec, B.grid, true, nullptr, s);
+    sp_launch_fft(B, G, B.grid, 1, false, nullptr, s, true);
     sp_launch_influence(B, G, B.grid, 1, true, nullptr, s);
     sp_launch_usum(B, G, 1, s);
-    sp_launch_fft(B, G, B.grid, 1, true, nullptr, s);
+    sp_launch_fft(B, G, B.grid, 1, true, nullptr, s, false);
     if (n > 0 && f)
         hipLaunchKernelGGL(k_sp_gather<true>, dim3((n + 3) / 4), dim3(256), 0, s, G, n, B.gi, B.th, q, mu, B.grid, phi,
                            ef, f);
@@ -758,12 +977,12 @@ int sp_largest_divisor(int n, int cap)
     return 1;
 }
 
-// radices of the transform passes (the 2s, then the 3s, then the 5s) and L, the lines per
-// workgroup: the largest divisor of ni with L K <= SP_FFT_ELEMS
+// radices of the transform passes (the 2s in pairs as 4s, then a lone 2, the 3s, the 5s)
+// and L, the lines per workgroup: the largest divisor of ni with L K <= SP_FFT_ELEMS
 void sp_axis_plan(SpmeAxis& A)
 {
     int K = A.K, nf = 0;
-    for (int r : {2, 3, 5})
+    for (int r : {4, 2, 3, 5})
         while (K % r == 0) {
             A.fac[nf++] = r;
             K /= r;
@@ -794,6 +1013,8 @@ void spme_free(SpmeBuffers& B)
 {
     sp_free(B.gi);
     sp_free(B.th);
+    sp_free(B.bin_off);
+    sp_free(B.bin_site);
     sp_free(B.grid);
     sp_free(B.bmod);
     sp_free(B.tw);
@@ -858,28 +1079,33 @@ int spme_plan(SpmeBuffers& B, SpmeGeom& G, const double cell[9], int order, cons
     G.p = order;
     for (int a = 0; a < 3; ++a) G.K[a] = grid[a];
     const int K1 = grid[0], K2 = grid[1], K3 = grid[2];
-    // tile rows: LDS of up to SP_MAX_SPREAD tiles within 60 000 bytes (the site lists take
+    // tile rows: LDS of up to SP_MAX_SPREAD fixed-point tiles, the two complex row buffers
+    // of the fused axis-2 pass and its twiddles within 60 000 bytes (the site lists take
     // 2 KiB more of a workgroup's 64 KiB), at most 16 rows, and at least 512 workgroups
     // where the grid allows
     {
         const int nsp = std::min(ngr, SP_MAX_SPREAD);
         int pick = 0, smallest = 1;
         for (int R = std::min(16, K2); R >= 1; --R) {
-            if (K2 % R || (long)nsp * R * K3 * 8 > 60000) continue;
+            if (K2 % R || (long)(nsp * 8 + 32) * R * K3 + 16L * K3 > 60000) continue;
             smallest = R;
             if (!pick && (long)K1 * (K2 / R) >= 512) pick = R;
         }
         G.rows = pick ? pick : smallest;
     }
     const long ng = (long)K1 * K2 * K3;
-    if (B.n_cap < n) {
+    if (B.n_cap < n || !B.gi) {
         sp_free(B.gi);
         sp_free(B.th);
+        sp_free(B.bin_site);
         B.n_cap = 0;
-        SPCHK(mbgpu_alloc(&B.gi, (size_t)n));
-        SPCHK(mbgpu_alloc(&B.th, (size_t)n * 9 * SPME_MAX_ORDER));
+        const size_t nn = (size_t)std::max(n, 1);
+        SPCHK(mbgpu_alloc(&B.gi, nn));
+        SPCHK(mbgpu_alloc(&B.th, nn * 9 * SPME_MAX_ORDER));
+        SPCHK(mbgpu_alloc(&B.bin_site, nn));
         B.n_cap = n;
     }
+    if (!B.bin_off) SPCHK(mbgpu_alloc(&B.bin_off, (size_t)SPME_MAX_GRID + 1));
     if (B.ng_cap < ng || B.ngr_cap < ngr) {
         const long cap = std::max(B.ng_cap, ng);
         const int gcap = std::max(B.ngr_cap, ngr);
