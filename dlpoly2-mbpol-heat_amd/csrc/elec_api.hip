@@ -387,6 +387,7 @@ void elec_release(ElecState& S)
     }
     delete S.spg;
     S.spg = nullptr;
+    cells_free(S.cells);
     S.ready = false;
 }
 
@@ -444,6 +445,8 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
     for (int i = 0; i < 9; ++i) same_k = same_k && S.k_cell[i] == cell[i];
     for (int i = 0; i < 3; ++i) same_k = same_k && S.k_kmax[i] == p->kmax[i];
     if (!same_k && build_klist(S, cell, p->kmax, p->alpha, err)) return -1;
+    if (cells_plan(S.cells, cell, p->rcut, 4 * nw, CELLS_MIN_SITES, err)) return -1;
+    S.P.cells = S.cells.V;
     if (S.recip) {
         // particle mesh: eleven grids (the final pass's), splines of every site
         if (!S.sp) S.sp = new SpmeBuffers();
@@ -497,7 +500,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     for (int attempt = 0;; ++attempt) {
         if (attempt) ECHK(hipMemcpyAsync(E.dip, E.dip0, sizeof(double) * 3 * P.n, hipMemcpyDeviceToDevice, s));
         ECHK(hipMemsetAsync(E.err, 0, 4 * sizeof(int), s));
-        int rc = el_launch_setup(E, P, s);
+        int rc = el_launch_setup(E, P, S.cells, s);
         if (rc) {
             err = std::string("electrostatics set-up kernels failed: ") + hipGetErrorString((hipError_t)rc);
             return rc;

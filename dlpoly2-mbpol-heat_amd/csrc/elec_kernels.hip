@@ -292,8 +292,11 @@ __global__ void k_el_sites(ElecDev P, double* __restrict__ x, double* __restrict
 }
 
 // Neighbour rows: every site of another water within rcut (superset by a relative 1e-9 of
-// rcut^2; each kernel applies the reference's own test), ascending partner index.  Row P
-// holds the partners of a polarizable site that are polarizable.  One wavefront per site.
+// rcut^2; each kernel applies the reference's own test), found in the 27 link cells around
+// the site when the box holds three cells per axis (the Verlet list over link cells of
+// parlink.f), else by a scan of every site; either way in an order fixed by the positions.
+// Row P holds the partners of a polarizable site that are polarizable.  One wavefront per
+// site.
 __global__ __launch_bounds__(64) void k_el_nbr(ElecDev P, const double* __restrict__ x,
                                                const double* __restrict__ y, const double* __restrict__ z,
                                                int* __restrict__ nbrA, int* __restrict__ cntA,
@@ -307,10 +310,10 @@ __global__ __launch_bounds__(64) void k_el_nbr(ElecDev P, const double* __restri
     const double rl2 = P.rcut * P.rcut * (1.0 + 1e-9);
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int nA = 0, nP = 0;
-    for (int base = 0; base < P.n; base += 64) {
-        const int j = base + lane;
+    // candidates j (or -1) one per lane, kept in lane order
+    auto take = [&](int j) {
         bool inA = false, inP = false;
-        if (j < P.n && water_of(P, j) != wi) {
+        if (j >= 0 && water_of(P, j) != wi) {
             double d[3];
             const double r2 = sep(P, x, y, z, i, j, d);
             inA = r2 < rl2;
@@ -327,6 +330,14 @@ __global__ __launch_bounds__(64) void k_el_nbr(ElecDev P, const double* __restri
         }
         nA += __popcll(bA);
         nP += __popcll(bP);
+    };
+    if (P.cells.on) {
+        // the 27 link cells around site i: cell order fixed, ascending index within a cell
+        for_neighbour_cells(P.cells, i, [&](int c0, int c1) {
+            for (int base = c0; base < c1; base += 64) take(base + lane < c1 ? P.cells.idx[base + lane] : -1);
+        });
+    } else {
+        for (int base = 0; base < P.n; base += 64) take(base + lane < P.n ? base + lane : -1);
     }
     if (lane == 0) {
         cntA[i] = min(nA, P.capA);
@@ -2102,10 +2113,14 @@ inline size_t kfield_lds(const ElecDev& P)
 inline size_t kfinal_lds(const ElecDev& P) { return sizeof(double) * ((size_t)P.kmaxch * EL_KF + 2 * (size_t)P.rpc + 2); }
 }
 
-int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s)
+int el_launch_setup(ElecBuffers& E, const ElecDev& P, CellList& cells, hipStream_t s)
 {
     const int nwb = ceil_div(P.nw, 64);
     hipLaunchKernelGGL(k_el_sites, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.dg);
+    if (P.cells.on) {
+        const int rc = cells_build(cells, P.n, E.x, E.y, E.z, 1, s);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_el_nbr, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.nbrA, E.cntA, E.nbrP, E.cntP, E.err);
     return (int)hipGetLastError();
 }

@@ -242,6 +242,11 @@ int ensure_clusters(mbgpu_ctx* c, long nd, long nt)
 int build_lists(mbgpu_ctx* c)
 {
     MbBuffers& B = c->B;
+    {
+        // link cells of the oxygens, at least the 2B cut-off (the larger one) wide
+        std::string err;
+        if (cells_plan(B.cells, c->geom.c, MBGPU_R2F, B.nw, CELLS_MIN_WATERS, err)) return fail(-1, "%s", err.c_str());
+    }
     for (int attempt = 0; attempt < 3; ++attempt) {
         if (B.dcap == 0 && ensure_clusters(c, 1024, 1024)) return -1;
         int rc = mb_launch_lists(B, c->geom, c->stream);
@@ -471,6 +476,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     dfree(B.clus3); dfree(B.xv3); dfree(B.part3); dfree(B.e3); dfree(B.g3); dfree(B.s3);
     dfree(B.is_water); dfree(B.nbr2l); dfree(B.cnt2l); dfree(B.kmask); dfree(B.rowoff); dfree(B.blk); dfree(B.scal); dfree(B.atomic_energy); dfree(B.heat_w); dfree(B.fout);
     dfree(B.cs2); dfree(B.cs3);
+    cells_free(B.cells);
     dfree(B.xv2f); dfree(B.part2f); dfree(B.cs2f); dfree(B.xv3f); dfree(B.part3f); dfree(B.cs3f);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
     dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);

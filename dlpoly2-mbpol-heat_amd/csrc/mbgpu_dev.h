@@ -34,6 +34,45 @@ struct CellGeom {
     double r[3];        // 1/cell(1), 1/cell(5), 1/cell(9)
 };
 
+// Link cells of one set of particles (cells.hip): nc[a] cells per axis in fractional
+// coordinates s_a = rv[a] . r wrapped into [0, 1); key[i] the cell of particle i, idx the
+// particles by cell (ascending index within a cell), start[c] where cell c begins.
+// on = 0: no cells (a search scans every particle).
+struct CellView {
+    int on;
+    int nc[3];
+    double rv[3][3];
+    const int* key;
+    const int* idx;
+    const int* start;
+};
+
+// f(first, end) for the ranges of idx of the 27 cells around particle i's cell, in a fixed
+// order (needs nc >= 3 on every axis, so that the 27 cells are distinct)
+template <class F>
+__device__ __forceinline__ void for_neighbour_cells(const CellView& V, int i, F&& f)
+{
+    const int k = V.key[i];
+    const int c2 = k % V.nc[2], c1 = (k / V.nc[2]) % V.nc[1], c0 = k / (V.nc[1] * V.nc[2]);
+    for (int d0 = -1; d0 <= 1; ++d0) {
+        int a0 = c0 + d0;
+        a0 += a0 < 0 ? V.nc[0] : 0;
+        a0 -= a0 >= V.nc[0] ? V.nc[0] : 0;
+        for (int d1 = -1; d1 <= 1; ++d1) {
+            int a1 = c1 + d1;
+            a1 += a1 < 0 ? V.nc[1] : 0;
+            a1 -= a1 >= V.nc[1] ? V.nc[1] : 0;
+            for (int d2 = -1; d2 <= 1; ++d2) {
+                int a2 = c2 + d2;
+                a2 += a2 < 0 ? V.nc[2] : 0;
+                a2 -= a2 >= V.nc[2] ? V.nc[2] : 0;
+                const int cell = (a0 * V.nc[1] + a1) * V.nc[2] + a2;
+                f(V.start[cell], V.start[cell + 1]);
+            }
+        }
+    }
+}
+
 // Minimum image of one vector exactly as image_vectors (mbpol.f90:1305-1361):
 // d - L*nint(d*(1/L)), nint rounding half away from zero, no FMA contraction.
 __device__ __forceinline__ void image_vec(const CellGeom& g, double& dx, double& dy, double& dz)

@@ -51,9 +51,33 @@ const double* pip2b_stream_values();
 const double* pip3b_stream_values();
 
 // Device buffers of one MB-pol evaluation context.
+// ---- link cells (cells.hip) ----
+// particles from which a search takes the cells (below, scanning them all is faster: at
+// 2 048 waters the scans of the MB-pol O-O lists and of the dispersion rows beat the cell
+// build, the electrostatics rows gain a little)
+#define CELLS_MIN_SITES 8192       // electrostatics neighbour rows
+#define CELLS_MIN_ATOMS 16384      // dispersion rows
+#define CELLS_MIN_WATERS 8192      // MB-pol O-O lists
+#define CELLS_MAX_PER_AXIS 16
+#define CELLS_MAX_CELLS (CELLS_MAX_PER_AXIS * CELLS_MAX_PER_AXIS * CELLS_MAX_PER_AXIS)
+struct CellList {
+    CellView V{};                  // what the kernels see (V.on = 0: no cells)
+    int ncell = 0;
+    int *key = nullptr, *sidx = nullptr, *start = nullptr;   // cell per particle, particles by cell
+    int n_cap = 0, ncell_cap = 0;
+};
+// cells at least `cutoff` wide for n particles in the cell (rows: cell vectors); leaves
+// V.on = 0 when an axis holds fewer than three or n < min_particles; -1 with err
+int cells_plan(CellList& C, const double cell[9], double cutoff, int n, int min_particles, std::string& err);
+// the cells of the first n particles (after cells_plan), coordinates of particle i at
+// x[i * stride], y[i * stride], z[i * stride], on stream s; 0 or a HIP error
+int cells_build(CellList& C, int n, const double* x, const double* y, const double* z, int stride, hipStream_t s);
+void cells_free(CellList& C);
+
 struct MbBuffers {
     int nw = 0, natms = 0;
     int cap2 = 0, cap3 = 0;        // neighbour capacities per water (6.5 A and 4.5 A lists)
+    CellList cells;                // link cells of the oxygens (O-O lists)
     long dcap = 0, tcap = 0;       // cluster capacities
     // geometry
     double* mon = nullptr;         // [nw][9] imaged monomers
@@ -100,6 +124,7 @@ struct MbBuffers {
 struct ElecDev {
     int nw, n, nreal, nrow;        // waters, sites (4nw), real atoms (3nw), rows carrying dipoles
     CellGeom g;
+    CellView cells;                // link cells of the sites (cells.on = 0: full scans)
     double rcut, alpha, gamma, q_o;
     double polr[3], polr2[3];      // O, H, M
     double athole, athole12, athole13, ascc, ascd;
@@ -166,7 +191,7 @@ struct ElecSpme {
     const SpmeGeom* G;
     double kap;          // V / (8 pi): from the full-space mesh sums to the half-space k sums
 };
-int el_launch_setup(ElecBuffers& E, const ElecDev& P, hipStream_t s);
+int el_launch_setup(ElecBuffers& E, const ElecDev& P, CellList& cells, hipStream_t s);
 int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, const ElecSpme* sp, hipStream_t s);
 int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, const ElecSpme* sp, hipStream_t s);
 int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34, double gln34, int refresh,
@@ -185,6 +210,7 @@ struct ElecState {
     int sp_order = 0, sp_grid[3] = {0, 0, 0};
     SpmeBuffers* sp = nullptr;          // particle-mesh buffers and geometry (recip = 1)
     SpmeGeom* spg = nullptr;
+    CellList cells;                     // link cells of the sites (neighbour rows)
     bool ready = false;                 // the last evaluation completed
     int last_iters = 0;                 // solver iterations of the last evaluation
     int nw = 0;
@@ -209,6 +235,7 @@ void elec_release(ElecState& S);
 struct WaterDev {
     int nw, nr;                    // waters, real atoms (3nw)
     CellGeom g;
+    CellView cells;                // link cells of the real atoms (cells.on = 0: full scans)
     int bond_key[2];               // O-H, H-H bond keys (0 none, 1 harm, 2 mors, 7 expo)
     double bond_prm[2][3];
     int ps_monomer, dispersion;    // switches
@@ -234,6 +261,7 @@ struct WaterBuffers {
 struct WaterState {
     WaterBuffers B;
     WaterDev P{};
+    CellList cells;                // link cells of the real atoms (dispersion rows)
     bool ready = false;            // the last evaluation completed
     int nw = 0;
     double tab_key[14];            // rcut, rvdw, c6, a6, c8, a8 of the tables

@@ -200,12 +200,40 @@ __global__ void k_image_monomers(int nw, int natms, const int* __restrict__ list
     for (int k = 0; k < 9; ++k) mon[9 * w + k] = m[k];
 }
 
-// O-O neighbour lists.  One wavefront per water i scans all partners in chunks of 64; a
-// ballot + popcount compacts the hits in ascending partner order.
+// Ascending order of the first len (<= 4 * 64) entries of row (distinct values), in place,
+// by one wavefront: each lane's entries get their rank among all entries.  Every lane reads
+// the whole row before any lane writes it.
+__device__ void wave_sort_row(int* __restrict__ row, int len)
+{
+    const int lane = threadIdx.x & 63;
+    __threadfence_block();   // the row's entries were stored by other lanes of this wavefront
+    int v[4], r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = lane + 64 * u;
+        v[u] = e < len ? row[e] : 0;
+        r[u] = 0;
+    }
+    for (int k = 0; k < len; ++k) {
+        const int w = row[k];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] += w < v[u];
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (lane + 64 * u < len) row[r[u]] = v[u];
+}
+
+// O-O neighbour lists.  One wavefront per water i scans its candidate partners in chunks of
+// 64 -- every water, or, when the box holds three link cells per axis (cells.on), the
+// waters of the 27 cells around i (the link-cell Verlet build of parlink.f) -- and a
+// ballot + popcount compacts the hits; the rows are in ascending partner order (sorted after
+// a link-cell scan).
 //   list 2:  j > i with |img(O_i - O_j)|^2 <= r2f^2          (do_dimer predicate)
 //   list 2l: j < i with the same predicate (the partners of which i is the second water)
 //   list 3:  j != i with sqrt(|img(O_lo - O_hi)|^2) <= r3f    (do_trimer edge test)
-__global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restrict__ mon, CellGeom g,
+__global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restrict__ mon, CellGeom g, CellView cv,
                                                    int cap2, int* __restrict__ nbr2, int* __restrict__ cnt2,
                                                    int* __restrict__ nbr2l, int* __restrict__ cnt2l,
                                                    int cap3, int* __restrict__ nbr3, int* __restrict__ cnt3,
@@ -216,10 +244,9 @@ __global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restr
     if (i >= nw) return;
     int n2 = 0, n2l = 0, n3 = 0;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int base = 0; base < nw; base += 64) {
-        const int j = base + lane;
+    auto take = [&](int j) {
         bool in2 = false, in2l = false, in3 = false;
-        if (j < nw && j != i) {
+        if (j >= 0 && j != i) {
             const int lo = min(i, j), hi = max(i, j);
             double dx = mon[9 * lo] - mon[9 * hi];
             double dy = mon[9 * lo + 1] - mon[9 * hi + 1];
@@ -247,6 +274,16 @@ __global__ __launch_bounds__(64) void k_neighbours(int nw, const double* __restr
         n2 += __popcll(b2);
         n2l += __popcll(b2l);
         n3 += __popcll(b3);
+    };
+    if (cv.on) {
+        for_neighbour_cells(cv, i, [&](int c0, int c1) {
+            for (int base = c0; base < c1; base += 64) take(base + lane < c1 ? cv.idx[base + lane] : -1);
+        });
+        wave_sort_row(nbr2 + (long)i * cap2, min(n2, cap2));
+        wave_sort_row(nbr2l + (long)i * cap2, min(n2l, cap2));
+        wave_sort_row(nbr3 + (long)i * cap3, min(n3, cap3));
+    } else {
+        for (int base = 0; base < nw; base += 64) take(base + lane < nw ? base + lane : -1);
     }
     if (lane == 0) {
         cnt2[i] = n2;
@@ -937,8 +974,12 @@ int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s)
     hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(64), 0, s, B.counts, 4);
     hipLaunchKernelGGL(k_image_monomers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.natms, B.list, B.x, B.y,
                        B.z, g, B.mon, B.wbase, B.counts + 2);
-    hipLaunchKernelGGL(k_neighbours, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cap2, B.nbr2, B.cnt2, B.nbr2l,
-                       B.cnt2l, 64, B.nbr3, B.cnt3, B.counts + 2);
+    if (B.cells.V.on) {
+        const int rc = cells_build(B.cells, nw, B.mon, B.mon + 1, B.mon + 2, 9, s);   // the oxygens
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_neighbours, dim3(nw), dim3(64), 0, s, nw, B.mon, g, B.cells.V, B.cap2, B.nbr2, B.cnt2,
+                       B.nbr2l, B.cnt2l, B.cap3, B.nbr3, B.cnt3, B.counts + 2);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, B.cnt2, nw, B.off2, B.counts + 0);
     hipLaunchKernelGGL(k_fill_dimers, dim3((nw + 127) / 128), dim3(128), 0, s, nw, B.cap2, B.nbr2, B.cnt2, B.off2,
                        B.dimers, B.dcap);

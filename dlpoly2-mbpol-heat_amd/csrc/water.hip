@@ -315,14 +315,15 @@ __global__ __launch_bounds__(256) void k_wt_disp(WaterDev P, const double* __res
     double a[20];
     for (int q = 0; q < 20; ++q) a[q] = 0.0;   // f3, e, W9, st6, vir
     if (P.dispersion) {
-        for (int j = lane; j < P.nr; j += 64) {
+        auto pair = [&](int j) {
+#pragma clang fp contract(off)
             const int tp = ti + ((j % 3) != 0);
-            if (j / 3 == wi || !((P.tmask >> tp) & 1)) continue;
+            if (j / 3 == wi || !((P.tmask >> tp) & 1)) return;
             const double rx = xi - x[j], ry = yi - y[j], rz = zi - z[j];
             double dx = rx, dy = ry, dz = rz;
             image_vec(P.g, dx, dy, dz);
             const double rsq = norm2_nc(dx, dy, dz);
-            if (!(P.rc2 > rsq)) continue;
+            if (!(P.rc2 > rsq)) return;
             const double rrr = sqrt(rsq);
             const int l = (int)(rrr * P.rdr);
             const double ppp = rrr * P.rdr - (double)l;
@@ -347,6 +348,14 @@ __global__ __launch_bounds__(256) void k_wt_disp(WaterDev P, const double* __res
             a[17] += d[1] * fv[2];
             a[18] += d[2] * fv[2];
             a[19] -= gamma * rsq;
+        };
+        if (P.cells.on) {
+            // the 27 link cells around atom i (fixed cell order, ascending index within one)
+            for_neighbour_cells(P.cells, i, [&](int c0, int c1) {
+                for (int k = c0 + lane; k < c1; k += 64) pair(P.cells.idx[k]);
+            });
+        } else {
+            for (int j = lane; j < P.nr; j += 64) pair(j);
         }
         for (int q = 0; q < 20; ++q) a[q] = wsum(a[q]);
     }
@@ -471,6 +480,7 @@ void water_release(WaterState& S)
     WaterBuffers& B = S.B;
     dfree(B.mono); dfree(B.wsc); dfree(B.asc); dfree(B.fw); dfree(B.eout); dfree(B.wout);
     dfree(B.vvv); dfree(B.ggg); dfree(B.psc); dfree(B.pcm); dfree(B.pidx); dfree(B.pidxm); dfree(B.scal);
+    cells_free(S.cells);
     S.nw = 0;
     S.have_tab = false;
     S.ready = false;
@@ -575,6 +585,12 @@ int water_configure(WaterState& S, int nw, const CellGeom& g, const double* cell
         for (int t = 0; t < 3; ++t)
             if (p->c6[t] != 0.0 || p->c8[t] != 0.0) P.tmask |= 1 << t;
     }
+    // link cells of the real atoms for the dispersion rows (the test is rsq < rvdw^2)
+    P.cells.on = 0;
+    if (p->dispersion) {
+        if (cells_plan(S.cells, cell, p->rvdw, 3 * nw, CELLS_MIN_ATOMS, err)) return -1;
+        P.cells = S.cells.V;
+    }
     // long-range correction on the host
     S.out[6] = S.out[7] = 0.0;
     if (p->long_range) {
@@ -594,6 +610,7 @@ int water_evaluate(WaterState& S, const double* x, const double* y, const double
     S.ready = false;
     hipLaunchKernelGGL(k_wt_monomer, dim3((P.nw + 3) / 4), dim3(256), 0, s, P, x, y, z, B.psc, B.pcm, B.pidx,
                        B.pidxm, B.mono, B.wsc);
+    if (P.cells.on) WCHK((hipError_t)cells_build(S.cells, P.nr, x, y, z, 1, s));
     hipLaunchKernelGGL(k_wt_disp, dim3((P.nr + 3) / 4), dim3(256), 0, s, P, x, y, z, B.vvv, B.ggg, B.mono, fx, fy,
                        fz, B.fw, B.eout, B.wout, B.asc);
     hipLaunchKernelGGL(k_wt_reduce, dim3(1), dim3(1024), 0, s, P.nw, P.nr, B.wsc, B.asc, B.scal);

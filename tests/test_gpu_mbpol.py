@@ -9,8 +9,10 @@ Bars (FP64 path):
   * leaves: within 1e-15 * S of the reference's golden vectors, S = sum |a_b monomial|
     (the cancellation scale of the polynomial; see test_oracle_leaves.py).
 At 2048 waters (beyond the oracle's few-second range) the checks are size-independent:
-list sanity, zero net force, energy decomposition sums, virial = -trace(stress),
-determinism.
+the complete dimer and trimer sets against the predicates evaluated for all pairs (the GPU
+scanning every water and, in a second run, searching six link cells per axis), zero net
+force, energy decomposition sums,
+virial = -trace(stress), determinism.
 """
 import numpy as np
 import pytest
@@ -189,8 +191,11 @@ def test_bad_device_list_is_an_error(pkg, box64):
     assert e1 == e0 and v1 == v0 and np.array_equal(s1, s0)
 
 
-def test_2048_size_independent_properties(pkg, waterbox):
+@pytest.mark.parametrize("cells", [False, True])
+def test_2048_size_independent_properties(pkg, waterbox, cells, monkeypatch):
     box = waterbox.box_2048()
+    if cells:
+        monkeypatch.setenv("MBGPU_CELLS_MIN", "1024")   # the O-O lists through link cells
     ctx, g = gpu_forces(pkg, box)
     d, t = g["dimers"], g["trimers"]
     assert len(d) > 30000 and len(t) > 100000
@@ -203,12 +208,28 @@ def test_2048_size_independent_properties(pkg, waterbox):
     assert abs(g["atomic_energy"].sum() - g["eng"]) <= 1e-10 * abs(g["eng"])
     s = g["stress"]
     assert abs(s[0] + s[4] + s[8] + g["vir"]) <= 1e-10 * abs(g["vir"])
-    # spot-check a sample of dimers and trimers against the oracle's predicates
+    # the complete sets against the predicates of mbpol.f90:496-503 (dimers) and 843-869
+    # (trimers: at least two O-O edges <= r3f), evaluated for all pairs in the same FP64
+    # order (lower index minus higher, d - L nint(d / L), (x^2 + y^2) + z^2)
     O = box.xyz[:box.nreal:3]
-    rng = np.random.default_rng(1)
-    for i, j in d[rng.choice(len(d), 200, replace=False)]:
-        x = O[i] - O[j]
-        x -= box.L * np.round(x / box.L)
-        assert (x ** 2).sum() <= 42.25
+    dd = O[:, None, :] - O[None, :, :]
+    dd = dd - box.L * np.round(dd * (1.0 / box.L))
+    r2 = (dd[..., 0] * dd[..., 0] + dd[..., 1] * dd[..., 1]) + dd[..., 2] * dd[..., 2]
+    iu, ju = np.triu_indices(box.nw, 1)
+    r2u = r2[iu, ju]
+    pair = ~(r2u > 6.5 * 6.5)
+    assert set(map(tuple, d.tolist())) == set(zip(iu[pair].tolist(), ju[pair].tolist()))
+    short = ~(np.sqrt(r2u) > 4.5)
+    nb = [[] for _ in range(box.nw)]
+    for a_, b_ in zip(iu[short].tolist(), ju[short].tolist()):
+        nb[a_].append(b_)
+        nb[b_].append(a_)
+    tri = set()
+    for c_ in range(box.nw):
+        ns = sorted(nb[c_])
+        for u in range(len(ns)):
+            for v in range(u + 1, len(ns)):
+                tri.add(tuple(sorted((c_, ns[u], ns[v]))))
+    assert set(map(tuple, t.tolist())) == tri
     _, g2 = gpu_forces(pkg, box, ctx)
     assert np.array_equal(g["f"], g2["f"])
