@@ -199,7 +199,7 @@ def serve_leg(pkg, args):
                mix=args.serve_mix, cluster_sizes=sizes,
                what="requests drawn from 32 PIMD beads x 256 H2O (L=19.72 A, N(0,0.05 A) displacements)"
                     + (" and, half of them, isolated clusters of n in [3,1024] waters, n log-uniform (random "
-                       "sequential addition at 0.997 g/cm^3, O-O >= 2.6 A, in a periodic box of side 2R+12 A)"
+                       "sequential addition at 0.997 g/cm^3, O-O >= 2.6 A; isolated: imcon 0, direct sums, no images)"
                        if clusters else "")
                     + ", Poisson arrivals, continuous batching over the workers' slots; latency = arrival to "
                       "forces on the host")

@@ -174,7 +174,8 @@ int build_table(ElecState& S, double alpha, double rcut, std::string& err)
     if (mxegrd < 1000) mxegrd = 1000;
     const double drewd = rcut / (double)(mxegrd - 4);
     std::vector<double> t((size_t)4 * (mxegrd + 1), 0.0);
-    for (int i = 1; i <= mxegrd; ++i) {
+    for (int i = 1; i <= mxegrd && alpha >= 1.0e-6; ++i) {
+        // (no Ewald split, alpha = 0: the -erf kernels vanish, as ewald3p.f sets them)
         const double rrr = (double)i * drewd;
         const double rsqi = 1.0 / rrr / rrr;
         const double a = alpha * rrr;
@@ -376,6 +377,30 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
 
 }  // namespace
 
+namespace {
+
+// An isolated cluster (imcon 0): no reciprocal space.  The buffers the k-sum set-up would
+// size are sized for the site count alone: the field partials (one chunk of 6 components)
+// and the block partials of the final passes.
+int alloc_direct(ElecState& S, std::string& err)
+{
+    ElecBuffers& E = S.E;
+    dfree(E.Fpart);
+    ECHK(dalloc(&E.Fpart, (size_t)6 * E.n));
+    const int nsb = (E.n + 255) / 256, nkb = (E.n + 3) / 4, nwb = (S.nw + 63) / 64;
+    dfree(E.blk);
+    E.nblk = nsb + 8 * nkb + 2 * nsb + 8 * E.n + 24 * nwb + 2 * nsb + 64;
+    ECHK(dalloc(&E.blk, (size_t)E.nblk));
+    E.K = E.nrun = 0;
+    E.kchunks = E.fchunks = 1;
+    E.rpc = E.kmaxch = E.frpc = E.fkmax = 0;
+    S.k_alpha = -1.0;   // a later periodic call builds its k list
+    S.P.rvolm = 0.0;
+    return 0;
+}
+
+}  // namespace
+
 void elec_release(ElecState& S)
 {
     free_buffers(S.E);
@@ -400,7 +425,18 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
         err = "null parameters";
         return -1;
     }
-    if (!(p->rcut > 0.0) || !(p->alpha > 1e-6) || p->kmax[0] < 1 || p->kmax[1] < 1 || p->kmax[2] < 1) {
+    if (g.imcon == 0) {
+        // an isolated cluster: direct sums over all site pairs within rcut, no Ewald split
+        if (!(p->rcut > 0.0) || p->alpha != 0.0) {
+            err = "electrostatics parameters: an isolated cluster (imcon 0) needs rcut > 0 and alpha = 0 "
+                  "(direct sums; rcut at least the cluster's extent)";
+            return -1;
+        }
+        if (S.recip) {
+            err = "electrostatics parameters: the particle mesh needs a periodic cell (imcon 1-3)";
+            return -1;
+        }
+    } else if (!(p->rcut > 0.0) || !(p->alpha > 1e-6) || p->kmax[0] < 1 || p->kmax[1] < 1 || p->kmax[2] < 1) {
         err = "electrostatics parameters: need rcut > 0, alpha > 1e-6 and kmax >= 1";
         return -1;
     }
@@ -441,11 +477,16 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
     P.compat_qd = p->compat_qd_column;
     if (S.tab_alpha != p->alpha || S.tab_rcut != p->rcut)
         if (build_table(S, p->alpha, p->rcut, err)) return -1;
-    bool same_k = S.k_alpha == p->alpha && S.E.K > 0;
-    for (int i = 0; i < 9; ++i) same_k = same_k && S.k_cell[i] == cell[i];
-    for (int i = 0; i < 3; ++i) same_k = same_k && S.k_kmax[i] == p->kmax[i];
-    if (!same_k && build_klist(S, cell, p->kmax, p->alpha, err)) return -1;
-    if (cells_plan(S.cells, cell, p->rcut, 4 * nw, CELLS_MIN_SITES, err)) return -1;
+    if (g.imcon == 0) {
+        if (alloc_direct(S, err)) return -1;
+    } else {
+        bool same_k = S.k_alpha == p->alpha && S.E.K > 0;
+        for (int i = 0; i < 9; ++i) same_k = same_k && S.k_cell[i] == cell[i];
+        for (int i = 0; i < 3; ++i) same_k = same_k && S.k_kmax[i] == p->kmax[i];
+        if (!same_k && build_klist(S, cell, p->kmax, p->alpha, err)) return -1;
+    }
+    S.cells.V.on = 0;   // (an isolated cluster has no cells to wrap)
+    if (g.imcon != 0 && cells_plan(S.cells, cell, p->rcut, 4 * nw, CELLS_MIN_SITES, err)) return -1;
     S.P.cells = S.cells.V;
     if (S.recip) {
         // particle mesh: eleven grids (the final pass's), splines of every site
@@ -489,8 +530,11 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     ElecBuffers& E = S.E;
     ElecDev& P = S.P;
     S.ready = false;
+    // the reciprocal space: the k-sum (sp null), the particle mesh, or none (an isolated
+    // cluster: sp with no mesh buffers)
     ElecSpme spx{S.sp, S.spg, S.spg ? S.spg->vol / (8.0 * PI_D) : 0.0};
-    const ElecSpme* sp = S.recip ? &spx : nullptr;
+    ElecSpme none{nullptr, nullptr, 0.0};
+    const ElecSpme* sp = P.g.imcon == 0 ? &none : S.recip ? &spx : nullptr;
     // The neighbour rows' capacity check is read back with the solver's first convergence
     // check instead of after the row build (one host round trip less): the rows are built
     // with clamped counts, so an overflowed attempt runs in bounds and is simply repeated
@@ -581,7 +625,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     }
     ECHK(hipMemcpyAsync(S.scal, E.scal, sizeof(double) * 41, hipMemcpyDeviceToHost, s));
     int spflag[2] = {0, 0};
-    if (sp) ECHK(hipMemcpyAsync(spflag, S.sp->flag, sizeof spflag, hipMemcpyDeviceToHost, s));
+    if (sp && sp->B) ECHK(hipMemcpyAsync(spflag, S.sp->flag, sizeof spflag, hipMemcpyDeviceToHost, s));
     ECHK(hipStreamSynchronize(s));
     if (spflag[0] == 1) {
         err = "particle mesh: a site position is not finite or too far from the cell to index the grid";

@@ -178,6 +178,10 @@ __device__ __forceinline__ void table_interp(const double* __restrict__ t, doubl
 // B functions computed directly (excluded pairs, rfield.f:365-374, ewald3p.f:100-110)
 __device__ __forceinline__ void bfun_direct(double alpha, double rr, double rsqi, double* bbb)
 {
+    if (alpha < 1.0e-6) {   // ewald3p.f: "if ewald parameter is too small then turn it off"
+        bbb[0] = bbb[1] = bbb[2] = bbb[3] = 0.0;
+        return;
+    }
     const double a = alpha * rr;
     bbb[0] = (erfcc_nr(a) - 1.0) / rr;
     const double exp2a = exp(-a * a);
@@ -2155,6 +2159,15 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, co
     hipLaunchKernelGGL(k_el_prep, dim3(P.n), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tab, E.nbrA, E.cntA, E.nbrP,
                        E.cntP, E.tens, E.efk);
     hipLaunchKernelGGL(k_el_intra_prep, dim3(nwb), dim3(64), 0, s, P, E.x, E.y, E.z, E.chge, E.tin, E.efk);
+    if (sp && !sp->B) {
+        // no reciprocal space (an isolated cluster): zero fields, in the layout of the k-sum
+        // field partials (one chunk), so that the reductions form efdd (the self term is 0
+        // without an Ewald split) and efdc
+        el_launch_zero(E.Fpart, 6L * P.n, s);
+        hipLaunchKernelGGL(k_el_kfield_reduce<true>, dim3(ceil_div(6L * P.n, 64)), dim3(256), 0, s, one_chunk(P),
+                           E.Fpart, E.dip, E.efdd, E.efdc, E.state);
+        return (int)hipGetLastError();
+    }
     if (sp) {
         // particle mesh: splines of every site, then the fields of the start dipoles (grid 0)
         // and of the charges (grid 1) at every site, in the layout of the k-sum field partials
@@ -2189,7 +2202,9 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, co
 
 int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, const ElecSpme* sp, hipStream_t s)
 {
-    if (!first && sp) {
+    if (!first && sp && !sp->B) {
+        // no reciprocal space: efdd keeps the preparation's zeros
+    } else if (!first && sp) {
         el_spme_dipole_field(E, *sp, P.nrow, E.dip, E.state, s);
         hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, one_chunk(P),
                            E.Fpart, E.dip, E.efdd, E.efdc, E.state);
@@ -2243,7 +2258,8 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, const ElecS
     hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
     if (sp) {
         // particle mesh: the reciprocal field of p, one chunk; no structure factors to carry
-        el_spme_dipole_field(E, *sp, P.nrow, p, E.state, s);
+        // (no reciprocal space: the preparation's zero partials stand)
+        if (sp->B) el_spme_dipole_field(E, *sp, P.nrow, p, E.state, s);
         const ElecDev Q = one_chunk(P);
         hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, Q, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
         hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, Q, it, nbq, blkq, E.cg_p, E.cg_q, E.cg_minv, E.dip,
@@ -2300,7 +2316,12 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     double* b_p3 = b_p2 + 8L * P.n;              // [nwb][8]
     double* b_qd = b_p3 + 8L * nwb;              // [nwb][16]
     hipLaunchKernelGGL(k_el_polar, dim3(nsb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.ep, b_pol);
-    if (sp) {
+    if (sp && !sp->B) {
+        // no reciprocal space: zero site sums and per-block partials for the k-sum's final
+        // kernel (forces, ESP, per-atom energy and stress of the reciprocal part all 0)
+        el_launch_zero(E.ksum, 15L * P.n, s);
+        el_launch_zero(b_ks, 8L * nkb, s);
+    } else if (sp) {
         // particle mesh: grids 0..4 hold q + mu, q, and mu_x, mu_y, mu_z as charges; the
         // stress kernels of grid 0 go to grids 5..10; every per-site quantity of the k-sum
         // site pass then comes from one gather (k_sp_elec_final), in the k-sum pass's units

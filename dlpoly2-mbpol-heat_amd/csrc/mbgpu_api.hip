@@ -83,8 +83,15 @@ void dfree(T*& p)
 
 bool make_geom(int imcon, const double* cell, CellGeom& g)
 {
-    if (imcon < 1 || imcon > 3) return false;
+    if (imcon < 0 || imcon > 3) return false;
     g.imcon = imcon;
+    if (imcon == 0) {
+        // an isolated cluster: no images; the cell (if any) is kept but not used
+        for (int i = 0; i < 9; ++i) g.c[i] = cell ? cell[i] : 0.0;
+        for (int i = 0; i < 3; ++i) g.r[i] = 0.0;
+        for (int i = 0; i < 9; ++i) g.rc[i] = 0.0;
+        return true;
+    }
     for (int i = 0; i < 9; ++i) g.c[i] = cell[i];
     g.r[0] = 1.0 / cell[0];
     g.r[1] = 1.0 / cell[4];
@@ -245,7 +252,9 @@ int build_lists(mbgpu_ctx* c)
     {
         // link cells of the oxygens, at least the 2B cut-off (the larger one) wide
         std::string err;
-        if (cells_plan(B.cells, c->geom.c, MBGPU_R2F, B.nw, CELLS_MIN_WATERS, err)) return fail(-1, "%s", err.c_str());
+        B.cells.V.on = 0;
+        if (c->geom.imcon != 0 && cells_plan(B.cells, c->geom.c, MBGPU_R2F, B.nw, CELLS_MIN_WATERS, err))
+            return fail(-1, "%s", err.c_str());
     }
     for (int attempt = 0; attempt < 3; ++attempt) {
         if (B.dcap == 0 && ensure_clusters(c, 1024, 1024)) return -1;
@@ -1163,6 +1172,7 @@ int mbgpu_time_elec(mbgpu_ctx* c, int reps, double* ms_sfac, double* flop_sfac)
     if (!c || !c->el.ready) return fail(-1, "mbgpu_time_elec: run mbgpu_polar_electrostatics first");
     if (reps < 1) return fail(-1, "reps must be positive");
     ElecState& S = c->el;
+    if (S.P.K < 1 || S.recip) return fail(-1, "mbgpu_time_elec: the last electrostatics call had no k-sum");
     int rc = el_time_sfac(S.E, S.P, reps, c->ev0, c->ev1, c->el_stream);
     if (rc) return fail(rc, "timing run failed");
     float ms = 0.f;

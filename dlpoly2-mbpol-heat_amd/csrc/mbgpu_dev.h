@@ -28,7 +28,8 @@ __host__ __device__ inline int water_base(const int* list, int w, int natms)
 }
 
 struct CellGeom {
-    int imcon;          // 1 cubic, 2 orthorhombic, 3 general (mbpol.f90:1319-1357)
+    int imcon;          // 0 none (an isolated cluster), 1 cubic, 2 orthorhombic, 3 general
+                        // (mbpol.f90:1319-1357)
     double c[9];        // cell vectors, Fortran cell(1..9)
     double rc[9];       // inverse (imcon 3)
     double r[3];        // 1/cell(1), 1/cell(5), 1/cell(9)
@@ -78,6 +79,7 @@ __device__ __forceinline__ void for_neighbour_cells(const CellView& V, int i, F&
 __device__ __forceinline__ void image_vec(const CellGeom& g, double& dx, double& dy, double& dz)
 {
 #pragma clang fp contract(off)
+    if (g.imcon == 0) return;   // an isolated cluster: no images
     if (g.imcon == 1) {
         dx = dx - g.c[0] * round(dx * g.r[0]);
         dy = dy - g.c[0] * round(dy * g.r[0]);

@@ -588,12 +588,13 @@ int water_configure(WaterState& S, int nw, const CellGeom& g, const double* cell
     // link cells of the real atoms for the dispersion rows (the test is rsq < rvdw^2)
     P.cells.on = 0;
     if (p->dispersion) {
-        if (cells_plan(S.cells, cell, p->rvdw, 3 * nw, CELLS_MIN_ATOMS, err)) return -1;
+        S.cells.V.on = 0;
+        if (g.imcon != 0 && cells_plan(S.cells, cell, p->rvdw, 3 * nw, CELLS_MIN_ATOMS, err)) return -1;
         P.cells = S.cells.V;
     }
     // long-range correction on the host
     S.out[6] = S.out[7] = 0.0;
-    if (p->long_range) {
+    if (p->long_range && g.imcon != 0) {   // lrcorrect.f:114: none without a periodic cell
         const double* a = cell;
         const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[3] * (a[1] * a[8] - a[2] * a[7]) +
                            a[6] * (a[1] * a[5] - a[2] * a[4]);

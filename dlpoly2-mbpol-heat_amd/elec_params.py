@@ -51,11 +51,7 @@ def for_box(L):
 
 
 def for_cluster(L):
-    """Ewald parameters for an isolated cluster in a cubic box of side L (waterbox.make_cluster):
-    rcut = min(9, L/2 - 0.05) A, alpha = 3.2 / rcut (erfc(3.2) = 6e-6 at the cutoff) and
-    kmax = ceil(1.3 alpha L) (exp(-(pi kmax / (alpha L))^2) < 1e-7), at most 25 per axis."""
-    import math
-    rcut = min(9.0, 0.5 * L - 0.05)
-    alpha = 3.2 / rcut
-    k = min(25, int(math.ceil(1.3 * alpha * L)))
-    return ElecParams(rcut=rcut, alpha=alpha, kmax=(k, k, k))
+    """Parameters for an isolated cluster (imcon 0) inside a cube of side L
+    (waterbox.make_cluster): direct sums, no Ewald split (alpha = 0), every site pair
+    interacting (rcut = L bounds the cluster's extent); kmax is not used."""
+    return ElecParams(rcut=float(L), alpha=0.0, kmax=(1, 1, 1))

@@ -7,7 +7,8 @@ context with its own HIP stream) of the request's size, so up to `workers` evalu
 on the GPU at once and a request that arrives while others run starts as soon as a worker
 is free: admission is per request, not per batch.
 
-Slots are pooled by size.  A worker reuses an idle slot of the request's size, creates one
+Slots are pooled by size and kind (periodic box or isolated cluster).  A worker reuses an
+idle slot of the request's size and kind, creates one
 when the pool has room under its memory budget, and otherwise evicts the least recently
 used idle slot of another size; when the busy slots alone fill the budget it waits for one
 to be released.  Every kernel sums in a fixed order and a context's results do not depend
@@ -30,10 +31,16 @@ from .engine import Slot
 MB = 1 << 20
 
 
-def slot_bytes(nw):
-    """Device memory of a slot of nw waters (estimate from the buffer sizes: cluster partials
-    dominate, about 0.55 MB per water at liquid density, plus fixed tables)."""
-    return int(0.6 * MB * nw + 24 * MB)
+def slot_bytes(nw, cluster=False):
+    """Device memory of a slot of nw waters (estimate from the buffer sizes: MB-pol cluster
+    partials dominate in a periodic box, about 0.55 MB per water at liquid density, plus
+    fixed tables).  An isolated cluster (imcon 0) takes every site pair into its
+    electrostatics rows: (4 nw)^2 dipole tensors and partner indices, 56 bytes a pair."""
+    return int(0.6 * MB * nw + 24 * MB + (56 * (4 * nw) ** 2 if cluster else 0))
+
+
+def _pool_key(box):
+    return (box.nw, box.imcon == 0)
 
 
 class Request:
@@ -87,15 +94,17 @@ class Server:
 
     # ---------------------------------------------------------------- slot pool ----
     def _acquire(self, nw, box):
-        """An idle slot for nw waters, or a new one within the budget (called with the lock)."""
+        """An idle slot for a configuration like box (nw waters, periodic or a cluster), or a
+        new one within the budget (called with the lock)."""
+        key = _pool_key(box)
         for sid, s in self._idle.items():
-            if s.box.nw == nw:
+            if _pool_key(s.box) == key:
                 del self._idle[sid]
                 return s, False
-        need = slot_bytes(nw)
+        need = slot_bytes(nw, box.imcon == 0)
         while self._used_bytes + need > self.budget and self._idle:
             _, victim = self._idle.popitem(last=False)    # least recently used
-            self._used_bytes -= slot_bytes(victim.box.nw)
+            self._used_bytes -= slot_bytes(victim.box.nw, victim.box.imcon == 0)
             self.slots_evicted += 1
             del victim
         if self._used_bytes + need > self.budget:
@@ -117,8 +126,9 @@ class Server:
                 if not self._queue:
                     return
                 req = self._queue.popleft()
-                if slot_bytes(req.box.nw) > self.budget:
-                    req.error = MemoryError(f"a slot of {req.box.nw} waters needs {slot_bytes(req.box.nw)} bytes, "
+                need = slot_bytes(req.box.nw, req.box.imcon == 0)
+                if need > self.budget:
+                    req.error = MemoryError(f"a slot of {req.box.nw} waters needs {need} bytes, "
                                             f"more than the server's budget of {self.budget}")
                     req.t_done = time.perf_counter()
                     req.done.set()
@@ -144,7 +154,7 @@ class Server:
                 req.error = e
                 if create and slot is None:
                     with self._cv:
-                        self._used_bytes -= slot_bytes(req.box.nw)
+                        self._used_bytes -= slot_bytes(req.box.nw, req.box.imcon == 0)
             req.t_done = time.perf_counter()
             if slot is not None:
                 self._release(slot)

@@ -67,6 +67,6 @@ def for_box(L):
 
 
 def for_cluster(L):
-    """Dispersion cutoff for an isolated cluster in a cubic box of side L: min(9, L/2 - 0.05)."""
-    r = min(9.0, 0.5 * L - 0.05)
-    return WaterParams(rvdw=r, rcut=r)
+    """Cutoffs for an isolated cluster (imcon 0): the model's 9 A dispersion cutoff (no
+    images to keep clear of); L is not used."""
+    return WaterParams(rvdw=9.0, rcut=9.0)

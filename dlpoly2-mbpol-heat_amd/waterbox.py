@@ -143,9 +143,9 @@ def make_cluster(n, seed, margin=12.0, dmin=2.6, temperature=300.0):
     """An isolated cluster of n waters built by random sequential addition: oxygens placed
     uniformly in a sphere holding n molecules at 0.997 g/cm^3, rejected below an O-O distance
     of dmin, each molecule a perturbed monomer with a random orientation as in
-    make_waterbox.  The cluster sits in a cubic periodic box of side 2R + margin (the
-    library's images need a periodic cell; with the margin no image is within the 2B/3B
-    cutoffs of another, and the Ewald sums of the box stand in for the open boundary)."""
+    make_waterbox.  The cluster is isolated (imcon 0: no images, direct sums); it is
+    centred in a cube of side L = 2R + margin, which bounds every site pair's distance and
+    is kept as the box's cell for bookkeeping only."""
     rng = np.random.Generator(np.random.PCG64(seed))
     R = (3.0 * n / (4.0 * np.pi * RHO_WATER)) ** (1.0 / 3.0)
     L = 2.0 * R + margin
@@ -184,7 +184,9 @@ def make_cluster(n, seed, margin=12.0, dmin=2.6, temperature=300.0):
     vel[:3 * n] = v
     mass = np.zeros(4 * n)
     mass[:3 * n] = mass_real
-    return WaterBox(n, L, xyz, vel, mass, np.arange(1, 3 * n + 1, dtype=np.int32))
+    box = WaterBox(n, L, xyz, vel, mass, np.arange(1, 3 * n + 1, dtype=np.int32))
+    box.imcon = 0
+    return box
 
 
 def displaced(box, sigma, seed):

@@ -70,8 +70,9 @@ const char* mbgpu_last_error(void);
  *   x/y/zzz: natms positions (Angstrom);
  *   f{x,y,z}: += MB-pol forces (internal units);
  *   engsrp, virsrp: += 2B+3B energy and virial; stress[9]: += stress.
- * imcon 1, 2, 3 as image_vectors (mbpol.f90:1319-1357); other values return an error
- * (the reference stops in fixme).  The per-atom energies (E/6 per dimer atom, E/9 per
+ * imcon 1, 2, 3 as image_vectors (mbpol.f90:1319-1357); imcon 0 is an isolated cluster
+ * (no images; the cell is not used; the reference stops in fixme there, mbpol.f90:1357,
+ * so these semantics are this library's); other values return an error.  The per-atom energies (E/6 per dimer atom, E/9 per
  * trimer atom) and the per-atom heat-flux tensors stay on the device for
  * mbgpu_heat_flux and can be read with mbgpu_get_decomposition. */
 int mbgpu_mbpol_forces(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm2, const int* listttm2,
@@ -179,7 +180,12 @@ typedef struct {
  *   niter: receives the number of solver iterations.
  * Per-atom energies, reciprocal-space stresses and force-matrix contractions stay on the
  * device and enter mbgpu_heat_flux automatically.  Errors: no convergence within
- * max_iter (the reference stops, forces.f:351-359), unsupported imcon, capacity. */
+ * max_iter (the reference stops, forces.f:351-359), unsupported imcon, capacity.
+ * imcon 0 (an isolated cluster; the reference rejects it, forces.f:271-273): no images and
+ * no Ewald split -- alpha must be 0, so that the real-space kernels are the bare ones (the
+ * -erf parts vanish, as ewald3p.f sets them below alpha 1e-6), there is no reciprocal sum
+ * and no self term; every site pair within rcut interacts, so rcut should cover the
+ * cluster; kmax is not used. */
 int mbgpu_polar_electrostatics(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_elec_params* p,
                                double* xxx, double* yyy, double* zzz, double* dipx, double* dipy, double* dipz,
                                double* chge, double* fxx, double* fyy, double* fzz, double* engcpe,
@@ -346,7 +352,8 @@ typedef struct {
  *             as dlpoly_cmd.f:2572-2574 does).
  * Per-atom energies (bond halves, pterm/3, dispersion halves; the long-range correction
  * has no per-atom share, its per-atom update is commented out in lrcorrect.f:226-232) and the
- * force-matrix contractions stay on the device and enter mbgpu_heat_flux automatically. */
+ * force-matrix contractions stay on the device and enter mbgpu_heat_flux automatically.
+ * imcon 0 (an isolated cluster): no images and no long-range correction (lrcorrect.f:114). */
 int mbgpu_water_terms(mbgpu_ctx* ctx, int imcon, const double cell[9], const mbgpu_water_params* p,
                       const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
                       double* fzz, double eng[4], double vir[4], double stress[9]);

@@ -48,6 +48,11 @@
  * values promoted to double. */
 double ora_erfcc(double x)
 {
+#ifdef ORA_EXACT_ERFC
+    /* the second build of this oracle (liboracle_exact.so): the C library's erfc in place of
+     * the reference's approximation, to separate what that approximation does from the rest */
+    return erfc(x);
+#endif
     const double z = fabs(x);
     const double t = 1.0 / (1.0 + 0.5 * z);
     double r = t * exp(-z * z - (double)1.26551223f +
@@ -145,6 +150,10 @@ void ora_erfc_table(double alpha, double rcut, double *t /* (mxegrd+1)*4 */)
         const double rsqi = 1.0 / rrr / rrr;
         const double a = alpha * rrr;
         double *e = t + 4 * i;
+        if (alpha < 1.0e-6) { /* no Ewald split (a cluster): the -erf kernels vanish, as ewald3p.f sets them */
+            e[0] = e[1] = e[2] = e[3] = 0.0;
+            continue;
+        }
         e[0] = (ora_erfcc(a) - 1.0) / rrr;
         const double exp2a = exp(-a * a);
         for (int n = 1; n <= 3; ++n) {
@@ -170,6 +179,10 @@ static void table_interp(const double *t, double drewd, double rr, double *bbb)
 /* direct B functions of excluded pairs (rfield.f:365-374, ewald3p.f:100-110) */
 static void bfun_direct(double alpha, double rr, double rsqi, double *bbb)
 {
+    if (alpha < 1.0e-6) { /* ewald3p.f: "if ewald parameter is too small then turn it off" */
+        bbb[0] = bbb[1] = bbb[2] = bbb[3] = 0.0;
+        return;
+    }
     const double a = alpha * rr;
     bbb[0] = (ora_erfcc(a) - 1.0) / rr;
     const double exp2a = exp(-a * a);
@@ -1067,7 +1080,7 @@ static void qdforce(ctx_t *C)
  * A->vir_qd and A->stress_qd_atoms set.  Positions as given (the M sites are not rebuilt). */
 int ora_qdforce(ora_elec_args *A)
 {
-    if (A->imcon < 1 || A->imcon > 3) return -1;
+    if (A->imcon < 0 || A->imcon > 3) return -1;
     if (!g_dms_loaded) return -3;
     ctx_t C;
     memset(&C, 0, sizeof C);
@@ -1095,9 +1108,23 @@ int ora_qdforce(ora_elec_args *A)
 
 /* ------------------------------------------------------------------- driver ---- */
 
+static int elec_forces_run(ora_elec_args *A);
+
 int ora_elec_forces(ora_elec_args *A)
 {
-    if (A->imcon < 1 || A->imcon > 3) return -1;
+    if (A->imcon < 0 || A->imcon > 3) return -1;
+    /* imcon 0, an isolated cluster: direct sums over all pairs within rcut, no Ewald split
+     * (alpha must be 0) and no reciprocal space */
+    if (A->imcon == 0 && A->alpha != 0.0) return -1;
+    const int kmax_in[3] = {A->kmax[0], A->kmax[1], A->kmax[2]};
+    if (A->imcon == 0) A->kmax[0] = A->kmax[1] = A->kmax[2] = 0;   /* restored on return */
+    const int rc_all = elec_forces_run(A);
+    for (int q = 0; q < 3; ++q) A->kmax[q] = kmax_in[q];
+    return rc_all;
+}
+
+static int elec_forces_run(ora_elec_args *A)
+{
     if (!g_dms_loaded) return -3;
     ctx_t C;
     memset(&C, 0, sizeof C);
@@ -1189,7 +1216,13 @@ int ora_elec_forces(ora_elec_args *A)
     C.enc = malloc(sizeof(double) * n * (km3 + 1));
     C.ens = malloc(sizeof(double) * n * (km3 + 1));
     C.fk = calloc(3 * n, sizeof(double));
-    recip_setup(&C);
+    if (A->imcon != 0) {
+        recip_setup(&C);
+    } else {
+        C.volm = 0.0;
+        C.rvolm = 0.0;
+        C.ralph = 0.0;
+    }
     A->volm = C.volm;
 
     C.efk = calloc(3 * n, sizeof(double));

@@ -71,6 +71,7 @@ static void invert3(const double *a, double *b)
 
 int ora_image_vectors(int imcon, const double *cell, int n, double *dx)
 {
+    if (imcon == 0) return 0; /* an isolated cluster: no images */
     if (imcon == 1) {
         const double a1 = 1.0 / cell[0];
         for (int i = 0; i < n; ++i)
@@ -291,7 +292,7 @@ static void do_trimer(ora_mbpol_args *A, double *w1, double *w2, double *w3,
 
 int ora_mbpol_forces(ora_mbpol_args *A)
 {
-    if (A->imcon < 1 || A->imcon > 3) return -1;
+    if (A->imcon < 0 || A->imcon > 3) return -1;
     const int nw = A->nw;
     double *mon = malloc(sizeof(double) * 9 * (nw > 0 ? nw : 1));
     for (int t = 0; t < 3 * nw; ++t) {

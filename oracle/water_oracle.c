@@ -271,7 +271,7 @@ static void fm_add(ora_water_args *A, const double *x, const double *y, const do
 
 int ora_water_terms(ora_water_args *A)
 {
-    if (A->imcon < 1 || A->imcon > 3) return -1;
+    if (A->imcon < 0 || A->imcon > 3) return -1;
     if (!g_ps_loaded) return -3;
     const int nw = A->nw, nreal = 3 * nw, n = 4 * nw;
     const double *x = A->x, *y = A->y, *z = A->z;
@@ -430,7 +430,8 @@ int ora_water_terms(ora_water_args *A)
     const double num[2] = {(double)nw, 2.0 * nw};
     const int tix[2][2] = {{0, 1}, {1, 2}};
     double elrc = 0.0, plrc = 0.0;
-    for (int ti = 0; ti < 2 && A->long_range; ++ti)
+    /* lrcorrect.f:114: no correction without a periodic cell (imcon 0) */
+    for (int ti = 0; ti < 2 && A->long_range && A->imcon != 0; ++ti)
         for (int tj = 0; tj <= ti; ++tj) {
             const int t = tix[ti][tj];
             const double r = A->rvdw;

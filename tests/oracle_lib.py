@@ -30,6 +30,25 @@ class MbpolArgs(ctypes.Structure):
 
 
 _ora = None
+_ora_exact = None
+
+
+def oracle_exact():
+    """The oracle built with the C library's erfc in place of the reference's approximation
+    (liboracle_exact.so): for checks of the mathematics that the approximation blurs."""
+    global _ora_exact
+    if _ora_exact is None:
+        so = os.path.join(ORACLE_DIR, "liboracle_exact.so")
+        if not os.path.exists(so):
+            subprocess.check_call(["make", "-C", ORACLE_DIR, "oracle"], stdout=subprocess.DEVNULL)
+        lib = ctypes.CDLL(so)
+        lib.ora_erfcc.restype = ctypes.c_double
+        lib.ora_erfcc.argtypes = [ctypes.c_double]
+        lib.ora_msite_distribute.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        if lib.ora_dms_load(os.path.join(DATA, "dms_table.txt").encode()) != 0:
+            raise RuntimeError("oracle could not load the DMS table")
+        _ora_exact = lib
+    return _ora_exact
 
 
 def oracle():
@@ -246,11 +265,12 @@ def site_xyz(box):
     return np.ascontiguousarray(box.xyz[:4 * box.nw], float).copy()
 
 
-def elec_forces(box, p, dip0=None, dense=False, xyz=None, cell=None, imcon=None, distribute=True):
+def elec_forces(box, p, dip0=None, dense=False, xyz=None, cell=None, imcon=None, distribute=True, exact_erfc=False):
     """Run the oracle's polarizable electrostatics on a WaterBox with ElecParams p.
     Returns a dict; per-atom arrays are redistributed from the M sites when
-    distribute=True (forces of real atoms, energies, ew1p stresses)."""
-    lib = elec_lib()
+    distribute=True (forces of real atoms, energies, ew1p stresses).  exact_erfc: the
+    oracle build with the exact erfc (oracle_exact)."""
+    lib = oracle_exact() if exact_erfc else elec_lib()
     nw = box.nw
     n = 4 * nw
     a = ElecArgs()

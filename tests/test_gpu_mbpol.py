@@ -151,7 +151,7 @@ def test_unsupported_imcon_is_an_error(pkg, box64):
     ctx = pkg.Context(box64.nw, box64.natms)
     f = np.zeros((3, box64.natms))
     with pytest.raises(pkg.MbgpuError, match="imcon"):
-        ctx.mbpol_forces(0, box64.cell, 3 * box64.nw, box64.listttm2, box64.x, box64.y, box64.z, f[0], f[1], f[2])
+        ctx.mbpol_forces(4, box64.cell, 3 * box64.nw, box64.listttm2, box64.x, box64.y, box64.z, f[0], f[1], f[2])
 
 
 def test_bad_list_is_an_error(pkg, box64):

@@ -141,4 +141,4 @@ def test_parameter_errors(pkg, waterbox, wp):
     with pytest.raises(pkg.MbgpuError, match="rvdw"):
         ctx.water_terms(1, box.cell, dataclasses.replace(wp.for_box(box.L), rcut=5.0), *a)
     with pytest.raises(pkg.MbgpuError, match="imcon"):
-        ctx.water_terms(0, box.cell, wp.for_box(box.L), *a)
+        ctx.water_terms(4, box.cell, wp.for_box(box.L), *a)

@@ -322,3 +322,35 @@ def test_rotation_invariance(box16, par16, res16):
     r = o.elec_forces(box16, par16, xyz=xyz @ R.T)
     assert abs(r["eng"] - res16["eng"]) <= 1e-8 * abs(res16["eng"])
     assert np.abs(r["f"] - res16["f"] @ R.T).max() <= 1e-7 * np.abs(res16["f"]).max()
+
+
+def test_cluster_direct_sum_is_the_open_boundary_limit_of_ewald():
+    """An isolated cluster (imcon 0: direct sums, alpha 0, no reciprocal space) against the
+    Ewald sum of the same cluster centred in periodic boxes of 200 and 300 A (alpha 0.05 1/A,
+    real space within L/2 - 1, kmax 20 / 30: converged) plus the term that turns the Ewald
+    sum's conducting boundary into vacuum, 2 pi |M|^2 / (3V) r4pie0 with M the cluster's
+    dipole (charges only, polr2 = 0, so that no induced dipole feels the surface field).
+    What is left is the boxes' higher multipoles, O(1/L^5): the residual falls by
+    (300/200)^5 = 7.6 and is 3e-8 of the energy at 300 A.  The Ewald sums use the oracle's
+    exact-erfc build: with the reference's erfc approximation the Ewald energy moves with
+    alpha by 1e-4 of this cluster's energy (the approximation's error times the large
+    cancelling intramolecular terms), which would hide the limit."""
+    import dataclasses
+    wb = importlib.import_module("dlpoly2-mbpol-heat_amd.waterbox")
+    ep = importlib.import_module("dlpoly2-mbpol-heat_amd.elec_params")
+    c = wb.make_cluster(12, seed=5)
+    pd = dataclasses.replace(ep.for_cluster(c.L), polr2=(0.0, 0.0, 0.0))
+    rd = o.elec_forces(c, pd)
+    res = []
+    for L, km in ((200.0, 20), (300.0, 30)):
+        big = wb.WaterBox(c.nw, L, c.xyz + (0.5 * L - 0.5 * c.L), c.vel.copy(), c.mass.copy(), c.listttm2.copy())
+        pe = dataclasses.replace(ep.ElecParams(rcut=0.5 * L - 1.0, alpha=0.05, kmax=(km, km, km)),
+                                 polr2=(0.0, 0.0, 0.0))
+        re = o.elec_forces(big, pe, exact_erfc=True)
+        assert np.abs(rd["chge"] - re["chge"]).max() <= 1e-12
+        M = (re["chge"][:, None] * re["xyz"]).sum(0)
+        surface = 2.0 * np.pi * (M @ M) / (3.0 * L ** 3) * 138935.4835
+        res.append(rd["eng"] - (re["eng"] + surface))
+    ratio = res[0] / res[1]
+    assert 6.0 < ratio < 9.5
+    assert abs(res[1]) <= 1e-7 * abs(rd["eng"])

@@ -11,9 +11,8 @@ coordinates of a 16-water box:
   is the derivative of the exact erfc, not of the approximation (ewald3p.f:104-114,
   erfcgenp.f).  The mismatch (about 2e-6 of the largest force) is a property of the
   reference's arithmetic, which this repository keeps for parity; with the exact erfc in
-  its place it falls to the differences' own noise (3e-5 absolute, measured on a scratch
-  copy of the oracle).  A dropped term, a wrong sign or a wrong factor would give a
-  mismatch of order one.
+  its place (the oracle's second build) it falls within 1e-8, the differences' own noise.
+  A dropped term, a wrong sign or a wrong factor would give a mismatch of order one.
 """
 import dataclasses
 import importlib
@@ -42,8 +41,9 @@ def test_forces_are_gradients_of_the_energies():
 
     def terms(b):
         m, e, w = o.mbpol_forces(b), o.elec_forces(b, pe), o.water_terms(b, pw)
+        x = o.elec_forces(b, pe, exact_erfc=True)
         return {"mbpol": (m["eng"], m["f"][:b.nreal]), "elec": (e["eng"], e["f"]),
-                "water": (w["eng_bnd"] + w["eng_ang"] + w["eng_srp"], w["f"])}
+                "water": (w["eng_bnd"] + w["eng_ang"] + w["eng_srp"], w["f"]), "elec_exact": (x["eng"], x["f"])}
 
     base = terms(box)
     worst = {k: 0.0 for k in base}
@@ -56,3 +56,4 @@ def test_forces_are_gradients_of_the_energies():
     print("max |F - FD| / max |F|:", {t: f"{v:.2e}" for t, v in rel.items()})
     assert rel["mbpol"] <= 1e-8 and rel["water"] <= 1e-8
     assert 1e-7 <= rel["elec"] <= 1e-5
+    assert rel["elec_exact"] <= 1e-8

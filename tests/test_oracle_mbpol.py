@@ -113,8 +113,10 @@ def _shift(box, atom, ax, h):
 
 
 def test_unsupported_imcon_is_an_error(box64):
+    """imcon 4-7 (truncated octahedron, ...) are not images this path handles: the reference
+    stops in fixme (mbpol.f90:1357); imcon 0 is the isolated cluster, without images."""
     import copy
     b = copy.deepcopy(box64)
-    b.imcon = 0
+    b.imcon = 4
     with pytest.raises(RuntimeError):
         o.mbpol_forces(b)
