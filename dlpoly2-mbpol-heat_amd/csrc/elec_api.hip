@@ -106,7 +106,8 @@ void free_buffers(ElecBuffers& E)
     dfree(E.x); dfree(E.y); dfree(E.z); dfree(E.chge); dfree(E.polr); dfree(E.polr2); dfree(E.dg); dfree(E.tab);
     dfree(E.nbrA); dfree(E.cntA); dfree(E.nbrP); dfree(E.cntP); dfree(E.tens); dfree(E.tin);
     dfree(E.dip); dfree(E.dip0); dfree(E.efk); dfree(E.emu); dfree(E.efdc); dfree(E.efdd);
-    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.cg_tmu); dfree(E.cg_smu); dfree(E.cgblk); dfree(E.ksum); dfree(E.state); dfree(E.zero_state);
+    dfree(E.cg_p); dfree(E.cg_q); dfree(E.cg_r); dfree(E.cg_z); dfree(E.cg_minv); dfree(E.cg_tmu); dfree(E.cg_smu); dfree(E.cgblk);
+    dfree(E.cg_u); dfree(E.cg_w); dfree(E.cg_m); dfree(E.cg_s); dfree(E.cg_sp); dfree(E.cg_su); dfree(E.cg_sq); dfree(E.ksum); dfree(E.state); dfree(E.zero_state);
     dfree(E.kv); dfree(E.klmn); dfree(E.runs); dfree(E.runk); dfree(E.phc); dfree(E.phs);
     dfree(E.S); dfree(E.Spart); dfree(E.Sq); dfree(E.kc); dfree(E.kf); dfree(E.Fpart);
     dfree(E.vk); dfree(E.vr); dfree(E.vs); dfree(E.vc); dfree(E.vdck); dfree(E.vdcr); dfree(E.vesp);
@@ -130,9 +131,12 @@ int alloc_sites(ElecState& S, int nw, std::string& err)
     ECHK(dalloc(&E.efdc, (size_t)3 * n)); ECHK(dalloc(&E.efdd, (size_t)3 * n));
     ECHK(dalloc(&E.cg_p, (size_t)3 * n)); ECHK(dalloc(&E.cg_q, (size_t)3 * n));
     ECHK(dalloc(&E.cg_r, (size_t)3 * n)); ECHK(dalloc(&E.cg_z, (size_t)3 * n));
+    ECHK(dalloc(&E.cg_u, (size_t)3 * n)); ECHK(dalloc(&E.cg_w, (size_t)3 * n));
+    ECHK(dalloc(&E.cg_m, (size_t)3 * n)); ECHK(dalloc(&E.cg_s, (size_t)3 * n));
     ECHK(dalloc(&E.cg_minv, (size_t)45 * nw));
     ECHK(dalloc(&E.cg_tmu, (size_t)3 * n));
-    ECHK(dalloc(&E.cgblk, (size_t)(9 * nw + 63) / 64 + 2 * ((nw + 27) / 28) + 3 * ((nw + 255) / 256) + 64));
+    // the solver's vector update writes 5 partials per block of 7 waters
+    ECHK(dalloc(&E.cgblk, (size_t)5 * ((nw + 6) / 7) + 64));
     ECHK(dalloc(&E.ksum, (size_t)16 * n));
     ECHK(dalloc(&E.state, 16)); ECHK(dalloc(&E.zero_state, 8));
     ECHK(dalloc(&E.vk, n)); ECHK(dalloc(&E.vr, n)); ECHK(dalloc(&E.vs, n)); ECHK(dalloc(&E.vc, n));
@@ -313,7 +317,13 @@ int build_klist(ElecState& S, const double* cell, const int* kmax, double alpha,
     E.dspc = (nrow + E.dchunks - 1) / E.dchunks;
     ECHK(dalloc(&E.S, (size_t)8 * K));
     dfree(E.cg_smu);
+    dfree(E.cg_sp);
+    dfree(E.cg_su);
+    dfree(E.cg_sq);
     ECHK(dalloc(&E.cg_smu, (size_t)6 * K));
+    ECHK(dalloc(&E.cg_sp, (size_t)6 * K));
+    ECHK(dalloc(&E.cg_su, (size_t)6 * K));
+    ECHK(dalloc(&E.cg_sq, (size_t)6 * K));
     ECHK(dalloc(&E.Sq, (size_t)2 * K));
     ECHK(dalloc(&E.kc, (size_t)16 * K));
     ECHK(dalloc(&E.kf, (size_t)16 * K));

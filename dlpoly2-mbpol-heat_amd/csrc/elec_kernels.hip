@@ -86,6 +86,52 @@ __device__ double chunk_sum64(const double* __restrict__ in, long cs, int nc, lo
     return t;
 }
 
+// Stopping test and scalars of the conjugate-gradient solver at the start of iteration `it`,
+// from the block partials of its last vector update: nb blocks of (r.u, w.u,
+// |polr2 r / r4pie0|^2, dipole count, max |m_i|).  Every wavefront that calls it sums the partials in
+// the same order and takes lane 0's sums, so all of a launch's wavefronts reach the same
+// decision without a grid-wide synchronization.  With `record`, lane 0 stores the iteration
+// count, the RMS step and then the done flag or the scalars of the update: beta = g / g_prev,
+// alpha = g / (d - beta g / alpha_prev) (first iteration: beta = 0, alpha = g / d), with
+// g = r.u and d = w.u (the Chronopoulos-Gear recurrence), and a power of two of about
+// 1 / max |m_i| for the particle mesh's spreading of m.  state = [iterations, rms step, done, g,
+// alpha, beta, -, d, -, m scale].  Returns true when the dipoles have converged.
+__device__ bool cg_step(const ElecDev& P, int it, int nb, const double* __restrict__ blk, double* __restrict__ state,
+                        bool record)
+{
+    const int lane = threadIdx.x & 63;
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = 0.0;
+    for (int k = lane; k < nb; k += 64) {
+        a += blk[5 * k];
+        b += blk[5 * k + 1];
+        c += blk[5 * k + 2];
+        d += blk[5 * k + 3];
+        e = fmax(e, blk[5 * k + 4]);
+    }
+    a = __shfl(wsum(a), 0, 64);
+    b = __shfl(wsum(b), 0, 64);
+    c = __shfl(wsum(c), 0, 64);
+    d = __shfl(wsum(d), 0, 64);
+    for (int o = 32; o > 0; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
+    const double rms = (1.0 - P.omega) * sqrt(c / (d == 0.0 ? 1.0 : d));
+    const bool done = rms <= P.toler;
+    if (record && lane == 0) {
+        state[0] = it;
+        state[1] = rms;
+        state[9] = (e > 0.0 && isfinite(e)) ? ldexp(1.0, -(ilogb(e) + 1)) : 1.0;
+        if (done) {
+            state[2] = 1.0;
+        } else {
+            const double beta = it ? a / state[3] : 0.0;
+            state[4] = it ? a / (b - beta * a / state[4]) : a / b;
+            state[5] = beta;
+            state[3] = a;
+            state[7] = b;
+        }
+    }
+    return done;
+}
+
 __device__ __forceinline__ int water_of(const ElecDev& P, int i) { return i < P.nreal ? i / 3 : i - P.nreal; }
 
 __device__ __forceinline__ double site_polr(const ElecDev& P, int i)
@@ -539,14 +585,21 @@ __device__ __forceinline__ void run_phase(double el_c, double el_s, double em_c,
 #define EL_SG 4                       // site groups (wavefronts) per structure-factor block
 #endif
 #define EL_HALF (EL_RUN / EL_SPLIT)
+// With cgblk (the conjugate-gradient solver's partials, cgnb blocks), the pass first takes the
+// solver's stopping test for iteration `it` (cg_step; block (0, 0) records the scalars) and
+// does nothing once the dipoles have converged.
 template <bool CHG>
 __global__ __launch_bounds__(64 * EL_SG) void k_el_sfac(ElecDev P, int spc, const int* __restrict__ runs,
                                                         const int* __restrict__ runk, const double* __restrict__ phc,
                                                         const double* __restrict__ phs, const double* __restrict__ chge,
                                                         const double* __restrict__ dip, double* __restrict__ Spart,
-                                                        const double* __restrict__ state)
+                                                        double* __restrict__ state, const double* __restrict__ cgblk,
+                                                        int cgnb, int it)
 {
     if (!CHG && state[2] != 0.0) return;
+    if (!CHG && cgblk &&
+        cg_step(P, it, cgnb, cgblk, state, blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 64))
+        return;
     extern __shared__ double lds[];
     constexpr int NT = 64 * EL_SG;
     constexpr int NV = CHG ? 8 : 6;
@@ -696,17 +749,15 @@ __global__ __launch_bounds__(64 * EL_SG) void k_el_sfac(ElecDev P, int spc, cons
 // (k_a akk ckcs, -k_a akk ckss) (ewald1p.f:749-757).  A block sums 64 (k, component)
 // elements (8 k vectors); lanes 8k'..8k'+7 of the first wave hold the 8 sums of one k.
 template <bool CHG>
-__global__ __launch_bounds__(256) void k_el_sreduce(ElecDev P, int schunks, const double* __restrict__ Spart,
-                                                    const double* __restrict__ kv, double* __restrict__ S,
-                                                    double* __restrict__ Sq, double* __restrict__ kc,
-                                                    const double* __restrict__ state)
+__device__ void sreduce_block(const ElecDev& P, int blk, int schunks, const double* __restrict__ Spart,
+                              const double* __restrict__ kv, double* __restrict__ S, double* __restrict__ Sq,
+                              double* __restrict__ kc)
 {
-    if (!CHG && state[2] != 0.0) return;
     // NV values per k in the partials (see k_el_sfac); a block sums EPB consecutive
     // elements, a whole number of k
     constexpr int NV = CHG ? 8 : 6, EPB = CHG ? 64 : 60, Q0 = CHG ? 0 : 2;
     const int lane = threadIdx.x & 63;
-    const long e = (long)blockIdx.x * EPB + lane;
+    const long e = (long)blk * EPB + lane;
     const double v = chunk_sum64(Spart, (long)NV * P.K, schunks, e, lane < EPB && e < (long)NV * P.K);
     if (threadIdx.x >= 64) return;
     const int base = lane - lane % NV;
@@ -741,6 +792,16 @@ __global__ __launch_bounds__(256) void k_el_sreduce(ElecDev P, int schunks, cons
     }
 }
 
+template <bool CHG>
+__global__ __launch_bounds__(256) void k_el_sreduce(ElecDev P, int schunks, const double* __restrict__ Spart,
+                                                    const double* __restrict__ kv, double* __restrict__ S,
+                                                    double* __restrict__ Sq, double* __restrict__ kc,
+                                                    const double* __restrict__ state)
+{
+    if (!CHG && state[2] != 0.0) return;
+    sreduce_block<CHG>(P, blockIdx.x, schunks, Spart, kv, S, Sq, kc);
+}
+
 // Reciprocal-space fields over a chunk of runs, site-parallel (ewald1p.f:749-769):
 // efdd_a += c_a skr + d_a ckr (dipole field), with CHG also efdc_a += e_a skr + f_a ckr
 // (charge field).  A block covers 128 sites (two per lane: j and j + 64) and EL_KG run
@@ -765,6 +826,17 @@ __global__ __launch_bounds__(64 * EL_KG) void k_el_kfield(ElecDev P, const int* 
     const int chunk = blockIdx.y;
     const int r0 = chunk * P.frpc, r1 = min(P.nrun, r0 + P.frpc);
     const int nr = max(0, r1 - r0);
+    // two sites per lane share every constant read; lanes past the last site compute on a
+    // clamped site and store nothing (every thread reaches the barriers below).  The n = 1
+    // phase factors are loaded before the staging barrier, which completes them: loaded
+    // later, the wait for them inside the k loop would also wait for the run prefetches.
+    const int nrow = CHG ? P.n : P.nrow;
+    const int ja = blockIdx.x * 128 + lane, jb = ja + 64;
+    const bool ha = ja < nrow, hb = jb < nrow;
+    const int jA = ha ? ja : 0, jB = hb ? jb : jA;
+    const int ks = P.kstride;
+    const long o1 = ((long)2 * ks + 1) * P.n;
+    const double ca1 = phc[o1 + jA], sa1 = phs[o1 + jA], cb1 = phc[o1 + jB], sb1 = phs[o1 + jB];
     double* sk = lds;                                          // [fkmax][NC]
     int* sr = reinterpret_cast<int*>(lds + (long)P.fkmax * NC); // [frpc][4] runs
     int* so = sr + 4 * P.frpc;                                  // [frpc] first k of each run - kbase
@@ -777,15 +849,6 @@ __global__ __launch_bounds__(64 * EL_KG) void k_el_kfield(ElecDev P, const int* 
     for (int idx = tid; idx < nr * 4; idx += NT) sr[idx] = runs[4 * r0 + idx];
     for (int idx = tid; idx < nr; idx += NT) so[idx] = runk[r0 + idx] - kbase;
     __syncthreads();
-    // two sites per lane share every constant read; lanes past the last site compute on a
-    // clamped site and store nothing (every thread reaches the barriers below)
-    const int nrow = CHG ? P.n : P.nrow;
-    const int ja = blockIdx.x * 128 + lane, jb = ja + 64;
-    const bool ha = ja < nrow, hb = jb < nrow;
-    const int jA = ha ? ja : 0, jB = hb ? jb : jA;
-    const int ks = P.kstride;
-    const long o1 = ((long)2 * ks + 1) * P.n;
-    const double ca1 = phc[o1 + jA], sa1 = phs[o1 + jA], cb1 = phc[o1 + jB], sb1 = phs[o1 + jB];
     double fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0}, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
     // the phase-table entries of a run are loaded one run ahead (double-buffered in
     // registers), so their latency overlaps the previous run's arithmetic
@@ -908,14 +971,10 @@ __global__ __launch_bounds__(256) void k_el_kfield_reduce(ElecDev P, const doubl
 
 // Real-space dipole field of one row (rfield.f:201-317 list pairs; 419-520 excluded
 // pairs): emu_i = sum_j T_ij mu_j.  One wavefront per site.
-__global__ __launch_bounds__(64) void k_el_emu(ElecDev P, const int* __restrict__ nbrP, const int* __restrict__ cntP,
-                                               const double* __restrict__ tens, const double* __restrict__ tin,
-                                               const double* __restrict__ dip, double* __restrict__ emu,
-                                               const double* __restrict__ state)
+__device__ void emu_row(const ElecDev& P, int i, int lane, const int* __restrict__ nbrP, const int* __restrict__ cntP,
+                        const double* __restrict__ tens, const double* __restrict__ tin,
+                        const double* __restrict__ dip, double* __restrict__ emu)
 {
-    if (state[2] != 0.0) return;
-    const int i = blockIdx.x;
-    const int lane = threadIdx.x;
     if (i >= P.nrow) return;
     const long nc = (long)P.n * P.capP;
     double e[3] = {0, 0, 0};
@@ -950,6 +1009,40 @@ __global__ __launch_bounds__(64) void k_el_emu(ElecDev P, const int* __restrict_
     for (int q = 0; q < 3; ++q) e[q] = wsum(e[q]);
     if (lane == 0)
         for (int q = 0; q < 3; ++q) emu[3 * i + q] = e[q];
+}
+
+// emu for four rows per block.  With cgblk (the conjugate-gradient solver's partials, cgnb
+// blocks), the solver's stopping test for iteration `it` comes first (cg_step; block 0
+// records the scalars) and the kernel does nothing once the dipoles have converged.
+__global__ __launch_bounds__(256) void k_el_emu(ElecDev P, const int* __restrict__ nbrP, const int* __restrict__ cntP,
+                                                const double* __restrict__ tens, const double* __restrict__ tin,
+                                                const double* __restrict__ dip, double* __restrict__ emu,
+                                                double* __restrict__ state, const double* __restrict__ cgblk, int cgnb,
+                                                int it)
+{
+    if (state[2] != 0.0) return;
+    if (cgblk && cg_step(P, it, cgnb, cgblk, state, blockIdx.x == 0 && threadIdx.x < 64)) return;
+    emu_row(P, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63, nbrP, cntP, tens, tin, dip, emu);
+}
+
+// One launch for two independent parts of an application of the dipole-field operator:
+// blocks [0, nsr) sum the dipole structure factors (k_el_sreduce<false>), the others form
+// the real-space field emu, four rows per block.
+__global__ __launch_bounds__(256) void k_el_sreduce_emu(ElecDev P, int nsr, int schunks, const double* __restrict__ Spart,
+                                                        const double* __restrict__ kv, double* __restrict__ S,
+                                                        double* __restrict__ Sq, double* __restrict__ kc,
+                                                        const int* __restrict__ nbrP, const int* __restrict__ cntP,
+                                                        const double* __restrict__ tens, const double* __restrict__ tin,
+                                                        const double* __restrict__ dip, double* __restrict__ emu,
+                                                        const double* __restrict__ state)
+{
+    if (state[2] != 0.0) return;
+    const int b = blockIdx.x;
+    if (b < nsr) {
+        sreduce_block<false>(P, b, schunks, Spart, kv, S, Sq, kc);
+        return;
+    }
+    emu_row(P, (b - nsr) * 4 + (threadIdx.x >> 6), threadIdx.x & 63, nbrP, cntP, tens, tin, dip, emu);
 }
 
 // SOR update (iter.f:58-89) and per-block sums of |dmu|^2 and of the dipole count.
@@ -1005,40 +1098,31 @@ __global__ __launch_bounds__(64) void k_el_converge(ElecDev P, int nb, const dou
 
 
 // ==================================================== conjugate-gradient solver ======
-// The induced dipoles solve A mu = E0 with A = r4pie0 / polr2 - T (T: the dipole-field
-// operator, emu + efdd; E0 = efk + efdc).  A is symmetric positive definite for Thole-
-// damped dipoles, so preconditioned conjugate gradients reach the SOR fixed point of
-// iter.f in far fewer iterations.  The preconditioner is block-Jacobi over waters: the
+// The induced dipoles solve A mu = E0 with A = D - T, D = r4pie0 / polr2 (per site), T the
+// dipole-field operator (emu + efdd) and E0 = efk + efdc.  A is symmetric positive definite
+// for Thole-damped dipoles, so preconditioned conjugate gradients reach the SOR fixed point
+// of iter.f in far fewer iterations.  The preconditioner M is block-Jacobi over waters: the
 // 9 x 9 block of each water's O, H1, H2 dipoles, which holds the strong intramolecular
 // couplings (half the iterations of a diagonal preconditioner at liquid density).  The
 // stopping test is the reference's: the RMS of the step an SOR update would take from the
 // current dipoles, (1 - omega) * rms(polr2 r / r4pie0), against toler (iter.f:95-101).
-// state = [iterations, rms step, done, r.z, alpha, beta, dipole count, -, r.z by iteration
-// parity (two slots)].  The scalar steps need no grid-wide synchronization: every block of
-// the kernel that needs alpha or beta sums the previous kernel's block partials itself, in
-// the same fixed order, and the current r.z sits in the slot of the iteration's parity
-// (passed by the host), which that kernel only reads; its block 0 writes the other slot.
-
-// Scalar steps of the solver from the summed partials (a, b, c):
-//   mode 0 (start):        r.z = a, rms test on b, dipole count c -> state[6]
-//   mode 1 (after T p):    alpha = r.z / p.q (a = p.q)
-//   mode 2 (after update): beta = r.z(new) / r.z = a / state[3], iteration count, rms test on b
-__device__ void cg_scalars(const ElecDev& P, int mode, double a, double b, double c, double* __restrict__ state)
-{
-    if (mode == 1) {
-        state[4] = state[3] / a;
-        return;
-    }
-    if (mode == 0) state[6] = c == 0.0 ? 1.0 : c;
-    const double rms = (1.0 - P.omega) * sqrt(b / state[6]);
-    if (mode == 2) {
-        state[5] = a / state[3];
-        state[0] += 1.0;
-    }
-    state[3] = a;
-    state[1] = rms;
-    if (rms <= P.toler) state[2] = 1.0;
-}
+//
+// The iteration is the Chronopoulos-Gear form of preconditioned conjugate gradients
+// (A. T. Chronopoulos and C. W. Gear, J. Comput. Appl. Math. 25 (1989) 153-168), in which
+// both inner products of an iteration are taken after its vector update, so that one
+// iteration is one application of A and one fused update with a single set of block
+// partials:
+//   n = A m
+//   z = n + beta z,  q = m + beta q,  s = w + beta s,  p = u + beta p
+//   mu += alpha p,  r -= alpha s,  u -= alpha q,  w -= alpha z,  m = M^-1 w
+//   partials of g = r.u and d = w.u, from which cg_step forms the next alpha and beta
+// (u = M^-1 r, w = A u, s = A p, q = M^-1 s, z = A q throughout).  It starts from
+// r = E0 - A mu0, u = M^-1 r and one extra application of A for w = A u.  T mu, which the
+// final passes need, follows mu as T mu += alpha (D p - s); with the Ewald sum over k the
+// dipole structure factors follow it too: S(q) = S(m) + beta S(q), S(p) = S(u) + beta S(p),
+// S(mu) += alpha S(p), S(u) -= alpha S(q).  The scalars need no grid-wide synchronization:
+// the first kernel of each iteration sums the update's block partials in every wavefront
+// (cg_step) and its first block records them for the update.
 
 // S(mu) of the start dipoles (the dipole part of the preparation's structure factors)
 __global__ __launch_bounds__(256) void k_cg_smu_init(ElecDev P, const double* __restrict__ S, double* __restrict__ Smu)
@@ -1175,186 +1259,169 @@ __device__ __forceinline__ void cg_precondition(const double* __restrict__ m, co
     }
 }
 
-// r = E0 + T mu0 - r4pie0 mu0 / polr2, z = M^-1 r, p = z; one thread per water; partials
-// (r.z, |polr2 r / r4pie0|^2, dipole count)
+// r = E0 + T mu0 - D mu0 (T mu0 = emu + efdd of the start dipoles), u = M^-1 r, T mu = T mu0;
+// the first application of A goes to m = u; z, q, s, p start at zero.  One thread per water.
 __global__ __launch_bounds__(256) void k_cg_init(ElecDev P, const double* __restrict__ efk,
                                                  const double* __restrict__ efdc, const double* __restrict__ emu,
                                                  const double* __restrict__ efdd, const double* __restrict__ dip,
                                                  const double* __restrict__ minv, double* __restrict__ r,
-                                                 double* __restrict__ z, double* __restrict__ pv,
-                                                 double* __restrict__ tmu, double* __restrict__ blk)
+                                                 double* __restrict__ u, double* __restrict__ m, double* __restrict__ z,
+                                                 double* __restrict__ qv, double* __restrict__ sv,
+                                                 double* __restrict__ pv, double* __restrict__ tmu)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[3] = {0.0, 0.0, 0.0};
-    if (w < P.nw) {
-        double rr[9], zz[9];
-        bool on[3];
+    if (w >= P.nw) return;
+    double rr[9], uu[9];
+    bool on[3];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-            const double p2 = site_polr2(P, 3 * w + s);
-            on[s] = p2 > P.toler;
+    for (int s = 0; s < 3; ++s) {
+        const double p2 = site_polr2(P, 3 * w + s);
+        on[s] = p2 > P.toler;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int o = 9 * w + 3 * s + c;
-                tmu[o] = emu[o] + efdd[o];
-                rr[3 * s + c] = on[s] ? (efk[o] + efdc[o] + emu[o] + efdd[o]) - R4PIE0 * dip[o] / p2 : 0.0;
-                const double zs = on[s] ? p2 * rr[3 * s + c] / R4PIE0 : 0.0;
-                v[1] += zs * zs;
-            }
-            v[2] += on[s] ? 1.0 : 0.0;
-        }
-        cg_precondition(minv + 45L * w, rr, on, zz);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            r[9 * w + q] = rr[q];
-            z[9 * w + q] = zz[q];
-            pv[9 * w + q] = zz[q];
-            v[0] += rr[q] * zz[q];
+        for (int c = 0; c < 3; ++c) {
+            const int o = 9 * w + 3 * s + c;
+            tmu[o] = emu[o] + efdd[o];
+            rr[3 * s + c] = on[s] ? (efk[o] + efdc[o] + emu[o] + efdd[o]) - R4PIE0 * dip[o] / p2 : 0.0;
         }
     }
-    block_sum_store(v, blk + 3 * blockIdx.x);
-}
-
-// Sum of nb block partials (value q of stride nv) on the first wavefront, in block order;
-// the result is returned to every thread of the block.
-template <int NV>
-__device__ double block_partials_sum(const double* __restrict__ blk, int nb, int q)
-{
-    __shared__ double res;
-    if (threadIdx.x < 64) {
-        double a = 0.0;
-        for (int k = threadIdx.x; k < nb; k += 64) a += blk[NV * k + q];
-        a = wsum(a);
-        if (threadIdx.x == 0) res = a;
+    cg_precondition(minv + 45L * w, rr, on, uu);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int o = 9 * w + q;
+        r[o] = rr[q];
+        u[o] = uu[q];
+        m[o] = uu[q];
+        z[o] = 0.0;
+        qv[o] = 0.0;
+        sv[o] = 0.0;
+        pv[o] = 0.0;
     }
-    __syncthreads();
-    const double r = res;
-    __syncthreads();
-    return r;
 }
 
-// Start of the solver from k_cg_init's partials: r.z, the stopping test, the dipole count;
-// r.z also into the parity-0 slot (single wavefront)
-__global__ __launch_bounds__(64) void k_cg_start(ElecDev P, int nb, const double* __restrict__ blk,
-                                                 double* __restrict__ state)
-{
-    double a = 0.0, b = 0.0, c = 0.0;
-    for (int k = threadIdx.x; k < nb; k += 64) {
-        a += blk[3 * k];
-        b += blk[3 * k + 1];
-        c += blk[3 * k + 2];
-    }
-    a = wsum(a);
-    b = wsum(b);
-    c = wsum(c);
-    if (threadIdx.x != 0) return;
-    cg_scalars(P, 0, a, b, c, state);
-    state[8] = a;
-}
-
-// The reciprocal dipole field of p from the chunk partials (efdd = scale * sum + self
-// term, as k_el_kfield_reduce), then q = A p = r4pie0 p / polr2 - (emu + efdd) and the
-// partials of p.q.  Element (component, site) as in
-// k_el_kfield_reduce, 64 per block.
-__global__ __launch_bounds__(256) void k_cg_apply(ElecDev P, const double* __restrict__ Fpart,
-                                                  const double* __restrict__ pv, const double* __restrict__ emu,
-                                                  double* __restrict__ efdd, double* __restrict__ qv,
-                                                  double* __restrict__ blk, const double* __restrict__ state)
+// The vector update of an iteration (INIT: of the start) from n = A m, with A m formed here
+// from the real-space field emu and the reciprocal field's chunk partials (efdd = scale *
+// sum + self term, as k_el_kfield_reduce).  A block takes CG_WPB waters (63 rows, one per
+// lane): its four wavefronts first add a quarter of the field chunks each, in chunk order,
+// and the first wavefront adds the quarters in order and updates the rows; the new w of
+// each water is shared through LDS for m = M^-1 w.  Block partials (r.u, w.u,
+// |polr2 r / r4pie0|^2, dipole count, max |m_i|) per water block.  INIT sets w = A u and
+// leaves mu, r, u as they are.  The blocks after the waters' (Ewald sum over k) carry the
+// dipole structure factors (INIT: S(u) = S(m), S(q) = S(p) = 0).
+#define CG_WPB 7
+template <bool INIT>
+__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int nbw, const double* __restrict__ Fpart,
+                                                   const double* __restrict__ emu, const double* __restrict__ minv,
+                                                   double* __restrict__ m, double* __restrict__ dip,
+                                                   double* __restrict__ r, double* __restrict__ u,
+                                                   double* __restrict__ wv, double* __restrict__ z,
+                                                   double* __restrict__ qv, double* __restrict__ sv,
+                                                   double* __restrict__ pv, double* __restrict__ tmu,
+                                                   const double* __restrict__ S, double* __restrict__ Sx,
+                                                   double* __restrict__ Sp, double* __restrict__ Su,
+                                                   double* __restrict__ Sqr, double* __restrict__ blk,
+                                                   const double* __restrict__ state)
 {
     if (state[2] != 0.0) return;
-    const int t = blockIdx.x * 64 + (threadIdx.x & 63);
-    const bool valid = t < 3 * P.nrow;
-    const int q = valid ? t / P.nrow : 0, j = valid ? t - q * P.nrow : 0;
-    const double sum = chunk_sum64(Fpart, 6L * P.n, P.fchunks, (long)q * P.n + j, valid);
-    double v[1] = {0.0};
-    if (threadIdx.x < 64 && valid) {
-        const double p2 = site_polr2(P, j);
-        const int o = 3 * j + q;
-        const double fd = (p2 > 1.e-6 ? P.scale * sum : 0.0) + P.self_dd * pv[o];
-        efdd[o] = fd;
-        const double qq = p2 > P.toler ? R4PIE0 * pv[o] / p2 - (emu[o] + fd) : 0.0;
-        qv[o] = qq;
-        v[0] = pv[o] * qq;
-    }
-    block_sum_store(v, blk + blockIdx.x);
-}
-
-// mu += alpha p, r -= alpha q, z = M^-1 r, partials (r.z, |polr2 r / r4pie0|^2); one
-// thread per (water, row), CG_WPB waters per block, the new r of a water shared through
-// LDS.  alpha = r.z / p.q, with p.q summed from k_cg_apply's nbq partials by every block.
-// The blocks after the waters' carry S(mu) along (S(mu) += alpha S(p)).
-#define CG_WPB 28
-__global__ __launch_bounds__(256) void k_cg_update(ElecDev P, int it, int nbq, const double* __restrict__ blkq,
-                                                   const double* __restrict__ pv, const double* __restrict__ qv,
-                                                   const double* __restrict__ minv, double* __restrict__ dip,
-                                                   double* __restrict__ r, double* __restrict__ z,
-                                                   const double* __restrict__ emu, const double* __restrict__ efdd,
-                                                   double* __restrict__ tmu, const double* __restrict__ Sp,
-                                                   double* __restrict__ Smu, double* __restrict__ blk,
-                                                   double* __restrict__ state)
-{
-    if (state[2] != 0.0) return;
-    __shared__ double rs[CG_WPB * 9];
+    const double alpha = INIT ? 0.0 : state[4], beta = INIT ? 0.0 : state[5];
     const int tid = threadIdx.x;
-    const double alpha = state[8 + (it & 1)] / block_partials_sum<1>(blkq, nbq, 0);
-    if (blockIdx.x == 0 && tid == 0) state[4] = alpha;
-    const int nbw = (P.nw + CG_WPB - 1) / CG_WPB;
     if ((int)blockIdx.x >= nbw) {
-        // blocks past the waters: the dipole structure factors follow mu (S(p) from this
-        // iteration's sreduce), 256 entries per block
         const long e = (long)(blockIdx.x - nbw) * 256 + tid;
-        if (e < 6L * P.K) Smu[e] += alpha * Sp[8 * (e / 6) + 2 + e % 6];
+        if (e >= 6L * P.K) return;
+        const double sm = S[8 * (e / 6) + 2 + e % 6];
+        if (INIT) {
+            Su[e] = sm;
+            Sqr[e] = 0.0;
+            Sp[e] = 0.0;
+        } else {
+            const double sq = sm + beta * Sqr[e];
+            const double sp = Su[e] + beta * Sp[e];
+            Sqr[e] = sq;
+            Sp[e] = sp;
+            Sx[e] += alpha * sp;
+            Su[e] -= alpha * sq;
+        }
         return;
     }
-    const int wl = tid / 9, i = tid - 9 * wl;
+    __shared__ double part[4][64];
+    __shared__ double ws[64];
+    __shared__ double wmax[4];
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wl = lane / 9, i = lane - 9 * wl;
     const int w = blockIdx.x * CG_WPB + wl;
-    const bool act = wl < CG_WPB && w < P.nw;
-    const int o = 9 * w + i;
-    const double p2 = act ? site_polr2(P, 3 * w + i / 3) : 0.0;
+    const bool act = lane < 9 * CG_WPB && w < P.nw;
+    const int o = 9 * w + i, j = 3 * w + i / 3, c = i % 3;
+    const double p2 = act ? site_polr2(P, j) : 0.0;
     const bool on = act && p2 > P.toler;
-    double rr = 0.0, zs = 0.0;
-    if (on) {
-        dip[o] += alpha * pv[o];
-        tmu[o] += alpha * (emu[o] + efdd[o]);      // T mu follows mu: T p is this iteration's field
-        rr = r[o] - alpha * qv[o];
-        zs = p2 * rr / R4PIE0;
+    {
+        // this wavefront's quarter of the field chunks
+        double sum = 0.0;
+        if (on && p2 > 1.e-6) {
+            const int c0 = (P.fchunks * wave) / 4, c1 = (P.fchunks * (wave + 1)) / 4;
+            const double* f = Fpart + (long)c * P.n + j;
+            const long cs = 6L * P.n;
+#pragma unroll 4
+            for (int ch = c0; ch < c1; ++ch) sum += f[ch * cs];
+        }
+        part[wave][lane] = sum;
     }
-    if (wl < CG_WPB) rs[tid] = rr;
     __syncthreads();
-    double zz = 0.0;
-    if (on) {
-        const double* m = minv + 45L * w;
-#pragma unroll
-        for (int j = 0; j < 9; ++j) zz += m[tri9(i, j)] * rs[9 * wl + j];
-        r[o] = rr;
-        z[o] = zz;
+    double rr = 0.0, uu = 0.0, wn = 0.0;
+    if (wave == 0 && on) {
+        const double sum = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+        const double mm = m[o];
+        const double nv = R4PIE0 * mm / p2 - (emu[o] + (p2 > 1.e-6 ? P.scale * sum : 0.0) + P.self_dd * mm);
+        if (INIT) {
+            wn = nv;
+            wv[o] = wn;
+            rr = r[o];
+            uu = u[o];
+        } else {
+            const double zz = nv + beta * z[o];
+            const double qq = mm + beta * qv[o];
+            const double ss = wv[o] + beta * sv[o];
+            const double pp = u[o] + beta * pv[o];
+            z[o] = zz;
+            qv[o] = qq;
+            sv[o] = ss;
+            pv[o] = pp;
+            dip[o] += alpha * pp;
+            tmu[o] += alpha * (R4PIE0 * pp / p2 - ss);
+            rr = r[o] - alpha * ss;
+            uu = u[o] - alpha * qq;
+            wn = wv[o] - alpha * zz;
+            r[o] = rr;
+            u[o] = uu;
+            wv[o] = wn;
+        }
     }
-    double v[2] = {rr * zz, zs * zs};
-    block_sum_store(v, blk + 2 * blockIdx.x);
+    if (wave == 0) ws[lane] = wn;
+    __syncthreads();
+    double mn = 0.0;
+    if (wave == 0 && on) {
+        const double* mi = minv + 45L * w;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) mn += mi[tri9(i, k)] * ws[9 * wl + k];
+        m[o] = mn;
+    }
+    const double zs = p2 * rr / R4PIE0;
+    const bool row = wave == 0 && on;
+    double v[4] = {rr * uu, wn * uu, row ? zs * zs : 0.0, (row && c == 0) ? 1.0 : 0.0};
+    block_sum_store(v, blk + 5 * blockIdx.x);
+    // max |m_i| of the block (a maximum: the same in any order)
+    double a = fabs(mn);
+    for (int o2 = 32; o2 > 0; o2 >>= 1) a = fmax(a, __shfl_xor(a, o2, 64));
+    if (lane == 0) wmax[wave] = a;
+    __syncthreads();
+    if (tid == 0) blk[5 * blockIdx.x + 4] = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
 }
 
-// p = z + beta p with beta = r.z(new) / r.z, r.z(new) summed from k_cg_update's nbu
-// partials by every block; block 0 records the iteration: the new r.z into the other
-// parity slot, beta, the iteration count and the stopping test.
-__global__ __launch_bounds__(256) void k_cg_direction(ElecDev P, int it, int nbu, const double* __restrict__ blku,
-                                                      const double* __restrict__ z, double* __restrict__ pv,
-                                                      double* __restrict__ state)
+// The stopping test and scalars of iteration `it` alone (cg_step in one wavefront), for
+// paths whose first kernel of the iteration does not take them.
+__global__ __launch_bounds__(64) void k_cg_check(ElecDev P, int it, int nb, const double* __restrict__ blk,
+                                                 double* __restrict__ state)
 {
     if (state[2] != 0.0) return;
-    const double rz = block_partials_sum<2>(blku, nbu, 0);
-    const double beta = rz / state[8 + (it & 1)];
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o < 3 * P.nrow) pv[o] = z[o] + beta * pv[o];
-    if (blockIdx.x != 0) return;
-    const double zz = block_partials_sum<2>(blku, nbu, 1);
-    if (threadIdx.x != 0) return;
-    const double rms = (1.0 - P.omega) * sqrt(zz / state[6]);
-    state[8 + ((it + 1) & 1)] = rz;
-    state[5] = beta;
-    state[3] = rz;
-    state[0] += 1.0;
-    state[1] = rms;
-    if (rms <= P.toler) state[2] = 1.0;
+    (void)cg_step(P, it, nb, blk, state, true);
 }
 
 
@@ -2132,17 +2199,19 @@ int el_launch_setup(ElecBuffers& E, const ElecDev& P, CellList& cells, hipStream
 // after the neighbour counts were checked by the host
 // The reciprocal field of the dipoles mu at the first n sites (particle mesh) into
 // Fpart[c * P.n + j] as a k-sum partial (times kap), for the chunk-sum readers with one chunk
+// (scale_state > 0: mu spread times the power of two in state[scale_state], see SpSpec)
 static void el_spme_dipole_field(ElecBuffers& E, const ElecSpme& sp, int n, const double* mu, const double* state,
-                                 hipStream_t s)
+                                 hipStream_t s, int scale_state = 0)
 {
     SpSpec spec{};
     spec.ngr = 1;
     spec.mu_on[0] = 1;
     spec.comp[0] = -1;
+    spec.scale_state = scale_state;
     double2* g = sp.B->grid;
     sp_launch_spread(*sp.B, *sp.G, n, nullptr, mu, spec, g, true, state, s);
     sp_launch_convolve(*sp.B, *sp.G, g, 1, true, state, s);
-    sp_launch_field(*sp.B, *sp.G, n, g, 1, sp.kap, E.Fpart, E.n, state, s);
+    sp_launch_field(*sp.B, *sp.G, n, g, 1, sp.kap, E.Fpart, E.n, state, s, scale_state);
 }
 
 static inline ElecDev one_chunk(const ElecDev& P)
@@ -2190,7 +2259,7 @@ int el_launch_prep(ElecBuffers& E, const ElecDev& P, const double* rcell_dev, co
                        E.phs);
     // charge structure factors and the iloop = 1 reciprocal fields
     hipLaunchKernelGGL(k_el_sfac<true>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.schunks), dim3(64 * EL_SG), sfac_lds(P), s, P, E.spc,
-                       E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
+                       E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state, nullptr, 0, 0);
     hipLaunchKernelGGL(k_el_sreduce<true>, dim3(ceil_div(8L * P.K, 64)), dim3(256), 0, s, P, E.schunks, E.Spart, E.kv, E.S,
                        E.Sq, E.kc, E.state);
     hipLaunchKernelGGL(k_el_kfield<true>, dim3(ceil_div(P.n, 128), P.fchunks), dim3(64 * EL_KG), kfield_lds(P), s, P, E.runs,
@@ -2212,7 +2281,7 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, const ElecS
         // reciprocal dipole field from the current dipoles (the first iteration's came
         // with the charge field)
         hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
-                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state);
+                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.state, nullptr, 0, 0);
         hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv,
                            E.S, E.Sq, E.kc, E.state);
         hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 128), P.fchunks), dim3(64 * EL_KG), kfield_lds(P), s, P,
@@ -2220,66 +2289,83 @@ int el_launch_iteration(ElecBuffers& E, const ElecDev& P, int first, const ElecS
         hipLaunchKernelGGL(k_el_kfield_reduce<false>, dim3(ceil_div(3L * P.nrow, 64)), dim3(256), 0, s, P, E.Fpart, E.dip,
                            E.efdd, E.efdc, E.state);
     }
-    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
+    hipLaunchKernelGGL(k_el_emu, dim3(ceil_div(P.nrow, 4)), dim3(256), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip,
+                       E.emu, E.state, nullptr, 0, 0);
     const int nb = ceil_div(P.nrow, 256);
     hipLaunchKernelGGL(k_el_sor, dim3(nb), dim3(256), 0, s, P, E.efk, E.emu, E.efdd, E.efdc, E.dip, E.blk, E.state);
     hipLaunchKernelGGL(k_el_converge, dim3(1), dim3(64), 0, s, P, nb, E.blk, E.state);
     return (int)hipGetLastError();
 }
 
-// T v into emu / efdd (real-space rows and reciprocal space) for the vector v
-// Block partials of the solver kernels in E.cgblk: k_cg_apply's at the start, then
-// k_cg_update's (disjoint, since every update block reads all of k_cg_apply's).
-static inline int cg_nbq(const ElecDev& P) { return ceil_div(3 * P.nrow, 64); }
+// The dipole-field operator applied to the solver's m (emu: real-space rows; Fpart: the
+// reciprocal field's partials), for the k-sum: structure factors (taking the stopping test
+// of iteration `it` first when `check`), their sum and emu in one launch, the field.
+static void el_cg_apply_ksum(ElecBuffers& E, const ElecDev& P, bool check, int it, hipStream_t s)
+{
+    const int nbw = ceil_div(P.nw, CG_WPB), nsr = ceil_div(6 * P.K, 60);
+    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG),
+                       sfac_lds(P), s, P, E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.cg_m, E.Spart, E.state,
+                       check ? E.cgblk : nullptr, nbw, it);
+    hipLaunchKernelGGL(k_el_sreduce_emu, dim3(nsr + ceil_div(P.nrow, 4)), dim3(256), 0, s, P, nsr, E.dchunks, E.Spart,
+                       E.kv, E.S, E.Sq, E.kc, E.nbrP, E.cntP, E.tens, E.tin, E.cg_m, E.emu, E.state);
+    hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 128), P.fchunks), dim3(64 * EL_KG), kfield_lds(P), s, P,
+                       E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
+}
+
+// The same for the particle mesh (emu first, with the stopping test when `check`) and for no
+// reciprocal space (emu alone; the preparation's zero field partials stand)
+static void el_cg_apply_other(ElecBuffers& E, const ElecDev& P, const ElecSpme& sp, bool check, int it, hipStream_t s)
+{
+    const int nbw = ceil_div(P.nw, CG_WPB);
+    hipLaunchKernelGGL(k_el_emu, dim3(ceil_div(P.nrow, 4)), dim3(256), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.cg_m,
+                       E.emu, E.state, check ? E.cgblk : nullptr, nbw, it);
+    if (sp.B) el_spme_dipole_field(E, sp, P.nrow, E.cg_m, E.state, s, 9);
+}
+
+static void el_cg_update(ElecBuffers& E, const ElecDev& P, const ElecSpme* sp, bool init, hipStream_t s)
+{
+    const int nbw = ceil_div(P.nw, CG_WPB);
+    const ElecDev Q = sp ? one_chunk(P) : P;
+    const int nsb = sp ? 0 : ceil_div(6 * P.K, 256);
+    if (init)
+        hipLaunchKernelGGL(k_cg_update<true>, dim3(nbw + nsb), dim3(256), 0, s, Q, nbw, E.Fpart, E.emu, E.cg_minv,
+                           E.cg_m, E.dip, E.cg_r, E.cg_u, E.cg_w, E.cg_z, E.cg_q, E.cg_s, E.cg_p, E.cg_tmu, E.S,
+                           E.cg_smu, E.cg_sp, E.cg_su, E.cg_sq, E.cgblk, E.state);
+    else
+        hipLaunchKernelGGL(k_cg_update<false>, dim3(nbw + nsb), dim3(256), 0, s, Q, nbw, E.Fpart, E.emu, E.cg_minv,
+                           E.cg_m, E.dip, E.cg_r, E.cg_u, E.cg_w, E.cg_z, E.cg_q, E.cg_s, E.cg_p, E.cg_tmu, E.S,
+                           E.cg_smu, E.cg_sp, E.cg_su, E.cg_sq, E.cgblk, E.state);
+}
 
 // conjugate-gradient start: residual of the start dipoles (their fields came with the
-// charge fields of the preparation, plus the real-space dipole field here)
+// charge fields of the preparation, plus the real-space dipole field here), u = M^-1 r, and
+// the first application of A, to u, for w = A u
 int el_launch_cg_init(ElecBuffers& E, const ElecDev& P, const ElecSpme* sp, hipStream_t s)
 {
     const int nbw = ceil_div(P.nw, 256);
     hipLaunchKernelGGL(k_cg_block, dim3(nbw), dim3(256), 0, s, P, E.x, E.y, E.z, E.cg_minv);
-    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip, E.emu, E.state);
+    hipLaunchKernelGGL(k_el_emu, dim3(ceil_div(P.nrow, 4)), dim3(256), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, E.dip,
+                       E.emu, E.state, nullptr, 0, 0);
     hipLaunchKernelGGL(k_cg_init, dim3(nbw), dim3(256), 0, s, P, E.efk, E.efdc, E.emu, E.efdd, E.dip, E.cg_minv, E.cg_r,
-                       E.cg_z, E.cg_p, E.cg_tmu, E.cgblk);
-    if (!sp) hipLaunchKernelGGL(k_cg_smu_init, dim3(ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, E.S, E.cg_smu);
-    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s, P, nbw, E.cgblk, E.state);
+                       E.cg_u, E.cg_m, E.cg_z, E.cg_q, E.cg_s, E.cg_p, E.cg_tmu);
+    if (!sp) {
+        hipLaunchKernelGGL(k_cg_smu_init, dim3(ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, E.S, E.cg_smu);
+        el_cg_apply_ksum(E, P, false, 0, s);
+    } else {
+        el_cg_apply_other(E, P, *sp, false, 0, s);
+    }
+    el_cg_update(E, P, sp, true, s);
     return (int)hipGetLastError();
 }
 
-// Iteration `it` (0-based): T p (real-space rows, structure factors, reciprocal field),
-// q = A p, the update of mu, r, z with alpha, then the new direction with beta and the
-// stopping test.
+// Iteration `it` (0-based): the stopping test and scalars from the last update's partials,
+// n = A m (real-space rows, structure factors, reciprocal field), then the vector update.
+// With the particle mesh or no reciprocal space the structure factors are not carried.
 int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, const ElecSpme* sp, hipStream_t s)
 {
-    const double* p = E.cg_p;
-    const int nbq = cg_nbq(P), nbu = ceil_div(P.nw, CG_WPB);
-    double* blkq = E.cgblk;
-    double* blku = E.cgblk + nbq;
-    hipLaunchKernelGGL(k_el_emu, dim3(P.nrow), dim3(64), 0, s, P, E.nbrP, E.cntP, E.tens, E.tin, p, E.emu, E.state);
-    if (sp) {
-        // particle mesh: the reciprocal field of p, one chunk; no structure factors to carry
-        // (no reciprocal space: the preparation's zero partials stand)
-        if (sp->B) el_spme_dipole_field(E, *sp, P.nrow, p, E.state, s);
-        const ElecDev Q = one_chunk(P);
-        hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, Q, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
-        hipLaunchKernelGGL(k_cg_update, dim3(nbu), dim3(256), 0, s, Q, it, nbq, blkq, E.cg_p, E.cg_q, E.cg_minv, E.dip,
-                           E.cg_r, E.cg_z, E.emu, E.efdd, E.cg_tmu, E.S, E.cg_smu, blku, E.state);
-        hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, it, nbu, blku, E.cg_z,
-                           E.cg_p, E.state);
-        return (int)hipGetLastError();
-    }
-    hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P, E.dspc,
-                       E.runs, E.runk, E.phc, E.phs, E.chge, p, E.Spart, E.state);
-    hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S,
-                       E.Sq, E.kc, E.state);
-    hipLaunchKernelGGL(k_el_kfield<false>, dim3(ceil_div(P.nrow, 128), P.fchunks), dim3(64 * EL_KG), kfield_lds(P), s, P,
-                       E.runs, E.runk, E.phc, E.phs, E.kc, E.Fpart, E.state);
-    hipLaunchKernelGGL(k_cg_apply, dim3(nbq), dim3(256), 0, s, P, E.Fpart, p, E.emu, E.efdd, E.cg_q, blkq, E.state);
-    hipLaunchKernelGGL(k_cg_update, dim3(nbu + ceil_div(6L * P.K, 256)), dim3(256), 0, s, P, it, nbq, blkq, E.cg_p,
-                       E.cg_q, E.cg_minv, E.dip,
-                       E.cg_r, E.cg_z, E.emu, E.efdd, E.cg_tmu, E.S, E.cg_smu, blku, E.state);
-    hipLaunchKernelGGL(k_cg_direction, dim3(ceil_div(3L * P.nrow, 256)), dim3(256), 0, s, P, it, nbu, blku, E.cg_z,
-                       E.cg_p, E.state);
+    if (sp) el_cg_apply_other(E, P, *sp, true, it, s);
+    else el_cg_apply_ksum(E, P, true, it, s);
+    el_cg_update(E, P, sp, false, s);
     return (int)hipGetLastError();
 }
 
@@ -2291,7 +2377,7 @@ int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEv
     (void)hipEventRecord(a, s);
     for (int r = 0; r < reps; ++r)
         hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
-                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.zero_state);
+                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.zero_state, nullptr, 0, 0);
     (void)hipEventRecord(b, s);
     return (int)hipGetLastError();
 }
@@ -2344,7 +2430,7 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     } else if (!refresh) {
         // structure factors of the converged dipoles (already computed with the refreshed fields)
         hipLaunchKernelGGL(k_el_sfac<false>, dim3(ceil_div((long)EL_SPLIT * P.nrun, 64), E.dchunks), dim3(64 * EL_SG), sfac_lds(P), s, P,
-                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, (const double*)E.zero_state);
+                           E.dspc, E.runs, E.runk, E.phc, E.phs, E.chge, E.dip, E.Spart, E.zero_state, nullptr, 0, 0);
         hipLaunchKernelGGL(k_el_sreduce<false>, dim3(ceil_div(6L * P.K, 60)), dim3(256), 0, s, P, E.dchunks, E.Spart, E.kv, E.S, E.Sq, E.kc,
                            (const double*)E.zero_state);
     }

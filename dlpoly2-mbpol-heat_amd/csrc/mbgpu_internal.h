@@ -155,8 +155,10 @@ struct ElecBuffers {
     double *dip = nullptr, *efk = nullptr, *emu = nullptr, *efdc = nullptr, *efdd = nullptr;  // [3n]
     double* dip0 = nullptr;                                    // [3n] start dipoles of the evaluation
     double *cg_p = nullptr, *cg_q = nullptr, *cg_r = nullptr, *cg_z = nullptr;                // [3n] solver vectors
+    double *cg_u = nullptr, *cg_w = nullptr, *cg_m = nullptr, *cg_s = nullptr;                // [3n]
     double* cg_minv = nullptr;                                 // [nw][45] inverse preconditioner blocks
     double *cg_tmu = nullptr, *cg_smu = nullptr;               // [3n] T mu, [K][6] S(mu) carried by the solver
+    double *cg_sp = nullptr, *cg_su = nullptr, *cg_sq = nullptr;   // [K][6] S(p), S(u), S(q) carried by the solver
     double* cgblk = nullptr;                                   // block partials of the solver kernels
     double* ksum = nullptr;                                    // [16][n] chunk sums of the final pass
     double* state = nullptr;                                   // [16] solver scalars
@@ -322,6 +324,11 @@ struct SpmeBuffers {
 struct SpSpec {
     int ngr;
     signed char q_on[SP_MAX_GRIDS], mu_on[SP_MAX_GRIDS], comp[SP_MAX_GRIDS];
+    // > 0: the dipoles are spread times state[scale_state] (a power of two, so exactly; 0 or
+    // less there means 1), which the field pass divides out again -- for dipole-only spreads
+    // of small vectors, whose contributions would otherwise fall below the fixed point's
+    // resolution
+    int scale_state;
 };
 // Geometry, tables, transform plans and buffers for n sites and ngr grids; 0 or -1 with err.
 int spme_plan(SpmeBuffers& B, SpmeGeom& G, const double cell[9], int order, const int grid[3], double alpha, int n,
@@ -353,7 +360,7 @@ void sp_launch_influence_stress(SpmeBuffers& B, const SpmeGeom& G, double2* src,
                                 hipStream_t s);
 // E = -grad phi of grid y at the first n sites into out[(3 y + c) * ostride + j] * fac
 void sp_launch_field(SpmeBuffers& B, const SpmeGeom& G, int n, const double2* grids, int ngr, double fac,
-                     double* out, long ostride, const double* state, hipStream_t s);
+                     double* out, long ostride, const double* state, hipStream_t s, int scale_state = 0);
 // every per-site reciprocal quantity of the polarizable electrostatics from the eleven
 // transformed grids of the final pass, as the k-sum site pass's partial sums (times kap):
 // ks[t * n + j] (t < 15) and per-block partials blk[8 * b + v] (see k_sp_elec_final)

@@ -395,6 +395,11 @@ __global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* 
     }
     __syncthreads();
     const int ncand = boff[p];
+    double msc = 1.0;
+    if (spec.scale_state > 0 && state) {
+        const double v = state[spec.scale_state];
+        msc = v > 0.0 ? v : 1.0;
+    }
     bool big = false;
     for (int base = 0; base < ncand; base += 256 * CU) {
         if (tid == 0) cnt = 0;
@@ -450,7 +455,7 @@ __global__ __launch_bounds__(256) void k_sp_tile(SpmeGeom G, int n, const int4* 
             const double qi = q ? q[i2] : 0.0;
             double mv[3] = {0.0, 0.0, 0.0};
             if (mu)
-                for (int cc = 0; cc < 3; ++cc) mv[cc] = mu[3 * i2 + cc];
+                for (int cc = 0; cc < 3; ++cc) mv[cc] = mu[3 * i2 + cc] * msc;
             double wa[3];
             for (int a = 0; a < 3; ++a) wa[a] = G.K[a] * (G.rv[a][0] * mv[0] + G.rv[a][1] * mv[1] + G.rv[a][2] * mv[2]);
             for (int gr = 0; gr < ngr; ++gr) {
@@ -733,12 +738,17 @@ __global__ __launch_bounds__(256) void k_sp_gather(SpmeGeom G, int n, const int4
 __global__ __launch_bounds__(256) void k_sp_field(SpmeGeom G, int n, const int4* __restrict__ gi,
                                                   const double* __restrict__ th, const double2* __restrict__ grids,
                                                   long gs, double fac, double* __restrict__ out, long os,
-                                                  const double* __restrict__ state)
+                                                  const double* __restrict__ state, int scale_state)
 {
     if (state && state[2] != 0.0) return;
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
+    if (scale_state > 0 && state) {
+        // the spreading's power-of-two scale of the dipoles, divided out exactly
+        const double v = state[scale_state];
+        if (v > 0.0) fac /= v;
+    }
     const int4 gg = gi[i];
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     if (gg.w) sp_site_sums<4>(G, gg, th + (long)i * 9 * SPME_MAX_ORDER, grids + blockIdx.y * gs, s);
@@ -921,11 +931,11 @@ void sp_launch_influence_stress(SpmeBuffers& B, const SpmeGeom& G, double2* src,
 }
 
 void sp_launch_field(SpmeBuffers& B, const SpmeGeom& G, int n, const double2* grids, int ngr, double fac,
-                     double* out, long ostride, const double* state, hipStream_t s)
+                     double* out, long ostride, const double* state, hipStream_t s, int scale_state)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_sp_field, dim3((unsigned)((n + 3) / 4), (unsigned)ngr), dim3(256), 0, s, G, n, B.gi, B.th,
-                           grids, B.ng_cap, fac, out, ostride, state);
+                           grids, B.ng_cap, fac, out, ostride, state, scale_state);
 }
 
 void sp_launch_elec_final(SpmeBuffers& B, const SpmeGeom& G, int n, const double* q, const double* mu,
