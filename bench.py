@@ -158,6 +158,32 @@ def large_box_leg(pkg, torch, reps):
     return out
 
 
+def large_md_leg(pkg, args):
+    """The large-box workflow (md.large_box_run): 2048 waters, particle mesh order 6 on 48^3,
+    --large-md-relax FP64 relaxation steps, --large-md-steps mixed-precision NVE steps at
+    0.2 fs with J every step, an FP64 shadow evaluation every 100 steps, the J
+    autocorrelation on the device."""
+    md = importlib.import_module("dlpoly2-mbpol-heat_amd.md")
+    box = pkg.waterbox.box_2048()
+
+    def progress(stage, step, d):
+        print(f"large-box {stage} step {step}: " + ", ".join(f"{k} {v:.3g}" for k, v in d.items()
+                                                               if isinstance(v, float)), file=sys.stderr, flush=True)
+
+    r = md.large_box_run(box, relax_steps=args.large_md_relax, nve_steps=args.large_md_steps, shadow_every=100,
+                         progress=progress)
+    keep = ("waters", "relax_steps", "nve_steps", "shadow_every", "dt_fs", "recip", "relax_ms_per_step",
+            "relax_temperature_last", "nve_ms_per_step", "drift_per_ps_over_KE0", "max_abs_dE_over_KE0",
+            "dipole_iterations_median", "max_rms_dF_kcal_mol_A", "max_rel_dJ", "acf_ms")
+    out = {k: r[k] for k in keep}
+    out["acf_lags"] = int(len(r["acf"]))
+    out["shadow_last"] = r["shadow"][-1]
+    out["what"] = ("2048 H2O (13^3 lattice, L = 39.43 A), FP64 relaxation with Berendsen scaling to 300 K, then NVE "
+                   "with the 2B/3B polynomials in FP32 and the heat current every step; every 100 steps the same "
+                   "configuration evaluated in FP64 (forces, energy, J compared); J autocorrelation on the device")
+    return out
+
+
 def serve_leg(pkg, args):
     """Config-3 style serving mix through serve.Server; returns latency and throughput."""
     import numpy as np
@@ -233,6 +259,14 @@ def main():
                     help="run the 2048-water leg alone and print its record (for profiling)")
     ap.add_argument("--large-reps", type=int, default=5,
                     help="evaluations per reciprocal method of the 2048-water leg (0: skip)")
+    ap.add_argument("--large-md-steps", type=int, default=1000,
+                    help="mixed-precision NVE steps of the large-box workflow (2048 waters, mesh 48^3, order 6, "
+                         "FP64 shadow evaluation every 100 steps, J autocorrelation on the device); the full "
+                         "workflow is 10000; 0 skips it")
+    ap.add_argument("--large-md-relax", type=int, default=500,
+                    help="FP64 relaxation steps (Berendsen scaling to 300 K) before the large-box NVE run")
+    ap.add_argument("--only-large-md", action="store_true",
+                    help="run the large-box workflow alone and print its record")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-mix", choices=["beads", "mixed"], default="mixed",
                     help="beads: PIMD beads of 256 waters only (same-size slots); mixed: beads and isolated "
@@ -272,6 +306,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.only_large:
         print(json.dumps({"large_box": large_box_leg(pkg, torch, max(1, args.large_reps))}), flush=True)
+        return
+    if args.only_large_md:
+        print(json.dumps({"large_box_md": large_md_leg(pkg, args)}), flush=True)
         return
     box = pkg.waterbox.box_512()
     ctx = pkg.Context(box.nw, box.natms)
@@ -418,6 +455,7 @@ def main():
     if args.serve_requests > 0:
         serving = serve_leg(pkg, args)
     large_out = large_box_leg(pkg, torch, args.large_reps) if args.large_reps > 0 else None
+    large_md = large_md_leg(pkg, args) if args.large_md_steps > 0 else None
     # headline: the fastest leg whose forces are within 1e-5 kcal/mol/A RMS of FP64
     lead = ("fp64 single", single, ms, "f64")
     if "fp64" in batched and batched["fp64"]["evals_per_s"] > single:
@@ -437,6 +475,8 @@ def main():
         legs["pimd"] = pimd_out
     if large_out is not None:
         legs["large_box"] = large_out
+    if large_md is not None:
+        legs["large_box_md"] = large_md
     if "mixed" in batched:
         m = batched["mixed"]
         legs["mixed_vs_fp64"] = {"rms_dF_kcal_mol_A": m["rms_dF"], "max_dF_kcal_mol_A": m["max_dF"],

@@ -12,6 +12,11 @@ energy, J).
 (heatcurrent.f90:730-740): the time as es12.5, then J_kin, J_pot and J_tot, each as
 3es15.7, in internal units times junit = 1e3.
 
+`large_box_run` is the large-box workflow: a thermostatted FP64 relaxation, then an NVE
+run with the polynomials in FP32 and the heat current every step, its autocorrelation on
+the device, and every `shadow_every` steps an FP64 evaluation of the same configuration
+(the shadow) against which the run's forces, energy and J are compared.
+
 `green_kubo` turns a J series into the heat-current autocorrelation (on the device,
 mbgpu_heat_acf) and the running Green-Kubo conductivity
 kappa(t) = 1 / (3 V k_B T^2) * int_0^t <J(0) . J(s)> ds (trapezoid rule), the analysis the
@@ -166,3 +171,117 @@ def green_kubo(ctx, jseries, dt_ps, volume_A3, temperature_K, ntau):
     acf = ctx.heat_acf(jseries, ntau)
     return dict(time_ps=dt_ps * np.arange(ntau), acf=acf,
                 kappa_W_mK=kappa_from_acf(acf, dt_ps, volume_A3, temperature_K))
+
+
+def _copy_state(src, dst):
+    """Positions, velocities and per-atom kinetic energies of slot src into slot dst (same
+    size), device to device; both slots' streams are idle when this is called."""
+    t = src.torch
+    t.cuda.synchronize(src.dev)
+    for a, b in ((src.x, dst.x), (src.y, dst.y), (src.z, dst.z), (src.v, dst.v), (src.ek, dst.ek)):
+        b.copy_(a)
+    t.cuda.synchronize(src.dev)
+
+
+def large_box_run(box, relax_steps=500, nve_steps=10000, shadow_every=100, dt_fs=0.2, temperature=300.0,
+                  tau_fs=20.0, recip=("spme", 6, (48, 48, 48)), ntau=None, progress=None):
+    """The large-box workflow on one configuration (box: a waterbox.WaterBox).
+
+    1. relax_steps steps of FP64 MD at dt_fs with Berendsen velocity scaling towards
+       `temperature` (time constant tau_fs): lambda^2 = 1 + dt / tau (T0 / T - 1), applied
+       to the velocities after each step, T from the kinetic energy over 3 N_atoms - 3
+       degrees of freedom;
+    2. nve_steps steps of NVE from the relaxed state with the 2B and 3B polynomials in FP32
+       (the rest in FP64), the heat current every step;
+    3. every shadow_every steps (and at step 0) the same positions and velocities evaluated
+       in FP64 by a second slot: RMS and max force difference, energy difference per
+       molecule and relative |J - J_FP64|;
+    4. the autocorrelation of the run's J_tot for ntau lags on the device.
+    The reciprocal space is `recip` (engine.Slot's argument; None: the Ewald sum over k).
+    progress(stage, step, dict) is called every shadow_every steps.  Returns a dict of the
+    series and summaries (internal units: energy 10 J/mol, J in (10 J/mol) A/ps)."""
+    import time as _time
+    from . import engine
+    dt_ps = dt_fs * 1.0e-3
+    nr = box.nreal
+    ndf = 3 * nr - 3
+    kb = 0.831446261815324          # Boltzmann's constant in 10 J/mol per K
+    out = dict(waters=box.nw, relax_steps=relax_steps, nve_steps=nve_steps, shadow_every=shadow_every,
+               dt_fs=dt_fs, recip=None if recip is None else list(recip))
+    # 1. relaxation (FP64)
+    fp64 = engine.Slot(box, precision=0, recip=recip)
+    md = NVE(fp64, dt_fs)
+    temps = []
+    t0 = _time.perf_counter()
+    for k in range(relax_steps):
+        r = md.step()
+        temp = 2.0 * r["kin"] / (ndf * kb)
+        lam = np.sqrt(max(0.0, 1.0 + dt_fs / tau_fs * (temperature / temp - 1.0)))
+        fp64.torch.cuda.synchronize(fp64.dev)
+        fp64.v.mul_(lam)
+        fp64.ek.mul_(lam * lam)
+        fp64.torch.cuda.synchronize(fp64.dev)
+        temps.append(temp)
+        if progress and (k + 1) % shadow_every == 0:
+            progress("relax", k + 1, dict(temperature=temp, pot=r["pot"]))
+    out["relax_ms_per_step"] = 1e3 * (_time.perf_counter() - t0) / max(1, relax_steps)
+    out["relax_temperature_last"] = temps[-1] if temps else None
+    # 2.-3. mixed-precision NVE with the FP64 shadow
+    mixed = engine.Slot(box, precision=3, recip=recip)
+    _copy_state(fp64, mixed)
+    shadow = fp64
+    nve = NVE(mixed, dt_fs)
+    e0, k0 = nve.pot + nve.kin, nve.kin
+    etot, jser, checks, iters = [], [], [], []
+
+    def shadow_check(step, jmix):
+        _copy_state(mixed, shadow)
+        rs = shadow.evaluate()
+        fm, fs = mixed.forces(), shadow.forces()
+        d = fm - fs
+        checks.append(dict(step=step, rms_dF_kcal_mol_A=float(np.sqrt((d ** 2).mean())) / 418.4,
+                           max_dF_kcal_mol_A=float(np.abs(d).max()) / 418.4,
+                           dE_kcal_mol_per_molecule=(nve.pot - rs["eng"]) / 418.4 / box.nw,
+                           rel_dJ=float(np.linalg.norm(jmix - rs["jtot"]) / np.linalg.norm(rs["jtot"]))))
+
+    shadow_check(0, _current_j(nve))
+    t0 = _time.perf_counter()
+    for k in range(nve_steps):
+        r = nve.step()
+        etot.append(r["tot"])
+        jser.append(r["jtot"])
+        iters.append(r["iterations"])
+        if (k + 1) % shadow_every == 0:
+            t1 = _time.perf_counter()
+            shadow_check(k + 1, r["jtot"])
+            t0 += _time.perf_counter() - t1          # the shadow is not part of the run's time
+            if progress:
+                progress("nve", k + 1, checks[-1])
+    out["nve_ms_per_step"] = 1e3 * (_time.perf_counter() - t0) / max(1, nve_steps)
+    etot = np.array(etot)
+    tt = dt_ps * np.arange(1, len(etot) + 1)
+    slope = float(np.polyfit(tt, etot, 1)[0]) if len(etot) > 1 else 0.0
+    out.update(drift_per_ps_over_KE0=slope / k0, max_abs_dE_over_KE0=float(np.abs(etot - e0).max() / k0),
+               dipole_iterations_median=float(np.median(iters)) if iters else None, shadow=checks,
+               max_rms_dF_kcal_mol_A=max(c["rms_dF_kcal_mol_A"] for c in checks),
+               max_rel_dJ=max(c["rel_dJ"] for c in checks))
+    jser = np.array(jser)
+    if ntau is None:
+        ntau = max(1, min(1000, len(jser) // 2))
+    t1 = _time.perf_counter()
+    acf = mixed.ctx.heat_acf(jser, ntau) if len(jser) else np.zeros(0)
+    out["acf_ms"] = 1e3 * (_time.perf_counter() - t1)
+    out["acf"] = acf
+    out["jtot"] = jser
+    out["etot"] = etot
+    mixed.close()
+    fp64.close()
+    return out
+
+
+def _current_j(nve):
+    """J of the NVE run's current state (the library's heat flux of the last evaluation)."""
+    s = nve.slot
+    v = s.v
+    _, _, jt = s.ctx.heat_flux_dev(v[0].data_ptr(), v[1].data_ptr(), v[2].data_ptr(), s.ek.data_ptr())
+    return np.array(jt)

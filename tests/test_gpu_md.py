@@ -7,6 +7,10 @@
   heatflux2 layout.
 * The heat-current autocorrelation on the device against numpy's direct sum (1e-12), host
   and device-pointer entries bit-identical, and the argument checks.
+* The large-box workflow (md.large_box_run) shortened on its own system, 2048 waters with
+  the particle mesh: the thermostat moves the temperature towards 300 K, the FP64 shadow
+  of the FP32-polynomial run differs by the FP32 polynomials' error (not more), and the
+  J autocorrelation on the device equals numpy's direct sum over the run's J series.
 """
 import importlib
 
@@ -105,3 +109,29 @@ def test_heat_current_autocorrelation(pkg):
     assert np.array_equal(ad.cpu().numpy(), got)
     with pytest.raises(pkg.MbgpuError, match="ntau"):
         ctx.heat_acf(j[:10], 11)
+
+
+def test_large_box_workflow_short(pkg, mods):
+    engine, md, wb = mods
+    box = wb.box_2048()
+    print("large-box workflow: 2048 waters, 40 relaxation + 200 mixed NVE steps, shadow every 100", flush=True)
+    # tau = dt: the velocities are rescaled to 300 K after every relaxation step, so the
+    # temperature seen after a step is one step's change away from 300 K
+    r = md.large_box_run(box, relax_steps=40, nve_steps=200, shadow_every=100, ntau=50, tau_fs=0.2)
+    print({k: r[k] for k in ("relax_temperature_last", "nve_ms_per_step", "max_abs_dE_over_KE0",
+                             "max_rms_dF_kcal_mol_A", "max_rel_dJ")}, flush=True)
+    assert 280.0 < r["relax_temperature_last"] < 330.0
+    # shadows at steps 0, 100, 200: the mixed run's forces, energy and J against FP64 at the
+    # same configuration differ by the FP32 polynomials' error (RMS 9e-3 kcal/mol/A at 512
+    # waters) and nothing else
+    assert [c["step"] for c in r["shadow"]] == [0, 100, 200]
+    for c in r["shadow"]:
+        assert 1e-4 < c["rms_dF_kcal_mol_A"] < 0.05
+        assert abs(c["dE_kcal_mol_per_molecule"]) < 0.05
+        assert c["rel_dJ"] < 1e-3
+    assert np.all(np.isfinite(r["etot"])) and r["max_abs_dE_over_KE0"] < 0.05
+    # the device autocorrelation of the run's own J series
+    j = r["jtot"]
+    assert j.shape == (200, 3)
+    ref = np.array([np.mean(np.sum(j[: len(j) - k] * j[k:], axis=1)) for k in range(50)])
+    assert np.abs(r["acf"] - ref).max() <= 1e-12 * np.abs(ref).max()
