@@ -413,6 +413,7 @@ int alloc_direct(ElecState& S, std::string& err)
 
 void elec_release(ElecState& S)
 {
+    elec_drop_graph(S);
     free_buffers(S.E);
     dfree(S.rcell);
     if (S.sp) {
@@ -534,7 +535,73 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
     return 0;
 }
 
+namespace {
+
+// Enqueue one attempt of an evaluation up to `nbatch` solver iterations: the start dipoles
+// kept in dip0 (first attempt) or restored from it (a repeat after the rows grew), the
+// rows, the preparation and the solver's start.  Kernels only: a graph may capture it.
+int enqueue_attempt(ElecState& S, const ElecSpme* sp, int solver, bool first, int nbatch, hipStream_t s)
+{
+    ElecBuffers& E = S.E;
+    const ElecDev& P = S.P;
+    if (first) el_launch_copy(E.dip0, E.dip, 3L * P.n, s);
+    else el_launch_copy(E.dip, E.dip0, 3L * P.n, s);
+    el_launch_zero(reinterpret_cast<double*>(E.err), 2, s);   // the four int flags
+    int rc = el_launch_setup(E, P, S.cells, s);
+    if (rc) return rc;
+    el_launch_zero(E.state, 16, s);
+    el_launch_zero(E.emu, 3L * P.n, s);
+    // M sites carry no induced dipole (polr2 = 0): clear any start value
+    el_launch_zero(E.dip + 3 * P.nreal, 3L * (P.n - P.nreal), s);
+    rc = el_launch_prep(E, P, S.rcell, sp, s);
+    if (!rc && solver == 1) rc = el_launch_cg_init(E, P, sp, s);
+    for (int b = 0; !rc && b < nbatch; ++b)
+        rc = solver == 1 ? el_launch_cg_iteration(E, P, b, sp, s) : el_launch_iteration(E, P, b == 0, sp, s);
+    return rc;
+}
+
+// The key of a captured evaluation: the bytes of everything its launches were recorded
+// with (kernel-argument blocks, buffer addresses and sizes, the mesh and link-cell plans)
+// and the run's own choices.
+std::vector<unsigned char> graph_key(const ElecState& S, int distribute, int solver, int nbatch)
+{
+    std::vector<unsigned char> k;
+    auto add = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), c, c + n);
+    };
+    add(&S.P, sizeof S.P);
+    add(&S.E, sizeof S.E);
+    add(&S.cells, sizeof S.cells);
+    add(&S.rcell, sizeof S.rcell);
+    const int v[5] = {distribute, solver, nbatch, S.recip, S.P.g.imcon};
+    add(v, sizeof v);
+    if (S.recip && S.sp && S.spg) {
+        add(S.sp, sizeof(SpmeBuffers));
+        add(S.spg, sizeof(SpmeGeom));
+    }
+    return k;
+}
+
+void drop_graph(ElecState& S)
+{
+    if (S.gexec) (void)hipGraphExecDestroy(S.gexec);
+    S.gexec = nullptr;
+    S.gkey.clear();
+}
+
+}  // namespace
+
+void elec_drop_graph(ElecState& S) { drop_graph(S); }
+
 // One evaluation with positions (real atoms) in E.x/y/z and start dipoles in E.dip.
+//
+// Once an evaluation has set the iteration count, the next ones run as one captured graph:
+// the start, last count + 2 solver iterations (a converged state turns the rest into no-ops)
+// and the final pass, then one read-back of the solver state, the row-overflow flag and the
+// scalars.  A graph that ends unconverged continues with iterations launched one by one and
+// runs the final pass again (it rewrites every output); a row overflow falls back to the
+// repeat below.  The graph is captured again whenever its key changes.
 int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStream_t s, std::string& err)
 {
     ElecBuffers& E = S.E;
@@ -545,53 +612,95 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     ElecSpme spx{S.sp, S.spg, S.spg ? S.spg->vol / (8.0 * PI_D) : 0.0};
     ElecSpme none{nullptr, nullptr, 0.0};
     const ElecSpme* sp = P.g.imcon == 0 ? &none : S.recip ? &spx : nullptr;
+    const double g34 = exp(gammln_host(0.75)), gln34 = gammln_host(0.75);
+    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int herr = 0;
+    bool need_final = true;
+    // Iterations in the graph: the last count + 2, kept while it covers the solve and is not
+    // far too long (iterations past convergence are no-ops of about a microsecond per kernel;
+    // a new capture costs more), so counts that drift by one or two do not capture again.
+    // MBGPU_GRAPH_BATCH=k caps it (a test aid for the continuation after a graph).
+    int gb = 0;
+    if (S.last_iters > 0) {
+        gb = std::min(S.last_iters + 2, 64);
+        if (S.gexec && S.gbatch >= S.last_iters && S.gbatch <= S.last_iters + 8) gb = S.gbatch;
+    }
+    if (const char* ev = getenv("MBGPU_GRAPH_BATCH")) gb = gb ? std::min(gb, std::max(1, atoi(ev))) : 0;
+    bool from_graph = false;
+    // MBGPU_GRAPHS=0 launches kernel by kernel whatever the context's setting (a test aid)
+    const char* gv = getenv("MBGPU_GRAPHS");
+    const bool graphs = S.graphs && !(gv && atoi(gv) == 0);
+    if (graphs && gb > 0) {
+        std::vector<unsigned char> key = graph_key(S, distribute, solver, gb);
+        if (!S.gexec || key != S.gkey) {
+            drop_graph(S);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            int rc = 0;
+            if (e == hipSuccess) {
+                rc = enqueue_attempt(S, sp, solver, true, gb, s);
+                if (!rc) rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, sp, s);
+                e = hipStreamEndCapture(s, &graph);
+            }
+            if (e == hipSuccess && !rc && graph) e = hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e != hipSuccess || rc || !S.gexec) {
+                // no graph for this state: launch kernel by kernel from now on
+                (void)hipGetLastError();
+                drop_graph(S);
+                S.graphs = 0;
+            } else {
+                S.gkey.swap(key);
+                S.gbatch = gb;
+                ++S.gcaptures;
+            }
+        }
+        if (S.gexec) {
+            ECHK(hipGraphLaunch(S.gexec, s));
+            ++S.greplays;
+            ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
+            ECHK(hipMemcpyAsync(&herr, E.err, sizeof(int), hipMemcpyDeviceToHost, s));
+            ECHK(hipStreamSynchronize(s));
+            from_graph = !herr;
+            need_final = !(st[2] != 0.0);
+        }
+    }
     // The neighbour rows' capacity check is read back with the solver's first convergence
     // check instead of after the row build (one host round trip less): the rows are built
     // with clamped counts, so an overflowed attempt runs in bounds and is simply repeated
     // with larger rows, from the same start dipoles (kept in dip0).
-    ECHK(hipMemcpyAsync(E.dip0, E.dip, sizeof(double) * 3 * P.n, hipMemcpyDeviceToDevice, s));
-    double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int attempt = 0;; ++attempt) {
-        if (attempt) ECHK(hipMemcpyAsync(E.dip, E.dip0, sizeof(double) * 3 * P.n, hipMemcpyDeviceToDevice, s));
-        ECHK(hipMemsetAsync(E.err, 0, 4 * sizeof(int), s));
-        int rc = el_launch_setup(E, P, S.cells, s);
-        if (rc) {
-            err = std::string("electrostatics set-up kernels failed: ") + hipGetErrorString((hipError_t)rc);
-            return rc;
-        }
-        ECHK(hipMemsetAsync(E.state, 0, 16 * sizeof(double), s));
-        ECHK(hipMemsetAsync(E.emu, 0, sizeof(double) * 3 * P.n, s));
-        // M sites carry no induced dipole (polr2 = 0): clear any start value
-        ECHK(hipMemsetAsync(E.dip + 3 * P.nreal, 0, sizeof(double) * 3 * (P.n - P.nreal), s));
-        rc = el_launch_prep(E, P, S.rcell, sp, s);
-        if (rc) {
-            err = std::string("electrostatics preparation kernels failed: ") + hipGetErrorString((hipError_t)rc);
-            return rc;
-        }
-        // solver loop in batches; a converged state turns the rest of a batch into no-ops.
-        // The first batch covers the last evaluation's count and two more (one host round
-        // trip in steady state; an iteration past convergence costs a few microseconds of
-        // no-op launches, a second round trip tens)
+    for (int attempt = from_graph ? -1 : (S.gexec && herr ? 1 : 0); attempt >= 0 || from_graph;) {
         int launched = 0, batch = S.last_iters > 0 ? std::min(S.last_iters + 2, 64) : (solver == 1 ? 8 : 12);
-        if (solver == 1) {
-            rc = el_launch_cg_init(E, P, sp, s);
+        int rc = 0;
+        if (from_graph) {
+            // continue after the graph's iterations
+            launched = S.gbatch;
+            batch = solver == 1 ? 4 : 8;
+            from_graph = false;
+            attempt = -1;
+            if (st[2] != 0.0) break;
+        } else {
+            if (attempt > 0) {
+                // the rows overflowed: grow them and repeat from the start dipoles
+                const int full = ((P.n + 63) / 64) * 64;
+                if (attempt > 2 || E.capA >= full) {
+                    err = "electrostatics neighbour rows exceed their capacity";
+                    return -2;
+                }
+                if (alloc_rows(S, std::min(full, 2 * E.capA), std::min(full, 2 * E.capP), err)) return -1;
+                P.capA = E.capA;
+                P.capP = E.capP;
+            }
+            rc = enqueue_attempt(S, sp, solver, attempt == 0, batch, s);
             if (rc) {
-                err = std::string("electrostatics solver kernels failed: ") + hipGetErrorString((hipError_t)rc);
+                err = std::string("electrostatics kernels failed: ") + hipGetErrorString((hipError_t)rc);
                 return rc;
             }
+            launched = batch;
+            batch = solver == 1 ? 4 : 8;
         }
         bool overflow = false;
         for (;;) {
-            for (int b = 0; b < batch; ++b) {
-                rc = solver == 1 ? el_launch_cg_iteration(E, P, launched, sp, s)
-                                 : el_launch_iteration(E, P, launched == 0, sp, s);
-                ++launched;
-                if (rc) {
-                    err = std::string("electrostatics iteration kernels failed: ") + hipGetErrorString((hipError_t)rc);
-                    return rc;
-                }
-            }
-            int herr = 0;
             ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
             ECHK(hipMemcpyAsync(&herr, E.err, sizeof(int), hipMemcpyDeviceToHost, s));
             ECHK(hipStreamSynchronize(s));
@@ -599,10 +708,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
                 overflow = true;
                 break;
             }
-            if (st[2] != 0.0) {
-                S.last_iters = (int)st[0];
-                break;
-            }
+            if (st[2] != 0.0) break;
             if (!(st[1] == st[1])) {
                 err = "induced dipoles became NaN";
                 return -3;
@@ -614,24 +720,28 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
                 err = buf;
                 return -3;
             }
-            batch = solver == 1 ? 4 : 8;
+            for (int b = 0; b < batch; ++b) {
+                rc = solver == 1 ? el_launch_cg_iteration(E, P, launched, sp, s)
+                                 : el_launch_iteration(E, P, launched == 0, sp, s);
+                ++launched;
+                if (rc) {
+                    err = std::string("electrostatics iteration kernels failed: ") +
+                          hipGetErrorString((hipError_t)rc);
+                    return rc;
+                }
+            }
         }
+        need_final = true;
         if (!overflow) break;
-        const int full = ((P.n + 63) / 64) * 64;
-        if (attempt >= 2 || E.capA >= full) {
-            err = "electrostatics neighbour rows exceed their capacity";
-            return -2;
-        }
-        if (alloc_rows(S, std::min(full, 2 * E.capA), std::min(full, 2 * E.capP), err)) return -1;
-        P.capA = E.capA;
-        P.capP = E.capP;
+        attempt = attempt < 0 ? 1 : attempt + 1;
     }
-    int rc = 0;
-    const double g34 = exp(gammln_host(0.75)), gln34 = gammln_host(0.75);
-    rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, sp, s);
-    if (rc) {
-        err = std::string("electrostatics final kernels failed: ") + hipGetErrorString((hipError_t)rc);
-        return rc;
+    S.last_iters = (int)st[0];
+    if (need_final) {
+        const int rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, sp, s);
+        if (rc) {
+            err = std::string("electrostatics final kernels failed: ") + hipGetErrorString((hipError_t)rc);
+            return rc;
+        }
     }
     ECHK(hipMemcpyAsync(S.scal, E.scal, sizeof(double) * 41, hipMemcpyDeviceToHost, s));
     int spflag[2] = {0, 0};

@@ -2490,3 +2490,16 @@ void el_launch_add(double* dst, const double* src, long n, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_el_add, dim3(ceil_div(n, 256)), dim3(256), 0, s, dst, src, n);
 }
+
+__global__ void k_el_copy(double* __restrict__ dst, const double* __restrict__ src, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// dst[i] = src[i], i < n (a kernel rather than a copy call, so that a captured graph holds
+// only kernel nodes)
+void el_launch_copy(double* dst, const double* src, long n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_el_copy, dim3(ceil_div(n, 256)), dim3(256), 0, s, dst, src, n);
+}

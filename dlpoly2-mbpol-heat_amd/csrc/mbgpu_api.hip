@@ -460,7 +460,12 @@ mbgpu_ctx* mbgpu_create(int nw, int natms)
     c->B.nw = nw;
     c->B.natms = natms;
     (void)hipGetDevice(&c->device);
-    if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->el_stream) != hipSuccess || alloc_fixed(c) ||
+    // the electrostatics' stream does not synchronize with the null stream: it is captured
+    // into graphs (mbgpu_set_graphs), and another thread's null-stream work during a capture
+    // would otherwise invalidate it; nothing relies on that implicit ordering, since every
+    // entry waits for its streams before it returns and takes inputs the caller has finished
+    if (hipStreamCreate(&c->stream) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->el_stream, hipStreamNonBlocking) != hipSuccess || alloc_fixed(c) ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
         std::string msg = g_err;
@@ -527,6 +532,23 @@ int mbgpu_set_elec_reciprocal(mbgpu_ctx* c, int mode, int order, const int grid[
     c->el.sp_order = order;
     for (int a = 0; a < 3; ++a) c->el.sp_grid[a] = grid[a];
     c->el.ready = false;
+    return 0;
+}
+
+int mbgpu_set_graphs(mbgpu_ctx* c, int on)
+{
+    if (!c) return fail(-1, "null context");
+    if (on != 0 && on != 1) return fail(-1, "graphs: 0 = launch kernel by kernel, 1 = capture and replay (got %d)", on);
+    c->el.graphs = on;
+    if (!on) elec_drop_graph(c->el);
+    return 0;
+}
+
+int mbgpu_get_graph_counts(mbgpu_ctx* c, long counts[2])
+{
+    if (!c || !counts) return fail(-1, "null context or output");
+    counts[0] = c->el.gcaptures;
+    counts[1] = c->el.greplays;
     return 0;
 }
 

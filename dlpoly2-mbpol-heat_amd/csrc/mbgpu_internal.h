@@ -5,6 +5,7 @@
 #include <stdlib.h>
 
 #include <string>
+#include <vector>
 #include <type_traits>
 
 #include "../../include/mbgpu.h"
@@ -203,6 +204,7 @@ int el_launch_cg_iteration(ElecBuffers& E, const ElecDev& P, int it, const ElecS
 int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEvent_t b, hipStream_t s);
 void el_launch_zero(double* p, long n, hipStream_t s);
 void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
+void el_launch_copy(double* dst, const double* src, long n, hipStream_t s);
 
 // Host-side state of the electrostatics of one context.
 struct ElecState {
@@ -224,12 +226,20 @@ struct ElecState {
     double scal[48];                    // host copy of the reduced scalars
     double out[20];                     // eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe, vir_cpe,
                                         // vir_ew1p, vir_qd, iterations, rms change, stress[9], pairs
+    // One evaluation's device work captured as a graph and replayed while its key (the
+    // launch parameters and every buffer address) is unchanged: elec_evaluate.
+    int graphs = 1;                     // 0: always launch kernel by kernel
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned char> gkey;
+    int gbatch = 0;                     // solver iterations in the graph
+    long gcaptures = 0, greplays = 0;   // graphs captured, replays launched
 };
 
 int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, const mbgpu_elec_params* p,
                    std::string& err);
 int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStream_t s, std::string& err);
 void elec_release(ElecState& S);
+void elec_drop_graph(ElecState& S);
 
 // ------------------------------------------------------------- monomer and dispersion ----
 

@@ -44,6 +44,8 @@ _SIGS = {
     "mbgpu_get_decomposition": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "mbgpu_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mbgpu_set_elec_reciprocal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "mbgpu_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "mbgpu_get_graph_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "mbgpu_get_lists": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _ip,
                                        ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "mbgpu_heat_flux": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -298,6 +300,17 @@ class Context:
         m = {"ewald": 0, "spme": 1}.get(mode, mode)
         g = (ctypes.c_int * 3)(*[int(k) for k in grid])
         _check(load_library().mbgpu_set_elec_reciprocal(self._h, int(m), int(order), g), "mbgpu_set_elec_reciprocal")
+
+    def set_graphs(self, on):
+        """Graph replay of the electrostatics: True (default) = capture one evaluation's kernels
+        once the iteration count is known and replay them; False = launch kernel by kernel."""
+        _check(load_library().mbgpu_set_graphs(self._h, 1 if on else 0), "mbgpu_set_graphs")
+
+    def graph_counts(self):
+        """(graphs captured, replays launched) by this context's electrostatics."""
+        c = (ctypes.c_long * 2)()
+        _check(load_library().mbgpu_get_graph_counts(self._h, c), "mbgpu_get_graph_counts")
+        return int(c[0]), int(c[1])
 
     def decomposition(self):
         ae = np.zeros(self.natms)

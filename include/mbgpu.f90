@@ -134,6 +134,18 @@ module mbgpu
       integer(c_int), intent(in) :: grid(3)
       integer(c_int) :: mbgpu_set_elec_reciprocal
     end function
+    function mbgpu_set_graphs(ctx, on) bind(c, name="mbgpu_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+      integer(c_int) :: mbgpu_set_graphs
+    end function
+    function mbgpu_get_graph_counts(ctx, counts) bind(c, name="mbgpu_get_graph_counts")
+      import :: c_int, c_long, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_long), intent(out) :: counts(2)
+      integer(c_int) :: mbgpu_get_graph_counts
+    end function
     function mbgpu_get_lists(ctx, dimers, dcap, nd, trimers, tcap, nt) bind(c, name="mbgpu_get_lists")
       import :: c_int, c_long, c_ptr
       type(c_ptr), value :: ctx, dimers, trimers

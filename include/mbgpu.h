@@ -80,6 +80,9 @@ int mbgpu_mbpol_forces(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm
                        double* fzz, double* engsrp, double* virsrp, double stress[9]);
 
 /* Same with device pointers (inputs resident in HBM; f{x,y,z} device arrays or NULL).
+ * Every _dev entry reads and writes its device arrays on the context's own streams and
+ * waits for them before it returns; the caller finishes writing the inputs first (its
+ * stream synchronized), since those streams are not ordered after other streams' work.
  * The device list is checked on the device before any atom array is read with it; a bad
  * entry returns -5 and no forces are added. */
 int mbgpu_mbpol_forces_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm2,
@@ -209,6 +212,16 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* ctx, int imcon, const double cell[
  * only; this carries the dipoles through the spread density.  Takes effect at the next
  * electrostatics call.  No counterpart in the reference for the polarizable path. */
 int mbgpu_set_elec_reciprocal(mbgpu_ctx* ctx, int mode, int order, const int grid[3]);
+
+/* Graph replay of the electrostatics in this context: 1 (default) = once an evaluation has
+ * set the solver's iteration count, later evaluations run as one captured HIP graph (the
+ * start, that count + 2 solver iterations, the final pass) with one read-back, captured
+ * again whenever the parameters, the cell or a buffer changes; results are bit-identical
+ * to launching kernel by kernel.  0 = launch kernel by kernel.  No counterpart in the
+ * reference (callers that evaluate configurations of one size many times: serving, MD). */
+int mbgpu_set_graphs(mbgpu_ctx* ctx, int on);
+/* counts[0]: graphs the electrostatics captured in this context, counts[1]: replays. */
+int mbgpu_get_graph_counts(mbgpu_ctx* ctx, long counts[2]);
 
 /* Results of the last electrostatics call (host arrays, any may be NULL):
  *   scal[20]: eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe, vir_cpe, vir_ew1p, vir_qd,

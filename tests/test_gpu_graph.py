@@ -1,0 +1,123 @@
+"""Graph replay of the electrostatics (mbgpu_set_graphs) against launching kernel by kernel.
+
+Once an evaluation has set the solver's iteration count, a context captures the next
+evaluation's electrostatics (the start, count + 2 iterations, the final pass) as one HIP
+graph and replays it while the parameters, the cell and the buffers stay the same.  Every
+replay must give exactly what launching the same kernels one by one gives:
+
+* several configurations in a row through engine.Slot (the electrostatics on their own
+  stream beside MB-pol and the water terms), with the program's own eager work and
+  synchronizations between the replays: energies, forces, J, dipole iterations and the
+  per-site electrostatics arrays bit-identical, for the Ewald sum over k, the particle mesh
+  and an isolated cluster, with both solvers;
+* a graph holding fewer iterations than the solve needs (MBGPU_GRAPH_BATCH): the solve
+  continues kernel by kernel and the final pass runs again, still bit-identical;
+* a change of parameters captures a new graph.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DETAILS = ("atomic_energy", "heat_w", "s_ew1p", "vesp", "chge", "f_sites")
+
+
+@pytest.fixture(scope="module")
+def mods(pkg):
+    m = lambda n: importlib.import_module(f"dlpoly2-mbpol-heat_amd.{n}")
+    return m("engine"), m("waterbox")
+
+
+def configurations(pkg, wb, kind):
+    if kind == "cluster":
+        base = wb.make_cluster(30, seed=8)
+        e, w = pkg.elec_params.for_cluster(base.L), pkg.water_params.for_cluster(base.L)
+    else:
+        base = wb.box_64()
+        e, w = pkg.elec_params.for_box(base.L), pkg.water_params.for_box(base.L)
+    boxes = [base] + [wb.displaced(base, 0.03, seed=70 + i) for i in range(4)]
+    if kind == "cluster":
+        for b in boxes:
+            b.imcon = 0
+    return boxes, e, w
+
+
+def same(ra, sa, rb, sb):
+    for k in ("eng", "vir", "eng_elec", "dipole_iterations"):
+        assert ra[k] == rb[k], k
+    for k in ("stress", "jtot", "jkin", "jpot"):
+        assert np.array_equal(ra[k], rb[k]), k
+    assert np.array_equal(sa.forces(), sb.forces())
+    da, db = sa.ctx.elec_details(), sb.ctx.elec_details()
+    for k in DETAILS:
+        assert np.array_equal(da[k], db[k]), k
+
+
+@pytest.mark.parametrize("kind", ["ksum", "mesh", "cluster"])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_replays_equal_kernel_by_kernel(pkg, mods, kind, solver):
+    import torch
+    engine, wb = mods
+    boxes, e, w = configurations(pkg, wb, kind)
+    recip = ("spme", 6, (32, 32, 32)) if kind == "mesh" else None
+    eager = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip)
+    eager.ctx.set_graphs(False)
+    graph = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip)
+    print(f"graph replays: {kind}, solver {solver}, {len(boxes)} configurations", flush=True)
+    scratch = torch.zeros(1 << 16, dtype=torch.float64, device="cuda")
+    for rep in range(2):
+        for b in boxes:
+            eager.load(b)
+            graph.load(b)
+            ra = eager.evaluate()
+            scratch.add_(1.0)                  # eager work of the program between the replays
+            torch.cuda.synchronize()
+            rb = graph.evaluate()
+            same(ra, eager, rb, graph)
+    caps, reps = graph.ctx.graph_counts()
+    assert eager.ctx.graph_counts() == (0, 0)
+    # the first evaluation ran kernel by kernel; the others replayed (a new iteration count
+    # captures again)
+    assert caps >= 1 and reps == 2 * len(boxes) - 1
+    print(f"{kind}, solver {solver}: {caps} capture(s), {reps} replays", flush=True)
+
+
+def test_short_graph_continues_kernel_by_kernel(pkg, mods, monkeypatch):
+    engine, wb = mods
+    boxes, e, w = configurations(pkg, wb, "ksum")
+    eager = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    eager.ctx.set_graphs(False)
+    monkeypatch.setenv("MBGPU_GRAPH_BATCH", "3")
+    graph = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    print("graph of 3 solver iterations, continued kernel by kernel", flush=True)
+    for b in boxes[:3]:
+        eager.load(b)
+        graph.load(b)
+        ra, rb = eager.evaluate(), graph.evaluate()
+        assert ra["dipole_iterations"] > 3
+        same(ra, eager, rb, graph)
+    assert graph.ctx.graph_counts()[1] == 2
+
+
+def test_new_parameters_capture_again(pkg, mods):
+    engine, wb = mods
+    boxes, e, w = configurations(pkg, wb, "ksum")
+    e2 = pkg.elec_params.ElecParams(rcut=5.5, alpha=0.6, kmax=(10, 10, 10))
+    eager = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    eager.ctx.set_graphs(False)
+    graph = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    print("graph replay across a change of electrostatics parameters", flush=True)
+    for params in (e, e2, e):
+        eager.set_params(params, w)
+        graph.set_params(params, w)
+        for _ in range(3):
+            ra, rb = eager.evaluate(), graph.evaluate()
+            same(ra, eager, rb, graph)
+    # three parameter sets: one capture each (more if a set's first solve outgrew the batch
+    # carried over from the previous set), every evaluation after the first a replay
+    caps, reps = graph.ctx.graph_counts()
+    assert 3 <= caps <= 6 and reps == 8
+    with pytest.raises(pkg.MbgpuError, match="graphs"):
+        pkg.mbgpu._check(pkg.mbgpu.load_library().mbgpu_set_graphs(graph.ctx._h, 2), "mbgpu_set_graphs")
