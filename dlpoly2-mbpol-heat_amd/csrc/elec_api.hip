@@ -635,6 +635,8 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         if (!S.gexec || key != S.gkey) {
             drop_graph(S);
             hipGraph_t graph = nullptr;
+            // no other entry runs while this thread captures (capture_gate_begin)
+            capture_gate_begin();
             hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             int rc = 0;
             if (e == hipSuccess) {
@@ -642,6 +644,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
                 if (!rc) rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, sp, s);
                 e = hipStreamEndCapture(s, &graph);
             }
+            capture_gate_end();
             if (e == hipSuccess && !rc && graph) e = hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0);
             if (graph) (void)hipGraphDestroy(graph);
             if (e != hipSuccess || rc || !S.gexec) {

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <atomic>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -17,9 +18,10 @@
 #include "mbgpu_internal.h"
 
 int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s);
-int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipStream_t s);
-int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gather, hipStream_t s);
-int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s);
+int mb_launch_dimers(MbBuffers& B, const int* ncnt, int ncap, const CellGeom& g, bool gather, hipStream_t s);
+int mb_launch_trimers(MbBuffers& B, const int* ncnt, int ncap, int blk_off, const CellGeom& g, bool gather,
+                      hipStream_t s);
+int mb_launch_accumulate(MbBuffers& B, double* fx, double* fy, double* fz, hipStream_t s);
 int md_launch_stage(int stage, int nr, double dt, const double* mass, double* x, double* y, double* z, double* vx,
                     double* vy, double* vz, double* fx, double* fy, double* fz, double* fsite, double* ekin,
                     double* blk, double* out, hipStream_t s);
@@ -34,6 +36,76 @@ int mb_launch_heat_flux(int nreal, const double* vx, const double* vy, const dou
 static std::atomic<long> g_alloc_serial{0};
 
 long mbgpu_alloc_serial() { return g_alloc_serial.fetch_add(1); }
+
+// Gate between the library's entries and a graph capture.  A capture must not overlap
+// another thread's null-stream work (a synchronous copy or set during a set-up, a free):
+// HIP rejects that work while any stream captures.  Every entry holds the gate shared while
+// it runs (the entries wait for their own streams before returning, so no library work is
+// in flight outside an entry); a capture holds it alone.  New entries wait while a capture
+// is pending, so a stream of entries cannot starve it.  An entry called from inside another
+// on the same thread does not take the gate again.
+namespace {
+class EntryGate {
+    std::mutex m_;
+    std::condition_variable cv_;
+    int readers_ = 0, waiting_ = 0;
+    bool writer_ = false;
+
+public:
+    void lock_shared()
+    {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return !writer_ && waiting_ == 0; });
+        ++readers_;
+    }
+    void unlock_shared()
+    {
+        std::lock_guard<std::mutex> l(m_);
+        if (--readers_ == 0) cv_.notify_all();
+    }
+    void lock()
+    {
+        std::unique_lock<std::mutex> l(m_);
+        ++waiting_;
+        cv_.wait(l, [&] { return !writer_ && readers_ == 0; });
+        --waiting_;
+        writer_ = true;
+    }
+    void unlock()
+    {
+        std::lock_guard<std::mutex> l(m_);
+        writer_ = false;
+        cv_.notify_all();
+    }
+};
+EntryGate g_gate;
+thread_local int g_gate_depth = 0;
+
+struct EntryGuard {
+    EntryGuard()
+    {
+        if (g_gate_depth++ == 0) g_gate.lock_shared();
+    }
+    ~EntryGuard()
+    {
+        if (--g_gate_depth == 0) g_gate.unlock_shared();
+    }
+};
+}  // namespace
+
+void capture_gate_begin()
+{
+    if (g_gate_depth > 0) g_gate.unlock_shared();
+    g_gate.lock();
+}
+
+void capture_gate_end()
+{
+    g_gate.unlock();
+    if (g_gate_depth > 0) g_gate.lock_shared();
+}
+
+
 
 int mbgpu_poison_this(long serial)
 {
@@ -279,9 +351,9 @@ int build_lists(mbgpu_ctx* c)
 int run_clusters(mbgpu_ctx* c, double* fx, double* fy, double* fz)
 {
     MbBuffers& B = c->B;
-    int rc = mb_launch_dimers(B, (int)c->nd, c->geom, true, c->stream);
-    if (!rc) rc = mb_launch_trimers(B, (int)c->nt, (int)c->nd, c->geom, true, c->stream);
-    if (!rc) rc = mb_launch_accumulate(B, (int)c->nd, (int)c->nt, fx, fy, fz, c->stream);
+    int rc = mb_launch_dimers(B, B.counts, (int)B.dcap, c->geom, true, c->stream);
+    if (!rc) rc = mb_launch_trimers(B, B.counts + 1, (int)B.tcap, (int)((B.dcap + 127) / 128), c->geom, true, c->stream);
+    if (!rc) rc = mb_launch_accumulate(B, fx, fy, fz, c->stream);
     if (rc) return fail(rc, "cluster kernels failed (%d)", rc);
     return 0;
 }
@@ -336,8 +408,11 @@ int leaf_eval(int nmon, const double* const* w, double* E, double* gout)
         for (int k = 0; k < 9; ++k) soa[9 * m + k] = w[m][k];
     double* dclus = nmon == 2 ? B.clus2 : B.clus3;
     HIPCHK(hipMemcpyAsync(dclus, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, c->stream));
-    int rc = nmon == 2 ? mb_launch_dimers(B, 1, c->geom, false, c->stream)
-                       : mb_launch_trimers(B, 1, 0, c->geom, false, c->stream);
+    // one cluster (the count 1 in counts[0] and counts[1]), stride 1
+    const int one[2] = {1, 1};
+    HIPCHK(hipMemcpyAsync(B.counts, one, sizeof one, hipMemcpyHostToDevice, c->stream));
+    int rc = nmon == 2 ? mb_launch_dimers(B, B.counts, 1, c->geom, false, c->stream)
+                       : mb_launch_trimers(B, B.counts + 1, 1, 0, c->geom, false, c->stream);
     if (rc) return fail(rc, "leaf kernels failed");
     HIPCHK(hipMemcpyAsync(E, nmon == 2 ? B.e2 : B.e3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(gout, nmon == 2 ? B.g2 : B.g3, sizeof(double) * 9 * nmon, hipMemcpyDeviceToHost,
@@ -447,6 +522,7 @@ int mbgpu_check_listttm2(int nttm2, const int* listttm2, int natms)
 
 mbgpu_ctx* mbgpu_create(int nw, int natms)
 {
+    EntryGuard gate_;
     if (nw < 1 || natms < 3 * nw) {
         fail(-1, "mbgpu_create: need nw >= 1 and natms >= 3*nw (got %d, %d)", nw, natms);
         return nullptr;
@@ -480,6 +556,7 @@ mbgpu_ctx* mbgpu_create(int nw, int natms)
 
 void mbgpu_destroy(mbgpu_ctx* c)
 {
+    EntryGuard gate_;
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->el_stream) (void)hipStreamSynchronize(c->el_stream);
@@ -509,6 +586,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
 
 int mbgpu_set_elec_reciprocal(mbgpu_ctx* c, int mode, int order, const int grid[3])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (mode == 0) {
         c->el.recip = 0;
@@ -537,6 +615,7 @@ int mbgpu_set_elec_reciprocal(mbgpu_ctx* c, int mode, int order, const int grid[
 
 int mbgpu_set_graphs(mbgpu_ctx* c, int on)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (on != 0 && on != 1) return fail(-1, "graphs: 0 = launch kernel by kernel, 1 = capture and replay (got %d)", on);
     c->el.graphs = on;
@@ -546,6 +625,7 @@ int mbgpu_set_graphs(mbgpu_ctx* c, int on)
 
 int mbgpu_get_graph_counts(mbgpu_ctx* c, long counts[2])
 {
+    EntryGuard gate_;
     if (!c || !counts) return fail(-1, "null context or output");
     counts[0] = c->el.gcaptures;
     counts[1] = c->el.greplays;
@@ -554,6 +634,7 @@ int mbgpu_get_graph_counts(mbgpu_ctx* c, long counts[2])
 
 int mbgpu_set_precision(mbgpu_ctx* c, int mode)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (mode < 0 || mode > 3)
         return fail(-1, "precision mode %d: bit 0 = FP32 2B polynomial, bit 1 = FP32 3B polynomial", mode);
@@ -577,6 +658,7 @@ int mbgpu_mbpol_forces_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nt
                            const double* x_dev, const double* y_dev, const double* z_dev, double* fx_dev,
                            double* fy_dev, double* fz_dev, double* engsrp, double* virsrp, double stress[9])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (nttm2 != 3 * c->B.nw) return fail(-1, "nttm2 = %d does not match the context (3*%d)", nttm2, c->B.nw);
     if (!make_geom(imcon, cell, c->geom))
@@ -599,6 +681,7 @@ int mbgpu_mbpol_forces(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
                        const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
                        double* fzz, double* engsrp, double* virsrp, double stress[9])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     const int na = c->B.natms;
     if (nttm2 != 3 * c->B.nw) return fail(-1, "nttm2 = %d does not match the context (3*%d)", nttm2, c->B.nw);
@@ -628,6 +711,7 @@ int mbgpu_mbpol_forces(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
 
 int mbgpu_get_decomposition(mbgpu_ctx* c, double* ae, double* hw, double* forces)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     const int na = c->B.natms;
     if (ae) HIPCHK(hipMemcpyAsync(ae, c->B.atomic_energy, sizeof(double) * na, hipMemcpyDeviceToHost, c->stream));
@@ -640,6 +724,7 @@ int mbgpu_get_decomposition(mbgpu_ctx* c, double* ae, double* hw, double* forces
 
 int mbgpu_get_lists(mbgpu_ctx* c, int* dimers, long dcap, long* nd, int* trimers, long tcap, long* nt)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (nd) *nd = c->nd;
     if (nt) *nt = c->nt;
@@ -657,6 +742,7 @@ int mbgpu_heat_flux(mbgpu_ctx* c, const double* vxx, const double* vyy, const do
                     const double* e_pot_extra, const double* w_extra, const double* stress_ew1p, double jkin[3],
                     double jpot[3], double jtot[3])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     const int nr = 3 * c->B.nw;
     double* v = c->dv;
@@ -690,6 +776,7 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const 
                         const double* e_pot_extra, const double* w_extra, const double* stress_ew1p, double jkin[3],
                         double jpot[3], double jtot[3])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     const int nr = 3 * c->B.nw;
     if (c->B.nblk_cap == 0 && ensure_clusters(c, 1, 1)) return -1;
@@ -722,6 +809,7 @@ int mbgpu_polar_electrostatics_dev(mbgpu_ctx* c, int imcon, const double cell[9]
                                    double* x_dev, double* y_dev, double* z_dev, double* dip_dev, double* f_dev,
                                    double* engcpe, double* vircpe, double stress[9], int* niter)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     const int nw = c->B.nw, n = 4 * nw;
     if (c->B.natms < n) return fail(-1, "the context holds %d sites; the electrostatics need 4*nw = %d", c->B.natms, n);
@@ -764,6 +852,7 @@ int mbgpu_polar_electrostatics(mbgpu_ctx* c, int imcon, const double cell[9], co
                                double* chge, double* fxx, double* fyy, double* fzz, double* engcpe, double* vircpe,
                                double stress[9], int* niter)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     const int nw = c->B.nw, n = 4 * nw;
     if (c->B.natms < n) return fail(-1, "the context holds %d sites; the electrostatics need 4*nw = %d", c->B.natms, n);
@@ -817,6 +906,7 @@ int mbgpu_polar_electrostatics(mbgpu_ctx* c, int imcon, const double cell[9], co
 int mbgpu_get_elec_details(mbgpu_ctx* c, double* scal, double* chge, double* vesp, double* e_real, double* w_real,
                            double* s_real, double* f_site)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!c->el.ready) return fail(-1, "no completed electrostatics evaluation in this context");
     const ElecBuffers& E = c->el.E;
@@ -837,6 +927,7 @@ int mbgpu_water_terms_dev(mbgpu_ctx* c, int imcon, const double cell[9], const m
                           const double* x_dev, const double* y_dev, const double* z_dev, double* fx_dev,
                           double* fy_dev, double* fz_dev, double eng[4], double vir[4], double stress[9])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!p) return fail(-1, "null water-term parameters");
     if (!x_dev || !y_dev || !z_dev) return fail(-1, "null position array");
@@ -870,6 +961,7 @@ int mbgpu_water_terms(mbgpu_ctx* c, int imcon, const double cell[9], const mbgpu
                       const double* xxx, const double* yyy, const double* zzz, double* fxx, double* fyy,
                       double* fzz, double eng[4], double vir[4], double stress[9])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!xxx || !yyy || !zzz) return fail(-1, "null position array");
     const int nr = 3 * c->B.nw;
@@ -894,6 +986,7 @@ int mbgpu_water_terms(mbgpu_ctx* c, int imcon, const double cell[9], const mbgpu
 
 int mbgpu_get_water_details(mbgpu_ctx* c, double* e_real, double* w_real, double* f_real)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!c->wt.ready) return fail(-1, "no completed water-terms evaluation in this context");
     const WaterBuffers& W = c->wt.B;
@@ -913,6 +1006,7 @@ int mbgpu_get_water_details(mbgpu_ctx* c, double* e_real, double* w_real, double
 
 int mbgpu_heat_check(mbgpu_ctx* c, const double* e_kin_dev, double out[12])
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!out) return fail(-1, "null output");
     const int na = c->B.natms, nr = 3 * c->B.nw;
@@ -941,6 +1035,7 @@ int mbgpu_verlet_stage(mbgpu_ctx* c, int stage, double dt, const double* mass_de
                        double* z_dev, double* vx_dev, double* vy_dev, double* vz_dev, double* fx_dev,
                        double* fy_dev, double* fz_dev, double* fsite_dev, double* ekin_dev, double* ekin_total)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (stage != 1 && stage != 2) return fail(-1, "stage must be 1 or 2 (got %d)", stage);
     if (!mass_dev || !x_dev || !y_dev || !z_dev || !vx_dev || !vy_dev || !vz_dev)
@@ -963,6 +1058,7 @@ int mbgpu_verlet_stage(mbgpu_ctx* c, int stage, double dt, const double* mass_de
 
 int mbgpu_heat_acf_dev(mbgpu_ctx* c, const double* j_dev, long nt, int ntau, double* acf_dev)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!j_dev || !acf_dev) return fail(-1, "null current or autocorrelation array");
     if (nt < 1 || ntau < 1 || ntau > nt) return fail(-1, "need 1 <= ntau <= nt (got nt %ld, ntau %d)", nt, ntau);
@@ -974,6 +1070,7 @@ int mbgpu_heat_acf_dev(mbgpu_ctx* c, const double* j_dev, long nt, int ntau, dou
 
 int mbgpu_heat_acf(mbgpu_ctx* c, const double* j, long nt, int ntau, double* acf)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!j || !acf) return fail(-1, "null current or autocorrelation array");
     if (nt < 1 || ntau < 1 || ntau > nt) return fail(-1, "need 1 <= ntau <= nt (got nt %ld, ntau %d)", nt, ntau);
@@ -997,6 +1094,7 @@ int mbgpu_heat_acf(mbgpu_ctx* c, const double* j, long nt, int ntau, double* acf
 int mbgpu_ring_matrices(int nbead, int langevin, double* nmode_to_cart, double* cart_to_nmode,
                         double* tcart_to_tnmode, double* lambda)
 {
+    EntryGuard gate_;
     if (nbead < 2 || nbead % 2 || nbead > MBGPU_RING_MAX_BEADS)
         return fail(-1, "nbead must be even, 2 <= nbead <= %d (got %d)", MBGPU_RING_MAX_BEADS, nbead);
     if (!nmode_to_cart || !cart_to_nmode || !tcart_to_tnmode || !lambda) return fail(-1, "null output array");
@@ -1053,6 +1151,7 @@ int mbgpu_ring_matrices(int nbead, int langevin, double* nmode_to_cart, double* 
 int mbgpu_ring_transform_dev(mbgpu_ctx* c, int nbead, long ncoord, const double* M, const double* in_dev,
                              double* out_dev)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (nbead < 1 || nbead > MBGPU_RING_MAX_BEADS) return fail(-1, "nbead out of range (got %d)", nbead);
     if (ncoord < 1) return fail(-1, "ncoord must be positive");
@@ -1075,6 +1174,7 @@ int mbgpu_ring_spring_dev(mbgpu_ctx* c, int nbead, int natom, double coeff, doub
                           const double* mass_dev, const double* x_dev, const double* xc_dev, double* f_dev,
                           double* epot_spring, double* epot_deriv)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (nbead < 1 || nbead > MBGPU_RING_MAX_BEADS) return fail(-1, "nbead out of range (got %d)", nbead);
     if (natom < 1) return fail(-1, "natom must be positive");
@@ -1102,6 +1202,7 @@ int mbgpu_ring_spring_dev_noalloc(mbgpu_ctx* c, int nbead, int natom, double coe
                                   const double* mass_dev, const double* x_dev, const double* xc_dev, double* f_dev,
                                   double* epot_spring, double* epot_deriv)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (nbead < 1 || nbead > MBGPU_RING_MAX_BEADS) return fail(-1, "nbead out of range (got %d)", nbead);
     if (natom < 1) return fail(-1, "natom must be positive");
@@ -1130,6 +1231,7 @@ int mbgpu_spme_recip_dev(mbgpu_ctx* c, const double cell[9], int order, const in
                          const double* x_dev, const double* y_dev, const double* z_dev, const double* q_dev,
                          const double* mu_dev, double* phi_dev, double* efield_dev, double* f_dev, double* energy)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (n < 0) return fail(-1, "n must not be negative");
     if (n > 0 && (!x_dev || !y_dev || !z_dev || !q_dev)) return fail(-1, "null position or charge array");
@@ -1152,6 +1254,7 @@ int mbgpu_spme_recip_dev(mbgpu_ctx* c, const double cell[9], int order, const in
 
 int mbgpu_get_dipoles(mbgpu_ctx* c, double* dip)
 {
+    EntryGuard gate_;
     if (!c) return fail(-1, "null context");
     if (!c->el.ready) return fail(-1, "no completed electrostatics evaluation in this context");
     if (dip)
@@ -1162,15 +1265,18 @@ int mbgpu_get_dipoles(mbgpu_ctx* c, double* dip)
 
 int mbgpu_time_forces(mbgpu_ctx* c, int reps, double* ms_per_eval, double* ms_pip3b)
 {
+    EntryGuard gate_;
     if (!c || !c->have_inputs) return fail(-1, "mbgpu_time_forces: run mbgpu_mbpol_forces first");
     MbBuffers& B = c->B;
     float total = 0.f, k3 = 0.f;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < reps; ++r) {
         if (build_lists(c)) return -1;
-        int rc = mb_launch_dimers(B, (int)c->nd, c->geom, true, c->stream);
-        if (!rc) rc = mb_launch_trimers(B, (int)c->nt, (int)c->nd, c->geom, true, c->stream);
-        if (!rc) rc = mb_launch_accumulate(B, (int)c->nd, (int)c->nt, nullptr, nullptr, nullptr, c->stream);
+        int rc = mb_launch_dimers(B, B.counts, (int)B.dcap, c->geom, true, c->stream);
+        if (!rc)
+            rc = mb_launch_trimers(B, B.counts + 1, (int)B.tcap, (int)((B.dcap + 127) / 128), c->geom, true,
+                                   c->stream);
+        if (!rc) rc = mb_launch_accumulate(B, nullptr, nullptr, nullptr, c->stream);
         if (rc) return fail(rc, "timing run failed");
         double s[16];
         HIPCHK(hipMemcpyAsync(s, B.scal, sizeof s, hipMemcpyDeviceToHost, c->stream));
@@ -1180,7 +1286,7 @@ int mbgpu_time_forces(mbgpu_ctx* c, int reps, double* ms_per_eval, double* ms_pi
     HIPCHK(hipEventElapsedTime(&total, c->ev0, c->ev1));
     // the 3-body polynomial kernel alone, same stream
     HIPCHK(hipEventRecord(c->ev2, c->stream));
-    for (int r = 0; r < reps; ++r) launch_pip3b_f64(B.xv3, B.part3, (int)c->nt, B.cs3, c->stream);
+    for (int r = 0; r < reps; ++r) launch_pip3b_f64(B.xv3, B.part3, B.counts + 1, (int)B.tcap, B.cs3, c->stream);
     HIPCHK(hipEventRecord(c->ev3, c->stream));
     HIPCHK(hipEventSynchronize(c->ev3));
     HIPCHK(hipEventElapsedTime(&k3, c->ev2, c->ev3));
@@ -1191,6 +1297,7 @@ int mbgpu_time_forces(mbgpu_ctx* c, int reps, double* ms_per_eval, double* ms_pi
 
 int mbgpu_time_elec(mbgpu_ctx* c, int reps, double* ms_sfac, double* flop_sfac)
 {
+    EntryGuard gate_;
     if (!c || !c->el.ready) return fail(-1, "mbgpu_time_elec: run mbgpu_polar_electrostatics first");
     if (reps < 1) return fail(-1, "reps must be positive");
     ElecState& S = c->el;
@@ -1216,6 +1323,7 @@ void mbpol_3b_cutoff(double* r) { *r = MBGPU_R3F; }
 
 void mbpol_2b_poly_(const double* w1, const double* w2, double* E, double* g1, double* g2, double* force_matrix)
 {
+    EntryGuard gate_;
     (void)force_matrix;
     const double d[3] = {w1[0] - w2[0], w1[1] - w2[1], w1[2] - w2[2]};
     if (sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) > MBGPU_R2F) {  // leaf early return
@@ -1238,6 +1346,7 @@ void mbpol_2b_poly_(const double* w1, const double* w2, double* E, double* g1, d
 void mbpol_3b_poly_(const double* w1, const double* w2, const double* w3, double* E, double* g1, double* g2,
                     double* g3, double* force_matrix)
 {
+    EntryGuard gate_;
     (void)force_matrix;
     const double* w[3] = {w1, w2, w3};
     double g[27];
@@ -1255,12 +1364,14 @@ void mbpol_3b_poly_(const double* w1, const double* w2, const double* w3, double
 
 void mbpol_2b_poly(const double* w1, const double* w2, double* E, double* g1, double* g2, double* fm)
 {
+    EntryGuard gate_;
     mbpol_2b_poly_(w1, w2, E, g1, g2, fm);
 }
 
 void mbpol_3b_poly(const double* w1, const double* w2, const double* w3, double* E, double* g1, double* g2,
                    double* g3, double* fm)
 {
+    EntryGuard gate_;
     mbpol_3b_poly_(w1, w2, w3, E, g1, g2, g3, fm);
 }
 

@@ -35,13 +35,14 @@ inline hipError_t mbgpu_alloc(T** p, size_t n)
     return e;
 }
 
-// Launch wrappers for the generated polynomial kernels (pip2b.hip / pip3b.hip).
-// xv: variables SoA [nvars][nc]; part: SoA [nslices][nvars+1][nc]; cs: coefficient stream.
-void launch_pip2b_f64(const double* xv, double* part, int nc, const double* cs, hipStream_t s);
-void launch_pip3b_f64(const double* xv, double* part, int nc, const double* cs, hipStream_t s);
+// Launch wrappers for the generated polynomial kernels (pip2b.hip / pip3b.hip): the first
+// *ncnt clusters (a device count) of buffers laid out with stride ncap, ncap lanes launched.
+// xv: variables SoA [nvars][ncap]; part: SoA [nslices][nvars+1][ncap]; cs: coefficient stream.
+void launch_pip2b_f64(const double* xv, double* part, const int* ncnt, int ncap, const double* cs, hipStream_t s);
+void launch_pip3b_f64(const double* xv, double* part, const int* ncnt, int ncap, const double* cs, hipStream_t s);
 // FP32 instantiations of the same code (mixed-precision mode)
-void launch_pip2b_f32(const float* xv, float* part, int nc, const float* cs, hipStream_t s);
-void launch_pip3b_f32(const float* xv, float* part, int nc, const float* cs, hipStream_t s);
+void launch_pip2b_f32(const float* xv, float* part, const int* ncnt, int ncap, const float* cs, hipStream_t s);
+void launch_pip3b_f32(const float* xv, float* part, const int* ncnt, int ncap, const float* cs, hipStream_t s);
 int pip2b_nslices();
 int pip3b_nslices();
 int pip2b_stream_len();
@@ -240,6 +241,9 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
 int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStream_t s, std::string& err);
 void elec_release(ElecState& S);
 void elec_drop_graph(ElecState& S);
+// A graph capture holds the library's entry gate alone (mbgpu_api.hip): call around it.
+void capture_gate_begin();
+void capture_gate_end();
 
 // ------------------------------------------------------------- monomer and dispersion ----
 

@@ -412,10 +412,11 @@ __global__ void k_fill_dimers(int nw, int cap2, const int* __restrict__ nbr2, co
 }
 
 // Re-image a dimer around its second water (mbpol.f90:496-509); SoA [18][nd].
-__global__ void k_gather2(int nd, const int* __restrict__ dimers, const double* __restrict__ mon, CellGeom g,
+__global__ void k_gather2(const int* __restrict__ ncnt, int ncap, const int* __restrict__ dimers, const double* __restrict__ mon, CellGeom g,
                           double* __restrict__ clus)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nd = min(*ncnt, ncap);
     if (d >= nd) return;
     double w1[9], w2[9];
     load_mon(mon, dimers[2 * d], w1);
@@ -430,16 +431,17 @@ __global__ void k_gather2(int nd, const int* __restrict__ dimers, const double* 
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        clus[(long)k * nd + d] = w1[k];
-        clus[(long)(9 + k) * nd + d] = w2[k];
+        clus[(long)k * ncap + d] = w1[k];
+        clus[(long)(9 + k) * ncap + d] = w2[k];
     }
 }
 
 // Re-image a trimer around its middle water (mbpol.f90:843-882); SoA [27][nt].
-__global__ void k_gather3(int nt, const int* __restrict__ trimers, const double* __restrict__ mon, CellGeom g,
+__global__ void k_gather3(const int* __restrict__ ncnt, int ncap, const int* __restrict__ trimers, const double* __restrict__ mon, CellGeom g,
                           double* __restrict__ clus)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = min(*ncnt, ncap);
     if (t >= nt) return;
     double w1[9], w2[9], w3[9];
     load_mon(mon, trimers[3 * t], w1);
@@ -460,21 +462,22 @@ __global__ void k_gather3(int nt, const int* __restrict__ trimers, const double*
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        clus[(long)k * nt + t] = w1[k];
-        clus[(long)(9 + k) * nt + t] = w2[k];
-        clus[(long)(18 + k) * nt + t] = w3[k];
+        clus[(long)k * ncap + t] = w1[k];
+        clus[(long)(9 + k) * ncap + t] = w2[k];
+        clus[(long)(18 + k) * ncap + t] = w3[k];
     }
 }
 
 // 2B variables (mbpol-2b-poly.cpp:358-445): SoA [31][nd]; xvf (may be NULL) receives the
 // same values rounded to FP32 for the mixed-precision polynomial
-__global__ void k_setup2(int nd, const double* __restrict__ clus, double* __restrict__ xv, float* __restrict__ xvf)
+__global__ void k_setup2(const int* __restrict__ ncnt, int ncap, const double* __restrict__ clus, double* __restrict__ xv, float* __restrict__ xvf)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nd = min(*ncnt, ncap);
     if (d >= nd) return;
     double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
 #pragma unroll
-    for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * nd + d];
+    for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * ncap + d];
     dimer_sites(w, s, a1, a2, b1, b2);
 #pragma unroll
     for (int n = 0; n < 31; ++n) {
@@ -484,8 +487,8 @@ __global__ void k_setup2(int nd, const double* __restrict__ clus, double* __rest
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         const double e = exp(c2_k[v.k] * ((double)v.d0 - r));
         const double val = v.kind ? e * (1.0 / r) : e;
-        xv[(long)n * nd + d] = val;
-        if (xvf) xvf[(long)n * nd + d] = (float)val;
+        xv[(long)n * ncap + d] = val;
+        if (xvf) xvf[(long)n * ncap + d] = (float)val;
     }
 }
 
@@ -530,7 +533,7 @@ __device__ __forceinline__ void cluster_raw_sum(const int* ids, const int* __res
     }
 }
 
-__global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* __restrict__ clus,
+__global__ __launch_bounds__(128) void k_finish2(const int* __restrict__ ncnt, int ncap, int nsl, const double* __restrict__ clus,
                                                  const double* __restrict__ xv, const double* __restrict__ part,
                                                  double* __restrict__ erec, double* __restrict__ grec,
                                                  double* __restrict__ blk, const int* __restrict__ dimers,
@@ -539,18 +542,19 @@ __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* 
                                                  double* __restrict__ srec)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nd = min(*ncnt, ncap);
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (d < nd && dimers) {
         double S[3];
         cluster_raw_sum<2>(dimers + 2L * d, wbase, x, y, z, S);
         srec[d] = S[0];
-        srec[(long)nd + d] = S[1];
-        srec[2L * nd + d] = S[2];
+        srec[(long)ncap + d] = S[1];
+        srec[2L * ncap + d] = S[2];
     }
     if (d < nd) {
         double w[18], s[30], a1[3], a2[3], b1[3], b2[3];
 #pragma unroll
-        for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * nd + d];
+        for (int k = 0; k < 18; ++k) w[k] = clus[(long)k * ncap + d];
         const double dOO[3] = {w[0] - w[9], w[1] - w[10], w[2] - w[11]};
         const double rOO = sqrt(dOO[0] * dOO[0] + dOO[1] * dOO[1] + dOO[2] * dOO[2]);
         double gout[18];
@@ -561,19 +565,19 @@ __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* 
         } else {
             dimer_sites(w, s, a1, a2, b1, b2);
             double E = 0.0;
-            for (int q = 0; q < nsl; ++q) E += part[((long)q * 32) * nd + d];
+            for (int q = 0; q < nsl; ++q) E += part[((long)q * 32) * ncap + d];
             double xg[30];
 #pragma unroll
             for (int k = 0; k < 30; ++k) xg[k] = 0.0;
 #pragma unroll
             for (int n = 0; n < 31; ++n) {
                 double gn = 0.0;
-                for (int q = 0; q < nsl; ++q) gn += part[((long)q * 32 + 1 + n) * nd + d];
+                for (int q = 0; q < nsl; ++q) gn += part[((long)q * 32 + 1 + n) * ncap + d];
                 const V2Def v = c2_vars[n];
                 const double dd[3] = {s[3 * v.s1] - s[3 * v.s2], s[3 * v.s1 + 1] - s[3 * v.s2 + 1],
                                       s[3 * v.s1 + 2] - s[3 * v.s2 + 2]};
                 const double r = sqrt(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
-                const double val = xv[(long)n * nd + d];
+                const double val = xv[(long)n * ncap + d];
                 const double k = c2_k[v.k];
                 double gg;
                 if (v.kind) {
@@ -606,7 +610,7 @@ __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* 
         }
         erec[d] = Eout;
 #pragma unroll
-        for (int k = 0; k < 18; ++k) grec[(long)k * nd + d] = gout[k];
+        for (int k = 0; k < 18; ++k) grec[(long)k * ncap + d] = gout[k];
         red[0] = Eout * MBGPU_ENGUNIT;
         cluster_virial<18>(w, gout, red);
     }
@@ -614,13 +618,14 @@ __global__ __launch_bounds__(128) void k_finish2(int nd, int nsl, const double* 
 }
 
 // 3B variables (mbpol-3b-poly.cpp:170-206): SoA [36][nt]; xvf as in k_setup2
-__global__ void k_setup3(int nt, const double* __restrict__ clus, double* __restrict__ xv, float* __restrict__ xvf)
+__global__ void k_setup3(const int* __restrict__ ncnt, int ncap, const double* __restrict__ clus, double* __restrict__ xv, float* __restrict__ xvf)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = min(*ncnt, ncap);
     if (t >= nt) return;
     double w[27];
 #pragma unroll
-    for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * nt + t];
+    for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * ncap + t];
 #pragma unroll
     for (int n = 0; n < 36; ++n) {
         const V3Def v = c3_vars[n];
@@ -628,14 +633,14 @@ __global__ void k_setup3(int nt, const double* __restrict__ clus, double* __rest
                      dz = w[3 * v.a1 + 2] - w[3 * v.a2 + 2];
         const double dd = sqrt(dx * dx + dy * dy + dz * dz);
         const double val = exp(-c3_k[v.cls] * (dd - c3_r0[v.cls]));
-        xv[(long)n * nt + t] = val;
-        if (xvf) xvf[(long)n * nt + t] = (float)val;
+        xv[(long)n * ncap + t] = val;
+        if (xvf) xvf[(long)n * ncap + t] = (float)val;
     }
 }
 
 // mbpol-3b-poly.cpp:208-312: switch s = sab*sac + sab*sbc + sac*sbc, chain rule, switch
 // gradients.  Record: E (kcal/mol) and dE/dx of the 9 atoms; block partials as k_finish2.
-__global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* __restrict__ clus,
+__global__ __launch_bounds__(128) void k_finish3(const int* __restrict__ ncnt, int ncap, int nsl, const double* __restrict__ clus,
                                                  const double* __restrict__ xv, const double* __restrict__ part,
                                                  double* __restrict__ erec, double* __restrict__ grec,
                                                  double* __restrict__ blk, const int* __restrict__ trimers,
@@ -644,20 +649,21 @@ __global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* 
                                                  double* __restrict__ srec)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = min(*ncnt, ncap);
     double red[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (t < nt && trimers) {
         double S[3];
         cluster_raw_sum<3>(trimers + 3L * t, wbase, x, y, z, S);
         srec[t] = S[0];
-        srec[(long)nt + t] = S[1];
-        srec[2L * nt + t] = S[2];
+        srec[(long)ncap + t] = S[1];
+        srec[2L * ncap + t] = S[2];
     }
     if (t < nt) {
         double w[27];
 #pragma unroll
-        for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * nt + t];
+        for (int k = 0; k < 27; ++k) w[k] = clus[(long)k * ncap + t];
         double E = 0.0;
-        for (int q = 0; q < nsl; ++q) E += part[((long)q * 37) * nt + t];
+        for (int q = 0; q < nsl; ++q) E += part[((long)q * 37) * ncap + t];
         double rab[3], rac[3], rbc[3];
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
@@ -679,14 +685,14 @@ __global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* 
 #pragma unroll
         for (int n = 0; n < 36; ++n) {
             double gn = 0.0;
-            for (int q = 0; q < nsl; ++q) gn += part[((long)q * 37 + 1 + n) * nt + t];
+            for (int q = 0; q < nsl; ++q) gn += part[((long)q * 37 + 1 + n) * ncap + t];
             gn *= sw;
             const V3Def v = c3_vars[n];
             const double dx[3] = {w[3 * v.a1] - w[3 * v.a2], w[3 * v.a1 + 1] - w[3 * v.a2 + 1],
                                   w[3 * v.a1 + 2] - w[3 * v.a2 + 2]};
             const double dd = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
             const double k = c3_k[v.cls];
-            const double gg = -k * gn * xv[(long)n * nt + t] / dd;
+            const double gg = -k * gn * xv[(long)n * ncap + t] / dd;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 gc[3 * v.a1 + i] += gg * dx[i];
@@ -705,7 +711,7 @@ __global__ __launch_bounds__(128) void k_finish3(int nt, int nsl, const double* 
         const double Eout = E * sw;
         erec[t] = Eout;
 #pragma unroll
-        for (int k = 0; k < 27; ++k) grec[(long)k * nt + t] = gc[k];
+        for (int k = 0; k < 27; ++k) grec[(long)k * ncap + t] = gc[k];
         red[0] = Eout * MBGPU_ENGUNIT;
         cluster_virial<27>(w, gc, red);
     }
@@ -881,21 +887,23 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather_water(GatherArgs A)
     }
 }
 
-// Sum the nsl slice partials of every (variable, cluster) into slice 0, in slice order.
-__global__ void k_slice_sum(long n, int nsl, double* __restrict__ part)
+// Sum the nsl slice partials of every (variable, cluster) into slice 0, in slice order:
+// n = rows x ncap elements, clusters at or past the count skipped.
+__global__ void k_slice_sum(long n, int nsl, const int* __restrict__ ncnt, int ncap, double* __restrict__ part)
 {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || i % ncap >= *ncnt) return;
     double s = part[i];
     for (int q = 1; q < nsl; ++q) s += part[q * n + i];
     part[i] = s;
 }
 
 // FP64 sum of the nsl FP32 slice partials of every (variable, cluster), in slice order.
-__global__ void k_slice_sum_f(long n, int nsl, const float* __restrict__ pf, double* __restrict__ part)
+__global__ void k_slice_sum_f(long n, int nsl, const int* __restrict__ ncnt, int ncap, const float* __restrict__ pf,
+                              double* __restrict__ part)
 {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || i % ncap >= *ncnt) return;
     double s = (double)pf[i];
     for (int q = 1; q < nsl; ++q) s += (double)pf[q * n + i];
     part[i] = s;
@@ -991,58 +999,63 @@ int mb_launch_lists(MbBuffers& B, const CellGeom& g, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-// Cluster pipeline of the dimers; block partials of (energy, virial, stress) go to
-// B.blk[0 .. 8*ceil(nd/128)).
-int mb_launch_dimers(MbBuffers& B, int nd, const CellGeom& g, bool gather, hipStream_t s)
+// Cluster pipeline of the dimers: ncnt (device) dimers of a buffer laid out with stride and
+// grid ncap, so that the launches do not depend on the count (a graph may hold them); block
+// partials of (energy, virial, stress) go to B.blk[0 .. 8*ceil(ncap/128)), zero for the
+// blocks past the count.
+int mb_launch_dimers(MbBuffers& B, const int* ncnt, int ncap, const CellGeom& g, bool gather, hipStream_t s)
 {
-    if (nd <= 0) return 0;
-    const int nb = (nd + 127) / 128;
-    if (gather) hipLaunchKernelGGL(k_gather2, dim3(nb), dim3(128), 0, s, nd, B.dimers, B.mon, g, B.clus2);
-    const long n = 32L * nd;
+    if (ncap <= 0) return 0;
+    const int nb = (ncap + 127) / 128;
+    if (gather) hipLaunchKernelGGL(k_gather2, dim3(nb), dim3(128), 0, s, ncnt, ncap, B.dimers, B.mon, g, B.clus2);
+    const long n = 32L * ncap;
     if (B.mixed & 1) {
-        hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2, B.xv2f);
-        launch_pip2b_f32(B.xv2f, B.part2f, nd, B.cs2f, s);
-        hipLaunchKernelGGL(k_slice_sum_f, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2f,
+        hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, ncnt, ncap, B.clus2, B.xv2, B.xv2f);
+        launch_pip2b_f32(B.xv2f, B.part2f, ncnt, ncap, B.cs2f, s);
+        hipLaunchKernelGGL(k_slice_sum_f, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), ncnt, ncap,
+                           B.part2f, B.part2);
+    } else {
+        hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, ncnt, ncap, B.clus2, B.xv2, nullptr);
+        launch_pip2b_f64(B.xv2, B.part2, ncnt, ncap, B.cs2, s);
+        hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), ncnt, ncap,
                            B.part2);
-    } else {
-        hipLaunchKernelGGL(k_setup2, dim3(nb), dim3(128), 0, s, nd, B.clus2, B.xv2, nullptr);
-        launch_pip2b_f64(B.xv2, B.part2, nd, B.cs2, s);
-        hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip2b_nslices(), B.part2);
     }
-    hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, nd, 1, B.clus2, B.xv2, B.part2, B.e2, B.g2, B.blk,
-                       gather ? B.dimers : nullptr, B.wbase, B.x, B.y, B.z, B.s2);
+    hipLaunchKernelGGL(k_finish2, dim3(nb), dim3(128), 0, s, ncnt, ncap, 1, B.clus2, B.xv2, B.part2, B.e2, B.g2,
+                       B.blk, gather ? B.dimers : nullptr, B.wbase, B.x, B.y, B.z, B.s2);
     return (int)hipGetLastError();
 }
 
-// Same for trimers; block partials after those of the dimers (offset from nd).
-int mb_launch_trimers(MbBuffers& B, int nt, int nd, const CellGeom& g, bool gather, hipStream_t s)
+// Same for trimers; block partials after blk_off blocks of the dimers'.
+int mb_launch_trimers(MbBuffers& B, const int* ncnt, int ncap, int blk_off, const CellGeom& g, bool gather,
+                      hipStream_t s)
 {
-    if (nt <= 0) return 0;
-    const int nb = (nt + 127) / 128;
-    const int nb2 = (nd + 127) / 128;
-    if (gather) hipLaunchKernelGGL(k_gather3, dim3(nb), dim3(128), 0, s, nt, B.trimers, B.mon, g, B.clus3);
-    const long n = 37L * nt;
+    if (ncap <= 0) return 0;
+    const int nb = (ncap + 127) / 128;
+    if (gather) hipLaunchKernelGGL(k_gather3, dim3(nb), dim3(128), 0, s, ncnt, ncap, B.trimers, B.mon, g, B.clus3);
+    const long n = 37L * ncap;
     if (B.mixed & 2) {
-        hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3, B.xv3f);
-        launch_pip3b_f32(B.xv3f, B.part3f, nt, B.cs3f, s);
-        hipLaunchKernelGGL(k_slice_sum_f, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3f,
-                           B.part3);
+        hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, ncnt, ncap, B.clus3, B.xv3, B.xv3f);
+        launch_pip3b_f32(B.xv3f, B.part3f, ncnt, ncap, B.cs3f, s);
+        hipLaunchKernelGGL(k_slice_sum_f, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), ncnt, ncap,
+                           B.part3f, B.part3);
     } else {
-        hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, nt, B.clus3, B.xv3, nullptr);
-        launch_pip3b_f64(B.xv3, B.part3, nt, B.cs3, s);
-        hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), B.part3);
+        hipLaunchKernelGGL(k_setup3, dim3(nb), dim3(128), 0, s, ncnt, ncap, B.clus3, B.xv3, nullptr);
+        launch_pip3b_f64(B.xv3, B.part3, ncnt, ncap, B.cs3, s);
+        hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, s, n, pip3b_nslices(), ncnt, ncap,
+                           B.part3);
     }
-    hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, nt, 1, B.clus3, B.xv3, B.part3, B.e3, B.g3,
-                       B.blk + 8L * nb2, gather ? B.trimers : nullptr, B.wbase, B.x, B.y, B.z, B.s3);
+    hipLaunchKernelGGL(k_finish3, dim3(nb), dim3(128), 0, s, ncnt, ncap, 1, B.clus3, B.xv3, B.part3, B.e3, B.g3,
+                       B.blk + 8L * blk_off, gather ? B.trimers : nullptr, B.wbase, B.x, B.y, B.z, B.s3);
     return (int)hipGetLastError();
 }
 
-// Per-water gather of the cluster records into per-atom forces, energies and heat-flux
-// tensors; scal[0..7] = 2B (energy, virial, stress 6), scal[8..15] = 3B.
-int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, double* fz, hipStream_t s)
+// Per-water gather of the cluster records (the lists' dimers and trimers, strides dcap and
+// tcap) into per-atom forces, energies and heat-flux tensors; scal[0..7] = 2B (energy,
+// virial, stress 6), scal[8..15] = 3B.
+int mb_launch_accumulate(MbBuffers& B, double* fx, double* fy, double* fz, hipStream_t s)
 {
     const int na = B.natms, nw = B.nw;
-    const int nb2 = (nd + 127) / 128, nb3 = (nt + 127) / 128;
+    const int nb2 = (int)((B.dcap + 127) / 128), nb3 = (int)((B.tcap + 127) / 128);
     if (nb2 + nb3 > B.nblk_cap) return -1;
     hipLaunchKernelGGL(k_zero_i, dim3((na + 1023) / 1024), dim3(256), 0, s, reinterpret_cast<int*>(B.is_water),
                        (na + 3) / 4);
@@ -1050,7 +1063,7 @@ int mb_launch_accumulate(MbBuffers& B, int nd, int nt, double* fx, double* fy, d
     hipLaunchKernelGGL(k_zero_other_sites, dim3((na + 255) / 256), dim3(256), 0, s, na, B.is_water, B.fout,
                        B.atomic_energy, B.heat_w);
     GatherArgs A;
-    A.nw = nw; A.nd = nd; A.nt = nt; A.cap2 = B.cap2;
+    A.nw = nw; A.nd = (int)B.dcap; A.nt = (int)B.tcap; A.cap2 = B.cap2;
     A.wbase = B.wbase; A.dimers = B.dimers; A.trimers = B.trimers;
     A.nbr2 = B.nbr2; A.cnt2 = B.cnt2; A.off2 = B.off2; A.nbr2l = B.nbr2l; A.cnt2l = B.cnt2l;
     A.nbr3 = B.nbr3; A.cnt3 = B.cnt3; A.toff = B.toff; A.tcnt = B.tcnt; A.rowoff = B.rowoff; A.kmask = B.kmask;
