@@ -594,6 +594,114 @@ void drop_graph(ElecState& S)
 
 void elec_drop_graph(ElecState& S) { drop_graph(S); }
 
+namespace {
+// the reciprocal space of an evaluation: the k-sum (null), the particle mesh, or none (an
+// isolated cluster: a description with no mesh buffers)
+const ElecSpme* recip_of(const ElecState& S, ElecSpme& spx, ElecSpme& none)
+{
+    spx = ElecSpme{S.sp, S.spg, S.spg ? S.spg->vol / (8.0 * PI_D) : 0.0};
+    none = ElecSpme{nullptr, nullptr, 0.0};
+    return S.P.g.imcon == 0 ? &none : S.recip ? &spx : nullptr;
+}
+}  // namespace
+
+// Solver iterations a captured evaluation holds (0 before the first evaluation set a count):
+// the last count + 2, kept while it covers the solve and is not far too long (iterations past
+// convergence are no-ops of about a microsecond per kernel; a new capture costs more), so
+// counts that drift by one or two do not capture again.  MBGPU_GRAPH_BATCH=k caps it (a test
+// aid for the continuation after a graph).
+int elec_batch(const ElecState& S)
+{
+    int gb = 0;
+    if (S.last_iters > 0) {
+        gb = std::min(S.last_iters + 2, 64);
+        if (S.gbatch >= S.last_iters && S.gbatch <= S.last_iters + 8) gb = S.gbatch;
+    }
+    if (const char* ev = getenv("MBGPU_GRAPH_BATCH")) gb = gb ? std::min(gb, std::max(1, atoi(ev))) : 0;
+    return gb;
+}
+
+// One evaluation's device work with gb solver iterations and the final pass (what a graph
+// holds), kernels only.
+int elec_enqueue_full(ElecState& S, int distribute, int solver, int gb, hipStream_t s)
+{
+    ElecSpme spx, none;
+    const ElecSpme* sp = recip_of(S, spx, none);
+    const double g34 = exp(gammln_host(0.75)), gln34 = gammln_host(0.75);
+    int rc = enqueue_attempt(S, sp, solver, true, gb, s);
+    if (!rc) rc = el_launch_final(S.E, S.P, distribute, g34, gln34, solver == 1, sp, s);
+    return rc;
+}
+
+void elec_key_append(const ElecState& S, int distribute, int solver, int gb, std::vector<unsigned char>& k)
+{
+    const std::vector<unsigned char> e = graph_key(S, distribute, solver, gb);
+    k.insert(k.end(), e.begin(), e.end());
+}
+
+// The host side of an evaluation's end: the particle mesh's flags, then the scalars of
+// S.out from the reduced sums in S.scal (read back by the caller) and the solver state st.
+int elec_finish(ElecState& S, const double st[8], const int spflag[2], std::string& err)
+{
+    const ElecDev& P = S.P;
+    const ElecBuffers& E = S.E;
+    if (spflag[0] == 1) {
+        err = "particle mesh: a site position is not finite or too far from the cell to index the grid";
+        return -1;
+    }
+    if (spflag[0] == 2) {
+        err = "particle mesh: a charge or dipole is too large for the fixed-point grid";
+        return -1;
+    }
+    const double* q = S.scal;
+    const double engsic = -R4PIE0 * P.alpha * q[9] / SQRPI;
+    const double eng1 = P.scal1 * q[1] + engsic;
+    const double engdum = q[2];
+    const double omg_xx = q[3], omg_yy = q[4], omg_zz = q[5], omg_xy = q[6], omg_xz = q[7], omg_yz = q[8];
+    double st9[9];
+    st9[0] = P.scal1 * (omg_xx + engdum);
+    st9[1] = P.scal1 * omg_xy;
+    st9[2] = P.scal1 * omg_xz;
+    st9[3] = st9[1];
+    st9[4] = P.scal1 * (omg_yy + engdum);
+    st9[5] = P.scal1 * omg_yz;
+    st9[6] = st9[2];
+    st9[7] = st9[5];
+    st9[8] = P.scal1 * (omg_zz + engdum);
+    const double vir1 = -P.scal1 * (omg_xx + omg_yy + omg_zz + 3.0 * engdum);
+    // pair, excluded-pair and qd sums: [energy, virial, xx, xy, xz, yy, yz, zz]
+    const double* pp[3] = {q + 11, q + 19, q + 27};
+    double vir = vir1;
+    for (int t = 0; t < 3; ++t) {
+        const double* r = pp[t];
+        vir += r[1];
+        st9[0] += r[2];
+        st9[1] += r[3];
+        st9[2] += r[4];
+        st9[3] += r[3];
+        st9[4] += r[5];
+        st9[5] += r[6];
+        st9[6] += r[4];
+        st9[7] += r[6];
+        st9[8] += r[7];
+    }
+    double* o = S.out;
+    o[0] = eng1;
+    o[1] = q[11];
+    o[2] = q[19];
+    o[3] = q[0];
+    o[4] = q[0] + eng1 + q[11] + q[19];
+    o[5] = vir;
+    o[6] = vir1;
+    o[7] = q[28];
+    o[8] = st[0];
+    o[9] = st[1];
+    for (int i = 0; i < 9; ++i) o[10 + i] = st9[i];
+    o[19] = (double)E.K;
+    S.ready = true;
+    return 0;
+}
+
 // One evaluation with positions (real atoms) in E.x/y/z and start dipoles in E.dip.
 //
 // Once an evaluation has set the iteration count, the next ones run as one captured graph:
@@ -616,16 +724,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int herr = 0;
     bool need_final = true;
-    // Iterations in the graph: the last count + 2, kept while it covers the solve and is not
-    // far too long (iterations past convergence are no-ops of about a microsecond per kernel;
-    // a new capture costs more), so counts that drift by one or two do not capture again.
-    // MBGPU_GRAPH_BATCH=k caps it (a test aid for the continuation after a graph).
-    int gb = 0;
-    if (S.last_iters > 0) {
-        gb = std::min(S.last_iters + 2, 64);
-        if (S.gexec && S.gbatch >= S.last_iters && S.gbatch <= S.last_iters + 8) gb = S.gbatch;
-    }
-    if (const char* ev = getenv("MBGPU_GRAPH_BATCH")) gb = gb ? std::min(gb, std::max(1, atoi(ev))) : 0;
+    const int gb = elec_batch(S);
     bool from_graph = false;
     // MBGPU_GRAPHS=0 launches kernel by kernel whatever the context's setting (a test aid)
     const char* gv = getenv("MBGPU_GRAPHS");
@@ -640,8 +739,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
             hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
             int rc = 0;
             if (e == hipSuccess) {
-                rc = enqueue_attempt(S, sp, solver, true, gb, s);
-                if (!rc) rc = el_launch_final(E, P, distribute, g34, gln34, solver == 1, sp, s);
+                rc = elec_enqueue_full(S, distribute, solver, gb, s);
                 e = hipStreamEndCapture(s, &graph);
             }
             capture_gate_end();
@@ -750,59 +848,5 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     int spflag[2] = {0, 0};
     if (sp && sp->B) ECHK(hipMemcpyAsync(spflag, S.sp->flag, sizeof spflag, hipMemcpyDeviceToHost, s));
     ECHK(hipStreamSynchronize(s));
-    if (spflag[0] == 1) {
-        err = "particle mesh: a site position is not finite or too far from the cell to index the grid";
-        return -1;
-    }
-    if (spflag[0] == 2) {
-        err = "particle mesh: a charge or dipole is too large for the fixed-point grid";
-        return -1;
-    }
-    const double* q = S.scal;
-    const double engsic = -R4PIE0 * P.alpha * q[9] / SQRPI;
-    const double eng1 = P.scal1 * q[1] + engsic;
-    const double engdum = q[2];
-    const double omg_xx = q[3], omg_yy = q[4], omg_zz = q[5], omg_xy = q[6], omg_xz = q[7], omg_yz = q[8];
-    double st9[9];
-    st9[0] = P.scal1 * (omg_xx + engdum);
-    st9[1] = P.scal1 * omg_xy;
-    st9[2] = P.scal1 * omg_xz;
-    st9[3] = st9[1];
-    st9[4] = P.scal1 * (omg_yy + engdum);
-    st9[5] = P.scal1 * omg_yz;
-    st9[6] = st9[2];
-    st9[7] = st9[5];
-    st9[8] = P.scal1 * (omg_zz + engdum);
-    const double vir1 = -P.scal1 * (omg_xx + omg_yy + omg_zz + 3.0 * engdum);
-    // pair, excluded-pair and qd sums: [energy, virial, xx, xy, xz, yy, yz, zz]
-    const double* pp[3] = {q + 11, q + 19, q + 27};
-    double vir = vir1;
-    for (int t = 0; t < 3; ++t) {
-        const double* r = pp[t];
-        vir += r[1];
-        st9[0] += r[2];
-        st9[1] += r[3];
-        st9[2] += r[4];
-        st9[3] += r[3];
-        st9[4] += r[5];
-        st9[5] += r[6];
-        st9[6] += r[4];
-        st9[7] += r[6];
-        st9[8] += r[7];
-    }
-    double* o = S.out;
-    o[0] = eng1;
-    o[1] = q[11];
-    o[2] = q[19];
-    o[3] = q[0];
-    o[4] = q[0] + eng1 + q[11] + q[19];
-    o[5] = vir;
-    o[6] = vir1;
-    o[7] = q[28];
-    o[8] = st[0];
-    o[9] = st[1];
-    for (int i = 0; i < 9; ++i) o[10 + i] = st9[i];
-    o[19] = (double)E.K;
-    S.ready = true;
-    return 0;
+    return elec_finish(S, st, spflag, err);
 }

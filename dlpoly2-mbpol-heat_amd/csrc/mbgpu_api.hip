@@ -153,6 +153,32 @@ void dfree(T*& p)
     p = nullptr;
 }
 
+// Packed scalars of one complete evaluation (mbgpu_evaluate_dev), in doubles: MB-pol's
+// dimer/trimer counts and flag (4 ints), MB-pol's energy/virial/stress sums (16), the solver
+// state (8), the electrostatics' error flag, their scalar sums (41), the particle mesh's input
+// flags (2 ints), the water terms' sums (18), the heat-flux sums (8).  Integers are exact.
+constexpr int PK_CNT = 0, PK_MB = 4, PK_ST = 20, PK_ERR = 28, PK_EL = 29, PK_SPF = 70, PK_WT = 72, PK_J = 90,
+              PK_N = 98;
+
+__global__ void k_pack_eval(const int* __restrict__ cnt, const double* __restrict__ mb,
+                            const double* __restrict__ st, const int* __restrict__ err,
+                            const double* __restrict__ els, const int* __restrict__ spf,
+                            const double* __restrict__ ws, const double* __restrict__ jd, double* __restrict__ out)
+{
+    const int t = threadIdx.x;
+    if (t >= PK_N) return;
+    double v;
+    if (t < PK_MB) v = (double)cnt[t];
+    else if (t < PK_ST) v = mb[t - PK_MB];
+    else if (t < PK_ERR) v = st[t - PK_ST];
+    else if (t < PK_EL) v = (double)err[0];
+    else if (t < PK_SPF) v = els[t - PK_EL];
+    else if (t < PK_WT) v = spf ? (double)spf[t - PK_SPF] : 0.0;
+    else if (t < PK_J) v = ws[t - PK_WT];
+    else v = jd[t - PK_J];
+    out[t] = v;
+}
+
 bool make_geom(int imcon, const double* cell, CellGeom& g)
 {
     if (imcon < 0 || imcon > 3) return false;
@@ -211,6 +237,18 @@ struct mbgpu_ctx {
     double *ring_m = nullptr, *ring_blk = nullptr, *ring_out = nullptr;
     long ring_m_cap = 0, ring_blk_cap = 0;
     SpmeBuffers sp;                   // particle-mesh Ewald reciprocal space
+    // a complete evaluation (mbgpu_evaluate_dev) captured as three graphs (the electrostatics
+    // on their stream; MB-pol and the water terms; the per-atom sums, heat flux and packed
+    // scalars after the join), replayed while their key is unchanged.  One graph with both
+    // branches runs them one after the other on this runtime; two launches on two streams
+    // overlap.
+    hipGraphExec_t fexec[3] = {nullptr, nullptr, nullptr};
+    std::vector<unsigned char> fkey;
+    long fcaptures = 0, freplays = 0;
+    // every scalar of a complete evaluation, packed on the device and read back in one copy
+    // into pinned host memory
+    double* dpk = nullptr;
+    double* hpk = nullptr;
 };
 
 namespace {
@@ -250,6 +288,8 @@ int alloc_fixed(mbgpu_ctx* c)
     HIPCHK(dalloc(&c->dw, (size_t)9 * na));
     HIPCHK(dalloc(&c->dsew, (size_t)9 * na));
     HIPCHK(dalloc(&c->dout, 16));
+    HIPCHK(dalloc(&c->dpk, PK_N));
+    HIPCHK(hipHostMalloc((void**)&c->hpk, sizeof(double) * PK_N, hipHostMallocDefault));
     // coefficient streams
     const int l2 = pip2b_stream_len(), l3 = pip3b_stream_len();
     HIPCHK(dalloc(&B.cs2, l2));
@@ -358,11 +398,9 @@ int run_clusters(mbgpu_ctx* c, double* fx, double* fy, double* fz)
     return 0;
 }
 
-int finish_scalars(mbgpu_ctx* c, double* engsrp, double* virsrp, double* stress)
+// MB-pol's energy, virial and stress (+=) from the 16 reduced sums s (2B then 3B)
+void add_mbpol_scalars(const double s[16], double* engsrp, double* virsrp, double* stress)
 {
-    double s[16];
-    HIPCHK(hipMemcpyAsync(s, c->B.scal, sizeof s, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
     if (engsrp) *engsrp += s[0] + s[8];
     if (virsrp) *virsrp += s[1] + s[9];
     if (stress) {
@@ -377,6 +415,14 @@ int finish_scalars(mbgpu_ctx* c, double* engsrp, double* virsrp, double* stress)
         stress[7] += t[4];
         stress[8] += t[5];
     }
+}
+
+int finish_scalars(mbgpu_ctx* c, double* engsrp, double* virsrp, double* stress)
+{
+    double s[16];
+    HIPCHK(hipMemcpyAsync(s, c->B.scal, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    add_mbpol_scalars(s, engsrp, virsrp, stress);
     return 0;
 }
 
@@ -435,7 +481,7 @@ void add_terms(mbgpu_ctx* c, const double*& x, const double* const* terms, int n
             x = terms[0];
             return;
         }
-        (void)hipMemcpyAsync(scratch, terms[0], sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream);
+        el_launch_copy(scratch, terms[0], n, c->stream);
         x = scratch;
         k = 1;
     }
@@ -508,6 +554,115 @@ double* noalloc_scratch(mbgpu_ctx* c)
 }  // namespace
 
 // ====================================================================== C ABI =======
+namespace {
+
+// The device work of one complete evaluation (mbgpu_evaluate_dev), kernels only, in three
+// parts.  The electrostatics, on their stream b: positions in, start dipoles, gb solver
+// iterations, the final pass, M sites and dipoles out.
+int full_enqueue_elec(mbgpu_ctx* c, const mbgpu_elec_params* ep, int gb, double* x, double* y, double* z,
+                      double* dip, hipStream_t b)
+{
+    ElecBuffers& E = c->el.E;
+    const int nw = c->B.nw, n = 4 * nw, nr = 3 * nw;
+    el_launch_copy(E.x, x, n, b);
+    el_launch_copy(E.y, y, n, b);
+    el_launch_copy(E.z, z, n, b);
+    if (dip) el_launch_copy(E.dip, dip, 3L * n, b);
+    else el_launch_zero(E.dip, 3L * n, b);
+    int rc = elec_enqueue_full(c->el, ep->distribute_msite, ep->solver, gb, b);
+    if (rc) return fail(rc, "electrostatics kernels failed: %s", hipGetErrorString((hipError_t)rc));
+    el_launch_copy(x + nr, E.x + nr, nw, b);
+    el_launch_copy(y + nr, E.y + nr, nw, b);
+    el_launch_copy(z + nr, E.z + nr, nw, b);
+    if (dip) el_launch_copy(dip, E.dip, 3L * n, b);
+    return 0;
+}
+
+// MB-pol's lists and clusters and the water terms, on the context's stream a.
+int full_enqueue_mbpol_water(mbgpu_ctx* c, double* x, double* y, double* z, hipStream_t a)
+{
+    MbBuffers& B = c->B;
+    int rc = mb_launch_lists(B, c->geom, a);
+    if (!rc) rc = mb_launch_dimers(B, B.counts, (int)B.dcap, c->geom, true, a);
+    if (!rc) rc = mb_launch_trimers(B, B.counts + 1, (int)B.tcap, (int)((B.dcap + 127) / 128), c->geom, true, a);
+    if (!rc) rc = mb_launch_accumulate(B, nullptr, nullptr, nullptr, a);
+    if (!rc) rc = water_enqueue(c->wt, x, y, z, nullptr, nullptr, nullptr, a);
+    if (rc) return fail(rc, "MB-pol or water-term kernels failed: %s", hipGetErrorString((hipError_t)rc));
+    return 0;
+}
+
+// After both: the per-atom terms of the electrostatics, then of the water terms
+// (with_model_terms' order), the heat flux over every term, and every scalar packed into dpk.
+int full_enqueue_tail(mbgpu_ctx* c, const double* vx, const double* vy, const double* vz, const double* ek,
+                      const int* spflag, hipStream_t a)
+{
+    MbBuffers& B = c->B;
+    ElecBuffers& E = c->el.E;
+    const long nr = 3L * B.nw;
+    const double* ep_ = nullptr;
+    const double* we_ = nullptr;
+    const double* se_ = nullptr;
+    const double* es[2] = {E.eout, c->wt.B.eout};
+    const double* ws[2] = {E.wout, c->wt.B.wout};
+    const double* ss[1] = {E.sout};
+    add_terms(c, ep_, es, 2, c->dep, nr);
+    add_terms(c, we_, ws, 2, c->dw, 9L * nr);
+    add_terms(c, se_, ss, 1, c->dsew, 9L * nr);
+    int rc = mb_launch_heat_flux((int)nr, vx, vy, vz, ek, B.atomic_energy, ep_, B.heat_w, we_, se_, B.blk,
+                                 B.nblk_cap, c->dout, a);
+    if (rc) return fail(rc, "heat-flux kernels failed");
+    hipLaunchKernelGGL(k_pack_eval, dim3(1), dim3(128), 0, a, B.counts, B.scal, E.state, E.err, E.scal, spflag,
+                       c->wt.B.scal, c->dout, c->dpk);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// One part captured on stream s into ex (kernel nodes only); 0 or an error code, ex null then.
+template <typename F>
+int capture_part(hipStream_t s, F&& enqueue, hipGraphExec_t& ex)
+{
+    hipGraph_t graph = nullptr;
+    int rc = 0;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = enqueue();
+        const hipError_t e2 = hipStreamEndCapture(s, &graph);
+        if (e2 != hipSuccess) e = e2;
+    }
+    if (e == hipSuccess && !rc && graph) e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess || rc || !ex) {
+        (void)hipGetLastError();
+        if (ex) (void)hipGraphExecDestroy(ex);
+        ex = nullptr;
+        return rc ? rc : (e != hipSuccess ? (int)e : -1);
+    }
+    return 0;
+}
+
+// The key of a captured complete evaluation: everything its launches were recorded with.
+std::vector<unsigned char> full_key(const mbgpu_ctx* c, const mbgpu_elec_params* ep, int gb, const void* const* ptrs,
+                                    int nptr)
+{
+    std::vector<unsigned char> k;
+    auto add = [&](const void* p, size_t n) {
+        const unsigned char* q = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), q, q + n);
+    };
+    add(&c->B, sizeof c->B);
+    add(&c->geom, sizeof c->geom);
+    add(&c->wt.B, sizeof c->wt.B);
+    add(&c->wt.P, sizeof c->wt.P);
+    add(&c->wt.cells, sizeof c->wt.cells);
+    add(ptrs, sizeof(void*) * nptr);
+    const int v[2] = {ep->distribute_msite, ep->solver};
+    add(v, sizeof v);
+    elec_key_append(c->el, ep->distribute_msite, ep->solver, gb, k);
+    return k;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* mbgpu_last_error(void) { return g_err.c_str(); }
@@ -560,6 +715,8 @@ void mbgpu_destroy(mbgpu_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->el_stream) (void)hipStreamSynchronize(c->el_stream);
+    for (hipGraphExec_t& g : c->fexec)
+        if (g) (void)hipGraphExecDestroy(g);
     MbBuffers& B = c->B;
     dfree(B.mon); dfree(B.wbase); dfree(B.nbr2); dfree(B.cnt2); dfree(B.off2); dfree(B.nbr3); dfree(B.cnt3);
     dfree(B.tcnt); dfree(B.toff); dfree(B.counts); dfree(B.dimers); dfree(B.trimers);
@@ -570,7 +727,8 @@ void mbgpu_destroy(mbgpu_ctx* c)
     cells_free(B.cells);
     dfree(B.xv2f); dfree(B.part2f); dfree(B.cs2f); dfree(B.xv3f); dfree(B.part3f); dfree(B.cs3f);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
-    dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout);
+    dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout); dfree(c->dpk);
+    if (c->hpk) (void)hipHostFree(c->hpk);
     dfree(c->ring_m); dfree(c->ring_blk); dfree(c->ring_out);
     spme_free(c->sp);
     elec_release(c->el);
@@ -802,6 +960,182 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const 
         jpot[a] = out[3 + a];
         jtot[a] = out[a] + out[3 + a];
     }
+    return 0;
+}
+
+int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
+                       const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev, double* y_dev,
+                       double* z_dev, double* dip_dev, const double* vx_dev, const double* vy_dev,
+                       const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT])
+{
+    EntryGuard gate_;
+    if (!c) return fail(-1, "null context");
+    if (!ep || !wp) return fail(-1, "null electrostatics or water-term parameters");
+    if (!out || !listttm2_dev || !x_dev || !y_dev || !z_dev || !vx_dev || !vy_dev || !vz_dev || !e_kin_dev)
+        return fail(-1, "null argument");
+    const int nw = c->B.nw;
+    if (nttm2 != 3 * nw) return fail(-1, "nttm2 = %d does not match the context (3*%d)", nttm2, nw);
+    if (c->B.natms < 4 * nw)
+        return fail(-1, "the context holds %d sites; the electrostatics need 4*nw = %d", c->B.natms, 4 * nw);
+    for (int i = 0; i < MBGPU_EVAL_OUT; ++i) out[i] = 0.0;
+    CellGeom g;
+    if (!make_geom(imcon, cell, g) || !make_geom(imcon, cell, c->geom))
+        return fail(-4, "imcon = %d is not supported by images (reference stops in fixme)", imcon);
+    // host-side set-up of the three parts: tables, k vectors, plans, buffers
+    MbBuffers& B = c->B;
+    B.list = const_cast<int*>(listttm2_dev);
+    B.x = x_dev;
+    B.y = y_dev;
+    B.z = z_dev;
+    c->have_inputs = true;
+    c->mb_ready = false;
+    std::string err;
+    B.cells.V.on = 0;
+    if (imcon != 0 && cells_plan(B.cells, c->geom.c, MBGPU_R2F, nw, CELLS_MIN_WATERS, err))
+        return fail(-1, "%s", err.c_str());
+    c->el.ready = false;
+    if (elec_configure(c->el, nw, g, cell, ep, err)) return fail(-1, "%s", err.c_str());
+    c->wt.ready = false;
+    if (water_configure(c->wt, nw, g, cell, wp, c->stream, err)) return fail(-1, "%s", err.c_str());
+    const int gb = elec_batch(c->el);
+    // one enqueue (a graph replay once captured) when every buffer has been sized by an
+    // evaluation through the separate entries
+    if (B.dcap > 0 && B.nblk_cap > 0 && gb > 0) {
+        const int* spf = c->el.recip && c->el.sp && imcon != 0 ? c->el.sp->flag : nullptr;
+        const void* ptrs[14] = {listttm2_dev, x_dev, y_dev, z_dev, dip_dev, vx_dev, vy_dev, vz_dev, e_kin_dev,
+                                c->dep, c->dw, c->dsew, spf, c->dpk};
+        const char* gv = getenv("MBGPU_GRAPHS");
+        const bool graphs = c->el.graphs && !(gv && atoi(gv) == 0);
+        hipStream_t a = c->stream, b = c->el_stream;
+        auto part_el = [&] { return full_enqueue_elec(c, ep, gb, x_dev, y_dev, z_dev, dip_dev, b); };
+        auto part_mb = [&] { return full_enqueue_mbpol_water(c, x_dev, y_dev, z_dev, a); };
+        auto part_tail = [&] { return full_enqueue_tail(c, vx_dev, vy_dev, vz_dev, e_kin_dev, spf, a); };
+        bool replay = false;
+        if (graphs) {
+            std::vector<unsigned char> key = full_key(c, ep, gb, ptrs, 14);
+            if (!c->fexec[0] || key != c->fkey) {
+                for (hipGraphExec_t& g : c->fexec)
+                    if (g) (void)hipGraphExecDestroy(g);
+                c->fexec[0] = c->fexec[1] = c->fexec[2] = nullptr;
+                c->fkey.clear();
+                capture_gate_begin();
+                int cr = capture_part(b, part_el, c->fexec[0]);
+                if (!cr) cr = capture_part(a, part_mb, c->fexec[1]);
+                if (!cr) cr = capture_part(a, part_tail, c->fexec[2]);
+                capture_gate_end();
+                if (cr) {
+                    // the runtime refused a capture: this context launches kernel by kernel from now on
+                    for (hipGraphExec_t& g : c->fexec)
+                        if (g) (void)hipGraphExecDestroy(g);
+                    c->fexec[0] = c->fexec[1] = c->fexec[2] = nullptr;
+                    c->el.graphs = 0;
+                } else {
+                    c->fkey.swap(key);
+                    c->el.gbatch = gb;
+                    ++c->fcaptures;
+                }
+            }
+            replay = c->fexec[0] != nullptr;
+        }
+        // fork: the electrostatics on b after the set-up enqueued on a; join before the tail
+        HIPCHK(hipEventRecord(c->ev2, a));
+        HIPCHK(hipStreamWaitEvent(b, c->ev2, 0));
+        int rc = 0;
+        if (replay) {
+            HIPCHK(hipGraphLaunch(c->fexec[0], b));
+            HIPCHK(hipEventRecord(c->ev3, b));
+            HIPCHK(hipGraphLaunch(c->fexec[1], a));
+            HIPCHK(hipStreamWaitEvent(a, c->ev3, 0));
+            HIPCHK(hipGraphLaunch(c->fexec[2], a));
+            ++c->freplays;
+        } else {
+            rc = part_el();
+            if (!rc) {
+                HIPCHK(hipEventRecord(c->ev3, b));
+                rc = part_mb();
+            }
+            if (!rc) {
+                HIPCHK(hipStreamWaitEvent(a, c->ev3, 0));
+                rc = part_tail();
+            }
+        }
+        if (rc) {
+            (void)hipStreamSynchronize(b);
+            (void)hipStreamSynchronize(a);
+            return rc;
+        }
+        const double* pk = c->hpk;
+        HIPCHK(hipMemcpyAsync(c->hpk, c->dpk, sizeof(double) * PK_N, hipMemcpyDeviceToHost, a));
+        HIPCHK(hipStreamSynchronize(a));
+        int cnt[4], spflag[2];
+        for (int i = 0; i < 4; ++i) cnt[i] = (int)pk[PK_CNT + i];
+        const double* mb = pk + PK_MB;
+        const double* st = pk + PK_ST;
+        const int herr = (int)pk[PK_ERR];
+        for (int i = 0; i < 41; ++i) c->el.scal[i] = pk[PK_EL + i];
+        spflag[0] = (int)pk[PK_SPF];
+        spflag[1] = (int)pk[PK_SPF + 1];
+        const double* wr = pk + PK_WT;
+        const double* jd = pk + PK_J;
+        // anything out of the ordinary (a list past its capacity or with a flag, neighbour rows
+        // overflowed, a solve longer than the graph holds) goes through the separate entries
+        const bool ok = cnt[2] == 0 && cnt[0] <= B.dcap && cnt[1] <= B.tcap && !herr && st[2] != 0.0;
+        if (ok) {
+            c->nd = cnt[0];
+            c->nt = cnt[1];
+            c->mb_ready = true;
+            add_mbpol_scalars(mb, &out[0], &out[1], &out[2]);
+            c->el.last_iters = (int)st[0];
+            if (elec_finish(c->el, st, spflag, err)) return fail(-1, "%s", err.c_str());
+            const double* o = c->el.out;
+            out[11] = o[4];
+            out[12] = o[5];
+            for (int i = 0; i < 9; ++i) out[13 + i] = o[10 + i];
+            out[22] = o[8];
+            water_finish(c->wt, wr);
+            const double* w = c->wt.out;
+            for (int i = 0; i < 4; ++i) {
+                out[23 + i] = w[2 * i];
+                out[27 + i] = w[2 * i + 1];
+            }
+            for (int i = 0; i < 9; ++i) out[31 + i] = c->wt.stress[i];
+            for (int a2 = 0; a2 < 3; ++a2) {
+                out[40 + a2] = jd[a2];
+                out[43 + a2] = jd[3 + a2];
+                out[46 + a2] = jd[a2] + jd[3 + a2];
+            }
+            out[49] = 1.0;
+            return 0;
+        }
+    }
+    // the separate entries: they size every buffer, repeat overflowed builds, continue
+    // unconverged solves and report errors
+    int niter = 0;
+    int rc = mbgpu_mbpol_forces_dev(c, imcon, cell, nttm2, listttm2_dev, x_dev, y_dev, z_dev, nullptr, nullptr,
+                                    nullptr, &out[0], &out[1], &out[2]);
+    if (rc) return rc;
+    rc = mbgpu_polar_electrostatics_dev(c, imcon, cell, ep, x_dev, y_dev, z_dev, dip_dev, nullptr, &out[11],
+                                        &out[12], &out[13], &niter);
+    if (rc) return rc;
+    out[22] = niter;
+    double ew[4], vw[4];
+    rc = mbgpu_water_terms_dev(c, imcon, cell, wp, x_dev, y_dev, z_dev, nullptr, nullptr, nullptr, ew, vw,
+                               &out[31]);
+    if (rc) return rc;
+    for (int i = 0; i < 4; ++i) {
+        out[23 + i] = ew[i];
+        out[27 + i] = vw[i];
+    }
+    return mbgpu_heat_flux_dev(c, vx_dev, vy_dev, vz_dev, e_kin_dev, nullptr, nullptr, nullptr, &out[40], &out[43],
+                               &out[46]);
+}
+
+int mbgpu_get_evaluate_graph_counts(mbgpu_ctx* c, long counts[2])
+{
+    EntryGuard gate_;
+    if (!c || !counts) return fail(-1, "null context or output");
+    counts[0] = c->fcaptures;
+    counts[1] = c->freplays;
     return 0;
 }
 

@@ -241,6 +241,13 @@ int elec_configure(ElecState& S, int nw, const CellGeom& g, const double* cell, 
 int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStream_t s, std::string& err);
 void elec_release(ElecState& S);
 void elec_drop_graph(ElecState& S);
+// pieces of an evaluation for a caller that enqueues it with other work (one graph of a
+// complete evaluation): the solver iterations a capture holds, the device work, the key
+// bytes of that work, and the host side of the end
+int elec_batch(const ElecState& S);
+int elec_enqueue_full(ElecState& S, int distribute, int solver, int gb, hipStream_t s);
+void elec_key_append(const ElecState& S, int distribute, int solver, int gb, std::vector<unsigned char>& k);
+int elec_finish(ElecState& S, const double st[8], const int spflag[2], std::string& err);
 // A graph capture holds the library's entry gate alone (mbgpu_api.hip): call around it.
 void capture_gate_begin();
 void capture_gate_end();
@@ -290,6 +297,9 @@ int water_configure(WaterState& S, int nw, const CellGeom& g, const double* cell
                     hipStream_t s, std::string& err);
 int water_evaluate(WaterState& S, const double* x, const double* y, const double* z, double* fx, double* fy,
                    double* fz, hipStream_t s, std::string& err);
+int water_enqueue(WaterState& S, const double* x, const double* y, const double* z, double* fx, double* fy,
+                  double* fz, hipStream_t s);
+void water_finish(WaterState& S, const double r[18]);
 void water_release(WaterState& S);
 
 // ---- smooth particle-mesh Ewald, charges and dipoles (spme.hip) ----

@@ -603,22 +603,41 @@ int water_configure(WaterState& S, int nw, const CellGeom& g, const double* cell
     return 0;
 }
 
+// The device work of an evaluation (kernels only): monomers, link cells, dispersion rows,
+// reductions into B.scal.
+int water_enqueue(WaterState& S, const double* x, const double* y, const double* z, double* fx, double* fy,
+                  double* fz, hipStream_t s)
+{
+    WaterBuffers& B = S.B;
+    const WaterDev& P = S.P;
+    hipLaunchKernelGGL(k_wt_monomer, dim3((P.nw + 3) / 4), dim3(256), 0, s, P, x, y, z, B.psc, B.pcm, B.pidx,
+                       B.pidxm, B.mono, B.wsc);
+    if (P.cells.on) {
+        const int rc = cells_build(S.cells, P.nr, x, y, z, 1, s);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_wt_disp, dim3((P.nr + 3) / 4), dim3(256), 0, s, P, x, y, z, B.vvv, B.ggg, B.mono, fx, fy,
+                       fz, B.fw, B.eout, B.wout, B.asc);
+    hipLaunchKernelGGL(k_wt_reduce, dim3(1), dim3(1024), 0, s, P.nw, P.nr, B.wsc, B.asc, B.scal);
+    return (int)hipGetLastError();
+}
+
 int water_evaluate(WaterState& S, const double* x, const double* y, const double* z, double* fx, double* fy,
                    double* fz, hipStream_t s, std::string& err)
 {
     WaterBuffers& B = S.B;
-    const WaterDev& P = S.P;
     S.ready = false;
-    hipLaunchKernelGGL(k_wt_monomer, dim3((P.nw + 3) / 4), dim3(256), 0, s, P, x, y, z, B.psc, B.pcm, B.pidx,
-                       B.pidxm, B.mono, B.wsc);
-    if (P.cells.on) WCHK((hipError_t)cells_build(S.cells, P.nr, x, y, z, 1, s));
-    hipLaunchKernelGGL(k_wt_disp, dim3((P.nr + 3) / 4), dim3(256), 0, s, P, x, y, z, B.vvv, B.ggg, B.mono, fx, fy,
-                       fz, B.fw, B.eout, B.wout, B.asc);
-    hipLaunchKernelGGL(k_wt_reduce, dim3(1), dim3(1024), 0, s, P.nw, P.nr, B.wsc, B.asc, B.scal);
-    WCHK(hipGetLastError());
+    WCHK((hipError_t)water_enqueue(S, x, y, z, fx, fy, fz, s));
     double r[18];
     WCHK(hipMemcpyAsync(r, B.scal, sizeof r, hipMemcpyDeviceToHost, s));
     WCHK(hipStreamSynchronize(s));
+    water_finish(S, r);
+    return 0;
+}
+
+// The host side of an evaluation's end: S.out and S.stress from the 18 reduced sums.
+void water_finish(WaterState& S, const double r[18])
+{
     S.out[0] = r[0];
     S.out[1] = r[1];
     S.out[2] = r[2];
@@ -629,5 +648,4 @@ int water_evaluate(WaterState& S, const double* x, const double* y, const double
     const double full[9] = {st[0], st[1], st[2], st[1], st[3], st[4], st[2], st[4], st[5]};
     for (int q = 0; q < 9; ++q) S.stress[q] = full[q];
     S.ready = true;
-    return 0;
 }

@@ -27,7 +27,7 @@ ENGUNIT = 418.4
 
 class Slot:
     def __init__(self, box, precision=0, solver=1, elec=None, water=None, device="cuda:0", overlap=True,
-                 recip=None):
+                 recip=None, fused=True):
         import torch
         self.torch = torch
         self.dev = torch.device(device)
@@ -40,6 +40,10 @@ class Slot:
         self.precision = precision
         self.solver = solver
         self.overlap = bool(overlap)
+        # fused: one library call per evaluation (mbgpu_evaluate_dev: every part, the two
+        # streams and, once the buffers are sized, one captured graph); otherwise the parts
+        # are called in turn (the electrostatics from a helper thread when overlap is set)
+        self.fused = bool(fused)
         self._helper = None
         self.set_params(elec or _ep.for_box(box.L), water or _wp.for_box(box.L))
         self.stream = torch.cuda.Stream(device=self.dev)
@@ -89,6 +93,17 @@ class Slot:
         the parts share no buffers and the heat flux starts after both have returned."""
         b, c = self.box, self.ctx
         x, y, z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+        if self.fused:
+            o = c.evaluate_dev(b.imcon, b.cell, 3 * b.nw, self.lst.data_ptr(), self.eparams, self.wparams, x, y, z,
+                               None, self.v[0].data_ptr(), self.v[1].data_ptr(), self.v[2].data_ptr(),
+                               self.ek.data_ptr())
+            e_mb, v_mb, s_mb = float(o[0]), float(o[1]), o[2:11].copy()
+            e_el, v_el, s_el, it = float(o[11]), float(o[12]), o[13:22].copy(), int(o[22])
+            e_w, v_w, s_w = [float(t) for t in o[23:27]], [float(t) for t in o[27:31]], o[31:40].copy()
+            jk, jp, jt = o[40:43].copy(), o[43:46].copy(), o[46:49].copy()
+            return dict(eng=e_mb + e_el + float(np.sum(e_w)), eng_mbpol=e_mb, eng_elec=e_el, eng_water=np.array(e_w),
+                        vir_water=np.array(v_w), vir=v_mb + v_el + float(np.sum(v_w)), stress=s_mb + s_el + s_w,
+                        dipole_iterations=it, jkin=jk, jpot=jp, jtot=jt)
         (e_el, v_el, s_el, it), ((e_mb, v_mb, s_mb), (e_w, v_w, s_w)) = self.split(
             lambda: c.polar_electrostatics_dev(b.imcon, b.cell, self.eparams, x, y, z, None),
             lambda: (c.mbpol_forces_dev(b.imcon, b.cell, self.lst.data_ptr(), x, y, z),

@@ -46,6 +46,11 @@ _SIGS = {
     "mbgpu_set_elec_reciprocal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "mbgpu_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "mbgpu_get_graph_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mbgpu_get_evaluate_graph_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "mbgpu_evaluate_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mbgpu_get_lists": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _ip,
                                        ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "mbgpu_heat_flux": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -305,6 +310,25 @@ class Context:
         """Graph replay of the electrostatics: True (default) = capture one evaluation's kernels
         once the iteration count is known and replay them; False = launch kernel by kernel."""
         _check(load_library().mbgpu_set_graphs(self._h, 1 if on else 0), "mbgpu_set_graphs")
+
+    def evaluate_dev(self, imcon, cell, nttm2, list_ptr, eparams, wparams, x_ptr, y_ptr, z_ptr, dip_ptr, vx_ptr,
+                     vy_ptr, vz_ptr, ek_ptr):
+        """mbgpu_evaluate_dev: one complete evaluation (MB-pol, electrostatics, water terms, heat
+        flux) of device arrays; dip_ptr may be None (dipoles from zero).  Returns the 64-value
+        output vector (see the header for its layout)."""
+        out = np.zeros(64)
+        cell = _f64(cell, 9)
+        rc = load_library().mbgpu_evaluate_dev(self._h, int(imcon), _p(cell), int(nttm2), list_ptr,
+                                               ctypes.byref(eparams), ctypes.byref(wparams), x_ptr, y_ptr, z_ptr,
+                                               dip_ptr, vx_ptr, vy_ptr, vz_ptr, ek_ptr, _p(out))
+        _check(rc, "mbgpu_evaluate_dev")
+        return out
+
+    def evaluate_graph_counts(self):
+        """(complete evaluations captured as graphs, replays) in this context."""
+        c = (ctypes.c_long * 2)()
+        _check(load_library().mbgpu_get_evaluate_graph_counts(self._h, c), "mbgpu_get_evaluate_graph_counts")
+        return int(c[0]), int(c[1])
 
     def graph_counts(self):
         """(graphs captured, replays launched) by this context's electrostatics."""

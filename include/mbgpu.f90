@@ -229,6 +229,23 @@ module mbgpu
       real(c_double) :: eng(4), vir(4), stress(9)
       integer(c_int) :: mbgpu_water_terms_dev
     end function
+    function mbgpu_evaluate_dev(ctx, imcon, cell, nttm2, listttm2_dev, ep, wp, x_dev, y_dev, z_dev, dip_dev, &
+                                vx_dev, vy_dev, vz_dev, e_kin_dev, out) bind(c, name="mbgpu_evaluate_dev")
+      import :: c_int, c_double, c_ptr, mbgpu_elec_params, mbgpu_water_params
+      type(c_ptr), value :: ctx, listttm2_dev, x_dev, y_dev, z_dev, dip_dev, vx_dev, vy_dev, vz_dev, e_kin_dev
+      integer(c_int), value :: imcon, nttm2
+      real(c_double), intent(in) :: cell(9)
+      type(mbgpu_elec_params), intent(in) :: ep
+      type(mbgpu_water_params), intent(in) :: wp
+      real(c_double), intent(out) :: out(64)
+      integer(c_int) :: mbgpu_evaluate_dev
+    end function
+    function mbgpu_get_evaluate_graph_counts(ctx, counts) bind(c, name="mbgpu_get_evaluate_graph_counts")
+      import :: c_int, c_long, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_long), intent(out) :: counts(2)
+      integer(c_int) :: mbgpu_get_evaluate_graph_counts
+    end function
     function mbgpu_get_water_details(ctx, e_real, w_real, f_real) bind(c, name="mbgpu_get_water_details")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, e_real, w_real, f_real

@@ -380,6 +380,37 @@ int mbgpu_water_terms_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], const
  * any may be NULL): per-atom energy, force-matrix contraction W_i[a][b], forces. */
 int mbgpu_get_water_details(mbgpu_ctx* ctx, double* e_real, double* w_real, double* f_real);
 
+/* One complete evaluation with device arrays: MB-pol (mbgpu_mbpol_forces_dev), the
+ * polarizable electrostatics (mbgpu_polar_electrostatics_dev, dipoles from dip_dev or from
+ * zero when it is NULL, written back when given), the water terms (mbgpu_water_terms_dev) and
+ * the heat flux over all of them (mbgpu_heat_flux_dev with no extras), in one call: the
+ * electrostatics run on the context's second stream beside MB-pol and the water terms, and
+ * the results come back in one read.  Forces stay in the context (mbgpu_get_decomposition,
+ * mbgpu_get_elec_details, mbgpu_get_water_details).  Once an evaluation through the separate
+ * entries has sized the buffers (the first call makes one), the device work is three captured
+ * HIP graphs (the electrostatics on the second stream; MB-pol and the water terms on the
+ * first; the per-atom sums, the heat flux and the packed scalars after both) replayed while
+ * the parameters, the cell and the buffers stay the same (unless mbgpu_set_graphs(ctx, 0));
+ * a list past its capacity, overflowed neighbour rows or a solve
+ * longer than the graph holds repeats the evaluation through the separate entries.  Results
+ * are bit-identical to the separate entries.  out[MBGPU_EVAL_OUT]:
+ *   [0] MB-pol energy, [1] its virial, [2..10] its stress;
+ *   [11] electrostatic energy (engcpe), [12] its virial, [13..21] its stress, [22] solver
+ *        iterations;
+ *   [23..26] water-term energies (bonds, angles/PS, dispersion, long-range tail),
+ *   [27..30] their virials, [31..39] their stress;
+ *   [40..42] J_kin, [43..45] J_pot, [46..48] J_tot; [49] 1 when one enqueue or replay ran,
+ *   0 when the separate entries did.
+ * No counterpart in the reference (its force routine calls the parts in turn, forces.f). */
+#define MBGPU_EVAL_OUT 64
+int mbgpu_evaluate_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
+                       const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev, double* y_dev,
+                       double* z_dev, double* dip_dev, const double* vx_dev, const double* vy_dev,
+                       const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT]);
+/* counts[0]: complete evaluations captured as graphs in this context, counts[1]: replays. */
+int mbgpu_get_evaluate_graph_counts(mbgpu_ctx* ctx, long counts[2]);
+
+
 /* Timing helper: runs the MB-pol force evaluation `reps` times on device-resident inputs
  * already uploaded by a previous mbgpu_mbpol_forces(_dev) call and returns the mean
  * milliseconds per evaluation, measured with HIP events on the library's stream. */

@@ -12,7 +12,11 @@ replay must give exactly what launching the same kernels one by one gives:
   and an isolated cluster, with both solvers;
 * a graph holding fewer iterations than the solve needs (MBGPU_GRAPH_BATCH): the solve
   continues kernel by kernel and the final pass runs again, still bit-identical;
-* a change of parameters captures a new graph.
+* a change of parameters captures a new graph;
+* the complete evaluation in one call (mbgpu_evaluate_dev, engine.Slot's default: MB-pol,
+  electrostatics, water terms and heat flux enqueued together and, from the second call on,
+  one captured graph) against the parts called one by one, bit for bit, with graphs and
+  without, and its fall-back to the separate entries when a graph holds too few iterations.
 """
 import importlib
 
@@ -62,9 +66,9 @@ def test_replays_equal_kernel_by_kernel(pkg, mods, kind, solver):
     engine, wb = mods
     boxes, e, w = configurations(pkg, wb, kind)
     recip = ("spme", 6, (32, 32, 32)) if kind == "mesh" else None
-    eager = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip)
+    eager = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip, fused=False)
     eager.ctx.set_graphs(False)
-    graph = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip)
+    graph = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip, fused=False)
     print(f"graph replays: {kind}, solver {solver}, {len(boxes)} configurations", flush=True)
     scratch = torch.zeros(1 << 16, dtype=torch.float64, device="cuda")
     for rep in range(2):
@@ -87,10 +91,10 @@ def test_replays_equal_kernel_by_kernel(pkg, mods, kind, solver):
 def test_short_graph_continues_kernel_by_kernel(pkg, mods, monkeypatch):
     engine, wb = mods
     boxes, e, w = configurations(pkg, wb, "ksum")
-    eager = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    eager = engine.Slot(boxes[0], solver=1, elec=e, water=w, fused=False)
     eager.ctx.set_graphs(False)
     monkeypatch.setenv("MBGPU_GRAPH_BATCH", "3")
-    graph = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    graph = engine.Slot(boxes[0], solver=1, elec=e, water=w, fused=False)
     print("graph of 3 solver iterations, continued kernel by kernel", flush=True)
     for b in boxes[:3]:
         eager.load(b)
@@ -105,9 +109,9 @@ def test_new_parameters_capture_again(pkg, mods):
     engine, wb = mods
     boxes, e, w = configurations(pkg, wb, "ksum")
     e2 = pkg.elec_params.ElecParams(rcut=5.5, alpha=0.6, kmax=(10, 10, 10))
-    eager = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    eager = engine.Slot(boxes[0], solver=1, elec=e, water=w, fused=False)
     eager.ctx.set_graphs(False)
-    graph = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    graph = engine.Slot(boxes[0], solver=1, elec=e, water=w, fused=False)
     print("graph replay across a change of electrostatics parameters", flush=True)
     for params in (e, e2, e):
         eager.set_params(params, w)
@@ -121,3 +125,56 @@ def test_new_parameters_capture_again(pkg, mods):
     assert 3 <= caps <= 6 and reps == 8
     with pytest.raises(pkg.MbgpuError, match="graphs"):
         pkg.mbgpu._check(pkg.mbgpu.load_library().mbgpu_set_graphs(graph.ctx._h, 2), "mbgpu_set_graphs")
+
+
+@pytest.mark.parametrize("kind", ["ksum", "mesh", "cluster"])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_complete_evaluation_in_one_call(pkg, mods, kind, solver, monkeypatch):
+    import torch
+    engine, wb = mods
+    boxes, e, w = configurations(pkg, wb, kind)
+    recip = ("spme", 6, (32, 32, 32)) if kind == "mesh" else None
+    parts = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip, fused=False)
+    parts.ctx.set_graphs(False)
+    one = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip)
+    print(f"one-call evaluation: {kind}, solver {solver}", flush=True)
+    scratch = torch.zeros(1 << 16, dtype=torch.float64, device="cuda")
+    for rep in range(2):
+        for b in boxes:
+            parts.load(b)
+            one.load(b)
+            ra = parts.evaluate()
+            scratch.add_(1.0)
+            torch.cuda.synchronize()
+            rb = one.evaluate()
+            same(ra, parts, rb, one)
+    caps, reps = one.ctx.evaluate_graph_counts()
+    # the first call sizes the buffers through the separate entries; every later one replays
+    assert caps >= 1 and reps == 2 * len(boxes) - 1
+    print(f"{kind}, solver {solver}: {caps} capture(s), {reps} replays of the complete evaluation", flush=True)
+    # the same single call without graphs (one enqueue, one read-back)
+    monkeypatch.setenv("MBGPU_GRAPHS", "0")
+    plain = engine.Slot(boxes[0], solver=solver, elec=e, water=w, recip=recip)
+    for b in boxes[:3]:
+        parts.load(b)
+        plain.load(b)
+        ra, rb = parts.evaluate(), plain.evaluate()
+        same(ra, parts, rb, plain)
+    assert plain.ctx.evaluate_graph_counts() == (0, 0)
+
+
+def test_one_call_falls_back_when_the_graph_is_short(pkg, mods, monkeypatch):
+    engine, wb = mods
+    boxes, e, w = configurations(pkg, wb, "ksum")
+    parts = engine.Slot(boxes[0], solver=1, elec=e, water=w, fused=False)
+    parts.ctx.set_graphs(False)
+    monkeypatch.setenv("MBGPU_GRAPH_BATCH", "3")
+    one = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    print("one-call evaluation with a 3-iteration graph: the separate entries finish", flush=True)
+    for b in boxes[:3]:
+        parts.load(b)
+        one.load(b)
+        ra, rb = parts.evaluate(), one.evaluate()
+        assert ra["dipole_iterations"] > 3
+        same(ra, parts, rb, one)
+    assert one.ctx.evaluate_graph_counts()[1] == 2
