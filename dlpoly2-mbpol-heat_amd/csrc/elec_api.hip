@@ -113,7 +113,7 @@ void free_buffers(ElecBuffers& E)
     dfree(E.vk); dfree(E.vr); dfree(E.vs); dfree(E.vc); dfree(E.vdck); dfree(E.vdcr); dfree(E.vesp);
     dfree(E.fk); dfree(E.f2); dfree(E.f3); dfree(E.fsite); dfree(E.e1); dfree(E.e2); dfree(E.e3); dfree(E.ep);
     dfree(E.s1); dfree(E.w2); dfree(E.w3); dfree(E.fmsum); dfree(E.fmx); dfree(E.eout); dfree(E.wout);
-    dfree(E.sout); dfree(E.blk); dfree(E.scal); dfree(E.err);
+    dfree(E.sout); dfree(E.blk); dfree(E.scal); dfree(E.err); dfree(E.pk);
     E.n = E.capA = E.capP = E.K = E.nrun = 0;
 }
 
@@ -148,6 +148,8 @@ int alloc_sites(ElecState& S, int nw, std::string& err)
     ECHK(dalloc(&E.fmsum, (size_t)3 * n)); ECHK(dalloc(&E.fmx, (size_t)9 * n));
     ECHK(dalloc(&E.eout, nreal)); ECHK(dalloc(&E.wout, (size_t)9 * nreal)); ECHK(dalloc(&E.sout, (size_t)9 * nreal));
     ECHK(dalloc(&E.scal, 64)); ECHK(dalloc(&E.err, 4)); ECHK(dalloc(&S.rcell, 9));
+    ECHK(dalloc(&E.pk, 64));
+    if (!S.hpk) ECHK(hipHostMalloc((void**)&S.hpk, sizeof(double) * 64, hipHostMallocDefault));
     ECHK(hipMemset(E.zero_state, 0, 8 * sizeof(double)));
     ECHK(hipMemset(E.w2, 0, sizeof(double) * 9 * n));
     ECHK(hipMemset(E.fmsum, 0, sizeof(double) * 3 * n));
@@ -416,6 +418,8 @@ void elec_release(ElecState& S)
     elec_drop_graph(S);
     free_buffers(S.E);
     dfree(S.rcell);
+    if (S.hpk) (void)hipHostFree(S.hpk);
+    S.hpk = nullptr;
     if (S.sp) {
         spme_free(*S.sp);
         delete S.sp;
@@ -724,6 +728,19 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
     double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int herr = 0;
     bool need_final = true;
+    // every read-back is one copy of the packed scalars into pinned memory (the solver state,
+    // the row-overflow flag, the reduced scalars, the mesh's input flags); `pack` launches
+    // the pack kernel first (a graph holds its own)
+    const int* spf = sp && sp->B ? S.sp->flag : nullptr;
+    auto readback = [&](bool pack) -> hipError_t {
+        if (pack) el_launch_pack(E.state, E.err, E.scal, spf, E.pk, s);
+        hipError_t e = hipMemcpyAsync(S.hpk, E.pk, sizeof(double) * EL_PK_N, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        for (int i = 0; i < 8; ++i) st[i] = S.hpk[EL_PK_ST + i];
+        herr = (int)S.hpk[EL_PK_ERR];
+        return hipSuccess;
+    };
     const int gb = elec_batch(S);
     bool from_graph = false;
     // MBGPU_GRAPHS=0 launches kernel by kernel whatever the context's setting (a test aid)
@@ -740,6 +757,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
             int rc = 0;
             if (e == hipSuccess) {
                 rc = elec_enqueue_full(S, distribute, solver, gb, s);
+                if (!rc) el_launch_pack(E.state, E.err, E.scal, spf, E.pk, s);
                 e = hipStreamEndCapture(s, &graph);
             }
             capture_gate_end();
@@ -759,9 +777,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         if (S.gexec) {
             ECHK(hipGraphLaunch(S.gexec, s));
             ++S.greplays;
-            ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
-            ECHK(hipMemcpyAsync(&herr, E.err, sizeof(int), hipMemcpyDeviceToHost, s));
-            ECHK(hipStreamSynchronize(s));
+            ECHK(readback(false));
             from_graph = !herr;
             need_final = !(st[2] != 0.0);
         }
@@ -802,9 +818,7 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
         }
         bool overflow = false;
         for (;;) {
-            ECHK(hipMemcpyAsync(st, E.state, sizeof st, hipMemcpyDeviceToHost, s));
-            ECHK(hipMemcpyAsync(&herr, E.err, sizeof(int), hipMemcpyDeviceToHost, s));
-            ECHK(hipStreamSynchronize(s));
+            ECHK(readback(true));
             if (herr) {
                 overflow = true;
                 break;
@@ -844,9 +858,13 @@ int elec_evaluate(ElecState& S, int max_iter, int distribute, int solver, hipStr
             return rc;
         }
     }
-    ECHK(hipMemcpyAsync(S.scal, E.scal, sizeof(double) * 41, hipMemcpyDeviceToHost, s));
-    int spflag[2] = {0, 0};
-    if (sp && sp->B) ECHK(hipMemcpyAsync(spflag, S.sp->flag, sizeof spflag, hipMemcpyDeviceToHost, s));
-    ECHK(hipStreamSynchronize(s));
+    if (need_final) {
+        // the scalars of the final pass (a graph that converged read them with its state)
+        const double keep[8] = {st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7]};
+        ECHK(readback(true));
+        for (int i = 0; i < 8; ++i) st[i] = keep[i];
+    }
+    for (int i = 0; i < 41; ++i) S.scal[i] = S.hpk[EL_PK_SCAL + i];
+    const int spflag[2] = {(int)S.hpk[EL_PK_SPF], (int)S.hpk[EL_PK_SPF + 1]};
     return elec_finish(S, st, spflag, err);
 }

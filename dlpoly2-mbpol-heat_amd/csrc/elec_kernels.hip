@@ -2499,6 +2499,25 @@ __global__ void k_el_copy(double* __restrict__ dst, const double* __restrict__ s
 
 // dst[i] = src[i], i < n (a kernel rather than a copy call, so that a captured graph holds
 // only kernel nodes)
+__global__ void k_el_pack(const double* __restrict__ state, const int* __restrict__ err,
+                          const double* __restrict__ scal, const int* __restrict__ spflag, double* __restrict__ out)
+{
+    const int t = threadIdx.x;
+    if (t >= EL_PK_N) return;
+    double v;
+    if (t < EL_PK_ERR) v = state[t - EL_PK_ST];
+    else if (t < EL_PK_SCAL) v = (double)err[0];
+    else if (t < EL_PK_SPF) v = scal[t - EL_PK_SCAL];
+    else v = spflag ? (double)spflag[t - EL_PK_SPF] : 0.0;
+    out[t] = v;
+}
+
+void el_launch_pack(const double* state, const int* err, const double* scal, const int* spflag, double* out,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_el_pack, dim3(1), dim3(64), 0, s, state, err, scal, spflag, out);
+}
+
 void el_launch_copy(double* dst, const double* src, long n, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_el_copy, dim3(ceil_div(n, 256)), dim3(256), 0, s, dst, src, n);

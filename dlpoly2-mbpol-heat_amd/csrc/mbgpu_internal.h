@@ -183,8 +183,9 @@ struct ElecBuffers {
     double *eout = nullptr, *wout = nullptr, *sout = nullptr;  // [nreal], [9 nreal] x2: per real atom
     double* blk = nullptr;                                     // [nblk][16] block partials
     int nblk = 0;
-    double* scal = nullptr;                                    // [32] reduced scalars
+    double* scal = nullptr;                                    // [64] reduced scalars (41 used)
     int* err = nullptr;                                        // [4] device error flags
+    double* pk = nullptr;                                      // [64] packed read-back (EL_PK_N used)
 };
 
 struct SpmeBuffers;
@@ -206,6 +207,11 @@ int el_time_sfac(ElecBuffers& E, const ElecDev& P, int reps, hipEvent_t a, hipEv
 void el_launch_zero(double* p, long n, hipStream_t s);
 void el_launch_add(double* dst, const double* src, long n, hipStream_t s);
 void el_launch_copy(double* dst, const double* src, long n, hipStream_t s);
+// Read-back of one electrostatics evaluation packed in doubles: the solver state (8), the
+// error flag, the 41 reduced scalars, the particle mesh's two input flags (0 without one).
+constexpr int EL_PK_ST = 0, EL_PK_ERR = 8, EL_PK_SCAL = 9, EL_PK_SPF = 50, EL_PK_N = 52;
+void el_launch_pack(const double* state, const int* err, const double* scal, const int* spflag, double* out,
+                    hipStream_t s);
 
 // Host-side state of the electrostatics of one context.
 struct ElecState {
@@ -225,6 +231,7 @@ struct ElecState {
     double k_alpha = -1.0, k_rcut = -1.0;
     double* rcell = nullptr;            // [9] device copy of the inverse cell
     double scal[48];                    // host copy of the reduced scalars
+    double* hpk = nullptr;              // pinned host copy of E.pk (EL_PK_N doubles)
     double out[20];                     // eng_ew1p, eng_ew2p, eng_ew3p, eng_polar, eng_cpe, vir_cpe,
                                         // vir_ew1p, vir_qd, iterations, rms change, stress[9], pairs
     // One evaluation's device work captured as a graph and replayed while its key (the
