@@ -472,7 +472,8 @@ int leaf_eval(int nmon, const double* const* w, double* E, double* gout)
 // energies and W) into the heat-flux extras.  `x` is either NULL or already a context
 // scratch buffer (then the terms are added in place); a NULL extra becomes the single
 // term itself, or the scratch buffer holding the sum of several.
-void add_terms(mbgpu_ctx* c, const double*& x, const double* const* terms, int nterms, double* scratch, long n)
+void add_terms(mbgpu_ctx* c, const double*& x, const double* const* terms, int nterms, double* scratch, long n,
+               hipStream_t s)
 {
     int k = 0;
     if (!x) {
@@ -481,11 +482,11 @@ void add_terms(mbgpu_ctx* c, const double*& x, const double* const* terms, int n
             x = terms[0];
             return;
         }
-        el_launch_copy(scratch, terms[0], n, c->stream);
+        el_launch_copy(scratch, terms[0], n, s);
         x = scratch;
         k = 1;
     }
-    for (; k < nterms; ++k) el_launch_add(const_cast<double*>(x), terms[k], n, c->stream);
+    for (; k < nterms; ++k) el_launch_add(const_cast<double*>(x), terms[k], n, s);
 }
 
 bool have_model_terms(const mbgpu_ctx* c) { return c->el.ready || c->wt.ready; }
@@ -504,9 +505,9 @@ void with_model_terms(mbgpu_ctx* c, const double*& ep, const double*& we, const 
         es[ne] = c->wt.B.eout;
         ws[ne++] = c->wt.B.wout;
     }
-    add_terms(c, ep, es, ne, c->dep, nr);
-    add_terms(c, we, ws, ne, c->dw, 9L * nr);
-    add_terms(c, se, ss, ns, c->dsew, 9L * nr);
+    add_terms(c, ep, es, ne, c->dep, nr, c->stream);
+    add_terms(c, we, ws, ne, c->dw, 9L * nr, c->stream);
+    add_terms(c, se, ss, ns, c->dsew, 9L * nr, c->stream);
 }
 
 }  // namespace
@@ -591,7 +592,7 @@ int full_enqueue_mbpol_water(mbgpu_ctx* c, double* x, double* y, double* z, hipS
     return 0;
 }
 
-// After both: the per-atom terms of the electrostatics, then of the water terms
+// After both, on stream a: the per-atom terms of the electrostatics, then of the water terms
 // (with_model_terms' order), the heat flux over every term, and every scalar packed into dpk.
 int full_enqueue_tail(mbgpu_ctx* c, const double* vx, const double* vy, const double* vz, const double* ek,
                       const int* spflag, hipStream_t a)
@@ -605,9 +606,9 @@ int full_enqueue_tail(mbgpu_ctx* c, const double* vx, const double* vy, const do
     const double* es[2] = {E.eout, c->wt.B.eout};
     const double* ws[2] = {E.wout, c->wt.B.wout};
     const double* ss[1] = {E.sout};
-    add_terms(c, ep_, es, 2, c->dep, nr);
-    add_terms(c, we_, ws, 2, c->dw, 9L * nr);
-    add_terms(c, se_, ss, 1, c->dsew, 9L * nr);
+    add_terms(c, ep_, es, 2, c->dep, nr, a);
+    add_terms(c, we_, ws, 2, c->dw, 9L * nr, a);
+    add_terms(c, se_, ss, 1, c->dsew, 9L * nr, a);
     int rc = mb_launch_heat_flux((int)nr, vx, vy, vz, ek, B.atomic_energy, ep_, B.heat_w, we_, se_, B.blk,
                                  B.nblk_cap, c->dout, a);
     if (rc) return fail(rc, "heat-flux kernels failed");
@@ -1009,7 +1010,7 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
         hipStream_t a = c->stream, b = c->el_stream;
         auto part_el = [&] { return full_enqueue_elec(c, ep, gb, x_dev, y_dev, z_dev, dip_dev, b); };
         auto part_mb = [&] { return full_enqueue_mbpol_water(c, x_dev, y_dev, z_dev, a); };
-        auto part_tail = [&] { return full_enqueue_tail(c, vx_dev, vy_dev, vz_dev, e_kin_dev, spf, a); };
+        auto part_tail = [&] { return full_enqueue_tail(c, vx_dev, vy_dev, vz_dev, e_kin_dev, spf, b); };
         bool replay = false;
         if (graphs) {
             std::vector<unsigned char> key = full_key(c, ep, gb, ptrs, 14);
@@ -1021,7 +1022,7 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
                 capture_gate_begin();
                 int cr = capture_part(b, part_el, c->fexec[0]);
                 if (!cr) cr = capture_part(a, part_mb, c->fexec[1]);
-                if (!cr) cr = capture_part(a, part_tail, c->fexec[2]);
+                if (!cr) cr = capture_part(b, part_tail, c->fexec[2]);
                 capture_gate_end();
                 if (cr) {
                     // the runtime refused a capture: this context launches kernel by kernel from now on
@@ -1037,27 +1038,24 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
             }
             replay = c->fexec[0] != nullptr;
         }
-        // fork: the electrostatics on b after the set-up enqueued on a; join before the tail
-        HIPCHK(hipEventRecord(c->ev2, a));
-        HIPCHK(hipStreamWaitEvent(b, c->ev2, 0));
+        // The electrostatics on b, MB-pol and the water terms on a, joined on the host: the
+        // tail follows the electrostatics on b once a has finished.  No stream waits on an
+        // event of the other: with more streams than hardware queues (contexts evaluated
+        // concurrently) such a wait holds up whatever other stream shares its queue.  The
+        // inputs are complete before the call, and the set-up above enqueues on a only.
         int rc = 0;
         if (replay) {
             HIPCHK(hipGraphLaunch(c->fexec[0], b));
-            HIPCHK(hipEventRecord(c->ev3, b));
             HIPCHK(hipGraphLaunch(c->fexec[1], a));
-            HIPCHK(hipStreamWaitEvent(a, c->ev3, 0));
-            HIPCHK(hipGraphLaunch(c->fexec[2], a));
             ++c->freplays;
         } else {
             rc = part_el();
-            if (!rc) {
-                HIPCHK(hipEventRecord(c->ev3, b));
-                rc = part_mb();
-            }
-            if (!rc) {
-                HIPCHK(hipStreamWaitEvent(a, c->ev3, 0));
-                rc = part_tail();
-            }
+            if (!rc) rc = part_mb();
+        }
+        if (!rc) {
+            HIPCHK(hipStreamSynchronize(a));
+            if (replay) HIPCHK(hipGraphLaunch(c->fexec[2], b));
+            else rc = part_tail();
         }
         if (rc) {
             (void)hipStreamSynchronize(b);
@@ -1065,8 +1063,8 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
             return rc;
         }
         const double* pk = c->hpk;
-        HIPCHK(hipMemcpyAsync(c->hpk, c->dpk, sizeof(double) * PK_N, hipMemcpyDeviceToHost, a));
-        HIPCHK(hipStreamSynchronize(a));
+        HIPCHK(hipMemcpyAsync(c->hpk, c->dpk, sizeof(double) * PK_N, hipMemcpyDeviceToHost, b));
+        HIPCHK(hipStreamSynchronize(b));
         int cnt[4], spflag[2];
         for (int i = 0; i < 4; ++i) cnt[i] = (int)pk[PK_CNT + i];
         const double* mb = pk + PK_MB;
