@@ -26,7 +26,9 @@ Legs (all timed over --steps steps after --warmup):
 `value` is the fastest leg whose RMS force error against FP64 is <= 1e-5 kcal/mol/A.
 
 The legs after them: NVE on the device (with the heat-current autocorrelation), a ring
-polymer of 8 beads of 256 waters, and same-size serving with p50/p99 latency.  Prints one
+polymer of 8 beads of 256 waters, serving of PIMD beads and isolated clusters with p50/p99
+latency (FP64, then FP32 polynomials), the 2048-water box by the k-sum and the particle mesh,
+and the 2048-water relaxation + NVE workflow.  Prints one
 JSON line: metric, value, unit, timing and configuration, a `roofline` object for the
 dominant kernel (achieved vs peak FLOP/s, HBM traffic per launch) and a `cpu_baseline`
 object (the CPU path timed on this host).
@@ -184,8 +186,9 @@ def large_md_leg(pkg, args):
     return out
 
 
-def serve_leg(pkg, args):
-    """Config-3 style serving mix through serve.Server; returns latency and throughput."""
+def serve_leg(pkg, args, precision=0):
+    """Config-3 style serving mix through serve.Server; returns latency and throughput.
+    precision 0: FP64; 3: FP32 polynomials (the mixed-precision path)."""
     import numpy as np
     serve = importlib.import_module("dlpoly2-mbpol-heat_amd.serve")
     wb, ep, wp = pkg.waterbox, pkg.elec_params, pkg.water_params
@@ -204,7 +207,7 @@ def serve_leg(pkg, args):
         else:
             b = beads[i % len(beads)]
             jobs.append((b, ep.for_box(b.L), wp.for_box(b.L), "bead"))
-    srv = serve.Server(workers=max(1, args.batch), solver=1 if args.solver == "cg" else 0)
+    srv = serve.Server(workers=max(1, args.batch), precision=precision, solver=1 if args.solver == "cg" else 0)
     # warm the pool, untimed: one request per worker at once, so every worker holds a slot,
     # then one request of every cluster size (slots of new sizes are still created and
     # evicted during the timed run, as other workers use the same size at once)
@@ -222,7 +225,7 @@ def serve_leg(pkg, args):
             out[f"{tag}_p50_ms"], out[f"{tag}_p99_ms"] = s["p50_ms"], s["p99_ms"]
     out.update(arrival_rate=args.serve_rate, wall_s=wall, evals_per_s=len(reqs) / wall, workers=max(1, args.batch),
                slots_created=srv.slots_created, slots_evicted=srv.slots_evicted,
-               mix=args.serve_mix, cluster_sizes=sizes,
+               mix=args.serve_mix, cluster_sizes=sizes, precision="fp64" if precision == 0 else "fp32 polynomials",
                what="requests drawn from 32 PIMD beads x 256 H2O (L=19.72 A, N(0,0.05 A) displacements)"
                     + (" and, half of them, isolated clusters of n in [3,1024] waters, n log-uniform (random "
                        "sequential addition at 0.997 g/cm^3, O-O >= 2.6 A; isolated: imcon 0, direct sums, no images)"
@@ -268,6 +271,8 @@ def main():
     ap.add_argument("--only-large-md", action="store_true",
                     help="run the large-box workflow alone and print its record")
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--no-serve-mixed", action="store_true",
+                    help="serve in FP64 only (by default the stream is served again with FP32 polynomials)")
     ap.add_argument("--serve-mix", choices=["beads", "mixed"], default="mixed",
                     help="beads: PIMD beads of 256 waters only (same-size slots); mixed: beads and isolated "
                          "clusters of 3-1024 waters, half and half")
@@ -451,9 +456,11 @@ def main():
     # serving leg: the mixed stream of PIMD beads (256 waters, L = 19.72 A, one configuration
     # displaced by N(0, 0.05 A) per bead) and isolated clusters (n log-uniform in [3, 1024]),
     # Poisson arrivals, continuous batching over --batch workers
-    serving = None
+    serving = serving_mixed = None
     if args.serve_requests > 0:
         serving = serve_leg(pkg, args)
+        if not args.no_serve_mixed:
+            serving_mixed = serve_leg(pkg, args, precision=3)
     large_out = large_box_leg(pkg, torch, args.large_reps) if args.large_reps > 0 else None
     large_md = large_md_leg(pkg, args) if args.large_md_steps > 0 else None
     # headline: the fastest leg whose forces are within 1e-5 kcal/mol/A RMS of FP64
@@ -469,6 +476,8 @@ def main():
         legs[f"{name}_batched_speedup_vs_fp64_single"] = b["evals_per_s"] / single
     if serving is not None:
         legs["serving"] = serving
+    if serving_mixed is not None:
+        legs["serving_mixed"] = serving_mixed
     if nve_out is not None:
         legs["nve"] = nve_out
     if pimd_out is not None:
