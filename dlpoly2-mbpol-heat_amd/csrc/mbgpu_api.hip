@@ -999,6 +999,7 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
     c->wt.ready = false;
     if (water_configure(c->wt, nw, g, cell, wp, c->stream, err)) return fail(-1, "%s", err.c_str());
     const int gb = elec_batch(c->el);
+    bool fused_ran = false;
     // one enqueue (a graph replay once captured) when every buffer has been sized by an
     // evaluation through the separate entries
     if (B.dcap > 0 && B.nblk_cap > 0 && gb > 0) {
@@ -1062,6 +1063,7 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
             (void)hipStreamSynchronize(a);
             return rc;
         }
+        fused_ran = true;
         const double* pk = c->hpk;
         HIPCHK(hipMemcpyAsync(c->hpk, c->dpk, sizeof(double) * PK_N, hipMemcpyDeviceToHost, b));
         HIPCHK(hipStreamSynchronize(b));
@@ -1107,7 +1109,13 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
         }
     }
     // the separate entries: they size every buffer, repeat overflowed builds, continue
-    // unconverged solves and report errors
+    // unconverged solves and report errors.  After a one-call attempt they start again from
+    // the caller's start dipoles (kept in the electrostatics' dip0; the attempt wrote its
+    // dipoles back into dip_dev).
+    if (fused_ran && dip_dev) {
+        el_launch_copy(dip_dev, c->el.E.dip0, 3L * 4 * nw, c->el_stream);
+        HIPCHK(hipStreamSynchronize(c->el_stream));
+    }
     int niter = 0;
     int rc = mbgpu_mbpol_forces_dev(c, imcon, cell, nttm2, listttm2_dev, x_dev, y_dev, z_dev, nullptr, nullptr,
                                     nullptr, &out[0], &out[1], &out[2]);

@@ -178,3 +178,35 @@ def test_one_call_falls_back_when_the_graph_is_short(pkg, mods, monkeypatch):
         assert ra["dipole_iterations"] > 3
         same(ra, parts, rb, one)
     assert one.ctx.evaluate_graph_counts()[1] == 2
+
+
+def test_one_call_fallback_starts_from_the_callers_dipoles(pkg, mods, monkeypatch):
+    import torch
+    engine, wb = mods
+    boxes, e, w = configurations(pkg, wb, "ksum")
+    monkeypatch.setenv("MBGPU_GRAPH_BATCH", "3")
+    parts = engine.Slot(boxes[0], solver=1, elec=e, water=w, fused=False)
+    parts.ctx.set_graphs(False)
+    one = engine.Slot(boxes[0], solver=1, elec=e, water=w)
+    # start dipoles: the converged dipoles of another configuration
+    d0 = torch.zeros(3 * boxes[0].natms, dtype=torch.float64, device="cuda")
+    b4 = boxes[4]
+    parts.load(b4)
+    torch.cuda.synchronize()
+    parts.ctx.polar_electrostatics_dev(b4.imcon, b4.cell, parts.eparams, parts.x.data_ptr(), parts.y.data_ptr(),
+                                       parts.z.data_ptr(), d0.data_ptr())
+    print("one-call evaluation from given start dipoles, falling back after a 3-iteration graph", flush=True)
+    for bx in boxes[:3]:
+        parts.load(bx)
+        one.load(bx)
+        da, db = d0.clone(), d0.clone()
+        torch.cuda.synchronize()
+        ee, _, _, ita = parts.ctx.polar_electrostatics_dev(bx.imcon, bx.cell, parts.eparams, parts.x.data_ptr(),
+                                                           parts.y.data_ptr(), parts.z.data_ptr(), da.data_ptr())
+        o = one.ctx.evaluate_dev(bx.imcon, bx.cell, 3 * bx.nw, one.lst.data_ptr(), one.eparams, one.wparams,
+                                 one.x.data_ptr(), one.y.data_ptr(), one.z.data_ptr(), db.data_ptr(),
+                                 one.v[0].data_ptr(), one.v[1].data_ptr(), one.v[2].data_ptr(), one.ek.data_ptr())
+        torch.cuda.synchronize()
+        assert ita > 3 and o[49] == 0.0
+        assert o[11] == ee and int(o[22]) == ita
+        assert torch.equal(da, db)
