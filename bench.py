@@ -407,7 +407,7 @@ def main():
     if args.nve_steps > 0:
         md = importlib.import_module("dlpoly2-mbpol-heat_amd.md")
         engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
-        nve = md.NVE(engine.Slot(box, solver=1 if args.solver == "cg" else 0), 0.2)
+        nve = md.NVE(engine.Slot(box, solver=1 if args.solver == "cg" else 0), 0.2, extrapolate=2)
         e0, k0 = nve.pot + nve.kin, nve.kin
         tot, its, jser = [], [], []
         torch.cuda.synchronize()
@@ -431,7 +431,8 @@ def main():
                    "max_abs_dE_over_KE0": float(np.abs(tot - e0).max() / k0),
                    "drift_per_ps_over_KE0": float(slope / k0), "dipole_iterations_median": float(np.median(its)),
                    "heat_acf": {"lags": ntau, "ms": 1e3 * tg, "what": "autocorrelation of the run's J_tot on the device"},
-                   "what": "512 H2O from the 300 K start, heat current every step, induced dipoles warm-started"}
+                   "what": "512 H2O from the 300 K start, heat current every step, induced dipoles started from "
+                           "the parabola through the last three steps' (md.NVE extrapolate=2)"}
     # PIMD leg: a ring polymer of 8 beads of 256 waters (300 K), every bead evaluated with the
     # complete force field on its own context (overlapping), springs and normal modes on the
     # device, velocity Verlet of the ring-polymer Hamiltonian

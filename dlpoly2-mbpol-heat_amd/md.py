@@ -85,11 +85,21 @@ class CheckWriter:
 
 
 class NVE:
-    def __init__(self, slot, dt_fs):
+    """extrapolate: the solver's start dipoles each step.  0: the previous step's converged
+    dipoles, as the reference keeps them; 1: the line through the last two (2 mu1 - mu2); 2:
+    the parabola through the last three (3 mu1 - 3 mu2 + mu3).  The solve runs to the same
+    stopping test from any start, so the extrapolation only saves iterations; the dipoles
+    differ from the warm-started ones within that test."""
+
+    def __init__(self, slot, dt_fs, extrapolate=0):
         import torch
         self.t = torch
         self.slot = slot
         self.dt = dt_fs * 1.0e-3                   # ps
+        if extrapolate not in (0, 1, 2):
+            raise ValueError("extrapolate is 0, 1 or 2")
+        self.extrapolate = extrapolate
+        self._hist = []                            # converged dipoles of the last steps, newest first
         box = slot.box
         nr, n = box.nreal, box.natms
         f64 = dict(dtype=torch.float64, device=slot.dev)
@@ -108,10 +118,26 @@ class NVE:
         s = self.slot
         return s.x.data_ptr(), s.y.data_ptr(), s.z.data_ptr()
 
+    def _start_dipoles(self):
+        h, t = self._hist, self.t
+        k = min(self.extrapolate, len(h) - 1)
+        if k == 1:
+            t.sub(2.0 * h[0], h[1], out=self.dip)
+        elif k == 2:
+            t.add(3.0 * (h[0] - h[1]), h[2], out=self.dip)
+        if k > 0:
+            t.cuda.current_stream(self.slot.dev).synchronize()   # read by the library's stream
+
+    def _remember(self):
+        if self.extrapolate > 0:
+            self._hist.insert(0, self.dip.clone())
+            del self._hist[self.extrapolate + 1:]
+
     def _forces(self):
         s, b, c = self.slot, self.slot.box, self.slot.ctx
         x, y, z = self._ptrs()
         f = self.f
+        self._start_dipoles()
         fp = (f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr())
         # MB-pol and the water terms add into f; the electrostatics write fsite (own stream)
         (e_el, _, _, self.iterations), ((e_mb, _, _), (e_w, _, _)) = s.split(
@@ -119,6 +145,7 @@ class NVE:
                                                self.fsite.data_ptr()),
             lambda: (c.mbpol_forces_dev(b.imcon, b.cell, s.lst.data_ptr(), x, y, z, *fp),
                      c.water_terms_dev(b.imcon, b.cell, s.wparams, x, y, z, *fp)))
+        self._remember()
         return e_mb + e_el + float(np.sum(e_w))
 
     def _kinetic(self):
@@ -184,7 +211,7 @@ def _copy_state(src, dst):
 
 
 def large_box_run(box, relax_steps=500, nve_steps=10000, shadow_every=100, dt_fs=0.2, temperature=300.0,
-                  tau_fs=20.0, recip=("spme", 6, (48, 48, 48)), ntau=None, progress=None):
+                  tau_fs=20.0, recip=("spme", 6, (48, 48, 48)), ntau=None, progress=None, extrapolate=2):
     """The large-box workflow on one configuration (box: a waterbox.WaterBox).
 
     1. relax_steps steps of FP64 MD at dt_fs with Berendsen velocity scaling towards
@@ -197,7 +224,8 @@ def large_box_run(box, relax_steps=500, nve_steps=10000, shadow_every=100, dt_fs
        in FP64 by a second slot: RMS and max force difference, energy difference per
        molecule and relative |J - J_FP64|;
     4. the autocorrelation of the run's J_tot for ntau lags on the device.
-    The reciprocal space is `recip` (engine.Slot's argument; None: the Ewald sum over k).
+    The reciprocal space is `recip` (engine.Slot's argument; None: the Ewald sum over k);
+    `extrapolate` is NVE's choice of start dipoles (2: the parabola through the last three).
     progress(stage, step, dict) is called every shadow_every steps.  Returns a dict of the
     series and summaries (internal units: energy 10 J/mol, J in (10 J/mol) A/ps)."""
     import time as _time
@@ -210,7 +238,7 @@ def large_box_run(box, relax_steps=500, nve_steps=10000, shadow_every=100, dt_fs
                dt_fs=dt_fs, recip=None if recip is None else list(recip))
     # 1. relaxation (FP64)
     fp64 = engine.Slot(box, precision=0, recip=recip)
-    md = NVE(fp64, dt_fs)
+    md = NVE(fp64, dt_fs, extrapolate=extrapolate)
     temps = []
     t0 = _time.perf_counter()
     for k in range(relax_steps):
@@ -230,7 +258,7 @@ def large_box_run(box, relax_steps=500, nve_steps=10000, shadow_every=100, dt_fs
     mixed = engine.Slot(box, precision=3, recip=recip)
     _copy_state(fp64, mixed)
     shadow = fp64
-    nve = NVE(mixed, dt_fs)
+    nve = NVE(mixed, dt_fs, extrapolate=extrapolate)
     e0, k0 = nve.pot + nve.kin, nve.kin
     etot, jser, checks, iters = [], [], [], []
 

@@ -135,3 +135,22 @@ def test_large_box_workflow_short(pkg, mods):
     assert j.shape == (200, 3)
     ref = np.array([np.mean(np.sum(j[: len(j) - k] * j[k:], axis=1)) for k in range(50)])
     assert np.abs(r["acf"] - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_extrapolated_start_dipoles(pkg, mods):
+    # the solve runs to the same stopping test from an extrapolated start: the trajectory
+    # agrees with the warm-started one within that test, and fewer iterations are needed
+    engine, md, wb = mods
+    box = wb.box_64()
+    runs = {}
+    for ex in (0, 2):
+        nve = md.NVE(engine.Slot(box), 0.2, extrapolate=ex)
+        print(f"nve extrapolate {ex}: 40 steps", flush=True)
+        rs = [nve.step() for _ in range(40)]
+        runs[ex] = (np.array([r["tot"] for r in rs]), np.array([r["iterations"] for r in rs]), nve.positions())
+    (e0, i0, x0), (e2, i2, x2) = runs[0], runs[2]
+    assert np.abs(e2 - e0).max() <= 1e-8 * np.abs(e0).max()
+    assert np.abs(x2 - x0).max() <= 1e-9
+    assert i2[5:].mean() < i0[5:].mean()
+    with pytest.raises(ValueError):
+        md.NVE(engine.Slot(box), 0.2, extrapolate=3)
