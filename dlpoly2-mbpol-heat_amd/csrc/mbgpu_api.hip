@@ -561,7 +561,7 @@ namespace {
 // parts.  The electrostatics, on their stream b: positions in, start dipoles, gb solver
 // iterations, the final pass, M sites and dipoles out.
 int full_enqueue_elec(mbgpu_ctx* c, const mbgpu_elec_params* ep, int gb, double* x, double* y, double* z,
-                      double* dip, hipStream_t b)
+                      double* dip, double* fsite, hipStream_t b)
 {
     ElecBuffers& E = c->el.E;
     const int nw = c->B.nw, n = 4 * nw, nr = 3 * nw;
@@ -576,24 +576,32 @@ int full_enqueue_elec(mbgpu_ctx* c, const mbgpu_elec_params* ep, int gb, double*
     el_launch_copy(y + nr, E.y + nr, nw, b);
     el_launch_copy(z + nr, E.z + nr, nw, b);
     if (dip) el_launch_copy(dip, E.dip, 3L * n, b);
+    if (fsite) el_launch_copy(fsite, E.fsite, 3L * n, b);
     return 0;
 }
 
 // MB-pol's lists and clusters and the water terms, on the context's stream a.
-int full_enqueue_mbpol_water(mbgpu_ctx* c, double* x, double* y, double* z, hipStream_t a)
+int full_enqueue_mbpol_water(mbgpu_ctx* c, double* x, double* y, double* z, double* fx, double* fy, double* fz,
+                             hipStream_t a)
 {
     MbBuffers& B = c->B;
+    if (fx) {
+        el_launch_zero(fx, B.natms, a);
+        el_launch_zero(fy, B.natms, a);
+        el_launch_zero(fz, B.natms, a);
+    }
     int rc = mb_launch_lists(B, c->geom, a);
     if (!rc) rc = mb_launch_dimers(B, B.counts, (int)B.dcap, c->geom, true, a);
     if (!rc) rc = mb_launch_trimers(B, B.counts + 1, (int)B.tcap, (int)((B.dcap + 127) / 128), c->geom, true, a);
-    if (!rc) rc = mb_launch_accumulate(B, nullptr, nullptr, nullptr, a);
-    if (!rc) rc = water_enqueue(c->wt, x, y, z, nullptr, nullptr, nullptr, a);
+    if (!rc) rc = mb_launch_accumulate(B, fx, fy, fz, a);
+    if (!rc) rc = water_enqueue(c->wt, x, y, z, fx, fy, fz, a);
     if (rc) return fail(rc, "MB-pol or water-term kernels failed: %s", hipGetErrorString((hipError_t)rc));
     return 0;
 }
 
 // After both, on stream a: the per-atom terms of the electrostatics, then of the water terms
-// (with_model_terms' order), the heat flux over every term, and every scalar packed into dpk.
+// (with_model_terms' order), the heat flux over every term (skipped without velocities), and
+// every scalar packed into dpk.
 int full_enqueue_tail(mbgpu_ctx* c, const double* vx, const double* vy, const double* vz, const double* ek,
                       const int* spflag, hipStream_t a)
 {
@@ -606,12 +614,16 @@ int full_enqueue_tail(mbgpu_ctx* c, const double* vx, const double* vy, const do
     const double* es[2] = {E.eout, c->wt.B.eout};
     const double* ws[2] = {E.wout, c->wt.B.wout};
     const double* ss[1] = {E.sout};
-    add_terms(c, ep_, es, 2, c->dep, nr, a);
-    add_terms(c, we_, ws, 2, c->dw, 9L * nr, a);
-    add_terms(c, se_, ss, 1, c->dsew, 9L * nr, a);
-    int rc = mb_launch_heat_flux((int)nr, vx, vy, vz, ek, B.atomic_energy, ep_, B.heat_w, we_, se_, B.blk,
-                                 B.nblk_cap, c->dout, a);
-    if (rc) return fail(rc, "heat-flux kernels failed");
+    if (vx) {
+        add_terms(c, ep_, es, 2, c->dep, nr, a);
+        add_terms(c, we_, ws, 2, c->dw, 9L * nr, a);
+        add_terms(c, se_, ss, 1, c->dsew, 9L * nr, a);
+    }
+    if (vx) {
+        const int rc = mb_launch_heat_flux((int)nr, vx, vy, vz, ek, B.atomic_energy, ep_, B.heat_w, we_, se_,
+                                           B.blk, B.nblk_cap, c->dout, a);
+        if (rc) return fail(rc, "heat-flux kernels failed");
+    }
     hipLaunchKernelGGL(k_pack_eval, dim3(1), dim3(128), 0, a, B.counts, B.scal, E.state, E.err, E.scal, spflag,
                        c->wt.B.scal, c->dout, c->dpk);
     HIPCHK(hipGetLastError());
@@ -964,17 +976,21 @@ int mbgpu_heat_flux_dev(mbgpu_ctx* c, const double* vx, const double* vy, const 
     return 0;
 }
 
-int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
-                       const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev, double* y_dev,
-                       double* z_dev, double* dip_dev, const double* vx_dev, const double* vy_dev,
-                       const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT])
+}  // extern "C"
+
+namespace {
+
+// mbgpu_evaluate_dev and mbgpu_evaluate_forces_dev: forces written into fx/fy/fz (MB-pol and
+// water terms, natms each) and fsite (electrostatics, 3 per site) when given; the heat flux
+// when the velocities are given.
+int evaluate_impl(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
+                  const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev, double* y_dev,
+                  double* z_dev, double* dip_dev, double* fx_dev, double* fy_dev, double* fz_dev,
+                  double* fsite_dev, const double* vx_dev, const double* vy_dev, const double* vz_dev,
+                  const double* e_kin_dev, double out[MBGPU_EVAL_OUT])
 {
-    EntryGuard gate_;
-    if (!c) return fail(-1, "null context");
-    if (!ep || !wp) return fail(-1, "null electrostatics or water-term parameters");
-    if (!out || !listttm2_dev || !x_dev || !y_dev || !z_dev || !vx_dev || !vy_dev || !vz_dev || !e_kin_dev)
-        return fail(-1, "null argument");
     const int nw = c->B.nw;
+    const bool heat = vx_dev != nullptr;
     if (nttm2 != 3 * nw) return fail(-1, "nttm2 = %d does not match the context (3*%d)", nttm2, nw);
     if (c->B.natms < 4 * nw)
         return fail(-1, "the context holds %d sites; the electrostatics need 4*nw = %d", c->B.natms, 4 * nw);
@@ -1004,17 +1020,17 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
     // evaluation through the separate entries
     if (B.dcap > 0 && B.nblk_cap > 0 && gb > 0) {
         const int* spf = c->el.recip && c->el.sp && imcon != 0 ? c->el.sp->flag : nullptr;
-        const void* ptrs[14] = {listttm2_dev, x_dev, y_dev, z_dev, dip_dev, vx_dev, vy_dev, vz_dev, e_kin_dev,
-                                c->dep, c->dw, c->dsew, spf, c->dpk};
+        const void* ptrs[18] = {listttm2_dev, x_dev, y_dev, z_dev, dip_dev, vx_dev, vy_dev, vz_dev, e_kin_dev,
+                                c->dep, c->dw, c->dsew, spf, c->dpk, fx_dev, fy_dev, fz_dev, fsite_dev};
         const char* gv = getenv("MBGPU_GRAPHS");
         const bool graphs = c->el.graphs && !(gv && atoi(gv) == 0);
         hipStream_t a = c->stream, b = c->el_stream;
-        auto part_el = [&] { return full_enqueue_elec(c, ep, gb, x_dev, y_dev, z_dev, dip_dev, b); };
-        auto part_mb = [&] { return full_enqueue_mbpol_water(c, x_dev, y_dev, z_dev, a); };
+        auto part_el = [&] { return full_enqueue_elec(c, ep, gb, x_dev, y_dev, z_dev, dip_dev, fsite_dev, b); };
+        auto part_mb = [&] { return full_enqueue_mbpol_water(c, x_dev, y_dev, z_dev, fx_dev, fy_dev, fz_dev, a); };
         auto part_tail = [&] { return full_enqueue_tail(c, vx_dev, vy_dev, vz_dev, e_kin_dev, spf, b); };
         bool replay = false;
         if (graphs) {
-            std::vector<unsigned char> key = full_key(c, ep, gb, ptrs, 14);
+            std::vector<unsigned char> key = full_key(c, ep, gb, ptrs, 18);
             if (!c->fexec[0] || key != c->fkey) {
                 for (hipGraphExec_t& g : c->fexec)
                     if (g) (void)hipGraphExecDestroy(g);
@@ -1099,11 +1115,12 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
                 out[27 + i] = w[2 * i + 1];
             }
             for (int i = 0; i < 9; ++i) out[31 + i] = c->wt.stress[i];
-            for (int a2 = 0; a2 < 3; ++a2) {
-                out[40 + a2] = jd[a2];
-                out[43 + a2] = jd[3 + a2];
-                out[46 + a2] = jd[a2] + jd[3 + a2];
-            }
+            if (heat)
+                for (int a2 = 0; a2 < 3; ++a2) {
+                    out[40 + a2] = jd[a2];
+                    out[43 + a2] = jd[3 + a2];
+                    out[46 + a2] = jd[a2] + jd[3 + a2];
+                }
             out[49] = 1.0;
             return 0;
         }
@@ -1116,24 +1133,72 @@ int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2,
         el_launch_copy(dip_dev, c->el.E.dip0, 3L * 4 * nw, c->el_stream);
         HIPCHK(hipStreamSynchronize(c->el_stream));
     }
+    // forces are written: cleared here, then added by the separate entries
+    if (fx_dev) {
+        el_launch_zero(fx_dev, c->B.natms, c->stream);
+        el_launch_zero(fy_dev, c->B.natms, c->stream);
+        el_launch_zero(fz_dev, c->B.natms, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (fsite_dev) {
+        el_launch_zero(fsite_dev, 3L * 4 * nw, c->el_stream);
+        HIPCHK(hipStreamSynchronize(c->el_stream));
+    }
     int niter = 0;
-    int rc = mbgpu_mbpol_forces_dev(c, imcon, cell, nttm2, listttm2_dev, x_dev, y_dev, z_dev, nullptr, nullptr,
-                                    nullptr, &out[0], &out[1], &out[2]);
+    int rc = mbgpu_mbpol_forces_dev(c, imcon, cell, nttm2, listttm2_dev, x_dev, y_dev, z_dev, fx_dev, fy_dev,
+                                    fz_dev, &out[0], &out[1], &out[2]);
     if (rc) return rc;
-    rc = mbgpu_polar_electrostatics_dev(c, imcon, cell, ep, x_dev, y_dev, z_dev, dip_dev, nullptr, &out[11],
+    rc = mbgpu_polar_electrostatics_dev(c, imcon, cell, ep, x_dev, y_dev, z_dev, dip_dev, fsite_dev, &out[11],
                                         &out[12], &out[13], &niter);
     if (rc) return rc;
     out[22] = niter;
     double ew[4], vw[4];
-    rc = mbgpu_water_terms_dev(c, imcon, cell, wp, x_dev, y_dev, z_dev, nullptr, nullptr, nullptr, ew, vw,
+    rc = mbgpu_water_terms_dev(c, imcon, cell, wp, x_dev, y_dev, z_dev, fx_dev, fy_dev, fz_dev, ew, vw,
                                &out[31]);
     if (rc) return rc;
     for (int i = 0; i < 4; ++i) {
         out[23 + i] = ew[i];
         out[27 + i] = vw[i];
     }
+    if (!heat) return 0;
     return mbgpu_heat_flux_dev(c, vx_dev, vy_dev, vz_dev, e_kin_dev, nullptr, nullptr, nullptr, &out[40], &out[43],
                                &out[46]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbgpu_evaluate_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
+                       const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev, double* y_dev,
+                       double* z_dev, double* dip_dev, const double* vx_dev, const double* vy_dev,
+                       const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT])
+{
+    EntryGuard gate_;
+    if (!c) return fail(-1, "null context");
+    if (!ep || !wp) return fail(-1, "null electrostatics or water-term parameters");
+    if (!out || !listttm2_dev || !x_dev || !y_dev || !z_dev || !vx_dev || !vy_dev || !vz_dev || !e_kin_dev)
+        return fail(-1, "null argument");
+    return evaluate_impl(c, imcon, cell, nttm2, listttm2_dev, ep, wp, x_dev, y_dev, z_dev, dip_dev, nullptr,
+                         nullptr, nullptr, nullptr, vx_dev, vy_dev, vz_dev, e_kin_dev, out);
+}
+
+int mbgpu_evaluate_forces_dev(mbgpu_ctx* c, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
+                              const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev,
+                              double* y_dev, double* z_dev, double* dip_dev, double* fx_dev, double* fy_dev,
+                              double* fz_dev, double* fsite_dev, const double* vx_dev, const double* vy_dev,
+                              const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT])
+{
+    EntryGuard gate_;
+    if (!c) return fail(-1, "null context");
+    if (!ep || !wp) return fail(-1, "null electrostatics or water-term parameters");
+    if (!out || !listttm2_dev || !x_dev || !y_dev || !z_dev || !fx_dev || !fy_dev || !fz_dev || !fsite_dev)
+        return fail(-1, "null argument");
+    const bool v0 = !vx_dev, v1 = !vy_dev, v2 = !vz_dev, v3 = !e_kin_dev;
+    if (!(v0 == v1 && v1 == v2 && v2 == v3))
+        return fail(-1, "velocities and kinetic energies: give all four arrays or none");
+    return evaluate_impl(c, imcon, cell, nttm2, listttm2_dev, ep, wp, x_dev, y_dev, z_dev, dip_dev, fx_dev, fy_dev,
+                         fz_dev, fsite_dev, vx_dev, vy_dev, vz_dev, e_kin_dev, out);
 }
 
 int mbgpu_get_evaluate_graph_counts(mbgpu_ctx* c, long counts[2])

@@ -51,6 +51,8 @@ _SIGS = {
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "mbgpu_evaluate_forces_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+                                  + [ctypes.c_void_p] * 16),
     "mbgpu_get_lists": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _ip,
                                        ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "mbgpu_heat_flux": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -322,6 +324,20 @@ class Context:
                                                ctypes.byref(eparams), ctypes.byref(wparams), x_ptr, y_ptr, z_ptr,
                                                dip_ptr, vx_ptr, vy_ptr, vz_ptr, ek_ptr, _p(out))
         _check(rc, "mbgpu_evaluate_dev")
+        return out
+
+    def evaluate_forces_dev(self, imcon, cell, nttm2, list_ptr, eparams, wparams, x_ptr, y_ptr, z_ptr, dip_ptr,
+                            fx_ptr, fy_ptr, fz_ptr, fsite_ptr, vx_ptr=None, vy_ptr=None, vz_ptr=None, ek_ptr=None):
+        """mbgpu_evaluate_forces_dev: evaluate_dev that also WRITES the forces (MB-pol and water
+        terms into fx/fy/fz, natms each; electrostatics into fsite, 3 per site); without
+        velocities no heat flux (out[40:49] zero).  Returns the 64-value output vector."""
+        out = np.zeros(64)
+        cell = _f64(cell, 9)
+        rc = load_library().mbgpu_evaluate_forces_dev(self._h, int(imcon), _p(cell), int(nttm2), list_ptr,
+                                                      ctypes.byref(eparams), ctypes.byref(wparams), x_ptr, y_ptr,
+                                                      z_ptr, dip_ptr, fx_ptr, fy_ptr, fz_ptr, fsite_ptr, vx_ptr,
+                                                      vy_ptr, vz_ptr, ek_ptr, _p(out))
+        _check(rc, "mbgpu_evaluate_forces_dev")
         return out
 
     def evaluate_graph_counts(self):

@@ -139,6 +139,14 @@ class NVE:
         f = self.f
         self._start_dipoles()
         fp = (f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr())
+        if s.fused:
+            # one library call (graph replays): MB-pol and water forces written into f, the
+            # electrostatics' into fsite; no heat flux here (J follows the second kick)
+            o = c.evaluate_forces_dev(b.imcon, b.cell, 3 * b.nw, s.lst.data_ptr(), s.eparams, s.wparams, x, y, z,
+                                      self.dip.data_ptr(), *fp, self.fsite.data_ptr())
+            self.iterations = int(o[22])
+            self._remember()
+            return float(o[0]) + float(o[11]) + float(np.sum(o[23:27]))
         # MB-pol and the water terms add into f; the electrostatics write fsite (own stream)
         (e_el, _, _, self.iterations), ((e_mb, _, _), (e_w, _, _)) = s.split(
             lambda: c.polar_electrostatics_dev(b.imcon, b.cell, s.eparams, x, y, z, self.dip.data_ptr(),

@@ -240,6 +240,19 @@ module mbgpu
       real(c_double), intent(out) :: out(64)
       integer(c_int) :: mbgpu_evaluate_dev
     end function
+    function mbgpu_evaluate_forces_dev(ctx, imcon, cell, nttm2, listttm2_dev, ep, wp, x_dev, y_dev, z_dev, &
+                                       dip_dev, fx_dev, fy_dev, fz_dev, fsite_dev, vx_dev, vy_dev, vz_dev, &
+                                       e_kin_dev, out) bind(c, name="mbgpu_evaluate_forces_dev")
+      import :: c_int, c_double, c_ptr, mbgpu_elec_params, mbgpu_water_params
+      type(c_ptr), value :: ctx, listttm2_dev, x_dev, y_dev, z_dev, dip_dev, fx_dev, fy_dev, fz_dev, fsite_dev
+      type(c_ptr), value :: vx_dev, vy_dev, vz_dev, e_kin_dev
+      integer(c_int), value :: imcon, nttm2
+      real(c_double), intent(in) :: cell(9)
+      type(mbgpu_elec_params), intent(in) :: ep
+      type(mbgpu_water_params), intent(in) :: wp
+      real(c_double), intent(out) :: out(64)
+      integer(c_int) :: mbgpu_evaluate_forces_dev
+    end function
     function mbgpu_get_evaluate_graph_counts(ctx, counts) bind(c, name="mbgpu_get_evaluate_graph_counts")
       import :: c_int, c_long, c_ptr
       type(c_ptr), value :: ctx

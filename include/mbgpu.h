@@ -407,6 +407,20 @@ int mbgpu_evaluate_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm
                        const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev, double* y_dev,
                        double* z_dev, double* dip_dev, const double* vx_dev, const double* vy_dev,
                        const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT]);
+/* mbgpu_evaluate_dev with forces, for integrators: fx/fy/fz_dev (natms entries each) receive
+ * the MB-pol and water-term forces and fsite_dev (3 interleaved values per site) the
+ * electrostatic forces (with distribute_msite, moved onto the real atoms).  The forces are
+ * WRITTEN, not added: a call that repeats its evaluation through the separate entries
+ * clears them first.  vx/vy/vz_dev and e_kin_dev may all be NULL: no heat flux then
+ * (out[40..48] = 0).  Results are bit-identical to mbgpu_mbpol_forces_dev,
+ * mbgpu_polar_electrostatics_dev and mbgpu_water_terms_dev called in turn on cleared force
+ * arrays.  No counterpart in the reference (its force routine calls the parts in turn,
+ * forces.f). */
+int mbgpu_evaluate_forces_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], int nttm2, const int* listttm2_dev,
+                              const mbgpu_elec_params* ep, const mbgpu_water_params* wp, double* x_dev,
+                              double* y_dev, double* z_dev, double* dip_dev, double* fx_dev, double* fy_dev,
+                              double* fz_dev, double* fsite_dev, const double* vx_dev, const double* vy_dev,
+                              const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT]);
 /* counts[0]: complete evaluations captured as graphs in this context, counts[1]: replays. */
 int mbgpu_get_evaluate_graph_counts(mbgpu_ctx* ctx, long counts[2]);
 

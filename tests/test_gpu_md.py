@@ -154,3 +154,22 @@ def test_extrapolated_start_dipoles(pkg, mods):
     assert i2[5:].mean() < i0[5:].mean()
     with pytest.raises(ValueError):
         md.NVE(engine.Slot(box), 0.2, extrapolate=3)
+
+
+def test_one_call_steps_equal_separate_entries(pkg, mods):
+    # md.NVE through mbgpu_evaluate_forces_dev (engine.Slot's default) against the same
+    # steps through the separate entries: bit-identical energies, J, positions, velocities
+    engine, md, wb = mods
+    box = wb.box_64()
+    runs = []
+    for fused in (False, True):
+        nve = md.NVE(engine.Slot(box, fused=fused), 0.2)
+        print(f"nve one call {fused}: 20 steps", flush=True)
+        rs = [nve.step() for _ in range(20)]
+        runs.append((rs, nve.positions(), nve.velocities(), nve.slot.ctx.evaluate_graph_counts()))
+    (ra, xa, va, _), (rb, xb, vb, counts) = runs
+    for a, b in zip(ra, rb):
+        assert a["pot"] == b["pot"] and a["kin"] == b["kin"] and a["iterations"] == b["iterations"]
+        assert np.array_equal(a["jtot"], b["jtot"])
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    assert counts[1] >= 15
