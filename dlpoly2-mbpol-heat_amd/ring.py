@@ -87,6 +87,11 @@ class RingPolymer:
         x, y, z = s.x.data_ptr(), s.y.data_ptr(), s.z.data_ptr()
         f = self.F[k]
         fp = (f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr())
+        if s.fused:
+            # one library call per bead (graph replays), forces written into F[k] and fsite[k]
+            o = c.evaluate_forces_dev(b.imcon, b.cell, 3 * b.nw, s.lst.data_ptr(), s.eparams, s.wparams, x, y, z,
+                                      self.dip[k].data_ptr(), *fp, self.fsite[k].data_ptr())
+            return float(o[0]) + float(o[11]) + float(np.sum(o[23:27])), int(o[22])
         (e_el, _, _, its), ((e_mb, _, _), (e_w, _, _)) = s.split(
             lambda: c.polar_electrostatics_dev(b.imcon, b.cell, s.eparams, x, y, z, self.dip[k].data_ptr(),
                                                self.fsite[k].data_ptr()),
