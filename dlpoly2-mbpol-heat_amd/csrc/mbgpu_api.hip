@@ -179,6 +179,17 @@ __global__ void k_pack_eval(const int* __restrict__ cnt, const double* __restric
     out[t] = v;
 }
 
+// Total force on real atom i, component q: (MB-pol + electrostatics) + water terms, in that
+// order (the order of the host sum over the three parts' details).
+__global__ void k_sum_forces(int nr, const double* __restrict__ fmb, const double* __restrict__ fel,
+                             const double* __restrict__ fwt, double* __restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nr) return;
+    const int i = t / 3, q = t - 3 * i;
+    out[t] = (fmb[t] + fel[t]) + fwt[(long)q * nr + i];
+}
+
 bool make_geom(int imcon, const double* cell, CellGeom& g)
 {
     if (imcon < 0 || imcon > 3) return false;
@@ -249,6 +260,7 @@ struct mbgpu_ctx {
     // into pinned host memory
     double* dpk = nullptr;
     double* hpk = nullptr;
+    double* dfsum = nullptr;          // total forces of the last evaluation (mbgpu_get_forces)
 };
 
 namespace {
@@ -289,6 +301,7 @@ int alloc_fixed(mbgpu_ctx* c)
     HIPCHK(dalloc(&c->dsew, (size_t)9 * na));
     HIPCHK(dalloc(&c->dout, 16));
     HIPCHK(dalloc(&c->dpk, PK_N));
+    HIPCHK(dalloc(&c->dfsum, (size_t)3 * na));
     HIPCHK(hipHostMalloc((void**)&c->hpk, sizeof(double) * PK_N, hipHostMallocDefault));
     // coefficient streams
     const int l2 = pip2b_stream_len(), l3 = pip3b_stream_len();
@@ -740,7 +753,7 @@ void mbgpu_destroy(mbgpu_ctx* c)
     cells_free(B.cells);
     dfree(B.xv2f); dfree(B.part2f); dfree(B.cs2f); dfree(B.xv3f); dfree(B.part3f); dfree(B.cs3f);
     dfree(c->hx); dfree(c->hy); dfree(c->hz); dfree(c->hlist);
-    dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout); dfree(c->dpk);
+    dfree(c->dv); dfree(c->dek); dfree(c->dep); dfree(c->dw); dfree(c->dsew); dfree(c->dout); dfree(c->dpk); dfree(c->dfsum);
     if (c->hpk) (void)hipHostFree(c->hpk);
     dfree(c->ring_m); dfree(c->ring_blk); dfree(c->ring_out);
     spme_free(c->sp);
@@ -1199,6 +1212,21 @@ int mbgpu_evaluate_forces_dev(mbgpu_ctx* c, int imcon, const double cell[9], int
         return fail(-1, "velocities and kinetic energies: give all four arrays or none");
     return evaluate_impl(c, imcon, cell, nttm2, listttm2_dev, ep, wp, x_dev, y_dev, z_dev, dip_dev, fx_dev, fy_dev,
                          fz_dev, fsite_dev, vx_dev, vy_dev, vz_dev, e_kin_dev, out);
+}
+
+int mbgpu_get_forces(mbgpu_ctx* c, double* f)
+{
+    EntryGuard gate_;
+    if (!c || !f) return fail(-1, "null context or output");
+    if (!c->mb_ready || !c->el.ready || !c->wt.ready)
+        return fail(-1, "no complete evaluation (MB-pol, electrostatics and water terms) in this context");
+    const int nr = 3 * c->B.nw;
+    hipLaunchKernelGGL(k_sum_forces, dim3((3 * nr + 255) / 256), dim3(256), 0, c->stream, nr, c->B.fout,
+                       c->el.E.fsite, c->wt.B.fw, c->dfsum);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(f, c->dfsum, sizeof(double) * 3 * nr, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int mbgpu_get_evaluate_graph_counts(mbgpu_ctx* c, long counts[2])

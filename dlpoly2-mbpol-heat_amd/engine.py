@@ -140,11 +140,8 @@ class Slot:
 
     def forces(self):
         """Total forces of the last evaluation on the real atoms, (3*nw, 3), internal units."""
-        nr = self.box.nreal
-        _, _, fm = self.ctx.decomposition()
-        fe = self.ctx.elec_details()["f_sites"]
-        fw = self.ctx.water_details()["f"]
-        return fm[:nr] + fe[:nr] + fw
+        # (MB-pol + electrostatics) + water terms, summed on the device in that order
+        return self.ctx.total_forces()
 
 
     def water_dipoles(self):

@@ -53,6 +53,7 @@ _SIGS = {
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mbgpu_evaluate_forces_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
                                   + [ctypes.c_void_p] * 16),
+    "mbgpu_get_forces": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "mbgpu_get_lists": (ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _ip,
                                        ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "mbgpu_heat_flux": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -339,6 +340,13 @@ class Context:
                                                       vy_ptr, vz_ptr, ek_ptr, _p(out))
         _check(rc, "mbgpu_evaluate_forces_dev")
         return out
+
+    def total_forces(self):
+        """Total forces of the last complete evaluation on the real atoms, (3*nw, 3):
+        (MB-pol + electrostatics) + water terms, summed on the device (mbgpu_get_forces)."""
+        f = np.zeros((3 * self.nw, 3))
+        _check(load_library().mbgpu_get_forces(self._h, _p(f)), "mbgpu_get_forces")
+        return f
 
     def evaluate_graph_counts(self):
         """(complete evaluations captured as graphs, replays) in this context."""

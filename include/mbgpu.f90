@@ -253,6 +253,12 @@ module mbgpu
       real(c_double), intent(out) :: out(64)
       integer(c_int) :: mbgpu_evaluate_forces_dev
     end function
+    function mbgpu_get_forces(ctx, f) bind(c, name="mbgpu_get_forces")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: f(*)
+      integer(c_int) :: mbgpu_get_forces
+    end function
     function mbgpu_get_evaluate_graph_counts(ctx, counts) bind(c, name="mbgpu_get_evaluate_graph_counts")
       import :: c_int, c_long, c_ptr
       type(c_ptr), value :: ctx

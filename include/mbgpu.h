@@ -423,6 +423,12 @@ int mbgpu_evaluate_forces_dev(mbgpu_ctx* ctx, int imcon, const double cell[9], i
                               const double* vz_dev, const double* e_kin_dev, double out[MBGPU_EVAL_OUT]);
 /* counts[0]: complete evaluations captured as graphs in this context, counts[1]: replays. */
 int mbgpu_get_evaluate_graph_counts(mbgpu_ctx* ctx, long counts[2]);
+/* Total forces of the last complete evaluation on the 3*nw real atoms, f[3*i + q] (host
+ * array): (MB-pol + electrostatics) + water terms per element, summed on the device in that
+ * order, one copy.  Needs an evaluation of all three parts since the context was created or
+ * resized.  The reference's counterpart is the summed fxx/fyy/fzz after forces
+ * (dlpoly_cmd.f:2287-2603); no separate routine. */
+int mbgpu_get_forces(mbgpu_ctx* ctx, double* f);
 
 
 /* Timing helper: runs the MB-pol force evaluation `reps` times on device-resident inputs

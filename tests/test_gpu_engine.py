@@ -204,3 +204,21 @@ def test_neighbour_rows_grow_and_repeat(pkg, waterbox, engine, monkeypatch):
     assert r1["eng"] == r0["eng"] and np.array_equal(r1["jtot"], r0["jtot"])
     assert np.array_equal(small.forces(), ref.forces())
     assert r1["dipole_iterations"] == r0["dipole_iterations"]
+
+
+def test_total_forces_equal_the_host_sum_of_the_parts(pkg):
+    # mbgpu_get_forces (engine.Slot.forces) against the sum of the three parts' details in
+    # the same order, through both evaluation paths
+    engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
+    box = pkg.waterbox.box_64()
+    for fused in (True, False):
+        slot = engine.Slot(box, fused=fused)
+        for _ in range(3):
+            slot.evaluate()
+        nr = box.nreal
+        _, _, fm = slot.ctx.decomposition()
+        fe = slot.ctx.elec_details()["f_sites"]
+        fw = slot.ctx.water_details()["f"]
+        assert np.array_equal(slot.forces(), fm[:nr] + fe[:nr] + fw)
+    with pytest.raises(pkg.MbgpuError, match="no complete evaluation"):
+        pkg.Context(8, 32).total_forces()
