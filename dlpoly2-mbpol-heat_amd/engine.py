@@ -63,11 +63,17 @@ class Slot:
         t = self.torch
         n, nr = box.natms, box.nreal
         f64 = dict(dtype=t.float64, device=self.dev)
+        m = 3 * n + 4 * nr
         with t.cuda.stream(self.stream):
-            self.x, self.y, self.z = (t.zeros(n, **f64) for _ in range(3))
+            # positions, velocities and kinetic energies are views of one buffer, so that a
+            # configuration is uploaded in one copy
+            self._dbuf = t.zeros(m, **f64)
+            self.x, self.y, self.z = self._dbuf[:n], self._dbuf[n:2 * n], self._dbuf[2 * n:3 * n]
+            self.v = self._dbuf[3 * n:3 * n + 3 * nr].view(3, nr)
+            self.ek = self._dbuf[3 * n + 3 * nr:]
             self.lst = t.zeros(nr, dtype=t.int32, device=self.dev)
-            self.v = t.zeros(3, nr, **f64)
-            self.ek = t.zeros(nr, **f64)
+        self._hbuf = t.empty(m, dtype=t.float64, pin_memory=True)   # pinned staging of load()
+        self._hlst = None                                            # the list last uploaded
 
     def load(self, box):
         """Upload a configuration of the same size (positions, list, velocities)."""
@@ -75,14 +81,17 @@ class Slot:
         if box.nw != self.box.nw:
             raise ValueError("a slot holds configurations of one size")
         self.box = box
-        nr = box.nreal
+        n, nr = box.natms, box.nreal
+        h = self._hbuf.numpy()        # free: the previous load() synchronized its copy
+        h[:n], h[n:2 * n], h[2 * n:3 * n] = box.x, box.y, box.z
+        h[3 * n:3 * n + 3 * nr] = box.vel[:nr].T.ravel()
+        h[3 * n + 3 * nr:] = 0.5 * box.mass[:nr] * (box.vel[:nr] ** 2).sum(1)
+        lst = np.ascontiguousarray(box.listttm2, dtype=np.int32)
         with t.cuda.stream(self.stream):
-            for dst, src in ((self.x, box.x), (self.y, box.y), (self.z, box.z)):
-                dst.copy_(t.as_tensor(np.ascontiguousarray(src), dtype=t.float64))
-            self.lst.copy_(t.as_tensor(np.ascontiguousarray(box.listttm2), dtype=t.int32))
-            self.v.copy_(t.as_tensor(np.ascontiguousarray(box.vel[:nr].T), dtype=t.float64))
-            ek = 0.5 * box.mass[:nr] * (box.vel[:nr] ** 2).sum(1)
-            self.ek.copy_(t.as_tensor(ek, dtype=t.float64))
+            self._dbuf.copy_(self._hbuf, non_blocking=True)
+            if self._hlst is None or not np.array_equal(lst, self._hlst):
+                self.lst.copy_(t.as_tensor(lst))
+                self._hlst = lst.copy()
         self.stream.synchronize()
 
     def evaluate(self):

@@ -80,7 +80,13 @@ class _Tensor:
     def __getitem__(self, i):
         return _Tensor(self.fake, None, f"{self.tag}[{i}]")
 
-    def copy_(self, src):
+    def view(self, *shape):
+        return _Tensor(self.fake, shape, f"{self.tag}[view]")
+
+    def numpy(self):
+        return self.host
+
+    def copy_(self, src, non_blocking=False):
         self.fake.log.append(("copy", self.fake.current, self.tag.split("[")[0]))
         return self
 
@@ -114,6 +120,12 @@ class _FakeTorch:
         self.log.append(("zero", self.current, tag))
         return _Tensor(self, shape, tag)
 
+    def empty(self, n, dtype=None, pin_memory=False):
+        # host staging (not an array the library reads)
+        t = _Tensor(self, (n,), "host")
+        t.host = np.zeros(n)
+        return t
+
     def as_tensor(self, a, dtype=None):
         return a
 
@@ -143,8 +155,9 @@ def test_slot_clears_and_uploads_on_its_own_stream(waterbox):
     box = waterbox.make_waterbox(8, 9.32, 2, seed=1)
     log = _alloc_and_load(engine.Slot, box)
     assert _unordered_writes(log) == []
-    # every array the library reads is among those cleared and loaded: x, y, z, list, v, ek
-    assert sum(op == "zero" for op, _, _ in log) == 6
+    # every array the library reads is among those cleared and loaded: the configuration
+    # buffer (x, y, z, v, ek are views of it) and the list
+    assert sum(op == "zero" for op, _, _ in log) == 2
     assert {tag for op, _, tag in log if op == "copy"} == {tag for op, _, tag in log if op == "zero"}
 
 
@@ -161,6 +174,14 @@ def test_order_check_catches_a_clear_on_the_default_stream(waterbox):
             self.lst = t.zeros(nr)
             self.v = t.zeros(3, nr)
             self.ek = t.zeros(nr)
+
+        def load(self, box):
+            # the upload of that arrangement: every array on the slot's stream, then a sync
+            t = self.torch
+            with t.cuda.stream(self.stream):
+                for a in (self.x, self.y, self.z, self.lst, self.v, self.ek):
+                    a.copy_(None)
+            self.stream.synchronize()
 
     box = waterbox.make_waterbox(8, 9.32, 2, seed=1)
     bad = _unordered_writes(_alloc_and_load(OldSlot, box))
