@@ -17,8 +17,10 @@ resident in HBM before the timed region:
 The read-back of the energy, virial and stress scalars is inside the step.
 
 Legs (all timed over --steps steps after --warmup):
-  * plain FP64, one configuration at a time (one step = one evaluation), with the time
-    split per part and the roofline of the dominant kernel;
+  * plain FP64, one configuration at a time (one step = one evaluation), through the four
+    separate entries in the reference's order, with the time split per part and the roofline
+    of the dominant kernel; then the same configuration through engine.Slot (one library
+    call per evaluation, graph replays);
   * batched FP64: --batch configurations (different seeds) evaluated concurrently, one
     library context, HIP stream and host thread each (one step = one evaluation of each);
   * batched mixed: the same with the 2B/3B polynomials in FP32, with its force, energy,
@@ -379,6 +381,19 @@ def main():
     n_sfac = (int(np.median(iters)) + 2) if iters else None
     single = args.steps / dt
 
+    # the same configuration through engine.Slot: one library call per evaluation
+    # (mbgpu_evaluate_dev, graph replays, electrostatics beside MB-pol and the water terms)
+    engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
+    one = engine.Slot(box, solver=1 if args.solver == "cg" else 0)
+    for _ in range(max(2, args.warmup)):
+        one.evaluate()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one.evaluate()
+    torch.cuda.synchronize()
+    single_one_call = args.steps / (time.perf_counter() - t0)
+    del one
 
     # batched legs: B independent 512-water configurations (different seeds), one context,
     # stream and host thread each, evaluated concurrently; FP64, then FP32 polynomials
@@ -466,12 +481,14 @@ def main():
     large_md = large_md_leg(pkg, args) if args.large_md_steps > 0 else None
     # headline: the fastest leg whose forces are within 1e-5 kcal/mol/A RMS of FP64
     lead = ("fp64 single", single, ms, "f64")
+    if single_one_call > single:
+        lead = ("fp64 single (one call)", single_one_call, 1e3 / single_one_call, "f64")
     if "fp64" in batched and batched["fp64"]["evals_per_s"] > single:
         lead = (f"fp64 batched x{args.batch}", batched["fp64"]["evals_per_s"], batched["fp64"]["ms_per_step"], "f64")
     if "mixed" in batched and batched["mixed"]["rms_dF"] <= 1e-5 and batched["mixed"]["evals_per_s"] > lead[1]:
         lead = (f"mixed batched x{args.batch}", batched["mixed"]["evals_per_s"], batched["mixed"]["ms_per_step"],
                 "f64+f32")
-    legs = {"fp64_single_evals_per_s": single}
+    legs = {"fp64_single_evals_per_s": single, "fp64_single_one_call_evals_per_s": single_one_call}
     for name, b in batched.items():
         legs[f"{name}_batched_evals_per_s"] = b["evals_per_s"]
         legs[f"{name}_batched_speedup_vs_fp64_single"] = b["evals_per_s"] / single
