@@ -2497,8 +2497,6 @@ __global__ void k_el_copy(double* __restrict__ dst, const double* __restrict__ s
     if (i < n) dst[i] = src[i];
 }
 
-// dst[i] = src[i], i < n (a kernel rather than a copy call, so that a captured graph holds
-// only kernel nodes)
 __global__ void k_el_pack(const double* __restrict__ state, const int* __restrict__ err,
                           const double* __restrict__ scal, const int* __restrict__ spflag, double* __restrict__ out)
 {
@@ -2518,6 +2516,40 @@ void el_launch_pack(const double* state, const int* err, const double* scal, con
     hipLaunchKernelGGL(k_el_pack, dim3(1), dim3(64), 0, s, state, err, scal, spflag, out);
 }
 
+// Up to EL_NSEG copies dst_k[i] = src_k[i], i < n_k (a null source clears dst_k) in one
+// launch, segment k in blockIdx.y: the small copies around an evaluation as one graph node.
+struct ElCopySegs {
+    double* dst[EL_NSEG];
+    const double* src[EL_NSEG];
+    long n[EL_NSEG];
+};
+
+__global__ void k_el_copy_segs(ElCopySegs S)
+{
+    const int k = blockIdx.y;
+    double* __restrict__ d = S.dst[k];
+    const double* __restrict__ src = S.src[k];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < S.n[k]; i += (long)gridDim.x * blockDim.x)
+        d[i] = src ? src[i] : 0.0;
+}
+
+void el_launch_copy_segs(int nseg, double* const* dst, const double* const* src, const long* n, hipStream_t s)
+{
+    ElCopySegs S{};
+    long nmax = 0;
+    for (int k = 0; k < nseg; ++k) {
+        S.dst[k] = dst[k];
+        S.src[k] = src[k];
+        S.n[k] = n[k];
+        nmax = n[k] > nmax ? n[k] : nmax;
+    }
+    if (nseg <= 0 || nmax <= 0) return;
+    const long blocks = ceil_div(nmax, 256) < 1024 ? ceil_div(nmax, 256) : 1024;
+    hipLaunchKernelGGL(k_el_copy_segs, dim3(blocks, nseg), dim3(256), 0, s, S);
+}
+
+// dst[i] = src[i], i < n (a kernel rather than a copy call, so that a captured graph holds
+// only kernel nodes)
 void el_launch_copy(double* dst, const double* src, long n, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_el_copy, dim3(ceil_div(n, 256)), dim3(256), 0, s, dst, src, n);

@@ -578,18 +578,33 @@ int full_enqueue_elec(mbgpu_ctx* c, const mbgpu_elec_params* ep, int gb, double*
 {
     ElecBuffers& E = c->el.E;
     const int nw = c->B.nw, n = 4 * nw, nr = 3 * nw;
-    el_launch_copy(E.x, x, n, b);
-    el_launch_copy(E.y, y, n, b);
-    el_launch_copy(E.z, z, n, b);
-    if (dip) el_launch_copy(E.dip, dip, 3L * n, b);
-    else el_launch_zero(E.dip, 3L * n, b);
+    {
+        // positions in; start dipoles in, or zero
+        double* d[4] = {E.x, E.y, E.z, E.dip};
+        const double* sr[4] = {x, y, z, dip};
+        const long len[4] = {n, n, n, 3L * n};
+        el_launch_copy_segs(4, d, sr, len, b);
+    }
     int rc = elec_enqueue_full(c->el, ep->distribute_msite, ep->solver, gb, b);
     if (rc) return fail(rc, "electrostatics kernels failed: %s", hipGetErrorString((hipError_t)rc));
-    el_launch_copy(x + nr, E.x + nr, nw, b);
-    el_launch_copy(y + nr, E.y + nr, nw, b);
-    el_launch_copy(z + nr, E.z + nr, nw, b);
-    if (dip) el_launch_copy(dip, E.dip, 3L * n, b);
-    if (fsite) el_launch_copy(fsite, E.fsite, 3L * n, b);
+    {
+        // M-site positions out; dipoles and per-site forces out when asked for
+        double* d[5] = {x + nr, y + nr, z + nr, nullptr, nullptr};
+        const double* sr[5] = {E.x + nr, E.y + nr, E.z + nr, nullptr, nullptr};
+        long len[5] = {nw, nw, nw, 0, 0};
+        int k = 3;
+        if (dip) {
+            d[k] = dip;
+            sr[k] = E.dip;
+            len[k++] = 3L * n;
+        }
+        if (fsite) {
+            d[k] = fsite;
+            sr[k] = E.fsite;
+            len[k++] = 3L * n;
+        }
+        el_launch_copy_segs(k, d, sr, len, b);
+    }
     return 0;
 }
 
