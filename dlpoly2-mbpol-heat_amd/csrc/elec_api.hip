@@ -548,15 +548,23 @@ int enqueue_attempt(ElecState& S, const ElecSpme* sp, int solver, bool first, in
 {
     ElecBuffers& E = S.E;
     const ElecDev& P = S.P;
-    if (first) el_launch_copy(E.dip0, E.dip, 3L * P.n, s);
-    else el_launch_copy(E.dip, E.dip0, 3L * P.n, s);
-    el_launch_zero(reinterpret_cast<double*>(E.err), 2, s);   // the four int flags
+    {
+        // the start dipoles kept (or restored), and the four int flags cleared
+        double* d[2] = {first ? E.dip0 : E.dip, reinterpret_cast<double*>(E.err)};
+        const double* sr[2] = {first ? E.dip : E.dip0, nullptr};
+        const long len[2] = {3L * P.n, 2};
+        el_launch_copy_segs(2, d, sr, len, s);
+    }
     int rc = el_launch_setup(E, P, S.cells, s);
     if (rc) return rc;
-    el_launch_zero(E.state, 16, s);
-    el_launch_zero(E.emu, 3L * P.n, s);
-    // M sites carry no induced dipole (polr2 = 0): clear any start value
-    el_launch_zero(E.dip + 3 * P.nreal, 3L * (P.n - P.nreal), s);
+    {
+        // solver state and T mu cleared; M sites carry no induced dipole (polr2 = 0): any
+        // start value cleared
+        double* d[3] = {E.state, E.emu, E.dip + 3 * P.nreal};
+        const double* sr[3] = {nullptr, nullptr, nullptr};
+        const long len[3] = {16, 3L * P.n, 3L * (P.n - P.nreal)};
+        el_launch_copy_segs(3, d, sr, len, s);
+    }
     rc = el_launch_prep(E, P, S.rcell, sp, s);
     if (!rc && solver == 1) rc = el_launch_cg_init(E, P, sp, s);
     for (int b = 0; !rc && b < nbatch; ++b)

@@ -2405,8 +2405,10 @@ int el_launch_final(ElecBuffers& E, const ElecDev& P, int distribute, double g34
     if (sp && !sp->B) {
         // no reciprocal space: zero site sums and per-block partials for the k-sum's final
         // kernel (forces, ESP, per-atom energy and stress of the reciprocal part all 0)
-        el_launch_zero(E.ksum, 15L * P.n, s);
-        el_launch_zero(b_ks, 8L * nkb, s);
+        double* d[2] = {E.ksum, b_ks};
+        const double* sr[2] = {nullptr, nullptr};
+        const long len[2] = {15L * P.n, 8L * nkb};
+        el_launch_copy_segs(2, d, sr, len, s);
     } else if (sp) {
         // particle mesh: grids 0..4 hold q + mu, q, and mu_x, mu_y, mu_z as charges; the
         // stress kernels of grid 0 go to grids 5..10; every per-site quantity of the k-sum
@@ -2546,6 +2548,42 @@ void el_launch_copy_segs(int nseg, double* const* dst, const double* const* src,
     if (nseg <= 0 || nmax <= 0) return;
     const long blocks = ceil_div(nmax, 256) < 1024 ? ceil_div(nmax, 256) : 1024;
     hipLaunchKernelGGL(k_el_copy_segs, dim3(blocks, nseg), dim3(256), 0, s, S);
+}
+
+// Up to EL_NSEG sums dst_k[i] = a_k[i] + b_k[i], i < n_k, in one launch (segment k in
+// blockIdx.y).
+struct ElSumSegs {
+    double* dst[EL_NSEG];
+    const double* a[EL_NSEG];
+    const double* b[EL_NSEG];
+    long n[EL_NSEG];
+};
+
+__global__ void k_el_sum_segs(ElSumSegs S)
+{
+    const int k = blockIdx.y;
+    double* __restrict__ d = S.dst[k];
+    const double* __restrict__ a = S.a[k];
+    const double* __restrict__ b = S.b[k];
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < S.n[k]; i += (long)gridDim.x * blockDim.x)
+        d[i] = a[i] + b[i];
+}
+
+void el_launch_sum_segs(int nseg, double* const* dst, const double* const* a, const double* const* b, const long* n,
+                        hipStream_t s)
+{
+    ElSumSegs S{};
+    long nmax = 0;
+    for (int k = 0; k < nseg; ++k) {
+        S.dst[k] = dst[k];
+        S.a[k] = a[k];
+        S.b[k] = b[k];
+        S.n[k] = n[k];
+        nmax = n[k] > nmax ? n[k] : nmax;
+    }
+    if (nseg <= 0 || nmax <= 0) return;
+    const long blocks = ceil_div(nmax, 256) < 1024 ? ceil_div(nmax, 256) : 1024;
+    hipLaunchKernelGGL(k_el_sum_segs, dim3(blocks, nseg), dim3(256), 0, s, S);
 }
 
 // dst[i] = src[i], i < n (a kernel rather than a copy call, so that a captured graph holds

@@ -636,16 +636,17 @@ int full_enqueue_tail(mbgpu_ctx* c, const double* vx, const double* vy, const do
     MbBuffers& B = c->B;
     ElecBuffers& E = c->el.E;
     const long nr = 3L * B.nw;
-    const double* ep_ = nullptr;
-    const double* we_ = nullptr;
-    const double* se_ = nullptr;
-    const double* es[2] = {E.eout, c->wt.B.eout};
-    const double* ws[2] = {E.wout, c->wt.B.wout};
-    const double* ss[1] = {E.sout};
+    // per-atom energies and W: electrostatics + water terms (with_model_terms' order, one
+    // launch); the reciprocal stresses are the electrostatics' own
+    const double* ep_ = c->dep;
+    const double* we_ = c->dw;
+    const double* se_ = E.sout;
     if (vx) {
-        add_terms(c, ep_, es, 2, c->dep, nr, a);
-        add_terms(c, we_, ws, 2, c->dw, 9L * nr, a);
-        add_terms(c, se_, ss, 1, c->dsew, 9L * nr, a);
+        double* d[2] = {c->dep, c->dw};
+        const double* ta[2] = {E.eout, E.wout};
+        const double* tb[2] = {c->wt.B.eout, c->wt.B.wout};
+        const long len[2] = {nr, 9L * nr};
+        el_launch_sum_segs(2, d, ta, tb, len, a);
     }
     if (vx) {
         const int rc = mb_launch_heat_flux((int)nr, vx, vy, vz, ek, B.atomic_energy, ep_, B.heat_w, we_, se_,

@@ -210,6 +210,9 @@ void el_launch_copy(double* dst, const double* src, long n, hipStream_t s);
 // nseg <= EL_NSEG copies (a null source clears its destination) in one launch.
 constexpr int EL_NSEG = 6;
 void el_launch_copy_segs(int nseg, double* const* dst, const double* const* src, const long* n, hipStream_t s);
+// nseg <= EL_NSEG sums dst_k = a_k + b_k in one launch.
+void el_launch_sum_segs(int nseg, double* const* dst, const double* const* a, const double* const* b, const long* n,
+                        hipStream_t s);
 // Read-back of one electrostatics evaluation packed in doubles: the solver state (8), the
 // error flag, the 41 reduced scalars, the particle mesh's two input flags (0 without one).
 constexpr int EL_PK_ST = 0, EL_PK_ERR = 8, EL_PK_SCAL = 9, EL_PK_SPF = 50, EL_PK_N = 52;
