@@ -190,6 +190,11 @@ __device__ double gammq_nr(double a, double x, double gln)
         }
         return 1.0 - sum * exp(-x + a * log(x) - gln);
     }
+    // the continued fraction is multiplied by exp(lf): below -750 that factor is exactly 0 in
+    // double precision (underflow), and so is the result whatever the fraction (finite for
+    // x >= a + 1), so the fraction is not evaluated
+    const double lf = -x + a * log(x) - gln;
+    if (lf < -750.0) return 0.0;
     double b = x + 1.0 - a, c = 1.0 / FPMIN, d = 1.0 / b, h = d;
     for (int i = 1; i <= 100; ++i) {
         const double an = -i * (i - a);
@@ -203,7 +208,7 @@ __device__ double gammq_nr(double a, double x, double gln)
         h = h * del;
         if (fabs(del - 1.0) < EPS) break;
     }
-    return exp(-x + a * log(x) - gln) * h;
+    return exp(lf) * h;
 }
 
 // 3-point interpolation of the B-function table (rfield.f:102-120, ewald2p.f:160-178)
