@@ -17,10 +17,10 @@ resident in HBM before the timed region:
 The read-back of the energy, virial and stress scalars is inside the step.
 
 Legs (all timed over --steps steps after --warmup):
-  * plain FP64, one configuration at a time (one step = one evaluation): first through
-    engine.Slot (one library call per evaluation, graph replays), then through the four
+  * plain FP64, one configuration at a time (one step = one evaluation): through the four
     separate entries in the reference's order, with the time split per part and the roofline
-    of the dominant kernel;
+    of the dominant kernel, then (that context released) through engine.Slot (one library
+    call per evaluation, graph replays);
   * batched FP64: --batch configurations (different seeds) evaluated concurrently, one
     library context, HIP stream and host thread each (one step = one evaluation of each);
   * batched mixed: the same with the 2B/3B polynomials in FP32, with its force, energy,
@@ -319,24 +319,6 @@ def main():
         print(json.dumps({"large_box_md": large_md_leg(pkg, args)}), flush=True)
         return
     box = pkg.waterbox.box_512()
-    # the configuration through engine.Slot: one library call per evaluation (mbgpu_evaluate_dev,
-    # graph replays, electrostatics beside MB-pol and the water terms), timed first: with HIP's
-    # 4 hardware queues, a context whose two streams are created after another context's
-    # measured 1.72 ms instead of 1.48 ms per evaluation (how the runtime maps streams to
-    # queues; 8 queues or this order give 1.48 ms)
-    engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
-    one = engine.Slot(box, solver=1 if args.solver == "cg" else 0)
-    for _ in range(max(2, args.warmup)):
-        one.evaluate()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one.evaluate()
-    torch.cuda.synchronize()
-    single_one_call = args.steps / (time.perf_counter() - t0)
-    one.close()
-    one.ctx.close()
-    del one
     ctx = pkg.Context(box.nw, box.natms)
     ep = pkg.elec_params.for_box(box.L)
     eparams = pkg.ElecParams.from_params(ep, distribute_msite=1, solver=1 if args.solver == "cg" else 0)
@@ -399,6 +381,26 @@ def main():
     achieved = flop_sf / (ms_sf * 1e-3) / 1e12
     n_sfac = (int(np.median(iters)) + 2) if iters else None
     single = args.steps / dt
+
+    # the same configuration through engine.Slot: one library call per evaluation
+    # (mbgpu_evaluate_dev, graph replays, electrostatics beside MB-pol and the water terms),
+    # after the separate entries' context is released: with HIP's 4 hardware queues a
+    # context whose two streams are created while another context's exist measured 1.72 ms
+    # instead of 1.48 ms per evaluation (how the runtime maps streams to queues)
+    ctx.close()
+    engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
+    one = engine.Slot(box, solver=1 if args.solver == "cg" else 0)
+    for _ in range(max(2, args.warmup)):
+        one.evaluate()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one.evaluate()
+    torch.cuda.synchronize()
+    single_one_call = args.steps / (time.perf_counter() - t0)
+    one.close()
+    one.ctx.close()
+    del one
 
     # batched legs: B independent 512-water configurations (different seeds), one context,
     # stream and host thread each, evaluated concurrently; FP64, then FP32 polynomials
