@@ -450,6 +450,8 @@ def main():
         t0g = time.perf_counter()
         gk = md.green_kubo(nve.slot.ctx, np.array(jser), 0.2e-3, box.L ** 3, 300.0, ntau)
         tg = time.perf_counter() - t0g
+        nve.slot.close()                 # release the context before the next leg creates its own
+        nve.slot.ctx.close()
         tot = np.array(tot)
         slope = np.polyfit(np.arange(1, len(tot) + 1) * 0.2e-3, tot - e0, 1)[0] if len(tot) > 1 else 0.0
         nve_out = {"steps": args.nve_steps, "dt_fs": 0.2, "ms_per_step": 1e3 * tn / args.nve_steps,

@@ -158,3 +158,4 @@ class RingPolymer:
             self._pool = None
         for s in self.slots:
             s.close()
+            s.ctx.close()
