@@ -432,7 +432,8 @@ def main():
     if args.nve_steps > 0:
         md = importlib.import_module("dlpoly2-mbpol-heat_amd.md")
         engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
-        nve = md.NVE(engine.Slot(box, solver=1 if args.solver == "cg" else 0), 0.2, extrapolate=2)
+        nve = md.NVE(engine.Slot(box, solver=1 if args.solver == "cg" else 0, dedicated_queues=True), 0.2,
+                     extrapolate=2)
         e0, k0 = nve.pot + nve.kin, nve.kin
         tot, its, jser = [], [], []
         torch.cuda.synchronize()

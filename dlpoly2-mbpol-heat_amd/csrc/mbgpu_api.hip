@@ -717,9 +717,15 @@ int mbgpu_check_listttm2(int nttm2, const int* listttm2, int natms)
     return 0;
 }
 
-mbgpu_ctx* mbgpu_create(int nw, int natms)
+mbgpu_ctx* mbgpu_create(int nw, int natms) { return mbgpu_create_queues(nw, natms, 0); }
+
+mbgpu_ctx* mbgpu_create_queues(int nw, int natms, int dedicated)
 {
     EntryGuard gate_;
+    if (dedicated != 0 && dedicated != 1) {
+        fail(-1, "mbgpu_create_queues: dedicated is 0 or 1 (got %d)", dedicated);
+        return nullptr;
+    }
     if (nw < 1 || natms < 3 * nw) {
         fail(-1, "mbgpu_create: need nw >= 1 and natms >= 3*nw (got %d, %d)", nw, natms);
         return nullptr;
@@ -737,8 +743,26 @@ mbgpu_ctx* mbgpu_create(int nw, int natms)
     // into graphs (mbgpu_set_graphs), and another thread's null-stream work during a capture
     // would otherwise invalidate it; nothing relies on that implicit ordering, since every
     // entry waits for its streams before it returns and takes inputs the caller has finished
-    if (hipStreamCreate(&c->stream) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->el_stream, hipStreamNonBlocking) != hipSuccess || alloc_fixed(c) ||
+    // With `dedicated`, both streams are created with a full CU mask, for which the runtime
+    // makes hardware queues of their own: the electrostatics then always overlap MB-pol and
+    // the water terms (streams of shared queues can land on one queue).  Such streams are
+    // blocking ones.
+    hipError_t se = hipSuccess;
+    if (dedicated) {
+        hipDeviceProp_t prop;
+        se = hipGetDeviceProperties(&prop, c->device);
+        const int ncu = se == hipSuccess ? prop.multiProcessorCount : 0, nwd = (ncu + 31) / 32;
+        std::vector<uint32_t> mask(nwd > 0 ? nwd : 1, 0xffffffffu);
+        if (ncu % 32) mask[nwd - 1] = (1u << (ncu % 32)) - 1u;
+        if (se == hipSuccess && ncu < 1) se = hipErrorInvalidDevice;
+        if (se == hipSuccess) se = hipExtStreamCreateWithCUMask(&c->stream, nwd, mask.data());
+        if (se == hipSuccess) se = hipExtStreamCreateWithCUMask(&c->el_stream, nwd, mask.data());
+    } else {
+        se = hipStreamCreate(&c->stream);
+        if (se == hipSuccess) se = hipStreamCreateWithFlags(&c->el_stream, hipStreamNonBlocking);
+    }
+    if (se != hipSuccess) fail((int)se, "streams: %s", hipGetErrorString(se));
+    if (se != hipSuccess || alloc_fixed(c) ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
         std::string msg = g_err;

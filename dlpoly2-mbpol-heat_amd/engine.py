@@ -27,12 +27,14 @@ ENGUNIT = 418.4
 
 class Slot:
     def __init__(self, box, precision=0, solver=1, elec=None, water=None, device="cuda:0", overlap=True,
-                 recip=None, fused=True):
+                 recip=None, fused=True, dedicated_queues=False):
         import torch
         self.torch = torch
         self.dev = torch.device(device)
         self.box = box
-        self.ctx = Context(box.nw, box.natms)
+        # dedicated_queues: the context's streams get hardware queues of their own (one
+        # latency-bound slot, e.g. an MD run); shared queues suit many concurrent slots
+        self.ctx = Context(box.nw, box.natms, dedicated_queues=dedicated_queues)
         self.ctx.set_precision(precision)
         # recip: None (the Ewald sum over k) or ("spme", order, grid)
         if recip is not None:

@@ -34,6 +34,7 @@ _SIGS = {
     "mbpol_3b_cutoff": (None, [_dp]),
     "mbgpu_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int]),
     "mbgpu_destroy": (None, [ctypes.c_void_p]),
+    "mbgpu_create_queues": (ctypes.c_void_p, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mbgpu_last_error": (ctypes.c_char_p, []),
     "mbgpu_check_listttm2": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int]),
     "mbgpu_mbpol_forces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp, _dp, _dp,
@@ -248,12 +249,15 @@ def ring_matrices(nbead, langevin=0):
 
 
 class Context:
-    """One device context for a system of nw waters and natms sites."""
+    """One device context for a system of nw waters and natms sites.  dedicated_queues: its
+    two streams get hardware queues of their own (mbgpu_create_queues), for one
+    latency-bound context; the default shares the process's queues (many contexts)."""
 
-    def __init__(self, nw, natms):
+    def __init__(self, nw, natms, dedicated_queues=False):
         lib = load_library()
         self.nw, self.natms = nw, natms
-        self._h = lib.mbgpu_create(nw, natms)
+        self.dedicated_queues = bool(dedicated_queues)
+        self._h = lib.mbgpu_create_queues(nw, natms, int(self.dedicated_queues))
         if not self._h:
             raise MbgpuError("mbgpu_create failed: " + lib.mbgpu_last_error().decode())
 

@@ -245,7 +245,7 @@ def large_box_run(box, relax_steps=500, nve_steps=10000, shadow_every=100, dt_fs
     out = dict(waters=box.nw, relax_steps=relax_steps, nve_steps=nve_steps, shadow_every=shadow_every,
                dt_fs=dt_fs, recip=None if recip is None else list(recip))
     # 1. relaxation (FP64)
-    fp64 = engine.Slot(box, precision=0, recip=recip)
+    fp64 = engine.Slot(box, precision=0, recip=recip, dedicated_queues=True)
     md = NVE(fp64, dt_fs, extrapolate=extrapolate)
     temps = []
     t0 = _time.perf_counter()
@@ -263,7 +263,7 @@ def large_box_run(box, relax_steps=500, nve_steps=10000, shadow_every=100, dt_fs
     out["relax_ms_per_step"] = 1e3 * (_time.perf_counter() - t0) / max(1, relax_steps)
     out["relax_temperature_last"] = temps[-1] if temps else None
     # 2.-3. mixed-precision NVE with the FP64 shadow
-    mixed = engine.Slot(box, precision=3, recip=recip)
+    mixed = engine.Slot(box, precision=3, recip=recip, dedicated_queues=True)
     _copy_state(fp64, mixed)
     shadow = fp64
     nve = NVE(mixed, dt_fs, extrapolate=extrapolate)

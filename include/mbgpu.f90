@@ -80,6 +80,11 @@ module mbgpu
       integer(c_int), value :: nw, natms
       type(c_ptr) :: mbgpu_create
     end function
+    function mbgpu_create_queues(nw, natms, dedicated) bind(c, name="mbgpu_create_queues")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nw, natms, dedicated
+      type(c_ptr) :: mbgpu_create_queues
+    end function
     subroutine mbgpu_destroy(ctx) bind(c, name="mbgpu_destroy")
       import :: c_ptr
       type(c_ptr), value :: ctx

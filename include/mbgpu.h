@@ -61,6 +61,13 @@ void mbpol_3b_cutoff(double* r);
 /* nw waters, natms sites in the per-atom arrays (4*nw with M sites).  Returns NULL on
  * failure (no GPU, out of memory, tables missing); see mbgpu_last_error(). */
 mbgpu_ctx* mbgpu_create(int nw, int natms);
+/* mbgpu_create with a choice of hardware queues: dedicated = 1 creates the context's two
+ * streams (MB-pol and water terms; electrostatics) with hardware queues of their own, so
+ * that they always run side by side, for one latency-bound context (an MD run); dedicated =
+ * 0 is mbgpu_create (queues shared among all streams of the process, the better choice
+ * for many concurrent contexts).  Results are identical either way.  No counterpart in the
+ * reference. */
+mbgpu_ctx* mbgpu_create_queues(int nw, int natms, int dedicated);
 void mbgpu_destroy(mbgpu_ctx* ctx);
 const char* mbgpu_last_error(void);
 

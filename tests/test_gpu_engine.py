@@ -222,3 +222,22 @@ def test_total_forces_equal_the_host_sum_of_the_parts(pkg):
         assert np.array_equal(slot.forces(), fm[:nr] + fe[:nr] + fw)
     with pytest.raises(pkg.MbgpuError, match="no complete evaluation"):
         pkg.Context(8, 32).total_forces()
+
+
+def test_dedicated_queues_give_the_same_results(pkg):
+    # mbgpu_create_queues(..., 1): the same kernels on streams with queues of their own
+    engine = importlib.import_module("dlpoly2-mbpol-heat_amd.engine")
+    box = pkg.waterbox.box_64()
+    a = engine.Slot(box)
+    b = engine.Slot(box, dedicated_queues=True)
+    for _ in range(3):
+        ra, rb = a.evaluate(), b.evaluate()
+        for k in ("eng", "vir", "dipole_iterations"):
+            assert ra[k] == rb[k], k
+        assert np.array_equal(ra["jtot"], rb["jtot"]) and np.array_equal(ra["stress"], rb["stress"])
+        assert np.array_equal(a.forces(), b.forces())
+    with pytest.raises(pkg.MbgpuError, match="dedicated"):
+        pkg.mbgpu.load_library()
+        h = pkg.mbgpu.load_library().mbgpu_create_queues(8, 32, 2)
+        assert not h
+        raise pkg.MbgpuError(pkg.mbgpu.load_library().mbgpu_last_error().decode())
